@@ -1,0 +1,110 @@
+/* quickses: SES (solvent excluded surface) meshes on AMD Instinct MI355X (gfx950).
+ *
+ * C ABI of libquickses.so.  The first five entry points are drop-in replacements of the
+ * reference's exported functions and keep their signatures and semantics; the rest are
+ * additions that the reference does not have.
+ *
+ * Types: qs_float3 is three packed floats (12 bytes, 4-byte aligned), the layout of the
+ * float3 the reference's ABI passes (CudaSurf.h:75-83).
+ */
+#ifndef QUICKSES_H
+#define QUICKSES_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    float x, y, z;
+} qs_float3;
+
+/* ---------------------------------------------------------------------------------------- */
+/* Reference-compatible entry points                                                          */
+
+/* Replaces API_computeSES (CudaSurf.h:77-78, CudaSurf.cu:787-841).
+ * resoSES: voxel size in Angstrom; atomPos/atomRad: N atom centres and radii (host memory);
+ * doSmoothing: number of Laplacian passes (<= 0: none).  out_vertices and out_triangles are
+ * ignored, as in the reference; results are read with the getters below.  *NVert = number of
+ * vertices, *NTri = number of triangle INDICES (3 per triangle) after dropping triangles with a
+ * repeated vertex.  N <= 1 gives an empty mesh.  On error the mesh is empty and
+ * API_lastError() is non-zero (the reference exits the process instead). */
+void API_computeSES(float resoSES, qs_float3 *atomPos, float *atomRad, unsigned int N, qs_float3 *out_vertices,
+                    unsigned int *NVert, int *out_triangles, unsigned int *NTri, int doSmoothing);
+
+/* Replaces API_getTriangles (CudaSurf.h:79, CudaSurf.cu:845-857): flat int32[NTri]; with
+ * invertTriangles the first two indices of every triangle are swapped in place. */
+int *API_getTriangles(bool invertTriangles);
+
+/* Replaces API_getVertices (CudaSurf.h:80, CudaSurf.cu:858-861): qs_float3[NVert]. */
+qs_float3 *API_getVertices(void);
+
+/* Replaces API_freeMesh (CudaSurf.h:81, CudaSurf.cu:867-872): frees the three result arrays. */
+void API_freeMesh(void);
+
+/* Replaces API_getAtomIdPerVert (CudaSurf.h:82, CudaSurf.cu:862-865): int32[NVert], the
+ * atom nearest to each vertex (before smoothing) among the 27 neighbour cells; -1 if none.
+ * By default the index into the caller's atom arrays; with qs_set_option("atom_ids_sorted", 1)
+ * the index into the cell-sorted atom order, which is what the reference returns. */
+int *API_getAtomIdPerVert(void);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Additions                                                                                   */
+
+/* Error code of the last call (0 = success) and its message. */
+int API_lastError(void);
+const char *qs_last_error_message(void);
+
+/* Options (process-wide, like the reference's globals):
+ *   "slice"           x-slab thickness in grid points, 0 = whole grid in one slab (default);
+ *                     the mesh does not depend on it (the reference's -s flag)
+ *   "lim_mode"        0 = polygonise every cube whose corners are valid (default),
+ *                     1 = stop at cube g-2r-2 as the reference's single slice does
+ *   "atom_ids_sorted" 0 = input atom index (default), 1 = cell-sorted index (reference)
+ *   "pass2_direct"    1 = evaluate field pass 2 with the (2r+1)^3 window kernel
+ *   "trace"           1 = print per-stage GPU times as JSON lines on stderr
+ *   "timing"          1 = record per-stage GPU times (hipEvents on the library's stream)
+ * Returns 0, or -1 for an unknown key. */
+int qs_set_option(const char *key, int value);
+int qs_get_option(const char *key);
+
+/* Device-resident form of API_computeSES: d_pos (3*N floats) and d_rad (N floats) are device
+ * pointers on the current HIP device; the mesh stays on the device in buffers owned by the
+ * library, valid until the next call.  *nvert, *nidx as for API_computeSES.  Returns 0 or an
+ * error code.  The call is synchronous. */
+int qs_compute_device(const float *d_pos, const float *d_rad, unsigned int N, float resoSES, int doSmoothing,
+                      unsigned int *nvert, unsigned int *nidx, float **d_vert, int **d_tri, int **d_atom_id);
+
+/* Per-stage GPU times of the last computation (needs option "timing" or "trace"): stage names
+ * comma-separated into names, milliseconds into ms; returns the number of stages. */
+int qs_stage_times(char *names, int names_cap, float *ms, int max_stages);
+
+/* Grid of the last computation: fparams = {origin x,y,z, dx, dxn}, iparams = {g, gn, r, lim}. */
+int qs_last_grid(float *fparams, int *iparams);
+
+/* Copies of the last computation's intermediate arrays to host memory (stage-wise checks):
+ *   "labels" uint8[g^3] (0 outside, 1 shell, 2 inside), "field" float[g^3],
+ *   "perm" int32[N] (cell-sorted order), "cells" int32[2*gn^3] (start,end pairs),
+ *   "vert_unsmoothed" float[3*NVert], "tri_all" int32[3*T] (before the degenerate filter).
+ * Returns the number of bytes copied, or -1. */
+int64_t qs_fetch(const char *what, void *host, int64_t bytes);
+
+/* Stage entry points on caller-given inputs (host memory):
+ * field pass 2 from labels; marching cubes + weld from a field. */
+int qs_stage_pass2(const float *fparams, const int *iparams, const uint8_t *labels, float *field);
+int qs_stage_mesh(const float *fparams, const int *iparams, const float *field, unsigned int *nvert,
+                  unsigned int *ntri, float *vert, int64_t vert_cap, int *tri, int64_t tri_cap);
+
+/* Seeded synthetic molecules (test and benchmark inputs). */
+int qs_synth_ball(uint64_t seed, int n, double density, double min_dist, float *pos, float *rad);
+int qs_synth_shell(uint64_t seed, int n, double r_in, double r_out, double min_dist, float *pos, float *rad);
+int qs_synth_jitter(uint64_t seed, int n, double sigma, const float *base, float *out);
+int qs_synth_sizes(uint64_t seed, int count, double lo, double hi, int *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

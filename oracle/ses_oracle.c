@@ -1,0 +1,786 @@
+/* CPU oracle for the SES mesh path -- TEST INFRASTRUCTURE ONLY (see ses_oracle.h).
+ *
+ * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off -fno-fast-math).  Every float
+ * expression below is written in the evaluation order of the reference kernel it restates, so
+ * that with contraction off each operation rounds exactly once, as IEEE single precision.
+ */
+#include "ses_oracle.h"
+#include "../quickses_amd/csrc/qs_mc_table.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+int or_num_threads(void) { return omp_get_max_threads(); }
+
+static inline float fmin_ref(float a, float b) { return (b < a) ? b : a; } /* std::min */
+static inline float fmax_ref(float a, float b) { return (a < b) ? b : a; } /* std::max */
+static inline int iclamp(int f, int a, int b) { return f < a ? a : (f > b ? b : f); }
+
+/* ------------------------------------------------------------------------------------------ */
+/* grid setup: getMinMax CudaSurf.cu:149-174, computeMaxDist 250-253, grids 548-576, range 629  */
+int or_grid_setup(const float *pos, const float *rad, int n, float reso, or_grid *G)
+{
+    memset(G, 0, sizeof(*G));
+    G->n_atoms = n;
+    G->reso = reso;
+    float vmin[3] = {100000.0f, 100000.0f, 100000.0f};
+    float vmax[3] = {-100000.0f, -100000.0f, -100000.0f};
+    float max_rad = 0.0f;
+    for (int a = 0; a < n; a++) {
+        for (int c = 0; c < 3; c++) {
+            vmin[c] = fmin_ref(vmin[c], pos[3 * a + c]);
+            vmax[c] = fmax_ref(vmax[c], pos[3 * a + c]);
+        }
+        max_rad = fmax_ref(max_rad, rad[a]);
+    }
+    for (int c = 0; c < 3; c++) {
+        G->minv[c] = vmin[c];
+        G->maxv[c] = vmax[c];
+    }
+    G->max_rad = max_rad;
+    if (n <= 1) return 1;
+    float ext = fmax_ref(vmax[0] - vmin[0], fmax_ref(vmax[1] - vmin[1], vmax[2] - vmin[2]));
+    float max_dist = ext + (2 * max_rad);
+    max_dist = max_dist + (4 * OR_PROBE);
+    G->max_dist = max_dist;
+    G->dx_n = OR_PROBE + max_rad;
+    for (int c = 0; c < 3; c++) {
+        float o = vmin[c] - max_rad;
+        G->origin[c] = o - OR_PROBE;
+    }
+    G->gn = (int)ceilf(max_dist / G->dx_n);
+    G->g = (int)ceilf(max_dist / reso);
+    G->r = (int)ceilf(OR_PROBE / reso);
+    return 0;
+}
+
+/* spaceToGrid Kernels.cu:119-123 (truncating float->int), flatten3DTo1D Kernels.cu:129-131 */
+static inline int cell_of(float p, float o, float dx) { return (int)((p - o) / dx); }
+
+/* hashAtoms Kernels.cu:185-216 */
+void or_hash(const or_grid *G, const float *pos, int n, int *hash)
+{
+    const int gn = G->gn;
+    for (int i = 0; i < n; i++) {
+        int cx = cell_of(pos[3 * i + 0], G->origin[0], G->dx_n);
+        int cy = cell_of(pos[3 * i + 1], G->origin[1], G->dx_n);
+        int cz = cell_of(pos[3 * i + 2], G->origin[2], G->dx_n);
+        hash[i] = (gn * gn * cx) + (gn * cy) + cz;
+    }
+}
+
+/* stable counting sort by cell id == thrust::sort(int2, compare_int2) with (cell, index)
+   canonical order (CudaSurf.cu:610-612); cell start/end as sortCell (Kernels.cu:306-341) */
+void or_sort_cells(const or_grid *G, const float *pos, const float *rad, int n, const int *hash,
+                   int *perm, int *cell_start, int *cell_end, float *sorted_xyzr)
+{
+    const int64_t ncell = (int64_t)G->gn * G->gn * G->gn;
+    int *cnt = (int *)calloc((size_t)ncell + 1, sizeof(int));
+    for (int i = 0; i < n; i++) cnt[hash[i] + 1]++;
+    for (int64_t c = 0; c < ncell; c++) cnt[c + 1] += cnt[c];
+    for (int64_t c = 0; c < ncell; c++) {
+        if (cnt[c + 1] > cnt[c]) {
+            cell_start[c] = cnt[c];
+            cell_end[c] = cnt[c + 1];
+        } else {
+            cell_start[c] = OR_EMPTY;
+            cell_end[c] = OR_EMPTY;
+        }
+    }
+    for (int i = 0; i < n; i++) perm[cnt[hash[i]]++] = i;
+    for (int k = 0; k < n; k++) {
+        int id = perm[k];
+        sorted_xyzr[4 * k + 0] = pos[3 * id + 0];
+        sorted_xyzr[4 * k + 1] = pos[3 * id + 1];
+        sorted_xyzr[4 * k + 2] = pos[3 * id + 2];
+        sorted_xyzr[4 * k + 3] = rad[id];
+    }
+    free(cnt);
+}
+
+/* fast_distance Kernels.cu:149-155, evaluated left to right, IEEE sqrtf */
+static inline float dist3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    float dx = ax - bx, dy = ay - by, dz = az - bz;
+    float x = dx * dx;
+    float y = dy * dy;
+    float z = dz * dz;
+    float s = x + y;
+    s = s + z;
+    return sqrtf(s);
+}
+
+/* gridToSpace Kernels.cu:124-127 */
+static inline float grid_pos(float o, int i, float dx)
+{
+    float t = (float)i * dx;
+    return o + t;
+}
+
+/* computeInOrOut Kernels.cu:344-392 (result is independent of traversal order) */
+static int in_or_out(const or_grid *G, const int *cs, const int *ce, const float *sx, float px,
+                     float py, float pz, float dx)
+{
+    const int gn = G->gn;
+    int cx = cell_of(px, G->origin[0], G->dx_n);
+    int cy = cell_of(py, G->origin[1], G->dx_n);
+    int cz = cell_of(pz, G->origin[2], G->dx_n);
+    int near_probe = 0;
+    for (int x = -1; x <= 1; x++) {
+        int ix = iclamp(cx + x, 0, gn - 1);
+        for (int y = -1; y <= 1; y++) {
+            int iy = iclamp(cy + y, 0, gn - 1);
+            for (int z = -1; z <= 1; z++) {
+                int iz = iclamp(cz + z, 0, gn - 1);
+                int h = (gn * gn * ix) + (gn * iy) + iz;
+                int s = cs[h], e = ce[h];
+                if (s < OR_EMPTY) {
+                    for (int id = s; id < e; id++) {
+                        const float *a = sx + 4 * (int64_t)id;
+                        float d = dist3(a[0], a[1], a[2], px, py, pz);
+                        if (d < a[3] - dx) return OR_IN;
+                        if (d < (OR_PROBE + a[3])) near_probe = 1;
+                    }
+                }
+            }
+        }
+    }
+    return near_probe ? OR_SHELL : OR_OUT;
+}
+
+/* probeIntersection Kernels.cu:394-475 on one slice covering the whole grid; the last layer of
+   each axis is never written and keeps its initial "outside" value (CudaSurf.cu:664).
+   Planes x in [x0, x1); labels indexed ((x - x0) * g + y) * g + z. */
+void or_pass1_range(const or_grid *G, const int *cs, const int *ce, const float *sx, int x0, int x1, uint8_t *labels)
+{
+    const int g = G->g;
+    const float dx = G->reso;
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int i = x0; i < x1; i++) {
+        for (int j = 0; j < g; j++) {
+            uint8_t *row = labels + ((int64_t)(i - x0) * g + j) * g;
+            for (int k = 0; k < g; k++) {
+                if (i >= g - 1 || j >= g - 1 || k >= g - 1) {
+                    row[k] = OR_OUT;
+                    continue;
+                }
+                float px = grid_pos(G->origin[0], i, dx);
+                float py = grid_pos(G->origin[1], j, dx);
+                float pz = grid_pos(G->origin[2], k, dx);
+                row[k] = (uint8_t)in_or_out(G, cs, ce, sx, px, py, pz, dx);
+            }
+        }
+    }
+}
+
+void or_pass1(const or_grid *G, const int *cs, const int *ce, const float *sx, uint8_t *labels)
+{
+    or_pass1_range(G, cs, ce, sx, 0, G->g, labels);
+}
+
+/* distanceFieldRefine Kernels.cu:478-545.  The reference reads the field in place; a voxel
+   counts as outside when its value exceeds probe-eps, which only pass-1 "outside" voxels do
+   (refined values are <= probe-dx), so the pass-1 labels give the same predicate.
+   labels cover planes [lx0, lx1) (which must hold [x0-r, x1+r) clipped to the grid); field
+   covers planes [x0, x1). */
+void or_pass2_range(const or_grid *G, const uint8_t *labels, int lx0, int lx1, int x0, int x1, float *field)
+{
+    const int g = G->g, r = G->r;
+    const float dx = G->reso;
+    const float *o = G->origin;
+    (void)lx1;
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int i = x0; i < x1; i++) {
+        for (int j = 0; j < g; j++) {
+            int64_t lbase = ((int64_t)(i - lx0) * g + j) * g;
+            int64_t fbase = ((int64_t)(i - x0) * g + j) * g;
+            for (int k = 0; k < g; k++) {
+                uint8_t l = labels[lbase + k];
+                if (l == OR_OUT) { field[fbase + k] = OR_PROBE; continue; }
+                if (l == OR_IN) { field[fbase + k] = -dx; continue; }
+                float vx = grid_pos(o[0], i, dx), vy = grid_pos(o[1], j, dx), vz = grid_pos(o[2], k, dx);
+                float min_dist = 100000.0f;
+                for (int a = -r; a <= r; a++) {
+                    int qx = iclamp(i + a, 0, g - 1);
+                    for (int b = -r; b <= r; b++) {
+                        int qy = iclamp(j + b, 0, g - 1);
+                        for (int c = -r; c <= r; c++) {
+                            int qz = iclamp(k + c, 0, g - 1);
+                            if (labels[((int64_t)(qx - lx0) * g + qy) * g + qz] != OR_OUT) continue;
+                            float d = dist3(grid_pos(o[0], qx, dx), grid_pos(o[1], qy, dx),
+                                            grid_pos(o[2], qz, dx), vx, vy, vz);
+                            min_dist = fminf(d, min_dist);
+                        }
+                    }
+                }
+                field[fbase + k] = (min_dist < 999.0f) ? (OR_PROBE - min_dist) : -dx;
+            }
+        }
+    }
+}
+
+void or_pass2(const or_grid *G, const uint8_t *labels, float *field)
+{
+    or_pass2_range(G, labels, 0, G->g, 0, G->g, field);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* marching cubes                                                                               */
+static const int CORNER_OFF[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                     {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+static const int EDGE_CORNERS[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                        {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+/* low end point and axis of each cube edge */
+static const int EDGE_LOW[12][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 1},
+                                    {0, 1, 1}, {0, 0, 1}, {0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}};
+static const int EDGE_AXIS[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+
+static inline int tri_entry(int ci, int k) { return (int)((QS_TRI_PACKED[ci] >> (4 * k)) & 0xF); }
+static inline int tri_count(int ci)
+{
+    int k = 0;
+    while (k < 16 && tri_entry(ci, k) != 0xF) k++;
+    return k / 3;
+}
+
+/* countVertexPerCell MarchingCubes.cu:571-658 (cube index) */
+void or_mc_classify(int g, int lim, const float *field, uint8_t *cubeidx)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < g; i++) {
+        for (int j = 0; j < g; j++) {
+            for (int k = 0; k < g; k++) {
+                int64_t id = ((int64_t)i * g + j) * g + k;
+                if (i >= lim || j >= lim || k >= lim) { cubeidx[id] = 0; continue; }
+                int ci = 0;
+                for (int c = 0; c < 8; c++) {
+                    int x = i + CORNER_OFF[c][0], y = j + CORNER_OFF[c][1], z = k + CORNER_OFF[c][2];
+                    float f = field[((int64_t)x * g + y) * g + z];
+                    ci += ((unsigned)(f < 0.0f)) << c;
+                }
+                cubeidx[id] = (uint8_t)ci;
+            }
+        }
+    }
+}
+
+void or_mesh_free(or_mesh *m)
+{
+    free(m->vert);
+    free(m->tri);
+    free(m->atom_id);
+    memset(m, 0, sizeof(*m));
+}
+
+/* edge (p, axis) lies on an emitted cube iff p[axis] < lim and the other coordinates <= lim */
+static inline int edge_in_window(int x, int y, int z, int axis, int lim)
+{
+    int p[3] = {x, y, z};
+    for (int c = 0; c < 3; c++) {
+        if (c == axis) { if (p[c] >= lim) return 0; }
+        else if (p[c] > lim) return 0;
+    }
+    return 1;
+}
+
+#define FIELD_AT(F, g, x, y, z) (F)[(((int64_t)(x)) * (g) + (y)) * (g) + (z)]
+
+/* vertex slots per grid point: 0,1,2 = edge along x,y,z from this point; 3 = the point itself
+   (used when an active edge's non-negative end is exactly zero) */
+static uint8_t slot_mask(const float *F, int g, int lim, int x, int y, int z)
+{
+    uint8_t m = 0;
+    int p[3] = {x, y, z};
+    float f = FIELD_AT(F, g, x, y, z);
+    for (int a = 0; a < 3; a++) {
+        int q[3] = {x, y, z};
+        q[a]++;
+        if (q[a] < g && edge_in_window(x, y, z, a, lim)) {
+            float fq = FIELD_AT(F, g, q[0], q[1], q[2]);
+            if ((f < 0.0f) != (fq < 0.0f)) {
+                float nonneg = (f < 0.0f) ? fq : f;
+                if (nonneg != 0.0f) m |= (uint8_t)(1u << a);
+            }
+        }
+    }
+    if (f == 0.0f) {
+        for (int a = 0; a < 3 && !(m & 8); a++) {
+            for (int s = -1; s <= 1; s += 2) {
+                int q[3] = {p[0], p[1], p[2]};
+                q[a] += s;
+                if (q[a] < 0 || q[a] >= g) continue;
+                int lo[3] = {p[0], p[1], p[2]};
+                if (s < 0) lo[a] = q[a];
+                if (!edge_in_window(lo[0], lo[1], lo[2], a, lim)) continue;
+                if (FIELD_AT(F, g, q[0], q[1], q[2]) < 0.0f) { m |= 8; break; }
+            }
+        }
+    }
+    return m;
+}
+
+static inline int popc4(unsigned m) { return (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1) + ((m >> 3) & 1); }
+
+int or_mc_edge_weld(const or_grid *G, int lim, const float *F, or_mesh *m)
+{
+    memset(m, 0, sizeof(*m));
+    const int g = G->g;
+    const float dx = G->reso;
+    const int64_t V = (int64_t)g * g * g;
+    uint8_t *mask = (uint8_t *)malloc((size_t)V);
+    int64_t *base = (int64_t *)malloc((size_t)V * sizeof(int64_t));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < g; i++)
+        for (int j = 0; j < g; j++)
+            for (int k = 0; k < g; k++) mask[((int64_t)i * g + j) * g + k] = slot_mask(F, g, lim, i, j, k);
+    int64_t nv = 0;
+    for (int64_t p = 0; p < V; p++) {
+        base[p] = nv;
+        nv += popc4(mask[p]);
+    }
+    m->n_vert = nv;
+    m->vert = (float *)malloc((size_t)(nv ? nv : 1) * 3 * sizeof(float));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < g; i++) {
+        for (int j = 0; j < g; j++) {
+            for (int k = 0; k < g; k++) {
+                int64_t p = ((int64_t)i * g + j) * g + k;
+                unsigned mk = mask[p];
+                if (!mk) continue;
+                int64_t v = base[p];
+                float A[3] = {grid_pos(G->origin[0], i, dx), grid_pos(G->origin[1], j, dx),
+                              grid_pos(G->origin[2], k, dx)};
+                float f0 = F[p];
+                for (int a = 0; a < 3; a++) {
+                    if (!(mk & (1u << a))) continue;
+                    int q[3] = {i, j, k};
+                    q[a]++;
+                    float f1 = FIELD_AT(F, g, q[0], q[1], q[2]);
+                    float B[3] = {A[0], A[1], A[2]};
+                    B[a] = A[a] + dx;
+                    /* vertexInterp + lerp MarchingCubes.cu:715-734, edge oriented low -> high */
+                    float t = (0.0f - f0) / (f1 - f0);
+                    for (int c = 0; c < 3; c++) {
+                        float dd = B[c] - A[c];
+                        float s = t * dd;
+                        m->vert[3 * v + c] = A[c] + s;
+                    }
+                    v++;
+                }
+                if (mk & 8) {
+                    for (int c = 0; c < 3; c++) m->vert[3 * v + c] = A[c];
+                    v++;
+                }
+            }
+        }
+    }
+    /* triangles: cubes in linear order, table order within a cube */
+    int64_t nt = 0;
+    uint8_t *ci_arr = (uint8_t *)malloc((size_t)V);
+    or_mc_classify(g, lim, F, ci_arr);
+    for (int64_t p = 0; p < V; p++) nt += tri_count(ci_arr[p]);
+    m->n_tri = nt;
+    m->tri = (int *)malloc((size_t)(nt ? nt : 1) * 3 * sizeof(int));
+    int64_t t = 0;
+    for (int i = 0; i < lim; i++) {
+        for (int j = 0; j < lim; j++) {
+            for (int k = 0; k < lim; k++) {
+                int ci = ci_arr[((int64_t)i * g + j) * g + k];
+                int ntri = tri_count(ci);
+                for (int tt = 0; tt < 3 * ntri; tt++) {
+                    int e = tri_entry(ci, tt);
+                    int q[3] = {i + EDGE_LOW[e][0], j + EDGE_LOW[e][1], k + EDGE_LOW[e][2]};
+                    int a = EDGE_AXIS[e];
+                    int q2[3] = {q[0], q[1], q[2]};
+                    q2[a]++;
+                    float fq = FIELD_AT(F, g, q[0], q[1], q[2]);
+                    float fq2 = FIELD_AT(F, g, q2[0], q2[1], q2[2]);
+                    const int *nn = (fq < 0.0f) ? q2 : q;
+                    float fnn = (fq < 0.0f) ? fq2 : fq;
+                    int64_t vid;
+                    if (fnn == 0.0f) {
+                        int64_t pn = ((int64_t)nn[0] * g + nn[1]) * g + nn[2];
+                        vid = base[pn] + popc4(mask[pn] & 7u);
+                    } else {
+                        int64_t pq = ((int64_t)q[0] * g + q[1]) * g + q[2];
+                        vid = base[pq] + popc4(mask[pq] & ((1u << a) - 1u));
+                    }
+                    m->tri[3 * t + (tt % 3)] = (int)vid;
+                    if (tt % 3 == 2) t++;
+                }
+            }
+        }
+    }
+    free(ci_arr);
+    free(mask);
+    free(base);
+    return 0;
+}
+
+/* generateTriangleVerticesSMEM MarchingCubes.cu:736-851 (every triangle emitted, see D1) */
+int or_mc_soup(const or_grid *G, int lim, const float *F, float **soup_out, int64_t *n_out)
+{
+    const int g = G->g;
+    const float dx = G->reso;
+    const int64_t V = (int64_t)g * g * g;
+    uint8_t *ci_arr = (uint8_t *)malloc((size_t)V);
+    or_mc_classify(g, lim, F, ci_arr);
+    int64_t nt = 0;
+    for (int64_t p = 0; p < V; p++) nt += tri_count(ci_arr[p]);
+    float *soup = (float *)malloc((size_t)(nt ? nt : 1) * 9 * sizeof(float));
+    int64_t w = 0;
+    for (int i = 0; i < lim; i++) {
+        for (int j = 0; j < lim; j++) {
+            for (int k = 0; k < lim; k++) {
+                int ci = ci_arr[((int64_t)i * g + j) * g + k];
+                if (ci == 0 || ci == 255) continue;
+                float p[3] = {grid_pos(G->origin[0], i, dx), grid_pos(G->origin[1], j, dx),
+                              grid_pos(G->origin[2], k, dx)};
+                float v[8][3], f[8];
+                for (int c = 0; c < 8; c++) {
+                    for (int a = 0; a < 3; a++) v[c][a] = p[a] + (CORNER_OFF[c][a] ? dx : 0.0f);
+                    f[c] = FIELD_AT(F, g, i + CORNER_OFF[c][0], j + CORNER_OFF[c][1], k + CORNER_OFF[c][2]);
+                }
+                int ntri = tri_count(ci);
+                for (int tt = 0; tt < 3 * ntri; tt++) {
+                    int e = tri_entry(ci, tt);
+                    int c0 = EDGE_CORNERS[e][0], c1 = EDGE_CORNERS[e][1];
+                    float t = (0.0f - f[c0]) / (f[c1] - f[c0]);
+                    for (int a = 0; a < 3; a++) {
+                        float dd = v[c1][a] - v[c0][a];
+                        float s = t * dd;
+                        float val = v[c0][a] + s;
+                        soup[3 * w + a] = val + 0.0f; /* + offsetPos, zero for one slice */
+                    }
+                    w++;
+                }
+            }
+        }
+    }
+    free(ci_arr);
+    *soup_out = soup;
+    *n_out = w;
+    return 0;
+}
+
+/* groupVertices MarchingCubes.cu:853-862 then thrust sort/unique/lower_bound CudaSurf.cu:444-457 */
+static int cmp_f3(const void *a, const void *b)
+{
+    const float *x = (const float *)a, *y = (const float *)b;
+    for (int c = 0; c < 3; c++) {
+        if (x[c] < y[c]) return -1;
+        if (y[c] < x[c]) return 1;
+    }
+    return 0;
+}
+
+int or_quantised_weld(const float *soup, int64_t n, or_mesh *m)
+{
+    memset(m, 0, sizeof(*m));
+    float *q = (float *)malloc((size_t)(n ? n : 1) * 3 * sizeof(float));
+    for (int64_t i = 0; i < 3 * n; i++) {
+        float s = soup[i] / OR_EPS;
+        q[i] = roundf(s) * OR_EPS;
+    }
+    float *u = (float *)malloc((size_t)(n ? n : 1) * 3 * sizeof(float));
+    memcpy(u, q, (size_t)n * 3 * sizeof(float));
+    qsort(u, (size_t)n, 3 * sizeof(float), cmp_f3);
+    int64_t nu = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (nu == 0 || cmp_f3(u + 3 * (nu - 1), u + 3 * i) != 0) {
+            memmove(u + 3 * nu, u + 3 * i, 3 * sizeof(float));
+            nu++;
+        }
+    }
+    m->n_vert = nu;
+    m->vert = u;
+    m->n_tri = n / 3;
+    m->tri = (int *)malloc((size_t)(n ? n : 1) * sizeof(int));
+    for (int64_t i = 0; i < n; i++) {
+        int64_t lo = 0, hi = nu;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) / 2;
+            if (cmp_f3(u + 3 * mid, q + 3 * i) < 0) lo = mid + 1;
+            else hi = mid;
+        }
+        m->tri[i] = (int)lo;
+    }
+    free(q);
+    return 0;
+}
+
+/* computeClosestAtom / closestAtomPerVertex Kernels.cu:547-604 */
+void or_closest_atom(const or_grid *G, const int *cs, const int *ce, const float *sx,
+                     const float *vert, int64_t nv, int *out)
+{
+    const int gn = G->gn;
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < nv; v++) {
+        float px = vert[3 * v], py = vert[3 * v + 1], pz = vert[3 * v + 2];
+        int cx = cell_of(px, G->origin[0], G->dx_n);
+        int cy = cell_of(py, G->origin[1], G->dx_n);
+        int cz = cell_of(pz, G->origin[2], G->dx_n);
+        int best = -1;
+        float min_d = 999999.0f;
+        for (int x = -1; x <= 1; x++) {
+            int ix = iclamp(cx + x, 0, gn - 1);
+            for (int y = -1; y <= 1; y++) {
+                int iy = iclamp(cy + y, 0, gn - 1);
+                for (int z = -1; z <= 1; z++) {
+                    int iz = iclamp(cz + z, 0, gn - 1);
+                    int h = (gn * gn * ix) + (gn * iy) + iz;
+                    int s = cs[h], e = ce[h];
+                    if (s < OR_EMPTY) {
+                        for (int id = s; id < e; id++) {
+                            const float *a = sx + 4 * (int64_t)id;
+                            float d = dist3(a[0], a[1], a[2], px, py, pz);
+                            if (d < min_d) { min_d = d; best = id; }
+                        }
+                    }
+                }
+            }
+        }
+        out[v] = best;
+    }
+}
+
+/* getNeighboorsVertices SmoothMesh.cpp:45-63 as CSR in push order, then smoothMeshLaplacian
+   SmoothMesh.cpp:96-124 */
+void or_smooth(float *vert, int64_t nv, const int *tri, int64_t nt, int times, int gs)
+{
+    if (times <= 0 || nv <= 0) return;
+    int64_t *start = (int64_t *)calloc((size_t)nv + 1, sizeof(int64_t));
+    for (int64_t t = 0; t < nt; t++)
+        for (int c = 0; c < 3; c++) start[tri[3 * t + c] + 1] += 2;
+    for (int64_t v = 0; v < nv; v++) start[v + 1] += start[v];
+    int *nb = (int *)malloc((size_t)(start[nv] ? start[nv] : 1) * sizeof(int));
+    int64_t *fill = (int64_t *)malloc((size_t)nv * sizeof(int64_t));
+    memcpy(fill, start, (size_t)nv * sizeof(int64_t));
+    for (int64_t t = 0; t < nt; t++) {
+        int x = tri[3 * t], y = tri[3 * t + 1], z = tri[3 * t + 2];
+        nb[fill[x]++] = y; nb[fill[x]++] = z;
+        nb[fill[y]++] = x; nb[fill[y]++] = z;
+        nb[fill[z]++] = x; nb[fill[z]++] = y;
+    }
+    free(fill);
+    float *src = gs ? vert : (float *)malloc((size_t)nv * 3 * sizeof(float));
+    for (int it = 0; it < times; it++) {
+        if (!gs) memcpy(src, vert, (size_t)nv * 3 * sizeof(float));
+        for (int64_t i = 0; i < nv; i++) {
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            int cnt = (int)(start[i + 1] - start[i]);
+            for (int64_t n = start[i]; n < start[i + 1]; n++) {
+                const float *q = src + 3 * (int64_t)nb[n];
+                sx = sx + q[0];
+                sy = sy + q[1];
+                sz = sz + q[2];
+            }
+            vert[3 * i + 0] = sx / (float)cnt;
+            vert[3 * i + 1] = sy / (float)cnt;
+            vert[3 * i + 2] = sz / (float)cnt;
+        }
+    }
+    if (!gs) free(src);
+    free(nb);
+    free(start);
+}
+
+int64_t or_filter_degenerate(int *tri, int64_t nt)
+{
+    int64_t w = 0;
+    for (int64_t t = 0; t < nt; t++) {
+        int x = tri[3 * t], y = tri[3 * t + 1], z = tri[3 * t + 2];
+        if (x != y && y != z && x != z) {
+            tri[3 * w] = x; tri[3 * w + 1] = y; tri[3 * w + 2] = z;
+            w++;
+        }
+    }
+    return w;
+}
+
+int or_compute_ses(const float *pos, const float *rad, int n, float reso, int smooth_times,
+                   const or_options *opt, or_mesh *out)
+{
+    memset(out, 0, sizeof(*out));
+    or_grid G;
+    if (or_grid_setup(pos, rad, n, reso, &G)) return 0;
+    const int64_t V = (int64_t)G.g * G.g * G.g;
+    const int64_t ncell = (int64_t)G.gn * G.gn * G.gn;
+    int *hash = (int *)malloc((size_t)n * sizeof(int));
+    int *perm = (int *)malloc((size_t)n * sizeof(int));
+    int *cs = (int *)malloc((size_t)ncell * sizeof(int));
+    int *ce = (int *)malloc((size_t)ncell * sizeof(int));
+    float *sx = (float *)malloc((size_t)n * 4 * sizeof(float));
+    or_hash(&G, pos, n, hash);
+    or_sort_cells(&G, pos, rad, n, hash, perm, cs, ce, sx);
+    uint8_t *labels = (uint8_t *)malloc((size_t)V);
+    float *field = (float *)malloc((size_t)V * sizeof(float));
+    or_pass1(&G, cs, ce, sx, labels);
+    or_pass2(&G, labels, field);
+    free(labels);
+    int lim = opt->lim_mode ? (G.g - 2 * G.r - 1) : (G.g - 1);
+    if (lim < 0) lim = 0;
+    if (opt->weld == 0) {
+        or_mc_edge_weld(&G, lim, field, out);
+    } else {
+        float *soup;
+        int64_t ns;
+        or_mc_soup(&G, lim, field, &soup, &ns);
+        or_quantised_weld(soup, ns, out);
+        free(soup);
+    }
+    free(field);
+    out->atom_id = (int *)malloc((size_t)(out->n_vert ? out->n_vert : 1) * sizeof(int));
+    or_closest_atom(&G, cs, ce, sx, out->vert, out->n_vert, out->atom_id);
+    if (!opt->atom_ids_sorted)
+        for (int64_t v = 0; v < out->n_vert; v++)
+            if (out->atom_id[v] >= 0) out->atom_id[v] = perm[out->atom_id[v]];
+    or_smooth(out->vert, out->n_vert, out->tri, out->n_tri, smooth_times, opt->smooth_gs);
+    out->n_tri = or_filter_degenerate(out->tri, out->n_tri);
+    free(hash); free(perm); free(cs); free(ce); free(sx);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* FP64: the same mathematics with positions, radii and distances in double precision.          */
+void or_field_fp64(const or_grid *G, const float *pos, const float *rad, int n, uint8_t *lab,
+                   double *field)
+{
+    const int g = G->g, r = G->r, gn = G->gn;
+    const double dx = (double)G->reso, probe = (double)OR_PROBE;
+    const double ox = G->origin[0], oy = G->origin[1], oz = G->origin[2];
+    const double dn = (double)G->dx_n;
+    /* exact cell lists rebuilt in double: atom in cell floor((p-o)/dn) */
+    int64_t ncell = (int64_t)gn * gn * gn;
+    int *cnt = (int *)calloc((size_t)ncell + 1, sizeof(int));
+    int *cell = (int *)malloc((size_t)n * sizeof(int));
+    for (int a = 0; a < n; a++) {
+        int c[3];
+        for (int d = 0; d < 3; d++) {
+            double o = (double)G->origin[d];
+            c[d] = iclamp((int)floor(((double)pos[3 * a + d] - o) / dn), 0, gn - 1);
+        }
+        cell[a] = (c[0] * gn + c[1]) * gn + c[2];
+        cnt[cell[a] + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; c++) cnt[c + 1] += cnt[c];
+    int *ids = (int *)malloc((size_t)(n ? n : 1) * sizeof(int));
+    int *fill = (int *)malloc((size_t)ncell * sizeof(int));
+    memcpy(fill, cnt, (size_t)ncell * sizeof(int));
+    for (int a = 0; a < n; a++) ids[fill[cell[a]]++] = a;
+    free(fill);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < g; i++) {
+        for (int j = 0; j < g; j++) {
+            for (int k = 0; k < g; k++) {
+                int64_t id = ((int64_t)i * g + j) * g + k;
+                if (i >= g - 1 || j >= g - 1 || k >= g - 1) { lab[id] = OR_OUT; continue; }
+                double p[3] = {ox + i * dx, oy + j * dx, oz + k * dx};
+                int c[3];
+                for (int d = 0; d < 3; d++)
+                    c[d] = (int)floor((p[d] - (double)G->origin[d]) / dn);
+                int res = OR_OUT;
+                for (int x = -1; x <= 1 && res != OR_IN; x++)
+                    for (int y = -1; y <= 1 && res != OR_IN; y++)
+                        for (int z = -1; z <= 1 && res != OR_IN; z++) {
+                            int cx = c[0] + x, cy = c[1] + y, cz = c[2] + z;
+                            if (cx < 0 || cy < 0 || cz < 0 || cx >= gn || cy >= gn || cz >= gn) continue;
+                            int h = (cx * gn + cy) * gn + cz;
+                            for (int q = cnt[h]; q < cnt[h + 1]; q++) {
+                                int a = ids[q];
+                                double ddx = pos[3 * a] - p[0], ddy = pos[3 * a + 1] - p[1], ddz = pos[3 * a + 2] - p[2];
+                                double d = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                                if (d < (double)rad[a] - dx) { res = OR_IN; break; }
+                                if (d < probe + (double)rad[a]) res = OR_SHELL;
+                            }
+                        }
+                lab[id] = (uint8_t)res;
+            }
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < g; i++) {
+        for (int j = 0; j < g; j++) {
+            for (int k = 0; k < g; k++) {
+                int64_t id = ((int64_t)i * g + j) * g + k;
+                if (lab[id] == OR_OUT) { field[id] = probe; continue; }
+                if (lab[id] == OR_IN) { field[id] = -dx; continue; }
+                int best = -1;
+                for (int a = -r; a <= r; a++) {
+                    int qx = i + a;
+                    if (qx < 0 || qx >= g) continue;
+                    for (int b = -r; b <= r; b++) {
+                        int qy = j + b;
+                        if (qy < 0 || qy >= g) continue;
+                        for (int c = -r; c <= r; c++) {
+                            int qz = k + c;
+                            if (qz < 0 || qz >= g) continue;
+                            if (lab[((int64_t)qx * g + qy) * g + qz] != OR_OUT) continue;
+                            int n2 = a * a + b * b + c * c;
+                            if (best < 0 || n2 < best) best = n2;
+                        }
+                    }
+                }
+                field[id] = (best >= 0) ? probe - dx * sqrt((double)best) : -dx;
+            }
+        }
+    }
+    free(cnt); free(cell); free(ids);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* CPU timing sample: grid setup, hashing, cell order, pass 1 and pass 2 on `planes` x-planes
+   through the middle of the grid (with the pass-2 halo), then the cube index of every cube of
+   those planes.  Returns wall seconds; *voxels = planes * g^2. */
+#include <time.h>
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+double or_bench_slab(const float *pos, const float *rad, int n, float reso, int planes, int *g_out, int64_t *voxels)
+{
+    double t0 = now_s();
+    or_grid G;
+    if (or_grid_setup(pos, rad, n, reso, &G)) return -1.0;
+    const int g = G.g, r = G.r;
+    if (planes > g - 1) planes = g - 1;
+    int x0 = g / 2 - planes / 2, x1 = x0 + planes;
+    int lx0 = x0 - r < 0 ? 0 : x0 - r, lx1 = x1 + 1 + r > g ? g : x1 + 1 + r;
+    const int64_t ncell = (int64_t)G.gn * G.gn * G.gn;
+    int *hash = (int *)malloc((size_t)n * sizeof(int));
+    int *perm = (int *)malloc((size_t)n * sizeof(int));
+    int *cs = (int *)malloc((size_t)ncell * sizeof(int));
+    int *ce = (int *)malloc((size_t)ncell * sizeof(int));
+    float *sx = (float *)malloc((size_t)n * 4 * sizeof(float));
+    or_hash(&G, pos, n, hash);
+    or_sort_cells(&G, pos, rad, n, hash, perm, cs, ce, sx);
+    int64_t plane = (int64_t)g * g;
+    uint8_t *lab = (uint8_t *)malloc((size_t)(lx1 - lx0) * plane);
+    float *fld = (float *)malloc((size_t)(planes + 1) * plane * sizeof(float));
+    or_pass1_range(&G, cs, ce, sx, lx0, lx1, lab);
+    or_pass2_range(&G, lab, lx0, lx1, x0, x1 + 1, fld);
+    /* cube indices of planes [x0, x1): corners from fld planes 0..planes */
+    int64_t active = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : active)
+    for (int i = 0; i < planes; i++)
+        for (int j = 0; j < g - 1; j++)
+            for (int k = 0; k < g - 1; k++) {
+                int ci = 0;
+                for (int c = 0; c < 8; c++) {
+                    float f = fld[((int64_t)(i + CORNER_OFF[c][0]) * g + j + CORNER_OFF[c][1]) * g + k + CORNER_OFF[c][2]];
+                    ci += ((unsigned)(f < 0.0f)) << c;
+                }
+                active += (ci != 0 && ci != 255);
+            }
+    double t1 = now_s();
+    free(hash); free(perm); free(cs); free(ce); free(sx); free(lab); free(fld);
+    if (g_out) *g_out = g;
+    if (voxels) *voxels = (int64_t)planes * plane;
+    (void)active;
+    return t1 - t0;
+}

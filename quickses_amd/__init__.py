@@ -1,0 +1,233 @@
+"""quickses_amd: SES meshes on AMD Instinct MI355X.
+
+Python mirror of the reference's C ABI (CudaSurf.h:75-83) over libquickses.so.  The library
+holds every kernel; this module only marshals arrays.  It raises if the library is missing:
+there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libquickses.so")
+_lib = None
+
+PROBE_RADIUS = 1.4
+ELEMENT_RADIUS = {"O": 1.52, "C": 1.70, "N": 1.55, "H": 1.20, "S": 1.80, "P": 1.80, "X": 1.40}  # CudaSurf.h:64-73
+
+
+class QuickSESError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libquickses.so (built by quickses_amd.build); raises if it is not there."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise QuickSESError("libquickses.so not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(LIB_PATH)
+    f3p = ctypes.POINTER(ctypes.c_float)
+    u32p = ctypes.POINTER(ctypes.c_uint)
+    L.API_computeSES.argtypes = [ctypes.c_float, f3p, f3p, ctypes.c_uint, f3p, u32p, ctypes.POINTER(ctypes.c_int),
+                                 u32p, ctypes.c_int]
+    L.API_computeSES.restype = None
+    L.API_getTriangles.argtypes = [ctypes.c_bool]
+    L.API_getTriangles.restype = ctypes.POINTER(ctypes.c_int)
+    L.API_getVertices.argtypes = []
+    L.API_getVertices.restype = ctypes.POINTER(ctypes.c_float)
+    L.API_getAtomIdPerVert.argtypes = []
+    L.API_getAtomIdPerVert.restype = ctypes.POINTER(ctypes.c_int)
+    L.API_freeMesh.argtypes = []
+    L.API_freeMesh.restype = None
+    L.API_lastError.restype = ctypes.c_int
+    L.qs_last_error_message.restype = ctypes.c_char_p
+    L.qs_set_option.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    L.qs_set_option.restype = ctypes.c_int
+    L.qs_get_option.argtypes = [ctypes.c_char_p]
+    L.qs_get_option.restype = ctypes.c_int
+    vp = ctypes.c_void_p
+    L.qs_compute_device.argtypes = [vp, vp, ctypes.c_uint, ctypes.c_float, ctypes.c_int, u32p, u32p,
+                                    ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    L.qs_compute_device.restype = ctypes.c_int
+    L.qs_last_grid.argtypes = [f3p, ctypes.POINTER(ctypes.c_int)]
+    L.qs_stage_times.argtypes = [ctypes.c_char_p, ctypes.c_int, f3p, ctypes.c_int]
+    L.qs_stage_times.restype = ctypes.c_int
+    L.qs_fetch.argtypes = [ctypes.c_char_p, vp, ctypes.c_int64]
+    L.qs_fetch.restype = ctypes.c_int64
+    L.qs_stage_pass2.argtypes = [f3p, ctypes.POINTER(ctypes.c_int), vp, vp]
+    L.qs_stage_pass2.restype = ctypes.c_int
+    L.qs_stage_mesh.argtypes = [f3p, ctypes.POINTER(ctypes.c_int), vp, u32p, u32p, vp, ctypes.c_int64, vp,
+                                ctypes.c_int64]
+    L.qs_stage_mesh.restype = ctypes.c_int
+    for name in ("qs_synth_ball", "qs_synth_shell"):
+        fn = getattr(L, name)
+        fn.restype = ctypes.c_int
+    L.qs_synth_ball.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp]
+    L.qs_synth_shell.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 vp, vp]
+    L.qs_synth_jitter.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, vp, vp]
+    L.qs_synth_jitter.restype = ctypes.c_int
+    L.qs_synth_sizes.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
+    L.qs_synth_sizes.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _check(rc=None):
+    L = lib()
+    err = L.API_lastError()
+    if err or (rc is not None and rc != 0):
+        raise QuickSESError("quickses error %d: %s" % (err or rc, L.qs_last_error_message().decode()))
+
+
+def set_option(key, value):
+    if lib().qs_set_option(key.encode(), int(value)) != 0:
+        raise KeyError(key)
+
+
+def get_option(key):
+    return lib().qs_get_option(key.encode())
+
+
+def compute_ses(pos, rad, reso=0.5, smooth=1, invert_triangles=False):
+    """API_computeSES + getters + API_freeMesh (CudaSurf.cu:787-872).
+
+    pos: (N, 3) float32, rad: (N,) float32.  Returns (vertices (NV, 3) float32,
+    triangles (NT, 3) int32, atom_ids (NV,) int32)."""
+    L = lib()
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+    rad = np.ascontiguousarray(rad, dtype=np.float32).reshape(-1)
+    if pos.shape[0] != rad.shape[0]:
+        raise ValueError("pos and rad disagree on the atom count")
+    nv = ctypes.c_uint(0)
+    nt = ctypes.c_uint(0)
+    L.API_computeSES(ctypes.c_float(reso), _fptr(pos), _fptr(rad), pos.shape[0], None, ctypes.byref(nv), None,
+                     ctypes.byref(nt), int(smooth))
+    err = L.API_lastError()
+    try:
+        if err:
+            raise QuickSESError("quickses error %d: %s" % (err, L.qs_last_error_message().decode()))
+        tp = L.API_getTriangles(bool(invert_triangles))
+        vp = L.API_getVertices()
+        ap = L.API_getAtomIdPerVert()
+        V = np.ctypeslib.as_array(vp, shape=(max(nv.value, 1) * 3,))[: nv.value * 3].reshape(-1, 3).copy()
+        T = np.ctypeslib.as_array(tp, shape=(max(nt.value, 1),))[: nt.value].reshape(-1, 3).copy()
+        A = np.ctypeslib.as_array(ap, shape=(max(nv.value, 1),))[: nv.value].copy()
+    finally:
+        L.API_freeMesh()
+    return V, T, A
+
+
+def compute_device(d_pos, d_rad, n, reso=0.5, smooth=1):
+    """qs_compute_device on device pointers; returns (nvert, nidx, d_vert, d_tri, d_atom) with
+    device pointers owned by the library (valid until the next call)."""
+    L = lib()
+    nv = ctypes.c_uint(0)
+    ni = ctypes.c_uint(0)
+    pv, pt, pa = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    rc = L.qs_compute_device(ctypes.c_void_p(d_pos), ctypes.c_void_p(d_rad), int(n), ctypes.c_float(reso), int(smooth),
+                             ctypes.byref(nv), ctypes.byref(ni), ctypes.byref(pv), ctypes.byref(pt), ctypes.byref(pa))
+    if rc:
+        raise QuickSESError("quickses error %d: %s" % (rc, L.qs_last_error_message().decode()))
+    return nv.value, ni.value, pv.value, pt.value, pa.value
+
+
+def stage_times():
+    """{stage: ms} of the last call (option "timing" must be on)."""
+    L = lib()
+    names = ctypes.create_string_buffer(4096)
+    ms = (ctypes.c_float * 64)()
+    k = L.qs_stage_times(names, 4096, ms, 64)
+    keys = names.value.decode().split(",")[:k]
+    return {key: ms[i] for i, key in enumerate(keys)}
+
+
+def last_grid():
+    L = lib()
+    fp = (ctypes.c_float * 5)()
+    ip = (ctypes.c_int * 4)()
+    L.qs_last_grid(fp, ip)
+    return dict(origin=(fp[0], fp[1], fp[2]), dx=fp[3], dxn=fp[4], g=ip[0], gn=ip[1], r=ip[2], lim=ip[3])
+
+
+def fetch(what, dtype, count):
+    out = np.empty(count, dtype=dtype)
+    n = lib().qs_fetch(what.encode(), ctypes.c_void_p(out.ctypes.data), out.nbytes)
+    if n < 0:
+        raise QuickSESError("fetch of %s failed" % what)
+    return out[: n // out.itemsize]
+
+
+def _grid_params(grid):
+    fp = (ctypes.c_float * 5)(grid["origin"][0], grid["origin"][1], grid["origin"][2], grid["dx"], grid["dxn"])
+    ip = (ctypes.c_int * 4)(grid["g"], grid["gn"], grid["r"], grid["lim"])
+    return fp, ip
+
+
+def stage_pass2(grid, labels):
+    """Field pass 2 on caller-given labels (uint8[g^3])."""
+    fp, ip = _grid_params(grid)
+    labels = np.ascontiguousarray(labels, dtype=np.uint8)
+    field = np.empty(labels.shape, dtype=np.float32)
+    rc = lib().qs_stage_pass2(fp, ip, ctypes.c_void_p(labels.ctypes.data), ctypes.c_void_p(field.ctypes.data))
+    _check(rc)
+    return field
+
+
+def stage_mesh(grid, field, vert_cap=None, tri_cap=None):
+    """Marching cubes + edge-id weld on a caller-given field; unsmoothed, unfiltered."""
+    fp, ip = _grid_params(grid)
+    field = np.ascontiguousarray(field, dtype=np.float32)
+    g = grid["g"]
+    vert_cap = vert_cap or max(3 * 4 * g * g * 8, 1024)
+    tri_cap = tri_cap or max(3 * 8 * g * g * 8, 1024)
+    vert = np.empty(vert_cap, dtype=np.float32)
+    tri = np.empty(tri_cap, dtype=np.int32)
+    nv = ctypes.c_uint(0)
+    nt = ctypes.c_uint(0)
+    rc = lib().qs_stage_mesh(fp, ip, ctypes.c_void_p(field.ctypes.data), ctypes.byref(nv), ctypes.byref(nt),
+                             ctypes.c_void_p(vert.ctypes.data), vert_cap, ctypes.c_void_p(tri.ctypes.data), tri_cap)
+    if rc == 5:
+        return stage_mesh(grid, field, 3 * nv.value + 3, 3 * nt.value + 3)
+    _check(rc)
+    return vert[: 3 * nv.value].reshape(-1, 3).copy(), tri[: 3 * nt.value].reshape(-1, 3).copy()
+
+
+def synth_ball(seed, n, density=0.05, min_dist=1.2):
+    """n atoms uniform in a ball at `density` atoms/A^3 with min-distance rejection (seeded)."""
+    pos = np.empty((n, 3), dtype=np.float32)
+    rad = np.empty(n, dtype=np.float32)
+    rc = lib().qs_synth_ball(seed, n, density, min_dist, ctypes.c_void_p(pos.ctypes.data), ctypes.c_void_p(rad.ctypes.data))
+    if rc:
+        raise QuickSESError("synth_ball failed")
+    return pos, rad
+
+
+def synth_shell(seed, n, r_in, r_out, min_dist=1.2):
+    pos = np.empty((n, 3), dtype=np.float32)
+    rad = np.empty(n, dtype=np.float32)
+    rc = lib().qs_synth_shell(seed, n, r_in, r_out, min_dist, ctypes.c_void_p(pos.ctypes.data),
+                              ctypes.c_void_p(rad.ctypes.data))
+    if rc:
+        raise QuickSESError("synth_shell failed")
+    return pos, rad
+
+
+def synth_jitter(seed, base, sigma=0.3):
+    base = np.ascontiguousarray(base, dtype=np.float32)
+    out = np.empty_like(base)
+    lib().qs_synth_jitter(seed, base.shape[0], sigma, ctypes.c_void_p(base.ctypes.data), ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
+def synth_sizes(seed, count, lo=500, hi=20000):
+    out = np.empty(count, dtype=np.int32)
+    lib().qs_synth_sizes(seed, count, lo, hi, ctypes.c_void_p(out.ctypes.data))
+    return out

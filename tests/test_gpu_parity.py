@@ -1,0 +1,137 @@
+"""GPU path (libquickses.so through the C ABI) against the CPU oracle, same seeded inputs.
+
+Integer outputs (atom order, cell ranges, labels, cube indices, vertex and triangle ids, atom
+ids) must be identical; the field and vertex positions too, because both sides evaluate the
+same single-precision formulas with one rounding per operation.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _ball(qs, seed, n):
+    return qs.synth_ball(seed, n)
+
+
+def _gpu_mesh(qs, pos, rad, reso, smooth, **opts):
+    saved = {k: qs.get_option(k) for k in opts}
+    try:
+        for k, v in opts.items():
+            qs.set_option(k, v)
+        return qs.compute_ses(pos, rad, reso, smooth)
+    finally:
+        for k, v in saved.items():
+            qs.set_option(k, v)
+
+
+def _same_mesh(a, b):
+    Va, Ta, Aa = a
+    Vb, Tb, Ab = b
+    assert Va.shape == Vb.shape, (Va.shape, Vb.shape)
+    assert Ta.shape == Tb.shape, (Ta.shape, Tb.shape)
+    assert np.array_equal(Ta, Tb)
+    assert np.array_equal(Va.view(np.uint32), Vb.view(np.uint32)), np.abs(Va - Vb).max()
+    assert np.array_equal(Aa, Ab)
+
+
+@pytest.mark.parametrize("seed,n,reso,smooth", [(1, 2000, 0.5, 1), (11, 300, 0.2, 1), (12, 500, 0.3, 2),
+                                                (13, 800, 1.0, 1), (14, 1200, 0.5, 0)])
+def test_end_to_end_matches_oracle(qs, orc, seed, n, reso, smooth):
+    pos, rad = _ball(qs, seed, n)
+    print("case seed=%d n=%d reso=%.2f smooth=%d" % (seed, n, reso, smooth), flush=True)
+    gpu = qs.compute_ses(pos, rad, reso, smooth)
+    ref = orc.compute_ses(pos, rad, reso, smooth)
+    assert len(gpu[0]) > 0
+    _same_mesh(gpu, ref)
+
+
+def test_stages_match_oracle(qs, orc):
+    pos, rad = _ball(qs, 1, 2000)
+    print("case stages seed=1 n=2000", flush=True)
+    qs.compute_ses(pos, rad, 0.5, 1)
+    grid = qs.last_grid()
+    G, perm, cs, ce, sx, lab, fld = orc.field(pos, rad, 0.5)
+    assert (grid["g"], grid["gn"], grid["r"]) == (G.g, G.gn, G.r)
+    assert np.array_equal(np.array(grid["origin"], np.float32), G.origin)
+    assert np.array_equal(qs.fetch("perm", np.int32, len(rad)), perm)
+    cells = qs.fetch("cells", np.int32, 2 * G.gn ** 3).reshape(-1, 2)
+    assert np.array_equal(cells[:, 0], cs) and np.array_equal(cells[:, 1], ce)
+    assert np.array_equal(qs.fetch("labels", np.uint8, G.g ** 3), lab)
+    f = qs.fetch("field", np.float32, G.g ** 3)
+    assert np.array_equal(f.view(np.uint32), fld.view(np.uint32))
+
+
+def test_pass2_tile_equals_direct(qs, orc):
+    pos, rad = _ball(qs, 21, 600)
+    for reso in (0.2, 0.5):
+        G, perm, cs, ce, sx, lab, fld = orc.field(pos, rad, reso)
+        params = G.params(G.g - 1)
+        print("case pass2 reso=%.1f g=%d" % (reso, G.g), flush=True)
+        tile = qs.stage_pass2(params, lab)
+        qs.set_option("pass2_direct", 1)
+        try:
+            direct = qs.stage_pass2(params, lab)
+        finally:
+            qs.set_option("pass2_direct", 0)
+        assert np.array_equal(tile.view(np.uint32), fld.view(np.uint32))
+        assert np.array_equal(direct.view(np.uint32), fld.view(np.uint32))
+
+
+def test_stage_mesh_matches_oracle(qs, orc):
+    pos, rad = _ball(qs, 22, 700)
+    G, perm, cs, ce, sx, lab, fld = orc.field(pos, rad, 0.3)
+    for lim in (G.g - 1, G.g - 2 * G.r - 1):
+        print("case mesh lim=%d" % lim, flush=True)
+        V, T = qs.stage_mesh(G.params(lim), fld)
+        Vo, To = orc.edge_weld(G, lim, fld)
+        assert np.array_equal(T, To)
+        assert np.array_equal(V.view(np.uint32), Vo.view(np.uint32))
+
+
+def test_reference_conventions(qs, orc):
+    pos, rad = _ball(qs, 2, 900)
+    print("case conventions", flush=True)
+    gpu = _gpu_mesh(qs, pos, rad, 0.5, 1, lim_mode=1, atom_ids_sorted=1)
+    ref = orc.compute_ses(pos, rad, 0.5, 1, lim_mode=1, atom_ids_sorted=True)
+    _same_mesh(gpu, ref)
+
+
+def test_invert_triangles(qs):
+    pos, rad = _ball(qs, 3, 400)
+    V, T, A = qs.compute_ses(pos, rad, 0.5, 1)
+    V2, T2, A2 = qs.compute_ses(pos, rad, 0.5, 1, invert_triangles=True)
+    assert np.array_equal(T2[:, [1, 0, 2]], T)
+
+
+def test_small_and_invalid_inputs(qs):
+    for n in (0, 1):
+        pos = np.zeros((n, 3), np.float32)
+        rad = np.full(n, 1.5, np.float32)
+        V, T, A = qs.compute_ses(pos, rad, 0.5, 1)
+        assert len(V) == 0 and len(T) == 0
+    pos = np.array([[0, 0, 0], [1.0, 0, 0]], np.float32)
+    rad = np.array([1.7, 1.52], np.float32)
+    V, T, A = qs.compute_ses(pos, rad, 0.5, 1)
+    assert len(T) > 0 and set(np.unique(A)) <= {0, 1}
+    bad = pos.copy()
+    bad[1, 2] = np.nan
+    with pytest.raises(qs.QuickSESError):
+        qs.compute_ses(bad, rad, 0.5, 1)
+    with pytest.raises(qs.QuickSESError):
+        qs.compute_ses(pos, rad, -0.5, 1)
+
+
+def test_device_entry_point(qs):
+    from quickses_amd import hipmem
+
+    pos, rad = _ball(qs, 1, 2000)
+    ref = qs.compute_ses(pos, rad, 0.5, 1)
+    dp = hipmem.DeviceArray(pos)
+    dr = hipmem.DeviceArray(rad)
+    nv, ni, pv, pt, pa = qs.compute_device(dp.ptr, dr.ptr, len(rad), 0.5, 1)
+    assert nv == len(ref[0]) and ni == 3 * len(ref[1])
+    V = hipmem.download(pv, (nv, 3), np.float32)
+    T = hipmem.download(pt, (ni // 3, 3), np.int32)
+    A = hipmem.download(pa, (nv,), np.int32)
+    assert np.array_equal(V, ref[0]) and np.array_equal(T, ref[1]) and np.array_equal(A, ref[2])
