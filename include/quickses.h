@@ -64,7 +64,8 @@ const char *qs_last_error_message(void);
  *   "lim_mode"        0 = polygonise every cube whose corners are valid (default),
  *                     1 = stop at cube g-2r-2 as the reference's single slice does
  *   "atom_ids_sorted" 0 = input atom index (default), 1 = cell-sorted index (reference)
- *   "pass2_direct"    1 = evaluate field pass 2 with the (2r+1)^3 window kernel
+ *   "pass1_direct"    1 = evaluate field pass 1 per voxel over its 27 cells (cross-check)
+ *   "pass2_direct"    1 = evaluate field pass 2 with the (2r+1)^3 window kernel (cross-check)
  *   "trace"           1 = print per-stage GPU times as JSON lines on stderr
  *   "timing"          1 = record per-stage GPU times (hipEvents on the library's stream)
  * Returns 0, or -1 for an unknown key. */

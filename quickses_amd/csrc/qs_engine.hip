@@ -66,6 +66,7 @@ struct Options {
     int slice = 0;
     int lim_mode = 0;
     int atom_ids_sorted = 0;
+    int pass1_direct = 0;
     int pass2_direct = 0;
     int trace = 0;
     int timing = 0;
@@ -248,10 +249,10 @@ struct Engine {
 
     DevBuf in_pos, in_rad, bb_part, bb_out;
     DevBuf key, count, start, cursor, perm, cse, sx;
-    DevBuf labels, field, pinfo;
-    DevBuf bs_tri, bs_vert, bo_tri, bo_vert, totals;
-    DevBuf vbase, vert, vert2, vert3, tri, tri_out, atom;
-    DevBuf deg, istart, icursor, inc, keep, keeppos;
+    DevBuf labels, outm, shellm, field, negm, zerom;
+    DevBuf s4, vcnt, tcnt, vbw, tbw, apos, alist, totals;
+    DevBuf vert, vert2, vert3, vkey, tri, tri_out, atom;
+    DevBuf keep, keeppos;
     DevBuf scan_tmp;
     DevBuf t_offs, t_norm, t_end;
     OffsetTables tables;
@@ -315,7 +316,8 @@ struct Engine {
     }
 
     int run(const float *d_pos, const float *d_rad, unsigned n, float reso, int smooth, const BBox *host_bbox);
-    int mesh_from_field(unsigned n_for_trace, StageTimer &tm, bool with_atoms);
+    int mesh_from_field(StageTimer &tm, bool with_atoms);
+    int pass2_tables();
 };
 
 Engine g_eng;
@@ -372,7 +374,7 @@ int Engine::run(const float *d_pos, const float *d_rad, unsigned n, float reso, 
     /* ---- atoms into cells ---- */
     QS_TRY(ensure(key, (size_t)n * 4));
     QS_TRY(ensure(count, ncell * 4));
-    QS_TRY(ensure(start, ncell * 4));
+    QS_TRY(ensure(start, (ncell + 1) * 4));
     QS_TRY(ensure(cursor, ncell * 4));
     QS_TRY(ensure(perm, (size_t)n * 4));
     QS_TRY(ensure(cse, ncell * 8));
@@ -380,7 +382,8 @@ int Engine::run(const float *d_pos, const float *d_rad, unsigned n, float reso, 
     QS_HIP(hipMemsetAsync(count.p, 0, ncell * 4, stream));
     QS_HIP(hipMemsetAsync(cursor.p, 0, ncell * 4, stream));
     qs_launch_hash_count(d_pos, (int)n, G, as<int>(key), as<uint32_t>(count), stream);
-    QS_TRY(scan(as<uint32_t>(count), as<uint32_t>(start), ncell, nullptr));
+    /* start[c] = atoms in cells < c; start[ncell] = n */
+    QS_TRY(scan(as<uint32_t>(count), as<uint32_t>(start), ncell, as<uint32_t>(start) + ncell));
     qs_launch_scatter(as<int>(key), (int)n, as<uint32_t>(start), as<uint32_t>(cursor), as<int>(perm), stream);
     qs_launch_cell_finalize(as<uint32_t>(start), as<uint32_t>(count), (int)ncell, as<int>(perm), as<int2>(cse), stream);
     qs_launch_gather_atoms(as<int>(perm), d_pos, d_rad, (int)n, as<float4>(sx), stream);
@@ -388,59 +391,39 @@ int Engine::run(const float *d_pos, const float *d_rad, unsigned n, float reso, 
     tm.mark("cells");
 
     /* ---- field ---- */
-    QS_TRY(ensure(labels, V));
+    const int W = (g + 63) / 64;
+    const size_t nwords = (size_t)g * g * W;
+    QS_TRY(ensure(outm, nwords * 8));
+    QS_TRY(ensure(shellm, nwords * 8));
     QS_TRY(ensure(field, V * 4));
-    qs_launch_pass1(G, as<int2>(cse), as<float4>(sx), as<uint8_t>(labels), 0, g, stream);
+    if (g_opt.pass1_direct)
+        qs_launch_pass1_direct(G, as<int2>(cse), as<float4>(sx), as<uint64_t>(outm), as<uint64_t>(shellm), W, 0, g, stream);
+    else
+        qs_launch_pass1(G, as<uint32_t>(start), as<float4>(sx), as<uint64_t>(outm), as<uint64_t>(shellm), W, 0, g, stream);
     QS_HIP(hipGetLastError());
     tm.mark("pass1");
-    bool direct = g_opt.pass2_direct || qs_pass2_lds_bytes(G.r) > 65536;
-    if (!direct) {
-        float cmax = 0.0f;
-        for (float o : {G.ox, G.oy, G.oz}) cmax = std::max(cmax, std::max(std::fabs(o), std::fabs(o + (float)g * G.dx)));
-        cmax = std::max(cmax, (float)g * G.dx);
-        float cmax2 = 1.0f;
-        while (cmax2 < cmax) cmax2 *= 2.0f;
-        if (tables.r != G.r || tables.dx != G.dx || tables.cmax != cmax2) {
-            build_offset_tables(G.r, G.dx, cmax2, tables);
-            QS_TRY(ensure(t_offs, tables.offs.size() * 4));
-            QS_TRY(ensure(t_norm, tables.norm_start.size() * 4));
-            QS_TRY(ensure(t_end, tables.enum_end.size() * 4));
-            QS_HIP(hipMemcpyAsync(t_offs.p, tables.offs.data(), tables.offs.size() * 4, hipMemcpyHostToDevice, stream));
-            QS_HIP(hipMemcpyAsync(t_norm.p, tables.norm_start.data(), tables.norm_start.size() * 4, hipMemcpyHostToDevice, stream));
-            QS_HIP(hipMemcpyAsync(t_end.p, tables.enum_end.data(), tables.enum_end.size() * 4, hipMemcpyHostToDevice, stream));
-            QS_HIP(hipStreamSynchronize(stream));
-        }
-        qs_launch_pass2_tile(G, as<uint8_t>(labels), as<float>(field), as<int>(t_offs), as<int>(t_norm), as<int>(t_end), 0,
-                             g, stream);
-    } else {
-        qs_launch_pass2_direct(G, as<uint8_t>(labels), as<float>(field), 0, g, stream);
-    }
+    QS_TRY(ensure(negm, nwords * 8));
+    QS_TRY(ensure(zerom, nwords * 8));
+    QS_TRY(pass2_tables());
+    qs_launch_pass2(G, as<uint64_t>(outm), as<uint64_t>(shellm), W, as<float>(field), as<uint32_t>(negm), as<uint32_t>(zerom),
+                    as<int>(t_offs), as<int>(t_norm), as<int>(t_end), 0, g, g_opt.pass2_direct != 0, stream);
     QS_HIP(hipGetLastError());
     tm.mark("pass2");
 
-    QS_TRY(mesh_from_field(n, tm, true));
+    QS_TRY(mesh_from_field(tm, true));
 
     /* ---- Laplacian passes (Jacobi) ---- */
     res_vert = as<float>(vert);
     if (smooth > 0 && nv > 0 && nt_all > 0) {
-        uint32_t nc = 3 * nt_all;
-        QS_TRY(ensure(deg, (size_t)nv * 4));
-        QS_TRY(ensure(istart, (size_t)nv * 4));
-        QS_TRY(ensure(icursor, (size_t)nv * 4));
-        QS_TRY(ensure(inc, (size_t)nc * 4));
+        const int W = (g + 63) / 64;
         QS_TRY(ensure(vert2, (size_t)nv * 12));
-        QS_HIP(hipMemsetAsync(deg.p, 0, (size_t)nv * 4, stream));
-        QS_HIP(hipMemsetAsync(icursor.p, 0, (size_t)nv * 4, stream));
-        qs_launch_inc_count(as<int>(tri), nc, as<uint32_t>(deg), stream);
-        QS_TRY(scan(as<uint32_t>(deg), as<uint32_t>(istart), nv, nullptr));
-        qs_launch_inc_fill(as<int>(tri), nc, as<uint32_t>(istart), as<uint32_t>(icursor), as<uint32_t>(inc), stream);
-        qs_launch_inc_sort(as<uint32_t>(istart), as<uint32_t>(deg), nv, as<uint32_t>(inc), stream);
         /* `vert` keeps the unsmoothed positions; passes alternate between vert2 and vert3 */
         if (smooth > 1) QS_TRY(ensure(vert3, (size_t)nv * 12));
         const float *src = as<float>(vert);
         float *dst = as<float>(vert2);
         for (int it = 0; it < smooth; it++) {
-            qs_launch_jacobi(as<int>(tri), as<uint32_t>(istart), as<uint32_t>(deg), as<uint32_t>(inc), nv, src, dst, stream);
+            qs_launch_smooth(G, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw), W,
+                             as<uint64_t>(vkey), nv, src, dst, stream);
             src = dst;
             dst = (dst == as<float>(vert2)) ? as<float>(vert3) : as<float>(vert2);
         }
@@ -468,36 +451,39 @@ int Engine::run(const float *d_pos, const float *d_rad, unsigned n, float reso, 
     return 0;
 }
 
-/* marching cubes on `field` (grid G), results into vert / tri / atom */
-int Engine::mesh_from_field(unsigned n_for_trace, StageTimer &tm, bool with_atoms)
+/* marching cubes on the sign bits of `field` (grid G), results into vert / vkey / tri / atom */
+int Engine::mesh_from_field(StageTimer &tm, bool with_atoms)
 {
-    (void)n_for_trace;
-    const int g = G.g;
-    const uint64_t V = (uint64_t)g * g * g;
-    const unsigned nb = qs_mc_blocks(V);
-    QS_TRY(ensure(pinfo, V * 2));
-    QS_TRY(ensure(bs_tri, (size_t)nb * 4));
-    QS_TRY(ensure(bs_vert, (size_t)nb * 4));
-    QS_TRY(ensure(bo_tri, (size_t)nb * 4));
-    QS_TRY(ensure(bo_vert, (size_t)nb * 4));
+    const int g = G.g, W = (g + 63) / 64;
+    const uint64_t nw = (uint64_t)g * g * W;
+    QS_TRY(ensure(s4, nw * 32));
+    QS_TRY(ensure(vcnt, nw * 4));
+    QS_TRY(ensure(tcnt, nw * 4));
+    QS_TRY(ensure(vbw, nw * 4));
+    QS_TRY(ensure(tbw, nw * 4));
+    QS_TRY(ensure(apos, nw * 4));
+    QS_TRY(ensure(alist, nw * 4));
     QS_TRY(ensure(totals, 16));
-    qs_launch_mc_count(G, as<float>(field), as<uint16_t>(pinfo), as<uint32_t>(bs_tri), as<uint32_t>(bs_vert), 0, V, stream);
-    QS_TRY(scan(as<uint32_t>(bs_tri), as<uint32_t>(bo_tri), nb, as<uint32_t>(totals)));
-    QS_TRY(scan(as<uint32_t>(bs_vert), as<uint32_t>(bo_vert), nb, as<uint32_t>(totals) + 1));
-    uint32_t tot[2];
-    QS_HIP(hipMemcpyAsync(tot, totals.p, 8, hipMemcpyDeviceToHost, stream));
+    qs_launch_mc_count(G, as<uint64_t>(negm), as<uint64_t>(zerom), W, as<uint64_t>(s4), as<uint32_t>(vcnt), as<uint32_t>(tcnt),
+                       0, g, stream);
+    QS_TRY(scan(as<uint32_t>(tcnt), as<uint32_t>(tbw), nw, as<uint32_t>(totals)));
+    QS_TRY(scan(as<uint32_t>(vcnt), as<uint32_t>(vbw), nw, as<uint32_t>(totals) + 1));
+    qs_launch_mc_flag(as<uint32_t>(vcnt), as<uint32_t>(tcnt), nw, as<uint32_t>(apos), stream);
+    QS_TRY(scan(as<uint32_t>(apos), as<uint32_t>(apos), nw, as<uint32_t>(totals) + 2));
+    qs_launch_mc_active(as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint32_t>(apos), nw, as<uint32_t>(alist), stream);
+    uint32_t tot[3];
+    QS_HIP(hipMemcpyAsync(tot, totals.p, 12, hipMemcpyDeviceToHost, stream));
     QS_HIP(hipStreamSynchronize(stream));
     tm.mark("mc_count");
     nt_all = tot[0];
     nv = tot[1];
+    const uint32_t nact = tot[2];
     if ((uint64_t)nt_all * 3 >= (1ull << 31) || nv >= (1u << 31)) return set_error(4, "mesh too large");
-    QS_TRY(ensure(vbase, V * 4));
     QS_TRY(ensure(vert, (size_t)std::max<uint32_t>(nv, 1) * 12));
+    QS_TRY(ensure(vkey, (size_t)std::max<uint32_t>(nv, 1) * 8));
     QS_TRY(ensure(tri, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
-    qs_launch_mc_verts(G, as<float>(field), as<uint16_t>(pinfo), as<uint32_t>(bo_vert), as<uint32_t>(vbase), as<float>(vert),
-                       0, V, stream);
-    qs_launch_mc_tris(G, as<float>(field), as<uint16_t>(pinfo), as<uint32_t>(vbase), as<uint32_t>(bo_tri), as<int>(tri), 0, V,
-                      stream);
+    qs_launch_mc_emit(G, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw), W, as<float>(field),
+                      as<uint32_t>(alist), nact, as<uint32_t>(tbw), as<float>(vert), as<uint64_t>(vkey), as<int>(tri), stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
     if (with_atoms) {
@@ -507,6 +493,27 @@ int Engine::mesh_from_field(unsigned n_for_trace, StageTimer &tm, bool with_atom
         QS_HIP(hipGetLastError());
         tm.mark("closest_atom");
     }
+    return 0;
+}
+
+/* candidate tables of field pass 2 for the current grid (built when r, dx or the coordinate
+   range change) */
+int Engine::pass2_tables()
+{
+    float cmax = 0.0f;
+    for (float o : {G.ox, G.oy, G.oz}) cmax = std::max(cmax, std::max(std::fabs(o), std::fabs(o + (float)G.g * G.dx)));
+    cmax = std::max(cmax, (float)G.g * G.dx);
+    float cmax2 = 1.0f;
+    while (cmax2 < cmax) cmax2 *= 2.0f;
+    if (tables.r == G.r && tables.dx == G.dx && tables.cmax == cmax2) return 0;
+    build_offset_tables(G.r, G.dx, cmax2, tables);
+    QS_TRY(ensure(t_offs, tables.offs.size() * 4));
+    QS_TRY(ensure(t_norm, tables.norm_start.size() * 4));
+    QS_TRY(ensure(t_end, tables.enum_end.size() * 4));
+    QS_HIP(hipMemcpyAsync(t_offs.p, tables.offs.data(), tables.offs.size() * 4, hipMemcpyHostToDevice, stream));
+    QS_HIP(hipMemcpyAsync(t_norm.p, tables.norm_start.data(), tables.norm_start.size() * 4, hipMemcpyHostToDevice, stream));
+    QS_HIP(hipMemcpyAsync(t_end.p, tables.enum_end.data(), tables.enum_end.size() * 4, hipMemcpyHostToDevice, stream));
+    QS_HIP(hipStreamSynchronize(stream));
     return 0;
 }
 
@@ -532,6 +539,7 @@ int qs_set_option(const char *key, int value)
     if (k == "slice") g_opt.slice = value;
     else if (k == "lim_mode") g_opt.lim_mode = value;
     else if (k == "atom_ids_sorted") g_opt.atom_ids_sorted = value;
+    else if (k == "pass1_direct") g_opt.pass1_direct = value;
     else if (k == "pass2_direct") g_opt.pass2_direct = value;
     else if (k == "trace") g_opt.trace = value;
     else if (k == "timing") g_opt.timing = value;
@@ -545,6 +553,7 @@ int qs_get_option(const char *key)
     if (k == "slice") return g_opt.slice;
     if (k == "lim_mode") return g_opt.lim_mode;
     if (k == "atom_ids_sorted") return g_opt.atom_ids_sorted;
+    if (k == "pass1_direct") return g_opt.pass1_direct;
     if (k == "pass2_direct") return g_opt.pass2_direct;
     if (k == "trace") return g_opt.trace;
     if (k == "timing") return g_opt.timing;
@@ -728,7 +737,22 @@ int64_t qs_fetch(const char *what, void *host, int64_t bytes)
     uint64_t nc = (uint64_t)G.gn * G.gn * G.gn;
     const void *src = nullptr;
     int64_t need = 0;
-    if (w == "labels") { src = E.labels.p; need = (int64_t)V; }
+    if (w == "labels") {
+        if (bytes < (int64_t)V || !E.outm.p) return -1;
+        const int g = G.g, W = (g + 63) / 64;
+        size_t nw = (size_t)g * g * W;
+        std::vector<uint64_t> mo(nw), ms(nw);
+        if (hipMemcpy(mo.data(), E.outm.p, nw * 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        if (hipMemcpy(ms.data(), E.shellm.p, nw * 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        uint8_t *out = (uint8_t *)host;
+        for (size_t row = 0; row < (size_t)g * g; row++)
+            for (int z = 0; z < g; z++) {
+                uint64_t bit = 1ull << (z & 63);
+                size_t wi = row * W + (z >> 6);
+                out[row * g + z] = (mo[wi] & bit) ? QS_OUT : ((ms[wi] & bit) ? QS_SHELL : QS_IN);
+            }
+        return (int64_t)V;
+    }
     else if (w == "field") { src = E.field.p; need = (int64_t)V * 4; }
     else if (w == "perm") { src = E.perm.p; need = (int64_t)E.n_atoms * 4; }
     else if (w == "cells") { src = E.cse.p; need = (int64_t)nc * 8; }
@@ -761,26 +785,21 @@ int qs_stage_pass2(const float *fp, const int *ip, const uint8_t *labels, float 
     QsGrid G;
     grid_from_params(fp, ip, G);
     uint64_t V = (uint64_t)G.g * G.g * G.g;
+    const int W = (G.g + 63) / 64;
+    const size_t nwords = (size_t)G.g * G.g * W;
     QS_TRY(ensure(E.labels, V));
+    QS_TRY(ensure(E.outm, nwords * 8));
+    QS_TRY(ensure(E.shellm, nwords * 8));
     QS_TRY(ensure(E.field, V * 4));
     QS_HIP(hipMemcpy(E.labels.p, labels, V, hipMemcpyHostToDevice));
-    bool direct = g_opt.pass2_direct || qs_pass2_lds_bytes(G.r) > 65536;
-    if (!direct) {
-        float cmax = 0.0f;
-        for (float o : {G.ox, G.oy, G.oz}) cmax = std::max(cmax, std::max(std::fabs(o), std::fabs(o + (float)G.g * G.dx)));
-        cmax = std::max(cmax, (float)G.g * G.dx);
-        build_offset_tables(G.r, G.dx, cmax, E.tables);
-        QS_TRY(ensure(E.t_offs, E.tables.offs.size() * 4));
-        QS_TRY(ensure(E.t_norm, E.tables.norm_start.size() * 4));
-        QS_TRY(ensure(E.t_end, E.tables.enum_end.size() * 4));
-        QS_HIP(hipMemcpy(E.t_offs.p, E.tables.offs.data(), E.tables.offs.size() * 4, hipMemcpyHostToDevice));
-        QS_HIP(hipMemcpy(E.t_norm.p, E.tables.norm_start.data(), E.tables.norm_start.size() * 4, hipMemcpyHostToDevice));
-        QS_HIP(hipMemcpy(E.t_end.p, E.tables.enum_end.data(), E.tables.enum_end.size() * 4, hipMemcpyHostToDevice));
-        qs_launch_pass2_tile(G, as<uint8_t>(E.labels), as<float>(E.field), as<int>(E.t_offs), as<int>(E.t_norm),
-                             as<int>(E.t_end), 0, G.g, E.stream);
-    } else {
-        qs_launch_pass2_direct(G, as<uint8_t>(E.labels), as<float>(E.field), 0, G.g, E.stream);
-    }
+    qs_launch_labels_to_masks(as<uint8_t>(E.labels), G.g, W, as<uint64_t>(E.outm), as<uint64_t>(E.shellm), E.stream);
+    QS_TRY(ensure(E.negm, nwords * 8));
+    QS_TRY(ensure(E.zerom, nwords * 8));
+    E.G = G;
+    QS_TRY(E.pass2_tables());
+    qs_launch_pass2(G, as<uint64_t>(E.outm), as<uint64_t>(E.shellm), W, as<float>(E.field), as<uint32_t>(E.negm),
+                    as<uint32_t>(E.zerom), as<int>(E.t_offs), as<int>(E.t_norm), as<int>(E.t_end), 0, G.g,
+                    g_opt.pass2_direct != 0, E.stream);
     QS_HIP(hipGetLastError());
     QS_HIP(hipStreamSynchronize(E.stream));
     QS_HIP(hipMemcpy(field, E.field.p, V * 4, hipMemcpyDeviceToHost));
@@ -797,8 +816,13 @@ int qs_stage_mesh(const float *fp, const int *ip, const float *field, unsigned i
     uint64_t V = (uint64_t)E.G.g * E.G.g * E.G.g;
     QS_TRY(ensure(E.field, V * 4));
     QS_HIP(hipMemcpy(E.field.p, field, V * 4, hipMemcpyHostToDevice));
+    const int W = (E.G.g + 63) / 64;
+    const size_t nwords = (size_t)E.G.g * E.G.g * W;
+    QS_TRY(ensure(E.negm, nwords * 8));
+    QS_TRY(ensure(E.zerom, nwords * 8));
+    qs_launch_field_signs(as<float>(E.field), E.G.g, W, as<uint32_t>(E.negm), as<uint32_t>(E.zerom), E.stream);
     StageTimer tm;
-    QS_TRY(E.mesh_from_field(0, tm, false));
+    QS_TRY(E.mesh_from_field(tm, false));
     QS_HIP(hipStreamSynchronize(E.stream));
     if (nvert) *nvert = E.nv;
     if (ntri) *ntri = E.nt_all;

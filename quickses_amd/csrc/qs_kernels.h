@@ -12,26 +12,28 @@ void qs_launch_hash_count(const float *pos, int n, QsGrid G, int *key, uint32_t 
 void qs_launch_scatter(const int *key, int n, const uint32_t *start, uint32_t *cursor, int *perm, hipStream_t s);
 void qs_launch_cell_finalize(const uint32_t *start, const uint32_t *count, int ncell, int *perm, int2 *cse, hipStream_t s);
 void qs_launch_gather_atoms(const int *perm, const float *pos, const float *rad, int n, float4 *sx, hipStream_t s);
-void qs_launch_pass1(QsGrid G, const int2 *cse, const float4 *sx, uint8_t *labels, int x0, int x1, hipStream_t s);
+void qs_launch_pass1(QsGrid G, const uint32_t *cstart, const float4 *sx, uint64_t *outm, uint64_t *shellm, int W, int x0,
+                     int x1, hipStream_t s);
+void qs_launch_pass1_direct(QsGrid G, const int2 *cse, const float4 *sx, uint64_t *outm, uint64_t *shellm, int W, int x0,
+                            int x1, hipStream_t s);
 size_t qs_pass2_lds_bytes(int r);
-void qs_launch_pass2_tile(QsGrid G, const uint8_t *labels, float *field, const int *offs, const int *norm_start,
-                          const int *enum_end, int x0, int x1, hipStream_t s);
-void qs_launch_pass2_direct(QsGrid G, const uint8_t *labels, float *field, int x0, int x1, hipStream_t s);
-unsigned qs_mc_blocks(uint64_t npoints);
-void qs_launch_mc_count(QsGrid G, const float *F, uint16_t *pinfo, uint32_t *bs_tri, uint32_t *bs_vert, uint64_t p0,
-                        uint64_t p1, hipStream_t s);
-void qs_launch_mc_verts(QsGrid G, const float *F, const uint16_t *pinfo, const uint32_t *bo_vert, uint32_t *vbase,
-                        float *vert, uint64_t p0, uint64_t p1, hipStream_t s);
-void qs_launch_mc_tris(QsGrid G, const float *F, const uint16_t *pinfo, const uint32_t *vbase, const uint32_t *bo_tri,
-                       int *tri, uint64_t p0, uint64_t p1, hipStream_t s);
+int qs_launch_pass2(QsGrid G, const uint64_t *outm, const uint64_t *shellm, int W, float *field, uint32_t *negm,
+                    uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end, int x0, int x1, bool direct,
+                    hipStream_t s);
+void qs_launch_field_signs(const float *F, int g, int W, uint32_t *negm, uint32_t *zerom, hipStream_t s);
+void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *outm, uint64_t *shellm, hipStream_t s);
+void qs_launch_mc_count(QsGrid G, const uint64_t *N, const uint64_t *Zm, int W, uint64_t *S4, uint32_t *vcnt, uint32_t *tcnt,
+                        int x0, int x1, hipStream_t s);
+void qs_launch_mc_flag(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *flag, hipStream_t s);
+void qs_launch_mc_active(const uint32_t *vcnt, const uint32_t *tcnt, const uint32_t *apos, uint64_t nw, uint32_t *list,
+                         hipStream_t s);
+void qs_launch_mc_emit(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W,
+                       const float *F, const uint32_t *list, uint32_t nact, const uint32_t *tbw, float *vert, uint64_t *vkey,
+                       int *tri, hipStream_t s);
+void qs_launch_smooth(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W,
+                      const uint64_t *vkey, uint32_t nv, const float *src, float *dst, hipStream_t s);
 void qs_launch_closest_atom(QsGrid G, const int2 *cse, const float4 *sx, const int *perm, const float *vert, uint32_t nv,
                             int sorted_ids, int *out, hipStream_t s);
-void qs_launch_inc_count(const int *tri, uint32_t ncorner, uint32_t *deg, hipStream_t s);
-void qs_launch_inc_fill(const int *tri, uint32_t ncorner, const uint32_t *start, uint32_t *cursor, uint32_t *inc,
-                        hipStream_t s);
-void qs_launch_inc_sort(const uint32_t *start, const uint32_t *deg, uint32_t nv, uint32_t *inc, hipStream_t s);
-void qs_launch_jacobi(const int *tri, const uint32_t *start, const uint32_t *deg, const uint32_t *inc, uint32_t nv,
-                      const float *src, float *dst, hipStream_t s);
 void qs_launch_tri_keep(const int *tri, uint32_t nt, uint32_t *keep, hipStream_t s);
 void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *pos, int *out, hipStream_t s);
 uint64_t qs_scan_tiles(uint64_t n);

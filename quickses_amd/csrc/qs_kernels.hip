@@ -256,16 +256,290 @@ __global__ void __launch_bounds__(256) k_gather_atoms(const int *__restrict__ pe
 
 /* ------------------------------------------------------------------------------------------ */
 /* field pass 1: probeIntersection + computeInOrOut Kernels.cu:344-475                         */
+/*
+ * Labels are kept as two bit masks per 64-voxel z-word of every (x, y) row: "outside" (no
+ * admissible atom within r + probe) and "shell" (within r + probe of one, within r - dx of none);
+ * a voxel in neither is inside.  Bits for z >= g are 0; the last layer of each axis is outside
+ * (never written by the reference, CudaSurf.cu:664).
+ *
+ * k_pass1_rows: one wavefront per row word of up to four adjacent rows.  The lanes take the
+ * candidate atoms of the rows (the atoms of the neighbour cells any of their voxels would visit),
+ * and each lane computes, for its atom and each row, the z-interval of voxels it reaches.  Along a row the single-precision distance
+ * sqrt((x^2 + y^2) + z^2) is non-increasing then non-decreasing in z (every rounding is
+ * monotone), so {z : d(z) < T} is an interval; it is located from the closed-form estimate and
+ * then pinned by evaluating the reference's float predicate at its ends.  A voxel counts an atom
+ * only if the atom's cell is one of the 27 (clamped) cells it would visit, as in the reference.
+ * The wave ORs the lanes' interval masks.  Cost per row word: a few hundred instructions,
+ * instead of ~44 candidate atoms per voxel.
+ */
 
-/* block = 4 (x) * 8 (y) * 8 (z) voxels */
-__global__ void __launch_bounds__(256) k_pass1(QsGrid G, const int2 *__restrict__ cse, const float4 *__restrict__ sx,
-                                               uint8_t *__restrict__ labels, int x0, int x1)
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
 {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    lo = __shfl_xor(lo, m, 64);
+    hi = __shfl_xor(hi, m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+/* bits [a, b] (inclusive, 0 <= a, b <= 63) */
+__device__ __forceinline__ uint64_t bit_range(int a, int b)
+{
+    if (a > b) return 0ull;
+    uint64_t hi = (b >= 63) ? ~0ull : ((2ull << b) - 1ull);
+    uint64_t lo = (1ull << a) - 1ull;
+    return hi & ~lo;
+}
+
+/* smallest float s with sqrtf(s) >= T (T > 0), so that for v >= 0: sqrtf(v) < T <=> v < s.
+   Every step uses the correctly rounded sqrtf, so the comparison is the reference's. */
+__device__ __forceinline__ float sqrt_threshold(float T)
+{
+    if (!(T > 0.0f)) return -1.0f;
+    float s = T * T;
+    for (int i = 0; i < 8 && sqrtf(s) < T; i++) s = __uint_as_float(__float_as_uint(s) + 1u);
+    for (int i = 0; i < 8 && s > 0.0f; i++) {
+        float q = __uint_as_float(__float_as_uint(s) - 1u);
+        if (sqrtf(q) >= T) s = q;
+        else break;
+    }
+    return s;
+}
+
+/* one atom seen from one row: fl(S + fl(e(z)^2)) with S = fl(x^2 + y^2), e(z) = az - pz(z) */
+struct RowAtom {
+    float az, S, dx, oz;
+    __device__ __forceinline__ float Zf(int z) const
+    {
+        float pz = grid_pos(oz, z, dx);
+        float e = az - pz;
+        return e * e;
+    }
+    __device__ __forceinline__ bool pred(int z, float ss) const
+    {
+        float v = S + Zf(z);
+        return v < ss;
+    }
+};
+
+/* exact [L, R] within [zlo, zhi] where the predicate holds; zb = its argmin in [zlo, zhi];
+   L > R if none.  zc is the atom's z in voxel units (estimate only). */
+__device__ __forceinline__ void row_interval(const RowAtom &A, float ss, int zlo, int zhi, int zb, float zc, float inv_dx,
+                                             int &L, int &R)
+{
+    L = 1;
+    R = 0;
+    if (zlo > zhi || !A.pred(zb, ss)) return;
+    float h2 = ss - A.S;
+    float h = (h2 > 0.0f) ? sqrtf(h2) * inv_dx : 0.0f;
+    int l = iclamp((int)ceilf(zc - h), zlo, zb);
+    if (A.pred(l, ss)) {
+        while (l > zlo && A.pred(l - 1, ss)) l--;
+    } else {
+        while (l < zb && !A.pred(l, ss)) l++;
+    }
+    int r = iclamp((int)floorf(zc + h), zb, zhi);
+    if (A.pred(r, ss)) {
+        while (r < zhi && A.pred(r + 1, ss)) r++;
+    } else {
+        while (r > zb && !A.pred(r, ss)) r--;
+    }
+    L = l;
+    R = r;
+}
+
+/* rows per wavefront: R consecutive y rows (R * dx <= neighbour-cell size, so they span at most
+   two neighbour cells in y and the candidate set is at most 3 x 4 cell columns) */
+#define P1_RMAX 4
+
+__global__ void __launch_bounds__(256) k_pass1_rows(QsGrid G, const uint32_t *__restrict__ cstart,
+                                                    const float4 *__restrict__ sx, uint64_t *__restrict__ outm,
+                                                    uint64_t *__restrict__ shellm, int W, int R, int x0, int x1)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int g = G.g, gn = G.gn;
+    const int w = blockIdx.x;
+    const int y0 = (blockIdx.y * 4 + wid) * R;
+    const int x = x0 + blockIdx.z;
+    if (y0 >= g || x >= x1) return; /* whole wavefront */
+    const int ny = min(R, g - y0);
+    const int z0 = w * 64;
+    const int nvalid = min(64, g - z0);
+    const uint64_t valid = (nvalid >= 64) ? ~0ull : ((1ull << nvalid) - 1ull);
+    if (x >= g - 1) {
+        if (lane < ny) {
+            size_t widx = ((size_t)x * g + y0 + lane) * W + w;
+            outm[widx] = valid;
+            shellm[widx] = 0ull;
+        }
+        return;
+    }
+    const float dx = G.dx, inv_dx = 1.0f / G.dx;
+    const float px = grid_pos(G.ox, x, dx);
+    const int cx = cell_of(px, G.ox, G.dxn);
+    float py[P1_RMAX];
+    int cyl[P1_RMAX], cyh[P1_RMAX];
+#pragma unroll
+    for (int k = 0; k < P1_RMAX; k++) {
+        int yk = min(y0 + min(k, ny - 1), g - 1);
+        py[k] = grid_pos(G.oy, yk, dx);
+        int cy = cell_of(py[k], G.oy, G.dxn);
+        cyl[k] = iclamp(cy - 1, 0, gn - 1);
+        cyh[k] = iclamp(cy + 1, 0, gn - 1);
+    }
+    /* this lane's voxel z cell; the wave's cells are monotone in the lane */
+    const int zl = min(z0 + lane, g - 1);
+    const int cz = cell_of(grid_pos(G.oz, zl, dx), G.oz, G.dxn);
+    const int czmin = __shfl(cz, 0, 64), czmax = __shfl(cz, 63, 64);
+    const int D = czmax - czmin + 1; /* distinct z cells + 1 */
+    /* tab (lane j) = number of lanes whose z cell is < czmin + j, j = 0..D */
+    int tab = (lane >= D) ? 64 : 0;
+    if (D < 64) {
+        for (int d = 1; d < D; d++) {
+            uint64_t m = __ballot(cz < czmin + d);
+            if (lane == d) tab = __popcll(m);
+        }
+    }
+    const int zc0 = iclamp(czmin - 1, 0, gn - 1), zc1 = iclamp(czmax + 1, 0, gn - 1);
+    const int xa = iclamp(cx - 1, 0, gn - 1), xb = iclamp(cx + 1, 0, gn - 1);
+    const int ya = cyl[0], yb = cyh[P1_RMAX - 1];
+    /* candidate atoms: up to 3 x 4 contiguous ranges of the cell-ordered array */
+    int rs[12], pre[13];
+    pre[0] = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int k = 4 * i + j;
+            int xc = xa + i, yc = ya + j;
+            int len = 0, st = 0;
+            if (xc <= xb && yc <= yb) {
+                int h0 = (xc * gn + yc) * gn + zc0;
+                st = (int)cstart[h0];
+                len = (int)cstart[h0 + (zc1 - zc0) + 1] - st;
+            }
+            rs[k] = st;
+            pre[k + 1] = pre[k] + len;
+        }
+    }
+    const int total = pre[12];
+    const int zmax = min(z0 + 63, g - 2); /* labelled voxels: z <= g-2 (z = g-1 is outside) */
+    uint64_t m_in[P1_RMAX], m_near[P1_RMAX];
+#pragma unroll
+    for (int k = 0; k < P1_RMAX; k++) m_in[k] = m_near[k] = 0ull;
+    for (int base = 0; base < total; base += 64) {
+        const int kk = base + lane;
+        int id = -1;
+#pragma unroll
+        for (int q = 0; q < 12; q++)
+            if (kk >= pre[q] && kk < pre[q + 1]) id = rs[q] + (kk - pre[q]);
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+        int acz = czmin, acy = 0;
+        if (id >= 0) {
+            a = sx[id];
+            acz = iclamp(cell_of(a.z, G.oz, G.dxn), 0, gn - 1);
+            acy = iclamp(cell_of(a.y, G.oy, G.dxn), 0, gn - 1);
+        }
+        /* voxels whose z cell admits the atom's z cell: lanes [La, Ra] */
+        int La, Ra;
+        if (D < 64) {
+            int j1 = acz - 1 - czmin, j2 = acz + 2 - czmin;
+            int t1 = __shfl(tab, iclamp(j1, 0, 63), 64);
+            int t2 = __shfl(tab, iclamp(j2, 0, 63), 64);
+            La = (j1 <= 0) ? 0 : (j1 >= D ? 64 : t1);
+            Ra = ((j2 >= D) || (acz == gn - 1)) ? 63 : ((j2 <= 0) ? -1 : t2 - 1);
+        } else {
+            const int alo = iclamp(cz - 1, 0, gn - 1), ahi = iclamp(cz + 1, 0, gn - 1);
+            int lo_i = 0, hi_i = 64;
+            for (int it = 0; it < 7; it++) {
+                int mid = (lo_i + hi_i) >> 1;
+                int v = __shfl(ahi, min(mid, 63), 64);
+                if (lo_i < hi_i) {
+                    if (v >= acz) hi_i = mid;
+                    else lo_i = mid + 1;
+                }
+            }
+            La = lo_i;
+            lo_i = 0;
+            hi_i = 64;
+            for (int it = 0; it < 7; it++) {
+                int mid = (lo_i + hi_i) >> 1;
+                int v = __shfl(alo, min(mid, 63), 64);
+                if (lo_i < hi_i) {
+                    if (v > acz) hi_i = mid;
+                    else lo_i = mid + 1;
+                }
+            }
+            Ra = lo_i - 1;
+        }
+        if (id < 0) continue;
+        const int zlo = z0 + La, zhi = min(z0 + Ra, zmax);
+        if (zlo > zhi) continue;
+        const float sN = sqrt_threshold(QS_PROBE + a.w);
+        const float sD = sqrt_threshold(a.w - dx);
+        float ex = a.x - px;
+        const float X = ex * ex;
+        const float zc = (a.z - G.oz) * inv_dx;
+        RowAtom A;
+        A.az = a.z;
+        A.dx = dx;
+        A.oz = G.oz;
+        const int zb0 = iclamp((int)floorf(zc), zlo, zhi);
+#pragma unroll
+        for (int k = 0; k < P1_RMAX; k++) {
+            if (k >= ny) break;
+            if (acy < cyl[k] || acy > cyh[k]) continue; /* row's y cell does not admit the atom */
+            float ey = a.y - py[k];
+            float Y = ey * ey;
+            A.S = X + Y;
+            if (!(A.S < sN)) continue;
+            int zb = zb0;
+            if (zb + 1 <= zhi && A.Zf(zb + 1) < A.Zf(zb)) zb = zb + 1;
+            int L, Rr;
+            row_interval(A, sN, zlo, zhi, zb, zc, inv_dx, L, Rr);
+            if (L > Rr) continue;
+            m_near[k] |= bit_range(L - z0, Rr - z0);
+            row_interval(A, sD, L, Rr, zb, zc, inv_dx, L, Rr);
+            if (L <= Rr) m_in[k] |= bit_range(L - z0, Rr - z0);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < P1_RMAX; k++) {
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) {
+            m_in[k] |= shfl_xor_u64(m_in[k], m);
+            m_near[k] |= shfl_xor_u64(m_near[k], m);
+        }
+    }
+    if (lane < ny) {
+        uint64_t mi = 0ull, mn = 0ull;
+#pragma unroll
+        for (int k = 0; k < P1_RMAX; k++)
+            if (lane == k) {
+                mi = m_in[k];
+                mn = m_near[k];
+            }
+        const int y = y0 + lane;
+        uint64_t lab = (zmax >= z0 && y < g - 1) ? bit_range(0, zmax - z0) : 0ull;
+        mi &= lab;
+        mn &= lab;
+        size_t widx = ((size_t)x * g + y) * W + w;
+        outm[widx] = valid & ~(mi | mn);
+        shellm[widx] = mn & ~mi;
+    }
+}
+
+/* per-voxel form (27 cells per voxel, as the reference); cross-check of k_pass1_rows */
+__global__ void __launch_bounds__(256) k_pass1_direct(QsGrid G, const int2 *__restrict__ cse, const float4 *__restrict__ sx,
+                                                      uint64_t *__restrict__ outm, uint64_t *__restrict__ shellm, int W,
+                                                      int x0, int x1)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int g = G.g;
-    int z = blockIdx.x * 8 + (threadIdx.x & 7);
-    int y = blockIdx.y * 8 + ((threadIdx.x >> 3) & 7);
-    int x = x0 + blockIdx.z * 4 + (threadIdx.x >> 6);
-    if (x >= x1 || y >= g || z >= g) return;
+    const int w = blockIdx.x;
+    const int y = blockIdx.y * 4 + wid;
+    const int x = x0 + blockIdx.z;
+    if (y >= g || x >= x1) return;
+    const int z = w * 64 + lane;
     uint8_t res = QS_OUT;
     if (x < g - 1 && y < g - 1 && z < g - 1) {
         const float dx = G.dx;
@@ -295,7 +569,13 @@ __global__ void __launch_bounds__(256) k_pass1(QsGrid G, const int2 *__restrict_
         }
         res = inside ? QS_IN : (near ? QS_SHELL : QS_OUT);
     }
-    labels[((size_t)x * g + y) * g + z] = res;
+    uint64_t mo = __ballot(z < g && res == QS_OUT);
+    uint64_t ms = __ballot(z < g && res == QS_SHELL);
+    if (lane == 0) {
+        size_t widx = ((size_t)x * g + y) * W + w;
+        outm[widx] = mo;
+        shellm[widx] = ms;
+    }
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -307,64 +587,130 @@ __global__ void __launch_bounds__(256) k_pass1(QsGrid G, const int2 *__restrict_
  * norms that differ give distances further apart than 2*delta, so the minimum is attained among
  * the outside voxels whose norm lies within [nmin, nmax(nmin)], nmin being the exact smallest
  * norm.  The tile kernel finds nmin with three separable integer passes in LDS (z by bit scan
- * of a ballot mask, then y, then x) and evaluates the reference's float distance only for those
- * candidates.  The result is the reference's value bit for bit.
+ * of the outside mask, then y, then x) and evaluates the reference's float distance only for
+ * those candidates.  The result is the reference's value bit for bit.
  */
 
 #define P2_TX 8
 #define P2_TY 8
 #define P2_TZ 32
-#define P2_INF 0xFFFF
+
+/* bits [s, s + len) of a row of W mask words (bit i <-> z = s + i), zero outside [0, 64 W) */
+__device__ __forceinline__ uint64_t row_bits(const uint64_t *__restrict__ rp, int W, int s, int len)
+{
+    int w0 = s >> 6; /* floor */
+    int sh = s & 63;
+    uint64_t a = (w0 >= 0 && w0 < W) ? rp[w0] : 0ull;
+    uint64_t v = a >> sh;
+    if (sh) {
+        uint64_t b = (w0 + 1 >= 0 && w0 + 1 < W) ? rp[w0 + 1] : 0ull;
+        v |= b << (64 - sh);
+    }
+    if (len < 64) v &= (1ull << len) - 1ull;
+    return v;
+}
 
 size_t qs_pass2_lds_bytes(int r)
 {
     size_t LX = P2_TX + 2 * r, LY = P2_TY + 2 * r;
-    return LX * LY * 8 + LX * LY * P2_TZ * 2 + LX * P2_TY * P2_TZ * 2;
+    size_t dt = (3 * r * r < 255) ? 1 : 2;
+    return LX * LY * 8 + P2_TX * P2_TY * 4 + LX * LY * P2_TZ * dt + LX * P2_TY * P2_TZ * dt;
 }
 
-__global__ void __launch_bounds__(256) k_pass2_tile(QsGrid G, const uint8_t *__restrict__ labels,
-                                                    float *__restrict__ field, const int *__restrict__ offs,
+/* sign (field < 0) and zero (field == 0) bits of the field, one ballot per wave-half: lanes 0-31
+   are the 32 voxels (gx, gy, tz0..tz0+31), lanes 32-63 the same z range of row gy + 1 */
+__device__ __forceinline__ void emit_sign_bits(uint32_t *__restrict__ negm, uint32_t *__restrict__ zerom, int W, int g,
+                                               int gx, int gy, int tz0, bool valid, float res)
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t n = __ballot(valid && res < 0.0f);
+    uint64_t zr = __ballot(valid && res == 0.0f);
+    if ((lane & 31) == 0 && valid) {
+        size_t w32 = ((size_t)gx * g + gy) * (2 * (size_t)W) + (tz0 >> 5);
+        negm[w32] = (uint32_t)(lane ? (n >> 32) : n);
+        zerom[w32] = (uint32_t)(lane ? (zr >> 32) : zr);
+    }
+}
+
+template <typename DT>
+__global__ void __launch_bounds__(256) k_pass2_tile(QsGrid G, const uint64_t *__restrict__ outm,
+                                                    const uint64_t *__restrict__ shellm, int W,
+                                                    float *__restrict__ field, uint32_t *__restrict__ negm,
+                                                    uint32_t *__restrict__ zerom, const int *__restrict__ offs,
                                                     const int *__restrict__ norm_start,
                                                     const int *__restrict__ enum_end, int x0, int x1)
 {
     extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ int s_flags;
+    constexpr int INF = (sizeof(DT) == 1) ? 0xFF : 0xFFFF;
     const int r = G.r, g = G.g;
     const int LX = P2_TX + 2 * r, LY = P2_TY + 2 * r, LZ = P2_TZ + 2 * r;
     uint64_t *rows = (uint64_t *)smem;
-    uint16_t *DZ = (uint16_t *)(smem + (size_t)LX * LY * 8);
-    uint16_t *DY = DZ + (size_t)LX * LY * P2_TZ;
+    uint32_t *own = (uint32_t *)(smem + (size_t)LX * LY * 8);
+    DT *DZ = (DT *)(smem + (size_t)LX * LY * 8 + P2_TX * P2_TY * 4);
+    DT *DY = DZ + (size_t)LX * LY * P2_TZ;
     const int tz0 = blockIdx.x * P2_TZ, ty0 = blockIdx.y * P2_TY, tx0 = x0 + blockIdx.z * P2_TX;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-
-    /* 1. outside mask of the haloed tile, one 64-bit ballot per (x,y) row */
-    for (int row = wid; row < LX * LY; row += 4) {
-        int lx = row / LY, ly = row - lx * LY;
-        int gx = tx0 - r + lx, gy = ty0 - r + ly, gz = tz0 - r + lane;
-        bool out = false;
-        if (lane < LZ && gx >= 0 && gx < g && gy >= 0 && gy < g && gz >= 0 && gz < g)
-            out = labels[((size_t)gx * g + gy) * g + gz] == QS_OUT;
-        uint64_t m = __ballot(out);
-        if (lane == 0) rows[row] = m;
-    }
+    if (threadIdx.x == 0) s_flags = 0;
     __syncthreads();
 
-    /* 2. z: squared distance to the nearest outside voxel within |c| <= r along the row */
+    /* 1. outside bits of the haloed tile; shell bits of the tile */
+    int fl = 0;
+    for (int row = threadIdx.x; row < LX * LY; row += 256) {
+        int lx = row / LY, ly = row - lx * LY;
+        int gx = tx0 - r + lx, gy = ty0 - r + ly;
+        uint64_t m = 0ull;
+        if (gx >= 0 && gx < g && gy >= 0 && gy < g) m = row_bits(outm + ((size_t)gx * g + gy) * W, W, tz0 - r, LZ);
+        rows[row] = m;
+        if (m) fl |= 2;
+    }
+    for (int t = threadIdx.x; t < P2_TX * P2_TY; t += 256) {
+        int tx = t / P2_TY, ty = t - tx * P2_TY;
+        int gx = tx0 + tx, gy = ty0 + ty;
+        uint32_t m = 0u;
+        if (gx < x1 && gx < g && gy < g) m = (uint32_t)row_bits(shellm + ((size_t)gx * g + gy) * W, W, tz0, P2_TZ);
+        own[t] = m;
+        if (m) fl |= 1;
+    }
+    if (fl) atomicOr(&s_flags, fl);
+    __syncthreads();
+    const int flags = s_flags;
+
+    const int ty = threadIdx.x / P2_TZ, tz = threadIdx.x - ty * P2_TZ;
+    const int gy = ty0 + ty, gz = tz0 + tz;
+    const bool vyz = gy < g && gz < g;
+    const float dx = G.dx;
+    if (flags != 3) {
+        /* no shell voxel, or no outside voxel within reach: outside -> probe, else -dx */
+        for (int tx = 0; tx < P2_TX; tx++) {
+            int gx = tx0 + tx;
+            if (gx >= x1 || gx >= g) break;
+            bool out = (rows[(size_t)(tx + r) * LY + (ty + r)] >> (tz + r)) & 1ull;
+            float res = out ? QS_PROBE : -dx;
+            if (vyz) field[((size_t)gx * g + gy) * g + gz] = res;
+            emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, res);
+        }
+        return;
+    }
+
+    /* 2. z: squared distance to the nearest outside voxel within |c| <= r along each row */
     for (int w = threadIdx.x; w < LX * LY * P2_TZ; w += 256) {
         int row = w / P2_TZ, z = w - row * P2_TZ;
         uint64_t m = rows[row];
-        int c = z + r;
-        int best = P2_INF;
-        uint64_t up = m >> c;
-        if (up) {
-            int d = __builtin_ctzll(up);
-            if (d <= r) best = d * d;
+        int best = INF;
+        if (m) {
+            int c = z + r;
+            uint64_t up = m >> c;
+            if (up) {
+                int d = __builtin_ctzll(up);
+                if (d <= r) best = d * d;
+            }
+            uint64_t dn = m & ((2ull << c) - 1ull);
+            if (dn) {
+                int d = c - (63 - __builtin_clzll(dn));
+                if (d <= r) best = min(best, d * d);
+            }
         }
-        uint64_t dn = m & ((2ull << c) - 1ull);
-        if (dn) {
-            int d = c - (63 - __builtin_clzll(dn));
-            if (d <= r) best = min(best, d * d);
-        }
-        DZ[w] = (uint16_t)best;
+        DZ[w] = (DT)best;
     }
     __syncthreads();
 
@@ -372,76 +718,258 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsGrid G, const uint8_t *__r
     for (int w = threadIdx.x; w < LX * P2_TY * P2_TZ; w += 256) {
         int lx = w / (P2_TY * P2_TZ);
         int rem = w - lx * (P2_TY * P2_TZ);
-        int ty = rem / P2_TZ, z = rem - ty * P2_TZ;
-        int best = P2_INF;
+        int yy = rem / P2_TZ, z = rem - yy * P2_TZ;
+        int best = INF;
         for (int b = -r; b <= r; b++) {
-            int v = DZ[((size_t)lx * LY + (ty + r + b)) * P2_TZ + z];
-            if (v != P2_INF) best = min(best, v + b * b);
+            int v = DZ[((size_t)lx * LY + (yy + r + b)) * P2_TZ + z];
+            if (v != INF) best = min(best, v + b * b);
         }
-        DY[w] = (uint16_t)best;
+        DY[w] = (DT)best;
     }
     __syncthreads();
 
     /* 4. x, then the field value of each output voxel */
-    const int ty = threadIdx.x / P2_TZ, tz = threadIdx.x - ty * P2_TZ;
-    const int gy = ty0 + ty, gz = tz0 + tz;
-    if (gy >= g || gz >= g) return;
-    const float dx = G.dx;
     const float vy = grid_pos(G.oy, gy, dx), vz = grid_pos(G.oz, gz, dx);
     for (int tx = 0; tx < P2_TX; tx++) {
         int gx = tx0 + tx;
         if (gx >= x1 || gx >= g) break;
         size_t vid = ((size_t)gx * g + gy) * g + gz;
-        uint8_t lab = labels[vid];
-        float out;
-        if (lab == QS_OUT) {
-            out = QS_PROBE;
-        } else if (lab == QS_IN) {
-            out = -dx;
+        bool out = (rows[(size_t)(tx + r) * LY + (ty + r)] >> (tz + r)) & 1ull;
+        bool shell = (own[tx * P2_TY + ty] >> tz) & 1u;
+        float res;
+        if (!vyz || out) {
+            res = QS_PROBE;
+        } else if (!shell) {
+            res = -dx;
         } else {
-            int nmin = P2_INF;
+            int nmin = INF;
             for (int a = -r; a <= r; a++) {
                 int v = DY[((size_t)(tx + r + a) * P2_TY + ty) * P2_TZ + tz];
-                if (v != P2_INF) nmin = min(nmin, v + a * a);
+                if (v != INF) nmin = min(nmin, v + a * a);
             }
-            if (nmin == P2_INF) {
-                out = -dx;
+            if (nmin == INF) {
+                res = -dx;
             } else {
                 const float vx = grid_pos(G.ox, gx, dx);
                 float min_dist = 100000.0f;
-                for (int k = norm_start[nmin]; k < enum_end[nmin]; k++) {
-                    int o = offs[k];
-                    int a = (int)(int8_t)(o & 0xFF), b = (int)(int8_t)((o >> 8) & 0xFF), c = (int)(int8_t)((o >> 16) & 0xFF);
-                    uint64_t m = rows[(size_t)(tx + r + a) * LY + (ty + r + b)];
-                    if (!((m >> (tz + r + c)) & 1ull)) continue;
-                    float d = dist3(grid_pos(G.ox, gx + a, dx), grid_pos(G.oy, gy + b, dx), grid_pos(G.oz, gz + c, dx),
-                                    vx, vy, vz);
-                    min_dist = fminf(d, min_dist);
+                if (enum_end[nmin] == norm_start[nmin + 1]) {
+                    /* only norm nmin can attain the minimum: trace the minimisers back through the
+                       separable passes (a with a^2 + DY = nmin, then b with b^2 + DZ = DY, then
+                       c = +-sqrt(DZ) where the voxel is outside) */
+                    for (int a = -r; a <= r; a++) {
+                        int dy = DY[((size_t)(tx + r + a) * P2_TY + ty) * P2_TZ + tz];
+                        if (dy == INF || dy + a * a != nmin) continue;
+                        for (int b = -r; b <= r; b++) {
+                            int dz = DZ[((size_t)(tx + r + a) * LY + (ty + r + b)) * P2_TZ + tz];
+                            if (dz == INF || dz + b * b != dy) continue;
+                            int c = (int)sqrtf((float)dz);
+                            uint64_t m = rows[(size_t)(tx + r + a) * LY + (ty + r + b)];
+                            const float qx = grid_pos(G.ox, gx + a, dx), qy = grid_pos(G.oy, gy + b, dx);
+                            if ((m >> (tz + r + c)) & 1ull)
+                                min_dist = fminf(dist3(qx, qy, grid_pos(G.oz, gz + c, dx), vx, vy, vz), min_dist);
+                            if (c && ((m >> (tz + r - c)) & 1ull))
+                                min_dist = fminf(dist3(qx, qy, grid_pos(G.oz, gz - c, dx), vx, vy, vz), min_dist);
+                        }
+                    }
+                } else {
+                    for (int k = norm_start[nmin]; k < enum_end[nmin]; k++) {
+                        int o = offs[k];
+                        int a = (int)(int8_t)(o & 0xFF), b = (int)(int8_t)((o >> 8) & 0xFF), c = (int)(int8_t)((o >> 16) & 0xFF);
+                        uint64_t m = rows[(size_t)(tx + r + a) * LY + (ty + r + b)];
+                        if (!((m >> (tz + r + c)) & 1ull)) continue;
+                        float d = dist3(grid_pos(G.ox, gx + a, dx), grid_pos(G.oy, gy + b, dx), grid_pos(G.oz, gz + c, dx),
+                                        vx, vy, vz);
+                        min_dist = fminf(d, min_dist);
+                    }
                 }
-                out = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
+                res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
             }
         }
-        field[vid] = out;
+        if (vyz) field[vid] = res;
+        emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, res);
     }
 }
 
+/* Field pass 2 for a fixed window half-width R: loops unrolled, y and x passes register-blocked
+   (each thread keeps one column of the previous pass in registers). */
+template <int R>
+__global__ void __launch_bounds__(256) k_pass2_r(QsGrid G, const uint64_t *__restrict__ outm,
+                                                 const uint64_t *__restrict__ shellm, int W, float *__restrict__ field,
+                                                 uint32_t *__restrict__ negm, uint32_t *__restrict__ zerom,
+                                                 const int *__restrict__ offs, const int *__restrict__ norm_start,
+                                                 const int *__restrict__ enum_end, int x0, int x1)
+{
+    constexpr int LX = P2_TX + 2 * R, LY = P2_TY + 2 * R, LZ = P2_TZ + 2 * R;
+    constexpr int INF = 255;
+    static_assert(LZ <= 64 && 3 * R * R < INF, "window too wide for this kernel");
+    __shared__ uint64_t rows[LX * LY];
+    __shared__ uint32_t own[P2_TX * P2_TY];
+    __shared__ uint8_t DZ[LX * LY * P2_TZ];
+    __shared__ uint8_t DY[LX * P2_TY * P2_TZ];
+    __shared__ int s_flags;
+    const int g = G.g;
+    const int tz0 = blockIdx.x * P2_TZ, ty0 = blockIdx.y * P2_TY, tx0 = x0 + blockIdx.z * P2_TX;
+    if (threadIdx.x == 0) s_flags = 0;
+    __syncthreads();
+
+    int fl = 0;
+    for (int row = threadIdx.x; row < LX * LY; row += 256) {
+        int lx = row / LY, ly = row - lx * LY;
+        int gx = tx0 - R + lx, gy = ty0 - R + ly;
+        uint64_t m = 0ull;
+        if (gx >= 0 && gx < g && gy >= 0 && gy < g) m = row_bits(outm + ((size_t)gx * g + gy) * W, W, tz0 - R, LZ);
+        rows[row] = m;
+        if (m) fl |= 2;
+    }
+    if (threadIdx.x < P2_TX * P2_TY) {
+        int tx = threadIdx.x / P2_TY, ty = threadIdx.x - tx * P2_TY;
+        int gx = tx0 + tx, gy = ty0 + ty;
+        uint32_t m = 0u;
+        if (gx < x1 && gx < g && gy < g) m = (uint32_t)row_bits(shellm + ((size_t)gx * g + gy) * W, W, tz0, P2_TZ);
+        own[threadIdx.x] = m;
+        if (m) fl |= 1;
+    }
+    if (fl) atomicOr(&s_flags, fl);
+    __syncthreads();
+    const int flags = s_flags;
+
+    const int ty = threadIdx.x >> 5, tz = threadIdx.x & 31;
+    const int gy = ty0 + ty, gz = tz0 + tz;
+    const bool vyz = gy < g && gz < g;
+    const float dx = G.dx;
+    if (flags != 3) {
+        /* no shell voxel, or no outside voxel within reach: outside -> probe, else -dx */
+        for (int tx = 0; tx < P2_TX; tx++) {
+            int gx = tx0 + tx;
+            if (gx >= x1 || gx >= g) break;
+            bool out = (rows[(tx + R) * LY + (ty + R)] >> (tz + R)) & 1ull;
+            float res = out ? QS_PROBE : -dx;
+            if (vyz) field[((size_t)gx * g + gy) * g + gz] = res;
+            emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, res);
+        }
+        return;
+    }
+
+    /* z: squared distance to the nearest outside voxel within |c| <= R along each row */
+    for (int w = threadIdx.x; w < LX * LY * P2_TZ; w += 256) {
+        int row = w >> 5, z = w & 31;
+        uint64_t m = rows[row];
+        int best = INF;
+        if (m) {
+            int c = z + R;
+            uint64_t up = m >> c;
+            if (up) {
+                int d = __builtin_ctzll(up);
+                if (d <= R) best = d * d;
+            }
+            uint64_t dn = m & ((2ull << c) - 1ull);
+            if (dn) {
+                int d = c - (63 - __builtin_clzll(dn));
+                if (d <= R) best = min(best, d * d);
+            }
+        }
+        DZ[w] = (uint8_t)best;
+    }
+    __syncthreads();
+
+    /* y, one (lx, z) column per thread */
+    for (int col = threadIdx.x; col < LX * P2_TZ; col += 256) {
+        const int lx = col >> 5, z = col & 31;
+        int v[LY];
+#pragma unroll
+        for (int ly = 0; ly < LY; ly++) v[ly] = DZ[(lx * LY + ly) * P2_TZ + z];
+#pragma unroll
+        for (int yy = 0; yy < P2_TY; yy++) {
+            int best = INF;
+#pragma unroll
+            for (int b = 0; b <= 2 * R; b++) best = min(best, v[yy + b] + (b - R) * (b - R));
+            DY[(lx * P2_TY + yy) * P2_TZ + z] = (uint8_t)best;
+        }
+    }
+    __syncthreads();
+
+    /* x, one (ty, tz) column per thread, then the field */
+    int u[LX];
+#pragma unroll
+    for (int lx = 0; lx < LX; lx++) u[lx] = DY[(lx * P2_TY + ty) * P2_TZ + tz];
+    const float vy = grid_pos(G.oy, gy, dx), vz = grid_pos(G.oz, gz, dx);
+#pragma unroll
+    for (int tx = 0; tx < P2_TX; tx++) {
+        const int gx = tx0 + tx;
+        if (gx >= x1 || gx >= g) break;
+        const bool out = (rows[(tx + R) * LY + (ty + R)] >> (tz + R)) & 1ull;
+        const bool shell = (own[tx * P2_TY + ty] >> tz) & 1u;
+        float res = out ? QS_PROBE : -dx;
+        if (vyz && shell && !out) {
+            int nmin = INF;
+#pragma unroll
+            for (int a = 0; a <= 2 * R; a++) nmin = min(nmin, u[tx + a] + (a - R) * (a - R));
+            if (nmin < INF) {
+                const float vx = grid_pos(G.ox, gx, dx);
+                float min_dist = 100000.0f;
+                if (enum_end[nmin] == norm_start[nmin + 1]) {
+                    /* trace the minimisers back through the separable passes */
+#pragma unroll 1
+                    for (int a = -R; a <= R; a++) {
+                        const int dy = DY[((tx + R + a) * P2_TY + ty) * P2_TZ + tz];
+                        if (dy + a * a != nmin) continue;
+#pragma unroll 1
+                        for (int b = -R; b <= R; b++) {
+                            const int dz = DZ[((tx + R + a) * LY + (ty + R + b)) * P2_TZ + tz];
+                            if (dz == INF || dz + b * b != dy) continue;
+                            const int c = (int)sqrtf((float)dz);
+                            const uint64_t m = rows[(tx + R + a) * LY + (ty + R + b)];
+                            const float qx = grid_pos(G.ox, gx + a, dx), qy = grid_pos(G.oy, gy + b, dx);
+                            if ((m >> (tz + R + c)) & 1ull)
+                                min_dist = fminf(dist3(qx, qy, grid_pos(G.oz, gz + c, dx), vx, vy, vz), min_dist);
+                            if (c && ((m >> (tz + R - c)) & 1ull))
+                                min_dist = fminf(dist3(qx, qy, grid_pos(G.oz, gz - c, dx), vx, vy, vz), min_dist);
+                        }
+                    }
+                } else {
+#pragma unroll 1
+                    for (int k = norm_start[nmin]; k < enum_end[nmin]; k++) {
+                        int o = offs[k];
+                        int a = (int)(int8_t)(o & 0xFF), b = (int)(int8_t)((o >> 8) & 0xFF), c = (int)(int8_t)((o >> 16) & 0xFF);
+                        uint64_t m = rows[(tx + R + a) * LY + (ty + R + b)];
+                        if (!((m >> (tz + R + c)) & 1ull)) continue;
+                        float d = dist3(grid_pos(G.ox, gx + a, dx), grid_pos(G.oy, gy + b, dx), grid_pos(G.oz, gz + c, dx),
+                                        vx, vy, vz);
+                        min_dist = fminf(d, min_dist);
+                    }
+                }
+                res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
+            }
+        }
+        if (vyz) field[((size_t)gx * g + gy) * g + gz] = res;
+        emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, res);
+    }
+}
+
+__device__ __forceinline__ bool mask_bit(const uint64_t *__restrict__ m, int g, int W, int x, int y, int z)
+{
+    return (m[((size_t)x * g + y) * W + (z >> 6)] >> (z & 63)) & 1ull;
+}
+
 /* direct (2r+1)^3 window, one thread per voxel: any r; also a cross-check of the tile kernel */
-__global__ void __launch_bounds__(256) k_pass2_direct(QsGrid G, const uint8_t *__restrict__ labels,
-                                                      float *__restrict__ field, int x0, int x1)
+__global__ void __launch_bounds__(256) k_pass2_direct(QsGrid G, const uint64_t *__restrict__ outm,
+                                                      const uint64_t *__restrict__ shellm, int W,
+                                                      float *__restrict__ field, uint32_t *__restrict__ negm,
+                                                      uint32_t *__restrict__ zerom, int x0, int x1)
 {
     const int g = G.g, r = G.r;
     int z = blockIdx.x * 64 + (threadIdx.x & 63);
     int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     int x = x0 + blockIdx.z;
-    if (x >= x1 || y >= g || z >= g) return;
+    if (x >= x1 || y >= g) return;
+    const bool valid = z < g;
     size_t vid = ((size_t)x * g + y) * g + z;
-    uint8_t lab = labels[vid];
     const float dx = G.dx;
-    float out;
-    if (lab == QS_OUT) {
-        out = QS_PROBE;
-    } else if (lab == QS_IN) {
-        out = -dx;
+    float res;
+    if (!valid || mask_bit(outm, g, W, x, y, z)) {
+        res = QS_PROBE;
+    } else if (!mask_bit(shellm, g, W, x, y, z)) {
+        res = -dx;
     } else {
         float vx = grid_pos(G.ox, x, dx), vy = grid_pos(G.oy, y, dx), vz = grid_pos(G.oz, z, dx);
         float min_dist = 100000.0f;
@@ -451,15 +979,64 @@ __global__ void __launch_bounds__(256) k_pass2_direct(QsGrid G, const uint8_t *_
                 int qy = iclamp(y + b, 0, g - 1);
                 for (int c = -r; c <= r; c++) {
                     int qz = iclamp(z + c, 0, g - 1);
-                    if (labels[((size_t)qx * g + qy) * g + qz] != QS_OUT) continue;
+                    if (!mask_bit(outm, g, W, qx, qy, qz)) continue;
                     float d = dist3(grid_pos(G.ox, qx, dx), grid_pos(G.oy, qy, dx), grid_pos(G.oz, qz, dx), vx, vy, vz);
                     min_dist = fminf(d, min_dist);
                 }
             }
         }
-        out = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
+        res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
     }
-    field[vid] = out;
+    if (valid) field[vid] = res;
+    uint64_t n = __ballot(valid && res < 0.0f), zr = __ballot(valid && res == 0.0f);
+    if ((threadIdx.x & 63) == 0) {
+        size_t w32 = ((size_t)x * g + y) * (2 * (size_t)W) + 2 * (size_t)blockIdx.x;
+        negm[w32] = (uint32_t)n;
+        negm[w32 + 1] = (uint32_t)(n >> 32);
+        zerom[w32] = (uint32_t)zr;
+        zerom[w32 + 1] = (uint32_t)(zr >> 32);
+    }
+}
+
+/* sign and zero bits of a caller-given field (stage-wise entry point) */
+__global__ void __launch_bounds__(256) k_field_signs(const float *__restrict__ F, int g, int W, uint32_t *__restrict__ negm,
+                                                     uint32_t *__restrict__ zerom)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int w = blockIdx.x, y = blockIdx.y * 4 + wid, x = blockIdx.z;
+    if (y >= g) return;
+    const int z = w * 64 + lane;
+    const float f = (z < g) ? F[((size_t)x * g + y) * g + z] : 1.0f;
+    uint64_t n = __ballot(f < 0.0f), zr = __ballot(f == 0.0f);
+    if (lane == 0) {
+        size_t w32 = ((size_t)x * g + y) * (2 * (size_t)W) + 2 * (size_t)w;
+        negm[w32] = (uint32_t)n;
+        negm[w32 + 1] = (uint32_t)(n >> 32);
+        zerom[w32] = (uint32_t)zr;
+        zerom[w32 + 1] = (uint32_t)(zr >> 32);
+    }
+}
+
+void qs_launch_field_signs(const float *F, int g, int W, uint32_t *negm, uint32_t *zerom, hipStream_t s)
+{
+    dim3 grid((unsigned)W, (unsigned)((g + 3) / 4), (unsigned)g);
+    hipLaunchKernelGGL(k_field_signs, grid, dim3(256), 0, s, F, g, W, negm, zerom);
+}
+
+/* labels (u8, 0 outside 1 shell 2 inside) -> masks, for stage-wise entry points */
+__global__ void __launch_bounds__(256) k_labels_to_masks(const uint8_t *__restrict__ labels, int g, int W,
+                                                         uint64_t *__restrict__ outm, uint64_t *__restrict__ shellm)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int w = blockIdx.x, y = blockIdx.y * 4 + wid, x = blockIdx.z;
+    if (y >= g) return;
+    int z = w * 64 + lane;
+    uint8_t l = (z < g) ? labels[((size_t)x * g + y) * g + z] : 0xFF;
+    uint64_t mo = __ballot(l == QS_OUT), ms = __ballot(l == QS_SHELL);
+    if (lane == 0) {
+        outm[((size_t)x * g + y) * W + w] = mo;
+        shellm[((size_t)x * g + y) * W + w] = ms;
+    }
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -473,196 +1050,249 @@ __global__ void __launch_bounds__(256) k_pass2_direct(QsGrid G, const uint8_t *_
  * exact.  Triangles follow cube order and the case table's order.
  */
 
-#define MC_NT 256
-#define MC_IPT 4
-#define MC_B (MC_NT * MC_IPT)
-
-struct P3 {
-    int x, y, z;
-};
-
-__device__ __forceinline__ P3 unflatten(uint64_t p, int g)
-{
-    uint32_t gg = (uint32_t)g;
-    uint64_t xy = p / gg;
-    P3 r;
-    r.z = (int)(p - xy * gg);
-    r.x = (int)(xy / gg);
-    r.y = (int)(xy - (uint64_t)r.x * gg);
-    return r;
-}
+/*
+ * Marching cubes on the sign bits of the field.  Field pass 2 writes, per 64-voxel row word,
+ * N (field < 0) and Z (field == 0).  For every row word k_mc_count forms with 64-bit logic
+ *   the four vertex-slot masks of its points (edge +x, +y, +z with a sign change inside the
+ *   emission window and a non-zero non-negative end; the point itself when it is zero and an
+ *   incident in-window edge ends negative), and
+ *   the mask of polygonised cubes (cube origin in [0, lim)^3, corners not all of one sign),
+ * and counts vertices and triangles.  An exclusive scan over row words in linear (x, y, w) order
+ * gives each word's first vertex and triangle id; inside a word, a point's vertices come after
+ * those of the points below it (popcounts of the slot masks), in slot order x, y, z, point --
+ * the (point, slot) order.  Emission and smoothing run only on words that own something.
+ */
 
 __device__ __forceinline__ float fat(const float *__restrict__ f, int g, int x, int y, int z)
 {
     return f[((size_t)x * g + y) * g + z];
 }
 
-/* edge (p, axis) lies on a polygonised cube iff p[axis] < lim and the other two <= lim */
-__device__ __forceinline__ bool edge_in_window(int x, int y, int z, int axis, int lim)
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
 {
-    bool ok = (axis == 0 ? x < lim : x <= lim);
-    ok = ok && (axis == 1 ? y < lim : y <= lim);
-    ok = ok && (axis == 2 ? z < lim : z <= lim);
-    return ok;
-}
-
-__device__ __forceinline__ unsigned slot_mask(const float *__restrict__ F, int g, int lim, int x, int y, int z)
-{
-    unsigned m = 0;
-    float f = fat(F, g, x, y, z);
-    bool fn = f < 0.0f;
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        int qx = x + (a == 0), qy = y + (a == 1), qz = z + (a == 2);
-        if ((a == 0 ? qx : (a == 1 ? qy : qz)) < g && edge_in_window(x, y, z, a, lim)) {
-            float fq = fat(F, g, qx, qy, qz);
-            if (fn != (fq < 0.0f)) {
-                float nonneg = fn ? fq : f;
-                if (nonneg != 0.0f) m |= 1u << a;
-            }
-        }
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(v, d, 64);
+        if (lane >= d) v += y;
     }
-    if (f == 0.0f) {
-        for (int a = 0; a < 3 && !(m & 8u); a++) {
-            for (int s = -1; s <= 1; s += 2) {
-                int qx = x + (a == 0 ? s : 0), qy = y + (a == 1 ? s : 0), qz = z + (a == 2 ? s : 0);
-                int qa = (a == 0 ? qx : (a == 1 ? qy : qz));
-                if (qa < 0 || qa >= g) continue;
-                int lx = (s < 0) ? qx : x, ly = (s < 0) ? qy : y, lz = (s < 0) ? qz : z;
-                if (!edge_in_window(lx, ly, lz, a, lim)) continue;
-                if (fat(F, g, qx, qy, qz) < 0.0f) {
-                    m |= 8u;
-                    break;
-                }
-            }
-        }
-    }
-    return m;
+    return v;
 }
 
-/* countVertexPerCell MarchingCubes.cu:571-658 (cube index) plus vertex slots; per-block sums */
-__global__ void __launch_bounds__(MC_NT) k_mc_count(QsGrid G, const float *__restrict__ F, uint16_t *__restrict__ pinfo,
-                                                    uint32_t *__restrict__ bs_tri, uint32_t *__restrict__ bs_vert,
-                                                    uint64_t p0, uint64_t p1)
+struct MCView {
+    const uint64_t *__restrict__ N;
+    const uint64_t *__restrict__ Zm;
+    const uint64_t *__restrict__ S4; /* 4 slot masks per row word */
+    const uint32_t *__restrict__ vbw; /* first vertex id per row word */
+    int g, W, lim;
+
+    __device__ __forceinline__ size_t widx(int x, int y, int z) const { return ((size_t)x * g + y) * W + (z >> 6); }
+    __device__ __forceinline__ bool nbit(int x, int y, int z) const { return (N[widx(x, y, z)] >> (z & 63)) & 1ull; }
+    __device__ __forceinline__ bool zbit(int x, int y, int z) const { return (Zm[widx(x, y, z)] >> (z & 63)) & 1ull; }
+    __device__ __forceinline__ int cube_index(int x, int y, int z) const
+    {
+        int ci = (int)nbit(x, y, z);
+        ci |= (int)nbit(x + 1, y, z) << 1;
+        ci |= (int)nbit(x + 1, y + 1, z) << 2;
+        ci |= (int)nbit(x, y + 1, z) << 3;
+        ci |= (int)nbit(x, y, z + 1) << 4;
+        ci |= (int)nbit(x + 1, y, z + 1) << 5;
+        ci |= (int)nbit(x + 1, y + 1, z + 1) << 6;
+        ci |= (int)nbit(x, y + 1, z + 1) << 7;
+        return ci;
+    }
+    /* vertex id of slot s (0,1,2 edge x,y,z; 3 point) of point (x, y, z) */
+    __device__ __forceinline__ uint32_t slot_id(int x, int y, int z, int sl) const
+    {
+        size_t wi = widx(x, y, z);
+        int b = z & 63;
+        uint64_t below = (1ull << b) - 1ull;
+        const uint64_t *m = S4 + 4 * wi;
+        uint64_t m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+        uint32_t id = vbw[wi] + (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below) + __popcll(m2 & below) +
+                                           __popcll(m3 & below));
+        if (sl > 0) id += (uint32_t)((m0 >> b) & 1ull);
+        if (sl > 1) id += (uint32_t)((m1 >> b) & 1ull);
+        if (sl > 2) id += (uint32_t)((m2 >> b) & 1ull);
+        return id;
+    }
+    /* vertex of cube edge e of cube (cx, cy, cz) */
+    __device__ __forceinline__ uint32_t corner_vid(int cx, int cy, int cz, int e) const
+    {
+        int qx = cx + c_edge_low[e][0], qy = cy + c_edge_low[e][1], qz = cz + c_edge_low[e][2];
+        int a = c_edge_axis[e];
+        int rx = qx + (a == 0), ry = qy + (a == 1), rz = qz + (a == 2);
+        bool qneg = nbit(qx, qy, qz);
+        int nx = qneg ? rx : qx, ny = qneg ? ry : qy, nz = qneg ? rz : qz;
+        if (zbit(nx, ny, nz)) return slot_id(nx, ny, nz, 3);
+        return slot_id(qx, qy, qz, a);
+    }
+};
+
+/* bits of word w for z < L */
+__device__ __forceinline__ uint64_t bits_below(int L, int z0)
 {
-    __shared__ uint32_t lds[MC_NT / 64 + 1];
+    int n = L - z0;
+    return n <= 0 ? 0ull : (n >= 64 ? ~0ull : ((1ull << n) - 1ull));
+}
+
+__device__ __forceinline__ uint64_t ld_word(const uint64_t *__restrict__ m, int g, int W, int x, int y, int w)
+{
+    if (x < 0 || y < 0 || x >= g || y >= g || w < 0 || w >= W) return 0ull;
+    return m[((size_t)x * g + y) * W + w];
+}
+
+/* countVertexPerCell MarchingCubes.cu:571-658 in word form: slot masks and per-word counts */
+__global__ void __launch_bounds__(256) k_mc_count(QsGrid G, const uint64_t *__restrict__ N, const uint64_t *__restrict__ Zm,
+                                                  int W, uint64_t *__restrict__ S4, uint32_t *__restrict__ vcnt,
+                                                  uint32_t *__restrict__ tcnt, int x0, int x1)
+{
     const int g = G.g, lim = G.lim;
-    uint64_t base = p0 + (uint64_t)blockIdx.x * MC_B;
-    uint32_t st = 0, sv = 0;
-#pragma unroll
-    for (int k = 0; k < MC_IPT; k++) {
-        uint64_t p = base + (uint64_t)k * MC_NT + threadIdx.x;
-        if (p >= p1) continue;
-        P3 q = unflatten(p, g);
-        unsigned ci = 0;
-        if (q.x < lim && q.y < lim && q.z < lim) {
-            ci = (unsigned)(fat(F, g, q.x, q.y, q.z) < 0.0f);
-            ci += (unsigned)(fat(F, g, q.x + 1, q.y, q.z) < 0.0f) * 2u;
-            ci += (unsigned)(fat(F, g, q.x + 1, q.y + 1, q.z) < 0.0f) * 4u;
-            ci += (unsigned)(fat(F, g, q.x, q.y + 1, q.z) < 0.0f) * 8u;
-            ci += (unsigned)(fat(F, g, q.x, q.y, q.z + 1) < 0.0f) * 16u;
-            ci += (unsigned)(fat(F, g, q.x + 1, q.y, q.z + 1) < 0.0f) * 32u;
-            ci += (unsigned)(fat(F, g, q.x + 1, q.y + 1, q.z + 1) < 0.0f) * 64u;
-            ci += (unsigned)(fat(F, g, q.x, q.y + 1, q.z + 1) < 0.0f) * 128u;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    const int x = x0 + blockIdx.y;
+    if (idx >= g * W || x >= x1) return;
+    const int y = idx / W, w = idx - y * W;
+    const int z0 = w * 64;
+    const size_t wi = ((size_t)x * g + y) * W + w;
+    const uint64_t valid = bits_below(g, z0);
+    const uint64_t wlt = bits_below(lim, z0) & valid;     /* z < lim */
+    const uint64_t wle = bits_below(lim + 1, z0) & valid; /* z <= lim */
+    const uint64_t zm1 = (w == 0) ? ~1ull : ~0ull;         /* z >= 1 */
+
+    const uint64_t n00 = ld_word(N, g, W, x, y, w), n00p = ld_word(N, g, W, x, y, w + 1), n00m = ld_word(N, g, W, x, y, w - 1);
+    const uint64_t z00 = ld_word(Zm, g, W, x, y, w), z00p = ld_word(Zm, g, W, x, y, w + 1);
+    const uint64_t n10 = ld_word(N, g, W, x + 1, y, w), z10 = ld_word(Zm, g, W, x + 1, y, w);
+    const uint64_t n01 = ld_word(N, g, W, x, y + 1, w), z01 = ld_word(Zm, g, W, x, y + 1, w);
+    const uint64_t n00_up = (n00 >> 1) | (n00p << 63); /* bit z = N[z+1] */
+    const uint64_t n00_dn = (n00 << 1) | (n00m >> 63); /* bit z = N[z-1] */
+    const uint64_t z00_up = (z00 >> 1) | (z00p << 63);
+
+    uint64_t ex = 0, ey = 0, ez = 0, pt = 0;
+    if (x + 1 < g && x < lim && y <= lim) {
+        uint64_t nz = (n00 & z10) | (~n00 & z00);
+        ex = (n00 ^ n10) & ~nz & wle;
+    }
+    if (y + 1 < g && x <= lim && y < lim) {
+        uint64_t nz = (n00 & z01) | (~n00 & z00);
+        ey = (n00 ^ n01) & ~nz & wle;
+    }
+    if (x <= lim && y <= lim) {
+        uint64_t nz = (n00 & z00_up) | (~n00 & z00);
+        ez = (n00 ^ n00_up) & ~nz & wlt;
+    }
+    if (z00) {
+        uint64_t nb = 0;
+        if (x + 1 < g && x < lim && y <= lim) nb |= n10 & wle;
+        if (x >= 1 && x - 1 < lim && y <= lim) nb |= ld_word(N, g, W, x - 1, y, w) & wle;
+        if (y + 1 < g && x <= lim && y < lim) nb |= n01 & wle;
+        if (y >= 1 && x <= lim && y - 1 < lim) nb |= ld_word(N, g, W, x, y - 1, w) & wle;
+        if (x <= lim && y <= lim) nb |= (n00_up & wlt) | (n00_dn & zm1 & wle);
+        pt = z00 & nb;
+    }
+    S4[4 * wi + 0] = ex;
+    S4[4 * wi + 1] = ey;
+    S4[4 * wi + 2] = ez;
+    S4[4 * wi + 3] = pt;
+    uint32_t nv = (uint32_t)(__popcll(ex) + __popcll(ey) + __popcll(ez) + __popcll(pt));
+    uint32_t nt = 0;
+    if (x < lim && y < lim) {
+        const uint64_t n11 = ld_word(N, g, W, x + 1, y + 1, w);
+        const uint64_t c0 = n00, c1 = n10, c2 = n11, c3 = n01;
+        const uint64_t c4 = n00_up;
+        const uint64_t c5 = (n10 >> 1) | (ld_word(N, g, W, x + 1, y, w + 1) << 63);
+        const uint64_t c6 = (n11 >> 1) | (ld_word(N, g, W, x + 1, y + 1, w + 1) << 63);
+        const uint64_t c7 = (n01 >> 1) | (ld_word(N, g, W, x, y + 1, w + 1) << 63);
+        const uint64_t all = c0 & c1 & c2 & c3 & c4 & c5 & c6 & c7;
+        const uint64_t any = c0 | c1 | c2 | c3 | c4 | c5 | c6 | c7;
+        uint64_t act = any & ~all & wlt;
+        while (act) {
+            int b = __builtin_ctzll(act);
+            act &= act - 1ull;
+            int ci = (int)((c0 >> b) & 1ull) | (int)((c1 >> b) & 1ull) << 1 | (int)((c2 >> b) & 1ull) << 2 |
+                     (int)((c3 >> b) & 1ull) << 3 | (int)((c4 >> b) & 1ull) << 4 | (int)((c5 >> b) & 1ull) << 5 |
+                     (int)((c6 >> b) & 1ull) << 6 | (int)((c7 >> b) & 1ull) << 7;
+            nt += (uint32_t)tri_count(ci);
         }
-        unsigned m = slot_mask(F, g, lim, q.x, q.y, q.z);
-        pinfo[p] = (uint16_t)(ci | (m << 8));
-        st += tri_count(ci);
-        sv += __popc(m);
     }
-    uint32_t t = block_sum<MC_NT>(st, lds);
-    uint32_t v = block_sum<MC_NT>(sv, lds);
-    if (threadIdx.x == 0) {
-        bs_tri[blockIdx.x] = t;
-        bs_vert[blockIdx.x] = v;
-    }
+    vcnt[wi] = nv;
+    tcnt[wi] = nt;
 }
 
-/* vertex ids and positions (vertexInterp/lerp MarchingCubes.cu:715-734, edge oriented low->high) */
-__global__ void __launch_bounds__(MC_NT) k_mc_verts(QsGrid G, const float *__restrict__ F,
-                                                    const uint16_t *__restrict__ pinfo,
-                                                    const uint32_t *__restrict__ bo_vert, uint32_t *__restrict__ vbase,
-                                                    float *__restrict__ vert, uint64_t p0, uint64_t p1)
+/* list of row words that own a vertex or a cube with triangles */
+__global__ void __launch_bounds__(256) k_mc_active(const uint32_t *__restrict__ vcnt, const uint32_t *__restrict__ tcnt,
+                                                   const uint32_t *__restrict__ apos, uint64_t nw, uint32_t *__restrict__ list)
 {
-    __shared__ uint32_t lds[MC_NT / 64 + 1];
-    const int g = G.g;
+    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nw) return;
+    if (vcnt[i] | tcnt[i]) list[apos[i]] = (uint32_t)i;
+}
+
+__global__ void __launch_bounds__(256) k_mc_flag(const uint32_t *__restrict__ vcnt, const uint32_t *__restrict__ tcnt,
+                                                 uint64_t nw, uint32_t *__restrict__ flag)
+{
+    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nw) return;
+    flag[i] = (vcnt[i] | tcnt[i]) ? 1u : 0u;
+}
+
+/* vertices (vertexInterp/lerp MarchingCubes.cu:715-734, edge oriented low->high) and triangles
+   (generateTriangleVerticesSMEM MarchingCubes.cu:736-851 order); one wavefront per active word */
+__global__ void __launch_bounds__(256) k_mc_emit(QsGrid G, MCView V, const float *__restrict__ F,
+                                                 const uint32_t *__restrict__ list, uint32_t nact,
+                                                 const uint32_t *__restrict__ tbw, float *__restrict__ vert,
+                                                 uint64_t *__restrict__ vkey, int *__restrict__ tri)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint32_t k = blockIdx.x * 4 + wid;
+    if (k >= nact) return;
+    const int g = G.g, W = V.W, lim = G.lim;
+    const uint32_t wi = list[k];
+    const int row = (int)(wi / (uint32_t)W), w = (int)(wi - (uint32_t)row * W);
+    const int x = row / g, y = row - x * g;
+    const int z = w * 64 + lane;
     const float dx = G.dx;
-    uint64_t base = p0 + (uint64_t)blockIdx.x * MC_B;
-    uint32_t run = bo_vert[blockIdx.x];
-    for (int k = 0; k < MC_IPT; k++) {
-        uint64_t p = base + (uint64_t)k * MC_NT + threadIdx.x;
-        unsigned m = (p < p1) ? (unsigned)(pinfo[p] >> 8) : 0u;
-        uint32_t tot;
-        uint32_t pre = block_excl_scan<MC_NT>((uint32_t)__popc(m), &tot, lds);
-        uint32_t v = run + pre;
-        run += tot;
-        if (p >= p1 || !m) continue;
-        vbase[p] = v;
-        P3 q = unflatten(p, g);
-        float A[3] = {grid_pos(G.ox, q.x, dx), grid_pos(G.oy, q.y, dx), grid_pos(G.oz, q.z, dx)};
-        float f0 = fat(F, g, q.x, q.y, q.z);
+    const uint64_t *m = V.S4 + 4 * (size_t)wi;
+    const uint64_t m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t v = V.vbw[wi] + (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below) + __popcll(m2 & below) +
+                                        __popcll(m3 & below));
+    const unsigned sl = (unsigned)((m0 >> lane) & 1ull) | (unsigned)((m1 >> lane) & 1ull) << 1 |
+                        (unsigned)((m2 >> lane) & 1ull) << 2 | (unsigned)((m3 >> lane) & 1ull) << 3;
+    if (sl) {
+        const uint64_t pkey = (((uint64_t)x * g + y) * g + z) << 2;
+        float A[3] = {grid_pos(G.ox, x, dx), grid_pos(G.oy, y, dx), grid_pos(G.oz, z, dx)};
+        float f0 = fat(F, g, x, y, z);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            if (!(m & (1u << a))) continue;
-            float f1 = fat(F, g, q.x + (a == 0), q.y + (a == 1), q.z + (a == 2));
+            if (!(sl & (1u << a))) continue;
+            float f1 = fat(F, g, x + (a == 0), y + (a == 1), z + (a == 2));
             float B[3] = {A[0], A[1], A[2]};
             B[a] = A[a] + dx;
             float t = (0.0f - f0) / (f1 - f0);
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                float dd = B[c] - A[c];
+            for (int c3 = 0; c3 < 3; c3++) {
+                float dd = B[c3] - A[c3];
                 float s = t * dd;
-                vert[3 * (size_t)v + c] = A[c] + s;
+                vert[3 * (size_t)v + c3] = A[c3] + s;
             }
+            vkey[v] = pkey | (uint64_t)a;
             v++;
         }
-        if (m & 8u) {
+        if (sl & 8u) {
             vert[3 * (size_t)v + 0] = A[0];
             vert[3 * (size_t)v + 1] = A[1];
             vert[3 * (size_t)v + 2] = A[2];
+            vkey[v] = pkey | 3ull;
         }
     }
-}
-
-/* triangles: generateTriangleVerticesSMEM MarchingCubes.cu:736-851 order, vertex ids by edge */
-__global__ void __launch_bounds__(MC_NT) k_mc_tris(QsGrid G, const float *__restrict__ F,
-                                                   const uint16_t *__restrict__ pinfo, const uint32_t *__restrict__ vbase,
-                                                   const uint32_t *__restrict__ bo_tri, int *__restrict__ tri, uint64_t p0,
-                                                   uint64_t p1)
-{
-    __shared__ uint32_t lds[MC_NT / 64 + 1];
-    const int g = G.g;
-    uint64_t base = p0 + (uint64_t)blockIdx.x * MC_B;
-    uint32_t run = bo_tri[blockIdx.x];
-    for (int k = 0; k < MC_IPT; k++) {
-        uint64_t p = base + (uint64_t)k * MC_NT + threadIdx.x;
-        unsigned ci = (p < p1) ? (unsigned)(pinfo[p] & 0xFF) : 0u;
-        int nt = tri_count(ci);
-        uint32_t tot;
-        uint32_t pre = block_excl_scan<MC_NT>((uint32_t)nt, &tot, lds);
-        uint32_t t0 = run + pre;
-        run += tot;
-        if (nt == 0) continue;
-        P3 q = unflatten(p, g);
-        uint64_t packed = c_tri[ci];
-        for (int e3 = 0; e3 < 3 * nt; e3++) {
-            int e = tri_entry(packed, e3);
-            int ax = q.x + c_edge_low[e][0], ay = q.y + c_edge_low[e][1], az = q.z + c_edge_low[e][2];
-            int axis = c_edge_axis[e];
-            int bx = ax + (axis == 0), by = ay + (axis == 1), bz = az + (axis == 2);
-            float fa = fat(F, g, ax, ay, az), fb = fat(F, g, bx, by, bz);
-            bool a_neg = fa < 0.0f;
-            float fnn = a_neg ? fb : fa;
-            uint32_t vid;
-            if (fnn == 0.0f) {
-                size_t pn = a_neg ? (((size_t)bx * g + by) * g + bz) : (((size_t)ax * g + ay) * g + az);
-                vid = vbase[pn] + (uint32_t)__popc((pinfo[pn] >> 8) & 7u);
-            } else {
-                size_t pa = ((size_t)ax * g + ay) * g + az;
-                vid = vbase[pa] + (uint32_t)__popc((pinfo[pa] >> 8) & ((1u << axis) - 1u));
-            }
-            tri[3 * (size_t)t0 + e3] = (int)vid;
-        }
+    int ci = 0;
+    if (x < lim && y < lim && z < lim) ci = V.cube_index(x, y, z);
+    const int nt = tri_count(ci);
+    uint32_t incl = wave_incl_scan((uint32_t)nt, lane);
+    uint32_t t0 = tbw[wi] + incl - (uint32_t)nt;
+    if (nt) {
+        const uint64_t packed = c_tri[ci];
+        for (int e3 = 0; e3 < 3 * nt; e3++)
+            tri[3 * (size_t)t0 + e3] = (int)V.corner_vid(x, y, z, tri_entry(packed, e3));
     }
 }
 
@@ -705,72 +1335,92 @@ __global__ void __launch_bounds__(256) k_closest_atom(QsGrid G, const int2 *__re
 /* Laplacian smoothing (SmoothMesh.cpp:45-63, 96-124) in Jacobi form on the GPU                 */
 /*
  * A vertex's neighbour list is, triangle by triangle in triangle order, the two other corners
- * in the order the reference pushes them (corner 0: 1,2; corner 1: 0,2; corner 2: 0,1).  The
- * incidence list (3*t + corner) of each vertex is sorted ascending, which is that order, and
- * the sums run in it, so a pass equals a host pass over the same list that reads the previous
- * pass's positions.
+ * in the order the reference pushes them (corner 0: 1,2; corner 1: 0,2; corner 2: 0,1).  Every
+ * triangle that uses an edge vertex lies in one of the 4 cubes around that edge, every triangle
+ * that uses a point vertex in one of the 8 cubes around that point, and triangle order is cube
+ * order then case-table order.  So the list is walked from the case table, cube by cube in
+ * order, without being stored; the sums run in that order, reading the previous pass.
  */
-__global__ void __launch_bounds__(256) k_inc_count(const int *__restrict__ tri, uint32_t ncorner, uint32_t *__restrict__ deg)
+__device__ __forceinline__ void push_pair(const float *__restrict__ src, uint32_t a, uint32_t b, float &sx, float &sy,
+                                          float &sz)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ncorner) return;
-    atomicAdd(&deg[tri[i]], 1u);
+    const float *q0 = src + 3 * (size_t)a;
+    sx = sx + q0[0];
+    sy = sy + q0[1];
+    sz = sz + q0[2];
+    const float *q1 = src + 3 * (size_t)b;
+    sx = sx + q1[0];
+    sy = sy + q1[1];
+    sz = sz + q1[2];
 }
 
-__global__ void __launch_bounds__(256) k_inc_fill(const int *__restrict__ tri, uint32_t ncorner,
-                                                  const uint32_t *__restrict__ start, uint32_t *__restrict__ cursor,
-                                                  uint32_t *__restrict__ inc)
+__global__ void __launch_bounds__(256) k_smooth(QsGrid G, MCView V, const uint64_t *__restrict__ vkey, uint32_t nv,
+                                                const float *__restrict__ src, float *__restrict__ dst)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ncorner) return;
-    int v = tri[i];
-    uint32_t slot = start[v] + atomicAdd(&cursor[v], 1u);
-    inc[slot] = i;
-}
-
-__global__ void __launch_bounds__(256) k_inc_sort(const uint32_t *__restrict__ start, const uint32_t *__restrict__ deg,
-                                                  uint32_t nv, uint32_t *__restrict__ inc)
-{
-    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
-    uint32_t s = start[v], k = deg[v];
-    for (uint32_t a = s + 1; a < s + k; a++) {
-        uint32_t x = inc[a];
-        uint32_t b = a;
-        while (b > s && inc[b - 1] > x) {
-            inc[b] = inc[b - 1];
-            b--;
-        }
-        inc[b] = x;
-    }
-}
-
-__global__ void __launch_bounds__(256) k_jacobi(const int *__restrict__ tri, const uint32_t *__restrict__ start,
-                                                const uint32_t *__restrict__ deg, const uint32_t *__restrict__ inc,
-                                                uint32_t nv, const float *__restrict__ src, float *__restrict__ dst)
-{
-    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nv) return;
-    uint32_t s = start[v], k = deg[v];
+    const int g = G.g, lim = G.lim;
+    const uint64_t key = vkey[v];
+    const int sl = (int)(key & 3ull);
+    const uint64_t p = key >> 2;
+    const uint64_t xy = p / (uint64_t)g;
+    const int z = (int)(p - xy * (uint64_t)g);
+    const int x = (int)(xy / (uint64_t)g), y = (int)(xy - (uint64_t)x * g);
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for (uint32_t a = s; a < s + k; a++) {
-        uint32_t e = inc[a];
-        uint32_t t = e / 3u, c = e - 3u * t;
-        int n0 = tri[3 * (size_t)t + (c == 0 ? 1 : 0)];
-        int n1 = tri[3 * (size_t)t + (c == 2 ? 1 : 2)];
-        const float *q0 = src + 3 * (size_t)n0;
-        sx = sx + q0[0];
-        sy = sy + q0[1];
-        sz = sz + q0[2];
-        const float *q1 = src + 3 * (size_t)n1;
-        sx = sx + q1[0];
-        sy = sy + q1[1];
-        sz = sz + q1[2];
+    uint32_t cnt = 0;
+    if (sl < 3) {
+        /* the 4 cubes around the edge in cube order, and the edge's number in each */
+        for (int k = 0; k < 4; k++) {
+            const int hi = k >> 1, lo = k & 1; /* offsets -1 / 0 of the two other axes, major first */
+            int cx = x, cy = y, cz = z, e;
+            if (sl == 0) {
+                cy -= 1 - hi;
+                cz -= 1 - lo;
+                e = (1 - hi) * 2 + (1 - lo) * 4; /* corner offset (0,dy,dz): 0, 2, 4, 6 */
+            } else if (sl == 1) {
+                cx -= 1 - hi;
+                cz -= 1 - lo;
+                e = (1 - lo) ? ((1 - hi) ? 5 : 7) : ((1 - hi) ? 1 : 3); /* (dx,0,dz): 3 1 7 5 */
+            } else {
+                cx -= 1 - hi;
+                cy -= 1 - lo;
+                e = (1 - hi) ? ((1 - lo) ? 10 : 9) : ((1 - lo) ? 11 : 8); /* (dx,dy,0): 8 9 10 11 */
+            }
+            if (cx < 0 || cy < 0 || cz < 0 || cx >= lim || cy >= lim || cz >= lim) continue;
+            const int ci = V.cube_index(cx, cy, cz);
+            const int nt = tri_count(ci);
+            const uint64_t packed = c_tri[ci];
+            for (int t = 0; t < nt; t++) {
+                const int e0 = tri_entry(packed, 3 * t), e1 = tri_entry(packed, 3 * t + 1), e2 = tri_entry(packed, 3 * t + 2);
+                if (e0 == e) push_pair(src, V.corner_vid(cx, cy, cz, e1), V.corner_vid(cx, cy, cz, e2), sx, sy, sz);
+                else if (e1 == e) push_pair(src, V.corner_vid(cx, cy, cz, e0), V.corner_vid(cx, cy, cz, e2), sx, sy, sz);
+                else if (e2 == e) push_pair(src, V.corner_vid(cx, cy, cz, e0), V.corner_vid(cx, cy, cz, e1), sx, sy, sz);
+                else continue;
+                cnt += 2;
+            }
+        }
+    } else {
+        /* a zero grid point: the 8 cubes around it, every corner that resolves to this vertex */
+        for (int k = 0; k < 8; k++) {
+            const int cx = x - 1 + (k >> 2), cy = y - 1 + ((k >> 1) & 1), cz = z - 1 + (k & 1);
+            if (cx < 0 || cy < 0 || cz < 0 || cx >= lim || cy >= lim || cz >= lim) continue;
+            const int ci = V.cube_index(cx, cy, cz);
+            const int nt = tri_count(ci);
+            const uint64_t packed = c_tri[ci];
+            for (int t = 0; t < nt; t++) {
+                const uint32_t a = V.corner_vid(cx, cy, cz, tri_entry(packed, 3 * t));
+                const uint32_t b = V.corner_vid(cx, cy, cz, tri_entry(packed, 3 * t + 1));
+                const uint32_t c = V.corner_vid(cx, cy, cz, tri_entry(packed, 3 * t + 2));
+                if (a == v) { push_pair(src, b, c, sx, sy, sz); cnt += 2; }
+                if (b == v) { push_pair(src, a, c, sx, sy, sz); cnt += 2; }
+                if (c == v) { push_pair(src, a, b, sx, sy, sz); cnt += 2; }
+            }
+        }
     }
-    float cnt = (float)(2u * k);
-    dst[3 * (size_t)v + 0] = sx / cnt;
-    dst[3 * (size_t)v + 1] = sy / cnt;
-    dst[3 * (size_t)v + 2] = sz / cnt;
+    const float fc = (float)cnt;
+    dst[3 * (size_t)v + 0] = sx / fc;
+    dst[3 * (size_t)v + 1] = sy / fc;
+    dst[3 * (size_t)v + 2] = sz / fc;
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -828,44 +1478,113 @@ void qs_launch_gather_atoms(const int *perm, const float *pos, const float *rad,
     hipLaunchKernelGGL(k_gather_atoms, dim3(nblk(n, 256)), dim3(256), 0, s, perm, pos, rad, n, sx);
 }
 
-void qs_launch_pass1(QsGrid G, const int2 *cse, const float4 *sx, uint8_t *labels, int x0, int x1, hipStream_t s)
+void qs_launch_pass1(QsGrid G, const uint32_t *cstart, const float4 *sx, uint64_t *outm, uint64_t *shellm, int W, int x0,
+                     int x1, hipStream_t s)
 {
-    dim3 grid(nblk(G.g, 8), nblk(G.g, 8), nblk(x1 - x0, 4));
-    hipLaunchKernelGGL(k_pass1, grid, dim3(256), 0, s, G, cse, sx, labels, x0, x1);
+    /* R rows per wavefront with R * dx <= neighbour-cell size */
+    int R = (int)(G.dxn / G.dx);
+    R = R < 1 ? 1 : (R > P1_RMAX ? P1_RMAX : R);
+    dim3 grid((unsigned)W, nblk(G.g, 4 * R), (unsigned)(x1 - x0));
+    hipLaunchKernelGGL(k_pass1_rows, grid, dim3(256), 0, s, G, cstart, sx, outm, shellm, W, R, x0, x1);
 }
 
-void qs_launch_pass2_tile(QsGrid G, const uint8_t *labels, float *field, const int *offs, const int *norm_start,
-                          const int *enum_end, int x0, int x1, hipStream_t s)
+void qs_launch_pass1_direct(QsGrid G, const int2 *cse, const float4 *sx, uint64_t *outm, uint64_t *shellm, int W, int x0,
+                            int x1, hipStream_t s)
+{
+    dim3 grid((unsigned)W, nblk(G.g, 4), (unsigned)(x1 - x0));
+    hipLaunchKernelGGL(k_pass1_direct, grid, dim3(256), 0, s, G, cse, sx, outm, shellm, W, x0, x1);
+}
+
+/* field pass 2: the fixed-R kernel for r <= 9, else the runtime-r tile kernel (LDS permitting),
+   else the direct window; returns the path taken (0, 1, 2) */
+int qs_launch_pass2(QsGrid G, const uint64_t *outm, const uint64_t *shellm, int W, float *field, uint32_t *negm,
+                    uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end, int x0, int x1, bool direct,
+                    hipStream_t s)
 {
     dim3 grid(nblk(G.g, P2_TZ), nblk(G.g, P2_TY), nblk(x1 - x0, P2_TX));
-    size_t lds = qs_pass2_lds_bytes(G.r);
-    hipLaunchKernelGGL(k_pass2_tile, grid, dim3(256), lds, s, G, labels, field, offs, norm_start, enum_end, x0, x1);
+    if (!direct) {
+#define QS_P2R(RR)                                                                                                         \
+    case RR:                                                                                                               \
+        hipLaunchKernelGGL(k_pass2_r<RR>, grid, dim3(256), 0, s, G, outm, shellm, W, field, negm, zerom, offs, norm_start, \
+                           enum_end, x0, x1);                                                                              \
+        return 0;
+        switch (G.r) {
+            QS_P2R(1)
+            QS_P2R(2)
+            QS_P2R(3)
+            QS_P2R(4)
+            QS_P2R(5)
+            QS_P2R(6)
+            QS_P2R(7)
+            QS_P2R(8)
+            QS_P2R(9)
+        default:
+            break;
+        }
+#undef QS_P2R
+        size_t lds = qs_pass2_lds_bytes(G.r);
+        if (lds <= 65536 && 2 * G.r + P2_TZ <= 64) {
+            hipLaunchKernelGGL(k_pass2_tile<uint16_t>, grid, dim3(256), lds, s, G, outm, shellm, W, field, negm, zerom, offs,
+                               norm_start, enum_end, x0, x1);
+            return 1;
+        }
+    }
+    dim3 gd(nblk(G.g, 64), nblk(G.g, 4), (unsigned)(x1 - x0));
+    hipLaunchKernelGGL(k_pass2_direct, gd, dim3(256), 0, s, G, outm, shellm, W, field, negm, zerom, x0, x1);
+    return 2;
 }
 
-void qs_launch_pass2_direct(QsGrid G, const uint8_t *labels, float *field, int x0, int x1, hipStream_t s)
+void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *outm, uint64_t *shellm, hipStream_t s)
 {
-    dim3 grid(nblk(G.g, 64), nblk(G.g, 4), (unsigned)(x1 - x0));
-    hipLaunchKernelGGL(k_pass2_direct, grid, dim3(256), 0, s, G, labels, field, x0, x1);
+    dim3 grid((unsigned)W, nblk(g, 4), (unsigned)g);
+    hipLaunchKernelGGL(k_labels_to_masks, grid, dim3(256), 0, s, labels, g, W, outm, shellm);
 }
 
-unsigned qs_mc_blocks(uint64_t npoints) { return nblk(npoints, MC_B); }
-
-void qs_launch_mc_count(QsGrid G, const float *F, uint16_t *pinfo, uint32_t *bs_tri, uint32_t *bs_vert, uint64_t p0,
-                        uint64_t p1, hipStream_t s)
+void qs_launch_mc_count(QsGrid G, const uint64_t *N, const uint64_t *Zm, int W, uint64_t *S4, uint32_t *vcnt, uint32_t *tcnt,
+                        int x0, int x1, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mc_count, dim3(qs_mc_blocks(p1 - p0)), dim3(MC_NT), 0, s, G, F, pinfo, bs_tri, bs_vert, p0, p1);
+    hipLaunchKernelGGL(k_mc_count, dim3(nblk((uint64_t)G.g * W, 256), (unsigned)(x1 - x0)), dim3(256), 0, s, G, N, Zm, W, S4,
+                       vcnt, tcnt, x0, x1);
 }
 
-void qs_launch_mc_verts(QsGrid G, const float *F, const uint16_t *pinfo, const uint32_t *bo_vert, uint32_t *vbase,
-                        float *vert, uint64_t p0, uint64_t p1, hipStream_t s)
+void qs_launch_mc_flag(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *flag, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mc_verts, dim3(qs_mc_blocks(p1 - p0)), dim3(MC_NT), 0, s, G, F, pinfo, bo_vert, vbase, vert, p0, p1);
+    hipLaunchKernelGGL(k_mc_flag, dim3(nblk(nw, 256)), dim3(256), 0, s, vcnt, tcnt, nw, flag);
 }
 
-void qs_launch_mc_tris(QsGrid G, const float *F, const uint16_t *pinfo, const uint32_t *vbase, const uint32_t *bo_tri,
-                       int *tri, uint64_t p0, uint64_t p1, hipStream_t s)
+void qs_launch_mc_active(const uint32_t *vcnt, const uint32_t *tcnt, const uint32_t *apos, uint64_t nw, uint32_t *list,
+                         hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mc_tris, dim3(qs_mc_blocks(p1 - p0)), dim3(MC_NT), 0, s, G, F, pinfo, vbase, bo_tri, tri, p0, p1);
+    hipLaunchKernelGGL(k_mc_active, dim3(nblk(nw, 256)), dim3(256), 0, s, vcnt, tcnt, apos, nw, list);
+}
+
+static MCView make_view(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W)
+{
+    MCView V;
+    V.N = N;
+    V.Zm = Zm;
+    V.S4 = S4;
+    V.vbw = vbw;
+    V.g = G.g;
+    V.W = W;
+    V.lim = G.lim;
+    return V;
+}
+
+void qs_launch_mc_emit(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W,
+                       const float *F, const uint32_t *list, uint32_t nact, const uint32_t *tbw, float *vert, uint64_t *vkey,
+                       int *tri, hipStream_t s)
+{
+    if (!nact) return;
+    hipLaunchKernelGGL(k_mc_emit, dim3(nblk(nact, 4)), dim3(256), 0, s, G, make_view(G, N, Zm, S4, vbw, W), F, list, nact, tbw,
+                       vert, vkey, tri);
+}
+
+void qs_launch_smooth(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W,
+                      const uint64_t *vkey, uint32_t nv, const float *src, float *dst, hipStream_t s)
+{
+    if (!nv) return;
+    hipLaunchKernelGGL(k_smooth, dim3(nblk(nv, 256)), dim3(256), 0, s, G, make_view(G, N, Zm, S4, vbw, W), vkey, nv, src, dst);
 }
 
 void qs_launch_closest_atom(QsGrid G, const int2 *cse, const float4 *sx, const int *perm, const float *vert, uint32_t nv,
@@ -873,32 +1592,6 @@ void qs_launch_closest_atom(QsGrid G, const int2 *cse, const float4 *sx, const i
 {
     if (!nv) return;
     hipLaunchKernelGGL(k_closest_atom, dim3(nblk(nv, 256)), dim3(256), 0, s, G, cse, sx, perm, vert, nv, sorted_ids, out);
-}
-
-void qs_launch_inc_count(const int *tri, uint32_t ncorner, uint32_t *deg, hipStream_t s)
-{
-    if (!ncorner) return;
-    hipLaunchKernelGGL(k_inc_count, dim3(nblk(ncorner, 256)), dim3(256), 0, s, tri, ncorner, deg);
-}
-
-void qs_launch_inc_fill(const int *tri, uint32_t ncorner, const uint32_t *start, uint32_t *cursor, uint32_t *inc,
-                        hipStream_t s)
-{
-    if (!ncorner) return;
-    hipLaunchKernelGGL(k_inc_fill, dim3(nblk(ncorner, 256)), dim3(256), 0, s, tri, ncorner, start, cursor, inc);
-}
-
-void qs_launch_inc_sort(const uint32_t *start, const uint32_t *deg, uint32_t nv, uint32_t *inc, hipStream_t s)
-{
-    if (!nv) return;
-    hipLaunchKernelGGL(k_inc_sort, dim3(nblk(nv, 256)), dim3(256), 0, s, start, deg, nv, inc);
-}
-
-void qs_launch_jacobi(const int *tri, const uint32_t *start, const uint32_t *deg, const uint32_t *inc, uint32_t nv,
-                      const float *src, float *dst, hipStream_t s)
-{
-    if (!nv) return;
-    hipLaunchKernelGGL(k_jacobi, dim3(nblk(nv, 256)), dim3(256), 0, s, tri, start, deg, inc, nv, src, dst);
 }
 
 void qs_launch_tri_keep(const int *tri, uint32_t nt, uint32_t *keep, hipStream_t s)

@@ -62,6 +62,54 @@ def test_stages_match_oracle(qs, orc):
     assert np.array_equal(f.view(np.uint32), fld.view(np.uint32))
 
 
+@pytest.mark.parametrize("seed,n,reso", [(31, 1500, 0.5), (32, 800, 0.2), (33, 600, 0.3), (34, 400, 1.0),
+                                         (35, 300, 0.15)])
+def test_field_matches_oracle(qs, orc, seed, n, reso):
+    pos, rad = _ball(qs, seed, n)
+    print("case field seed=%d n=%d reso=%.2f" % (seed, n, reso), flush=True)
+    qs.compute_ses(pos, rad, reso, 0)
+    G, perm, cs, ce, sx, lab, fld = orc.field(pos, rad, reso)
+    assert np.array_equal(qs.fetch("labels", np.uint8, G.g ** 3), lab)
+    assert np.array_equal(qs.fetch("field", np.float32, G.g ** 3).view(np.uint32), fld.view(np.uint32))
+
+
+def _lattice():
+    """atoms on a 1.5 A lattice: many voxel-atom distances tie exactly at the thresholds"""
+    ax = np.arange(-4, 5, dtype=np.float32) * np.float32(1.5)
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32)
+    keep = np.linalg.norm(pos, axis=1) < 6.1
+    pos = pos[keep]
+    rad = np.where(np.arange(len(pos)) % 3 == 0, 1.5, 1.7).astype(np.float32)
+    return pos, rad
+
+
+@pytest.mark.parametrize("reso", [0.5, 0.25, 0.1])
+def test_lattice_ties_match_oracle(qs, orc, reso):
+    pos, rad = _lattice()
+    print("case lattice reso=%.2f" % reso, flush=True)
+    gpu = qs.compute_ses(pos, rad, reso, 1)
+    G, perm, cs, ce, sx, lab, fld = orc.field(pos, rad, reso)
+    assert np.array_equal(qs.fetch("labels", np.uint8, G.g ** 3), lab)
+    assert np.array_equal(qs.fetch("field", np.float32, G.g ** 3).view(np.uint32), fld.view(np.uint32))
+    _same_mesh(gpu, orc.compute_ses(pos, rad, reso, 1))
+
+
+def test_pass1_rows_equal_direct(qs):
+    pos, rad = _ball(qs, 36, 1200)
+    for reso in (0.5, 0.2):
+        print("case pass1 direct reso=%.1f" % reso, flush=True)
+        qs.compute_ses(pos, rad, reso, 0)
+        g = qs.last_grid()["g"]
+        rows = qs.fetch("labels", np.uint8, g ** 3)
+        qs.set_option("pass1_direct", 1)
+        try:
+            qs.compute_ses(pos, rad, reso, 0)
+        finally:
+            qs.set_option("pass1_direct", 0)
+        assert np.array_equal(qs.fetch("labels", np.uint8, g ** 3), rows)
+
+
 def test_pass2_tile_equals_direct(qs, orc):
     pos, rad = _ball(qs, 21, 600)
     for reso in (0.2, 0.5):
