@@ -251,7 +251,7 @@ struct Engine {
     DevBuf key, count, start, cursor, perm, cse, sx;
     DevBuf labels, outm, shellm, field, negm, zerom;
     DevBuf s4, vcnt, tcnt, vbw, tbw, apos, alist, totals;
-    DevBuf vert, vert2, vert3, vkey, tri, tri_out, atom;
+    DevBuf vert, vert2, vert3, vkey, cidx, tbase, tri, tri_out, atom;
     DevBuf keep, keeppos;
     DevBuf scan_tmp;
     DevBuf t_offs, t_norm, t_end;
@@ -260,7 +260,7 @@ struct Engine {
     /* last result */
     QsGrid G{};
     unsigned n_atoms = 0;
-    uint32_t nv = 0, nt_all = 0, nt = 0;
+    uint32_t nv = 0, nt_all = 0, nt = 0, n_active = 0;
     float *res_vert = nullptr;
     int *res_tri = nullptr, *res_atom = nullptr;
 
@@ -423,7 +423,7 @@ int Engine::run(const float *d_pos, const float *d_rad, unsigned n, float reso, 
         float *dst = as<float>(vert2);
         for (int it = 0; it < smooth; it++) {
             qs_launch_smooth(G, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw), W,
-                             as<uint64_t>(vkey), nv, src, dst, stream);
+                             as<uint64_t>(vkey), nv, as<uint8_t>(cidx), as<uint32_t>(tbase), as<int>(tri), src, dst, stream);
             src = dst;
             dst = (dst == as<float>(vert2)) ? as<float>(vert3) : as<float>(vert2);
         }
@@ -451,7 +451,7 @@ int Engine::run(const float *d_pos, const float *d_rad, unsigned n, float reso, 
     return 0;
 }
 
-/* marching cubes on the sign bits of `field` (grid G), results into vert / vkey / tri / atom */
+/* marching cubes on the sign bits of `field` (grid G), results into vert / tri / atom */
 int Engine::mesh_from_field(StageTimer &tm, bool with_atoms)
 {
     const int g = G.g, W = (g + 63) / 64;
@@ -482,8 +482,13 @@ int Engine::mesh_from_field(StageTimer &tm, bool with_atoms)
     QS_TRY(ensure(vert, (size_t)std::max<uint32_t>(nv, 1) * 12));
     QS_TRY(ensure(vkey, (size_t)std::max<uint32_t>(nv, 1) * 8));
     QS_TRY(ensure(tri, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
+    const uint64_t V = (uint64_t)g * g * g;
+    QS_TRY(ensure(cidx, V));
+    QS_TRY(ensure(tbase, V * 4));
+    n_active = nact;
     qs_launch_mc_emit(G, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw), W, as<float>(field),
-                      as<uint32_t>(alist), nact, as<uint32_t>(tbw), as<float>(vert), as<uint64_t>(vkey), as<int>(tri), stream);
+                      as<uint32_t>(alist), nact, as<uint32_t>(tbw), as<float>(vert), as<uint64_t>(vkey), as<int>(tri),
+                      as<uint8_t>(cidx), as<uint32_t>(tbase), stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
     if (with_atoms) {

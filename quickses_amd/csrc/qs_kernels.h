@@ -29,9 +29,10 @@ void qs_launch_mc_active(const uint32_t *vcnt, const uint32_t *tcnt, const uint3
                          hipStream_t s);
 void qs_launch_mc_emit(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W,
                        const float *F, const uint32_t *list, uint32_t nact, const uint32_t *tbw, float *vert, uint64_t *vkey,
-                       int *tri, hipStream_t s);
+                       int *tri, uint8_t *cidx, uint32_t *tbase, hipStream_t s);
 void qs_launch_smooth(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W,
-                      const uint64_t *vkey, uint32_t nv, const float *src, float *dst, hipStream_t s);
+                      const uint64_t *vkey, uint32_t nv, const uint8_t *cidx, const uint32_t *tbase, const int *tri,
+                      const float *src, float *dst, hipStream_t s);
 void qs_launch_closest_atom(QsGrid G, const int2 *cse, const float4 *sx, const int *perm, const float *vert, uint32_t nv,
                             int sorted_ids, int *out, hipStream_t s);
 void qs_launch_tri_keep(const int *tri, uint32_t nt, uint32_t *keep, hipStream_t s);

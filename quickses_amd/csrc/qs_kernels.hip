@@ -1234,40 +1234,138 @@ __global__ void __launch_bounds__(256) k_mc_flag(const uint32_t *__restrict__ vc
     flag[i] = (vcnt[i] | tcnt[i]) ? 1u : 0u;
 }
 
+/* The 3 x 3 x 3 row words around a row word (rows x-1..x+1, y-1..y+1, words w-1..w+1): sign,
+   zero and slot masks and first vertex ids, staged in LDS once per wavefront so that cube
+   indices and corner vertex ids are LDS lookups. */
+struct Hood {
+    uint64_t N[27], Z[27], S[27][4];
+    uint32_t vb[27];
+};
+
+__device__ __forceinline__ void load_hood(Hood &H, const MCView &V, int x, int y, int w, int lane)
+{
+    for (int t = lane; t < 27 * 7; t += 64) {
+        const int e = t / 7, f = t - 7 * (t / 7);
+        const int xx = x + e / 9 - 1, yy = y + (e / 3) % 3 - 1, ww = w + e % 3 - 1;
+        const bool ok = xx >= 0 && yy >= 0 && xx < V.g && yy < V.g && ww >= 0 && ww < V.W;
+        const size_t wi = ok ? (((size_t)xx * V.g + yy) * V.W + ww) : 0;
+        if (f == 0) H.N[e] = ok ? V.N[wi] : 0ull;
+        else if (f == 1) H.Z[e] = ok ? V.Zm[wi] : 0ull;
+        else if (f < 6) H.S[e][f - 2] = ok ? V.S4[4 * wi + (f - 2)] : 0ull;
+        else H.vb[e] = ok ? V.vbw[wi] : 0u;
+    }
+}
+
+struct HoodView {
+    const Hood *H;
+    int x, y, w;
+    __device__ __forceinline__ int ent(int px, int py, int pz) const
+    {
+        return ((px - x + 1) * 3 + (py - y + 1)) * 3 + ((pz >> 6) - w + 1);
+    }
+    __device__ __forceinline__ bool nbit(int px, int py, int pz) const { return (H->N[ent(px, py, pz)] >> (pz & 63)) & 1ull; }
+    __device__ __forceinline__ bool zbit(int px, int py, int pz) const { return (H->Z[ent(px, py, pz)] >> (pz & 63)) & 1ull; }
+    __device__ __forceinline__ int cube_index(int cx, int cy, int cz) const
+    {
+        int ci = (int)nbit(cx, cy, cz);
+        ci |= (int)nbit(cx + 1, cy, cz) << 1;
+        ci |= (int)nbit(cx + 1, cy + 1, cz) << 2;
+        ci |= (int)nbit(cx, cy + 1, cz) << 3;
+        ci |= (int)nbit(cx, cy, cz + 1) << 4;
+        ci |= (int)nbit(cx + 1, cy, cz + 1) << 5;
+        ci |= (int)nbit(cx + 1, cy + 1, cz + 1) << 6;
+        ci |= (int)nbit(cx, cy + 1, cz + 1) << 7;
+        return ci;
+    }
+    __device__ __forceinline__ uint32_t slot_id(int px, int py, int pz, int sl) const
+    {
+        const int e = ent(px, py, pz), b = pz & 63;
+        const uint64_t below = (1ull << b) - 1ull;
+        const uint64_t m0 = H->S[e][0], m1 = H->S[e][1], m2 = H->S[e][2], m3 = H->S[e][3];
+        uint32_t id = H->vb[e] + (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below) + __popcll(m2 & below) +
+                                            __popcll(m3 & below));
+        if (sl > 0) id += (uint32_t)((m0 >> b) & 1ull);
+        if (sl > 1) id += (uint32_t)((m1 >> b) & 1ull);
+        if (sl > 2) id += (uint32_t)((m2 >> b) & 1ull);
+        return id;
+    }
+    __device__ __forceinline__ uint32_t corner_vid(int cx, int cy, int cz, int e) const
+    {
+        const int qx = cx + c_edge_low[e][0], qy = cy + c_edge_low[e][1], qz = cz + c_edge_low[e][2];
+        const int a = c_edge_axis[e];
+        const bool qneg = nbit(qx, qy, qz);
+        const int nx = qneg ? qx + (a == 0) : qx, ny = qneg ? qy + (a == 1) : qy, nz = qneg ? qz + (a == 2) : qz;
+        if (zbit(nx, ny, nz)) return slot_id(nx, ny, nz, 3);
+        return slot_id(qx, qy, qz, a);
+    }
+};
+
+__device__ __forceinline__ void decode_word(uint32_t wi, int g, int W, int &x, int &y, int &w)
+{
+    const int row = (int)(wi / (uint32_t)W);
+    w = (int)(wi - (uint32_t)row * W);
+    x = row / g;
+    y = row - x * g;
+}
+
+/* the 2 x 2 x 2 row words (rows x..x+1, y..y+1, words w..w+1) that a word's cubes touch */
+__device__ __forceinline__ void load_hood8(Hood &H, const MCView &V, int x, int y, int w, int lane)
+{
+    if (lane < 8 * 7) {
+        const int e8 = lane / 7, f = lane - 7 * (lane / 7);
+        const int dx = e8 >> 2, dy = (e8 >> 1) & 1, dw = e8 & 1;
+        const int e = ((dx + 1) * 3 + (dy + 1)) * 3 + (dw + 1); /* slot in the 3x3x3 layout */
+        const int xx = x + dx, yy = y + dy, ww = w + dw;
+        const bool ok = xx < V.g && yy < V.g && ww < V.W;
+        const size_t wi = ok ? (((size_t)xx * V.g + yy) * V.W + ww) : 0;
+        if (f == 0) H.N[e] = ok ? V.N[wi] : 0ull;
+        else if (f == 1) H.Z[e] = ok ? V.Zm[wi] : 0ull;
+        else if (f < 6) H.S[e][f - 2] = ok ? V.S4[4 * wi + (f - 2)] : 0ull;
+        else H.vb[e] = ok ? V.vbw[wi] : 0u;
+    }
+}
+
 /* vertices (vertexInterp/lerp MarchingCubes.cu:715-734, edge oriented low->high) and triangles
-   (generateTriangleVerticesSMEM MarchingCubes.cu:736-851 order); one wavefront per active word */
+   (generateTriangleVerticesSMEM MarchingCubes.cu:736-851 order); one wavefront per active word.
+   Each polygonised cube also records its case and first triangle id (read by the smoothing
+   pass), and each vertex its (point, slot) key. */
 __global__ void __launch_bounds__(256) k_mc_emit(QsGrid G, MCView V, const float *__restrict__ F,
                                                  const uint32_t *__restrict__ list, uint32_t nact,
                                                  const uint32_t *__restrict__ tbw, float *__restrict__ vert,
-                                                 uint64_t *__restrict__ vkey, int *__restrict__ tri)
+                                                 uint64_t *__restrict__ vkey, int *__restrict__ tri,
+                                                 uint8_t *__restrict__ cidx, uint32_t *__restrict__ tbase)
 {
+    __shared__ Hood hoods[4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint32_t k = blockIdx.x * 4 + wid;
-    if (k >= nact) return;
+    const bool active = k < nact;
     const int g = G.g, W = V.W, lim = G.lim;
-    const uint32_t wi = list[k];
-    const int row = (int)(wi / (uint32_t)W), w = (int)(wi - (uint32_t)row * W);
-    const int x = row / g, y = row - x * g;
+    const uint32_t wi = active ? list[k] : 0u;
+    int x = 0, y = 0, w = 0;
+    decode_word(wi, g, W, x, y, w);
+    if (active) load_hood8(hoods[wid], V, x, y, w, lane);
+    __syncthreads();
+    if (!active) return;
+    const HoodView hv{&hoods[wid], x, y, w};
     const int z = w * 64 + lane;
     const float dx = G.dx;
-    const uint64_t *m = V.S4 + 4 * (size_t)wi;
-    const uint64_t m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
-    const uint64_t below = (1ull << lane) - 1ull;
-    uint32_t v = V.vbw[wi] + (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below) + __popcll(m2 & below) +
-                                        __popcll(m3 & below));
+    const Hood &H = hoods[wid];
+    const uint64_t m0 = H.S[13][0], m1 = H.S[13][1], m2 = H.S[13][2], m3 = H.S[13][3];
     const unsigned sl = (unsigned)((m0 >> lane) & 1ull) | (unsigned)((m1 >> lane) & 1ull) << 1 |
                         (unsigned)((m2 >> lane) & 1ull) << 2 | (unsigned)((m3 >> lane) & 1ull) << 3;
+    const size_t pidx = ((size_t)x * g + y) * g + z;
     if (sl) {
-        const uint64_t pkey = (((uint64_t)x * g + y) * g + z) << 2;
+        uint32_t v = hv.slot_id(x, y, z, 0);
+        const uint64_t pkey = (uint64_t)pidx << 2;
         float A[3] = {grid_pos(G.ox, x, dx), grid_pos(G.oy, y, dx), grid_pos(G.oz, z, dx)};
-        float f0 = fat(F, g, x, y, z);
+        const float f0 = fat(F, g, x, y, z);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             if (!(sl & (1u << a))) continue;
-            float f1 = fat(F, g, x + (a == 0), y + (a == 1), z + (a == 2));
+            const float f1 = fat(F, g, x + (a == 0), y + (a == 1), z + (a == 2));
             float B[3] = {A[0], A[1], A[2]};
             B[a] = A[a] + dx;
-            float t = (0.0f - f0) / (f1 - f0);
+            const float t = (0.0f - f0) / (f1 - f0);
 #pragma unroll
             for (int c3 = 0; c3 < 3; c3++) {
                 float dd = B[c3] - A[c3];
@@ -1285,14 +1383,15 @@ __global__ void __launch_bounds__(256) k_mc_emit(QsGrid G, MCView V, const float
         }
     }
     int ci = 0;
-    if (x < lim && y < lim && z < lim) ci = V.cube_index(x, y, z);
+    if (x < lim && y < lim && z < lim) ci = hv.cube_index(x, y, z);
     const int nt = tri_count(ci);
-    uint32_t incl = wave_incl_scan((uint32_t)nt, lane);
-    uint32_t t0 = tbw[wi] + incl - (uint32_t)nt;
+    const uint32_t incl = wave_incl_scan((uint32_t)nt, lane);
+    const uint32_t t0 = tbw[wi] + incl - (uint32_t)nt;
     if (nt) {
+        cidx[pidx] = (uint8_t)ci;
+        tbase[pidx] = t0;
         const uint64_t packed = c_tri[ci];
-        for (int e3 = 0; e3 < 3 * nt; e3++)
-            tri[3 * (size_t)t0 + e3] = (int)V.corner_vid(x, y, z, tri_entry(packed, e3));
+        for (int e3 = 0; e3 < 3 * nt; e3++) tri[3 * (size_t)t0 + e3] = (int)hv.corner_vid(x, y, z, tri_entry(packed, e3));
     }
 }
 
@@ -1354,8 +1453,13 @@ __device__ __forceinline__ void push_pair(const float *__restrict__ src, uint32_
     sz = sz + q1[2];
 }
 
+/* one Jacobi pass, one thread per vertex.  Polygonised cubes carry their case (cidx) and first
+   triangle id (tbase), so each incident triangle's other corners are read from the triangle
+   array. */
 __global__ void __launch_bounds__(256) k_smooth(QsGrid G, MCView V, const uint64_t *__restrict__ vkey, uint32_t nv,
-                                                const float *__restrict__ src, float *__restrict__ dst)
+                                                const uint8_t *__restrict__ cidx, const uint32_t *__restrict__ tbase,
+                                                const int *__restrict__ tri, const float *__restrict__ src,
+                                                float *__restrict__ dst)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
@@ -1369,7 +1473,8 @@ __global__ void __launch_bounds__(256) k_smooth(QsGrid G, MCView V, const uint64
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     uint32_t cnt = 0;
     if (sl < 3) {
-        /* the 4 cubes around the edge in cube order, and the edge's number in each */
+        /* the 4 cubes around the edge in cube order, and the edge's number in each; all of them
+           that are polygonised contain the sign change, so their records exist */
         for (int k = 0; k < 4; k++) {
             const int hi = k >> 1, lo = k & 1; /* offsets -1 / 0 of the two other axes, major first */
             int cx = x, cy = y, cz = z, e;
@@ -1387,30 +1492,33 @@ __global__ void __launch_bounds__(256) k_smooth(QsGrid G, MCView V, const uint64
                 e = (1 - hi) ? ((1 - lo) ? 10 : 9) : ((1 - lo) ? 11 : 8); /* (dx,dy,0): 8 9 10 11 */
             }
             if (cx < 0 || cy < 0 || cz < 0 || cx >= lim || cy >= lim || cz >= lim) continue;
-            const int ci = V.cube_index(cx, cy, cz);
+            const size_t c = ((size_t)cx * g + cy) * g + cz;
+            const int ci = cidx[c];
+            const uint32_t tb = tbase[c];
             const int nt = tri_count(ci);
             const uint64_t packed = c_tri[ci];
             for (int t = 0; t < nt; t++) {
-                const int e0 = tri_entry(packed, 3 * t), e1 = tri_entry(packed, 3 * t + 1), e2 = tri_entry(packed, 3 * t + 2);
-                if (e0 == e) push_pair(src, V.corner_vid(cx, cy, cz, e1), V.corner_vid(cx, cy, cz, e2), sx, sy, sz);
-                else if (e1 == e) push_pair(src, V.corner_vid(cx, cy, cz, e0), V.corner_vid(cx, cy, cz, e2), sx, sy, sz);
-                else if (e2 == e) push_pair(src, V.corner_vid(cx, cy, cz, e0), V.corner_vid(cx, cy, cz, e1), sx, sy, sz);
-                else continue;
+                const int j = (tri_entry(packed, 3 * t) == e) ? 0
+                              : (tri_entry(packed, 3 * t + 1) == e) ? 1
+                              : (tri_entry(packed, 3 * t + 2) == e) ? 2 : -1;
+                if (j < 0) continue;
+                const int *tt = tri + 3 * (size_t)(tb + t);
+                push_pair(src, (uint32_t)tt[j == 0 ? 1 : 0], (uint32_t)tt[j == 2 ? 1 : 2], sx, sy, sz);
                 cnt += 2;
             }
         }
     } else {
-        /* a zero grid point: the 8 cubes around it, every corner that resolves to this vertex */
+        /* a zero grid point: the 8 cubes around it, every corner that is this vertex */
         for (int k = 0; k < 8; k++) {
             const int cx = x - 1 + (k >> 2), cy = y - 1 + ((k >> 1) & 1), cz = z - 1 + (k & 1);
             if (cx < 0 || cy < 0 || cz < 0 || cx >= lim || cy >= lim || cz >= lim) continue;
             const int ci = V.cube_index(cx, cy, cz);
             const int nt = tri_count(ci);
-            const uint64_t packed = c_tri[ci];
+            if (!nt) continue;
+            const uint32_t tb = tbase[((size_t)cx * g + cy) * g + cz];
             for (int t = 0; t < nt; t++) {
-                const uint32_t a = V.corner_vid(cx, cy, cz, tri_entry(packed, 3 * t));
-                const uint32_t b = V.corner_vid(cx, cy, cz, tri_entry(packed, 3 * t + 1));
-                const uint32_t c = V.corner_vid(cx, cy, cz, tri_entry(packed, 3 * t + 2));
+                const int *tt = tri + 3 * (size_t)(tb + t);
+                const uint32_t a = (uint32_t)tt[0], b = (uint32_t)tt[1], c = (uint32_t)tt[2];
                 if (a == v) { push_pair(src, b, c, sx, sy, sz); cnt += 2; }
                 if (b == v) { push_pair(src, a, c, sx, sy, sz); cnt += 2; }
                 if (c == v) { push_pair(src, a, b, sx, sy, sz); cnt += 2; }
@@ -1573,18 +1681,20 @@ static MCView make_view(QsGrid G, const uint64_t *N, const uint64_t *Zm, const u
 
 void qs_launch_mc_emit(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W,
                        const float *F, const uint32_t *list, uint32_t nact, const uint32_t *tbw, float *vert, uint64_t *vkey,
-                       int *tri, hipStream_t s)
+                       int *tri, uint8_t *cidx, uint32_t *tbase, hipStream_t s)
 {
     if (!nact) return;
     hipLaunchKernelGGL(k_mc_emit, dim3(nblk(nact, 4)), dim3(256), 0, s, G, make_view(G, N, Zm, S4, vbw, W), F, list, nact, tbw,
-                       vert, vkey, tri);
+                       vert, vkey, tri, cidx, tbase);
 }
 
 void qs_launch_smooth(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W,
-                      const uint64_t *vkey, uint32_t nv, const float *src, float *dst, hipStream_t s)
+                      const uint64_t *vkey, uint32_t nv, const uint8_t *cidx, const uint32_t *tbase, const int *tri,
+                      const float *src, float *dst, hipStream_t s)
 {
     if (!nv) return;
-    hipLaunchKernelGGL(k_smooth, dim3(nblk(nv, 256)), dim3(256), 0, s, G, make_view(G, N, Zm, S4, vbw, W), vkey, nv, src, dst);
+    hipLaunchKernelGGL(k_smooth, dim3(nblk(nv, 256)), dim3(256), 0, s, G, make_view(G, N, Zm, S4, vbw, W), vkey, nv, cidx,
+                       tbase, tri, src, dst);
 }
 
 void qs_launch_closest_atom(QsGrid G, const int2 *cse, const float4 *sx, const int *perm, const float *vert, uint32_t nv,
