@@ -59,8 +59,6 @@ int API_lastError(void);
 const char *qs_last_error_message(void);
 
 /* Options (process-wide, like the reference's globals):
- *   "slice"           x-slab thickness in grid points, 0 = whole grid in one slab (default);
- *                     the mesh does not depend on it (the reference's -s flag)
  *   "lim_mode"        0 = polygonise every cube whose corners are valid (default),
  *                     1 = stop at cube g-2r-2 as the reference's single slice does
  *   "atom_ids_sorted" 0 = input atom index (default), 1 = cell-sorted index (reference)
@@ -79,14 +77,39 @@ int qs_get_option(const char *key);
 int qs_compute_device(const float *d_pos, const float *d_rad, unsigned int N, float resoSES, int doSmoothing,
                       unsigned int *nvert, unsigned int *nidx, float **d_vert, int **d_tri, int **d_atom_id);
 
+/* Batched form: nmol molecules in one sequence of launches.  Molecule m owns atoms
+ * [atom_off[m], atom_off[m+1]) of the concatenated pos (3 floats per atom) / rad arrays and is
+ * meshed at voxel size reso[m]; its mesh equals what qs_compute_device gives for it alone.
+ * atom_off has nmol+1 entries (host memory), starting at 0.  Molecules with fewer than 2 atoms
+ * get an empty mesh.  Results stay on the device (library buffers, valid until the next call):
+ * vertices of molecule m are [vert_off[m], vert_off[m+1]) of *d_vert, its triangles
+ * [tri_off[m], tri_off[m+1]) of *d_tri (3 ints each, holding indices into *d_vert), atom ids
+ * local to the molecule.  vert_off / tri_off (host, nmol+1 entries each) may be NULL.
+ * Synchronous.  Returns 0 or an error code. */
+int qs_compute_batch_device(const float *d_pos, const float *d_rad, const unsigned int *atom_off, int nmol,
+                            const float *reso, int doSmoothing, unsigned int *vert_off, unsigned int *tri_off,
+                            float **d_vert, int **d_tri, int **d_atom_id);
+
+/* Batched form on host arrays: per-molecule vertex and triangle counts into nvert[m], ntri[m]
+ * (host, nmol entries each, may be NULL); qs_batch_mesh then copies molecule m's mesh out with
+ * vertex indices local to the molecule (vert: 3*nvert[m] floats, tri: 3*ntri[m] ints, atom_id:
+ * nvert[m] ints; any may be NULL). */
+int qs_compute_batch(const float *pos, const float *rad, const unsigned int *atom_off, int nmol, const float *reso,
+                     int doSmoothing, unsigned int *nvert, unsigned int *ntri);
+int qs_batch_mesh(int m, float *vert, int *tri, int *atom_id);
+
 /* Per-stage GPU times of the last computation (needs option "timing" or "trace"): stage names
  * comma-separated into names, milliseconds into ms; returns the number of stages. */
 int qs_stage_times(char *names, int names_cap, float *ms, int max_stages);
 
-/* Grid of the last computation: fparams = {origin x,y,z, dx, dxn}, iparams = {g, gn, r, lim}. */
+/* Grid of the last computation (its first molecule, or molecule m of the last batch):
+ * fparams = {origin x,y,z, dx, dxn}, iparams = {g, gn, r, lim}.  Returns 0, or -1 for no such
+ * molecule. */
 int qs_last_grid(float *fparams, int *iparams);
+int qs_batch_grid(int m, float *fparams, int *iparams);
 
-/* Copies of the last computation's intermediate arrays to host memory (stage-wise checks):
+/* Copies of the last computation's intermediate arrays to host memory (stage-wise checks; for
+ * a batch, the concatenated arrays, and "labels" of its first molecule):
  *   "labels" uint8[g^3] (0 outside, 1 shell, 2 inside), "field" float[g^3],
  *   "perm" int32[N] (cell-sorted order), "cells" int32[2*gn^3] (start,end pairs),
  *   "vert_unsmoothed" float[3*NVert], "tri_all" int32[3*T] (before the degenerate filter).

@@ -52,7 +52,16 @@ def lib():
     L.qs_compute_device.argtypes = [vp, vp, ctypes.c_uint, ctypes.c_float, ctypes.c_int, u32p, u32p,
                                     ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.qs_compute_device.restype = ctypes.c_int
+    L.qs_compute_batch_device.argtypes = [vp, vp, u32p, ctypes.c_int, f3p, ctypes.c_int, u32p, u32p,
+                                          ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    L.qs_compute_batch_device.restype = ctypes.c_int
+    L.qs_compute_batch.argtypes = [f3p, f3p, u32p, ctypes.c_int, f3p, ctypes.c_int, u32p, u32p]
+    L.qs_compute_batch.restype = ctypes.c_int
+    L.qs_batch_mesh.argtypes = [ctypes.c_int, vp, vp, vp]
+    L.qs_batch_mesh.restype = ctypes.c_int
     L.qs_last_grid.argtypes = [f3p, ctypes.POINTER(ctypes.c_int)]
+    L.qs_batch_grid.argtypes = [ctypes.c_int, f3p, ctypes.POINTER(ctypes.c_int)]
+    L.qs_batch_grid.restype = ctypes.c_int
     L.qs_stage_times.argtypes = [ctypes.c_char_p, ctypes.c_int, f3p, ctypes.c_int]
     L.qs_stage_times.restype = ctypes.c_int
     L.qs_fetch.argtypes = [ctypes.c_char_p, vp, ctypes.c_int64]
@@ -137,6 +146,79 @@ def compute_device(d_pos, d_rad, n, reso=0.5, smooth=1):
     if rc:
         raise QuickSESError("quickses error %d: %s" % (rc, L.qs_last_error_message().decode()))
     return nv.value, ni.value, pv.value, pt.value, pa.value
+
+
+def _batch_desc(counts, reso):
+    nmol = len(counts)
+    off = np.zeros(nmol + 1, dtype=np.uint32)
+    off[1:] = np.cumsum(np.asarray(counts, dtype=np.uint64))
+    if np.ndim(reso) == 0:
+        reso = np.full(nmol, reso, dtype=np.float32)
+    reso = np.ascontiguousarray(reso, dtype=np.float32)
+    if reso.shape != (nmol,):
+        raise ValueError("one voxel size per molecule")
+    return off, reso
+
+
+def compute_batch(molecules, reso=0.5, smooth=1):
+    """qs_compute_batch + qs_batch_mesh: molecules is a list of (pos (N,3), rad (N,)) pairs,
+    reso a voxel size or one per molecule.  Returns a list of (vertices, triangles, atom_ids),
+    each equal to compute_ses on that molecule alone."""
+    L = lib()
+    pos = [np.ascontiguousarray(p, dtype=np.float32).reshape(-1, 3) for p, _ in molecules]
+    rad = [np.ascontiguousarray(r, dtype=np.float32).reshape(-1) for _, r in molecules]
+    for p, r in zip(pos, rad):
+        if p.shape[0] != r.shape[0]:
+            raise ValueError("pos and rad disagree on the atom count")
+    off, reso = _batch_desc([p.shape[0] for p in pos], reso)
+    nmol = len(pos)
+    P = np.ascontiguousarray(np.concatenate(pos) if nmol else np.zeros((0, 3), np.float32))
+    R = np.ascontiguousarray(np.concatenate(rad) if nmol else np.zeros(0, np.float32))
+    nv = np.zeros(max(nmol, 1), dtype=np.uint32)
+    nt = np.zeros(max(nmol, 1), dtype=np.uint32)
+    u32p = ctypes.POINTER(ctypes.c_uint)
+    rc = L.qs_compute_batch(_fptr(P), _fptr(R), off.ctypes.data_as(u32p), nmol, _fptr(reso), int(smooth),
+                            nv.ctypes.data_as(u32p), nt.ctypes.data_as(u32p))
+    _check(rc)
+    out = []
+    for m in range(nmol):
+        V = np.empty((int(nv[m]), 3), dtype=np.float32)
+        T = np.empty((int(nt[m]), 3), dtype=np.int32)
+        A = np.empty(int(nv[m]), dtype=np.int32)
+        rc = L.qs_batch_mesh(m, ctypes.c_void_p(V.ctypes.data), ctypes.c_void_p(T.ctypes.data),
+                             ctypes.c_void_p(A.ctypes.data))
+        _check(rc)
+        out.append((V, T, A))
+    return out
+
+
+def compute_batch_device(d_pos, d_rad, counts, reso=0.5, smooth=1):
+    """qs_compute_batch_device on device pointers: counts = atoms per molecule (concatenated in
+    d_pos / d_rad).  Returns (vert_off, tri_off, d_vert, d_tri, d_atom): per-molecule offsets
+    (nmol+1 each, triangles counted in triangles) and library-owned device pointers."""
+    L = lib()
+    off, reso = _batch_desc(counts, reso)
+    nmol = len(counts)
+    vo = np.zeros(nmol + 1, dtype=np.uint32)
+    to = np.zeros(nmol + 1, dtype=np.uint32)
+    u32p = ctypes.POINTER(ctypes.c_uint)
+    pv, pt, pa = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    rc = L.qs_compute_batch_device(ctypes.c_void_p(d_pos), ctypes.c_void_p(d_rad), off.ctypes.data_as(u32p), nmol,
+                                   _fptr(reso), int(smooth), vo.ctypes.data_as(u32p), to.ctypes.data_as(u32p),
+                                   ctypes.byref(pv), ctypes.byref(pt), ctypes.byref(pa))
+    if rc:
+        raise QuickSESError("quickses error %d: %s" % (rc, L.qs_last_error_message().decode()))
+    return vo, to, pv.value, pt.value, pa.value
+
+
+def batch_grid(m):
+    """Grid of molecule m of the last batch."""
+    L = lib()
+    fp = (ctypes.c_float * 5)()
+    ip = (ctypes.c_int * 4)()
+    if L.qs_batch_grid(int(m), fp, ip) != 0:
+        raise QuickSESError("no molecule %d in the last batch" % m)
+    return dict(origin=(fp[0], fp[1], fp[2]), dx=fp[3], dxn=fp[4], g=ip[0], gn=ip[1], r=ip[2], lim=ip[3])
 
 
 def stage_times():

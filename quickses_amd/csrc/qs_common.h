@@ -24,4 +24,26 @@ struct QsGrid {
     int lim;          /* cubes (i,j,k) with i,j,k < lim are polygonised */
 };
 
+/* One molecule of a batch: its grid and where its rows live in the concatenated arrays. */
+struct QsMol {
+    QsGrid G;
+    uint32_t atom_off, n_atoms; /* atoms (input order and cell order) */
+    uint32_t cell_off, W;       /* first neighbour cell; 64-voxel words per grid row */
+    uint64_t word_off;          /* first row word (bit masks, slot masks, counts) */
+    uint64_t vox_off;           /* first voxel (field, cube records) */
+    uint32_t vert_off, n_vert;  /* vertices (set after counting) */
+    uint32_t tri_off, n_tri;    /* triangles (set after counting) */
+    int rows_per_wave;          /* field pass 1: rows per wavefront */
+    int pad;
+};
+
+/* A launch over a batch: block b belongs to molecule mid[j] (or j) where
+   bpre[j] <= b < bpre[j+1]. */
+struct QsBatch {
+    const QsMol *mols;
+    const uint32_t *bpre;
+    const int *mid;
+    int n;
+};
+
 #endif

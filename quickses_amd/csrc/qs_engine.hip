@@ -8,8 +8,10 @@
  *   MC count, scans, vertices, triangles (CudaSurf.cu:373-458)
  *   closest atom (CudaSurf.cu:460-467), Laplacian passes (CudaSurf.cu:743)
  *   degenerate filter (CudaSurf.cu:820-829)
- * The whole grid is one slice; x-slabs ("slice" option) bound the per-slab field memory and
- * give the same mesh.
+ * The whole grid is one slice.  A batch of molecules runs as one sequence of launches over
+ * concatenated arrays: each molecule owns a range of neighbour cells, row words, voxels,
+ * vertices and triangles (QsMol), and each launch maps its blocks to molecules by a prefix of
+ * per-molecule block counts (QsBatch).
  */
 #include <hip/hip_runtime.h>
 
@@ -63,7 +65,6 @@ int set_error(int code, const char *fmt, ...)
 /* options                                                                                      */
 
 struct Options {
-    int slice = 0;
     int lim_mode = 0;
     int atom_ids_sorted = 0;
     int pass1_direct = 0;
@@ -243,23 +244,50 @@ struct StageTimer {
     }
 };
 
+/* host description of one molecule of a batch */
+struct MolIn {
+    uint32_t atom_off = 0, n = 0;
+    float reso = 0.5f;
+};
+
+/* field-pass-2 candidate tables on the device, kept for reuse by later calls */
+struct P2Tables {
+    int r;
+    float dx, cmax;
+    DevBuf offs, norm, end;
+};
+
+/* molecules sharing a field-pass-2 kernel (path, r) and candidate tables (dx, coordinate range) */
+struct P2Group {
+    int path, r;
+    float dx, cmax;
+    std::vector<int> mols;
+    size_t bpre_at = 0, mid_at = 0;
+    uint32_t nblocks = 0;
+    P2Tables *tab = nullptr;
+};
+
 struct Engine {
     int device = -1;
     hipStream_t stream = nullptr;
 
-    DevBuf in_pos, in_rad, bb_part, bb_out;
-    DevBuf key, count, start, cursor, perm, cse, sx;
+    DevBuf in_pos, in_rad, d_off, bb_part, bb_out, d_mols, d_meta;
+    DevBuf key, count, start, cursor, perm, cse, sx, aux;
     DevBuf labels, outm, shellm, field, negm, zerom;
-    DevBuf s4, vcnt, tcnt, vbw, tbw, apos, alist, totals;
+    DevBuf s4, vcnt, tcnt, vbw, tbw, apos, alist, totals, bounds, bidx;
     DevBuf vert, vert2, vert3, vkey, cidx, tbase, tri, tri_out, atom;
     DevBuf keep, keeppos;
     DevBuf scan_tmp;
-    DevBuf t_offs, t_norm, t_end;
-    OffsetTables tables;
+    std::vector<P2Group> groups;
+    std::vector<P2Tables *> tables;
+    /* host staging of uploads that kernels read later in the same call */
+    std::vector<uint32_t> h_meta, h_vpre;
 
     /* last result */
-    QsGrid G{};
-    unsigned n_atoms = 0;
+    std::vector<QsMol> mols;
+    std::vector<uint32_t> vert_off, tri_off; /* per molecule, nmol + 1; triangles after the filter */
+    uint32_t n_atoms_total = 0;
+    uint64_t ncell_total = 0, nw_total = 0, vox_total = 0;
     uint32_t nv = 0, nt_all = 0, nt = 0, n_active = 0;
     float *res_vert = nullptr;
     int *res_tri = nullptr, *res_atom = nullptr;
@@ -281,7 +309,6 @@ struct Engine {
             if (d_total) QS_HIP(hipMemsetAsync(d_total, 0, sizeof(uint32_t), stream));
             return 0;
         }
-        /* level sizes */
         std::vector<uint64_t> lv;
         uint64_t m = n;
         while (qs_scan_tiles(m) > 1) {
@@ -297,7 +324,6 @@ struct Engine {
             lvl_ptr.push_back(p);
             p += (x + 63) / 64 * 64;
         }
-        /* reduce down the levels */
         const uint32_t *src = in;
         uint64_t len = n;
         for (size_t l = 0; l < lv.size(); l++) {
@@ -305,7 +331,6 @@ struct Engine {
             src = lvl_ptr[l];
             len = lv[l];
         }
-        /* scan back up */
         for (int l = (int)lv.size() - 1; l >= 0; l--) {
             const uint32_t *offs = (l + 1 < (int)lv.size()) ? lvl_ptr[l + 1] : nullptr;
             qs_launch_tile_scan(lvl_ptr[l], lvl_ptr[l], lv[l], offs, nullptr, stream);
@@ -315,147 +340,205 @@ struct Engine {
         return 0;
     }
 
-    int run(const float *d_pos, const float *d_rad, unsigned n, float reso, int smooth, const BBox *host_bbox);
-    int mesh_from_field(StageTimer &tm, bool with_atoms);
-    int pass2_tables();
+    QsBatch batch(size_t bpre_at, size_t mid_at, int n) const
+    {
+        QsBatch B;
+        B.mols = (const QsMol *)d_mols.p;
+        B.bpre = (const uint32_t *)d_meta.p + bpre_at;
+        B.mid = mid_at ? (const int *)((const uint32_t *)d_meta.p + mid_at) : nullptr;
+        B.n = n;
+        return B;
+    }
+
+    int setup(const std::vector<MolIn> &in, const std::vector<BBox> &bb);
+    int build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &at_wpre,
+                   uint32_t &nb_atoms, uint32_t &nb_p1, uint32_t &nb_mc);
+    int run_batch(const float *d_pos, const float *d_rad, const std::vector<MolIn> &in, int smooth,
+                  const std::vector<BBox> *host_bbox);
+    int field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm);
+    int mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer &tm, bool with_atoms);
+    int group_tables(P2Group &grp);
 };
 
 Engine g_eng;
 
-int Engine::run(const float *d_pos, const float *d_rad, unsigned n, float reso, int smooth, const BBox *host_bbox)
+/* per-molecule grids and offsets into the concatenated arrays */
+int Engine::setup(const std::vector<MolIn> &in, const std::vector<BBox> &bb)
 {
-    nv = nt = nt_all = 0;
-    n_atoms = n;
-    res_vert = nullptr;
-    res_tri = nullptr;
-    res_atom = nullptr;
-    if (!(reso > 0.0f) || !std::isfinite(reso)) return set_error(3, "voxel size must be a positive finite number");
-    if (n <= 1) return 0; /* CudaSurf.cu:539-543 */
-    if (n > (1u << 30)) return set_error(3, "too many atoms (%u)", n);
-
-    StageTimer tm;
-    tm.on = g_opt.trace != 0 || g_opt.timing != 0;
-    tm.s = stream;
-    tm.mark("start");
-
-    /* ---- bbox and grid ---- */
-    BBox bb;
-    if (host_bbox) {
-        bb = *host_bbox;
-    } else {
-        int nparts = (int)std::min<unsigned>((n + 255) / 256, 1024);
-        QS_TRY(ensure(bb_part, (size_t)nparts * 8 * sizeof(float)));
-        QS_TRY(ensure(bb_out, 8 * sizeof(float)));
-        qs_launch_bbox(d_pos, d_rad, (int)n, as<float>(bb_part), nparts, as<float>(bb_out), stream);
-        float h[8];
-        QS_HIP(hipMemcpyAsync(h, bb_out.p, sizeof(h), hipMemcpyDeviceToHost, stream));
-        QS_HIP(hipStreamSynchronize(stream));
-        if (h[7] != 0.0f) return set_error(3, "atom positions or radii contain NaN or Inf");
-        for (int c = 0; c < 3; c++) {
-            bb.vmin[c] = h[c];
-            bb.vmax[c] = h[3 + c];
+    const int nmol = (int)in.size();
+    mols.assign(nmol, QsMol{});
+    uint64_t cell = 0, word = 0, vox = 0;
+    n_atoms_total = 0;
+    for (int m = 0; m < nmol; m++) {
+        QsMol &M = mols[m];
+        M.atom_off = in[m].atom_off;
+        M.n_atoms = in[m].n;
+        n_atoms_total += in[m].n;
+        if (!(in[m].reso > 0.0f) || !std::isfinite(in[m].reso))
+            return set_error(3, "molecule %d: voxel size must be a positive finite number", m);
+        if (in[m].n > 1) {
+            for (int c = 0; c < 3; c++)
+                if (!(bb[m].vmin[c] > -100000.0f && bb[m].vmax[c] < 100000.0f && bb[m].vmin[c] <= bb[m].vmax[c]))
+                    return set_error(3, "molecule %d: atom coordinates must lie strictly within (-1e5, 1e5) Angstrom", m);
+            grid_from_bbox(bb[m], in[m].reso, M.G);
+            QsGrid &G = M.G;
+            if (G.g < 2 || G.gn < 1) return set_error(3, "molecule %d: degenerate grid (g=%d)", m, G.g);
+            if ((uint64_t)G.g * G.g * G.g >= (1ull << 32)) return set_error(4, "molecule %d: grid of %d^3 points", m, G.g);
+            int lim = g_opt.lim_mode ? (G.g - 2 * G.r - 1) : (G.g - 1);
+            G.lim = std::max(lim, 0);
+            M.W = (uint32_t)((G.g + 63) / 64);
+            M.rows_per_wave = qs_pass1_rows_per_wave(G);
+        } else {
+            /* N <= 1: empty mesh (CudaSurf.cu:539-543); its atoms sit in one cell and no grid is run */
+            M.G = QsGrid{0.f, 0.f, 0.f, in[m].reso, 1.0f, 0, 1, 1, 0};
+            M.W = 0;
+            M.rows_per_wave = 1;
         }
-        bb.max_rad = h[6];
+        const uint64_t gg = (uint64_t)M.G.g;
+        M.cell_off = (uint32_t)cell;
+        M.word_off = word;
+        M.vox_off = vox;
+        cell += (uint64_t)M.G.gn * M.G.gn * M.G.gn;
+        word += gg * gg * M.W;
+        vox += gg * gg * gg;
     }
-    tm.mark("bbox");
-    for (int c = 0; c < 3; c++)
-        if (!(bb.vmin[c] > -100000.0f && bb.vmax[c] < 100000.0f && bb.vmin[c] <= bb.vmax[c]))
-            return set_error(3, "atom coordinates must lie strictly within (-1e5, 1e5) Angstrom");
-    grid_from_bbox(bb, reso, G);
-    const int g = G.g, gn = G.gn;
-    if (g < 2 || gn < 1) return set_error(3, "degenerate grid (g=%d)", g);
-    const uint64_t V = (uint64_t)g * g * g;
-    const uint64_t ncell = (uint64_t)gn * gn * gn;
-    if (V >= (1ull << 32)) return set_error(4, "grid of %d^3 points exceeds 2^32 points", g);
-    if (ncell >= (1ull << 31)) return set_error(4, "neighbour grid too large");
-    int lim = g_opt.lim_mode ? (g - 2 * G.r - 1) : (g - 1);
-    G.lim = std::max(lim, 0);
-
-    /* ---- atoms into cells ---- */
-    QS_TRY(ensure(key, (size_t)n * 4));
-    QS_TRY(ensure(count, ncell * 4));
-    QS_TRY(ensure(start, (ncell + 1) * 4));
-    QS_TRY(ensure(cursor, ncell * 4));
-    QS_TRY(ensure(perm, (size_t)n * 4));
-    QS_TRY(ensure(cse, ncell * 8));
-    QS_TRY(ensure(sx, (size_t)n * 16));
-    QS_HIP(hipMemsetAsync(count.p, 0, ncell * 4, stream));
-    QS_HIP(hipMemsetAsync(cursor.p, 0, ncell * 4, stream));
-    qs_launch_hash_count(d_pos, (int)n, G, as<int>(key), as<uint32_t>(count), stream);
-    /* start[c] = atoms in cells < c; start[ncell] = n */
-    QS_TRY(scan(as<uint32_t>(count), as<uint32_t>(start), ncell, as<uint32_t>(start) + ncell));
-    qs_launch_scatter(as<int>(key), (int)n, as<uint32_t>(start), as<uint32_t>(cursor), as<int>(perm), stream);
-    qs_launch_cell_finalize(as<uint32_t>(start), as<uint32_t>(count), (int)ncell, as<int>(perm), as<int2>(cse), stream);
-    qs_launch_gather_atoms(as<int>(perm), d_pos, d_rad, (int)n, as<float4>(sx), stream);
-    QS_HIP(hipGetLastError());
-    tm.mark("cells");
-
-    /* ---- field ---- */
-    const int W = (g + 63) / 64;
-    const size_t nwords = (size_t)g * g * W;
-    QS_TRY(ensure(outm, nwords * 8));
-    QS_TRY(ensure(shellm, nwords * 8));
-    QS_TRY(ensure(field, V * 4));
-    if (g_opt.pass1_direct)
-        qs_launch_pass1_direct(G, as<int2>(cse), as<float4>(sx), as<uint64_t>(outm), as<uint64_t>(shellm), W, 0, g, stream);
-    else
-        qs_launch_pass1(G, as<uint32_t>(start), as<float4>(sx), as<uint64_t>(outm), as<uint64_t>(shellm), W, 0, g, stream);
-    QS_HIP(hipGetLastError());
-    tm.mark("pass1");
-    QS_TRY(ensure(negm, nwords * 8));
-    QS_TRY(ensure(zerom, nwords * 8));
-    QS_TRY(pass2_tables());
-    qs_launch_pass2(G, as<uint64_t>(outm), as<uint64_t>(shellm), W, as<float>(field), as<uint32_t>(negm), as<uint32_t>(zerom),
-                    as<int>(t_offs), as<int>(t_norm), as<int>(t_end), 0, g, g_opt.pass2_direct != 0, stream);
-    QS_HIP(hipGetLastError());
-    tm.mark("pass2");
-
-    QS_TRY(mesh_from_field(tm, true));
-
-    /* ---- Laplacian passes (Jacobi) ---- */
-    res_vert = as<float>(vert);
-    if (smooth > 0 && nv > 0 && nt_all > 0) {
-        const int W = (g + 63) / 64;
-        QS_TRY(ensure(vert2, (size_t)nv * 12));
-        /* `vert` keeps the unsmoothed positions; passes alternate between vert2 and vert3 */
-        if (smooth > 1) QS_TRY(ensure(vert3, (size_t)nv * 12));
-        const float *src = as<float>(vert);
-        float *dst = as<float>(vert2);
-        for (int it = 0; it < smooth; it++) {
-            qs_launch_smooth(G, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw), W,
-                             as<uint64_t>(vkey), nv, as<uint8_t>(cidx), as<uint32_t>(tbase), as<int>(tri), src, dst, stream);
-            src = dst;
-            dst = (dst == as<float>(vert2)) ? as<float>(vert3) : as<float>(vert2);
-        }
-        QS_HIP(hipGetLastError());
-        res_vert = (float *)src;
-        tm.mark("smooth");
-    }
-
-    /* ---- degenerate filter ---- */
-    QS_TRY(ensure(keep, (size_t)std::max<uint32_t>(nt_all, 1) * 4));
-    QS_TRY(ensure(keeppos, (size_t)std::max<uint32_t>(nt_all, 1) * 4));
-    QS_TRY(ensure(tri_out, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
-    QS_TRY(ensure(totals, 16));
-    qs_launch_tri_keep(as<int>(tri), nt_all, as<uint32_t>(keep), stream);
-    QS_TRY(scan(as<uint32_t>(keep), as<uint32_t>(keeppos), nt_all, as<uint32_t>(totals)));
-    qs_launch_tri_compact(as<int>(tri), nt_all, as<uint32_t>(keeppos), as<int>(tri_out), stream);
-    uint32_t kept = 0;
-    QS_HIP(hipMemcpyAsync(&kept, totals.p, 4, hipMemcpyDeviceToHost, stream));
-    QS_HIP(hipStreamSynchronize(stream));
-    tm.mark("filter");
-    nt = kept;
-    res_tri = as<int>(tri_out);
-    res_atom = as<int>(atom);
-    tm.report(G, n);
+    if (cell >= (1ull << 31)) return set_error(4, "neighbour grids too large for one batch");
+    if (word >= (1ull << 32)) return set_error(4, "grids too large for one batch");
+    ncell_total = cell;
+    nw_total = word;
+    vox_total = vox;
     return 0;
 }
 
-/* marching cubes on the sign bits of `field` (grid G), results into vert / tri / atom */
-int Engine::mesh_from_field(StageTimer &tm, bool with_atoms)
+/* block prefixes of the launches (one uint32 array, uploaded once) */
+int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &at_wpre,
+                       uint32_t &nb_atoms, uint32_t &nb_p1, uint32_t &nb_mc)
 {
-    const int g = G.g, W = (g + 63) / 64;
-    const uint64_t nw = (uint64_t)g * g * W;
+    const int nmol = (int)mols.size();
+    std::vector<uint32_t> &meta = h_meta;
+    meta.assign(1, 0u); /* index 0 is never a prefix start (mid_at == 0 means none) */
+    auto prefix = [&](auto blocks_of, uint32_t &total) {
+        size_t at = meta.size();
+        uint64_t acc = 0;
+        for (int m = 0; m < nmol; m++) {
+            meta.push_back((uint32_t)acc);
+            acc += blocks_of(mols[m]);
+        }
+        meta.push_back((uint32_t)acc);
+        total = (uint32_t)acc;
+        return at;
+    };
+    at_atoms = prefix([](const QsMol &M) { return (uint64_t)((M.n_atoms + 255) / 256); }, nb_atoms);
+    const bool p1d = g_opt.pass1_direct != 0;
+    at_p1 = prefix([&](const QsMol &M) { return (uint64_t)(M.G.g ? qs_pass1_blocks(M.G, (int)M.W, M.rows_per_wave, p1d) : 0); },
+                   nb_p1);
+    at_mc = prefix([](const QsMol &M) { return (uint64_t)(M.G.g ? qs_mc_count_blocks(M.G, (int)M.W) : 0); }, nb_mc);
+    /* word prefix (64-bit) stored as pairs */
+    if (meta.size() & 1) meta.push_back(0u);
+    at_wpre = meta.size();
+    for (int m = 0; m <= nmol; m++) {
+        uint64_t w = (m < nmol) ? mols[m].word_off : nw_total;
+        meta.push_back((uint32_t)w);
+        meta.push_back((uint32_t)(w >> 32));
+    }
+    /* field pass 2 groups: same kernel path, r, dx and coordinate range */
+    groups.resize(0);
+    std::vector<P2Group> gs;
+    for (int m = 0; m < nmol; m++) {
+        const QsGrid &G = mols[m].G;
+        if (!G.g) continue;
+        float cmax = 0.0f;
+        for (float o : {G.ox, G.oy, G.oz}) cmax = std::max(cmax, std::max(std::fabs(o), std::fabs(o + (float)G.g * G.dx)));
+        cmax = std::max(cmax, (float)G.g * G.dx);
+        float cmax2 = 1.0f;
+        while (cmax2 < cmax) cmax2 *= 2.0f;
+        const int path = qs_pass2_path(G.r, g_opt.pass2_direct != 0);
+        size_t k = 0;
+        for (; k < gs.size(); k++)
+            if (gs[k].path == path && gs[k].r == G.r && gs[k].dx == G.dx && gs[k].cmax == cmax2) break;
+        if (k == gs.size()) {
+            gs.emplace_back();
+            gs.back().path = path;
+            gs.back().r = G.r;
+            gs.back().dx = G.dx;
+            gs.back().cmax = cmax2;
+        }
+        gs[k].mols.push_back(m);
+    }
+    for (auto &grp : gs) {
+        grp.bpre_at = meta.size();
+        uint64_t acc = 0;
+        for (int m : grp.mols) {
+            meta.push_back((uint32_t)acc);
+            acc += qs_pass2_blocks(mols[m].G, grp.path);
+        }
+        meta.push_back((uint32_t)acc);
+        grp.nblocks = (uint32_t)acc;
+        grp.mid_at = meta.size();
+        for (int m : grp.mols) meta.push_back((uint32_t)m);
+    }
+    groups.swap(gs);
+    return 0;
+}
+
+/* candidate tables of field pass 2 for one group (built once per (r, dx, coordinate range)) */
+int Engine::group_tables(P2Group &grp)
+{
+    for (P2Tables *t : tables)
+        if (t->r == grp.r && t->dx == grp.dx && t->cmax == grp.cmax) {
+            grp.tab = t;
+            return 0;
+        }
+    OffsetTables T;
+    build_offset_tables(grp.r, grp.dx, grp.cmax, T);
+    P2Tables *t = new P2Tables{grp.r, grp.dx, grp.cmax, DevBuf{}, DevBuf{}, DevBuf{}};
+    tables.push_back(t);
+    QS_TRY(ensure(t->offs, T.offs.size() * 4));
+    QS_TRY(ensure(t->norm, T.norm_start.size() * 4));
+    QS_TRY(ensure(t->end, T.enum_end.size() * 4));
+    /* synchronous copies: T is freed on return */
+    QS_HIP(hipMemcpy(t->offs.p, T.offs.data(), T.offs.size() * 4, hipMemcpyHostToDevice));
+    QS_HIP(hipMemcpy(t->norm.p, T.norm_start.data(), T.norm_start.size() * 4, hipMemcpyHostToDevice));
+    QS_HIP(hipMemcpy(t->end.p, T.enum_end.data(), T.enum_end.size() * 4, hipMemcpyHostToDevice));
+    grp.tab = t;
+    return 0;
+}
+
+/* field passes 1 and 2 over every molecule of the batch (cells must be ready) */
+int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
+{
+    const int nmol = (int)mols.size();
+    QS_TRY(ensure(outm, std::max<uint64_t>(nw_total, 1) * 8));
+    QS_TRY(ensure(shellm, std::max<uint64_t>(nw_total, 1) * 8));
+    QS_TRY(ensure(field, std::max<uint64_t>(vox_total, 1) * 4));
+    QS_TRY(ensure(negm, std::max<uint64_t>(nw_total, 1) * 8));
+    QS_TRY(ensure(zerom, std::max<uint64_t>(nw_total, 1) * 8));
+    if (nb_p1) {
+        if (g_opt.pass1_direct)
+            qs_launch_pass1_direct(batch(at_p1, 0, nmol), nb_p1, as<int2>(cse), as<float4>(sx), as<uint64_t>(outm),
+                                   as<uint64_t>(shellm), stream);
+        else
+            qs_launch_pass1(batch(at_p1, 0, nmol), nb_p1, as<uint32_t>(start), as<float4>(sx), as<float4>(aux),
+                            as<uint64_t>(outm), as<uint64_t>(shellm), stream);
+        QS_HIP(hipGetLastError());
+    }
+    tm.mark("pass1");
+    for (auto &grp : groups) {
+        QS_TRY(group_tables(grp));
+        qs_launch_pass2(batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path, as<uint64_t>(outm),
+                        as<uint64_t>(shellm), as<float>(field), as<uint32_t>(negm), as<uint32_t>(zerom), as<int>(grp.tab->offs),
+                        as<int>(grp.tab->norm), as<int>(grp.tab->end), stream);
+        QS_HIP(hipGetLastError());
+    }
+    tm.mark("pass2");
+    return 0;
+}
+
+/* marching cubes over every molecule (sign masks must be ready) */
+int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer &tm, bool with_atoms)
+{
+    const int nmol = (int)mols.size();
+    const uint64_t nw = std::max<uint64_t>(nw_total, 1);
     QS_TRY(ensure(s4, nw * 32));
     QS_TRY(ensure(vcnt, nw * 4));
     QS_TRY(ensure(tcnt, nw * 4));
@@ -464,61 +547,199 @@ int Engine::mesh_from_field(StageTimer &tm, bool with_atoms)
     QS_TRY(ensure(apos, nw * 4));
     QS_TRY(ensure(alist, nw * 4));
     QS_TRY(ensure(totals, 16));
-    qs_launch_mc_count(G, as<uint64_t>(negm), as<uint64_t>(zerom), W, as<uint64_t>(s4), as<uint32_t>(vcnt), as<uint32_t>(tcnt),
-                       0, g, stream);
-    QS_TRY(scan(as<uint32_t>(tcnt), as<uint32_t>(tbw), nw, as<uint32_t>(totals)));
-    QS_TRY(scan(as<uint32_t>(vcnt), as<uint32_t>(vbw), nw, as<uint32_t>(totals) + 1));
-    qs_launch_mc_flag(as<uint32_t>(vcnt), as<uint32_t>(tcnt), nw, as<uint32_t>(apos), stream);
-    QS_TRY(scan(as<uint32_t>(apos), as<uint32_t>(apos), nw, as<uint32_t>(totals) + 2));
-    qs_launch_mc_active(as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint32_t>(apos), nw, as<uint32_t>(alist), stream);
-    uint32_t tot[3];
-    QS_HIP(hipMemcpyAsync(tot, totals.p, 12, hipMemcpyDeviceToHost, stream));
+    if (nb_mc)
+        qs_launch_mc_count(batch(at_mc, 0, nmol), nb_mc, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
+                           as<uint32_t>(vcnt), as<uint32_t>(tcnt), stream);
+    QS_TRY(scan(as<uint32_t>(tcnt), as<uint32_t>(tbw), nw_total, as<uint32_t>(totals)));
+    QS_TRY(scan(as<uint32_t>(vcnt), as<uint32_t>(vbw), nw_total, as<uint32_t>(totals) + 1));
+    if (nw_total) qs_launch_mc_flag(as<uint32_t>(vcnt), as<uint32_t>(tcnt), nw_total, as<uint32_t>(apos), stream);
+    QS_TRY(scan(as<uint32_t>(apos), as<uint32_t>(apos), nw_total, as<uint32_t>(totals) + 2));
+    if (nw_total) qs_launch_mc_active(as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint32_t>(apos), nw_total, as<uint32_t>(alist),
+                                      stream);
+    /* per-molecule first vertex and triangle: the scans at each molecule's first word */
+    std::vector<uint64_t> idx(nmol);
+    for (int m = 0; m < nmol; m++) idx[m] = std::min<uint64_t>(mols[m].word_off, nw_total ? nw_total - 1 : 0);
+    QS_TRY(ensure(bidx, ((size_t)nmol + 1) * 8));
+    QS_TRY(ensure(bounds, ((size_t)nmol + 1) * 8));
+    std::vector<uint32_t> hb(2 * (size_t)nmol + 3);
+    if (nmol && nw_total) {
+        QS_HIP(hipMemcpyAsync(bidx.p, idx.data(), (size_t)nmol * 8, hipMemcpyHostToDevice, stream));
+        qs_launch_gather_u32(as<uint32_t>(vbw), as<uint64_t>(bidx), nmol, as<uint32_t>(bounds), stream);
+        qs_launch_gather_u32(as<uint32_t>(tbw), as<uint64_t>(bidx), nmol, as<uint32_t>(bounds) + nmol, stream);
+        QS_HIP(hipMemcpyAsync(hb.data(), bounds.p, (size_t)nmol * 8, hipMemcpyDeviceToHost, stream));
+    }
+    QS_HIP(hipMemcpyAsync(hb.data() + 2 * (size_t)nmol, totals.p, 12, hipMemcpyDeviceToHost, stream));
     QS_HIP(hipStreamSynchronize(stream));
     tm.mark("mc_count");
-    nt_all = tot[0];
-    nv = tot[1];
-    const uint32_t nact = tot[2];
+    nt_all = hb[2 * (size_t)nmol];
+    nv = hb[2 * (size_t)nmol + 1];
+    const uint32_t nact = hb[2 * (size_t)nmol + 2];
+    n_active = nact;
     if ((uint64_t)nt_all * 3 >= (1ull << 31) || nv >= (1u << 31)) return set_error(4, "mesh too large");
+    std::vector<uint32_t> &vpre = h_vpre;
+    vpre.assign(nmol + 1, 0u);
+    std::vector<uint32_t> tpre(nmol + 1);
+    for (int m = 0; m < nmol; m++) {
+        const bool last_empty = mols[m].word_off >= nw_total;
+        vpre[m] = last_empty ? nv : hb[m];
+        tpre[m] = last_empty ? nt_all : hb[nmol + m];
+    }
+    vpre[nmol] = nv;
+    tpre[nmol] = nt_all;
+    for (int m = 0; m < nmol; m++) {
+        mols[m].vert_off = vpre[m];
+        mols[m].n_vert = vpre[m + 1] - vpre[m];
+        mols[m].tri_off = tpre[m];
+        mols[m].n_tri = tpre[m + 1] - tpre[m];
+    }
+    vert_off = vpre;
+    QS_HIP(hipMemcpyAsync(d_mols.p, mols.data(), mols.size() * sizeof(QsMol), hipMemcpyHostToDevice, stream));
+    /* vertex prefix for per-vertex kernels lives after the molecule table's prefixes */
+    QS_TRY(ensure(d_off, ((size_t)nmol + 1) * 4 * 2 + 64));
+    uint32_t *d_vpre = as<uint32_t>(d_off) + (nmol + 1);
+    QS_HIP(hipMemcpyAsync(d_vpre, vpre.data(), (size_t)(nmol + 1) * 4, hipMemcpyHostToDevice, stream));
     QS_TRY(ensure(vert, (size_t)std::max<uint32_t>(nv, 1) * 12));
     QS_TRY(ensure(vkey, (size_t)std::max<uint32_t>(nv, 1) * 8));
     QS_TRY(ensure(tri, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
-    const uint64_t V = (uint64_t)g * g * g;
-    QS_TRY(ensure(cidx, V));
-    QS_TRY(ensure(tbase, V * 4));
-    n_active = nact;
-    qs_launch_mc_emit(G, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw), W, as<float>(field),
-                      as<uint32_t>(alist), nact, as<uint32_t>(tbw), as<float>(vert), as<uint64_t>(vkey), as<int>(tri),
-                      as<uint8_t>(cidx), as<uint32_t>(tbase), stream);
+    QS_TRY(ensure(cidx, std::max<uint64_t>(vox_total, 1)));
+    QS_TRY(ensure(tbase, std::max<uint64_t>(vox_total, 1) * 4));
+    const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
+    qs_launch_mc_emit(as<QsMol>(d_mols), wpre, nmol, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
+                      as<uint32_t>(vbw), as<float>(field), as<uint32_t>(alist), nact, as<uint32_t>(tbw), as<float>(vert),
+                      as<uint64_t>(vkey), as<int>(tri), as<uint8_t>(cidx), as<uint32_t>(tbase), stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
     if (with_atoms) {
         QS_TRY(ensure(atom, (size_t)std::max<uint32_t>(nv, 1) * 4));
-        qs_launch_closest_atom(G, as<int2>(cse), as<float4>(sx), as<int>(perm), as<float>(vert), nv, g_opt.atom_ids_sorted,
-                               as<int>(atom), stream);
+        qs_launch_closest_atom(as<QsMol>(d_mols), d_vpre, nmol, as<int2>(cse), as<float4>(sx), as<int>(perm), as<float>(vert),
+                               nv, g_opt.atom_ids_sorted, as<int>(atom), stream);
         QS_HIP(hipGetLastError());
         tm.mark("closest_atom");
     }
     return 0;
 }
 
-/* candidate tables of field pass 2 for the current grid (built when r, dx or the coordinate
-   range change) */
-int Engine::pass2_tables()
+int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<MolIn> &in, int smooth,
+                      const std::vector<BBox> *host_bbox)
 {
-    float cmax = 0.0f;
-    for (float o : {G.ox, G.oy, G.oz}) cmax = std::max(cmax, std::max(std::fabs(o), std::fabs(o + (float)G.g * G.dx)));
-    cmax = std::max(cmax, (float)G.g * G.dx);
-    float cmax2 = 1.0f;
-    while (cmax2 < cmax) cmax2 *= 2.0f;
-    if (tables.r == G.r && tables.dx == G.dx && tables.cmax == cmax2) return 0;
-    build_offset_tables(G.r, G.dx, cmax2, tables);
-    QS_TRY(ensure(t_offs, tables.offs.size() * 4));
-    QS_TRY(ensure(t_norm, tables.norm_start.size() * 4));
-    QS_TRY(ensure(t_end, tables.enum_end.size() * 4));
-    QS_HIP(hipMemcpyAsync(t_offs.p, tables.offs.data(), tables.offs.size() * 4, hipMemcpyHostToDevice, stream));
-    QS_HIP(hipMemcpyAsync(t_norm.p, tables.norm_start.data(), tables.norm_start.size() * 4, hipMemcpyHostToDevice, stream));
-    QS_HIP(hipMemcpyAsync(t_end.p, tables.enum_end.data(), tables.enum_end.size() * 4, hipMemcpyHostToDevice, stream));
+    const int nmol = (int)in.size();
+    nv = nt = nt_all = n_active = 0;
+    res_vert = nullptr;
+    res_tri = nullptr;
+    res_atom = nullptr;
+    vert_off.assign(nmol + 1, 0);
+    tri_off.assign(nmol + 1, 0);
+    mols.clear();
+    if (nmol == 0) return 0;
+    StageTimer tm;
+    tm.on = g_opt.trace != 0 || g_opt.timing != 0;
+    tm.s = stream;
+    tm.mark("start");
+
+    /* ---- bounding boxes (getMinMax CudaSurf.cu:149-174) ---- */
+    std::vector<BBox> bb(nmol);
+    if (host_bbox) {
+        bb = *host_bbox;
+    } else {
+        std::vector<uint32_t> off(nmol + 1);
+        for (int m = 0; m < nmol; m++) off[m] = in[m].atom_off;
+        off[nmol] = in[nmol - 1].atom_off + in[nmol - 1].n;
+        QS_TRY(ensure(d_off, ((size_t)nmol + 1) * 4 * 2 + 64));
+        QS_TRY(ensure(bb_out, (size_t)nmol * 8 * sizeof(float)));
+        QS_TRY(ensure(bb_part, qs_bbox_part_floats(nmol) * sizeof(float)));
+        QS_HIP(hipMemcpyAsync(d_off.p, off.data(), (nmol + 1) * 4, hipMemcpyHostToDevice, stream));
+        qs_launch_bbox(d_pos, d_rad, as<uint32_t>(d_off), nmol, as<float>(bb_part), as<float>(bb_out), stream);
+        std::vector<float> h((size_t)nmol * 8);
+        QS_HIP(hipMemcpyAsync(h.data(), bb_out.p, h.size() * 4, hipMemcpyDeviceToHost, stream));
+        QS_HIP(hipStreamSynchronize(stream));
+        for (int m = 0; m < nmol; m++) {
+            if (in[m].n > 1 && h[8 * m + 7] != 0.0f)
+                return set_error(3, "molecule %d: atom positions or radii contain NaN or Inf", m);
+            for (int c = 0; c < 3; c++) {
+                bb[m].vmin[c] = h[8 * m + c];
+                bb[m].vmax[c] = h[8 * m + 3 + c];
+            }
+            bb[m].max_rad = h[8 * m + 6];
+        }
+    }
+    tm.mark("bbox");
+    QS_TRY(setup(in, bb));
+    size_t at_atoms, at_p1, at_mc, at_wpre;
+    uint32_t nb_atoms, nb_p1, nb_mc;
+    QS_TRY(build_meta(at_atoms, at_p1, at_mc, at_wpre, nb_atoms, nb_p1, nb_mc));
+    QS_TRY(ensure(d_mols, mols.size() * sizeof(QsMol)));
+    QS_TRY(ensure(d_meta, h_meta.size() * 4));
+    QS_HIP(hipMemcpyAsync(d_mols.p, mols.data(), mols.size() * sizeof(QsMol), hipMemcpyHostToDevice, stream));
+    QS_HIP(hipMemcpyAsync(d_meta.p, h_meta.data(), h_meta.size() * 4, hipMemcpyHostToDevice, stream));
+
+    /* ---- atoms into cells (CudaSurf.cu:594-622), global over the batch ---- */
+    const uint32_t n = n_atoms_total;
+    const uint64_t ncell = ncell_total;
+    QS_TRY(ensure(key, (size_t)std::max<uint32_t>(n, 1) * 4));
+    QS_TRY(ensure(count, ncell * 4));
+    QS_TRY(ensure(start, (ncell + 1) * 4));
+    QS_TRY(ensure(cursor, ncell * 4));
+    QS_TRY(ensure(perm, (size_t)std::max<uint32_t>(n, 1) * 4));
+    QS_TRY(ensure(cse, ncell * 8));
+    QS_TRY(ensure(sx, (size_t)std::max<uint32_t>(n, 1) * 16));
+    QS_TRY(ensure(aux, (size_t)std::max<uint32_t>(n, 1) * 16));
+    QS_HIP(hipMemsetAsync(count.p, 0, ncell * 4, stream));
+    QS_HIP(hipMemsetAsync(cursor.p, 0, ncell * 4, stream));
+    if (nb_atoms) qs_launch_hash_count(batch(at_atoms, 0, nmol), nb_atoms, d_pos, as<int>(key), as<uint32_t>(count), stream);
+    /* start[c] = atoms in cells < c (global); start[ncell] = n */
+    QS_TRY(scan(as<uint32_t>(count), as<uint32_t>(start), ncell, as<uint32_t>(start) + ncell));
+    if (n) {
+        qs_launch_scatter(as<int>(key), (int)n, as<uint32_t>(start), as<uint32_t>(cursor), as<int>(perm), stream);
+        qs_launch_cell_finalize(as<uint32_t>(start), as<uint32_t>(count), (int)ncell, as<int>(perm), as<int2>(cse), stream);
+        qs_launch_gather_atoms(as<int>(perm), d_pos, d_rad, (int)n, as<float4>(sx), stream);
+        qs_launch_atom_aux(batch(at_atoms, 0, nmol), nb_atoms, as<float4>(sx), as<float4>(aux), stream);
+    }
+    QS_HIP(hipGetLastError());
+    tm.mark("cells");
+
+    /* ---- field, mesh ---- */
+    QS_TRY(field_stage(nb_p1, at_p1, tm));
+    QS_TRY(mesh_stage(at_mc, nb_mc, at_wpre, tm, true));
+
+    /* ---- Laplacian passes (Jacobi) ---- */
+    res_vert = as<float>(vert);
+    const uint32_t *d_vpre = as<uint32_t>(d_off) + (nmol + 1);
+    if (smooth > 0 && nv > 0 && nt_all > 0) {
+        QS_TRY(ensure(vert2, (size_t)nv * 12));
+        /* `vert` keeps the unsmoothed positions; passes alternate between vert2 and vert3 */
+        if (smooth > 1) QS_TRY(ensure(vert3, (size_t)nv * 12));
+        const float *src = as<float>(vert);
+        float *dst = as<float>(vert2);
+        for (int it = 0; it < smooth; it++) {
+            qs_launch_smooth(as<QsMol>(d_mols), d_vpre, nmol, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
+                             as<uint32_t>(vbw), as<uint64_t>(vkey), nv, as<uint8_t>(cidx), as<uint32_t>(tbase), as<int>(tri),
+                             src, dst, stream);
+            src = dst;
+            dst = (dst == as<float>(vert2)) ? as<float>(vert3) : as<float>(vert2);
+        }
+        QS_HIP(hipGetLastError());
+        res_vert = (float *)src;
+        tm.mark("smooth");
+    }
+
+    /* ---- degenerate filter (CudaSurf.cu:820-829), per-molecule kept counts ---- */
+    QS_TRY(ensure(keep, (size_t)std::max<uint32_t>(nt_all, 1) * 4));
+    QS_TRY(ensure(keeppos, ((size_t)nt_all + 1) * 4));
+    QS_TRY(ensure(tri_out, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
+    qs_launch_tri_keep(as<int>(tri), nt_all, as<uint32_t>(keep), stream);
+    QS_TRY(scan(as<uint32_t>(keep), as<uint32_t>(keeppos), nt_all, as<uint32_t>(keeppos) + nt_all));
+    qs_launch_tri_compact(as<int>(tri), nt_all, as<uint32_t>(keeppos), as<int>(tri_out), stream);
+    std::vector<uint64_t> idx(nmol + 1);
+    for (int m = 0; m <= nmol; m++) idx[m] = (m < nmol) ? mols[m].tri_off : nt_all;
+    QS_HIP(hipMemcpyAsync(bidx.p, idx.data(), (size_t)(nmol + 1) * 8, hipMemcpyHostToDevice, stream));
+    qs_launch_gather_u32(as<uint32_t>(keeppos), as<uint64_t>(bidx), nmol + 1, as<uint32_t>(bounds), stream);
+    QS_HIP(hipMemcpyAsync(tri_off.data(), bounds.p, (size_t)(nmol + 1) * 4, hipMemcpyDeviceToHost, stream));
     QS_HIP(hipStreamSynchronize(stream));
+    tm.mark("filter");
+    nt = tri_off[nmol];
+    res_tri = as<int>(tri_out);
+    res_atom = as<int>(atom);
+    tm.report(mols[0].G, n);
     return 0;
 }
 
@@ -527,6 +748,56 @@ float *g_vertices = nullptr;
 int *g_triangles = nullptr;
 int *g_atom_ids = nullptr;
 unsigned g_ntri_idx = 0, g_nvert = 0;
+
+/* one-molecule batch with a given grid (stage-wise entry points) */
+int single_grid(Engine &E, const QsGrid &G, size_t &at_mc, uint32_t &nb_mc, size_t &at_wpre)
+{
+    E.mols.assign(1, QsMol{});
+    QsMol &M = E.mols[0];
+    M.G = G;
+    M.W = (uint32_t)((G.g + 63) / 64);
+    M.rows_per_wave = qs_pass1_rows_per_wave(G);
+    E.n_atoms_total = 0;
+    E.ncell_total = (uint64_t)G.gn * G.gn * G.gn;
+    E.nw_total = (uint64_t)G.g * G.g * M.W;
+    E.vox_total = (uint64_t)G.g * G.g * G.g;
+    E.vert_off.assign(2, 0);
+    E.tri_off.assign(2, 0);
+    size_t at_atoms, at_p1;
+    uint32_t nb_atoms, nb_p1;
+    QS_TRY(E.build_meta(at_atoms, at_p1, at_mc, at_wpre, nb_atoms, nb_p1, nb_mc));
+    QS_TRY(ensure(E.d_mols, sizeof(QsMol)));
+    QS_TRY(ensure(E.d_meta, E.h_meta.size() * 4));
+    QS_HIP(hipMemcpyAsync(E.d_mols.p, E.mols.data(), sizeof(QsMol), hipMemcpyHostToDevice, E.stream));
+    QS_HIP(hipMemcpyAsync(E.d_meta.p, E.h_meta.data(), E.h_meta.size() * 4, hipMemcpyHostToDevice, E.stream));
+    return 0;
+}
+
+int check_batch_args(const unsigned *atom_off, int nmol, const float *reso, std::vector<MolIn> &in)
+{
+    if (nmol < 0 || (nmol > 0 && (!atom_off || !reso))) return set_error(3, "bad batch description");
+    in.resize(nmol);
+    for (int m = 0; m < nmol; m++) {
+        if (atom_off[m + 1] < atom_off[m]) return set_error(3, "atom offsets must be non-decreasing");
+        in[m].atom_off = atom_off[m];
+        in[m].n = atom_off[m + 1] - atom_off[m];
+        in[m].reso = reso[m];
+    }
+    if (nmol && atom_off[0] != 0) return set_error(3, "atom offsets must start at 0");
+    if (nmol && atom_off[nmol] > (1u << 30)) return set_error(3, "too many atoms (%u)", atom_off[nmol]);
+    return 0;
+}
+
+void clear_host_result()
+{
+    free(g_vertices);
+    free(g_triangles);
+    free(g_atom_ids);
+    g_vertices = nullptr;
+    g_triangles = nullptr;
+    g_atom_ids = nullptr;
+    g_nvert = g_ntri_idx = 0;
+}
 
 }  // namespace
 
@@ -541,8 +812,7 @@ const char *qs_last_error_message(void) { return g_errmsg; }
 int qs_set_option(const char *key, int value)
 {
     std::string k = key ? key : "";
-    if (k == "slice") g_opt.slice = value;
-    else if (k == "lim_mode") g_opt.lim_mode = value;
+    if (k == "lim_mode") g_opt.lim_mode = value;
     else if (k == "atom_ids_sorted") g_opt.atom_ids_sorted = value;
     else if (k == "pass1_direct") g_opt.pass1_direct = value;
     else if (k == "pass2_direct") g_opt.pass2_direct = value;
@@ -555,7 +825,6 @@ int qs_set_option(const char *key, int value)
 int qs_get_option(const char *key)
 {
     std::string k = key ? key : "";
-    if (k == "slice") return g_opt.slice;
     if (k == "lim_mode") return g_opt.lim_mode;
     if (k == "atom_ids_sorted") return g_opt.atom_ids_sorted;
     if (k == "pass1_direct") return g_opt.pass1_direct;
@@ -574,15 +843,10 @@ void API_computeSES(float resoSES, qs_float3 *atomPos, float *atomRad, unsigned 
     g_errmsg[0] = 0;
     if (NVert) *NVert = 0;
     if (NTri) *NTri = 0;
-    g_nvert = g_ntri_idx = 0;
     /* results of a previous call that were not freed are released (the reference leaks them) */
-    free(g_vertices);
-    free(g_triangles);
-    free(g_atom_ids);
-    g_vertices = nullptr;
-    g_triangles = nullptr;
-    g_atom_ids = nullptr;
+    clear_host_result();
     auto fail = [&]() {
+        clear_host_result();
         g_vertices = (float *)malloc(12);
         g_triangles = (int *)malloc(12);
         g_atom_ids = (int *)malloc(4);
@@ -596,60 +860,53 @@ void API_computeSES(float resoSES, qs_float3 *atomPos, float *atomRad, unsigned 
         fail();
         return;
     }
-    if (g_eng.init()) {
+    if (N > (1u << 30)) {
+        set_error(3, "too many atoms (%u)", N);
         fail();
         return;
     }
-    BBox bb{};
-    int nonfinite = 0;
     const float *hp = (const float *)atomPos;
-    if (N > 1) {
-        bbox_host(hp, atomRad, N, bb, nonfinite);
-        if (nonfinite) {
-            set_error(3, "atom positions or radii contain NaN or Inf");
-            fail();
-            return;
-        }
-    }
-    int rc = 0;
-    if (N > 1) {
-        Engine &E = g_eng;
-        rc = ensure(E.in_pos, (size_t)N * 12);
-        if (!rc) rc = ensure(E.in_rad, (size_t)N * 4);
-        if (!rc && hipMemcpyAsync(E.in_pos.p, hp, (size_t)N * 12, hipMemcpyHostToDevice, E.stream) != hipSuccess)
-            rc = set_error(2, "H2D copy failed");
-        if (!rc && hipMemcpyAsync(E.in_rad.p, atomRad, (size_t)N * 4, hipMemcpyHostToDevice, E.stream) != hipSuccess)
-            rc = set_error(2, "H2D copy failed");
-        if (!rc) rc = E.run(as<float>(E.in_pos), as<float>(E.in_rad), N, resoSES, doSmoothing, &bb);
-    } else {
-        g_eng.nv = g_eng.nt = 0;
-    }
-    if (rc) {
+    std::vector<BBox> bb(1);
+    int nonfinite = 0;
+    bbox_host(hp, atomRad, N, bb[0], nonfinite);
+    if (nonfinite) {
+        set_error(3, "atom positions or radii contain NaN or Inf");
         fail();
         return;
     }
     Engine &E = g_eng;
+    int rc = E.init();
+    if (!rc) rc = ensure(E.in_pos, (size_t)N * 12);
+    if (!rc) rc = ensure(E.in_rad, (size_t)N * 4);
+    if (!rc && hipMemcpyAsync(E.in_pos.p, hp, (size_t)N * 12, hipMemcpyHostToDevice, E.stream) != hipSuccess)
+        rc = set_error(2, "H2D copy failed");
+    if (!rc && hipMemcpyAsync(E.in_rad.p, atomRad, (size_t)N * 4, hipMemcpyHostToDevice, E.stream) != hipSuccess)
+        rc = set_error(2, "H2D copy failed");
+    std::vector<MolIn> in(1);
+    in[0].atom_off = 0;
+    in[0].n = N;
+    in[0].reso = resoSES;
+    if (!rc) rc = E.run_batch(as<float>(E.in_pos), as<float>(E.in_rad), in, doSmoothing, &bb);
+    if (rc) {
+        fail();
+        return;
+    }
     g_nvert = E.nv;
     g_ntri_idx = 3 * E.nt;
     g_vertices = (float *)malloc((size_t)std::max<uint32_t>(E.nv, 1) * 12);
     g_triangles = (int *)malloc((size_t)std::max<uint32_t>(E.nt, 1) * 12);
     g_atom_ids = (int *)malloc((size_t)std::max<uint32_t>(E.nv, 1) * 4);
+    bool ok = true;
     if (E.nv) {
-        if (hipMemcpyAsync(g_vertices, E.res_vert, (size_t)E.nv * 12, hipMemcpyDeviceToHost, E.stream) != hipSuccess ||
-            hipMemcpyAsync(g_atom_ids, E.res_atom, (size_t)E.nv * 4, hipMemcpyDeviceToHost, E.stream) != hipSuccess) {
-            set_error(2, "D2H copy failed");
-            g_nvert = g_ntri_idx = 0;
-            return;
-        }
+        ok = hipMemcpyAsync(g_vertices, E.res_vert, (size_t)E.nv * 12, hipMemcpyDeviceToHost, E.stream) == hipSuccess &&
+             hipMemcpyAsync(g_atom_ids, E.res_atom, (size_t)E.nv * 4, hipMemcpyDeviceToHost, E.stream) == hipSuccess;
     }
-    if (E.nt && hipMemcpyAsync(g_triangles, E.res_tri, (size_t)E.nt * 12, hipMemcpyDeviceToHost, E.stream) != hipSuccess) {
+    if (ok && E.nt)
+        ok = hipMemcpyAsync(g_triangles, E.res_tri, (size_t)E.nt * 12, hipMemcpyDeviceToHost, E.stream) == hipSuccess;
+    if (ok) ok = hipStreamSynchronize(E.stream) == hipSuccess;
+    if (!ok) {
         set_error(2, "D2H copy failed");
-        g_nvert = g_ntri_idx = 0;
-        return;
-    }
-    if (hipStreamSynchronize(E.stream) != hipSuccess) {
-        set_error(2, "stream synchronisation failed");
-        g_nvert = g_ntri_idx = 0;
+        fail();
         return;
     }
     if (NVert) *NVert = g_nvert;
@@ -668,16 +925,7 @@ qs_float3 *API_getVertices(void) { return (qs_float3 *)g_vertices; }
 
 int *API_getAtomIdPerVert(void) { return g_atom_ids; }
 
-void API_freeMesh(void)
-{
-    free(g_vertices);
-    free(g_triangles);
-    free(g_atom_ids);
-    g_vertices = nullptr;
-    g_triangles = nullptr;
-    g_atom_ids = nullptr;
-    g_nvert = g_ntri_idx = 0;
-}
+void API_freeMesh(void) { clear_host_result(); }
 
 int qs_compute_device(const float *d_pos, const float *d_rad, unsigned int N, float resoSES, int doSmoothing,
                       unsigned int *nvert, unsigned int *nidx, float **d_vert, int **d_tri, int **d_atom_id)
@@ -686,13 +934,84 @@ int qs_compute_device(const float *d_pos, const float *d_rad, unsigned int N, fl
     g_errmsg[0] = 0;
     if (nvert) *nvert = 0;
     if (nidx) *nidx = 0;
+    if (N > (1u << 30)) return set_error(3, "too many atoms (%u)", N);
     QS_TRY(g_eng.init());
-    QS_TRY(g_eng.run(d_pos, d_rad, N, resoSES, doSmoothing, nullptr));
+    std::vector<MolIn> in(1);
+    in[0].atom_off = 0;
+    in[0].n = N;
+    in[0].reso = resoSES;
+    QS_TRY(g_eng.run_batch(d_pos, d_rad, in, doSmoothing, nullptr));
     if (nvert) *nvert = g_eng.nv;
     if (nidx) *nidx = 3 * g_eng.nt;
     if (d_vert) *d_vert = g_eng.res_vert;
     if (d_tri) *d_tri = g_eng.res_tri;
     if (d_atom_id) *d_atom_id = g_eng.res_atom;
+    return 0;
+}
+
+int qs_compute_batch_device(const float *d_pos, const float *d_rad, const unsigned int *atom_off, int nmol,
+                            const float *reso, int doSmoothing, unsigned int *vert_off, unsigned int *tri_off,
+                            float **d_vert, int **d_tri, int **d_atom_id)
+{
+    g_err = 0;
+    g_errmsg[0] = 0;
+    std::vector<MolIn> in;
+    QS_TRY(check_batch_args(atom_off, nmol, reso, in));
+    QS_TRY(g_eng.init());
+    QS_TRY(g_eng.run_batch(d_pos, d_rad, in, doSmoothing, nullptr));
+    for (int m = 0; m <= nmol; m++) {
+        if (vert_off) vert_off[m] = g_eng.vert_off[m];
+        if (tri_off) tri_off[m] = g_eng.tri_off[m];
+    }
+    if (d_vert) *d_vert = g_eng.res_vert;
+    if (d_tri) *d_tri = g_eng.res_tri;
+    if (d_atom_id) *d_atom_id = g_eng.res_atom;
+    return 0;
+}
+
+int qs_compute_batch(const float *pos, const float *rad, const unsigned int *atom_off, int nmol, const float *reso,
+                     int doSmoothing, unsigned int *nvert, unsigned int *ntri)
+{
+    g_err = 0;
+    g_errmsg[0] = 0;
+    std::vector<MolIn> in;
+    QS_TRY(check_batch_args(atom_off, nmol, reso, in));
+    const unsigned n = nmol ? atom_off[nmol] : 0;
+    if (n && (!pos || !rad)) return set_error(3, "null atom arrays");
+    std::vector<BBox> bb(nmol);
+    for (int m = 0; m < nmol; m++) {
+        int nonfinite = 0;
+        bbox_host(pos + 3 * (size_t)in[m].atom_off, rad + in[m].atom_off, in[m].n, bb[m], nonfinite);
+        if (in[m].n > 1 && nonfinite) return set_error(3, "molecule %d: atom positions or radii contain NaN or Inf", m);
+    }
+    Engine &E = g_eng;
+    QS_TRY(E.init());
+    QS_TRY(ensure(E.in_pos, (size_t)std::max(n, 1u) * 12));
+    QS_TRY(ensure(E.in_rad, (size_t)std::max(n, 1u) * 4));
+    if (n) {
+        QS_HIP(hipMemcpyAsync(E.in_pos.p, pos, (size_t)n * 12, hipMemcpyHostToDevice, E.stream));
+        QS_HIP(hipMemcpyAsync(E.in_rad.p, rad, (size_t)n * 4, hipMemcpyHostToDevice, E.stream));
+    }
+    QS_TRY(E.run_batch(as<float>(E.in_pos), as<float>(E.in_rad), in, doSmoothing, &bb));
+    for (int m = 0; m < nmol; m++) {
+        if (nvert) nvert[m] = E.vert_off[m + 1] - E.vert_off[m];
+        if (ntri) ntri[m] = E.tri_off[m + 1] - E.tri_off[m];
+    }
+    return 0;
+}
+
+int qs_batch_mesh(int m, float *vert, int *tri, int *atom_id)
+{
+    Engine &E = g_eng;
+    if (m < 0 || m + 1 >= (int)E.vert_off.size() || m + 1 >= (int)E.tri_off.size()) return set_error(3, "no molecule %d", m);
+    const uint32_t v0 = E.vert_off[m], nvm = E.vert_off[m + 1] - v0;
+    const uint32_t t0 = E.tri_off[m], ntm = E.tri_off[m + 1] - t0;
+    if (nvm && vert) QS_HIP(hipMemcpy(vert, E.res_vert + 3 * (size_t)v0, (size_t)nvm * 12, hipMemcpyDeviceToHost));
+    if (nvm && atom_id) QS_HIP(hipMemcpy(atom_id, E.res_atom + v0, (size_t)nvm * 4, hipMemcpyDeviceToHost));
+    if (ntm && tri) {
+        QS_HIP(hipMemcpy(tri, E.res_tri + 3 * (size_t)t0, (size_t)ntm * 12, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < 3 * (size_t)ntm; i++) tri[i] -= (int)v0;
+    }
     return 0;
 }
 
@@ -714,9 +1033,10 @@ int qs_stage_times(char *names, int names_cap, float *ms, int max_stages)
     return k;
 }
 
-int qs_last_grid(float *fp, int *ip)
+int qs_batch_grid(int m, float *fp, int *ip)
 {
-    const QsGrid &G = g_eng.G;
+    if (m < 0 || m >= (int)g_eng.mols.size()) return -1;
+    const QsGrid &G = g_eng.mols[m].G;
     if (fp) {
         fp[0] = G.ox;
         fp[1] = G.oy;
@@ -733,22 +1053,26 @@ int qs_last_grid(float *fp, int *ip)
     return 0;
 }
 
+int qs_last_grid(float *fp, int *ip) { return qs_batch_grid(0, fp, ip); }
+
 int64_t qs_fetch(const char *what, void *host, int64_t bytes)
 {
     Engine &E = g_eng;
     std::string w = what ? what : "";
-    const QsGrid &G = E.G;
-    uint64_t V = (uint64_t)G.g * G.g * G.g;
-    uint64_t nc = (uint64_t)G.gn * G.gn * G.gn;
+    if (E.mols.empty()) return -1;
+    const QsGrid &G = E.mols[0].G;
+    const uint64_t V = E.vox_total;
     const void *src = nullptr;
     int64_t need = 0;
     if (w == "labels") {
-        if (bytes < (int64_t)V || !E.outm.p) return -1;
+        /* molecule 0 of the last run, reconstructed from the outside / shell masks */
+        const uint64_t V0 = (uint64_t)G.g * G.g * G.g;
+        if (bytes < (int64_t)V0 || !E.outm.p) return -1;
         const int g = G.g, W = (g + 63) / 64;
         size_t nw = (size_t)g * g * W;
         std::vector<uint64_t> mo(nw), ms(nw);
-        if (hipMemcpy(mo.data(), E.outm.p, nw * 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
-        if (hipMemcpy(ms.data(), E.shellm.p, nw * 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        if (nw && hipMemcpy(mo.data(), E.outm.p, nw * 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        if (nw && hipMemcpy(ms.data(), E.shellm.p, nw * 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
         uint8_t *out = (uint8_t *)host;
         for (size_t row = 0; row < (size_t)g * g; row++)
             for (int z = 0; z < g; z++) {
@@ -756,11 +1080,11 @@ int64_t qs_fetch(const char *what, void *host, int64_t bytes)
                 size_t wi = row * W + (z >> 6);
                 out[row * g + z] = (mo[wi] & bit) ? QS_OUT : ((ms[wi] & bit) ? QS_SHELL : QS_IN);
             }
-        return (int64_t)V;
+        return (int64_t)V0;
     }
     else if (w == "field") { src = E.field.p; need = (int64_t)V * 4; }
-    else if (w == "perm") { src = E.perm.p; need = (int64_t)E.n_atoms * 4; }
-    else if (w == "cells") { src = E.cse.p; need = (int64_t)nc * 8; }
+    else if (w == "perm") { src = E.perm.p; need = (int64_t)E.n_atoms_total * 4; }
+    else if (w == "cells") { src = E.cse.p; need = (int64_t)E.ncell_total * 8; }
     else if (w == "vert_unsmoothed") { src = E.vert.p; need = (int64_t)E.nv * 12; }
     else if (w == "tri_all") { src = E.tri.p; need = (int64_t)E.nt_all * 12; }
     else return -1;
@@ -789,23 +1113,30 @@ int qs_stage_pass2(const float *fp, const int *ip, const uint8_t *labels, float 
     QS_TRY(E.init());
     QsGrid G;
     grid_from_params(fp, ip, G);
-    uint64_t V = (uint64_t)G.g * G.g * G.g;
-    const int W = (G.g + 63) / 64;
-    const size_t nwords = (size_t)G.g * G.g * W;
+    if (G.g < 2 || (uint64_t)G.g * G.g * G.g >= (1ull << 32)) return set_error(3, "bad grid");
+    size_t at_mc, at_wpre;
+    uint32_t nb_mc;
+    QS_TRY(single_grid(E, G, at_mc, nb_mc, at_wpre));
+    const uint64_t V = E.vox_total;
+    const int W = (int)E.mols[0].W;
     QS_TRY(ensure(E.labels, V));
-    QS_TRY(ensure(E.outm, nwords * 8));
-    QS_TRY(ensure(E.shellm, nwords * 8));
-    QS_TRY(ensure(E.field, V * 4));
-    QS_HIP(hipMemcpy(E.labels.p, labels, V, hipMemcpyHostToDevice));
+    QS_TRY(ensure(E.outm, E.nw_total * 8));
+    QS_TRY(ensure(E.shellm, E.nw_total * 8));
+    QS_HIP(hipMemcpyAsync(E.labels.p, labels, V, hipMemcpyHostToDevice, E.stream));
     qs_launch_labels_to_masks(as<uint8_t>(E.labels), G.g, W, as<uint64_t>(E.outm), as<uint64_t>(E.shellm), E.stream);
-    QS_TRY(ensure(E.negm, nwords * 8));
-    QS_TRY(ensure(E.zerom, nwords * 8));
-    E.G = G;
-    QS_TRY(E.pass2_tables());
-    qs_launch_pass2(G, as<uint64_t>(E.outm), as<uint64_t>(E.shellm), W, as<float>(E.field), as<uint32_t>(E.negm),
-                    as<uint32_t>(E.zerom), as<int>(E.t_offs), as<int>(E.t_norm), as<int>(E.t_end), 0, G.g,
-                    g_opt.pass2_direct != 0, E.stream);
     QS_HIP(hipGetLastError());
+    /* pass 1 is skipped: the labels stand in for it */
+    QS_TRY(ensure(E.field, V * 4));
+    QS_TRY(ensure(E.negm, E.nw_total * 8));
+    QS_TRY(ensure(E.zerom, E.nw_total * 8));
+    for (auto &grp : E.groups) {
+        QS_TRY(E.group_tables(grp));
+        qs_launch_pass2(E.batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path,
+                        as<uint64_t>(E.outm), as<uint64_t>(E.shellm), as<float>(E.field), as<uint32_t>(E.negm),
+                        as<uint32_t>(E.zerom), as<int>(grp.tab->offs), as<int>(grp.tab->norm), as<int>(grp.tab->end),
+                        E.stream);
+        QS_HIP(hipGetLastError());
+    }
     QS_HIP(hipStreamSynchronize(E.stream));
     QS_HIP(hipMemcpy(field, E.field.p, V * 4, hipMemcpyDeviceToHost));
     return 0;
@@ -817,17 +1148,21 @@ int qs_stage_mesh(const float *fp, const int *ip, const float *field, unsigned i
     g_err = 0;
     Engine &E = g_eng;
     QS_TRY(E.init());
-    grid_from_params(fp, ip, E.G);
-    uint64_t V = (uint64_t)E.G.g * E.G.g * E.G.g;
+    QsGrid G;
+    grid_from_params(fp, ip, G);
+    if (G.g < 2 || (uint64_t)G.g * G.g * G.g >= (1ull << 32)) return set_error(3, "bad grid");
+    size_t at_mc, at_wpre;
+    uint32_t nb_mc;
+    QS_TRY(single_grid(E, G, at_mc, nb_mc, at_wpre));
+    const uint64_t V = E.vox_total;
     QS_TRY(ensure(E.field, V * 4));
-    QS_HIP(hipMemcpy(E.field.p, field, V * 4, hipMemcpyHostToDevice));
-    const int W = (E.G.g + 63) / 64;
-    const size_t nwords = (size_t)E.G.g * E.G.g * W;
-    QS_TRY(ensure(E.negm, nwords * 8));
-    QS_TRY(ensure(E.zerom, nwords * 8));
-    qs_launch_field_signs(as<float>(E.field), E.G.g, W, as<uint32_t>(E.negm), as<uint32_t>(E.zerom), E.stream);
+    QS_HIP(hipMemcpyAsync(E.field.p, field, V * 4, hipMemcpyHostToDevice, E.stream));
+    const int W = (int)E.mols[0].W;
+    QS_TRY(ensure(E.negm, E.nw_total * 8));
+    QS_TRY(ensure(E.zerom, E.nw_total * 8));
+    qs_launch_field_signs(as<float>(E.field), G.g, W, as<uint32_t>(E.negm), as<uint32_t>(E.zerom), E.stream);
     StageTimer tm;
-    QS_TRY(E.mesh_from_field(tm, false));
+    QS_TRY(E.mesh_stage(at_mc, nb_mc, at_wpre, tm, false));
     QS_HIP(hipStreamSynchronize(E.stream));
     if (nvert) *nvert = E.nv;
     if (ntri) *ntri = E.nt_all;

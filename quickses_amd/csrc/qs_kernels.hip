@@ -59,6 +59,42 @@ __device__ __forceinline__ int tri_count(int ci)
     return k / 3;
 }
 
+/* molecule of block b of a batched launch, and the block's index inside the molecule */
+__device__ __forceinline__ int batch_mol(const QsBatch &B, uint32_t b, uint32_t &local)
+{
+    int lo = 0, hi = B.n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (B.bpre[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    local = b - B.bpre[lo];
+    return B.mid ? B.mid[lo] : lo;
+}
+
+/* index of the segment of a sorted prefix array (n + 1 entries) that holds item v */
+__device__ __forceinline__ int seg_of(const uint32_t *__restrict__ pre, int n, uint32_t v)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int seg_of64(const uint64_t *__restrict__ pre, int n, uint64_t v)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 /* exclusive scan over the block; returns this thread's prefix, *total the block sum */
 template <int NT>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total, uint32_t *lds)
@@ -147,21 +183,33 @@ __global__ void __launch_bounds__(SCAN_NT) k_tile_scan(const uint32_t *in, uint3
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* bounding box (getMinMax CudaSurf.cu:149-174): partial per block, then one block             */
+/* bounding box (getMinMax CudaSurf.cu:149-174): partial boxes per molecule, then one wave    */
 
 #define BB_NT 256
 #define BB_NV 8 /* min x,y,z, max x,y,z, max radius, count of non-finite inputs */
-__global__ void __launch_bounds__(BB_NT) k_bbox_partial(const float *__restrict__ pos, const float *__restrict__ rad,
-                                                        int n, float *__restrict__ part)
+#define BB_P 32 /* partial boxes per molecule */
+
+__device__ __forceinline__ void bb_combine(float *v, const float *w)
+{
+    for (int c = 0; c < 3; c++) v[c] = fminf(v[c], w[c]);
+    for (int c = 3; c < 7; c++) v[c] = fmaxf(v[c], w[c]);
+    v[7] += w[7];
+}
+
+/* block (p, m): atoms off[m] + p*BB_NT + k*BB_P*BB_NT + t of molecule m -> part[m][p] */
+__global__ void __launch_bounds__(BB_NT) k_bbox_part(const float *__restrict__ pos, const float *__restrict__ rad,
+                                                     const uint32_t *__restrict__ off, float *__restrict__ part)
 {
     __shared__ float s[BB_NV][BB_NT];
+    const int m = blockIdx.y, p = blockIdx.x;
     float v[BB_NV] = {100000.0f, 100000.0f, 100000.0f, -100000.0f, -100000.0f, -100000.0f, 0.0f, 0.0f};
-    for (int i = blockIdx.x * BB_NT + threadIdx.x; i < n; i += gridDim.x * BB_NT) {
+    const uint32_t end = off[m + 1];
+    for (uint32_t i = off[m] + p * BB_NT + threadIdx.x; i < end; i += BB_P * BB_NT) {
         for (int c = 0; c < 3; c++) {
-            float p = pos[3 * i + c];
-            v[c] = fminf(v[c], p);
-            v[3 + c] = fmaxf(v[3 + c], p);
-            if (!isfinite(p)) v[7] += 1.0f;
+            float q = pos[3 * (size_t)i + c];
+            v[c] = fminf(v[c], q);
+            v[3 + c] = fmaxf(v[3 + c], q);
+            if (!isfinite(q)) v[7] += 1.0f;
         }
         float ra = rad[i];
         v[6] = fmaxf(v[6], ra);
@@ -170,43 +218,52 @@ __global__ void __launch_bounds__(BB_NT) k_bbox_partial(const float *__restrict_
     for (int c = 0; c < BB_NV; c++) s[c][threadIdx.x] = v[c];
     __syncthreads();
     for (int st = BB_NT / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st)
+        if (threadIdx.x < st) {
+            float w[BB_NV];
             for (int c = 0; c < BB_NV; c++) {
-                float a = s[c][threadIdx.x], b = s[c][threadIdx.x + st];
-                s[c][threadIdx.x] = (c < 3) ? fminf(a, b) : (c < 7 ? fmaxf(a, b) : a + b);
+                v[c] = s[c][threadIdx.x];
+                w[c] = s[c][threadIdx.x + st];
             }
+            bb_combine(v, w);
+            for (int c = 0; c < BB_NV; c++) s[c][threadIdx.x] = v[c];
+        }
         __syncthreads();
     }
-    if (threadIdx.x < BB_NV) part[BB_NV * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+    if (threadIdx.x < BB_NV) part[((size_t)m * BB_P + p) * BB_NV + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void __launch_bounds__(BB_NT) k_bbox_final(const float *__restrict__ part, int nparts, float *__restrict__ out)
+/* one wavefront per molecule: combine its BB_P partial boxes */
+__global__ void __launch_bounds__(64) k_bbox_final(const float *__restrict__ part, float *__restrict__ out)
 {
-    if (threadIdx.x != 0) return;
-    float v[BB_NV] = {100000.0f, 100000.0f, 100000.0f, -100000.0f, -100000.0f, -100000.0f, 0.0f, 0.0f};
-    for (int b = 0; b < nparts; b++)
-        for (int c = 0; c < BB_NV; c++) {
-            float p = part[BB_NV * b + c];
-            v[c] = (c < 3) ? fminf(v[c], p) : (c < 7 ? fmaxf(v[c], p) : v[c] + p);
-        }
-    for (int c = 0; c < BB_NV; c++) out[c] = v[c];
+    const int m = blockIdx.x, lane = threadIdx.x;
+    if (lane >= BB_NV) return;
+    float acc = part[(size_t)m * BB_P * BB_NV + lane];
+    for (int p = 1; p < BB_P; p++) {
+        float w = part[((size_t)m * BB_P + p) * BB_NV + lane];
+        acc = (lane < 3) ? fminf(acc, w) : (lane < 7 ? fmaxf(acc, w) : acc + w);
+    }
+    out[BB_NV * m + lane] = acc;
 }
 
 /* ------------------------------------------------------------------------------------------ */
 /* atom hashing and cell ordering                                                              */
 
-/* hashAtoms Kernels.cu:185-216: cell id x-major; count atoms per cell */
-__global__ void __launch_bounds__(256) k_hash_count(const float *__restrict__ pos, int n, QsGrid G,
-                                                    int *__restrict__ key, uint32_t *__restrict__ count)
+/* hashAtoms Kernels.cu:185-216: cell id x-major (global over the batch); count atoms per cell */
+__global__ void __launch_bounds__(256) k_hash_count(QsBatch B, const float *__restrict__ pos, int *__restrict__ key,
+                                                    uint32_t *__restrict__ count)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    uint32_t lb;
+    const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
+    const uint32_t i = lb * 256 + threadIdx.x;
+    if (i >= M.n_atoms) return;
+    const QsGrid G = M.G;
+    const size_t ga = (size_t)M.atom_off + i;
     /* in range for every atom by construction of the grid; clamped only as a memory guard */
-    int cx = iclamp(cell_of(pos[3 * i + 0], G.ox, G.dxn), 0, G.gn - 1);
-    int cy = iclamp(cell_of(pos[3 * i + 1], G.oy, G.dxn), 0, G.gn - 1);
-    int cz = iclamp(cell_of(pos[3 * i + 2], G.oz, G.dxn), 0, G.gn - 1);
-    int h = (G.gn * G.gn * cx) + (G.gn * cy) + cz;
-    key[i] = h;
+    int cx = iclamp(cell_of(pos[3 * ga + 0], G.ox, G.dxn), 0, G.gn - 1);
+    int cy = iclamp(cell_of(pos[3 * ga + 1], G.oy, G.dxn), 0, G.gn - 1);
+    int cz = iclamp(cell_of(pos[3 * ga + 2], G.oz, G.dxn), 0, G.gn - 1);
+    int h = (int)M.cell_off + ((G.gn * G.gn * cx) + (G.gn * cy) + cz);
+    key[ga] = h;
     atomicAdd(&count[h], 1u);
 }
 
@@ -347,19 +404,54 @@ __device__ __forceinline__ void row_interval(const RowAtom &A, float ss, int zlo
     R = r;
 }
 
+/* per sorted atom, once: the exact float thresholds of the near (r + probe) and inside (r - dx)
+   tests, and its neighbour-cell y and z (packed) */
+__global__ void __launch_bounds__(256) k_atom_aux(QsBatch B, const float4 *__restrict__ sx, float4 *__restrict__ aux)
+{
+    uint32_t lb;
+    const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
+    const uint32_t i = lb * 256 + threadIdx.x;
+    if (i >= M.n_atoms) return;
+    const QsGrid G = M.G;
+    const size_t k = (size_t)M.atom_off + i;
+    const float4 a = sx[k];
+    const int acy = iclamp(cell_of(a.y, G.oy, G.dxn), 0, G.gn - 1);
+    const int acz = iclamp(cell_of(a.z, G.oz, G.dxn), 0, G.gn - 1);
+    aux[k] = make_float4(sqrt_threshold(QS_PROBE + a.w), sqrt_threshold(a.w - G.dx), __int_as_float((acy << 16) | acz), 0.0f);
+}
+
 /* rows per wavefront: R consecutive y rows (R * dx <= neighbour-cell size, so they span at most
    two neighbour cells in y and the candidate set is at most 3 x 4 cell columns) */
 #define P1_RMAX 4
+#define P1_QCAP 128
 
-__global__ void __launch_bounds__(256) k_pass1_rows(QsGrid G, const uint32_t *__restrict__ cstart,
-                                                    const float4 *__restrict__ sx, uint64_t *__restrict__ outm,
-                                                    uint64_t *__restrict__ shellm, int W, int R, int x0, int x1)
+__device__ __forceinline__ void wave_lds_sync()
 {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(256) k_pass1_rows(QsBatch B, const uint32_t *__restrict__ cstart_all,
+                                                    const float4 *__restrict__ sx, const float4 *__restrict__ aux,
+                                                    uint64_t *__restrict__ outm_all, uint64_t *__restrict__ shellm_all)
+{
+    __shared__ int queue_all[4][P1_QCAP];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int g = G.g, gn = G.gn;
-    const int w = blockIdx.x;
-    const int y0 = (blockIdx.y * 4 + wid) * R;
-    const int x = x0 + blockIdx.z;
+    int *queue = queue_all[wid];
+    uint32_t lb;
+    const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
+    const QsGrid G = M.G;
+    const int g = G.g, gn = G.gn, W = (int)M.W, R = M.rows_per_wave;
+    const uint32_t *__restrict__ cstart = cstart_all + M.cell_off;
+    uint64_t *__restrict__ outm = outm_all + M.word_off;
+    uint64_t *__restrict__ shellm = shellm_all + M.word_off;
+    const int nby = (g + 4 * R - 1) / (4 * R);
+    const int w = (int)(lb % (uint32_t)W);
+    const int by = (int)((lb / (uint32_t)W) % (uint32_t)nby);
+    const int x = (int)(lb / ((uint32_t)W * nby));
+    const int x1 = g;
+    const int y0 = (by * 4 + wid) * R;
     if (y0 >= g || x >= x1) return; /* whole wavefront */
     const int ny = min(R, g - y0);
     const int z0 = w * 64;
@@ -426,18 +518,18 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsGrid G, const uint32_t *__
     uint64_t m_in[P1_RMAX], m_near[P1_RMAX];
 #pragma unroll
     for (int k = 0; k < P1_RMAX; k++) m_in[k] = m_near[k] = 0ull;
-    for (int base = 0; base < total; base += 64) {
-        const int kk = base + lane;
-        int id = -1;
-#pragma unroll
-        for (int q = 0; q < 12; q++)
-            if (kk >= pre[q] && kk < pre[q + 1]) id = rs[q] + (kk - pre[q]);
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    /* interval work for up to 64 queued atoms, one per lane */
+    auto process = [&](int count) {
+        const int id = (lane < count) ? queue[lane] : -1;
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), ax = make_float4(0.f, 0.f, 0.f, 0.f);
         int acz = czmin, acy = 0;
         if (id >= 0) {
             a = sx[id];
-            acz = iclamp(cell_of(a.z, G.oz, G.dxn), 0, gn - 1);
-            acy = iclamp(cell_of(a.y, G.oy, G.dxn), 0, gn - 1);
+            ax = aux[id];
+            const int packed = __float_as_int(ax.z);
+            acy = packed >> 16;
+            acz = packed & 0xFFFF;
         }
         /* voxels whose z cell admits the atom's z cell: lanes [La, Ra] */
         int La, Ra;
@@ -471,12 +563,11 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsGrid G, const uint32_t *__
             }
             Ra = lo_i - 1;
         }
-        if (id < 0) continue;
+        if (id < 0) return;
         const int zlo = z0 + La, zhi = min(z0 + Ra, zmax);
-        if (zlo > zhi) continue;
-        const float sN = sqrt_threshold(QS_PROBE + a.w);
-        const float sD = sqrt_threshold(a.w - dx);
-        float ex = a.x - px;
+        if (zlo > zhi) return;
+        const float sN = ax.x, sD = ax.y;
+        const float ex = a.x - px;
         const float X = ex * ex;
         const float zc = (a.z - G.oz) * inv_dx;
         RowAtom A;
@@ -498,10 +589,49 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsGrid G, const uint32_t *__
             row_interval(A, sN, zlo, zhi, zb, zc, inv_dx, L, Rr);
             if (L > Rr) continue;
             m_near[k] |= bit_range(L - z0, Rr - z0);
+            if (!(A.S < sD)) continue;
             row_interval(A, sD, L, Rr, zb, zc, inv_dx, L, Rr);
             if (L <= Rr) m_in[k] |= bit_range(L - z0, Rr - z0);
         }
+    };
+
+    /* candidates that can reach one of the rows (xy distance below r + probe) are queued */
+    int qn = 0;
+    const float pymin = py[0], pymax = py[ny - 1];
+    for (int base = 0; base < total; base += 64) {
+        const int kk = base + lane;
+        int id = -1;
+#pragma unroll
+        for (int q = 0; q < 12; q++)
+            if (kk >= pre[q] && kk < pre[q + 1]) id = rs[q] + (kk - pre[q]);
+        bool keep = false;
+        if (id >= 0) {
+            const float4 a = sx[id];
+            const float sN = aux[id].x;
+            const float ex = a.x - px;
+            const float X = ex * ex;
+            const float yy = fminf(fmaxf(a.y, pymin), pymax); /* nearest row y */
+            const float ey = a.y - yy;
+            const float Y = ey * ey;
+            keep = (X + Y) < sN * 1.0001f + 1e-6f; /* conservative: exact tests follow */
+        }
+        const uint64_t km = __ballot(keep);
+        if (keep) queue[qn + __popcll(km & ((1ull << lane) - 1ull))] = id;
+        qn += __popcll(km);
+        wave_lds_sync();
+        if (qn >= 64) {
+            process(64);
+            wave_lds_sync();
+            const int rest = qn - 64;
+            int moved = (lane < rest) ? queue[64 + lane] : 0;
+            wave_lds_sync();
+            if (lane < rest) queue[lane] = moved;
+            wave_lds_sync();
+            qn = rest;
+        }
     }
+    if (qn > 0) process(qn);
+
 #pragma unroll
     for (int k = 0; k < P1_RMAX; k++) {
 #pragma unroll
@@ -529,16 +659,23 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsGrid G, const uint32_t *__
 }
 
 /* per-voxel form (27 cells per voxel, as the reference); cross-check of k_pass1_rows */
-__global__ void __launch_bounds__(256) k_pass1_direct(QsGrid G, const int2 *__restrict__ cse, const float4 *__restrict__ sx,
-                                                      uint64_t *__restrict__ outm, uint64_t *__restrict__ shellm, int W,
-                                                      int x0, int x1)
+__global__ void __launch_bounds__(256) k_pass1_direct(QsBatch B, const int2 *__restrict__ cse_all,
+                                                      const float4 *__restrict__ sx, uint64_t *__restrict__ outm_all,
+                                                      uint64_t *__restrict__ shellm_all)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int g = G.g;
-    const int w = blockIdx.x;
-    const int y = blockIdx.y * 4 + wid;
-    const int x = x0 + blockIdx.z;
-    if (y >= g || x >= x1) return;
+    uint32_t lb;
+    const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
+    const QsGrid G = M.G;
+    const int g = G.g, W = (int)M.W;
+    const int2 *__restrict__ cse = cse_all + M.cell_off;
+    uint64_t *__restrict__ outm = outm_all + M.word_off;
+    uint64_t *__restrict__ shellm = shellm_all + M.word_off;
+    const int nby = (g + 3) / 4;
+    const int w = (int)(lb % (uint32_t)W);
+    const int y = (int)((lb / (uint32_t)W) % (uint32_t)nby) * 4 + wid;
+    const int x = (int)(lb / ((uint32_t)W * nby));
+    if (y >= g || x >= g) return;
     const int z = w * 64 + lane;
     uint8_t res = QS_OUT;
     if (x < g - 1 && y < g - 1 && z < g - 1) {
@@ -633,23 +770,33 @@ __device__ __forceinline__ void emit_sign_bits(uint32_t *__restrict__ negm, uint
 }
 
 template <typename DT>
-__global__ void __launch_bounds__(256) k_pass2_tile(QsGrid G, const uint64_t *__restrict__ outm,
-                                                    const uint64_t *__restrict__ shellm, int W,
-                                                    float *__restrict__ field, uint32_t *__restrict__ negm,
-                                                    uint32_t *__restrict__ zerom, const int *__restrict__ offs,
-                                                    const int *__restrict__ norm_start,
-                                                    const int *__restrict__ enum_end, int x0, int x1)
+__global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *__restrict__ outm_all,
+                                                    const uint64_t *__restrict__ shellm_all, float *__restrict__ field_all,
+                                                    uint32_t *__restrict__ negm_all, uint32_t *__restrict__ zerom_all,
+                                                    const int *__restrict__ offs, const int *__restrict__ norm_start,
+                                                    const int *__restrict__ enum_end)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ int s_flags;
     constexpr int INF = (sizeof(DT) == 1) ? 0xFF : 0xFFFF;
-    const int r = G.r, g = G.g;
+    uint32_t lb;
+    const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
+    const QsGrid G = M.G;
+    const int r = G.r, g = G.g, W = (int)M.W;
+    const uint64_t *__restrict__ outm = outm_all + M.word_off;
+    const uint64_t *__restrict__ shellm = shellm_all + M.word_off;
+    float *__restrict__ field = field_all + M.vox_off;
+    uint32_t *__restrict__ negm = negm_all + 2 * M.word_off;
+    uint32_t *__restrict__ zerom = zerom_all + 2 * M.word_off;
+    const int x0 = 0, x1 = g;
+    const uint32_t nbx = (g + P2_TZ - 1) / P2_TZ, nby = (g + P2_TY - 1) / P2_TY;
+    const int bidx = (int)(lb % nbx), bidy = (int)((lb / nbx) % nby), bidz = (int)(lb / (nbx * nby));
     const int LX = P2_TX + 2 * r, LY = P2_TY + 2 * r, LZ = P2_TZ + 2 * r;
     uint64_t *rows = (uint64_t *)smem;
     uint32_t *own = (uint32_t *)(smem + (size_t)LX * LY * 8);
     DT *DZ = (DT *)(smem + (size_t)LX * LY * 8 + P2_TX * P2_TY * 4);
     DT *DY = DZ + (size_t)LX * LY * P2_TZ;
-    const int tz0 = blockIdx.x * P2_TZ, ty0 = blockIdx.y * P2_TY, tx0 = x0 + blockIdx.z * P2_TX;
+    const int tz0 = bidx * P2_TZ, ty0 = bidy * P2_TY, tx0 = x0 + bidz * P2_TX;
     if (threadIdx.x == 0) s_flags = 0;
     __syncthreads();
 
@@ -793,11 +940,11 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsGrid G, const uint64_t *__
 /* Field pass 2 for a fixed window half-width R: loops unrolled, y and x passes register-blocked
    (each thread keeps one column of the previous pass in registers). */
 template <int R>
-__global__ void __launch_bounds__(256) k_pass2_r(QsGrid G, const uint64_t *__restrict__ outm,
-                                                 const uint64_t *__restrict__ shellm, int W, float *__restrict__ field,
-                                                 uint32_t *__restrict__ negm, uint32_t *__restrict__ zerom,
+__global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__restrict__ outm_all,
+                                                 const uint64_t *__restrict__ shellm_all, float *__restrict__ field_all,
+                                                 uint32_t *__restrict__ negm_all, uint32_t *__restrict__ zerom_all,
                                                  const int *__restrict__ offs, const int *__restrict__ norm_start,
-                                                 const int *__restrict__ enum_end, int x0, int x1)
+                                                 const int *__restrict__ enum_end)
 {
     constexpr int LX = P2_TX + 2 * R, LY = P2_TY + 2 * R, LZ = P2_TZ + 2 * R;
     constexpr int INF = 255;
@@ -807,8 +954,18 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsGrid G, const uint64_t *__res
     __shared__ uint8_t DZ[LX * LY * P2_TZ];
     __shared__ uint8_t DY[LX * P2_TY * P2_TZ];
     __shared__ int s_flags;
-    const int g = G.g;
-    const int tz0 = blockIdx.x * P2_TZ, ty0 = blockIdx.y * P2_TY, tx0 = x0 + blockIdx.z * P2_TX;
+    uint32_t lb;
+    const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
+    const QsGrid G = M.G;
+    const int g = G.g, W = (int)M.W;
+    const uint64_t *__restrict__ outm = outm_all + M.word_off;
+    const uint64_t *__restrict__ shellm = shellm_all + M.word_off;
+    float *__restrict__ field = field_all + M.vox_off;
+    uint32_t *__restrict__ negm = negm_all + 2 * M.word_off;
+    uint32_t *__restrict__ zerom = zerom_all + 2 * M.word_off;
+    const int x1 = g;
+    const uint32_t nbx = (g + P2_TZ - 1) / P2_TZ, nby = (g + P2_TY - 1) / P2_TY;
+    const int tz0 = (int)(lb % nbx) * P2_TZ, ty0 = (int)((lb / nbx) % nby) * P2_TY, tx0 = (int)(lb / (nbx * nby)) * P2_TX;
     if (threadIdx.x == 0) s_flags = 0;
     __syncthreads();
 
@@ -952,15 +1109,25 @@ __device__ __forceinline__ bool mask_bit(const uint64_t *__restrict__ m, int g, 
 }
 
 /* direct (2r+1)^3 window, one thread per voxel: any r; also a cross-check of the tile kernel */
-__global__ void __launch_bounds__(256) k_pass2_direct(QsGrid G, const uint64_t *__restrict__ outm,
-                                                      const uint64_t *__restrict__ shellm, int W,
-                                                      float *__restrict__ field, uint32_t *__restrict__ negm,
-                                                      uint32_t *__restrict__ zerom, int x0, int x1)
+__global__ void __launch_bounds__(256) k_pass2_direct(QsBatch B, const uint64_t *__restrict__ outm_all,
+                                                      const uint64_t *__restrict__ shellm_all, float *__restrict__ field_all,
+                                                      uint32_t *__restrict__ negm_all, uint32_t *__restrict__ zerom_all)
 {
-    const int g = G.g, r = G.r;
-    int z = blockIdx.x * 64 + (threadIdx.x & 63);
-    int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    int x = x0 + blockIdx.z;
+    uint32_t lb;
+    const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
+    const QsGrid G = M.G;
+    const int g = G.g, r = G.r, W = (int)M.W;
+    const uint64_t *__restrict__ outm = outm_all + M.word_off;
+    const uint64_t *__restrict__ shellm = shellm_all + M.word_off;
+    float *__restrict__ field = field_all + M.vox_off;
+    uint32_t *__restrict__ negm = negm_all + 2 * M.word_off;
+    uint32_t *__restrict__ zerom = zerom_all + 2 * M.word_off;
+    const uint32_t nbx = (g + 63) / 64, nby = (g + 3) / 4;
+    const int bidx = (int)(lb % nbx);
+    int z = bidx * 64 + (threadIdx.x & 63);
+    int y = (int)((lb / nbx) % nby) * 4 + (threadIdx.x >> 6);
+    int x = (int)(lb / (nbx * nby));
+    const int x1 = g;
     if (x >= x1 || y >= g) return;
     const bool valid = z < g;
     size_t vid = ((size_t)x * g + y) * g + z;
@@ -990,7 +1157,7 @@ __global__ void __launch_bounds__(256) k_pass2_direct(QsGrid G, const uint64_t *
     if (valid) field[vid] = res;
     uint64_t n = __ballot(valid && res < 0.0f), zr = __ballot(valid && res == 0.0f);
     if ((threadIdx.x & 63) == 0) {
-        size_t w32 = ((size_t)x * g + y) * (2 * (size_t)W) + 2 * (size_t)blockIdx.x;
+        size_t w32 = ((size_t)x * g + y) * (2 * (size_t)W) + 2 * (size_t)bidx;
         negm[w32] = (uint32_t)n;
         negm[w32 + 1] = (uint32_t)(n >> 32);
         zerom[w32] = (uint32_t)zr;
@@ -1142,14 +1309,23 @@ __device__ __forceinline__ uint64_t ld_word(const uint64_t *__restrict__ m, int 
 }
 
 /* countVertexPerCell MarchingCubes.cu:571-658 in word form: slot masks and per-word counts */
-__global__ void __launch_bounds__(256) k_mc_count(QsGrid G, const uint64_t *__restrict__ N, const uint64_t *__restrict__ Zm,
-                                                  int W, uint64_t *__restrict__ S4, uint32_t *__restrict__ vcnt,
-                                                  uint32_t *__restrict__ tcnt, int x0, int x1)
+__global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__restrict__ N_all,
+                                                  const uint64_t *__restrict__ Zm_all, uint64_t *__restrict__ S4_all,
+                                                  uint32_t *__restrict__ vcnt_all, uint32_t *__restrict__ tcnt_all)
 {
-    const int g = G.g, lim = G.lim;
-    const int idx = blockIdx.x * 256 + threadIdx.x;
-    const int x = x0 + blockIdx.y;
-    if (idx >= g * W || x >= x1) return;
+    uint32_t lb;
+    const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
+    const QsGrid G = M.G;
+    const int g = G.g, lim = G.lim, W = (int)M.W;
+    const uint64_t *__restrict__ N = N_all + M.word_off;
+    const uint64_t *__restrict__ Zm = Zm_all + M.word_off;
+    uint64_t *__restrict__ S4 = S4_all + 4 * M.word_off;
+    uint32_t *__restrict__ vcnt = vcnt_all + M.word_off;
+    uint32_t *__restrict__ tcnt = tcnt_all + M.word_off;
+    const uint32_t nbx = ((uint32_t)g * W + 255) / 256;
+    const int idx = (int)(lb % nbx) * 256 + threadIdx.x;
+    const int x = (int)(lb / nbx);
+    if (idx >= g * W || x >= g) return;
     const int y = idx / W, w = idx - y * W;
     const int z0 = w * 64;
     const size_t wi = ((size_t)x * g + y) * W + w;
@@ -1329,20 +1505,35 @@ __device__ __forceinline__ void load_hood8(Hood &H, const MCView &V, int x, int 
    (generateTriangleVerticesSMEM MarchingCubes.cu:736-851 order); one wavefront per active word.
    Each polygonised cube also records its case and first triangle id (read by the smoothing
    pass), and each vertex its (point, slot) key. */
-__global__ void __launch_bounds__(256) k_mc_emit(QsGrid G, MCView V, const float *__restrict__ F,
-                                                 const uint32_t *__restrict__ list, uint32_t nact,
-                                                 const uint32_t *__restrict__ tbw, float *__restrict__ vert,
+__global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre, int nmol,
+                                                 const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
+                                                 const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
+                                                 const float *__restrict__ F_all, const uint32_t *__restrict__ list,
+                                                 uint32_t nact, const uint32_t *__restrict__ tbw, float *__restrict__ vert,
                                                  uint64_t *__restrict__ vkey, int *__restrict__ tri,
-                                                 uint8_t *__restrict__ cidx, uint32_t *__restrict__ tbase)
+                                                 uint8_t *__restrict__ cidx_all, uint32_t *__restrict__ tbase_all)
 {
     __shared__ Hood hoods[4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint32_t k = blockIdx.x * 4 + wid;
     const bool active = k < nact;
-    const int g = G.g, W = V.W, lim = G.lim;
     const uint32_t wi = active ? list[k] : 0u;
+    const QsMol &M = mols[seg_of64(wpre, nmol, wi)];
+    const QsGrid G = M.G;
+    const int g = G.g, W = (int)M.W, lim = G.lim;
+    MCView V;
+    V.N = N_all + M.word_off;
+    V.Zm = Zm_all + M.word_off;
+    V.S4 = S4_all + 4 * M.word_off;
+    V.vbw = vbw_all + M.word_off;
+    V.g = g;
+    V.W = W;
+    V.lim = lim;
+    const float *__restrict__ F = F_all + M.vox_off;
+    uint8_t *__restrict__ cidx = cidx_all + M.vox_off;
+    uint32_t *__restrict__ tbase = tbase_all + M.vox_off;
     int x = 0, y = 0, w = 0;
-    decode_word(wi, g, W, x, y, w);
+    decode_word((uint32_t)(wi - M.word_off), g, W, x, y, w);
     if (active) load_hood8(hoods[wid], V, x, y, w, lane);
     __syncthreads();
     if (!active) return;
@@ -1397,12 +1588,17 @@ __global__ void __launch_bounds__(256) k_mc_emit(QsGrid G, MCView V, const float
 
 /* ------------------------------------------------------------------------------------------ */
 /* closest atom per vertex (computeClosestAtom Kernels.cu:547-604)                              */
-__global__ void __launch_bounds__(256) k_closest_atom(QsGrid G, const int2 *__restrict__ cse, const float4 *__restrict__ sx,
-                                                      const int *__restrict__ perm, const float *__restrict__ vert,
-                                                      uint32_t nv, int sorted_ids, int *__restrict__ out)
+__global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
+                                                      int nmol, const int2 *__restrict__ cse_all,
+                                                      const float4 *__restrict__ sx, const int *__restrict__ perm,
+                                                      const float *__restrict__ vert, uint32_t nv, int sorted_ids,
+                                                      int *__restrict__ out)
 {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
+    const QsMol &M = mols[seg_of(vpre, nmol, v)];
+    const QsGrid G = M.G;
+    const int2 *__restrict__ cse = cse_all + M.cell_off;
     float px = vert[3 * (size_t)v], py = vert[3 * (size_t)v + 1], pz = vert[3 * (size_t)v + 2];
     int cx = cell_of(px, G.ox, G.dxn), cy = cell_of(py, G.oy, G.dxn), cz = cell_of(pz, G.oz, G.dxn);
     const int gn = G.gn;
@@ -1427,7 +1623,8 @@ __global__ void __launch_bounds__(256) k_closest_atom(QsGrid G, const int2 *__re
             }
         }
     }
-    out[v] = (best >= 0 && !sorted_ids) ? perm[best] : best;
+    /* atom indices are returned relative to the molecule's first atom */
+    out[v] = (best < 0) ? -1 : ((sorted_ids ? best : perm[best]) - (int)M.atom_off);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -1456,14 +1653,29 @@ __device__ __forceinline__ void push_pair(const float *__restrict__ src, uint32_
 /* one Jacobi pass, one thread per vertex.  Polygonised cubes carry their case (cidx) and first
    triangle id (tbase), so each incident triangle's other corners are read from the triangle
    array. */
-__global__ void __launch_bounds__(256) k_smooth(QsGrid G, MCView V, const uint64_t *__restrict__ vkey, uint32_t nv,
-                                                const uint8_t *__restrict__ cidx, const uint32_t *__restrict__ tbase,
+__global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre, int nmol,
+                                                const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
+                                                const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
+                                                const uint64_t *__restrict__ vkey, uint32_t nv,
+                                                const uint8_t *__restrict__ cidx_all, const uint32_t *__restrict__ tbase_all,
                                                 const int *__restrict__ tri, const float *__restrict__ src,
                                                 float *__restrict__ dst)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
+    const QsMol &M = mols[seg_of(vpre, nmol, v)];
+    const QsGrid G = M.G;
     const int g = G.g, lim = G.lim;
+    MCView V;
+    V.N = N_all + M.word_off;
+    V.Zm = Zm_all + M.word_off;
+    V.S4 = S4_all + 4 * M.word_off;
+    V.vbw = vbw_all + M.word_off;
+    V.g = g;
+    V.W = (int)M.W;
+    V.lim = lim;
+    const uint8_t *__restrict__ cidx = cidx_all + M.vox_off;
+    const uint32_t *__restrict__ tbase = tbase_all + M.vox_off;
     const uint64_t key = vkey[v];
     const int sl = (int)(key & 3ull);
     const uint64_t p = key >> 2;
@@ -1560,15 +1772,18 @@ __global__ void __launch_bounds__(256) k_tri_compact(const int *__restrict__ tri
 
 static inline unsigned nblk(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-void qs_launch_bbox(const float *pos, const float *rad, int n, float *part, int nparts, float *out, hipStream_t s)
+size_t qs_bbox_part_floats(int nmol) { return (size_t)nmol * BB_P * BB_NV; }
+
+void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, int nmol, float *part, float *out,
+                    hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(BB_NT), 0, s, pos, rad, n, part);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(BB_NT), 0, s, part, nparts, out);
+    hipLaunchKernelGGL(k_bbox_part, dim3(BB_P, nmol), dim3(BB_NT), 0, s, pos, rad, off, part);
+    hipLaunchKernelGGL(k_bbox_final, dim3(nmol), dim3(64), 0, s, part, out);
 }
 
-void qs_launch_hash_count(const float *pos, int n, QsGrid G, int *key, uint32_t *count, hipStream_t s)
+void qs_launch_hash_count(QsBatch B, uint32_t nblocks, const float *pos, int *key, uint32_t *count, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_hash_count, dim3(nblk(n, 256)), dim3(256), 0, s, pos, n, G, key, count);
+    hipLaunchKernelGGL(k_hash_count, dim3(nblocks), dim3(256), 0, s, B, pos, key, count);
 }
 
 void qs_launch_scatter(const int *key, int n, const uint32_t *start, uint32_t *cursor, int *perm, hipStream_t s)
@@ -1586,37 +1801,64 @@ void qs_launch_gather_atoms(const int *perm, const float *pos, const float *rad,
     hipLaunchKernelGGL(k_gather_atoms, dim3(nblk(n, 256)), dim3(256), 0, s, perm, pos, rad, n, sx);
 }
 
-void qs_launch_pass1(QsGrid G, const uint32_t *cstart, const float4 *sx, uint64_t *outm, uint64_t *shellm, int W, int x0,
-                     int x1, hipStream_t s)
+void qs_launch_atom_aux(QsBatch B, uint32_t nblocks, const float4 *sx, float4 *aux, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_atom_aux, dim3(nblocks), dim3(256), 0, s, B, sx, aux);
+}
+
+int qs_pass1_rows_per_wave(QsGrid G)
 {
     /* R rows per wavefront with R * dx <= neighbour-cell size */
     int R = (int)(G.dxn / G.dx);
-    R = R < 1 ? 1 : (R > P1_RMAX ? P1_RMAX : R);
-    dim3 grid((unsigned)W, nblk(G.g, 4 * R), (unsigned)(x1 - x0));
-    hipLaunchKernelGGL(k_pass1_rows, grid, dim3(256), 0, s, G, cstart, sx, outm, shellm, W, R, x0, x1);
+    return R < 1 ? 1 : (R > P1_RMAX ? P1_RMAX : R);
 }
 
-void qs_launch_pass1_direct(QsGrid G, const int2 *cse, const float4 *sx, uint64_t *outm, uint64_t *shellm, int W, int x0,
-                            int x1, hipStream_t s)
+uint32_t qs_pass1_blocks(QsGrid G, int W, int R, bool direct)
 {
-    dim3 grid((unsigned)W, nblk(G.g, 4), (unsigned)(x1 - x0));
-    hipLaunchKernelGGL(k_pass1_direct, grid, dim3(256), 0, s, G, cse, sx, outm, shellm, W, x0, x1);
+    if (direct) R = 1;
+    return (uint32_t)W * nblk(G.g, 4 * R) * (uint32_t)G.g;
 }
 
-/* field pass 2: the fixed-R kernel for r <= 9, else the runtime-r tile kernel (LDS permitting),
-   else the direct window; returns the path taken (0, 1, 2) */
-int qs_launch_pass2(QsGrid G, const uint64_t *outm, const uint64_t *shellm, int W, float *field, uint32_t *negm,
-                    uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end, int x0, int x1, bool direct,
-                    hipStream_t s)
+void qs_launch_pass1(QsBatch B, uint32_t nblocks, const uint32_t *cstart, const float4 *sx, const float4 *aux, uint64_t *outm,
+                     uint64_t *shellm, hipStream_t s)
 {
-    dim3 grid(nblk(G.g, P2_TZ), nblk(G.g, P2_TY), nblk(x1 - x0, P2_TX));
-    if (!direct) {
-#define QS_P2R(RR)                                                                                                         \
-    case RR:                                                                                                               \
-        hipLaunchKernelGGL(k_pass2_r<RR>, grid, dim3(256), 0, s, G, outm, shellm, W, field, negm, zerom, offs, norm_start, \
-                           enum_end, x0, x1);                                                                              \
-        return 0;
-        switch (G.r) {
+    hipLaunchKernelGGL(k_pass1_rows, dim3(nblocks), dim3(256), 0, s, B, cstart, sx, aux, outm, shellm);
+}
+
+void qs_launch_pass1_direct(QsBatch B, uint32_t nblocks, const int2 *cse, const float4 *sx, uint64_t *outm, uint64_t *shellm,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pass1_direct, dim3(nblocks), dim3(256), 0, s, B, cse, sx, outm, shellm);
+}
+
+/* field pass 2 kernel choice for half-window r: 0 fixed-R kernel (r <= 9), 1 runtime-r tile
+   kernel, 2 direct window */
+int qs_pass2_path(int r, bool direct)
+{
+    if (direct) return 2;
+    if (r >= 1 && r <= 9) return 0;
+    if (qs_pass2_lds_bytes(r) <= 65536 && 2 * r + P2_TZ <= 64) return 1;
+    return 2;
+}
+
+uint32_t qs_pass2_blocks(QsGrid G, int path)
+{
+    if (path == 2) return nblk(G.g, 64) * nblk(G.g, 4) * (uint32_t)G.g;
+    return nblk(G.g, P2_TZ) * nblk(G.g, P2_TY) * nblk(G.g, P2_TX);
+}
+
+/* one launch for a group of molecules sharing r (and so the path) */
+void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm, float *field,
+                     uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
+                     hipStream_t s)
+{
+    if (path == 0) {
+#define QS_P2R(RR)                                                                                                    \
+    case RR:                                                                                                          \
+        hipLaunchKernelGGL(k_pass2_r<RR>, dim3(nblocks), dim3(256), 0, s, B, outm, shellm, field, negm, zerom, offs, \
+                           norm_start, enum_end);                                                                     \
+        return;
+        switch (r) {
             QS_P2R(1)
             QS_P2R(2)
             QS_P2R(3)
@@ -1630,16 +1872,13 @@ int qs_launch_pass2(QsGrid G, const uint64_t *outm, const uint64_t *shellm, int 
             break;
         }
 #undef QS_P2R
-        size_t lds = qs_pass2_lds_bytes(G.r);
-        if (lds <= 65536 && 2 * G.r + P2_TZ <= 64) {
-            hipLaunchKernelGGL(k_pass2_tile<uint16_t>, grid, dim3(256), lds, s, G, outm, shellm, W, field, negm, zerom, offs,
-                               norm_start, enum_end, x0, x1);
-            return 1;
-        }
     }
-    dim3 gd(nblk(G.g, 64), nblk(G.g, 4), (unsigned)(x1 - x0));
-    hipLaunchKernelGGL(k_pass2_direct, gd, dim3(256), 0, s, G, outm, shellm, W, field, negm, zerom, x0, x1);
-    return 2;
+    if (path <= 1) {
+        hipLaunchKernelGGL(k_pass2_tile<uint16_t>, dim3(nblocks), dim3(256), qs_pass2_lds_bytes(r), s, B, outm, shellm, field,
+                           negm, zerom, offs, norm_start, enum_end);
+        return;
+    }
+    hipLaunchKernelGGL(k_pass2_direct, dim3(nblocks), dim3(256), 0, s, B, outm, shellm, field, negm, zerom);
 }
 
 void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *outm, uint64_t *shellm, hipStream_t s)
@@ -1648,11 +1887,12 @@ void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *ou
     hipLaunchKernelGGL(k_labels_to_masks, grid, dim3(256), 0, s, labels, g, W, outm, shellm);
 }
 
-void qs_launch_mc_count(QsGrid G, const uint64_t *N, const uint64_t *Zm, int W, uint64_t *S4, uint32_t *vcnt, uint32_t *tcnt,
-                        int x0, int x1, hipStream_t s)
+uint32_t qs_mc_count_blocks(QsGrid G, int W) { return nblk((uint64_t)G.g * W, 256) * (uint32_t)G.g; }
+
+void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
+                        uint32_t *tcnt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mc_count, dim3(nblk((uint64_t)G.g * W, 256), (unsigned)(x1 - x0)), dim3(256), 0, s, G, N, Zm, W, S4,
-                       vcnt, tcnt, x0, x1);
+    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(256), 0, s, B, N, Zm, S4, vcnt, tcnt);
 }
 
 void qs_launch_mc_flag(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *flag, hipStream_t s)
@@ -1666,42 +1906,30 @@ void qs_launch_mc_active(const uint32_t *vcnt, const uint32_t *tcnt, const uint3
     hipLaunchKernelGGL(k_mc_active, dim3(nblk(nw, 256)), dim3(256), 0, s, vcnt, tcnt, apos, nw, list);
 }
 
-static MCView make_view(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W)
-{
-    MCView V;
-    V.N = N;
-    V.Zm = Zm;
-    V.S4 = S4;
-    V.vbw = vbw;
-    V.g = G.g;
-    V.W = W;
-    V.lim = G.lim;
-    return V;
-}
-
-void qs_launch_mc_emit(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W,
-                       const float *F, const uint32_t *list, uint32_t nact, const uint32_t *tbw, float *vert, uint64_t *vkey,
-                       int *tri, uint8_t *cidx, uint32_t *tbase, hipStream_t s)
+void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, int nmol, const uint64_t *N, const uint64_t *Zm,
+                       const uint64_t *S4, const uint32_t *vbw, const float *F, const uint32_t *list, uint32_t nact,
+                       const uint32_t *tbw, float *vert, uint64_t *vkey, int *tri, uint8_t *cidx, uint32_t *tbase, hipStream_t s)
 {
     if (!nact) return;
-    hipLaunchKernelGGL(k_mc_emit, dim3(nblk(nact, 4)), dim3(256), 0, s, G, make_view(G, N, Zm, S4, vbw, W), F, list, nact, tbw,
+    hipLaunchKernelGGL(k_mc_emit, dim3(nblk(nact, 4)), dim3(256), 0, s, mols, wpre, nmol, N, Zm, S4, vbw, F, list, nact, tbw,
                        vert, vkey, tri, cidx, tbase);
 }
 
-void qs_launch_smooth(QsGrid G, const uint64_t *N, const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, int W,
-                      const uint64_t *vkey, uint32_t nv, const uint8_t *cidx, const uint32_t *tbase, const int *tri,
-                      const float *src, float *dst, hipStream_t s)
+void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, int nmol, const uint64_t *N, const uint64_t *Zm,
+                      const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
+                      const uint32_t *tbase, const int *tri, const float *src, float *dst, hipStream_t s)
 {
     if (!nv) return;
-    hipLaunchKernelGGL(k_smooth, dim3(nblk(nv, 256)), dim3(256), 0, s, G, make_view(G, N, Zm, S4, vbw, W), vkey, nv, cidx,
-                       tbase, tri, src, dst);
+    hipLaunchKernelGGL(k_smooth, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, nmol, N, Zm, S4, vbw, vkey, nv, cidx, tbase,
+                       tri, src, dst);
 }
 
-void qs_launch_closest_atom(QsGrid G, const int2 *cse, const float4 *sx, const int *perm, const float *vert, uint32_t nv,
-                            int sorted_ids, int *out, hipStream_t s)
+void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, int nmol, const int2 *cse, const float4 *sx,
+                            const int *perm, const float *vert, uint32_t nv, int sorted_ids, int *out, hipStream_t s)
 {
     if (!nv) return;
-    hipLaunchKernelGGL(k_closest_atom, dim3(nblk(nv, 256)), dim3(256), 0, s, G, cse, sx, perm, vert, nv, sorted_ids, out);
+    hipLaunchKernelGGL(k_closest_atom, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, nmol, cse, sx, perm, vert, nv,
+                       sorted_ids, out);
 }
 
 void qs_launch_tri_keep(const int *tri, uint32_t nt, uint32_t *keep, hipStream_t s)
@@ -1728,4 +1956,18 @@ void qs_launch_tile_scan(const uint32_t *in, uint32_t *out, uint64_t n, const ui
                          hipStream_t s)
 {
     hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)qs_scan_tiles(n)), dim3(SCAN_NT), 0, s, in, out, n, offsets, total);
+}
+
+/* value of a uint32 array at given indices (per-molecule boundaries of scans) */
+__global__ void k_gather_u32(const uint32_t *__restrict__ src, const uint64_t *__restrict__ idx, int n,
+                             uint32_t *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
+}
+
+void qs_launch_gather_u32(const uint32_t *src, const uint64_t *idx, int n, uint32_t *out, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_u32, dim3(nblk(n, 256)), dim3(256), 0, s, src, idx, n, out);
 }

@@ -97,7 +97,8 @@ extern "C" {
 
 /* N atoms uniform in a ball of radius (3N / (4 pi rho))^(1/3); 0 on success */
 int qs_synth_ball(uint64_t seed, int n, double density, double min_dist, float *pos, float *rad) {
-    if (n <= 0 || density <= 0) return -1;
+    if (n < 0 || density <= 0) return -1;
+    if (n == 0) return 0;
     double R = std::cbrt(3.0 * n / (4.0 * M_PI * density));
     return synth(seed, n, 0.0, R, min_dist, pos, rad);
 }
