@@ -46,12 +46,131 @@ def cpu_baseline(pos, rad, voxel, target_s=12.0):
                        "molecule (%d voxels, %.1f s)" % (planes, g, v, t))
 
 
+def estimated_voxels(pos, rad, voxel):
+    """g^3 of the molecule's grid by the grid rule (extent + 2 max radius + 4 probe)."""
+    import numpy as np
+
+    ext = float(np.max(pos.max(axis=0) - pos.min(axis=0)))
+    g = int(np.ceil((ext + 2.0 * float(rad.max()) + 4 * 1.4) / voxel))
+    return g ** 3
+
+
+def chunks_by_voxels(mols, voxel, budget):
+    """Consecutive groups of molecules whose grids together stay under `budget` voxels."""
+    out, cur, acc = [], [], 0
+    for pr in mols:
+        v = estimated_voxels(pr[0], pr[1], voxel)
+        if cur and acc + v > budget:
+            out.append(cur)
+            cur, acc = [], 0
+        cur.append(pr)
+        acc += v
+    if cur:
+        out.append(cur)
+    return out
+
+
+def config4_sets(qs, n_mol=256, n_frames=1000):
+    """256 molecules (sizes log-uniform 500-20,000, size seed 4, molecule m seed 4000+m) and
+    n_frames trajectory frames of one 10,000-atom molecule (seed 4500; frame f jittered by
+    N(0, 0.3 A) per coordinate with seed 5000+f)."""
+    sizes = qs.synth_sizes(4, n_mol, 500, 20000)
+    mols = [qs.synth_ball(4000 + m, int(n)) for m, n in enumerate(sizes)]
+    base_pos, base_rad = qs.synth_ball(4500, 10000)
+    frames = [(qs.synth_jitter(5000 + f, base_pos, 0.3), base_rad) for f in range(n_frames)]
+    return mols, frames
+
+
+def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_000_000):
+    """meshes/s of the plain path (one call per molecule) against the batched path (one launch
+    sequence per chunk of molecules) on the two streams of the batched workload, at two
+    boundaries: device-resident (atoms already in HBM, meshes left in HBM) and host arrays in /
+    host arrays out (API_computeSES per molecule against qs_compute_batch + one bulk copy)."""
+    import numpy as np
+
+    mols, frames = config4_sets(qs, 256, n_frames)
+    res = {}
+    for name, items in (("molecules_256", mols), ("frames_%d" % n_frames, frames)):
+        vox = sum(estimated_voxels(p, r, voxel) for p, r in items)
+        chunks = chunks_by_voxels(items, voxel, budget)
+        counts = [len(r) for _, r in items]
+        off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        d_pos = hipmem.DeviceArray(np.concatenate([p for p, _ in items]).astype(np.float32))
+        d_rad = hipmem.DeviceArray(np.concatenate([r for _, r in items]).astype(np.float32))
+        bounds, k = [], 0
+        for ch in chunks:
+            bounds.append((k, k + len(ch)))
+            k += len(ch)
+        qs.compute_batch(chunks[0][:4], voxel, smooth)  # warm-up
+        # device-resident: batched
+        hipmem.synchronize()
+        t0 = time.perf_counter()
+        nv_bd = 0
+        for a, b in bounds:
+            vo, *_ = qs.compute_batch_device(d_pos.ptr + 12 * int(off[a]), d_rad.ptr + 4 * int(off[a]), counts[a:b], voxel,
+                                             smooth)
+            nv_bd += int(vo[-1])
+        hipmem.synchronize()
+        tbd = time.perf_counter() - t0
+        qs.set_option("timing", 1)
+        a, b = bounds[0]
+        qs.compute_batch_device(d_pos.ptr + 12 * int(off[a]), d_rad.ptr + 4 * int(off[a]), counts[a:b], voxel, smooth)
+        chunk_stage_ms = {k: round(v, 3) for k, v in qs.stage_times().items()}
+        qs.set_option("timing", 0)
+        # device-resident: plain, one call per molecule
+        t0 = time.perf_counter()
+        nv_pd = 0
+        for m in range(len(items)):
+            nv, *_ = qs.compute_device(d_pos.ptr + 12 * int(off[m]), d_rad.ptr + 4 * int(off[m]), counts[m], voxel, smooth)
+            nv_pd += nv
+        hipmem.synchronize()
+        tpd = time.perf_counter() - t0
+        d_pos.free()
+        d_rad.free()
+        # host in, host out
+        t0 = time.perf_counter()
+        nv_b = 0
+        for ch in chunks:
+            out = qs.compute_batch(ch, voxel, smooth)
+            nv_b += sum(len(o[0]) for o in out)
+        tb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        nv_p = 0
+        for p, r in items:
+            nv_p += len(qs.compute_ses(p, r, voxel, smooth)[0])
+        tp = time.perf_counter() - t0
+        if not nv_b == nv_p == nv_bd == nv_pd:
+            raise RuntimeError("batched and plain meshes differ in size (%d %d %d %d vertices)" % (nv_b, nv_p, nv_bd, nv_pd))
+        n = len(items)
+        print("batched stream %s: %d meshes, device-resident plain %.3f s batched %.3f s, host-to-host plain %.3f s "
+              "batched %.3f s" % (name, n, tpd, tbd, tp, tb), file=sys.stderr, flush=True)
+        res[name] = dict(meshes=n, voxels=vox, launch_sequences=len(chunks), vertices=nv_b,
+                         first_sequence=dict(meshes=bounds[0][1] - bounds[0][0], stage_ms=chunk_stage_ms),
+                         device_resident=dict(plain_meshes_per_s=round(n / tpd, 2), batched_meshes_per_s=round(n / tbd, 2),
+                                              batched_voxels_per_s=round(vox / tbd, 1), speedup=round(tpd / tbd, 3)),
+                         host_to_host=dict(plain_meshes_per_s=round(n / tp, 2), batched_meshes_per_s=round(n / tb, 2),
+                                           batched_voxels_per_s=round(vox / tb, 1), speedup=round(tp / tb, 3)))
+    streams = list(res.values())
+    for key in ("device_resident", "host_to_host"):
+        tot = sum(v["meshes"] for v in streams)
+        tp = sum(v["meshes"] / v[key]["plain_meshes_per_s"] for v in streams)
+        tb = sum(v["meshes"] / v[key]["batched_meshes_per_s"] for v in streams)
+        res.setdefault("all", {"meshes": tot})[key] = dict(plain_meshes_per_s=round(tot / tp, 2),
+                                                           batched_meshes_per_s=round(tot / tb, 2),
+                                                           speedup=round(tp / tb, 3))
+    res["workload"] = ("256 molecules (500-20,000 atoms, log-uniform) and %d trajectory frames of a 10,000-atom "
+                       "molecule, voxel %.1f A, %d Laplacian pass" % (n_frames, voxel, smooth))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain streams")
+    ap.add_argument("--frames", type=int, default=1000, help="trajectory frames of the batched workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the SES path runs on a single device and this build has no "
@@ -110,6 +229,10 @@ def main():
         qs.compute_ses(pos, rad, W["voxel"], W["smooth"])
     api_s = (time.perf_counter() - t2) / api_steps
 
+    batched = None
+    if not args.no_batch:
+        batched = bench_batched(qs, hipmem, n_frames=args.frames)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pos, rad, W["voxel"])
@@ -135,6 +258,7 @@ def main():
         "api_host_to_host_voxels_per_s": round(V / api_s, 1),
         "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         "roofline": roofline,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

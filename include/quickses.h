@@ -97,6 +97,12 @@ int qs_compute_batch_device(const float *d_pos, const float *d_rad, const unsign
 int qs_compute_batch(const float *pos, const float *rad, const unsigned int *atom_off, int nmol, const float *reso,
                      int doSmoothing, unsigned int *nvert, unsigned int *ntri);
 int qs_batch_mesh(int m, float *vert, int *tri, int *atom_id);
+/* Offsets of the last batch (host arrays of nmol+1 entries, may be NULL); returns nmol, or a
+ * negative error code when there is no batch result. */
+int qs_batch_offsets(unsigned int *vert_off, unsigned int *tri_off);
+/* All meshes of the last batch at once: vertices (3 * vert_off[nmol] floats), triangles
+ * (3 * tri_off[nmol] ints, vertex indices local to each molecule), atom ids (vert_off[nmol]). */
+int qs_batch_fetch(float *vert, int *tri, int *atom_id);
 
 /* Per-stage GPU times of the last computation (needs option "timing" or "trace"): stage names
  * comma-separated into names, milliseconds into ms; returns the number of stages. */

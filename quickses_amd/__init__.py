@@ -59,6 +59,10 @@ def lib():
     L.qs_compute_batch.restype = ctypes.c_int
     L.qs_batch_mesh.argtypes = [ctypes.c_int, vp, vp, vp]
     L.qs_batch_mesh.restype = ctypes.c_int
+    L.qs_batch_fetch.argtypes = [vp, vp, vp]
+    L.qs_batch_fetch.restype = ctypes.c_int
+    L.qs_batch_offsets.argtypes = [u32p, u32p]
+    L.qs_batch_offsets.restype = ctypes.c_int
     L.qs_last_grid.argtypes = [f3p, ctypes.POINTER(ctypes.c_int)]
     L.qs_batch_grid.argtypes = [ctypes.c_int, f3p, ctypes.POINTER(ctypes.c_int)]
     L.qs_batch_grid.restype = ctypes.c_int
@@ -180,16 +184,45 @@ def compute_batch(molecules, reso=0.5, smooth=1):
     rc = L.qs_compute_batch(_fptr(P), _fptr(R), off.ctypes.data_as(u32p), nmol, _fptr(reso), int(smooth),
                             nv.ctypes.data_as(u32p), nt.ctypes.data_as(u32p))
     _check(rc)
-    out = []
-    for m in range(nmol):
-        V = np.empty((int(nv[m]), 3), dtype=np.float32)
-        T = np.empty((int(nt[m]), 3), dtype=np.int32)
-        A = np.empty(int(nv[m]), dtype=np.int32)
-        rc = L.qs_batch_mesh(m, ctypes.c_void_p(V.ctypes.data), ctypes.c_void_p(T.ctypes.data),
-                             ctypes.c_void_p(A.ctypes.data))
-        _check(rc)
-        out.append((V, T, A))
-    return out
+    vo = np.zeros(nmol + 1, dtype=np.int64)
+    to = np.zeros(nmol + 1, dtype=np.int64)
+    vo[1:] = np.cumsum(nv[:nmol])
+    to[1:] = np.cumsum(nt[:nmol])
+    V = np.empty((int(vo[-1]), 3), dtype=np.float32)
+    T = np.empty((int(to[-1]), 3), dtype=np.int32)
+    A = np.empty(int(vo[-1]), dtype=np.int32)
+    rc = L.qs_batch_fetch(ctypes.c_void_p(V.ctypes.data), ctypes.c_void_p(T.ctypes.data), ctypes.c_void_p(A.ctypes.data))
+    _check(rc)
+    return [(V[vo[m]:vo[m + 1]], T[to[m]:to[m + 1]], A[vo[m]:vo[m + 1]]) for m in range(nmol)]
+
+
+def batch_offsets():
+    """(vert_off, tri_off) of the last batch: nmol + 1 entries each (triangles counted in triangles)."""
+    L = lib()
+    n = L.qs_batch_offsets(None, None)
+    if n < 0:
+        _check(n)
+    vo = np.zeros(n + 1, dtype=np.uint32)
+    to = np.zeros(n + 1, dtype=np.uint32)
+    u32p = ctypes.POINTER(ctypes.c_uint)
+    L.qs_batch_offsets(vo.ctypes.data_as(u32p), to.ctypes.data_as(u32p))
+    return vo, to
+
+
+def batch_mesh(m):
+    """Mesh of molecule m of the last batch (vertex indices local to it), copied to host."""
+    L = lib()
+    vo, to = batch_offsets()
+    if not 0 <= m < len(vo) - 1:
+        raise QuickSESError("no molecule %d in the last batch" % m)
+    nv, nt = int(vo[m + 1] - vo[m]), int(to[m + 1] - to[m])
+    V = np.empty((nv, 3), dtype=np.float32)
+    T = np.empty((nt, 3), dtype=np.int32)
+    A = np.empty(nv, dtype=np.int32)
+    rc = L.qs_batch_mesh(int(m), ctypes.c_void_p(V.ctypes.data), ctypes.c_void_p(T.ctypes.data),
+                         ctypes.c_void_p(A.ctypes.data))
+    _check(rc)
+    return V, T, A
 
 
 def compute_batch_device(d_pos, d_rad, counts, reso=0.5, smooth=1):

@@ -276,7 +276,7 @@ struct Engine {
     DevBuf labels, outm, shellm, field, negm, zerom;
     DevBuf s4, vcnt, tcnt, vbw, tbw, apos, alist, totals, bounds, bidx;
     DevBuf vert, vert2, vert3, vkey, cidx, tbase, tri, tri_out, atom;
-    DevBuf keep, keeppos;
+    DevBuf keep, keeppos, tri_loc;
     DevBuf scan_tmp;
     std::vector<P2Group> groups;
     std::vector<P2Tables *> tables;
@@ -289,6 +289,7 @@ struct Engine {
     uint32_t n_atoms_total = 0;
     uint64_t ncell_total = 0, nw_total = 0, vox_total = 0;
     uint32_t nv = 0, nt_all = 0, nt = 0, n_active = 0;
+    bool batch_valid = false; /* result arrays, vert_off / tri_off and their device copies belong to run_batch */
     float *res_vert = nullptr;
     int *res_tri = nullptr, *res_atom = nullptr;
 
@@ -623,6 +624,7 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
                       const std::vector<BBox> *host_bbox)
 {
     const int nmol = (int)in.size();
+    batch_valid = false;
     nv = nt = nt_all = n_active = 0;
     res_vert = nullptr;
     res_tri = nullptr;
@@ -630,7 +632,10 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     vert_off.assign(nmol + 1, 0);
     tri_off.assign(nmol + 1, 0);
     mols.clear();
-    if (nmol == 0) return 0;
+    if (nmol == 0) {
+        batch_valid = true;
+        return 0;
+    }
     StageTimer tm;
     tm.on = g_opt.trace != 0 || g_opt.timing != 0;
     tm.s = stream;
@@ -739,6 +744,7 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     nt = tri_off[nmol];
     res_tri = as<int>(tri_out);
     res_atom = as<int>(atom);
+    batch_valid = true; /* device copies: vertex offsets after the molecule prefixes in d_off, triangle offsets in bounds */
     tm.report(mols[0].G, n);
     return 0;
 }
@@ -761,6 +767,7 @@ int single_grid(Engine &E, const QsGrid &G, size_t &at_mc, uint32_t &nb_mc, size
     E.ncell_total = (uint64_t)G.gn * G.gn * G.gn;
     E.nw_total = (uint64_t)G.g * G.g * M.W;
     E.vox_total = (uint64_t)G.g * G.g * G.g;
+    E.batch_valid = false;
     E.vert_off.assign(2, 0);
     E.tri_off.assign(2, 0);
     size_t at_atoms, at_p1;
@@ -1000,9 +1007,40 @@ int qs_compute_batch(const float *pos, const float *rad, const unsigned int *ato
     return 0;
 }
 
+int qs_batch_offsets(unsigned int *vert_off, unsigned int *tri_off)
+{
+    Engine &E = g_eng;
+    if (!E.batch_valid) return -set_error(3, "no batch result");
+    const int nmol = (int)E.mols.size();
+    for (int m = 0; m <= nmol; m++) {
+        if (vert_off) vert_off[m] = E.vert_off[m];
+        if (tri_off) tri_off[m] = E.tri_off[m];
+    }
+    return nmol;
+}
+
+int qs_batch_fetch(float *vert, int *tri, int *atom_id)
+{
+    Engine &E = g_eng;
+    if (!E.batch_valid) return set_error(3, "no batch result");
+    const int nmol = (int)E.mols.size();
+    if (E.nv && vert) QS_HIP(hipMemcpy(vert, E.res_vert, (size_t)E.nv * 12, hipMemcpyDeviceToHost));
+    if (E.nv && atom_id) QS_HIP(hipMemcpy(atom_id, E.res_atom, (size_t)E.nv * 4, hipMemcpyDeviceToHost));
+    if (E.nt && tri) {
+        QS_TRY(ensure(E.tri_loc, (size_t)E.nt * 12));
+        qs_launch_tri_localize(E.res_tri, E.nt, as<uint32_t>(E.bounds), as<uint32_t>(E.d_off) + (nmol + 1), nmol,
+                               as<int>(E.tri_loc), E.stream);
+        QS_HIP(hipGetLastError());
+        QS_HIP(hipMemcpyAsync(tri, E.tri_loc.p, (size_t)E.nt * 12, hipMemcpyDeviceToHost, E.stream));
+        QS_HIP(hipStreamSynchronize(E.stream));
+    }
+    return 0;
+}
+
 int qs_batch_mesh(int m, float *vert, int *tri, int *atom_id)
 {
     Engine &E = g_eng;
+    if (!E.batch_valid) return set_error(3, "no batch result");
     if (m < 0 || m + 1 >= (int)E.vert_off.size() || m + 1 >= (int)E.tri_off.size()) return set_error(3, "no molecule %d", m);
     const uint32_t v0 = E.vert_off[m], nvm = E.vert_off[m + 1] - v0;
     const uint32_t t0 = E.tri_off[m], ntm = E.tri_off[m + 1] - t0;

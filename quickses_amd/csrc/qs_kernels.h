@@ -59,6 +59,8 @@ void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, int nmol, const u
 /* degenerate-triangle filter */
 void qs_launch_tri_keep(const int *tri, uint32_t nt, uint32_t *keep, hipStream_t s);
 void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *pos, int *out, hipStream_t s);
+void qs_launch_tri_localize(const int *tri, uint32_t nt, const uint32_t *tpre, const uint32_t *vpre, int nmol, int *out,
+                            hipStream_t s);
 
 /* device-wide exclusive scan pieces */
 uint64_t qs_scan_tiles(uint64_t n);

@@ -1767,6 +1767,16 @@ __global__ void __launch_bounds__(256) k_tri_compact(const int *__restrict__ tri
     }
 }
 
+/* triangles of a batch with vertex indices local to their molecule */
+__global__ void __launch_bounds__(256) k_tri_localize(const int *__restrict__ tri, uint32_t nt, const uint32_t *__restrict__ tpre,
+                                                      const uint32_t *__restrict__ vpre, int nmol, int *__restrict__ out)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int v0 = (int)vpre[seg_of(tpre, nmol, t)];
+    for (int c = 0; c < 3; c++) out[3 * (size_t)t + c] = tri[3 * (size_t)t + c] - v0;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* launchers                                                                                    */
 
@@ -1936,6 +1946,13 @@ void qs_launch_tri_keep(const int *tri, uint32_t nt, uint32_t *keep, hipStream_t
 {
     if (!nt) return;
     hipLaunchKernelGGL(k_tri_keep, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, keep);
+}
+
+void qs_launch_tri_localize(const int *tri, uint32_t nt, const uint32_t *tpre, const uint32_t *vpre, int nmol, int *out,
+                            hipStream_t s)
+{
+    if (nt == 0) return;
+    hipLaunchKernelGGL(k_tri_localize, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, tpre, vpre, nmol, out);
 }
 
 void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *pos, int *out, hipStream_t s)

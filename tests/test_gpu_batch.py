@@ -103,3 +103,13 @@ def test_batch_errors(qs):
     with pytest.raises(qs.QuickSESError):
         qs.compute_batch([(pos, rad)], -1.0, 1)
     assert qs.compute_batch([], 0.5, 1) == []
+
+
+def test_batch_getters_agree(qs):
+    mols = _mixed(qs)
+    out = qs.compute_batch(mols, 0.5, 1)
+    vo, to = qs.batch_offsets()
+    assert len(vo) == len(mols) + 1
+    for m in range(len(mols)):
+        assert vo[m + 1] - vo[m] == len(out[m][0]) and to[m + 1] - to[m] == len(out[m][1])
+        _same_mesh(qs.batch_mesh(m), out[m])
