@@ -163,6 +163,124 @@ def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_0
     return res
 
 
+def config5_requests(qs, k):
+    """k surface requests: atom counts log-uniform 1,000-200,000 (size seed 5, request i seed
+    50000+i), voxel size uniform over {0.3, 0.5, 1.0} A, one Laplacian pass."""
+    import numpy as np
+
+    sizes = qs.synth_sizes(5, k, 1000, 200000)
+    rng = np.random.default_rng(5)
+    voxels = rng.choice([0.3, 0.5, 1.0], size=k)
+    return [(qs.synth_ball(50000 + i, int(n)), float(v)) for i, (n, v) in enumerate(zip(sizes, voxels))]
+
+
+def _pct(xs, q):
+    import numpy as np
+
+    return round(float(np.percentile(np.asarray(xs), q)), 3)
+
+
+def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=32 << 30):
+    """Poisson arrivals at `load` x the plain path's capacity.  Plain path: each request alone,
+    atoms copied to HBM and meshed by qs_compute_device (mesh left in HBM), first come first
+    served.  Served path: the request server (continuous batching over `workers` engines, HBM
+    budget admission, meshes resident in a device arena).  Latency = arrival -> mesh resident on
+    the device."""
+    import threading
+
+    import numpy as np
+
+    reqs = config5_requests(qs, k)
+    smooth = 1
+
+    def plain_one(pos, rad, reso):
+        dp, dr = hipmem.DeviceArray(pos), hipmem.DeviceArray(rad)
+        try:
+            qs.compute_device(dp.ptr, dr.ptr, len(rad), reso, smooth)
+        finally:
+            dp.free()
+            dr.free()
+
+    plain_one(reqs[0][0][0], reqs[0][0][1], reqs[0][1])  # warm-up
+    t0 = time.perf_counter()
+    for (pos, rad), reso in reqs:
+        plain_one(pos, rad, reso)
+    t_plain_serial = time.perf_counter() - t0
+    capacity = k / t_plain_serial
+    lam = load * capacity
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=k))
+
+    def wait_until(start, t):
+        while True:
+            d = start + t - time.perf_counter()
+            if d <= 0:
+                return
+            time.sleep(min(d, 0.0005) if d > 0.0002 else 0)
+
+    # plain path under the same arrivals: one request at a time, first come first served
+    lat_plain = []
+    start = time.perf_counter()
+    for ((pos, rad), reso), ta in zip(reqs, arrivals):
+        wait_until(start, ta)
+        plain_one(pos, rad, reso)
+        lat_plain.append(1e3 * (time.perf_counter() - start - ta))
+    wall_plain = time.perf_counter() - start
+
+    # served path
+    lat, queue, bsize = [], [], []
+    with qs.Server(hbm_budget_bytes=budget, result_bytes=arena, workers=workers) as srv:
+        w = srv.submit(reqs[0][0][0][:2000], reqs[0][0][1][:2000], 1.0, smooth)  # warm-up of every engine
+        srv.poll(w, -1)
+        srv.release(w)
+        ids = [None] * k
+        got = threading.Event()
+
+        def poller():
+            for i in range(k):
+                while ids[i] is None:
+                    time.sleep(0.0002)
+                inf = srv.poll(ids[i], -1)
+                lat.append(inf["latency_ms"])
+                queue.append(inf["queue_ms"])
+                bsize.append(inf["batch_size"])
+                srv.release(ids[i])
+            got.set()
+
+        th = threading.Thread(target=poller)
+        th.start()
+        start = time.perf_counter()
+        for i, (((pos, rad), reso), ta) in enumerate(zip(reqs, arrivals)):
+            wait_until(start, ta)
+            ids[i] = srv.submit(pos, rad, reso, smooth)
+        th.join()
+        wall_served = time.perf_counter() - start
+        st = srv.stats()
+        # saturated: every request submitted at once; throughput of the server against the plain
+        # path's serial capacity
+        start = time.perf_counter()
+        ids = [srv.submit(pos, rad, reso, smooth) for (pos, rad), reso in reqs]
+        lat_sat = []
+        for rid in ids:
+            lat_sat.append(srv.poll(rid, -1)["latency_ms"])
+            srv.release(rid)
+        wall_sat = time.perf_counter() - start
+    return dict(
+        requests=k, arrival_rate_per_s=round(lam, 2), load=load,
+        plain_capacity_meshes_per_s=round(capacity, 2),
+        plain=dict(meshes_per_s=round(k / wall_plain, 2), p50_ms=_pct(lat_plain, 50), p99_ms=_pct(lat_plain, 99)),
+        served=dict(meshes_per_s=round(k / wall_served, 2), p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99),
+                    queue_p50_ms=_pct(queue, 50), mean_batch=round(float(np.mean(bsize)), 2),
+                    batches=st["batches"] - 1, workers=workers, max_busy_workers=st["max_busy_workers"],
+                    hbm_budget_GiB=budget >> 30),
+        saturated=dict(plain_meshes_per_s=round(capacity, 2), served_meshes_per_s=round(k / wall_sat, 2),
+                       speedup=round(wall_sat and (k / wall_sat) / capacity, 3), served_p50_ms=_pct(lat_sat, 50),
+                       served_p99_ms=_pct(lat_sat, 99)),
+        workload="config-5 stream: atom counts log-uniform 1,000-200,000, voxel {0.3, 0.5, 1.0} A, 1 Laplacian "
+                 "pass, Poisson arrivals; latency = arrival -> mesh resident in HBM",
+    )
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +289,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain streams")
     ap.add_argument("--frames", type=int, default=1000, help="trajectory frames of the batched workload")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
+    ap.add_argument("--requests", type=int, default=400, help="requests of the served stream")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the SES path runs on a single device and this build has no "
@@ -233,6 +353,11 @@ def main():
     if not args.no_batch:
         batched = bench_batched(qs, hipmem, n_frames=args.frames)
 
+    served = None
+    if not args.no_serve:
+        served = bench_served(qs, hipmem, k=args.requests)
+        print("served: %s" % json.dumps(served), file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pos, rad, W["voxel"])
@@ -259,6 +384,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         "roofline": roofline,
         "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

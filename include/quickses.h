@@ -128,6 +128,32 @@ int qs_stage_pass2(const float *fparams, const int *iparams, const uint8_t *labe
 int qs_stage_mesh(const float *fparams, const int *iparams, const float *field, unsigned int *nvert,
                   unsigned int *ntri, float *vert, int64_t vert_cap, int *tri, int64_t tri_cap);
 
+/* Request server (no reference counterpart; the reference meshes one molecule per synchronous
+ * call).  Requests of any size and voxel size are submitted at any time; `workers` threads,
+ * each with its own engine and HIP stream, take pending requests in arrival order while their
+ * estimated device memory fits hbm_budget_bytes / workers, mesh them as one batch and keep
+ * each mesh in a device-resident result arena of result_bytes until it is released.  Batches
+ * of different workers overlap on the device.  A result equals API_computeSES on the same
+ * request alone, with triangle indices local to the request.
+ *   qs_serve_open    starts the workers on the current device; one server per process
+ *   qs_serve_submit  copies the request and returns its id (>= 0) or -error
+ *   qs_serve_poll    0 = done (device pointers into the arena, valid until release;
+ *                    latency_ms[3] = {submit -> resident, submit -> batch start, batch size}),
+ *                    1 = not finished after wait_ms (-1 waits), < 0 = error
+ *   qs_serve_fetch   waits, then copies the mesh to host arrays (any may be NULL)
+ *   qs_serve_release frees the request's result
+ *   qs_serve_stats   {batches, requests batched, largest batch, most workers busy at once,
+ *                    arena bytes in use, arena bytes, requests pending}; returns 7
+ *   qs_serve_close   finishes pending work, stops the workers, frees everything */
+int qs_serve_open(uint64_t hbm_budget_bytes, uint64_t result_bytes, int workers);
+int64_t qs_serve_submit(const float *pos, const float *rad, unsigned int N, float reso, int doSmoothing);
+int qs_serve_poll(int64_t id, int wait_ms, unsigned int *nvert, unsigned int *ntri, float **d_vert, int **d_tri,
+                  int **d_atom_id, double *latency_ms);
+int qs_serve_fetch(int64_t id, float *vert, int *tri, int *atom_id);
+int qs_serve_release(int64_t id);
+int qs_serve_stats(uint64_t *out, int n);
+int qs_serve_close(void);
+
 /* Seeded synthetic molecules (test and benchmark inputs). */
 int qs_synth_ball(uint64_t seed, int n, double density, double min_dist, float *pos, float *rad);
 int qs_synth_shell(uint64_t seed, int n, double r_in, double r_out, double min_dist, float *pos, float *rad);

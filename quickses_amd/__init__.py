@@ -75,6 +75,22 @@ def lib():
     L.qs_stage_mesh.argtypes = [f3p, ctypes.POINTER(ctypes.c_int), vp, u32p, u32p, vp, ctypes.c_int64, vp,
                                 ctypes.c_int64]
     L.qs_stage_mesh.restype = ctypes.c_int
+    i64 = ctypes.c_int64
+    L.qs_serve_open.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]
+    L.qs_serve_open.restype = ctypes.c_int
+    L.qs_serve_submit.argtypes = [f3p, f3p, ctypes.c_uint, ctypes.c_float, ctypes.c_int]
+    L.qs_serve_submit.restype = i64
+    L.qs_serve_poll.argtypes = [i64, ctypes.c_int, u32p, u32p, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                ctypes.POINTER(ctypes.c_double)]
+    L.qs_serve_poll.restype = ctypes.c_int
+    L.qs_serve_fetch.argtypes = [i64, vp, vp, vp]
+    L.qs_serve_fetch.restype = ctypes.c_int
+    L.qs_serve_release.argtypes = [i64]
+    L.qs_serve_release.restype = ctypes.c_int
+    L.qs_serve_stats.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]
+    L.qs_serve_stats.restype = ctypes.c_int
+    L.qs_serve_close.argtypes = []
+    L.qs_serve_close.restype = ctypes.c_int
     for name in ("qs_synth_ball", "qs_synth_shell"):
         fn = getattr(L, name)
         fn.restype = ctypes.c_int
@@ -252,6 +268,80 @@ def batch_grid(m):
     if L.qs_batch_grid(int(m), fp, ip) != 0:
         raise QuickSESError("no molecule %d in the last batch" % m)
     return dict(origin=(fp[0], fp[1], fp[2]), dx=fp[3], dxn=fp[4], g=ip[0], gn=ip[1], r=ip[2], lim=ip[3])
+
+
+class Server:
+    """Request server (qs_serve_*): submit surface requests at any time, poll for device-resident
+    meshes.  Use as a context manager, or call close()."""
+
+    def __init__(self, hbm_budget_bytes=16 << 30, result_bytes=8 << 30, workers=2):
+        L = lib()
+        rc = L.qs_serve_open(int(hbm_budget_bytes), int(result_bytes), int(workers))
+        if rc:
+            raise QuickSESError("quickses error %d: %s" % (rc, L.qs_last_error_message().decode()))
+        self.open = True
+
+    def submit(self, pos, rad, reso=0.5, smooth=1):
+        L = lib()
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        rad = np.ascontiguousarray(rad, dtype=np.float32).reshape(-1)
+        if pos.shape[0] != rad.shape[0]:
+            raise ValueError("pos and rad disagree on the atom count")
+        rid = L.qs_serve_submit(_fptr(pos), _fptr(rad), pos.shape[0], ctypes.c_float(reso), int(smooth))
+        if rid < 0:
+            raise QuickSESError("quickses error %d: %s" % (-rid, L.qs_last_error_message().decode()))
+        return rid
+
+    def poll(self, rid, wait_ms=-1):
+        """None if not finished within wait_ms, else dict(nvert, ntri, d_vert, d_tri, d_atom,
+        latency_ms, queue_ms, batch_size)."""
+        L = lib()
+        nv, nt = ctypes.c_uint(0), ctypes.c_uint(0)
+        pv, pt, pa = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        lat = (ctypes.c_double * 3)()
+        rc = L.qs_serve_poll(int(rid), int(wait_ms), ctypes.byref(nv), ctypes.byref(nt), ctypes.byref(pv),
+                             ctypes.byref(pt), ctypes.byref(pa), lat)
+        if rc == 1:
+            return None
+        if rc:
+            raise QuickSESError("quickses error %d: %s" % (-rc, L.qs_last_error_message().decode()))
+        return dict(nvert=nv.value, ntri=nt.value, d_vert=pv.value, d_tri=pt.value, d_atom=pa.value,
+                    latency_ms=lat[0], queue_ms=lat[1], batch_size=int(lat[2]))
+
+    def fetch(self, rid):
+        """(vertices, triangles, atom_ids) on the host; waits for the request."""
+        L = lib()
+        info = self.poll(rid, -1)
+        V = np.empty((info["nvert"], 3), dtype=np.float32)
+        T = np.empty((info["ntri"], 3), dtype=np.int32)
+        A = np.empty(info["nvert"], dtype=np.int32)
+        rc = L.qs_serve_fetch(int(rid), ctypes.c_void_p(V.ctypes.data), ctypes.c_void_p(T.ctypes.data),
+                              ctypes.c_void_p(A.ctypes.data))
+        _check(rc)
+        return V, T, A
+
+    def release(self, rid):
+        rc = lib().qs_serve_release(int(rid))
+        if rc:
+            raise QuickSESError("quickses error %d: %s" % (-rc, lib().qs_last_error_message().decode()))
+
+    def stats(self):
+        out = (ctypes.c_uint64 * 7)()
+        lib().qs_serve_stats(out, 7)
+        keys = ("batches", "requests_batched", "largest_batch", "max_busy_workers", "arena_used", "arena_bytes",
+                "pending")
+        return {k: int(out[i]) for i, k in enumerate(keys)}
+
+    def close(self):
+        if self.open:
+            lib().qs_serve_close()
+            self.open = False
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def stage_times():

@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["qs_kernels.hip", "qs_engine.hip", "qs_synth.cpp"]
+SOURCES = ["qs_kernels.hip", "qs_engine.hip", "qs_serve.hip", "qs_synth.cpp"]
 
 
 def _run(cmd, cwd=None):

@@ -1,0 +1,288 @@
+/* Host engine of the SES path (internal to libquickses.so): error state, options, device
+ * buffers, grid setup and the batched stage sequence (Engine).  Used by the C ABI
+ * (qs_engine.hip) and the request server (qs_serve.hip). */
+#ifndef QS_ENGINE_H
+#define QS_ENGINE_H
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "qs_common.h"
+#include "qs_kernels.h"
+
+namespace qs {
+
+/* ------------------------------------------------------------------------------------------ */
+/* errors                                                                                       */
+
+/* error state of the calling thread */
+extern thread_local int g_err;
+extern thread_local char g_errmsg[512];
+
+int set_error(int code, const char *fmt, ...);
+
+#define QS_HIP(call)                                                                                     \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return set_error(2, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+#define QS_TRY(expr)          \
+    do {                      \
+        int rc_ = (expr);     \
+        if (rc_) return rc_;  \
+    } while (0)
+
+/* ------------------------------------------------------------------------------------------ */
+/* options                                                                                      */
+
+struct Options {
+    int lim_mode = 0;
+    int atom_ids_sorted = 0;
+    int pass1_direct = 0;
+    int pass2_direct = 0;
+    int trace = 0;
+    int timing = 0;
+};
+extern Options g_opt;
+
+/* ------------------------------------------------------------------------------------------ */
+/* grow-only device buffers                                                                     */
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+/* grow (never shrink) to at least `bytes`; contents are not kept */
+/* headroom of a grown buffer: bytes >> g_grow_shift extra (3 = 1/8; 0 doubles, for threads
+   that must rarely reallocate, since freeing device memory waits for the whole device) */
+extern thread_local int g_grow_shift;
+int ensure(DevBuf &b, size_t bytes);
+void release(DevBuf &b);
+
+template <class T>
+inline T *as(DevBuf &b) { return (T *)b.p; }
+
+/* ------------------------------------------------------------------------------------------ */
+/* grid setup: the reference's host formulas, evaluated in single precision                    */
+
+struct BBox {
+    float vmin[3], vmax[3], max_rad;
+};
+
+void bbox_host(const float *pos, const float *rad, unsigned n, BBox &b, int &nonfinite);
+
+void grid_from_bbox(const BBox &b, float reso, QsGrid &G);
+
+/* ------------------------------------------------------------------------------------------ */
+/* pass-2 candidate tables: window offsets sorted by squared norm, and for each minimal norm the
+   end of the range of norms whose float distance can still be the minimum                      */
+
+struct OffsetTables {
+    int r = -1;
+    float dx = 0, cmax = 0, bound = 0;
+    std::vector<int> offs, norm_start, enum_end;
+};
+
+void build_offset_tables(int r, float dx, float coord_max, OffsetTables &T);
+
+/* ------------------------------------------------------------------------------------------ */
+/* engine                                                                                       */
+
+/* per-stage times of the calling thread's last timed run */
+extern thread_local std::vector<std::pair<std::string, float>> g_stage_ms;
+
+struct StageTimer {
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::vector<std::pair<std::string, hipEvent_t>> ev;
+    void mark(const char *name)
+    {
+        if (!on) return;
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        hipEventRecord(e, s);
+        ev.emplace_back(name, e);
+    }
+    void report(const QsGrid &G, unsigned n)
+    {
+        if (!on || ev.size() < 2) return;
+        hipEventSynchronize(ev.back().second);
+        g_stage_ms.clear();
+        for (size_t i = 1; i < ev.size(); i++) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, ev[i - 1].second, ev[i].second);
+            g_stage_ms.emplace_back(ev[i].first, ms);
+        }
+        if (!g_opt.trace) {
+            for (auto &p : ev) hipEventDestroy(p.second);
+            ev.clear();
+            return;
+        }
+        std::string line = "{\"qs_trace\": {\"n_atoms\": " + std::to_string(n) + ", \"g\": " + std::to_string(G.g);
+        for (size_t i = 1; i < ev.size(); i++) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, ev[i - 1].second, ev[i].second);
+            char buf[96];
+            snprintf(buf, sizeof(buf), ", \"%s_ms\": %.4f", ev[i].first.c_str(), ms);
+            line += buf;
+        }
+        line += "}}";
+        fprintf(stderr, "%s\n", line.c_str());
+        for (auto &p : ev) hipEventDestroy(p.second);
+        ev.clear();
+    }
+};
+
+/* host description of one molecule of a batch */
+struct MolIn {
+    uint32_t atom_off = 0, n = 0;
+    float reso = 0.5f;
+};
+
+/* field-pass-2 candidate tables on the device, kept for reuse by later calls */
+struct P2Tables {
+    int r;
+    float dx, cmax;
+    DevBuf offs, norm, end;
+};
+
+/* molecules sharing a field-pass-2 kernel (path, r) and candidate tables (dx, coordinate range) */
+struct P2Group {
+    int path, r;
+    float dx, cmax;
+    std::vector<int> mols;
+    size_t bpre_at = 0, mid_at = 0;
+    uint32_t nblocks = 0;
+    P2Tables *tab = nullptr;
+};
+
+struct Engine {
+    int device = -1;
+    hipStream_t stream = nullptr;
+
+    DevBuf in_pos, in_rad, d_off, bb_part, bb_out, d_mols, d_meta;
+    DevBuf key, count, start, cursor, perm, cse, sx, aux;
+    DevBuf labels, outm, shellm, field, negm, zerom;
+    DevBuf s4, vcnt, tcnt, vbw, tbw, apos, alist, totals, bounds, bidx;
+    DevBuf vert, vert2, vert3, vkey, cidx, tbase, tri, tri_out, atom;
+    DevBuf keep, keeppos, tri_loc;
+    DevBuf scan_tmp;
+    std::vector<P2Group> groups;
+    std::vector<P2Tables *> tables;
+    /* host staging of uploads that kernels read later in the same call */
+    std::vector<uint32_t> h_meta, h_vpre;
+
+    /* last result */
+    std::vector<QsMol> mols;
+    std::vector<uint32_t> vert_off, tri_off; /* per molecule, nmol + 1; triangles after the filter */
+    uint32_t n_atoms_total = 0;
+    uint64_t ncell_total = 0, nw_total = 0, vox_total = 0;
+    uint32_t nv = 0, nt_all = 0, nt = 0, n_active = 0;
+    bool batch_valid = false; /* result arrays, vert_off / tri_off and their device copies belong to run_batch */
+    float *res_vert = nullptr;
+    int *res_tri = nullptr, *res_atom = nullptr;
+
+    /* frees every device buffer and the stream (the engine can be initialised again) */
+    void release_all()
+    {
+        for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start, &cursor,
+                          &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
+                          &vbw, &tbw, &apos, &alist, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey, &cidx,
+                          &tbase, &tri, &tri_out, &atom, &keep, &keeppos, &tri_loc, &scan_tmp})
+            release(*b);
+        for (P2Tables *t : tables) {
+            release(t->offs);
+            release(t->norm);
+            release(t->end);
+            delete t;
+        }
+        tables.clear();
+        groups.clear();
+        if (stream) hipStreamDestroy(stream);
+        stream = nullptr;
+        device = -1;
+        batch_valid = false;
+    }
+
+    int init()
+    {
+        int dev = 0;
+        QS_HIP(hipGetDevice(&dev));
+        if (device == dev && stream) return 0;
+        device = dev;
+        QS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        return 0;
+    }
+
+    /* exclusive scan of n uint32 (in may equal out); total written to d_total if given */
+    int scan(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *d_total)
+    {
+        if (n == 0) {
+            if (d_total) QS_HIP(hipMemsetAsync(d_total, 0, sizeof(uint32_t), stream));
+            return 0;
+        }
+        std::vector<uint64_t> lv;
+        uint64_t m = n;
+        while (qs_scan_tiles(m) > 1) {
+            m = qs_scan_tiles(m);
+            lv.push_back(m);
+        }
+        size_t need = 0;
+        for (uint64_t x : lv) need += (x + 63) / 64 * 64;
+        QS_TRY(ensure(scan_tmp, std::max<size_t>(need, 64) * sizeof(uint32_t)));
+        std::vector<uint32_t *> lvl_ptr;
+        uint32_t *p = as<uint32_t>(scan_tmp);
+        for (uint64_t x : lv) {
+            lvl_ptr.push_back(p);
+            p += (x + 63) / 64 * 64;
+        }
+        const uint32_t *src = in;
+        uint64_t len = n;
+        for (size_t l = 0; l < lv.size(); l++) {
+            qs_launch_tile_reduce(src, len, lvl_ptr[l], stream);
+            src = lvl_ptr[l];
+            len = lv[l];
+        }
+        for (int l = (int)lv.size() - 1; l >= 0; l--) {
+            const uint32_t *offs = (l + 1 < (int)lv.size()) ? lvl_ptr[l + 1] : nullptr;
+            qs_launch_tile_scan(lvl_ptr[l], lvl_ptr[l], lv[l], offs, nullptr, stream);
+        }
+        qs_launch_tile_scan(in, out, n, lv.empty() ? nullptr : lvl_ptr[0], d_total, stream);
+        QS_HIP(hipGetLastError());
+        return 0;
+    }
+
+    QsBatch batch(size_t bpre_at, size_t mid_at, int n) const
+    {
+        QsBatch B;
+        B.mols = (const QsMol *)d_mols.p;
+        B.bpre = (const uint32_t *)d_meta.p + bpre_at;
+        B.mid = mid_at ? (const int *)((const uint32_t *)d_meta.p + mid_at) : nullptr;
+        B.n = n;
+        return B;
+    }
+
+    int setup(const std::vector<MolIn> &in, const std::vector<BBox> &bb);
+    int build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &at_wpre,
+                   uint32_t &nb_atoms, uint32_t &nb_p1, uint32_t &nb_mc);
+    /* allocate the workspace for batches up to these totals in one go */
+    int reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts, int nmol);
+    int run_batch(const float *d_pos, const float *d_rad, const std::vector<MolIn> &in, int smooth,
+                  const std::vector<BBox> *host_bbox);
+    int field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm);
+    int mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer &tm, bool with_atoms);
+    int group_tables(P2Group &grp);
+};
+
+
+}  // namespace qs
+
+#endif

@@ -36,40 +36,44 @@ float draw_radius(SplitMix64 &rng) {
     return 1.80f;
 }
 
-/* min-distance rejection on a hash grid of `cell`-sized cells */
+/* min-distance rejection on a dense grid of `cell`-sized cells covering [-extent, extent]^3
+   (per-cell linked lists of accepted points) */
 struct RejectGrid {
-    double cell;
-    std::unordered_map<uint64_t, std::vector<int>> bins;
+    double cell, lo;
+    int64_t dim;
+    std::vector<int> head, next;
     std::vector<double> pts;
-    explicit RejectGrid(double c) : cell(c) {}
-    static uint64_t key(int64_t x, int64_t y, int64_t z) {
-        return ((uint64_t)(x & 0x1FFFFF) << 42) | ((uint64_t)(y & 0x1FFFFF) << 21) | (uint64_t)(z & 0x1FFFFF);
+    RejectGrid(double c, double extent) : cell(c), lo(-extent - c)
+    {
+        dim = (int64_t)std::ceil((2.0 * extent + 2.0 * c) / c) + 2;
+        head.assign((size_t)(dim * dim * dim), -1);
     }
+    int64_t idx(int64_t x, int64_t y, int64_t z) const { return (x * dim + y) * dim + z; }
+    int64_t coord(double v) const { return (int64_t)std::floor((v - lo) / cell); }
     bool ok(double x, double y, double z, double dmin) const {
-        int64_t cx = (int64_t)std::floor(x / cell), cy = (int64_t)std::floor(y / cell), cz = (int64_t)std::floor(z / cell);
+        int64_t cx = coord(x), cy = coord(y), cz = coord(z);
         double d2 = dmin * dmin;
         for (int a = -1; a <= 1; a++)
             for (int b = -1; b <= 1; b++)
-                for (int c = -1; c <= 1; c++) {
-                    auto it = bins.find(key(cx + a, cy + b, cz + c));
-                    if (it == bins.end()) continue;
-                    for (int id : it->second) {
+                for (int c = -1; c <= 1; c++)
+                    for (int id = head[(size_t)idx(cx + a, cy + b, cz + c)]; id >= 0; id = next[id]) {
                         double ex = pts[3 * id] - x, ey = pts[3 * id + 1] - y, ez = pts[3 * id + 2] - z;
                         if (ex * ex + ey * ey + ez * ez < d2) return false;
                     }
-                }
         return true;
     }
     void add(double x, double y, double z) {
         int id = (int)(pts.size() / 3);
         pts.push_back(x); pts.push_back(y); pts.push_back(z);
-        bins[key((int64_t)std::floor(x / cell), (int64_t)std::floor(y / cell), (int64_t)std::floor(z / cell))].push_back(id);
+        size_t k = (size_t)idx(coord(x), coord(y), coord(z));
+        next.push_back(head[k]);
+        head[k] = id;
     }
 };
 
 int synth(uint64_t seed, int n, double r_in, double r_out, double dmin, float *pos, float *rad) {
     SplitMix64 rng(seed);
-    RejectGrid grid(dmin > 0 ? dmin : 1.0);
+    RejectGrid grid(dmin > 0 ? dmin : 1.0, dmin > 0 ? r_out : 0.0);
     int have = 0;
     const double ri2 = r_in * r_in, ro2 = r_out * r_out;
     uint64_t tries = 0, max_tries = (uint64_t)n * 1000ULL + 100000ULL;

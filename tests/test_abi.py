@@ -91,3 +91,16 @@ def test_synthetic_shell_and_sizes(qs):
     fr = qs.synth_jitter(5000, base, 0.3)
     dev = (fr - base).astype(np.float64)
     assert abs(dev.std() - 0.3) < 0.02 and abs(dev.mean()) < 0.02
+
+
+def test_server_without_gpu_returns_errors(qs):
+    from quickses_amd import hipmem
+
+    if hipmem.device_count() > 0:
+        pytest.skip("a GPU is present")
+    L = qs.lib()
+    pos, rad = qs.synth_ball(3, 20)
+    assert L.qs_serve_submit(qs._fptr(pos), qs._fptr(rad), 20, 0.5, 1) < 0  # no server open
+    with pytest.raises(qs.QuickSESError):
+        qs.Server(hbm_budget_bytes=1 << 30, result_bytes=1 << 26, workers=1)
+    assert L.qs_serve_close() < 0
