@@ -154,6 +154,20 @@ int qs_serve_release(int64_t id);
 int qs_serve_stats(uint64_t *out, int n);
 int qs_serve_close(void);
 
+/* File formats of the reference's command-line tool (host only, no GPU).
+ * qs_pdb_read: atoms of a PDB file as the reference's reader takes them (cpdb/cpdb.cpp:34-158,
+ *   211-275; radii by element, CudaSurf.h:64-73): "ATOM  " records up to the first "END*"
+ *   record, alternate locations other than 'A' / ' ' dropped, hydrogens kept, file order.
+ *   *pos (3 floats per atom) and *rad are malloc'ed (release with qs_free).  Returns 0, or 2
+ *   if the file cannot be read.
+ * qs_obj_write: the reference's OBJ layout (writeToObj CudaSurf.cu:329-371): "v %.3f %.3f %.3f"
+ *   lines, an empty line, then "f b+1 a+1 c+1" for each triangle (a, b, c).  Returns 0 or 2.
+ * qs_io_error: message of the last failure of either. */
+int qs_pdb_read(const char *path, float **pos, float **rad, unsigned int *n);
+int qs_obj_write(const char *path, const float *vert, unsigned int nvert, const int *tri, unsigned int ntri);
+const char *qs_io_error(void);
+void qs_free(void *p);
+
 /* Seeded synthetic molecules (test and benchmark inputs). */
 int qs_synth_ball(uint64_t seed, int n, double density, double min_dist, float *pos, float *rad);
 int qs_synth_shell(uint64_t seed, int n, double r_in, double r_out, double min_dist, float *pos, float *rad);

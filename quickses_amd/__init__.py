@@ -75,6 +75,13 @@ def lib():
     L.qs_stage_mesh.argtypes = [f3p, ctypes.POINTER(ctypes.c_int), vp, u32p, u32p, vp, ctypes.c_int64, vp,
                                 ctypes.c_int64]
     L.qs_stage_mesh.restype = ctypes.c_int
+    L.qs_pdb_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(f3p), ctypes.POINTER(f3p), u32p]
+    L.qs_pdb_read.restype = ctypes.c_int
+    L.qs_obj_write.argtypes = [ctypes.c_char_p, vp, ctypes.c_uint, vp, ctypes.c_uint]
+    L.qs_obj_write.restype = ctypes.c_int
+    L.qs_io_error.restype = ctypes.c_char_p
+    L.qs_free.argtypes = [vp]
+    L.qs_free.restype = None
     i64 = ctypes.c_int64
     L.qs_serve_open.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]
     L.qs_serve_open.restype = ctypes.c_int
@@ -268,6 +275,38 @@ def batch_grid(m):
     if L.qs_batch_grid(int(m), fp, ip) != 0:
         raise QuickSESError("no molecule %d in the last batch" % m)
     return dict(origin=(fp[0], fp[1], fp[2]), dx=fp[3], dxn=fp[4], g=ip[0], gn=ip[1], r=ip[2], lim=ip[3])
+
+
+CLI_PATH = os.path.join(_HERE, "quickses")
+
+
+def read_pdb(path):
+    """(pos (N, 3) float32, rad (N,) float32) of a PDB file, read as the reference's tool reads it."""
+    L = lib()
+    pp, rp = ctypes.POINTER(ctypes.c_float)(), ctypes.POINTER(ctypes.c_float)()
+    n = ctypes.c_uint(0)
+    rc = L.qs_pdb_read(os.fsencode(path), ctypes.byref(pp), ctypes.byref(rp), ctypes.byref(n))
+    if rc:
+        raise QuickSESError(L.qs_io_error().decode())
+    try:
+        N = n.value
+        pos = np.ctypeslib.as_array(pp, shape=(max(N, 1) * 3,))[: 3 * N].reshape(-1, 3).copy()
+        rad = np.ctypeslib.as_array(rp, shape=(max(N, 1),))[:N].copy()
+    finally:
+        L.qs_free(ctypes.cast(pp, ctypes.c_void_p))
+        L.qs_free(ctypes.cast(rp, ctypes.c_void_p))
+    return pos, rad
+
+
+def write_obj(path, vertices, triangles):
+    """Write a mesh in the reference tool's OBJ layout."""
+    L = lib()
+    V = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+    T = np.ascontiguousarray(triangles, dtype=np.int32).reshape(-1, 3)
+    rc = L.qs_obj_write(os.fsencode(path), ctypes.c_void_p(V.ctypes.data), V.shape[0], ctypes.c_void_p(T.ctypes.data),
+                        T.shape[0])
+    if rc:
+        raise QuickSESError(L.qs_io_error().decode())
 
 
 class Server:

@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["qs_kernels.hip", "qs_engine.hip", "qs_serve.hip", "qs_synth.cpp"]
+SOURCES = ["qs_kernels.hip", "qs_engine.hip", "qs_serve.hip", "qs_io.cpp", "qs_synth.cpp"]
 
 
 def _run(cmd, cwd=None):
@@ -59,7 +59,8 @@ def build_cli():
     if not os.path.exists(src):
         return None
     tmp = CLI + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-o", tmp, src, "-L" + PKG, "-lquickses", "-Wl,-rpath,$ORIGIN"])
+    _run(["g++", "-O2", "-std=c++17", "-o", tmp, src, "-L" + PKG, "-lquickses", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath-link,/opt/rocm/lib"])
     os.replace(tmp, CLI)
     return CLI
 
