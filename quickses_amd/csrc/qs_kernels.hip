@@ -15,10 +15,26 @@
 
 __constant__ uint64_t c_tri[256] = QS_TRI_PACKED_DATA;
 
-/* cube corners and edges (see qs_mc_table.h) */
-__constant__ int8_t c_edge_low[12][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 1},
-                                         {0, 1, 1}, {0, 0, 1}, {0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}};
-__constant__ int8_t c_edge_axis[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+/* cube corners and edges (see qs_mc_table.h): edge e runs from corner offset kEdgeLow[e] along
+   axis kEdgeAxis[e]; packed into bit masks so that per-lane lookups are shifts, not loads */
+constexpr int8_t kEdgeLow[12][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 1},
+                                    {0, 1, 1}, {0, 0, 1}, {0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}};
+constexpr int8_t kEdgeAxis[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+constexpr uint32_t edge_low_bits(int c)
+{
+    uint32_t m = 0;
+    for (int e = 0; e < 12; e++) m |= (uint32_t)(kEdgeLow[e][c] != 0) << e;
+    return m;
+}
+constexpr uint32_t edge_axis_bits()
+{
+    uint32_t m = 0;
+    for (int e = 0; e < 12; e++) m |= (uint32_t)kEdgeAxis[e] << (2 * e);
+    return m;
+}
+constexpr uint32_t kEdgeLX = edge_low_bits(0), kEdgeLY = edge_low_bits(1), kEdgeLZ = edge_low_bits(2);
+constexpr uint32_t kEdgeAX = edge_axis_bits();
+static_assert(kEdgeLX == 0x622u && kEdgeLY == 0xC44u && kEdgeLZ == 0xF0u && kEdgeAX == 0xAA4444u, "edge tables");
 
 /* ------------------------------------------------------------------------------------------ */
 /* small helpers                                                                                */
@@ -49,15 +65,14 @@ __device__ __forceinline__ float dist3(float ax, float ay, float az, float bx, f
 
 __device__ __forceinline__ int tri_entry(uint64_t packed, int k) { return (int)((packed >> (4 * k)) & 0xF); }
 
-/* number of triangles of case ci: the first nibble equal to 0xF, divided by 3 */
-__device__ __forceinline__ int tri_count(int ci)
+/* number of triangles of a packed case: nibble k is 0xF exactly for k >= 3 * ntri (k < 16) */
+__device__ __forceinline__ int tri_count_packed(uint64_t p)
 {
-    uint64_t p = c_tri[ci];
-    /* nibble k is 0xF for k >= 3*ntri; count leading non-0xF nibbles */
-    int k = 0;
-    while (k < 15 && tri_entry(p, k) != 0xF) k += 3;
-    return k / 3;
+    const uint64_t f = p & (p >> 1) & (p >> 2) & (p >> 3) & 0x1111111111111111ull; /* bit 4k: nibble k is 0xF */
+    return (int)(__builtin_ctzll(f | (1ull << 63)) >> 2) / 3;
 }
+
+__device__ __forceinline__ int tri_count(int ci) { return tri_count_packed(c_tri[ci]); }
 
 /* molecule of block b of a batched launch, and the block's index inside the molecule */
 __device__ __forceinline__ int batch_mol(const QsBatch &B, uint32_t b, uint32_t &local)
@@ -1285,8 +1300,8 @@ struct MCView {
     /* vertex of cube edge e of cube (cx, cy, cz) */
     __device__ __forceinline__ uint32_t corner_vid(int cx, int cy, int cz, int e) const
     {
-        int qx = cx + c_edge_low[e][0], qy = cy + c_edge_low[e][1], qz = cz + c_edge_low[e][2];
-        int a = c_edge_axis[e];
+        int qx = cx + (int)((kEdgeLX >> e) & 1u), qy = cy + (int)((kEdgeLY >> e) & 1u), qz = cz + (int)((kEdgeLZ >> e) & 1u);
+        int a = (int)((kEdgeAX >> (2 * e)) & 3u);
         int rx = qx + (a == 0), ry = qy + (a == 1), rz = qz + (a == 2);
         bool qneg = nbit(qx, qy, qz);
         int nx = qneg ? rx : qx, ny = qneg ? ry : qy, nz = qneg ? rz : qz;
@@ -1467,8 +1482,9 @@ struct HoodView {
     }
     __device__ __forceinline__ uint32_t corner_vid(int cx, int cy, int cz, int e) const
     {
-        const int qx = cx + c_edge_low[e][0], qy = cy + c_edge_low[e][1], qz = cz + c_edge_low[e][2];
-        const int a = c_edge_axis[e];
+        const int qx = cx + (int)((kEdgeLX >> e) & 1u), qy = cy + (int)((kEdgeLY >> e) & 1u),
+                  qz = cz + (int)((kEdgeLZ >> e) & 1u);
+        const int a = (int)((kEdgeAX >> (2 * e)) & 3u);
         const bool qneg = nbit(qx, qy, qz);
         const int nx = qneg ? qx + (a == 0) : qx, ny = qneg ? qy + (a == 1) : qy, nz = qneg ? qz + (a == 2) : qz;
         if (zbit(nx, ny, nz)) return slot_id(nx, ny, nz, 3);
@@ -1501,6 +1517,14 @@ __device__ __forceinline__ void load_hood8(Hood &H, const MCView &V, int x, int 
     }
 }
 
+/* per wavefront of k_mc_emit: owner cube (lane) of each of the word's triangles (at most 5 per
+   cube), and each cube's packed case and first triangle within the word */
+struct EmitLds {
+    uint64_t pk[64];
+    uint16_t ex[64];
+    uint8_t owner[64 * 5];
+};
+
 /* vertices (vertexInterp/lerp MarchingCubes.cu:715-734, edge oriented low->high) and triangles
    (generateTriangleVerticesSMEM MarchingCubes.cu:736-851 order); one wavefront per active word.
    Each polygonised cube also records its case and first triangle id (read by the smoothing
@@ -1514,6 +1538,7 @@ __global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols,
                                                  uint8_t *__restrict__ cidx_all, uint32_t *__restrict__ tbase_all)
 {
     __shared__ Hood hoods[4];
+    __shared__ EmitLds elds[4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint32_t k = blockIdx.x * 4 + wid;
     const bool active = k < nact;
@@ -1575,14 +1600,33 @@ __global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols,
     }
     int ci = 0;
     if (x < lim && y < lim && z < lim) ci = hv.cube_index(x, y, z);
-    const int nt = tri_count(ci);
-    const uint32_t incl = wave_incl_scan((uint32_t)nt, lane);
-    const uint32_t t0 = tbw[wi] + incl - (uint32_t)nt;
+    const uint64_t packed = c_tri[ci];
+    const int nt = tri_count_packed(packed);
+    /* exclusive prefix of the (< 8) triangle counts over the wave, by bit planes */
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t ex = 0, ntw = 0;
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        const uint64_t m = __ballot((nt >> b) & 1);
+        ex += (uint32_t)__popcll(m & below) << b;
+        ntw += (uint32_t)__popcll(m) << b;
+    }
+    const uint32_t tw = tbw[wi];
+    EmitLds &L = elds[wid];
     if (nt) {
         cidx[pidx] = (uint8_t)ci;
-        tbase[pidx] = t0;
-        const uint64_t packed = c_tri[ci];
-        for (int e3 = 0; e3 < 3 * nt; e3++) tri[3 * (size_t)t0 + e3] = (int)hv.corner_vid(x, y, z, tri_entry(packed, e3));
+        tbase[pidx] = tw + ex;
+        for (int i = 0; i < nt; i++) L.owner[ex + i] = (uint8_t)lane;
+    }
+    L.pk[lane] = packed;
+    L.ex[lane] = (uint16_t)ex;
+    wave_lds_sync();
+    /* one triangle corner per lane: coalesced stores, no per-cube serial loop */
+    for (uint32_t j = (uint32_t)lane; j < 3 * ntw; j += 64) {
+        const uint32_t tt = j / 3u, cc = j - 3u * tt;
+        const int c = L.owner[tt];
+        const int e = tri_entry(L.pk[c], 3 * (int)(tt - L.ex[c]) + (int)cc);
+        tri[3 * (size_t)(tw + tt) + cc] = (int)hv.corner_vid(x, y, w * 64 + c, e);
     }
 }
 
