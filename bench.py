@@ -25,12 +25,47 @@ WORKLOAD = dict(name="plain single-molecule SES, 50,000 atoms, voxel 0.2 A", see
 
 
 def algorithmic_bytes(stage, V, N, ncell):
-    """Bytes a stage must move at minimum (HBM), per launch."""
+    """Bytes a kernel must move at minimum (HBM) per launch, for the single-kernel stages.
+
+    pass1: writes the outside and shell bit masks (2 bits per voxel), reads the cell-ordered
+           atoms with their thresholds (32 B per atom) and the cell starts (4 B per cell).
+    pass2: reads the two bit masks, writes the f32 field (4 B per voxel) and the sign and zero
+           bit masks (2 bits per voxel)."""
     if stage == "pass1":
-        return V * 1 + N * 16 + ncell * 8  # label store; atoms and cell ranges read once
+        return V // 4 + 32 * N + 4 * ncell
     if stage == "pass2":
-        return V * (1 + 4)  # label load, field store
+        return V * 9 // 2
     return None
+
+
+STAGE_KERNEL = {"pass1": "k_pass1_rows", "pass2": "k_pass2_r"}
+
+
+def profiled_traffic(stage):
+    """HBM bytes per launch of a stage's kernel from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by the same counter on a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_ses_50k_0.2A.txt")
+    if stage not in STAGE_KERNEL or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic ") and ("kernel=%s" % STAGE_KERNEL[stage]) in line:
+                for tok in line.split():
+                    if tok.startswith("bytes_per_launch="):
+                        return int(tok.split("=", 1)[1])
+    return None
+
+
+def copy_bandwidth(qs, hipmem, nbytes=1 << 30, reps=3):
+    """GB/s (read + write) of the library's plain copy kernel over nbytes."""
+    a, b = hipmem.alloc(nbytes), hipmem.alloc(nbytes)
+    try:
+        qs.copy_device(b, a, nbytes)
+        ms = min(qs.copy_device(b, a, nbytes) for _ in range(reps))
+    finally:
+        hipmem.free(a)
+        hipmem.free(b)
+    return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
 def cpu_baseline(pos, rad, voxel, target_s=12.0):
@@ -309,40 +344,52 @@ def main():
     pos, rad = qs.synth_ball(W["seed"], W["atoms"])
     d_pos = hipmem.DeviceArray(pos)
     d_rad = hipmem.DeviceArray(rad)
-    qs.set_option("timing", 1)
 
     def step():
         return qs.compute_device(d_pos.ptr, d_rad.ptr, W["atoms"], W["voxel"], W["smooth"])
 
+    qs.set_option("timing", 0)
     for _ in range(args.warmup):
         step()
     grid = qs.last_grid()
     g, gn = grid["g"], grid["gn"]
     V = g ** 3
-    stage_sum = {}
     hipmem.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         nv, nidx, *_ = step()
-        for k, v in qs.stage_times().items():
-            stage_sum[k] = stage_sum.get(k, 0.0) + v
     hipmem.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
     ms_per_step = 1e3 * elapsed / args.steps
-    stage_avg = {k: v / args.steps for k, v in stage_sum.items()}
+
+    # per-stage times from HIP events on the library's stream, over extra steps
+    qs.set_option("timing", 1)
+    stage_sum = {}
+    n_stage = max(3, min(10, args.steps))
+    for _ in range(n_stage):
+        step()
+        for k, v in qs.stage_times().items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    qs.set_option("timing", 0)
+    stage_avg = {k: v / n_stage for k, v in stage_sum.items()}
 
     # dominant single-kernel stage against the HBM roofline
     kern = {k: v for k, v in stage_avg.items() if algorithmic_bytes(k, V, W["atoms"], gn ** 3) is not None}
     dom = max(kern, key=kern.get)
     abytes = algorithmic_bytes(dom, V, W["atoms"], gn ** 3)
     achieved = abytes / (kern[dom] * 1e-3) / 1e9
+    copy_gbs = copy_bandwidth(qs, hipmem)
+    fp_bytes = sum(algorithmic_bytes(k, V, W["atoms"], gn ** 3) for k in ("pass1", "pass2"))
+    fp_ms = stage_avg["pass1"] + stage_avg["pass2"]
     roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, avg_ms=round(kern[dom], 4),
-                    algorithmic_bytes=abytes)
+                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=profiled_traffic(dom), avg_ms=round(kern[dom], 4),
+                    algorithmic_bytes=abytes, copy_kernel_GBps=round(copy_gbs, 1),
+                    frac_of_copy=round(achieved / copy_gbs, 5),
+                    field_pass=dict(kernels="pass1 + pass2", algorithmic_bytes=fp_bytes, ms=round(fp_ms, 4),
+                                    achieved=round(fp_bytes / (fp_ms * 1e-3) / 1e9, 2)))
 
     # host-in / host-out rate (PCIe included) of the reference-compatible entry point
-    qs.set_option("timing", 0)
     t2 = time.perf_counter()
     api_steps = max(1, min(5, args.steps))
     for _ in range(api_steps):

@@ -104,6 +104,11 @@ int qs_batch_offsets(unsigned int *vert_off, unsigned int *tri_off);
  * (3 * tri_off[nmol] ints, vertex indices local to each molecule), atom ids (vert_off[nmol]). */
 int qs_batch_fetch(float *vert, int *tri, int *atom_id);
 
+/* Device-to-device copy of `bytes` (multiple of 16, 16-byte aligned device pointers) by a
+ * plain vector-copy kernel on the library's stream; *ms = its time.  A measurement reference:
+ * hardware traffic counters of this kernel calibrate those of the SES kernels. */
+int qs_copy_device(void *dst, const void *src, uint64_t bytes, float *ms);
+
 /* Per-stage GPU times of the last computation (needs option "timing" or "trace"): stage names
  * comma-separated into names, milliseconds into ms; returns the number of stages. */
 int qs_stage_times(char *names, int names_cap, float *ms, int max_stages);

@@ -63,6 +63,8 @@ def lib():
     L.qs_batch_fetch.restype = ctypes.c_int
     L.qs_batch_offsets.argtypes = [u32p, u32p]
     L.qs_batch_offsets.restype = ctypes.c_int
+    L.qs_copy_device.argtypes = [vp, vp, ctypes.c_uint64, f3p]
+    L.qs_copy_device.restype = ctypes.c_int
     L.qs_last_grid.argtypes = [f3p, ctypes.POINTER(ctypes.c_int)]
     L.qs_batch_grid.argtypes = [ctypes.c_int, f3p, ctypes.POINTER(ctypes.c_int)]
     L.qs_batch_grid.restype = ctypes.c_int
@@ -381,6 +383,15 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def copy_device(d_dst, d_src, nbytes):
+    """Plain device copy kernel (traffic-counter reference); returns its time in ms."""
+    L = lib()
+    ms = ctypes.c_float(0)
+    rc = L.qs_copy_device(ctypes.c_void_p(d_dst), ctypes.c_void_p(d_src), int(nbytes), ctypes.byref(ms))
+    _check(rc)
+    return ms.value
 
 
 def stage_times():

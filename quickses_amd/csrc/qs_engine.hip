@@ -881,6 +881,27 @@ int qs_batch_mesh(int m, float *vert, int *tri, int *atom_id)
     return 0;
 }
 
+int qs_copy_device(void *dst, const void *src, uint64_t bytes, float *ms)
+{
+    g_err = 0;
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) return set_error(3, "copy: 16-byte sizes and addresses");
+    QS_TRY(g_eng.init());
+    hipEvent_t a, b;
+    QS_HIP(hipEventCreate(&a));
+    QS_HIP(hipEventCreate(&b));
+    QS_HIP(hipEventRecord(a, g_eng.stream));
+    qs_launch_copy(src, dst, bytes, g_eng.stream);
+    QS_HIP(hipGetLastError());
+    QS_HIP(hipEventRecord(b, g_eng.stream));
+    QS_HIP(hipEventSynchronize(b));
+    float t = 0.0f;
+    QS_HIP(hipEventElapsedTime(&t, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (ms) *ms = t;
+    return 0;
+}
+
 int qs_stage_times(char *names, int names_cap, float *ms, int max_stages)
 {
     int k = 0;

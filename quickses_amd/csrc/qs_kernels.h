@@ -67,6 +67,7 @@ uint64_t qs_scan_tiles(uint64_t n);
 void qs_launch_tile_reduce(const uint32_t *in, uint64_t n, uint32_t *sums, hipStream_t s);
 void qs_launch_tile_scan(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *offsets, uint32_t *total,
                          hipStream_t s);
+void qs_launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
 void qs_launch_gather_u32(const uint32_t *src, const uint64_t *idx, int n, uint32_t *out, hipStream_t s);
 
 #endif

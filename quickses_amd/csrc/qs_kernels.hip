@@ -1811,6 +1811,12 @@ __global__ void __launch_bounds__(256) k_tri_compact(const int *__restrict__ tri
     }
 }
 
+/* plain device copy (16-byte vectors, grid-stride): the reference point for traffic counters */
+__global__ void __launch_bounds__(256) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n16)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 /* triangles of a batch with vertex indices local to their molecule */
 __global__ void __launch_bounds__(256) k_tri_localize(const int *__restrict__ tri, uint32_t nt, const uint32_t *__restrict__ tpre,
                                                       const uint32_t *__restrict__ vpre, int nmol, int *__restrict__ out)
@@ -1990,6 +1996,14 @@ void qs_launch_tri_keep(const int *tri, uint32_t nt, uint32_t *keep, hipStream_t
 {
     if (!nt) return;
     hipLaunchKernelGGL(k_tri_keep, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, keep);
+}
+
+void qs_launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s)
+{
+    const uint64_t n16 = bytes / 16;
+    if (!n16) return;
+    const uint64_t blocks = std::min<uint64_t>((n16 + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst, n16);
 }
 
 void qs_launch_tri_localize(const int *tri, uint32_t nt, const uint32_t *tpre, const uint32_t *vpre, int nmol, int *out,

@@ -68,6 +68,18 @@ class DeviceArray:
             pass
 
 
+def alloc(nbytes):
+    """Uninitialised device allocation; returns its address (free with free())."""
+    p = ctypes.c_void_p()
+    _ck(hip().hipMalloc(ctypes.byref(p), max(int(nbytes), 1)), "hipMalloc")
+    return p.value
+
+
+def free(ptr):
+    if ptr:
+        hip().hipFree(ctypes.c_void_p(ptr))
+
+
 def download(ptr, shape, dtype):
     out = np.empty(shape, dtype=dtype)
     if out.nbytes:
