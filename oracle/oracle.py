@@ -59,6 +59,8 @@ def lib():
         _lib.or_compute_ses.argtypes = [vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.POINTER(_Opt),
                                         ctypes.POINTER(_Mesh)]
         _lib.or_field_fp64.argtypes = [ctypes.POINTER(_Grid), vp, vp, ctypes.c_int, vp, vp]
+        _lib.or_mc_vertices_fp64.argtypes = [ctypes.POINTER(_Grid), ctypes.c_int, vp, vp, vp, vp]
+        _lib.or_mc_vertices_fp64.restype = ctypes.c_int64
         _lib.or_mesh_free.argtypes = [ctypes.POINTER(_Mesh)]
         _lib.or_num_threads.restype = ctypes.c_int
         _lib.or_bench_slab.argtypes = [vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -215,6 +217,17 @@ def field_fp64(G, pos, rad):
     f = np.empty(G.g ** 3, np.float64)
     lib().or_field_fp64(ctypes.byref(G._s), _p(pos), _p(rad), len(rad), _p(lab), _p(f))
     return lab, f
+
+
+def vertices_fp64(G, lim, fld, f64, nvert):
+    """(FP64 positions, grid point) of the vertices of edge_weld(G, lim, fld), placed on their
+    edges by the FP64 field f64."""
+    out = np.empty((nvert, 3), np.float64)
+    point = np.empty(nvert, np.int64)
+    n = lib().or_mc_vertices_fp64(ctypes.byref(G._s), lim, _p(np.ascontiguousarray(fld, np.float32)),
+                                  _p(np.ascontiguousarray(f64, np.float64)), _p(out), _p(point))
+    assert n == nvert
+    return out, point
 
 
 def num_threads():

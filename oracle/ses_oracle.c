@@ -643,6 +643,45 @@ int or_compute_ses(const float *pos, const float *rad, int n, float reso, int sm
     return 0;
 }
 
+/* The vertices of or_mc_edge_weld(G, lim, F), in the same order, placed on the same edges but
+   interpolated in double precision from a second field F64 (the FP64 field): the FP32 mesh's
+   topology with FP64 positions.  Where F64 does not change sign along the edge (a tie at a
+   grid point), the parameter is clamped to the edge.  point[v] (if given) = the grid point
+   the vertex belongs to.  Returns the number of vertices. */
+int64_t or_mc_vertices_fp64(const or_grid *G, int lim, const float *F, const double *F64, double *out, int64_t *point)
+{
+    const int g = G->g;
+    const double dx = (double)G->reso;
+    int64_t v = 0;
+    for (int i = 0; i < g; i++)
+        for (int j = 0; j < g; j++)
+            for (int k = 0; k < g; k++) {
+                unsigned mk = slot_mask(F, g, lim, i, j, k);
+                if (!mk) continue;
+                const int64_t p = ((int64_t)i * g + j) * g + k;
+                const double A[3] = {(double)G->origin[0] + i * dx, (double)G->origin[1] + j * dx,
+                                     (double)G->origin[2] + k * dx};
+                const double f0 = F64[p];
+                for (int a = 0; a < 3; a++) {
+                    if (!(mk & (1u << a))) continue;
+                    int q[3] = {i, j, k};
+                    q[a]++;
+                    const double f1 = F64[((int64_t)q[0] * g + q[1]) * g + q[2]];
+                    double t = (f1 != f0) ? (0.0 - f0) / (f1 - f0) : 0.0;
+                    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+                    for (int c = 0; c < 3; c++) out[3 * v + c] = A[c] + (c == a ? t * dx : 0.0);
+                    if (point) point[v] = p;
+                    v++;
+                }
+                if (mk & 8) {
+                    for (int c = 0; c < 3; c++) out[3 * v + c] = A[c];
+                    if (point) point[v] = p;
+                    v++;
+                }
+            }
+    return v;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* FP64: the same mathematics with positions, radii and distances in double precision.          */
 void or_field_fp64(const or_grid *G, const float *pos, const float *rad, int n, uint8_t *lab,

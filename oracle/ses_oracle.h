@@ -96,6 +96,11 @@ void or_mesh_free(or_mesh *m);
    window [0,lim)^3, or a zero-valued grid point (snap). Vertices in key order, triangles in
    cube order / table order. */
 int or_mc_edge_weld(const or_grid *G, int lim, const float *field, or_mesh *m);
+/* vertices of or_mc_edge_weld(G, lim, field) (same order, same edges) interpolated in double
+   from field64 (parameter clamped to the edge); returns their count (out holds 3 doubles per
+   vertex; point, if not NULL, the grid point of each vertex) */
+int64_t or_mc_vertices_fp64(const or_grid *G, int lim, const float *field, const double *field64, double *out,
+                            int64_t *point);
 
 /* Reference-form soup (one vertex per triangle corner, per-cube interpolation) and the
    reference's quantised weld: vertices sorted by quantised position. */
