@@ -24,6 +24,12 @@ struct QsGrid {
     int lim;          /* cubes (i,j,k) with i,j,k < lim are polygonised */
 };
 
+/* Field pass 2: bit n is set when a window minimum of squared integer norm n can only be
+   attained by offsets of norm n (no longer offset is within rounding of it), n < 256. */
+struct QsNormFlags {
+    uint64_t w[4];
+};
+
 /* One molecule of a batch: its grid and where its rows live in the concatenated arrays. */
 struct QsMol {
     QsGrid G;

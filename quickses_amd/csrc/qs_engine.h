@@ -152,6 +152,7 @@ struct P2Tables {
     int r;
     float dx, cmax;
     DevBuf offs, norm, end;
+    QsNormFlags flags;
 };
 
 /* molecules sharing a field-pass-2 kernel (path, r) and candidate tables (dx, coordinate range) */

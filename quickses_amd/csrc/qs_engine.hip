@@ -312,7 +312,9 @@ int Engine::group_tables(P2Group &grp)
         }
     OffsetTables T;
     build_offset_tables(grp.r, grp.dx, grp.cmax, T);
-    P2Tables *t = new P2Tables{grp.r, grp.dx, grp.cmax, DevBuf{}, DevBuf{}, DevBuf{}};
+    P2Tables *t = new P2Tables{grp.r, grp.dx, grp.cmax, DevBuf{}, DevBuf{}, DevBuf{}, QsNormFlags{{0, 0, 0, 0}}};
+    for (int n = 0; n < 256 && n + 1 < (int)T.norm_start.size(); n++)
+        if (T.enum_end[n] == T.norm_start[n + 1]) t->flags.w[n >> 6] |= 1ull << (n & 63);
     tables.push_back(t);
     QS_TRY(ensure(t->offs, T.offs.size() * 4));
     QS_TRY(ensure(t->norm, T.norm_start.size() * 4));
@@ -348,7 +350,7 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
         QS_TRY(group_tables(grp));
         qs_launch_pass2(batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path, as<uint64_t>(outm),
                         as<uint64_t>(shellm), as<float>(field), as<uint32_t>(negm), as<uint32_t>(zerom), as<int>(grp.tab->offs),
-                        as<int>(grp.tab->norm), as<int>(grp.tab->end), stream);
+                        as<int>(grp.tab->norm), as<int>(grp.tab->end), grp.tab->flags, stream);
         QS_HIP(hipGetLastError());
     }
     tm.mark("pass2");
@@ -1021,7 +1023,7 @@ int qs_stage_pass2(const float *fp, const int *ip, const uint8_t *labels, float 
         qs_launch_pass2(E.batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path,
                         as<uint64_t>(E.outm), as<uint64_t>(E.shellm), as<float>(E.field), as<uint32_t>(E.negm),
                         as<uint32_t>(E.zerom), as<int>(grp.tab->offs), as<int>(grp.tab->norm), as<int>(grp.tab->end),
-                        E.stream);
+                        grp.tab->flags, E.stream);
         QS_HIP(hipGetLastError());
     }
     QS_HIP(hipStreamSynchronize(E.stream));

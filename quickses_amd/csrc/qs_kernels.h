@@ -32,7 +32,7 @@ int qs_pass2_path(int r, bool direct);
 uint32_t qs_pass2_blocks(QsGrid G, int path);
 void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm, float *field,
                      uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
-                     hipStream_t s);
+                     QsNormFlags nf, hipStream_t s);
 
 /* stage-wise helpers (one molecule) */
 void qs_launch_field_signs(const float *F, int g, int W, uint32_t *negm, uint32_t *zerom, hipStream_t s);

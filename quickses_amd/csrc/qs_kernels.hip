@@ -959,7 +959,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
                                                  const uint64_t *__restrict__ shellm_all, float *__restrict__ field_all,
                                                  uint32_t *__restrict__ negm_all, uint32_t *__restrict__ zerom_all,
                                                  const int *__restrict__ offs, const int *__restrict__ norm_start,
-                                                 const int *__restrict__ enum_end)
+                                                 const int *__restrict__ enum_end, QsNormFlags nf)
 {
     constexpr int LX = P2_TX + 2 * R, LY = P2_TY + 2 * R, LZ = P2_TZ + 2 * R;
     constexpr int INF = 255;
@@ -1079,12 +1079,17 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
             if (nmin < INF) {
                 const float vx = grid_pos(G.ox, gx, dx);
                 float min_dist = 100000.0f;
-                if (enum_end[nmin] == norm_start[nmin + 1]) {
-                    /* trace the minimisers back through the separable passes */
-#pragma unroll 1
-                    for (int a = -R; a <= R; a++) {
-                        const int dy = DY[((tx + R + a) * P2_TY + ty) * P2_TZ + tz];
-                        if (dy + a * a != nmin) continue;
+                const uint64_t fw = nmin < 64 ? nf.w[0] : (nmin < 128 ? nf.w[1] : (nmin < 192 ? nf.w[2] : nf.w[3]));
+                if ((fw >> (nmin & 63)) & 1ull) {
+                    /* trace the minimisers back through the separable passes: the x offsets
+                       attaining nmin from the registers, then their y offsets */
+                    uint32_t amask = 0;
+#pragma unroll
+                    for (int a = 0; a <= 2 * R; a++) amask |= (uint32_t)(u[tx + a] + (a - R) * (a - R) == nmin) << a;
+                    while (amask) {
+                        const int a = __builtin_ctz(amask) - R;
+                        amask &= amask - 1u;
+                        const int dy = nmin - a * a;
 #pragma unroll 1
                         for (int b = -R; b <= R; b++) {
                             const int dz = DZ[((tx + R + a) * LY + (ty + R + b)) * P2_TZ + tz];
@@ -1910,13 +1915,13 @@ uint32_t qs_pass2_blocks(QsGrid G, int path)
 /* one launch for a group of molecules sharing r (and so the path) */
 void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm, float *field,
                      uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
-                     hipStream_t s)
+                     QsNormFlags nf, hipStream_t s)
 {
     if (path == 0) {
 #define QS_P2R(RR)                                                                                                    \
     case RR:                                                                                                          \
         hipLaunchKernelGGL(k_pass2_r<RR>, dim3(nblocks), dim3(256), 0, s, B, outm, shellm, field, negm, zerom, offs, \
-                           norm_start, enum_end);                                                                     \
+                           norm_start, enum_end, nf);                                                                 \
         return;
         switch (r) {
             QS_P2R(1)
