@@ -452,7 +452,8 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsBatch B, const uint32_t *_
                                                     uint64_t *__restrict__ outm_all, uint64_t *__restrict__ shellm_all)
 {
     __shared__ int queue_all[4][P1_QCAP];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    /* wave-uniform values go through readfirstlane so that they live in scalar registers */
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int *queue = queue_all[wid];
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
@@ -496,7 +497,7 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsBatch B, const uint32_t *_
     /* this lane's voxel z cell; the wave's cells are monotone in the lane */
     const int zl = min(z0 + lane, g - 1);
     const int cz = cell_of(grid_pos(G.oz, zl, dx), G.oz, G.dxn);
-    const int czmin = __shfl(cz, 0, 64), czmax = __shfl(cz, 63, 64);
+    const int czmin = __builtin_amdgcn_readfirstlane(cz), czmax = __builtin_amdgcn_readlane(cz, 63);
     const int D = czmax - czmin + 1; /* distinct z cells + 1 */
     /* tab (lane j) = number of lanes whose z cell is < czmin + j, j = 0..D */
     int tab = (lane >= D) ? 64 : 0;
