@@ -52,6 +52,17 @@ __device__ __forceinline__ float grid_pos(float o, int i, float dx)
 __device__ __forceinline__ int cell_of(float p, float o, float dx) { return (int)((p - o) / dx); }
 
 /* fast_distance Kernels.cu:149-155, left to right, IEEE sqrt */
+/* the squared sum under dist3's root, with dist3's roundings */
+__device__ __forceinline__ float dist3_sq(float ax, float ay, float az, float bx, float by, float bz)
+{
+    float ex = ax - bx, ey = ay - by, ez = az - bz;
+    float x = ex * ex;
+    float y = ey * ey;
+    float z = ez * ez;
+    float s = x + y;
+    return s + z;
+}
+
 __device__ __forceinline__ float dist3(float ax, float ay, float az, float bx, float by, float bz)
 {
     float ex = ax - bx, ey = ay - by, ez = az - bz;
@@ -1545,10 +1556,11 @@ __global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols,
 {
     __shared__ Hood hoods[4];
     __shared__ EmitLds elds[4];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    /* everything up to the lane's z is wave-uniform: scalar registers */
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t k = blockIdx.x * 4 + wid;
     const bool active = k < nact;
-    const uint32_t wi = active ? list[k] : 0u;
+    const uint32_t wi = __builtin_amdgcn_readfirstlane(active ? list[k] : 0u);
     const QsMol &M = mols[seg_of64(wpre, nmol, wi)];
     const QsGrid G = M.G;
     const int g = G.g, W = (int)M.W, lim = G.lim;
@@ -1565,9 +1577,9 @@ __global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols,
     uint32_t *__restrict__ tbase = tbase_all + M.vox_off;
     int x = 0, y = 0, w = 0;
     decode_word((uint32_t)(wi - M.word_off), g, W, x, y, w);
-    if (active) load_hood8(hoods[wid], V, x, y, w, lane);
-    __syncthreads();
-    if (!active) return;
+    if (!active) return; /* whole wavefront; each wave has its own hood, so no block barrier */
+    load_hood8(hoods[wid], V, x, y, w, lane);
+    wave_lds_sync();
     const HoodView hv{&hoods[wid], x, y, w};
     const int z = w * 64 + lane;
     const float dx = G.dx;
@@ -1652,8 +1664,11 @@ __global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ 
     float px = vert[3 * (size_t)v], py = vert[3 * (size_t)v + 1], pz = vert[3 * (size_t)v + 2];
     int cx = cell_of(px, G.ox, G.dxn), cy = cell_of(py, G.oy, G.dxn), cz = cell_of(pz, G.oz, G.dxn);
     const int gn = G.gn;
+    /* the reference keeps the first atom, in traversal order, whose rounded distance is strictly
+       below the best so far.  sqrtf is monotonic, so an atom can only win if its squared sum is
+       below the best one's; the root is taken only then and compared as the reference does. */
     int best = -1;
-    float min_d = 999999.0f;
+    float min_d = 999999.0f, min_s = INFINITY;
     for (int a = -1; a <= 1; a++) {
         int ix = iclamp(cx + a, 0, gn - 1);
         for (int b = -1; b <= 1; b++) {
@@ -1664,10 +1679,14 @@ __global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ 
                 if (se.x >= QS_EMPTY) continue;
                 for (int id = se.x; id < se.y; id++) {
                     float4 at = sx[id];
-                    float d = dist3(at.x, at.y, at.z, px, py, pz);
-                    if (d < min_d) {
-                        min_d = d;
-                        best = id;
+                    const float sq = dist3_sq(at.x, at.y, at.z, px, py, pz);
+                    if (sq < min_s) {
+                        const float d = sqrtf(sq);
+                        if (d < min_d) {
+                            min_d = d;
+                            min_s = sq;
+                            best = id;
+                        }
                     }
                 }
             }
