@@ -11,6 +11,8 @@
 
 #include "qs_common.h"
 #include "qs_kernels.h"
+#include <type_traits>
+
 #include "qs_mc_table.h"
 
 __constant__ uint64_t c_tri[256] = QS_TRI_PACKED_DATA;
@@ -966,6 +968,26 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *_
 
 /* Field pass 2 for a fixed window half-width R: loops unrolled, y and x passes register-blocked
    (each thread keeps one column of the previous pass in registers). */
+/* squared distance from bit c of row m to its nearest set bit within R, or INF */
+template <int R, int INF>
+__device__ __forceinline__ int zdist2(uint64_t m, int c)
+{
+    int best = INF;
+    if (m) {
+        const uint64_t up = m >> c;
+        if (up) {
+            const int d = __builtin_ctzll(up);
+            if (d <= R) best = d * d;
+        }
+        const uint64_t dn = m & ((2ull << c) - 1ull);
+        if (dn) {
+            const int d = c - (63 - __builtin_clzll(dn));
+            if (d <= R) best = min(best, d * d);
+        }
+    }
+    return best;
+}
+
 template <int R>
 __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__restrict__ outm_all,
                                                  const uint64_t *__restrict__ shellm_all, float *__restrict__ field_all,
@@ -978,8 +1000,10 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     static_assert(LZ <= 64 && 3 * R * R < INF, "window too wide for this kernel");
     __shared__ uint64_t rows[LX * LY];
     __shared__ uint32_t own[P2_TX * P2_TY];
-    __shared__ uint8_t DZ[LX * LY * P2_TZ];
     __shared__ uint8_t DY[LX * P2_TY * P2_TZ];
+    /* y offsets (bit b + R) attaining each DY value */
+    using BMask = typename std::conditional<(2 * R + 1 <= 16), uint16_t, uint32_t>::type;
+    __shared__ BMask BM[LX * P2_TY * P2_TZ];
     __shared__ int s_flags;
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
@@ -1034,40 +1058,23 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         return;
     }
 
-    /* z: squared distance to the nearest outside voxel within |c| <= R along each row */
-    for (int w = threadIdx.x; w < LX * LY * P2_TZ; w += 256) {
-        int row = w >> 5, z = w & 31;
-        uint64_t m = rows[row];
-        int best = INF;
-        if (m) {
-            int c = z + R;
-            uint64_t up = m >> c;
-            if (up) {
-                int d = __builtin_ctzll(up);
-                if (d <= R) best = d * d;
-            }
-            uint64_t dn = m & ((2ull << c) - 1ull);
-            if (dn) {
-                int d = c - (63 - __builtin_clzll(dn));
-                if (d <= R) best = min(best, d * d);
-            }
-        }
-        DZ[w] = (uint8_t)best;
-    }
-    __syncthreads();
-
-    /* y, one (lx, z) column per thread */
+    /* z then y, one (lx, z) column per thread: the squared z distance of each row to its nearest
+       outside voxel within R, straight from the row bits, then the y minima over 2R+1 rows */
     for (int col = threadIdx.x; col < LX * P2_TZ; col += 256) {
         const int lx = col >> 5, z = col & 31;
         int v[LY];
 #pragma unroll
-        for (int ly = 0; ly < LY; ly++) v[ly] = DZ[(lx * LY + ly) * P2_TZ + z];
+        for (int ly = 0; ly < LY; ly++) v[ly] = zdist2<R, INF>(rows[lx * LY + ly], z + R);
 #pragma unroll
         for (int yy = 0; yy < P2_TY; yy++) {
             int best = INF;
 #pragma unroll
             for (int b = 0; b <= 2 * R; b++) best = min(best, v[yy + b] + (b - R) * (b - R));
+            uint32_t bm = 0;
+#pragma unroll
+            for (int b = 0; b <= 2 * R; b++) bm |= (uint32_t)(v[yy + b] + (b - R) * (b - R) == best) << b;
             DY[(lx * P2_TY + yy) * P2_TZ + z] = (uint8_t)best;
+            BM[(lx * P2_TY + yy) * P2_TZ + z] = (BMask)bm;
         }
     }
     __syncthreads();
@@ -1102,10 +1109,11 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
                         const int a = __builtin_ctz(amask) - R;
                         amask &= amask - 1u;
                         const int dy = nmin - a * a;
-#pragma unroll 1
-                        for (int b = -R; b <= R; b++) {
-                            const int dz = DZ[((tx + R + a) * LY + (ty + R + b)) * P2_TZ + tz];
-                            if (dz == INF || dz + b * b != dy) continue;
+                        uint32_t bmask = BM[((tx + R + a) * P2_TY + ty) * P2_TZ + tz];
+                        while (bmask) {
+                            const int b = __builtin_ctz(bmask) - R;
+                            bmask &= bmask - 1u;
+                            const int dz = dy - b * b;
                             const int c = (int)sqrtf((float)dz);
                             const uint64_t m = rows[(tx + R + a) * LY + (ty + R + b)];
                             const float qx = grid_pos(G.ox, gx + a, dx), qy = grid_pos(G.oy, gy + b, dx);
