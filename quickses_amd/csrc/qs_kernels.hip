@@ -760,6 +760,7 @@ __global__ void __launch_bounds__(256) k_pass1_direct(QsBatch B, const int2 *__r
 #define P2_TX 8
 #define P2_TY 8
 #define P2_TZ 32
+#define P2R_TX 16 /* x extent of the fixed-radius kernel's tiles (halo cost (16 + 2R) / 16 in x) */
 
 /* bits [s, s + len) of a row of W mask words (bit i <-> z = s + i), zero outside [0, 64 W) */
 __device__ __forceinline__ uint64_t row_bits(const uint64_t *__restrict__ rp, int W, int s, int len)
@@ -995,11 +996,11 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
                                                  const int *__restrict__ offs, const int *__restrict__ norm_start,
                                                  const int *__restrict__ enum_end, QsNormFlags nf)
 {
-    constexpr int LX = P2_TX + 2 * R, LY = P2_TY + 2 * R, LZ = P2_TZ + 2 * R;
+    constexpr int LX = P2R_TX + 2 * R, LY = P2_TY + 2 * R, LZ = P2_TZ + 2 * R;
     constexpr int INF = 255;
     static_assert(LZ <= 64 && 3 * R * R < INF, "window too wide for this kernel");
     __shared__ uint64_t rows[LX * LY];
-    __shared__ uint32_t own[P2_TX * P2_TY];
+    __shared__ uint32_t own[P2R_TX * P2_TY];
     __shared__ uint8_t DY[LX * P2_TY * P2_TZ];
     /* y offsets (bit b + R) attaining each DY value */
     using BMask = typename std::conditional<(2 * R + 1 <= 16), uint16_t, uint32_t>::type;
@@ -1016,7 +1017,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     uint32_t *__restrict__ zerom = zerom_all + 2 * M.word_off;
     const int x1 = g;
     const uint32_t nbx = (g + P2_TZ - 1) / P2_TZ, nby = (g + P2_TY - 1) / P2_TY;
-    const int tz0 = (int)(lb % nbx) * P2_TZ, ty0 = (int)((lb / nbx) % nby) * P2_TY, tx0 = (int)(lb / (nbx * nby)) * P2_TX;
+    const int tz0 = (int)(lb % nbx) * P2_TZ, ty0 = (int)((lb / nbx) % nby) * P2_TY, tx0 = (int)(lb / (nbx * nby)) * P2R_TX;
     if (threadIdx.x == 0) s_flags = 0;
     __syncthreads();
 
@@ -1029,7 +1030,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         rows[row] = m;
         if (m) fl |= 2;
     }
-    if (threadIdx.x < P2_TX * P2_TY) {
+    if (threadIdx.x < P2R_TX * P2_TY) {
         int tx = threadIdx.x / P2_TY, ty = threadIdx.x - tx * P2_TY;
         int gx = tx0 + tx, gy = ty0 + ty;
         uint32_t m = 0u;
@@ -1047,7 +1048,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     const float dx = G.dx;
     if (flags != 3) {
         /* no shell voxel, or no outside voxel within reach: outside -> probe, else -dx */
-        for (int tx = 0; tx < P2_TX; tx++) {
+        for (int tx = 0; tx < P2R_TX; tx++) {
             int gx = tx0 + tx;
             if (gx >= x1 || gx >= g) break;
             bool out = (rows[(tx + R) * LY + (ty + R)] >> (tz + R)) & 1ull;
@@ -1085,7 +1086,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     for (int lx = 0; lx < LX; lx++) u[lx] = DY[(lx * P2_TY + ty) * P2_TZ + tz];
     const float vy = grid_pos(G.oy, gy, dx), vz = grid_pos(G.oz, gz, dx);
 #pragma unroll
-    for (int tx = 0; tx < P2_TX; tx++) {
+    for (int tx = 0; tx < P2R_TX; tx++) {
         const int gx = tx0 + tx;
         if (gx >= x1 || gx >= g) break;
         const bool out = (rows[(tx + R) * LY + (ty + R)] >> (tz + R)) & 1ull;
@@ -1937,7 +1938,7 @@ int qs_pass2_path(int r, bool direct)
 uint32_t qs_pass2_blocks(QsGrid G, int path)
 {
     if (path == 2) return nblk(G.g, 64) * nblk(G.g, 4) * (uint32_t)G.g;
-    return nblk(G.g, P2_TZ) * nblk(G.g, P2_TY) * nblk(G.g, P2_TX);
+    return nblk(G.g, P2_TZ) * nblk(G.g, P2_TY) * nblk(G.g, path == 0 ? P2R_TX : P2_TX);
 }
 
 /* one launch for a group of molecules sharing r (and so the path) */
