@@ -1555,44 +1555,47 @@ struct EmitLds {
    (generateTriangleVerticesSMEM MarchingCubes.cu:736-851 order); one wavefront per active word.
    Each polygonised cube also records its case and first triangle id (read by the smoothing
    pass), and each vertex its (point, slot) key. */
-__global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre, int nmol,
-                                                 const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
-                                                 const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
-                                                 const float *__restrict__ F_all, const uint32_t *__restrict__ list,
-                                                 uint32_t nact, const uint32_t *__restrict__ tbw, float *__restrict__ vert,
-                                                 uint64_t *__restrict__ vkey, int *__restrict__ tri,
-                                                 uint8_t *__restrict__ cidx_all, uint32_t *__restrict__ tbase_all)
-{
-    __shared__ Hood hoods[4];
-    __shared__ EmitLds elds[4];
-    /* everything up to the lane's z is wave-uniform: scalar registers */
-    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t k = blockIdx.x * 4 + wid;
-    const bool active = k < nact;
-    const uint32_t wi = __builtin_amdgcn_readfirstlane(active ? list[k] : 0u);
-    const QsMol &M = mols[seg_of64(wpre, nmol, wi)];
-    const QsGrid G = M.G;
-    const int g = G.g, W = (int)M.W, lim = G.lim;
+/* one active word's vertices and triangles; its hood is staged in LDS */
+struct EmitWord {
     MCView V;
-    V.N = N_all + M.word_off;
-    V.Zm = Zm_all + M.word_off;
-    V.S4 = S4_all + 4 * M.word_off;
-    V.vbw = vbw_all + M.word_off;
-    V.g = g;
-    V.W = W;
-    V.lim = lim;
-    const float *__restrict__ F = F_all + M.vox_off;
-    uint8_t *__restrict__ cidx = cidx_all + M.vox_off;
-    uint32_t *__restrict__ tbase = tbase_all + M.vox_off;
-    int x = 0, y = 0, w = 0;
-    decode_word((uint32_t)(wi - M.word_off), g, W, x, y, w);
-    if (!active) return; /* whole wavefront; each wave has its own hood, so no block barrier */
-    load_hood8(hoods[wid], V, x, y, w, lane);
-    wave_lds_sync();
-    const HoodView hv{&hoods[wid], x, y, w};
+    QsGrid G;
+    const float *F;
+    uint8_t *cidx;
+    uint32_t *tbase;
+    uint32_t wi;
+    int x, y, w;
+};
+
+__device__ __forceinline__ void emit_setup(EmitWord &E, const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre,
+                                           int nmol, const uint64_t *N_all, const uint64_t *Zm_all, const uint64_t *S4_all,
+                                           const uint32_t *vbw_all, const float *F_all, uint8_t *cidx_all,
+                                           uint32_t *tbase_all, uint32_t wi)
+{
+    const QsMol &M = mols[seg_of64(wpre, nmol, wi)];
+    E.G = M.G;
+    E.V.N = N_all + M.word_off;
+    E.V.Zm = Zm_all + M.word_off;
+    E.V.S4 = S4_all + 4 * M.word_off;
+    E.V.vbw = vbw_all + M.word_off;
+    E.V.g = E.G.g;
+    E.V.W = (int)M.W;
+    E.V.lim = E.G.lim;
+    E.F = F_all + M.vox_off;
+    E.cidx = cidx_all + M.vox_off;
+    E.tbase = tbase_all + M.vox_off;
+    E.wi = wi;
+    decode_word((uint32_t)(wi - M.word_off), E.G.g, (int)M.W, E.x, E.y, E.w);
+}
+
+__device__ __forceinline__ void emit_word(const EmitWord &E, const Hood &H, EmitLds &L, int lane,
+                                          const uint32_t *__restrict__ tbw, float *__restrict__ vert,
+                                          uint64_t *__restrict__ vkey, int *__restrict__ tri)
+{
+    const int x = E.x, y = E.y, w = E.w, g = E.G.g, lim = E.G.lim;
+    const HoodView hv{&H, x, y, w};
     const int z = w * 64 + lane;
-    const float dx = G.dx;
-    const Hood &H = hoods[wid];
+    const float dx = E.G.dx;
+    const float *__restrict__ F = E.F;
     const uint64_t m0 = H.S[13][0], m1 = H.S[13][1], m2 = H.S[13][2], m3 = H.S[13][3];
     const unsigned sl = (unsigned)((m0 >> lane) & 1ull) | (unsigned)((m1 >> lane) & 1ull) << 1 |
                         (unsigned)((m2 >> lane) & 1ull) << 2 | (unsigned)((m3 >> lane) & 1ull) << 3;
@@ -1600,7 +1603,7 @@ __global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols,
     if (sl) {
         uint32_t v = hv.slot_id(x, y, z, 0);
         const uint64_t pkey = (uint64_t)pidx << 2;
-        float A[3] = {grid_pos(G.ox, x, dx), grid_pos(G.oy, y, dx), grid_pos(G.oz, z, dx)};
+        float A[3] = {grid_pos(E.G.ox, x, dx), grid_pos(E.G.oy, y, dx), grid_pos(E.G.oz, z, dx)};
         const float f0 = fat(F, g, x, y, z);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
@@ -1638,11 +1641,10 @@ __global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols,
         ex += (uint32_t)__popcll(m & below) << b;
         ntw += (uint32_t)__popcll(m) << b;
     }
-    const uint32_t tw = tbw[wi];
-    EmitLds &L = elds[wid];
+    const uint32_t tw = tbw[E.wi];
     if (nt) {
-        cidx[pidx] = (uint8_t)ci;
-        tbase[pidx] = tw + ex;
+        E.cidx[pidx] = (uint8_t)ci;
+        E.tbase[pidx] = tw + ex;
         for (int i = 0; i < nt; i++) L.owner[ex + i] = (uint8_t)lane;
     }
     L.pk[lane] = packed;
@@ -1655,6 +1657,38 @@ __global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols,
         const int e = tri_entry(L.pk[c], 3 * (int)(tt - L.ex[c]) + (int)cc);
         tri[3 * (size_t)(tw + tt) + cc] = (int)hv.corner_vid(x, y, w * 64 + c, e);
     }
+    wave_lds_sync(); /* L is reused by the wave's next word */
+}
+
+/* EMIT_K consecutive active words per wavefront: their neighbourhood loads are issued together,
+   so the wave waits on global memory once for all of them */
+#define EMIT_K 2
+__global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre, int nmol,
+                                                 const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
+                                                 const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
+                                                 const float *__restrict__ F_all, const uint32_t *__restrict__ list,
+                                                 uint32_t nact, const uint32_t *__restrict__ tbw, float *__restrict__ vert,
+                                                 uint64_t *__restrict__ vkey, int *__restrict__ tri,
+                                                 uint8_t *__restrict__ cidx_all, uint32_t *__restrict__ tbase_all)
+{
+    __shared__ Hood hoods[4][EMIT_K];
+    __shared__ EmitLds elds[4];
+    /* everything up to the lane's z is wave-uniform: scalar registers */
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t k0 = (blockIdx.x * 4 + wid) * EMIT_K;
+    if (k0 >= nact) return; /* whole wavefront; each wave has its own hoods, so no block barrier */
+    EmitWord E[EMIT_K];
+#pragma unroll
+    for (int j = 0; j < EMIT_K; j++) {
+        const uint32_t k = min(k0 + j, nact - 1);
+        const uint32_t wi = __builtin_amdgcn_readfirstlane(list[k]);
+        emit_setup(E[j], mols, wpre, nmol, N_all, Zm_all, S4_all, vbw_all, F_all, cidx_all, tbase_all, wi);
+        if (k0 + j < nact) load_hood8(hoods[wid][j], E[j].V, E[j].x, E[j].y, E[j].w, lane);
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < EMIT_K; j++)
+        if (k0 + j < nact) emit_word(E[j], hoods[wid][j], elds[wid], lane, tbw, vert, vkey, tri);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -2005,8 +2039,8 @@ void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, int nmol, const 
                        const uint32_t *tbw, float *vert, uint64_t *vkey, int *tri, uint8_t *cidx, uint32_t *tbase, hipStream_t s)
 {
     if (!nact) return;
-    hipLaunchKernelGGL(k_mc_emit, dim3(nblk(nact, 4)), dim3(256), 0, s, mols, wpre, nmol, N, Zm, S4, vbw, F, list, nact, tbw,
-                       vert, vkey, tri, cidx, tbase);
+    hipLaunchKernelGGL(k_mc_emit, dim3(nblk(nact, 4 * EMIT_K)), dim3(256), 0, s, mols, wpre, nmol, N, Zm, S4, vbw, F, list,
+                       nact, tbw, vert, vkey, tri, cidx, tbase);
 }
 
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, int nmol, const uint64_t *N, const uint64_t *Zm,
