@@ -198,6 +198,32 @@ def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_0
     return res
 
 
+def bench_other_plain(qs, hipmem, reps=5):
+    """Plain device-resident meshes of the other single-molecule configurations: config 1 (2,000
+    atoms, 0.5 A, 1 pass) and config 3 (1,000,000 atoms on a shell of radii 120 and 186.7 A,
+    0.5 A, 2 passes)."""
+    out = {}
+    for name, kind, seed, n, voxel, smooth in (("config1_2k_0.5A", "ball", 1, 2000, 0.5, 1),
+                                               ("config3_1M_shell_0.5A", "shell", 3, 1000000, 0.5, 2)):
+        pos, rad = qs.synth_ball(seed, n) if kind == "ball" else qs.synth_shell(seed, n, 120.0, 186.7)
+        dp, dr = hipmem.DeviceArray(pos), hipmem.DeviceArray(rad)
+        try:
+            qs.compute_device(dp.ptr, dr.ptr, n, voxel, smooth)
+            hipmem.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                nv, ni, *_ = qs.compute_device(dp.ptr, dr.ptr, n, voxel, smooth)
+            hipmem.synchronize()
+            dt = (time.perf_counter() - t0) / reps
+        finally:
+            dp.free()
+            dr.free()
+        g = qs.last_grid()["g"]
+        out[name] = dict(atoms=n, voxel_A=voxel, laplacian_passes=smooth, grid=g, ms_per_mesh=round(1e3 * dt, 3),
+                         voxels_per_s=round(g ** 3 / dt, 1), vertices=nv, triangles=ni // 3)
+    return out
+
+
 def config5_requests(qs, k):
     """k surface requests: atom counts log-uniform 1,000-200,000 (size seed 5, request i seed
     50000+i), voxel size uniform over {0.3, 0.5, 1.0} A, one Laplacian pass."""
@@ -396,6 +422,8 @@ def main():
         qs.compute_ses(pos, rad, W["voxel"], W["smooth"])
     api_s = (time.perf_counter() - t2) / api_steps
 
+    other = bench_other_plain(qs, hipmem)
+
     batched = None
     if not args.no_batch:
         batched = bench_batched(qs, hipmem, n_frames=args.frames)
@@ -430,6 +458,7 @@ def main():
         "api_host_to_host_voxels_per_s": round(V / api_s, 1),
         "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         "roofline": roofline,
+        "other_configs": other,
         "batched": batched,
         "served": served,
         "cpu_baseline": cpu,

@@ -434,7 +434,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     tm.mark("mc_emit");
     if (with_atoms) {
         QS_TRY(ensure(atom, (size_t)std::max<uint32_t>(nv, 1) * 4));
-        qs_launch_closest_atom(as<QsMol>(d_mols), d_vpre, nmol, as<int2>(cse), as<float4>(sx), as<int>(perm), as<float>(vert),
+        qs_launch_closest_atom(as<QsMol>(d_mols), d_vpre, nmol, as<uint32_t>(start), as<float4>(sx), as<int>(perm), as<float>(vert),
                                nv, g_opt.atom_ids_sorted, as<int>(atom), stream);
         QS_HIP(hipGetLastError());
         tm.mark("closest_atom");
@@ -550,6 +550,27 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     }
 
     /* ---- degenerate filter (CudaSurf.cu:820-829), per-molecule kept counts ---- */
+    /* count first: without degenerate triangles (the usual case) the filter is the identity */
+    QS_HIP(hipMemsetAsync(as<uint32_t>(totals) + 3, 0, 4, stream));
+    qs_launch_tri_count_degenerate(as<int>(tri), nt_all, as<uint32_t>(totals) + 3, stream);
+    uint32_t ndeg = 0;
+    QS_HIP(hipMemcpyAsync(&ndeg, as<uint32_t>(totals) + 3, 4, hipMemcpyDeviceToHost, stream));
+    QS_HIP(hipStreamSynchronize(stream));
+    if (ndeg == 0) {
+        for (int m = 0; m < nmol; m++) tri_off[m] = mols[m].tri_off;
+        tri_off[nmol] = nt_all;
+        /* device copy of the offsets (read by the batch getters); the staging vector is next
+           written after this call's stream work has been waited for */
+        h_trioff = tri_off;
+        QS_HIP(hipMemcpyAsync(bounds.p, h_trioff.data(), (size_t)(nmol + 1) * 4, hipMemcpyHostToDevice, stream));
+        tm.mark("filter");
+        nt = nt_all;
+        res_tri = as<int>(tri);
+        res_atom = as<int>(atom);
+        batch_valid = true;
+        tm.report(mols[0].G, n);
+        return 0;
+    }
     QS_TRY(ensure(keep, (size_t)std::max<uint32_t>(nt_all, 1) * 4));
     QS_TRY(ensure(keeppos, ((size_t)nt_all + 1) * 4));
     QS_TRY(ensure(tri_out, (size_t)std::max<uint32_t>(nt_all, 1) * 12));

@@ -50,7 +50,7 @@ void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, int nmol, const 
                        const uint32_t *tbw, float *vert, uint64_t *vkey, int *tri, uint8_t *cidx, uint32_t *tbase, hipStream_t s);
 
 /* per-vertex stages */
-void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, int nmol, const int2 *cse, const float4 *sx,
+void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, int nmol, const uint32_t *cstart, const float4 *sx,
                             const int *perm, const float *vert, uint32_t nv, int sorted_ids, int *out, hipStream_t s);
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, int nmol, const uint64_t *N, const uint64_t *Zm,
                       const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
@@ -58,6 +58,7 @@ void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, int nmol, const u
 
 /* degenerate-triangle filter */
 void qs_launch_tri_keep(const int *tri, uint32_t nt, uint32_t *keep, hipStream_t s);
+void qs_launch_tri_count_degenerate(const int *tri, uint32_t nt, uint32_t *count, hipStream_t s);
 void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *pos, int *out, hipStream_t s);
 void qs_launch_tri_localize(const int *tri, uint32_t nt, const uint32_t *tpre, const uint32_t *vpre, int nmol, int *out,
                             hipStream_t s);

@@ -112,6 +112,16 @@ __device__ __forceinline__ int seg_of(const uint32_t *__restrict__ pre, int n, u
     return lo;
 }
 
+/* seg_of for a wavefront whose items usually share one segment: the first active lane's
+   segment is found with scalar loads, the others fall back to their own search */
+__device__ __forceinline__ int seg_of_wave(const uint32_t *__restrict__ pre, int n, uint32_t v)
+{
+    const uint32_t v0 = __builtin_amdgcn_readfirstlane(v);
+    const int m0 = __builtin_amdgcn_readfirstlane(seg_of(pre, n, v0));
+    if (v >= pre[m0] && v < pre[m0 + 1]) return m0;
+    return seg_of(pre, n, v);
+}
+
 __device__ __forceinline__ int seg_of64(const uint64_t *__restrict__ pre, int n, uint64_t v)
 {
     int lo = 0, hi = n - 1;
@@ -1694,42 +1704,42 @@ __global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols,
 /* ------------------------------------------------------------------------------------------ */
 /* closest atom per vertex (computeClosestAtom Kernels.cu:547-604)                              */
 __global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
-                                                      int nmol, const int2 *__restrict__ cse_all,
+                                                      int nmol, const uint32_t *__restrict__ cstart_all,
                                                       const float4 *__restrict__ sx, const int *__restrict__ perm,
                                                       const float *__restrict__ vert, uint32_t nv, int sorted_ids,
                                                       int *__restrict__ out)
 {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
-    const QsMol &M = mols[seg_of(vpre, nmol, v)];
+    const QsMol &M = mols[seg_of_wave(vpre, nmol, v)];
     const QsGrid G = M.G;
-    const int2 *__restrict__ cse = cse_all + M.cell_off;
+    const uint32_t *__restrict__ cstart = cstart_all + M.cell_off;
     float px = vert[3 * (size_t)v], py = vert[3 * (size_t)v + 1], pz = vert[3 * (size_t)v + 2];
     int cx = cell_of(px, G.ox, G.dxn), cy = cell_of(py, G.oy, G.dxn), cz = cell_of(pz, G.oz, G.dxn);
     const int gn = G.gn;
-    /* the reference keeps the first atom, in traversal order, whose rounded distance is strictly
-       below the best so far.  sqrtf is monotonic, so an atom can only win if its squared sum is
-       below the best one's; the root is taken only then and compared as the reference does. */
+    /* The reference visits the 27 clamped cells x-major, z fastest, and keeps the first atom whose
+       rounded distance is strictly below the best so far.  The three z cells of an (x, y) column
+       are consecutive cells, so their atoms are one contiguous range in that same order; a cell
+       the clamp visits twice cannot improve on its first visit.  sqrtf is monotonic, so an atom
+       can only win if its squared sum is below the best one's; the root is taken only then. */
+    const int z0 = iclamp(cz - 1, 0, gn - 1), z1 = iclamp(cz + 1, 0, gn - 1);
     int best = -1;
     float min_d = 999999.0f, min_s = INFINITY;
     for (int a = -1; a <= 1; a++) {
-        int ix = iclamp(cx + a, 0, gn - 1);
+        const int ix = iclamp(cx + a, 0, gn - 1);
         for (int b = -1; b <= 1; b++) {
-            int iy = iclamp(cy + b, 0, gn - 1);
-            for (int c = -1; c <= 1; c++) {
-                int iz = iclamp(cz + c, 0, gn - 1);
-                int2 se = cse[(gn * gn * ix) + (gn * iy) + iz];
-                if (se.x >= QS_EMPTY) continue;
-                for (int id = se.x; id < se.y; id++) {
-                    float4 at = sx[id];
-                    const float sq = dist3_sq(at.x, at.y, at.z, px, py, pz);
-                    if (sq < min_s) {
-                        const float d = sqrtf(sq);
-                        if (d < min_d) {
-                            min_d = d;
-                            min_s = sq;
-                            best = id;
-                        }
+            const int iy = iclamp(cy + b, 0, gn - 1);
+            const int h = (gn * gn * ix) + (gn * iy);
+            const int lo = (int)cstart[h + z0], hi = (int)cstart[h + z1 + 1];
+            for (int id = lo; id < hi; id++) {
+                const float4 at = sx[id];
+                const float sq = dist3_sq(at.x, at.y, at.z, px, py, pz);
+                if (sq < min_s) {
+                    const float d = sqrtf(sq);
+                    if (d < min_d) {
+                        min_d = d;
+                        min_s = sq;
+                        best = id;
                     }
                 }
             }
@@ -1775,7 +1785,7 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
-    const QsMol &M = mols[seg_of(vpre, nmol, v)];
+    const QsMol &M = mols[seg_of_wave(vpre, nmol, v)];
     const QsGrid G = M.G;
     const int g = G.g, lim = G.lim;
     MCView V;
@@ -1863,6 +1873,21 @@ __global__ void __launch_bounds__(256) k_tri_keep(const int *__restrict__ tri, u
     if (t >= nt) return;
     int x = tri[3 * (size_t)t], y = tri[3 * (size_t)t + 1], z = tri[3 * (size_t)t + 2];
     keep[t] = (x != y && y != z && x != z) ? 1u : 0u;
+}
+
+/* number of degenerate triangles (a repeated corner) added to *count: one atomic per wavefront
+   that has any */
+__global__ void __launch_bounds__(256) k_tri_count_degenerate(const int *__restrict__ tri, uint32_t nt,
+                                                              uint32_t *__restrict__ count)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool deg = false;
+    if (t < nt) {
+        const int x = tri[3 * (size_t)t], y = tri[3 * (size_t)t + 1], z = tri[3 * (size_t)t + 2];
+        deg = !(x != y && y != z && x != z);
+    }
+    const uint64_t m = __ballot(deg);
+    if (m && (threadIdx.x & 63) == (uint32_t)(__builtin_ctzll(m))) atomicAdd(count, (uint32_t)__popcll(m));
 }
 
 __global__ void __launch_bounds__(256) k_tri_compact(const int *__restrict__ tri, uint32_t nt, const uint32_t *__restrict__ pos,
@@ -2052,12 +2077,18 @@ void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, int nmol, const u
                        tri, src, dst);
 }
 
-void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, int nmol, const int2 *cse, const float4 *sx,
+void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, int nmol, const uint32_t *cstart, const float4 *sx,
                             const int *perm, const float *vert, uint32_t nv, int sorted_ids, int *out, hipStream_t s)
 {
     if (!nv) return;
-    hipLaunchKernelGGL(k_closest_atom, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, nmol, cse, sx, perm, vert, nv,
+    hipLaunchKernelGGL(k_closest_atom, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, nmol, cstart, sx, perm, vert, nv,
                        sorted_ids, out);
+}
+
+void qs_launch_tri_count_degenerate(const int *tri, uint32_t nt, uint32_t *count, hipStream_t s)
+{
+    if (!nt) return;
+    hipLaunchKernelGGL(k_tri_count_degenerate, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, count);
 }
 
 void qs_launch_tri_keep(const int *tri, uint32_t nt, uint32_t *keep, hipStream_t s)
