@@ -1800,15 +1800,20 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
     const uint32_t *__restrict__ tbase = tbase_all + M.vox_off;
     const uint64_t key = vkey[v];
     const int sl = (int)(key & 3ull);
-    const uint64_t p = key >> 2;
-    const uint64_t xy = p / (uint64_t)g;
-    const int z = (int)(p - xy * (uint64_t)g);
-    const int x = (int)(xy / (uint64_t)g), y = (int)(xy - (uint64_t)x * g);
+    /* key = grid point index << 2 | slot; a molecule's grid has fewer than 2^32 points */
+    const uint32_t p = (uint32_t)(key >> 2), ug = (uint32_t)g;
+    const uint32_t xy = p / ug;
+    const int z = (int)(p - xy * ug);
+    const int x = (int)(xy / ug), y = (int)(xy - (uint32_t)x * ug);
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     uint32_t cnt = 0;
     if (sl < 3) {
         /* the 4 cubes around the edge in cube order, and the edge's number in each; all of them
-           that are polygonised contain the sign change, so their records exist */
+           that are polygonised contain the sign change, so their records exist.  The records of
+           all four are loaded before any is used. */
+        int ci4[4], e4[4];
+        uint32_t tb4[4];
+#pragma unroll
         for (int k = 0; k < 4; k++) {
             const int hi = k >> 1, lo = k & 1; /* offsets -1 / 0 of the two other axes, major first */
             int cx = x, cy = y, cz = z, e;
@@ -1825,12 +1830,20 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
                 cy -= 1 - lo;
                 e = (1 - hi) ? ((1 - lo) ? 10 : 9) : ((1 - lo) ? 11 : 8); /* (dx,dy,0): 8 9 10 11 */
             }
+            e4[k] = e;
+            ci4[k] = 0;
+            tb4[k] = 0;
             if (cx < 0 || cy < 0 || cz < 0 || cx >= lim || cy >= lim || cz >= lim) continue;
             const size_t c = ((size_t)cx * g + cy) * g + cz;
-            const int ci = cidx[c];
-            const uint32_t tb = tbase[c];
-            const int nt = tri_count(ci);
+            ci4[k] = cidx[c];
+            tb4[k] = tbase[c];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int ci = ci4[k], e = e4[k];
+            const uint32_t tb = tb4[k];
             const uint64_t packed = c_tri[ci];
+            const int nt = tri_count_packed(packed);
             for (int t = 0; t < nt; t++) {
                 const int j = (tri_entry(packed, 3 * t) == e) ? 0
                               : (tri_entry(packed, 3 * t + 1) == e) ? 1
