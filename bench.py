@@ -241,7 +241,7 @@ def _pct(xs, q):
     return round(float(np.percentile(np.asarray(xs), q)), 3)
 
 
-def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=32 << 30):
+def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=32 << 30, max_batch_voxels=0):
     """Poisson arrivals at `load` x the plain path's capacity.  Plain path: each request alone,
     atoms copied to HBM and meshed by qs_compute_device (mesh left in HBM), first come first
     served.  Served path: the request server (continuous batching over `workers` engines, HBM
@@ -290,7 +290,7 @@ def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=
 
     # served path
     lat, queue, bsize = [], [], []
-    with qs.Server(hbm_budget_bytes=budget, result_bytes=arena, workers=workers) as srv:
+    with qs.Server(hbm_budget_bytes=budget, result_bytes=arena, workers=workers, max_batch_voxels=max_batch_voxels) as srv:
         w = srv.submit(reqs[0][0][0][:2000], reqs[0][0][1][:2000], 1.0, smooth)  # warm-up of every engine
         srv.poll(w, -1)
         srv.release(w)
@@ -333,7 +333,7 @@ def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=
         served=dict(meshes_per_s=round(k / wall_served, 2), p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99),
                     queue_p50_ms=_pct(queue, 50), mean_batch=round(float(np.mean(bsize)), 2),
                     batches=st["batches"] - 1, workers=workers, max_busy_workers=st["max_busy_workers"],
-                    hbm_budget_GiB=budget >> 30),
+                    hbm_budget_GiB=budget >> 30, max_batch_voxels=max_batch_voxels),
         saturated=dict(plain_meshes_per_s=round(capacity, 2), served_meshes_per_s=round(k / wall_sat, 2),
                        speedup=round(wall_sat and (k / wall_sat) / capacity, 3), served_p50_ms=_pct(lat_sat, 50),
                        served_p99_ms=_pct(lat_sat, 99)),
@@ -352,6 +352,9 @@ def main():
     ap.add_argument("--frames", type=int, default=1000, help="trajectory frames of the batched workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
     ap.add_argument("--requests", type=int, default=400, help="requests of the served stream")
+    ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--serve-cap", type=int, default=150_000_000, help="grid points per served batch (0 = no cap)")
+    ap.add_argument("--no-other", action="store_true", help="skip configs 1 and 3")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the SES path runs on a single device and this build has no "
@@ -422,7 +425,7 @@ def main():
         qs.compute_ses(pos, rad, W["voxel"], W["smooth"])
     api_s = (time.perf_counter() - t2) / api_steps
 
-    other = bench_other_plain(qs, hipmem)
+    other = None if args.no_other else bench_other_plain(qs, hipmem)
 
     batched = None
     if not args.no_batch:
@@ -430,7 +433,8 @@ def main():
 
     served = None
     if not args.no_serve:
-        served = bench_served(qs, hipmem, k=args.requests)
+        served = bench_served(qs, hipmem, k=args.requests, workers=args.serve_workers,
+                              max_batch_voxels=args.serve_cap)
         print("served: %s" % json.dumps(served), file=sys.stderr, flush=True)
 
     cpu = None

@@ -149,6 +149,9 @@ int qs_stage_mesh(const float *fparams, const int *iparams, const float *field, 
  *   qs_serve_release frees the request's result
  *   qs_serve_stats   {batches, requests batched, largest batch, most workers busy at once,
  *                    arena bytes in use, arena bytes, requests pending}; returns 7
+ *   qs_serve_set     "max_batch_voxels": cap on the grid points of one batch (0 = none, the
+ *                    default; a request larger than the cap runs alone); "max_batch": cap on
+ *                    the requests of one batch (1..256, default 256)
  *   qs_serve_close   finishes pending work, stops the workers, frees everything */
 int qs_serve_open(uint64_t hbm_budget_bytes, uint64_t result_bytes, int workers);
 int64_t qs_serve_submit(const float *pos, const float *rad, unsigned int N, float reso, int doSmoothing);
@@ -156,6 +159,7 @@ int qs_serve_poll(int64_t id, int wait_ms, unsigned int *nvert, unsigned int *nt
                   int **d_atom_id, double *latency_ms);
 int qs_serve_fetch(int64_t id, float *vert, int *tri, int *atom_id);
 int qs_serve_release(int64_t id);
+int qs_serve_set(const char *key, uint64_t value);
 int qs_serve_stats(uint64_t *out, int n);
 int qs_serve_close(void);
 

@@ -96,6 +96,8 @@ def lib():
     L.qs_serve_fetch.restype = ctypes.c_int
     L.qs_serve_release.argtypes = [i64]
     L.qs_serve_release.restype = ctypes.c_int
+    L.qs_serve_set.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
+    L.qs_serve_set.restype = ctypes.c_int
     L.qs_serve_stats.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]
     L.qs_serve_stats.restype = ctypes.c_int
     L.qs_serve_close.argtypes = []
@@ -315,12 +317,19 @@ class Server:
     """Request server (qs_serve_*): submit surface requests at any time, poll for device-resident
     meshes.  Use as a context manager, or call close()."""
 
-    def __init__(self, hbm_budget_bytes=16 << 30, result_bytes=8 << 30, workers=2):
+    def __init__(self, hbm_budget_bytes=16 << 30, result_bytes=8 << 30, workers=2, max_batch_voxels=0):
         L = lib()
         rc = L.qs_serve_open(int(hbm_budget_bytes), int(result_bytes), int(workers))
         if rc:
             raise QuickSESError("quickses error %d: %s" % (rc, L.qs_last_error_message().decode()))
         self.open = True
+        if max_batch_voxels:
+            self.set("max_batch_voxels", max_batch_voxels)
+
+    def set(self, key, value):
+        rc = lib().qs_serve_set(key.encode(), int(value))
+        if rc:
+            raise QuickSESError("quickses error %d: %s" % (-rc, lib().qs_last_error_message().decode()))
 
     def submit(self, pos, rad, reso=0.5, smooth=1):
         L = lib()

@@ -41,6 +41,7 @@ struct Request {
     int smooth = 1;
     BBox bb{};
     uint64_t est_bytes = 0;
+    uint64_t voxels = 0; /* g^3: the request's share of a batch's work */
     Clock::time_point t_submit, t_start, t_done;
     ReqState state = PENDING;
     char err[256] = "";
@@ -109,6 +110,7 @@ struct Server {
     Arena arena;
     uint64_t budget_per_worker = 0;
     int max_batch = 256;
+    uint64_t max_batch_voxels = 0; /* 0: no cap */
     int device = 0;
     bool closing = false;
     int64_t next_id = 0;
@@ -234,17 +236,20 @@ void worker_main(Server *S, int w)
             if (S->pending.empty()) return; /* closing */
             /* arrival order; the batch ends at the first request that does not fit the budget
                (a request larger than the budget runs alone) or asks for another pass count */
-            uint64_t acc = 0;
+            uint64_t acc = 0, accv = 0;
             const int smooth = S->pending.front()->smooth;
             const auto now = Clock::now();
             while (!S->pending.empty() && (int)batch.size() < S->max_batch) {
                 Request *r = S->pending.front();
                 if (r->smooth != smooth) break;
                 if (!batch.empty() && acc + r->est_bytes > S->budget_per_worker) break;
+                /* work cap: a batch stays short, so requests arriving meanwhile wait less */
+                if (!batch.empty() && S->max_batch_voxels && accv + r->voxels > S->max_batch_voxels) break;
                 S->pending.pop_front();
                 r->state = RUNNING;
                 r->t_start = now;
                 acc += r->est_bytes;
+                accv += r->voxels;
                 batch.push_back(r);
             }
             S->batches++;
@@ -326,6 +331,7 @@ int64_t qs_serve_submit(const float *pos, const float *rad, unsigned int N, floa
             return -set_error(4, "grid of %d^3 points", G.g);
         }
         r->est_bytes = estimate_bytes(G, N);
+        r->voxels = (uint64_t)G.g * G.g * G.g;
         r->pos.assign(pos, pos + 3 * (size_t)N);
         r->rad.assign(rad, rad + N);
     } else {
@@ -400,6 +406,18 @@ int qs_serve_release(int64_t id)
     if (r->arena_bytes) S->arena.release(r->arena_off, r->arena_bytes);
     S->reqs.erase(it);
     delete r;
+    return 0;
+}
+
+int qs_serve_set(const char *key, uint64_t value)
+{
+    if (check_server()) return -g_err;
+    Server *S = g_srv;
+    std::lock_guard<std::mutex> lk(S->mu);
+    const std::string k = key ? key : "";
+    if (k == "max_batch_voxels") S->max_batch_voxels = value;
+    else if (k == "max_batch") S->max_batch = (int)std::max<uint64_t>(1, std::min<uint64_t>(value, 256));
+    else return -set_error(3, "unknown server setting %s", k.c_str());
     return 0;
 }
 

@@ -30,11 +30,12 @@ def _requests(qs, k, seed=900):
     return out
 
 
-@pytest.mark.parametrize("workers,budget", [(2, 8 << 30), (1, 256 << 20), (3, 96 << 20)])
-def test_served_results_equal_alone(qs, workers, budget):
+@pytest.mark.parametrize("workers,budget,cap", [(2, 8 << 30, 0), (1, 256 << 20, 0), (3, 96 << 20, 0),
+                                                (2, 8 << 30, 3_000_000)])
+def test_served_results_equal_alone(qs, workers, budget, cap):
     reqs = _requests(qs, 14)
-    print("case serve workers=%d budget=%d" % (workers, budget), flush=True)
-    with qs.Server(hbm_budget_bytes=budget, result_bytes=1 << 30, workers=workers) as srv:
+    print("case serve workers=%d budget=%d cap=%d" % (workers, budget, cap), flush=True)
+    with qs.Server(hbm_budget_bytes=budget, result_bytes=1 << 30, workers=workers, max_batch_voxels=cap) as srv:
         ids = []
         for i, ((pos, rad), reso, smooth) in enumerate(reqs):
             ids.append(srv.submit(pos, rad, reso, smooth))
@@ -47,7 +48,7 @@ def test_served_results_equal_alone(qs, workers, budget):
             srv.release(r)
         assert srv.stats()["arena_used"] == 0
     assert st["requests_batched"] == len(reqs)
-    if budget < (1 << 30):
+    if budget < (1 << 30) or cap:
         assert st["batches"] > 1
     for ((pos, rad), reso, smooth), got, inf in zip(reqs, served, info):
         assert inf["latency_ms"] >= inf["queue_ms"] >= 0.0 and inf["batch_size"] >= 1
@@ -80,6 +81,8 @@ def test_served_errors(qs):
             srv.submit(bad, rad, 0.5, 1)
         with pytest.raises(qs.QuickSESError):
             srv.submit(pos, rad, 0.0, 1)
+        with pytest.raises(qs.QuickSESError):
+            srv.set("no_such_setting", 1)
         # a mesh larger than the 1 MiB result arena fails cleanly and frees nothing it did not take
         rid = srv.submit(pos, rad, 0.3, 1)
         with pytest.raises(qs.QuickSESError, match="arena"):
