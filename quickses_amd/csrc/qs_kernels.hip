@@ -368,6 +368,24 @@ __global__ void __launch_bounds__(256) k_gather_atoms(const int *__restrict__ pe
  * instead of ~44 candidate atoms per voxel.
  */
 
+/* OR of v over the 64 lanes (all active), returned wave-uniform: row shifts inside each 16-lane
+   row, then row broadcasts, leave the total in lane 63 */
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v)
+{
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true); /* row_shr:1 */
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true); /* row_shr:2 */
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true); /* row_shr:4 */
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true); /* row_shr:8 */
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false); /* row_bcast:15 */
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false); /* row_bcast:31 */
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ uint64_t wave_or_u64(uint64_t v)
+{
+    return ((uint64_t)wave_or_u32((uint32_t)(v >> 32)) << 32) | wave_or_u32((uint32_t)v);
+}
+
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
 {
     uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
@@ -673,11 +691,8 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsBatch B, const uint32_t *_
 
 #pragma unroll
     for (int k = 0; k < P1_RMAX; k++) {
-#pragma unroll
-        for (int m = 32; m > 0; m >>= 1) {
-            m_in[k] |= shfl_xor_u64(m_in[k], m);
-            m_near[k] |= shfl_xor_u64(m_near[k], m);
-        }
+        m_in[k] = wave_or_u64(m_in[k]);
+        m_near[k] = wave_or_u64(m_near[k]);
     }
     if (lane < ny) {
         uint64_t mi = 0ull, mn = 0ull;
