@@ -43,11 +43,18 @@ struct QsMol {
     int pad;
 };
 
+/* Coarse indexes of prefix arrays: coarse[k] = the segment that holds item k << shift, with
+   shift QS_COARSE_BLOCKS for block prefixes and QS_COARSE_ITEMS for row-word and vertex
+   prefixes. */
+#define QS_COARSE_BLOCKS 6
+#define QS_COARSE_ITEMS 12
+
 /* A launch over a batch: block b belongs to molecule mid[j] (or j) where
-   bpre[j] <= b < bpre[j+1]. */
+   bpre[j] <= b < bpre[j+1]; coarse indexes bpre. */
 struct QsBatch {
     const QsMol *mols;
     const uint32_t *bpre;
+    const uint32_t *coarse;
     const int *mid;
     int n;
 };

@@ -266,6 +266,7 @@ struct Engine {
         QsBatch B;
         B.mols = (const QsMol *)d_mols.p;
         B.bpre = (const uint32_t *)d_meta.p + bpre_at;
+        B.coarse = B.bpre + n + 1; /* the coarse index follows each prefix */
         B.mid = mid_at ? (const int *)((const uint32_t *)d_meta.p + mid_at) : nullptr;
         B.n = n;
         return B;

@@ -231,6 +231,20 @@ int Engine::setup(const std::vector<MolIn> &in, const std::vector<BBox> &bb)
     return 0;
 }
 
+/* coarse index of a prefix array pre[0..n] (pre[n] = total): entry k = the segment holding item
+   k << shift */
+template <class T>
+void append_coarse(std::vector<uint32_t> &out, const T *pre, int n, uint64_t total, int shift)
+{
+    const uint64_t K = (total >> shift) + 1;
+    int j = 0;
+    for (uint64_t k = 0; k < K; k++) {
+        const uint64_t t = k << shift;
+        while (j + 1 < n && (uint64_t)pre[j + 1] <= t) j++;
+        out.push_back((uint32_t)j);
+    }
+}
+
 /* block prefixes of the launches (one uint32 array, uploaded once) */
 int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &at_wpre,
                        uint32_t &nb_atoms, uint32_t &nb_p1, uint32_t &nb_mc)
@@ -238,6 +252,7 @@ int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &a
     const int nmol = (int)mols.size();
     std::vector<uint32_t> &meta = h_meta;
     meta.assign(1, 0u); /* index 0 is never a prefix start (mid_at == 0 means none) */
+    /* each block prefix (nmol + 1 entries) is followed by its coarse index */
     auto prefix = [&](auto blocks_of, uint32_t &total) {
         size_t at = meta.size();
         uint64_t acc = 0;
@@ -247,6 +262,8 @@ int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &a
         }
         meta.push_back((uint32_t)acc);
         total = (uint32_t)acc;
+        std::vector<uint32_t> pre(meta.begin() + at, meta.end());
+        append_coarse(meta, pre.data(), nmol, acc, QS_COARSE_BLOCKS);
         return at;
     };
     at_atoms = prefix([](const QsMol &M) { return (uint64_t)((M.n_atoms + 255) / 256); }, nb_atoms);
@@ -254,14 +271,17 @@ int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &a
     at_p1 = prefix([&](const QsMol &M) { return (uint64_t)(M.G.g ? qs_pass1_blocks(M.G, (int)M.W, M.rows_per_wave, p1d) : 0); },
                    nb_p1);
     at_mc = prefix([](const QsMol &M) { return (uint64_t)(M.G.g ? qs_mc_count_blocks(M.G, (int)M.W) : 0); }, nb_mc);
-    /* word prefix (64-bit) stored as pairs */
+    /* word prefix (64-bit) stored as pairs, then its coarse index */
     if (meta.size() & 1) meta.push_back(0u);
     at_wpre = meta.size();
+    std::vector<uint64_t> wp(nmol + 1);
     for (int m = 0; m <= nmol; m++) {
         uint64_t w = (m < nmol) ? mols[m].word_off : nw_total;
+        wp[m] = w;
         meta.push_back((uint32_t)w);
         meta.push_back((uint32_t)(w >> 32));
     }
+    append_coarse(meta, wp.data(), nmol, nw_total, QS_COARSE_ITEMS);
     /* field pass 2 groups: same kernel path, r, dx and coordinate range */
     groups.resize(0);
     std::vector<P2Group> gs;
@@ -295,6 +315,8 @@ int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &a
         }
         meta.push_back((uint32_t)acc);
         grp.nblocks = (uint32_t)acc;
+        std::vector<uint32_t> pre(meta.begin() + grp.bpre_at, meta.end());
+        append_coarse(meta, pre.data(), (int)grp.mols.size(), acc, QS_COARSE_BLOCKS);
         grp.mid_at = meta.size();
         for (int m : grp.mols) meta.push_back((uint32_t)m);
     }
@@ -417,24 +439,30 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     }
     vert_off = vpre;
     QS_HIP(hipMemcpyAsync(d_mols.p, mols.data(), mols.size() * sizeof(QsMol), hipMemcpyHostToDevice, stream));
-    /* vertex prefix for per-vertex kernels lives after the molecule table's prefixes */
-    QS_TRY(ensure(d_off, ((size_t)nmol + 1) * 4 * 2 + 64));
+    /* vertex prefix and its coarse index for the per-vertex kernels, after the atom offsets */
+    {
+        std::vector<uint32_t> vc;
+        append_coarse(vc, vpre.data(), nmol, nv, QS_COARSE_ITEMS);
+        vpre.insert(vpre.end(), vc.begin(), vc.end());
+    }
+    QS_TRY(ensure(d_off, ((size_t)nmol + 1) * 4 + vpre.size() * 4 + 64));
     uint32_t *d_vpre = as<uint32_t>(d_off) + (nmol + 1);
-    QS_HIP(hipMemcpyAsync(d_vpre, vpre.data(), (size_t)(nmol + 1) * 4, hipMemcpyHostToDevice, stream));
+    QS_HIP(hipMemcpyAsync(d_vpre, vpre.data(), vpre.size() * 4, hipMemcpyHostToDevice, stream));
     QS_TRY(ensure(vert, (size_t)std::max<uint32_t>(nv, 1) * 12));
     QS_TRY(ensure(vkey, (size_t)std::max<uint32_t>(nv, 1) * 8));
     QS_TRY(ensure(tri, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
     QS_TRY(ensure(cidx, std::max<uint64_t>(vox_total, 1)));
     QS_TRY(ensure(tbase, std::max<uint64_t>(vox_total, 1) * 4));
     const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
-    qs_launch_mc_emit(as<QsMol>(d_mols), wpre, nmol, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
+    const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 2 * ((size_t)nmol + 1);
+    qs_launch_mc_emit(as<QsMol>(d_mols), wpre, wcoarse, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
                       as<uint32_t>(vbw), as<float>(field), as<uint32_t>(alist), nact, as<uint32_t>(tbw), as<float>(vert),
                       as<uint64_t>(vkey), as<int>(tri), as<uint8_t>(cidx), as<uint32_t>(tbase), stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
     if (with_atoms) {
         QS_TRY(ensure(atom, (size_t)std::max<uint32_t>(nv, 1) * 4));
-        qs_launch_closest_atom(as<QsMol>(d_mols), d_vpre, nmol, as<uint32_t>(start), as<float4>(sx), as<int>(perm), as<float>(vert),
+        qs_launch_closest_atom(as<QsMol>(d_mols), d_vpre, d_vpre + nmol + 1, as<uint32_t>(start), as<float4>(sx), as<int>(perm), as<float>(vert),
                                nv, g_opt.atom_ids_sorted, as<int>(atom), stream);
         QS_HIP(hipGetLastError());
         tm.mark("closest_atom");
@@ -538,7 +566,7 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
         const float *src = as<float>(vert);
         float *dst = as<float>(vert2);
         for (int it = 0; it < smooth; it++) {
-            qs_launch_smooth(as<QsMol>(d_mols), d_vpre, nmol, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
+            qs_launch_smooth(as<QsMol>(d_mols), d_vpre, d_vpre + nmol + 1, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
                              as<uint32_t>(vbw), as<uint64_t>(vkey), nv, as<uint8_t>(cidx), as<uint32_t>(tbase), as<int>(tri),
                              src, dst, stream);
             src = dst;

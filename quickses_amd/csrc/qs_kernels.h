@@ -1,5 +1,6 @@
 /* Launchers of the HIP kernels (qs_kernels.hip).  Every per-molecule kernel runs over a batch:
- * a QsBatch maps each block to a molecule (QsMol) and to a block index inside it. */
+ * a QsBatch maps each block to a molecule (QsMol) and to a block index inside it.  Prefix
+ * arrays over blocks, row words or vertices come with coarse indexes (qs_common.h). */
 #ifndef QS_KERNELS_H
 #define QS_KERNELS_H
 
@@ -45,14 +46,15 @@ void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const ui
 void qs_launch_mc_flag(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *flag, hipStream_t s);
 void qs_launch_mc_active(const uint32_t *vcnt, const uint32_t *tcnt, const uint32_t *apos, uint64_t nw, uint32_t *list,
                          hipStream_t s);
-void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, int nmol, const uint64_t *N, const uint64_t *Zm,
+void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint64_t *N, const uint64_t *Zm,
                        const uint64_t *S4, const uint32_t *vbw, const float *F, const uint32_t *list, uint32_t nact,
                        const uint32_t *tbw, float *vert, uint64_t *vkey, int *tri, uint8_t *cidx, uint32_t *tbase, hipStream_t s);
 
 /* per-vertex stages */
-void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, int nmol, const uint32_t *cstart, const float4 *sx,
+void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint32_t *cstart,
+                            const float4 *sx,
                             const int *perm, const float *vert, uint32_t nv, int sorted_ids, int *out, hipStream_t s);
-void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, int nmol, const uint64_t *N, const uint64_t *Zm,
+void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
                       const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
                       const uint32_t *tbase, const int *tri, const float *src, float *dst, hipStream_t s);
 

@@ -87,17 +87,24 @@ __device__ __forceinline__ int tri_count_packed(uint64_t p)
 
 __device__ __forceinline__ int tri_count(int ci) { return tri_count_packed(c_tri[ci]); }
 
+/* Segment lookups.  A prefix array pre (n + 1 non-decreasing entries, pre[n] = total) comes with
+   a coarse index: coarse[k] = the segment holding item k << shift.  Segment of item v: start at
+   coarse[v >> shift] and step forward past the segments that end at or before v (segments are
+   long next to 2^shift items, so this is usually no step at all). */
+template <int SHIFT, typename T>
+__device__ __forceinline__ int seg_coarse(const T *__restrict__ pre, const uint32_t *__restrict__ coarse, T v)
+{
+    int j = (int)coarse[v >> SHIFT];
+    while (pre[j + 1] <= v) j++;
+    return j;
+}
+
 /* molecule of block b of a batched launch, and the block's index inside the molecule */
 __device__ __forceinline__ int batch_mol(const QsBatch &B, uint32_t b, uint32_t &local)
 {
-    int lo = 0, hi = B.n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (B.bpre[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    local = b - B.bpre[lo];
-    return B.mid ? B.mid[lo] : lo;
+    const int j = seg_coarse<QS_COARSE_BLOCKS>(B.bpre, B.coarse, b);
+    local = b - B.bpre[j];
+    return B.mid ? B.mid[j] : j;
 }
 
 /* index of the segment of a sorted prefix array (n + 1 entries) that holds item v */
@@ -112,14 +119,14 @@ __device__ __forceinline__ int seg_of(const uint32_t *__restrict__ pre, int n, u
     return lo;
 }
 
-/* seg_of for a wavefront whose items usually share one segment: the first active lane's
-   segment is found with scalar loads, the others fall back to their own search */
-__device__ __forceinline__ int seg_of_wave(const uint32_t *__restrict__ pre, int n, uint32_t v)
+/* segment of v for a wavefront whose items usually share one: the first active lane's segment
+   is found with scalar loads, the others fall back to their own lookup */
+__device__ __forceinline__ int seg_of_wave(const uint32_t *__restrict__ pre, const uint32_t *__restrict__ coarse, uint32_t v)
 {
     const uint32_t v0 = __builtin_amdgcn_readfirstlane(v);
-    const int m0 = __builtin_amdgcn_readfirstlane(seg_of(pre, n, v0));
+    const int m0 = __builtin_amdgcn_readfirstlane(seg_coarse<QS_COARSE_ITEMS>(pre, coarse, v0));
     if (v >= pre[m0] && v < pre[m0 + 1]) return m0;
-    return seg_of(pre, n, v);
+    return seg_coarse<QS_COARSE_ITEMS>(pre, coarse, v);
 }
 
 __device__ __forceinline__ int seg_of64(const uint64_t *__restrict__ pre, int n, uint64_t v)
@@ -1592,11 +1599,11 @@ struct EmitWord {
 };
 
 __device__ __forceinline__ void emit_setup(EmitWord &E, const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre,
-                                           int nmol, const uint64_t *N_all, const uint64_t *Zm_all, const uint64_t *S4_all,
+                                           const uint32_t *__restrict__ wcoarse, const uint64_t *N_all, const uint64_t *Zm_all, const uint64_t *S4_all,
                                            const uint32_t *vbw_all, const float *F_all, uint8_t *cidx_all,
                                            uint32_t *tbase_all, uint32_t wi)
 {
-    const QsMol &M = mols[seg_of64(wpre, nmol, wi)];
+    const QsMol &M = mols[seg_coarse<QS_COARSE_ITEMS>(wpre, wcoarse, (uint64_t)wi)];
     E.G = M.G;
     E.V.N = N_all + M.word_off;
     E.V.Zm = Zm_all + M.word_off;
@@ -1688,7 +1695,8 @@ __device__ __forceinline__ void emit_word(const EmitWord &E, const Hood &H, Emit
 /* EMIT_K consecutive active words per wavefront: their neighbourhood loads are issued together,
    so the wave waits on global memory once for all of them */
 #define EMIT_K 2
-__global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre, int nmol,
+__global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre,
+                                                 const uint32_t *__restrict__ wcoarse,
                                                  const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
                                                  const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
                                                  const float *__restrict__ F_all, const uint32_t *__restrict__ list,
@@ -1707,7 +1715,7 @@ __global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols,
     for (int j = 0; j < EMIT_K; j++) {
         const uint32_t k = min(k0 + j, nact - 1);
         const uint32_t wi = __builtin_amdgcn_readfirstlane(list[k]);
-        emit_setup(E[j], mols, wpre, nmol, N_all, Zm_all, S4_all, vbw_all, F_all, cidx_all, tbase_all, wi);
+        emit_setup(E[j], mols, wpre, wcoarse, N_all, Zm_all, S4_all, vbw_all, F_all, cidx_all, tbase_all, wi);
         if (k0 + j < nact) load_hood8(hoods[wid][j], E[j].V, E[j].x, E[j].y, E[j].w, lane);
     }
     wave_lds_sync();
@@ -1719,14 +1727,14 @@ __global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols,
 /* ------------------------------------------------------------------------------------------ */
 /* closest atom per vertex (computeClosestAtom Kernels.cu:547-604)                              */
 __global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
-                                                      int nmol, const uint32_t *__restrict__ cstart_all,
+                                                      const uint32_t *__restrict__ vcoarse, const uint32_t *__restrict__ cstart_all,
                                                       const float4 *__restrict__ sx, const int *__restrict__ perm,
                                                       const float *__restrict__ vert, uint32_t nv, int sorted_ids,
                                                       int *__restrict__ out)
 {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
-    const QsMol &M = mols[seg_of_wave(vpre, nmol, v)];
+    const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
     const QsGrid G = M.G;
     const uint32_t *__restrict__ cstart = cstart_all + M.cell_off;
     float px = vert[3 * (size_t)v], py = vert[3 * (size_t)v + 1], pz = vert[3 * (size_t)v + 2];
@@ -1790,7 +1798,8 @@ __device__ __forceinline__ void push_pair(const float *__restrict__ src, uint32_
 /* one Jacobi pass, one thread per vertex.  Polygonised cubes carry their case (cidx) and first
    triangle id (tbase), so each incident triangle's other corners are read from the triangle
    array. */
-__global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre, int nmol,
+__global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
+                                                const uint32_t *__restrict__ vcoarse,
                                                 const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
                                                 const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
                                                 const uint64_t *__restrict__ vkey, uint32_t nv,
@@ -1800,7 +1809,7 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
-    const QsMol &M = mols[seg_of_wave(vpre, nmol, v)];
+    const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
     const QsGrid G = M.G;
     const int g = G.g, lim = G.lim;
     MCView V;
@@ -2087,29 +2096,30 @@ void qs_launch_mc_active(const uint32_t *vcnt, const uint32_t *tcnt, const uint3
     hipLaunchKernelGGL(k_mc_active, dim3(nblk(nw, 256)), dim3(256), 0, s, vcnt, tcnt, apos, nw, list);
 }
 
-void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, int nmol, const uint64_t *N, const uint64_t *Zm,
+void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint64_t *N, const uint64_t *Zm,
                        const uint64_t *S4, const uint32_t *vbw, const float *F, const uint32_t *list, uint32_t nact,
                        const uint32_t *tbw, float *vert, uint64_t *vkey, int *tri, uint8_t *cidx, uint32_t *tbase, hipStream_t s)
 {
     if (!nact) return;
-    hipLaunchKernelGGL(k_mc_emit, dim3(nblk(nact, 4 * EMIT_K)), dim3(256), 0, s, mols, wpre, nmol, N, Zm, S4, vbw, F, list,
+    hipLaunchKernelGGL(k_mc_emit, dim3(nblk(nact, 4 * EMIT_K)), dim3(256), 0, s, mols, wpre, wcoarse, N, Zm, S4, vbw, F, list,
                        nact, tbw, vert, vkey, tri, cidx, tbase);
 }
 
-void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, int nmol, const uint64_t *N, const uint64_t *Zm,
+void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
                       const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
                       const uint32_t *tbase, const int *tri, const float *src, float *dst, hipStream_t s)
 {
     if (!nv) return;
-    hipLaunchKernelGGL(k_smooth, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, nmol, N, Zm, S4, vbw, vkey, nv, cidx, tbase,
+    hipLaunchKernelGGL(k_smooth, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, N, Zm, S4, vbw, vkey, nv, cidx, tbase,
                        tri, src, dst);
 }
 
-void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, int nmol, const uint32_t *cstart, const float4 *sx,
+void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint32_t *cstart,
+                            const float4 *sx,
                             const int *perm, const float *vert, uint32_t nv, int sorted_ids, int *out, hipStream_t s)
 {
     if (!nv) return;
-    hipLaunchKernelGGL(k_closest_atom, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, nmol, cstart, sx, perm, vert, nv,
+    hipLaunchKernelGGL(k_closest_atom, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, cstart, sx, perm, vert, nv,
                        sorted_ids, out);
 }
 
