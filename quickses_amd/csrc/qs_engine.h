@@ -167,7 +167,9 @@ struct P2Group {
 
 struct Engine {
     int device = -1;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr, side = nullptr;
+    hipEvent_t ev_emit = nullptr, ev_side = nullptr;
+    bool side_pending = false; /* closest-atom work on `side` not yet joined into `stream` */
 
     DevBuf in_pos, in_rad, d_off, bb_part, bb_out, d_mols, d_meta;
     DevBuf key, count, start, cursor, perm, cse, sx, aux;
@@ -209,6 +211,11 @@ struct Engine {
         groups.clear();
         if (stream) hipStreamDestroy(stream);
         stream = nullptr;
+        if (side) hipStreamDestroy(side);
+        side = nullptr;
+        if (ev_emit) hipEventDestroy(ev_emit);
+        if (ev_side) hipEventDestroy(ev_side);
+        ev_emit = ev_side = nullptr;
         device = -1;
         batch_valid = false;
     }
@@ -220,6 +227,10 @@ struct Engine {
         if (device == dev && stream) return 0;
         device = dev;
         QS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        /* a second stream (default priority) for the closest-atom pass, which overlaps smoothing */
+        QS_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        QS_HIP(hipEventCreateWithFlags(&ev_emit, hipEventDisableTiming));
+        QS_HIP(hipEventCreateWithFlags(&ev_side, hipEventDisableTiming));
         return 0;
     }
 

@@ -462,9 +462,21 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     tm.mark("mc_emit");
     if (with_atoms) {
         QS_TRY(ensure(atom, (size_t)std::max<uint32_t>(nv, 1) * 4));
+        /* the closest-atom pass only reads the unsmoothed vertices: without stage timing it runs
+           on the side stream, overlapping the smoothing passes, and is joined before the filter */
+        const bool overlap = !tm.on;
+        hipStream_t cs = overlap ? side : stream;
+        if (overlap) {
+            QS_HIP(hipEventRecord(ev_emit, stream));
+            QS_HIP(hipStreamWaitEvent(side, ev_emit, 0));
+        }
         qs_launch_closest_atom(as<QsMol>(d_mols), d_vpre, d_vpre + nmol + 1, as<uint32_t>(start), as<float4>(sx), as<int>(perm), as<float>(vert),
-                               nv, g_opt.atom_ids_sorted, as<int>(atom), stream);
+                               nv, g_opt.atom_ids_sorted, as<int>(atom), cs);
         QS_HIP(hipGetLastError());
+        if (overlap) {
+            QS_HIP(hipEventRecord(ev_side, side));
+            side_pending = true;
+        }
         tm.mark("closest_atom");
     }
     return 0;
@@ -575,6 +587,12 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
         QS_HIP(hipGetLastError());
         res_vert = (float *)src;
         tm.mark("smooth");
+    }
+
+    /* the closest-atom pass joins the main stream */
+    if (side_pending) {
+        QS_HIP(hipStreamWaitEvent(stream, ev_side, 0));
+        side_pending = false;
     }
 
     /* ---- degenerate filter (CudaSurf.cu:820-829), per-molecule kept counts ---- */
