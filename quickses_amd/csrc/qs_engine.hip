@@ -158,7 +158,7 @@ int Engine::reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts
     cells += 1;
     verts += 1;
     for (DevBuf *b : {&outm, &shellm, &negm, &zerom}) QS_TRY(ensure(*b, nw * 8));
-    for (DevBuf *b : {&vcnt, &tcnt, &vbw, &tbw, &apos, &alist}) QS_TRY(ensure(*b, nw * 4));
+    for (DevBuf *b : {&vcnt, &tcnt, &vbw, &tbw, &alist}) QS_TRY(ensure(*b, nw * 4));
     QS_TRY(ensure(s4, nw * 32));
     QS_TRY(ensure(field, vox * 4));
     QS_TRY(ensure(tbase, vox * 4));
@@ -389,18 +389,19 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_TRY(ensure(tcnt, nw * 4));
     QS_TRY(ensure(vbw, nw * 4));
     QS_TRY(ensure(tbw, nw * 4));
-    QS_TRY(ensure(apos, nw * 4));
     QS_TRY(ensure(alist, nw * 4));
     QS_TRY(ensure(totals, 16));
     if (nb_mc)
         qs_launch_mc_count(batch(at_mc, 0, nmol), nb_mc, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
                            as<uint32_t>(vcnt), as<uint32_t>(tcnt), stream);
-    QS_TRY(scan(as<uint32_t>(tcnt), as<uint32_t>(tbw), nw_total, as<uint32_t>(totals)));
-    QS_TRY(scan(as<uint32_t>(vcnt), as<uint32_t>(vbw), nw_total, as<uint32_t>(totals) + 1));
-    if (nw_total) qs_launch_mc_flag(as<uint32_t>(vcnt), as<uint32_t>(tcnt), nw_total, as<uint32_t>(apos), stream);
-    QS_TRY(scan(as<uint32_t>(apos), as<uint32_t>(apos), nw_total, as<uint32_t>(totals) + 2));
-    if (nw_total) qs_launch_mc_active(as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint32_t>(apos), nw_total, as<uint32_t>(alist),
-                                      stream);
+    if (nw_total) {
+        QS_TRY(ensure(scan_tmp, 3 * qs_scan_tiles(nw_total) * sizeof(uint32_t)));
+        qs_launch_mc_scans(as<uint32_t>(vcnt), as<uint32_t>(tcnt), nw_total, as<uint32_t>(scan_tmp), as<uint32_t>(vbw),
+                           as<uint32_t>(tbw), as<uint32_t>(alist), as<uint32_t>(totals), stream);
+        QS_HIP(hipGetLastError());
+    } else {
+        QS_HIP(hipMemsetAsync(totals.p, 0, 12, stream));
+    }
     /* per-molecule first vertex and triangle: the scans at each molecule's first word */
     std::vector<uint64_t> idx(nmol);
     for (int m = 0; m < nmol; m++) idx[m] = std::min<uint64_t>(mols[m].word_off, nw_total ? nw_total - 1 : 0);

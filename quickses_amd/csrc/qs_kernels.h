@@ -71,6 +71,10 @@ void qs_launch_tile_reduce(const uint32_t *in, uint64_t n, uint32_t *sums, hipSt
 void qs_launch_tile_scan(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *offsets, uint32_t *total,
                          hipStream_t s);
 void qs_launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
+/* marching-cubes word scans: vbw, tbw, the active-word list and totals {triangles, vertices,
+   active words}; sums holds 3 * qs_scan_tiles(nw) entries */
+void qs_launch_mc_scans(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
+                        uint32_t *tbw, uint32_t *list, uint32_t *totals, hipStream_t s);
 void qs_launch_gather_u32(const uint32_t *src, const uint64_t *idx, int n, uint32_t *out, hipStream_t s);
 
 #endif

@@ -227,6 +227,98 @@ __global__ void __launch_bounds__(SCAN_NT) k_tile_scan(const uint32_t *in, uint3
     if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = off + tot;
 }
 
+/* Marching-cubes word scans in one pass: per row word its vertex count, triangle count and
+   "active" flag (either nonzero) are scanned together.  Plane 0 = triangles, 1 = vertices,
+   2 = active words; tile sums are stored plane-major (3 x ntiles). */
+__global__ void __launch_bounds__(SCAN_NT) k_mc_reduce3(const uint32_t *__restrict__ vcnt, const uint32_t *__restrict__ tcnt,
+                                                        uint64_t n, uint32_t *__restrict__ sums, uint64_t ntiles)
+{
+    __shared__ uint32_t lds[SCAN_NT / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint32_t sv = 0, st = 0, sa = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        const uint64_t i = base + (uint64_t)k * SCAN_NT + threadIdx.x;
+        if (i < n) {
+            const uint32_t v = vcnt[i], t = tcnt[i];
+            sv += v;
+            st += t;
+            sa += (v | t) ? 1u : 0u;
+        }
+    }
+    const uint32_t tt = block_sum<SCAN_NT>(st, lds);
+    const uint32_t tv = block_sum<SCAN_NT>(sv, lds);
+    const uint32_t ta = block_sum<SCAN_NT>(sa, lds);
+    if (threadIdx.x == 0) {
+        sums[blockIdx.x] = tt;
+        sums[ntiles + blockIdx.x] = tv;
+        sums[2 * ntiles + blockIdx.x] = ta;
+    }
+}
+
+/* block p: exclusive scan of plane p of the tile sums in place, total to totals[p] */
+__global__ void __launch_bounds__(SCAN_NT) k_scan_planes(uint32_t *__restrict__ sums, uint64_t ntiles,
+                                                         uint32_t *__restrict__ totals)
+{
+    __shared__ uint32_t lds[SCAN_NT / 64 + 1];
+    uint32_t *plane = sums + (uint64_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (uint64_t b0 = 0; b0 < ntiles; b0 += SCAN_TILE) {
+        const uint64_t base = b0 + (uint64_t)threadIdx.x * SCAN_IPT;
+        uint32_t v[SCAN_IPT], s = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_IPT; k++) {
+            v[k] = (base + k < ntiles) ? plane[base + k] : 0u;
+            s += v[k];
+        }
+        uint32_t tot;
+        uint32_t run = carry + block_excl_scan<SCAN_NT>(s, &tot, lds);
+#pragma unroll
+        for (int k = 0; k < SCAN_IPT; k++) {
+            if (base + k < ntiles) plane[base + k] = run;
+            run += v[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+/* per word: first vertex (vbw) and first triangle (tbw); active words listed in order */
+__global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict__ vcnt, const uint32_t *__restrict__ tcnt,
+                                                      uint64_t n, const uint32_t *__restrict__ sums, uint64_t ntiles,
+                                                      uint32_t *__restrict__ vbw, uint32_t *__restrict__ tbw,
+                                                      uint32_t *__restrict__ list)
+{
+    __shared__ uint32_t lds[SCAN_NT / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_IPT;
+    uint32_t v[SCAN_IPT], t[SCAN_IPT];
+    uint32_t sv = 0, st = 0, sa = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        const uint64_t i = base + k;
+        v[k] = (i < n) ? vcnt[i] : 0u;
+        t[k] = (i < n) ? tcnt[i] : 0u;
+        sv += v[k];
+        st += t[k];
+        sa += (v[k] | t[k]) ? 1u : 0u;
+    }
+    uint32_t tot;
+    uint32_t rt = sums[blockIdx.x] + block_excl_scan<SCAN_NT>(st, &tot, lds);
+    uint32_t rv = sums[ntiles + blockIdx.x] + block_excl_scan<SCAN_NT>(sv, &tot, lds);
+    uint32_t ra = sums[2 * ntiles + blockIdx.x] + block_excl_scan<SCAN_NT>(sa, &tot, lds);
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        const uint64_t i = base + k;
+        if (i < n) {
+            vbw[i] = rv;
+            tbw[i] = rt;
+            if (v[k] | t[k]) list[ra++] = (uint32_t)i;
+        }
+        rv += v[k];
+        rt += t[k];
+    }
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* bounding box (getMinMax CudaSurf.cu:149-174): partial boxes per molecule, then one wave    */
 
@@ -2157,6 +2249,16 @@ void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *pos, int
 }
 
 /* scan helpers */
+void qs_launch_mc_scans(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
+                        uint32_t *tbw, uint32_t *list, uint32_t *totals, hipStream_t s)
+{
+    const uint64_t ntiles = qs_scan_tiles(nw);
+    if (!nw) return;
+    hipLaunchKernelGGL(k_mc_reduce3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles);
+    hipLaunchKernelGGL(k_scan_planes, dim3(3), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
+    hipLaunchKernelGGL(k_mc_scan3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles, vbw, tbw, list);
+}
+
 uint64_t qs_scan_tiles(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
 void qs_launch_tile_reduce(const uint32_t *in, uint64_t n, uint32_t *sums, hipStream_t s)
