@@ -43,9 +43,6 @@ void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *ou
 uint32_t qs_mc_count_blocks(QsGrid G, int W);
 void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
                         uint32_t *tcnt, hipStream_t s);
-void qs_launch_mc_flag(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *flag, hipStream_t s);
-void qs_launch_mc_active(const uint32_t *vcnt, const uint32_t *tcnt, const uint32_t *apos, uint64_t nw, uint32_t *list,
-                         hipStream_t s);
 void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint64_t *N, const uint64_t *Zm,
                        const uint64_t *S4, const uint32_t *vbw, const float *F, const uint32_t *list, uint32_t nact,
                        const uint32_t *tbw, float *vert, uint64_t *vkey, int *tri, uint8_t *cidx, uint32_t *tbase, hipStream_t s);

@@ -129,17 +129,6 @@ __device__ __forceinline__ int seg_of_wave(const uint32_t *__restrict__ pre, con
     return seg_coarse<QS_COARSE_ITEMS>(pre, coarse, v);
 }
 
-__device__ __forceinline__ int seg_of64(const uint64_t *__restrict__ pre, int n, uint64_t v)
-{
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= v) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 /* exclusive scan over the block; returns this thread's prefix, *total the block sum */
 template <int NT>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total, uint32_t *lds)
@@ -483,14 +472,6 @@ __device__ __forceinline__ uint32_t wave_or_u32(uint32_t v)
 __device__ __forceinline__ uint64_t wave_or_u64(uint64_t v)
 {
     return ((uint64_t)wave_or_u32((uint32_t)(v >> 32)) << 32) | wave_or_u32((uint32_t)v);
-}
-
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
-{
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-    lo = __shfl_xor(lo, m, 64);
-    hi = __shfl_xor(hi, m, 64);
-    return ((uint64_t)hi << 32) | lo;
 }
 
 /* bits [a, b] (inclusive, 0 <= a, b <= 63) */
@@ -1400,16 +1381,6 @@ __device__ __forceinline__ float fat(const float *__restrict__ f, int g, int x, 
     return f[((size_t)x * g + y) * g + z];
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
-{
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(v, d, 64);
-        if (lane >= d) v += y;
-    }
-    return v;
-}
-
 struct MCView {
     const uint64_t *__restrict__ N;
     const uint64_t *__restrict__ Zm;
@@ -1558,44 +1529,14 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
     tcnt[wi] = nt;
 }
 
-/* list of row words that own a vertex or a cube with triangles */
-__global__ void __launch_bounds__(256) k_mc_active(const uint32_t *__restrict__ vcnt, const uint32_t *__restrict__ tcnt,
-                                                   const uint32_t *__restrict__ apos, uint64_t nw, uint32_t *__restrict__ list)
-{
-    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= nw) return;
-    if (vcnt[i] | tcnt[i]) list[apos[i]] = (uint32_t)i;
-}
-
-__global__ void __launch_bounds__(256) k_mc_flag(const uint32_t *__restrict__ vcnt, const uint32_t *__restrict__ tcnt,
-                                                 uint64_t nw, uint32_t *__restrict__ flag)
-{
-    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= nw) return;
-    flag[i] = (vcnt[i] | tcnt[i]) ? 1u : 0u;
-}
-
-/* The 3 x 3 x 3 row words around a row word (rows x-1..x+1, y-1..y+1, words w-1..w+1): sign,
-   zero and slot masks and first vertex ids, staged in LDS once per wavefront so that cube
-   indices and corner vertex ids are LDS lookups. */
+/* Row words around a row word, indexed as a 3 x 3 x 3 block (rows x-1..x+1, y-1..y+1, words
+   w-1..w+1): sign, zero and slot masks and first vertex ids.  Emission stages the 2 x 2 x 2
+   words its cubes touch (x..x+1, y..y+1, w..w+1) in LDS, so that cube indices and corner
+   vertex ids are LDS lookups. */
 struct Hood {
     uint64_t N[27], Z[27], S[27][4];
     uint32_t vb[27];
 };
-
-__device__ __forceinline__ void load_hood(Hood &H, const MCView &V, int x, int y, int w, int lane)
-{
-    for (int t = lane; t < 27 * 7; t += 64) {
-        const int e = t / 7, f = t - 7 * (t / 7);
-        const int xx = x + e / 9 - 1, yy = y + (e / 3) % 3 - 1, ww = w + e % 3 - 1;
-        const bool ok = xx >= 0 && yy >= 0 && xx < V.g && yy < V.g && ww >= 0 && ww < V.W;
-        const size_t wi = ok ? (((size_t)xx * V.g + yy) * V.W + ww) : 0;
-        if (f == 0) H.N[e] = ok ? V.N[wi] : 0ull;
-        else if (f == 1) H.Z[e] = ok ? V.Zm[wi] : 0ull;
-        else if (f < 6) H.S[e][f - 2] = ok ? V.S4[4 * wi + (f - 2)] : 0ull;
-        else H.vb[e] = ok ? V.vbw[wi] : 0u;
-    }
-}
 
 struct HoodView {
     const Hood *H;
@@ -2175,17 +2116,6 @@ void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const ui
                         uint32_t *tcnt, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(256), 0, s, B, N, Zm, S4, vcnt, tcnt);
-}
-
-void qs_launch_mc_flag(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *flag, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_mc_flag, dim3(nblk(nw, 256)), dim3(256), 0, s, vcnt, tcnt, nw, flag);
-}
-
-void qs_launch_mc_active(const uint32_t *vcnt, const uint32_t *tcnt, const uint32_t *apos, uint64_t nw, uint32_t *list,
-                         hipStream_t s)
-{
-    hipLaunchKernelGGL(k_mc_active, dim3(nblk(nw, 256)), dim3(256), 0, s, vcnt, tcnt, apos, nw, list);
 }
 
 void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint64_t *N, const uint64_t *Zm,
