@@ -113,3 +113,25 @@ def test_batch_getters_agree(qs):
     for m in range(len(mols)):
         assert vo[m + 1] - vo[m] == len(out[m][0]) and to[m + 1] - to[m] == len(out[m][1])
         _same_mesh(qs.batch_mesh(m), out[m])
+
+
+@pytest.mark.parametrize("opt", ["lim_mode", "atom_ids_sorted", "pass1_direct", "pass2_direct"])
+def test_batch_options_match_oracle(qs, orc, opt):
+    """Each option in a batch: the per-voxel cross-check kernels and the reference's window and
+    atom-id conventions give every molecule the oracle's mesh."""
+    mols = [qs.synth_ball(800 + i, n) for i, n in enumerate([300, 700])]
+    reso = [0.5, 1.0]
+    old = qs.get_option(opt)
+    qs.set_option(opt, 1)
+    try:
+        print("case batch option %s" % opt, flush=True)
+        out = qs.compute_batch(mols, reso, 1)
+    finally:
+        qs.set_option(opt, old)
+    kw = {}
+    if opt == "lim_mode":
+        kw["lim_mode"] = 1
+    if opt == "atom_ids_sorted":
+        kw["atom_ids_sorted"] = True
+    for m, ((pos, rad), r) in enumerate(zip(mols, reso)):
+        _same_mesh(out[m], orc.compute_ses(pos, rad, r, 1, **kw))
