@@ -68,7 +68,7 @@ def copy_bandwidth(qs, hipmem, nbytes=1 << 30, reps=3):
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline(pos, rad, voxel, target_s=12.0):
+def cpu_baseline(pos, rad, voxel, target_s=25.0):
     from oracle import oracle  # CPU baseline leg only
 
     t, g, v = oracle.bench_slab(pos, rad, voxel, 4)
