@@ -583,22 +583,24 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsBatch B, const uint32_t *_
     const uint32_t *__restrict__ cstart = cstart_all + M.cell_off;
     uint64_t *__restrict__ outm = outm_all + M.word_off;
     uint64_t *__restrict__ shellm = shellm_all + M.word_off;
+    /* one wavefront per group of R rows, looping over the rows' W words: the group's setup
+       (molecule, rows, x and y cells) is paid once */
     const int nby = (g + 4 * R - 1) / (4 * R);
-    const int w = (int)(lb % (uint32_t)W);
-    const int by = (int)((lb / (uint32_t)W) % (uint32_t)nby);
-    const int x = (int)(lb / ((uint32_t)W * nby));
+    const int by = (int)(lb % (uint32_t)nby);
+    const int x = (int)(lb / (uint32_t)nby);
     const int x1 = g;
     const int y0 = (by * 4 + wid) * R;
     if (y0 >= g || x >= x1) return; /* whole wavefront */
     const int ny = min(R, g - y0);
-    const int z0 = w * 64;
-    const int nvalid = min(64, g - z0);
-    const uint64_t valid = (nvalid >= 64) ? ~0ull : ((1ull << nvalid) - 1ull);
     if (x >= g - 1) {
-        if (lane < ny) {
-            size_t widx = ((size_t)x * g + y0 + lane) * W + w;
-            outm[widx] = valid;
-            shellm[widx] = 0ull;
+        for (int w = 0; w < W; w++) {
+            const int nvalid = min(64, g - w * 64);
+            const uint64_t valid = (nvalid >= 64) ? ~0ull : ((1ull << nvalid) - 1ull);
+            if (lane < ny) {
+                size_t widx = ((size_t)x * g + y0 + lane) * W + w;
+                outm[widx] = valid;
+                shellm[widx] = 0ull;
+            }
         }
         return;
     }
@@ -615,180 +617,186 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsBatch B, const uint32_t *_
         cyl[k] = iclamp(cy - 1, 0, gn - 1);
         cyh[k] = iclamp(cy + 1, 0, gn - 1);
     }
-    /* this lane's voxel z cell; the wave's cells are monotone in the lane */
-    const int zl = min(z0 + lane, g - 1);
-    const int cz = cell_of(grid_pos(G.oz, zl, dx), G.oz, G.dxn);
-    const int czmin = __builtin_amdgcn_readfirstlane(cz), czmax = __builtin_amdgcn_readlane(cz, 63);
-    const int D = czmax - czmin + 1; /* distinct z cells + 1 */
-    /* tab (lane j) = number of lanes whose z cell is < czmin + j, j = 0..D */
-    int tab = (lane >= D) ? 64 : 0;
-    if (D < 64) {
-        for (int d = 1; d < D; d++) {
-            uint64_t m = __ballot(cz < czmin + d);
-            if (lane == d) tab = __popcll(m);
-        }
-    }
-    const int zc0 = iclamp(czmin - 1, 0, gn - 1), zc1 = iclamp(czmax + 1, 0, gn - 1);
-    const int xa = iclamp(cx - 1, 0, gn - 1), xb = iclamp(cx + 1, 0, gn - 1);
-    const int ya = cyl[0], yb = cyh[P1_RMAX - 1];
-    /* candidate atoms: up to 3 x 4 contiguous ranges of the cell-ordered array */
-    int rs[12], pre[13];
-    pre[0] = 0;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            int k = 4 * i + j;
-            int xc = xa + i, yc = ya + j;
-            int len = 0, st = 0;
-            if (xc <= xb && yc <= yb) {
-                int h0 = (xc * gn + yc) * gn + zc0;
-                st = (int)cstart[h0];
-                len = (int)cstart[h0 + (zc1 - zc0) + 1] - st;
-            }
-            rs[k] = st;
-            pre[k + 1] = pre[k] + len;
-        }
-    }
-    const int total = pre[12];
-    const int zmax = min(z0 + 63, g - 2); /* labelled voxels: z <= g-2 (z = g-1 is outside) */
-    uint64_t m_in[P1_RMAX], m_near[P1_RMAX];
-#pragma unroll
-    for (int k = 0; k < P1_RMAX; k++) m_in[k] = m_near[k] = 0ull;
-
-    /* interval work for up to 64 queued atoms, one per lane */
-    auto process = [&](int count) {
-        const int id = (lane < count) ? queue[lane] : -1;
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), ax = make_float4(0.f, 0.f, 0.f, 0.f);
-        int acz = czmin, acy = 0;
-        if (id >= 0) {
-            a = sx[id];
-            ax = aux[id];
-            const int packed = __float_as_int(ax.z);
-            acy = packed >> 16;
-            acz = packed & 0xFFFF;
-        }
-        /* voxels whose z cell admits the atom's z cell: lanes [La, Ra] */
-        int La, Ra;
+    for (int w = 0; w < W; w++) {
+        const int z0 = w * 64;
+        const int nvalid = min(64, g - z0);
+        const uint64_t valid = (nvalid >= 64) ? ~0ull : ((1ull << nvalid) - 1ull);
+        wave_lds_sync(); /* the previous word's queue reads are done */
+        /* this lane's voxel z cell; the wave's cells are monotone in the lane */
+        const int zl = min(z0 + lane, g - 1);
+        const int cz = cell_of(grid_pos(G.oz, zl, dx), G.oz, G.dxn);
+        const int czmin = __builtin_amdgcn_readfirstlane(cz), czmax = __builtin_amdgcn_readlane(cz, 63);
+        const int D = czmax - czmin + 1; /* distinct z cells + 1 */
+        /* tab (lane j) = number of lanes whose z cell is < czmin + j, j = 0..D */
+        int tab = (lane >= D) ? 64 : 0;
         if (D < 64) {
-            int j1 = acz - 1 - czmin, j2 = acz + 2 - czmin;
-            int t1 = __shfl(tab, iclamp(j1, 0, 63), 64);
-            int t2 = __shfl(tab, iclamp(j2, 0, 63), 64);
-            La = (j1 <= 0) ? 0 : (j1 >= D ? 64 : t1);
-            Ra = ((j2 >= D) || (acz == gn - 1)) ? 63 : ((j2 <= 0) ? -1 : t2 - 1);
-        } else {
-            const int alo = iclamp(cz - 1, 0, gn - 1), ahi = iclamp(cz + 1, 0, gn - 1);
-            int lo_i = 0, hi_i = 64;
-            for (int it = 0; it < 7; it++) {
-                int mid = (lo_i + hi_i) >> 1;
-                int v = __shfl(ahi, min(mid, 63), 64);
-                if (lo_i < hi_i) {
-                    if (v >= acz) hi_i = mid;
-                    else lo_i = mid + 1;
-                }
+            for (int d = 1; d < D; d++) {
+                uint64_t m = __ballot(cz < czmin + d);
+                if (lane == d) tab = __popcll(m);
             }
-            La = lo_i;
-            lo_i = 0;
-            hi_i = 64;
-            for (int it = 0; it < 7; it++) {
-                int mid = (lo_i + hi_i) >> 1;
-                int v = __shfl(alo, min(mid, 63), 64);
-                if (lo_i < hi_i) {
-                    if (v > acz) hi_i = mid;
-                    else lo_i = mid + 1;
+        }
+        const int zc0 = iclamp(czmin - 1, 0, gn - 1), zc1 = iclamp(czmax + 1, 0, gn - 1);
+        const int xa = iclamp(cx - 1, 0, gn - 1), xb = iclamp(cx + 1, 0, gn - 1);
+        const int ya = cyl[0], yb = cyh[P1_RMAX - 1];
+        /* candidate atoms: up to 3 x 4 contiguous ranges of the cell-ordered array */
+        int rs[12], pre[13];
+        pre[0] = 0;
+    #pragma unroll
+        for (int i = 0; i < 3; i++) {
+    #pragma unroll
+            for (int j = 0; j < 4; j++) {
+                int k = 4 * i + j;
+                int xc = xa + i, yc = ya + j;
+                int len = 0, st = 0;
+                if (xc <= xb && yc <= yb) {
+                    int h0 = (xc * gn + yc) * gn + zc0;
+                    st = (int)cstart[h0];
+                    len = (int)cstart[h0 + (zc1 - zc0) + 1] - st;
                 }
+                rs[k] = st;
+                pre[k + 1] = pre[k] + len;
             }
-            Ra = lo_i - 1;
         }
-        if (id < 0) return;
-        const int zlo = z0 + La, zhi = min(z0 + Ra, zmax);
-        if (zlo > zhi) return;
-        const float sN = ax.x, sD = ax.y;
-        const float ex = a.x - px;
-        const float X = ex * ex;
-        const float zc = (a.z - G.oz) * inv_dx;
-        RowAtom A;
-        A.az = a.z;
-        A.dx = dx;
-        A.oz = G.oz;
-        const int zb0 = iclamp((int)floorf(zc), zlo, zhi);
-#pragma unroll
-        for (int k = 0; k < P1_RMAX; k++) {
-            if (k >= ny) break;
-            if (acy < cyl[k] || acy > cyh[k]) continue; /* row's y cell does not admit the atom */
-            float ey = a.y - py[k];
-            float Y = ey * ey;
-            A.S = X + Y;
-            if (!(A.S < sN)) continue;
-            int zb = zb0;
-            if (zb + 1 <= zhi && A.Zf(zb + 1) < A.Zf(zb)) zb = zb + 1;
-            int L, Rr;
-            row_interval(A, sN, zlo, zhi, zb, zc, inv_dx, L, Rr);
-            if (L > Rr) continue;
-            m_near[k] |= bit_range(L - z0, Rr - z0);
-            if (!(A.S < sD)) continue;
-            row_interval(A, sD, L, Rr, zb, zc, inv_dx, L, Rr);
-            if (L <= Rr) m_in[k] |= bit_range(L - z0, Rr - z0);
-        }
-    };
+        const int total = pre[12];
+        const int zmax = min(z0 + 63, g - 2); /* labelled voxels: z <= g-2 (z = g-1 is outside) */
+        uint64_t m_in[P1_RMAX], m_near[P1_RMAX];
+    #pragma unroll
+        for (int k = 0; k < P1_RMAX; k++) m_in[k] = m_near[k] = 0ull;
 
-    /* candidates that can reach one of the rows (xy distance below r + probe) are queued */
-    int qn = 0;
-    const float pymin = py[0], pymax = py[ny - 1];
-    for (int base = 0; base < total; base += 64) {
-        const int kk = base + lane;
-        int id = -1;
-#pragma unroll
-        for (int q = 0; q < 12; q++)
-            if (kk >= pre[q] && kk < pre[q + 1]) id = rs[q] + (kk - pre[q]);
-        bool keep = false;
-        if (id >= 0) {
-            const float4 a = sx[id];
-            const float sN = aux[id].x;
+        /* interval work for up to 64 queued atoms, one per lane */
+        auto process = [&](int count) {
+            const int id = (lane < count) ? queue[lane] : -1;
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), ax = make_float4(0.f, 0.f, 0.f, 0.f);
+            int acz = czmin, acy = 0;
+            if (id >= 0) {
+                a = sx[id];
+                ax = aux[id];
+                const int packed = __float_as_int(ax.z);
+                acy = packed >> 16;
+                acz = packed & 0xFFFF;
+            }
+            /* voxels whose z cell admits the atom's z cell: lanes [La, Ra] */
+            int La, Ra;
+            if (D < 64) {
+                int j1 = acz - 1 - czmin, j2 = acz + 2 - czmin;
+                int t1 = __shfl(tab, iclamp(j1, 0, 63), 64);
+                int t2 = __shfl(tab, iclamp(j2, 0, 63), 64);
+                La = (j1 <= 0) ? 0 : (j1 >= D ? 64 : t1);
+                Ra = ((j2 >= D) || (acz == gn - 1)) ? 63 : ((j2 <= 0) ? -1 : t2 - 1);
+            } else {
+                const int alo = iclamp(cz - 1, 0, gn - 1), ahi = iclamp(cz + 1, 0, gn - 1);
+                int lo_i = 0, hi_i = 64;
+                for (int it = 0; it < 7; it++) {
+                    int mid = (lo_i + hi_i) >> 1;
+                    int v = __shfl(ahi, min(mid, 63), 64);
+                    if (lo_i < hi_i) {
+                        if (v >= acz) hi_i = mid;
+                        else lo_i = mid + 1;
+                    }
+                }
+                La = lo_i;
+                lo_i = 0;
+                hi_i = 64;
+                for (int it = 0; it < 7; it++) {
+                    int mid = (lo_i + hi_i) >> 1;
+                    int v = __shfl(alo, min(mid, 63), 64);
+                    if (lo_i < hi_i) {
+                        if (v > acz) hi_i = mid;
+                        else lo_i = mid + 1;
+                    }
+                }
+                Ra = lo_i - 1;
+            }
+            if (id < 0) return;
+            const int zlo = z0 + La, zhi = min(z0 + Ra, zmax);
+            if (zlo > zhi) return;
+            const float sN = ax.x, sD = ax.y;
             const float ex = a.x - px;
             const float X = ex * ex;
-            const float yy = fminf(fmaxf(a.y, pymin), pymax); /* nearest row y */
-            const float ey = a.y - yy;
-            const float Y = ey * ey;
-            keep = (X + Y) < sN * 1.0001f + 1e-6f; /* conservative: exact tests follow */
-        }
-        const uint64_t km = __ballot(keep);
-        if (keep) queue[qn + __popcll(km & ((1ull << lane) - 1ull))] = id;
-        qn += __popcll(km);
-        wave_lds_sync();
-        if (qn >= 64) {
-            process(64);
-            wave_lds_sync();
-            const int rest = qn - 64;
-            int moved = (lane < rest) ? queue[64 + lane] : 0;
-            wave_lds_sync();
-            if (lane < rest) queue[lane] = moved;
-            wave_lds_sync();
-            qn = rest;
-        }
-    }
-    if (qn > 0) process(qn);
-
-#pragma unroll
-    for (int k = 0; k < P1_RMAX; k++) {
-        m_in[k] = wave_or_u64(m_in[k]);
-        m_near[k] = wave_or_u64(m_near[k]);
-    }
-    if (lane < ny) {
-        uint64_t mi = 0ull, mn = 0ull;
-#pragma unroll
-        for (int k = 0; k < P1_RMAX; k++)
-            if (lane == k) {
-                mi = m_in[k];
-                mn = m_near[k];
+            const float zc = (a.z - G.oz) * inv_dx;
+            RowAtom A;
+            A.az = a.z;
+            A.dx = dx;
+            A.oz = G.oz;
+            const int zb0 = iclamp((int)floorf(zc), zlo, zhi);
+    #pragma unroll
+            for (int k = 0; k < P1_RMAX; k++) {
+                if (k >= ny) break;
+                if (acy < cyl[k] || acy > cyh[k]) continue; /* row's y cell does not admit the atom */
+                float ey = a.y - py[k];
+                float Y = ey * ey;
+                A.S = X + Y;
+                if (!(A.S < sN)) continue;
+                int zb = zb0;
+                if (zb + 1 <= zhi && A.Zf(zb + 1) < A.Zf(zb)) zb = zb + 1;
+                int L, Rr;
+                row_interval(A, sN, zlo, zhi, zb, zc, inv_dx, L, Rr);
+                if (L > Rr) continue;
+                m_near[k] |= bit_range(L - z0, Rr - z0);
+                if (!(A.S < sD)) continue;
+                row_interval(A, sD, L, Rr, zb, zc, inv_dx, L, Rr);
+                if (L <= Rr) m_in[k] |= bit_range(L - z0, Rr - z0);
             }
-        const int y = y0 + lane;
-        uint64_t lab = (zmax >= z0 && y < g - 1) ? bit_range(0, zmax - z0) : 0ull;
-        mi &= lab;
-        mn &= lab;
-        size_t widx = ((size_t)x * g + y) * W + w;
-        outm[widx] = valid & ~(mi | mn);
-        shellm[widx] = mn & ~mi;
+        };
+
+        /* candidates that can reach one of the rows (xy distance below r + probe) are queued */
+        int qn = 0;
+        const float pymin = py[0], pymax = py[ny - 1];
+        for (int base = 0; base < total; base += 64) {
+            const int kk = base + lane;
+            int id = -1;
+    #pragma unroll
+            for (int q = 0; q < 12; q++)
+                if (kk >= pre[q] && kk < pre[q + 1]) id = rs[q] + (kk - pre[q]);
+            bool keep = false;
+            if (id >= 0) {
+                const float4 a = sx[id];
+                const float sN = aux[id].x;
+                const float ex = a.x - px;
+                const float X = ex * ex;
+                const float yy = fminf(fmaxf(a.y, pymin), pymax); /* nearest row y */
+                const float ey = a.y - yy;
+                const float Y = ey * ey;
+                keep = (X + Y) < sN * 1.0001f + 1e-6f; /* conservative: exact tests follow */
+            }
+            const uint64_t km = __ballot(keep);
+            if (keep) queue[qn + __popcll(km & ((1ull << lane) - 1ull))] = id;
+            qn += __popcll(km);
+            wave_lds_sync();
+            if (qn >= 64) {
+                process(64);
+                wave_lds_sync();
+                const int rest = qn - 64;
+                int moved = (lane < rest) ? queue[64 + lane] : 0;
+                wave_lds_sync();
+                if (lane < rest) queue[lane] = moved;
+                wave_lds_sync();
+                qn = rest;
+            }
+        }
+        if (qn > 0) process(qn);
+
+    #pragma unroll
+        for (int k = 0; k < P1_RMAX; k++) {
+            m_in[k] = wave_or_u64(m_in[k]);
+            m_near[k] = wave_or_u64(m_near[k]);
+        }
+        if (lane < ny) {
+            uint64_t mi = 0ull, mn = 0ull;
+    #pragma unroll
+            for (int k = 0; k < P1_RMAX; k++)
+                if (lane == k) {
+                    mi = m_in[k];
+                    mn = m_near[k];
+                }
+            const int y = y0 + lane;
+            uint64_t lab = (zmax >= z0 && y < g - 1) ? bit_range(0, zmax - z0) : 0ull;
+            mi &= lab;
+            mn &= lab;
+            size_t widx = ((size_t)x * g + y) * W + w;
+            outm[widx] = valid & ~(mi | mn);
+            shellm[widx] = mn & ~mi;
+        }
     }
 }
 
@@ -2038,8 +2046,8 @@ int qs_pass1_rows_per_wave(QsGrid G)
 
 uint32_t qs_pass1_blocks(QsGrid G, int W, int R, bool direct)
 {
-    if (direct) R = 1;
-    return (uint32_t)W * nblk(G.g, 4 * R) * (uint32_t)G.g;
+    if (direct) return (uint32_t)W * nblk(G.g, 4) * (uint32_t)G.g;
+    return nblk(G.g, 4 * R) * (uint32_t)G.g;
 }
 
 void qs_launch_pass1(QsBatch B, uint32_t nblocks, const uint32_t *cstart, const float4 *sx, const float4 *aux, uint64_t *outm,
