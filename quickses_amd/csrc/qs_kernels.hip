@@ -627,14 +627,6 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsBatch B, const uint32_t *_
         const int cz = cell_of(grid_pos(G.oz, zl, dx), G.oz, G.dxn);
         const int czmin = __builtin_amdgcn_readfirstlane(cz), czmax = __builtin_amdgcn_readlane(cz, 63);
         const int D = czmax - czmin + 1; /* distinct z cells + 1 */
-        /* tab (lane j) = number of lanes whose z cell is < czmin + j, j = 0..D */
-        int tab = (lane >= D) ? 64 : 0;
-        if (D < 64) {
-            for (int d = 1; d < D; d++) {
-                uint64_t m = __ballot(cz < czmin + d);
-                if (lane == d) tab = __popcll(m);
-            }
-        }
         const int zc0 = iclamp(czmin - 1, 0, gn - 1), zc1 = iclamp(czmax + 1, 0, gn - 1);
         const int xa = iclamp(cx - 1, 0, gn - 1), xb = iclamp(cx + 1, 0, gn - 1);
         const int ya = cyl[0], yb = cyh[P1_RMAX - 1];
@@ -659,6 +651,23 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsBatch B, const uint32_t *_
         }
         const int total = pre[12];
         const int zmax = min(z0 + 63, g - 2); /* labelled voxels: z <= g-2 (z = g-1 is outside) */
+        if (total == 0) {
+            /* no atom can reach these rows: every voxel is outside */
+            if (lane < ny) {
+                const size_t widx = ((size_t)x * g + y0 + lane) * W + w;
+                outm[widx] = valid;
+                shellm[widx] = 0ull;
+            }
+            continue;
+        }
+        /* tab (lane j) = number of lanes whose z cell is < czmin + j, j = 0..D */
+        int tab = (lane >= D) ? 64 : 0;
+        if (D < 64) {
+            for (int d = 1; d < D; d++) {
+                uint64_t m = __ballot(cz < czmin + d);
+                if (lane == d) tab = __popcll(m);
+            }
+        }
         uint64_t m_in[P1_RMAX], m_near[P1_RMAX];
     #pragma unroll
         for (int k = 0; k < P1_RMAX; k++) m_in[k] = m_near[k] = 0ull;
