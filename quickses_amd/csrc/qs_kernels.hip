@@ -1546,13 +1546,12 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
     tcnt[wi] = nt;
 }
 
-/* Row words around a row word, indexed as a 3 x 3 x 3 block (rows x-1..x+1, y-1..y+1, words
-   w-1..w+1): sign, zero and slot masks and first vertex ids.  Emission stages the 2 x 2 x 2
-   words its cubes touch (x..x+1, y..y+1, w..w+1) in LDS, so that cube indices and corner
-   vertex ids are LDS lookups. */
+/* The 2 x 2 x 2 row words that a row word's cubes touch (rows x..x+1, y..y+1, words w..w+1):
+   sign, zero and slot masks and first vertex ids, staged in LDS by emission so that cube indices
+   and corner vertex ids are LDS lookups.  Entry ((px - x) * 2 + (py - y)) * 2 + (word - w). */
 struct Hood {
-    uint64_t N[27], Z[27], S[27][4];
-    uint32_t vb[27];
+    uint64_t N[8], Z[8], S[8][4];
+    uint32_t vb[8];
 };
 
 struct HoodView {
@@ -1560,7 +1559,7 @@ struct HoodView {
     int x, y, w;
     __device__ __forceinline__ int ent(int px, int py, int pz) const
     {
-        return ((px - x + 1) * 3 + (py - y + 1)) * 3 + ((pz >> 6) - w + 1);
+        return ((px - x) * 2 + (py - y)) * 2 + ((pz >> 6) - w);
     }
     __device__ __forceinline__ bool nbit(int px, int py, int pz) const { return (H->N[ent(px, py, pz)] >> (pz & 63)) & 1ull; }
     __device__ __forceinline__ bool zbit(int px, int py, int pz) const { return (H->Z[ent(px, py, pz)] >> (pz & 63)) & 1ull; }
@@ -1614,7 +1613,7 @@ __device__ __forceinline__ void load_hood8(Hood &H, const MCView &V, int x, int 
     if (lane < 8 * 7) {
         const int e8 = lane / 7, f = lane - 7 * (lane / 7);
         const int dx = e8 >> 2, dy = (e8 >> 1) & 1, dw = e8 & 1;
-        const int e = ((dx + 1) * 3 + (dy + 1)) * 3 + (dw + 1); /* slot in the 3x3x3 layout */
+        const int e = e8;
         const int xx = x + dx, yy = y + dy, ww = w + dw;
         const bool ok = xx < V.g && yy < V.g && ww < V.W;
         const size_t wi = ok ? (((size_t)xx * V.g + yy) * V.W + ww) : 0;
@@ -1678,7 +1677,7 @@ __device__ __forceinline__ void emit_word(const EmitWord &E, const Hood &H, Emit
     const int z = w * 64 + lane;
     const float dx = E.G.dx;
     const float *__restrict__ F = E.F;
-    const uint64_t m0 = H.S[13][0], m1 = H.S[13][1], m2 = H.S[13][2], m3 = H.S[13][3];
+    const uint64_t m0 = H.S[0][0], m1 = H.S[0][1], m2 = H.S[0][2], m3 = H.S[0][3];
     const unsigned sl = (unsigned)((m0 >> lane) & 1ull) | (unsigned)((m1 >> lane) & 1ull) << 1 |
                         (unsigned)((m2 >> lane) & 1ull) << 2 | (unsigned)((m3 >> lane) & 1ull) << 3;
     const size_t pidx = ((size_t)x * g + y) * g + z;
