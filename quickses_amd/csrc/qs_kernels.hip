@@ -99,6 +99,21 @@ __device__ __forceinline__ int seg_coarse(const T *__restrict__ pre, const uint3
     return j;
 }
 
+/* Workgroups are dealt to the eight XCDs round-robin (block b runs on XCD b % 8), and each XCD
+   has its own L2.  This renumbers the blocks of an nb-block launch in groups of 8*XCD_CHUNK: inside a
+   group each XCD runs XCD_CHUNK consecutive logical blocks, so neighbouring tiles share an L2 (their halo
+   reads hit there and their partial output lines merge there before going to HBM), while the
+   groups still spread every region of the grid over all eight XCDs.  A bijection on [0, nb): the
+   last, incomplete group keeps its order. */
+constexpr uint32_t XCD_CHUNK = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    constexpr uint32_t G = 8 * XCD_CHUNK;
+    if (b >= nb / G * G) return b;
+    const uint32_t w = b % G;
+    return b - w + (w & 7u) * XCD_CHUNK + (w >> 3);
+}
+
 /* molecule of block b of a batched launch, and the block's index inside the molecule */
 __device__ __forceinline__ int batch_mol(const QsBatch &B, uint32_t b, uint32_t &local)
 {
@@ -1129,7 +1144,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     __shared__ BMask BM[LX * P2_TY * P2_TZ];
     __shared__ int s_flags;
     uint32_t lb;
-    const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
+    const QsMol &M = B.mols[batch_mol(B, xcd_block(blockIdx.x, gridDim.x), lb)];
     const QsGrid G = M.G;
     const int g = G.g, W = (int)M.W;
     const uint64_t *__restrict__ outm = outm_all + M.word_off;
@@ -1467,7 +1482,7 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
                                                   uint32_t *__restrict__ vcnt_all, uint32_t *__restrict__ tcnt_all)
 {
     uint32_t lb;
-    const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
+    const QsMol &M = B.mols[batch_mol(B, xcd_block(blockIdx.x, gridDim.x), lb)];
     const QsGrid G = M.G;
     const int g = G.g, lim = G.lim, W = (int)M.W;
     const uint64_t *__restrict__ N = N_all + M.word_off;
@@ -1757,7 +1772,7 @@ __global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols,
     __shared__ EmitLds elds[4];
     /* everything up to the lane's z is wave-uniform: scalar registers */
     const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t k0 = (blockIdx.x * 4 + wid) * EMIT_K;
+    const uint32_t k0 = (xcd_block(blockIdx.x, gridDim.x) * 4 + wid) * EMIT_K;
     if (k0 >= nact) return; /* whole wavefront; each wave has its own hoods, so no block barrier */
     EmitWord E[EMIT_K];
 #pragma unroll
