@@ -1137,12 +1137,11 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     constexpr int INF = 255;
     static_assert(LZ <= 64 && 3 * R * R < INF, "window too wide for this kernel");
     __shared__ uint64_t rows[LX * LY];
-    __shared__ uint32_t own[P2R_TX * P2_TY];
+    __shared__ uint32_t own[P2R_TX * P2_TY], ownout[P2R_TX * P2_TY];
     __shared__ uint8_t DY[LX * P2_TY * P2_TZ];
     /* y offsets (bit b + R) attaining each DY value */
     using BMask = typename std::conditional<(2 * R + 1 <= 16), uint16_t, uint32_t>::type;
     __shared__ BMask BM[LX * P2_TY * P2_TZ];
-    __shared__ int s_flags;
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, xcd_block(blockIdx.x, gridDim.x), lb)];
     const QsGrid G = M.G;
@@ -1155,40 +1154,32 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     const int x1 = g;
     const uint32_t nbx = (g + P2_TZ - 1) / P2_TZ, nby = (g + P2_TY - 1) / P2_TY;
     const int tz0 = (int)(lb % nbx) * P2_TZ, ty0 = (int)((lb / nbx) % nby) * P2_TY, tx0 = (int)(lb / (nbx * nby)) * P2R_TX;
-    if (threadIdx.x == 0) s_flags = 0;
-    __syncthreads();
-
-    int fl = 0;
-    for (int row = threadIdx.x; row < LX * LY; row += 256) {
-        int lx = row / LY, ly = row - lx * LY;
-        int gx = tx0 - R + lx, gy = ty0 - R + ly;
-        uint64_t m = 0ull;
-        if (gx >= 0 && gx < g && gy >= 0 && gy < g) m = row_bits(outm + ((size_t)gx * g + gy) * W, W, tz0 - R, LZ);
-        rows[row] = m;
-        if (m) fl |= 2;
-    }
-    if (threadIdx.x < P2R_TX * P2_TY) {
-        int tx = threadIdx.x / P2_TY, ty = threadIdx.x - tx * P2_TY;
-        int gx = tx0 + tx, gy = ty0 + ty;
-        uint32_t m = 0u;
-        if (gx < x1 && gx < g && gy < g) m = (uint32_t)row_bits(shellm + ((size_t)gx * g + gy) * W, W, tz0, P2_TZ);
-        own[threadIdx.x] = m;
-        if (m) fl |= 1;
-    }
-    if (fl) atomicOr(&s_flags, fl);
-    __syncthreads();
-    const int flags = s_flags;
-
     const int ty = threadIdx.x >> 5, tz = threadIdx.x & 31;
     const int gy = ty0 + ty, gz = tz0 + tz;
     const bool vyz = gy < g && gz < g;
     const float dx = G.dx;
-    if (flags != 3) {
-        /* no shell voxel, or no outside voxel within reach: outside -> probe, else -dx */
+
+    /* the tile's own shell bits (threads 0-127) and outside bits (threads 128-255) first: a tile
+       without a shell voxel (outside the molecule or deep inside it) is finished from them alone,
+       without its halo */
+    static_assert(P2R_TX * P2_TY == 128, "one thread per own row of each mask");
+    int has_shell;
+    {
+        const int t = threadIdx.x & 127, tx = t / P2_TY, ty_ = t - tx * P2_TY;
+        const int gx = tx0 + tx, gy_ = ty0 + ty_;
+        const uint64_t *__restrict__ src = threadIdx.x < 128 ? shellm : outm;
+        uint32_t m = 0u;
+        if (gx < x1 && gx < g && gy_ < g) m = (uint32_t)row_bits(src + ((size_t)gx * g + gy_) * W, W, tz0, P2_TZ);
+        if (threadIdx.x < 128) own[t] = m;
+        else ownout[t] = m;
+        has_shell = __syncthreads_or(threadIdx.x < 128 && m != 0u);
+    }
+    if (!has_shell) {
+        /* outside -> probe, else -dx */
         for (int tx = 0; tx < P2R_TX; tx++) {
             int gx = tx0 + tx;
             if (gx >= x1 || gx >= g) break;
-            bool out = (rows[(tx + R) * LY + (ty + R)] >> (tz + R)) & 1ull;
+            bool out = (ownout[tx * P2_TY + ty] >> tz) & 1u;
             float res = out ? QS_PROBE : -dx;
             if (vyz) field[((size_t)gx * g + gy) * g + gz] = res;
             emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, res);
@@ -1196,8 +1187,66 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         return;
     }
 
+    int fl = 0;
+#pragma unroll
+    for (int row = threadIdx.x; row < LX * LY; row += 256) {
+        int lx = row / LY, ly = row - lx * LY;
+        int gx = tx0 - R + lx, gy_ = ty0 - R + ly;
+        uint64_t m = 0ull;
+        if (gx >= 0 && gx < g && gy_ >= 0 && gy_ < g) m = row_bits(outm + ((size_t)gx * g + gy_) * W, W, tz0 - R, LZ);
+        rows[row] = m;
+        fl |= (m != 0ull);
+    }
+    if (!__syncthreads_or(fl)) {
+        /* no outside voxel within reach: every voxel of the tile is -dx */
+        for (int tx = 0; tx < P2R_TX; tx++) {
+            int gx = tx0 + tx;
+            if (gx >= x1 || gx >= g) break;
+            if (vyz) field[((size_t)gx * g + gy) * g + gz] = -dx;
+            emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, -dx);
+        }
+        return;
+    }
+
     /* z then y, one (lx, z) column per thread: the squared z distance of each row to its nearest
        outside voxel within R, straight from the row bits, then the y minima over 2R+1 rows */
+    if constexpr (2 * R + 1 <= 16) {
+        /* two y outputs per 32-bit register (16-bit halves): the tap sums, minima and equality
+           masks in packed 16-bit integer arithmetic; every value stays below 2^9 */
+        typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+        u16x2 one = {1, 1};
+        asm volatile("" : "+v"(one)); /* opaque, so that the mask bits stay packed arithmetic */
+        for (int col = threadIdx.x; col < LX * P2_TZ; col += 256) {
+            const int lx = col >> 5, z = col & 31;
+            int v[LY];
+#pragma unroll
+            for (int ly = 0; ly < LY; ly++) v[ly] = zdist2<R, INF>(rows[lx * LY + ly], z + R);
+            u16x2 P[LY - 1];
+#pragma unroll
+            for (int k = 0; k < LY - 1; k++) P[k] = __builtin_bit_cast(u16x2, (uint32_t)v[k] | ((uint32_t)v[k + 1] << 16));
+#pragma unroll
+            for (int yy = 0; yy < P2_TY; yy += 2) {
+                u16x2 sb[2 * R + 1];
+                u16x2 best = {(uint16_t)INF, (uint16_t)INF};
+#pragma unroll
+                for (int b = 0; b <= 2 * R; b++) {
+                    const uint16_t c = (uint16_t)((b - R) * (b - R));
+                    sb[b] = P[yy + b] + (u16x2){c, c};
+                    best = __builtin_elementwise_min(best, sb[b]);
+                }
+                u16x2 ne = {0, 0};
+#pragma unroll
+                for (int b = 0; b <= 2 * R; b++)
+                    ne |= __builtin_elementwise_min(sb[b] - best, one) << (u16x2){(uint16_t)b, (uint16_t)b};
+                const uint32_t bb = __builtin_bit_cast(uint32_t, best);
+                const uint32_t mm = ~__builtin_bit_cast(uint32_t, ne) & (((1u << (2 * R + 1)) - 1u) * 0x10001u);
+                DY[(lx * P2_TY + yy) * P2_TZ + z] = (uint8_t)bb;
+                DY[(lx * P2_TY + yy + 1) * P2_TZ + z] = (uint8_t)(bb >> 16);
+                BM[(lx * P2_TY + yy) * P2_TZ + z] = (BMask)(mm & 0xFFFFu);
+                BM[(lx * P2_TY + yy + 1) * P2_TZ + z] = (BMask)(mm >> 16);
+            }
+        }
+    } else
     for (int col = threadIdx.x; col < LX * P2_TZ; col += 256) {
         const int lx = col >> 5, z = col & 31;
         int v[LY];
