@@ -63,6 +63,8 @@ const char *qs_last_error_message(void);
  *                     1 = stop at cube g-2r-2 as the reference's single slice does
  *   "atom_ids_sorted" 0 = input atom index (default), 1 = cell-sorted index (reference)
  *   "pass1_direct"    1 = evaluate field pass 1 per voxel over its 27 cells (cross-check)
+ *   "pass1_rows"      1 = evaluate field pass 1 by row gather, one wavefront per group of rows,
+ *                     instead of the default slab scatter (cross-check)
  *   "pass2_direct"    1 = evaluate field pass 2 with the (2r+1)^3 window kernel (cross-check)
  *   "trace"           1 = print per-stage GPU times as JSON lines on stderr
  *   "timing"          1 = record per-stage GPU times (hipEvents on the library's stream)

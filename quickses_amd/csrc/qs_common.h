@@ -39,8 +39,8 @@ struct QsMol {
     uint64_t vox_off;           /* first voxel (field, cube records) */
     uint32_t vert_off, n_vert;  /* vertices (set after counting) */
     uint32_t tri_off, n_tri;    /* triangles (set after counting) */
-    int rows_per_wave;          /* field pass 1: rows per wavefront */
-    int pad;
+    int rows_per_wave;          /* field pass 1, row kernel: rows per wavefront */
+    int slab_rows;              /* field pass 1, slab kernel: y rows per block */
 };
 
 /* Coarse indexes of prefix arrays: coarse[k] = the segment that holds item k << shift, with

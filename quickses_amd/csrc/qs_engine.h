@@ -46,6 +46,7 @@ struct Options {
     int lim_mode = 0;
     int atom_ids_sorted = 0;
     int pass1_direct = 0;
+    int pass1_rows = 0;
     int pass2_direct = 0;
     int trace = 0;
     int timing = 0;
@@ -189,6 +190,7 @@ struct Engine {
     uint32_t n_atoms_total = 0;
     uint64_t ncell_total = 0, nw_total = 0, vox_total = 0;
     uint32_t nv = 0, nt_all = 0, nt = 0, n_active = 0;
+    size_t p1_lds = 0; /* dynamic LDS of the field pass 1 launch (largest molecule of the batch) */
     bool batch_valid = false; /* result arrays, vert_off / tri_off and their device copies belong to run_batch */
     float *res_vert = nullptr;
     int *res_tri = nullptr, *res_atom = nullptr;

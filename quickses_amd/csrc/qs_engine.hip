@@ -209,11 +209,13 @@ int Engine::setup(const std::vector<MolIn> &in, const std::vector<BBox> &bb)
             G.lim = std::max(lim, 0);
             M.W = (uint32_t)((G.g + 63) / 64);
             M.rows_per_wave = qs_pass1_rows_per_wave(G);
+            M.slab_rows = qs_pass1_slab_rows(G, (int)M.W);
         } else {
             /* N <= 1: empty mesh (CudaSurf.cu:539-543); its atoms sit in one cell and no grid is run */
             M.G = QsGrid{0.f, 0.f, 0.f, in[m].reso, 1.0f, 0, 1, 1, 0};
             M.W = 0;
             M.rows_per_wave = 1;
+            M.slab_rows = 1;
         }
         const uint64_t gg = (uint64_t)M.G.g;
         M.cell_off = (uint32_t)cell;
@@ -267,9 +269,11 @@ int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &a
         return at;
     };
     at_atoms = prefix([](const QsMol &M) { return (uint64_t)((M.n_atoms + 255) / 256); }, nb_atoms);
-    const bool p1d = g_opt.pass1_direct != 0;
-    at_p1 = prefix([&](const QsMol &M) { return (uint64_t)(M.G.g ? qs_pass1_blocks(M.G, (int)M.W, M.rows_per_wave, p1d) : 0); },
-                   nb_p1);
+    const int p1mode = qs_pass1_mode(g_opt.pass1_direct != 0, g_opt.pass1_rows != 0);
+    at_p1 = prefix([&](const QsMol &M) { return (uint64_t)(M.G.g ? qs_pass1_blocks(M, p1mode) : 0); }, nb_p1);
+    p1_lds = 0;
+    for (const QsMol &M : mols)
+        if (M.G.g) p1_lds = std::max(p1_lds, qs_pass1_lds_bytes(M, p1mode));
     at_mc = prefix([](const QsMol &M) { return (uint64_t)(M.G.g ? qs_mc_count_blocks(M.G, (int)M.W) : 0); }, nb_mc);
     /* word prefix (64-bit) stored as pairs, then its coarse index */
     if (meta.size() & 1) meta.push_back(0u);
@@ -359,12 +363,13 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
     QS_TRY(ensure(negm, std::max<uint64_t>(nw_total, 1) * 8));
     QS_TRY(ensure(zerom, std::max<uint64_t>(nw_total, 1) * 8));
     if (nb_p1) {
-        if (g_opt.pass1_direct)
+        const int p1mode = qs_pass1_mode(g_opt.pass1_direct != 0, g_opt.pass1_rows != 0);
+        if (p1mode == 2)
             qs_launch_pass1_direct(batch(at_p1, 0, nmol), nb_p1, as<int2>(cse), as<float4>(sx), as<uint64_t>(outm),
                                    as<uint64_t>(shellm), stream);
         else
-            qs_launch_pass1(batch(at_p1, 0, nmol), nb_p1, as<uint32_t>(start), as<float4>(sx), as<float4>(aux),
-                            as<uint64_t>(outm), as<uint64_t>(shellm), stream);
+            qs_launch_pass1(batch(at_p1, 0, nmol), nb_p1, p1mode, p1_lds, as<uint32_t>(start), as<float4>(sx),
+                            as<float4>(aux), as<uint64_t>(outm), as<uint64_t>(shellm), stream);
         QS_HIP(hipGetLastError());
     }
     tm.mark("pass1");
@@ -661,6 +666,7 @@ int single_grid(Engine &E, const QsGrid &G, size_t &at_mc, uint32_t &nb_mc, size
     M.G = G;
     M.W = (uint32_t)((G.g + 63) / 64);
     M.rows_per_wave = qs_pass1_rows_per_wave(G);
+    M.slab_rows = qs_pass1_slab_rows(G, (int)M.W);
     E.n_atoms_total = 0;
     E.ncell_total = (uint64_t)G.gn * G.gn * G.gn;
     E.nw_total = (uint64_t)G.g * G.g * M.W;
@@ -720,6 +726,7 @@ int qs_set_option(const char *key, int value)
     if (k == "lim_mode") g_opt.lim_mode = value;
     else if (k == "atom_ids_sorted") g_opt.atom_ids_sorted = value;
     else if (k == "pass1_direct") g_opt.pass1_direct = value;
+    else if (k == "pass1_rows") g_opt.pass1_rows = value;
     else if (k == "pass2_direct") g_opt.pass2_direct = value;
     else if (k == "trace") g_opt.trace = value;
     else if (k == "timing") g_opt.timing = value;
@@ -733,6 +740,7 @@ int qs_get_option(const char *key)
     if (k == "lim_mode") return g_opt.lim_mode;
     if (k == "atom_ids_sorted") return g_opt.atom_ids_sorted;
     if (k == "pass1_direct") return g_opt.pass1_direct;
+    if (k == "pass1_rows") return g_opt.pass1_rows;
     if (k == "pass2_direct") return g_opt.pass2_direct;
     if (k == "trace") return g_opt.trace;
     if (k == "timing") return g_opt.timing;

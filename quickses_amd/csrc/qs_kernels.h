@@ -21,8 +21,13 @@ void qs_launch_atom_aux(QsBatch B, uint32_t nblocks, const float4 *sx, float4 *a
 
 /* field pass 1 */
 int qs_pass1_rows_per_wave(QsGrid G);
-uint32_t qs_pass1_blocks(QsGrid G, int W, int R, bool direct);
-void qs_launch_pass1(QsBatch B, uint32_t nblocks, const uint32_t *cstart, const float4 *sx, const float4 *aux, uint64_t *outm,
+int qs_pass1_slab_rows(QsGrid G, int W);
+/* field pass 1 kernel: 0 slab scatter (default), 1 row gather, 2 per-voxel */
+int qs_pass1_mode(bool direct, bool rows);
+uint32_t qs_pass1_blocks(const QsMol &M, int mode);
+size_t qs_pass1_lds_bytes(const QsMol &M, int mode);
+void qs_launch_pass1(QsBatch B, uint32_t nblocks, int mode, size_t lds_bytes, const uint32_t *cstart, const float4 *sx,
+                     const float4 *aux, uint64_t *outm,
                      uint64_t *shellm, hipStream_t s);
 void qs_launch_pass1_direct(QsBatch B, uint32_t nblocks, const int2 *cse, const float4 *sx, uint64_t *outm, uint64_t *shellm,
                             hipStream_t s);

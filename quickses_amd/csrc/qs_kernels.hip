@@ -498,6 +498,13 @@ __device__ __forceinline__ uint64_t bit_range(int a, int b)
     return hi & ~lo;
 }
 
+/* bits of the word starting at z0 for z < L */
+__device__ __forceinline__ uint64_t bits_below(int L, int z0)
+{
+    int n = L - z0;
+    return n <= 0 ? 0ull : (n >= 64 ? ~0ull : ((1ull << n) - 1ull));
+}
+
 /* smallest float s with sqrtf(s) >= T (T > 0), so that for v >= 0: sqrtf(v) < T <=> v < s.
    Every step uses the correctly rounded sqrtf, so the comparison is the reference's. */
 __device__ __forceinline__ float sqrt_threshold(float T)
@@ -821,6 +828,231 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsBatch B, const uint32_t *_
             outm[widx] = valid & ~(mi | mn);
             shellm[widx] = mn & ~mi;
         }
+    }
+}
+
+/* OR bits [L, R] (absolute z) into a row's words in LDS */
+__device__ __forceinline__ void lds_or_range(uint64_t *row, int L, int R)
+{
+    for (int w = L >> 6; w <= (R >> 6); w++) {
+        const int z0 = w << 6;
+        atomicOr((unsigned long long *)&row[w], (unsigned long long)bit_range(max(L, z0) - z0, min(R, z0 + 63) - z0));
+    }
+}
+
+/* first index i in [0, n) with t[i] >= v (n if none); t non-decreasing */
+__device__ __forceinline__ int lds_lower_bound(const int *t, int n, int v)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (t[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+/* Field pass 1 by scatter into LDS: one block per slab of up to P1S_MAXROWS consecutive y rows of
+   one x plane, all z.  Each candidate atom (x cells cx-1..cx+1 times the slab's y cells, every z
+   cell) takes the rows whose y cell admits it and, lane by lane, ORs its exact near and inside
+   z-intervals (the ones of k_pass1_rows) into the slab's two masks.  The z-intervals are clipped
+   to the voxels whose z cell admits the atom, so each voxel sees exactly the atoms of its 27
+   neighbour cells, as in probeIntersection (Kernels.cu:394-475); inside wins over near, which
+   makes the result independent of the order of the ORs. */
+#define P1S_MAXROWS 64
+#define P1S_MAXCOL 256
+__global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *__restrict__ cstart_all,
+                                                    const float4 *__restrict__ sx, const float4 *__restrict__ aux,
+                                                    uint64_t *__restrict__ outm_all, uint64_t *__restrict__ shellm_all)
+{
+    extern __shared__ __align__(16) uint64_t p1s_dyn[];
+    __shared__ uint32_t colpre[P1S_MAXCOL + 1], colst[P1S_MAXCOL];
+    __shared__ int q_id[4][P1_QCAP], q_rows[4][P1_QCAP], q_z[4][P1_QCAP];
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t lb;
+    const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
+    const QsGrid G = M.G;
+    const int g = G.g, gn = G.gn, W = (int)M.W, YB = M.slab_rows;
+    const uint32_t *__restrict__ cstart = cstart_all + M.cell_off;
+    uint64_t *__restrict__ outm = outm_all + M.word_off;
+    uint64_t *__restrict__ shellm = shellm_all + M.word_off;
+    const int nyb = (g + YB - 1) / YB;
+    const int x = (int)(lb / (uint32_t)nyb);
+    const int y0 = (int)(lb % (uint32_t)nyb) * YB;
+    if (x >= g) return; /* whole block */
+    const int ny = min(YB, g - y0);
+    const size_t wbase = ((size_t)x * g + y0) * W;
+    if (x >= g - 1) {
+        /* the last plane is not labelled: outside */
+        for (int t = threadIdx.x; t < ny * W; t += 256) {
+            const int w = t % W;
+            outm[wbase + t] = bits_below(g, w * 64);
+            shellm[wbase + t] = 0ull;
+        }
+        return;
+    }
+    uint64_t *nearw = p1s_dyn, *insw = p1s_dyn + (size_t)YB * W;
+    /* yfirst[c] (zfirst[c]) = first y (z) whose neighbour cell is >= c, g if none; c = 0..gn+1 */
+    int *yfirst = (int *)(p1s_dyn + 2 * (size_t)YB * W), *zfirst = yfirst + (gn + 2);
+    for (int t = threadIdx.x; t < ny * W; t += 256) nearw[t] = insw[t] = 0ull;
+    for (int t = threadIdx.x; t < gn + 2; t += 256) yfirst[t] = zfirst[t] = g;
+    __syncthreads();
+    const float dx = G.dx, inv_dx = 1.0f / G.dx;
+    for (int i = threadIdx.x; i < g; i += 256) {
+        const int cy = min(cell_of(grid_pos(G.oy, i, dx), G.oy, G.dxn), gn + 1);
+        const int cyp = i ? min(cell_of(grid_pos(G.oy, i - 1, dx), G.oy, G.dxn), gn + 1) : -1;
+        for (int c = cyp + 1; c <= cy; c++) yfirst[c] = i;
+        const int cz = min(cell_of(grid_pos(G.oz, i, dx), G.oz, G.dxn), gn + 1);
+        const int czp = i ? min(cell_of(grid_pos(G.oz, i - 1, dx), G.oz, G.dxn), gn + 1) : -1;
+        for (int c = czp + 1; c <= cz; c++) zfirst[c] = i;
+    }
+    /* labelled rows of the slab: y <= g - 2 */
+    const int ylast = min(y0 + ny - 1, g - 2);
+    const float px = grid_pos(G.ox, x, dx);
+    const int cx = cell_of(px, G.ox, G.dxn);
+    const int xa = iclamp(cx - 1, 0, gn - 1), xb = iclamp(cx + 1, 0, gn - 1);
+    const int ya = iclamp(cell_of(grid_pos(G.oy, y0, dx), G.oy, G.dxn) - 1, 0, gn - 1);
+    const int yb = iclamp(cell_of(grid_pos(G.oy, max(ylast, y0), dx), G.oy, G.dxn) + 1, 0, gn - 1);
+    const int nyc = yb - ya + 1, ncol = (xb - xa + 1) * nyc;
+    /* candidate columns: every z cell of (xc, yc), contiguous in the cell-ordered atoms */
+    uint32_t len = 0;
+    if ((int)threadIdx.x < ncol) {
+        const int xc = xa + (int)threadIdx.x / nyc, yc = ya + (int)threadIdx.x % nyc;
+        const size_t h0 = ((size_t)xc * gn + yc) * gn;
+        colst[threadIdx.x] = cstart[h0];
+        len = cstart[h0 + gn] - cstart[h0];
+    }
+    /* exclusive prefix of the column lengths (ncol <= 256) */
+    {
+        uint32_t v = len;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(v, o, 64);
+            if (lane >= o) v += t;
+        }
+        __shared__ uint32_t wsum[4];
+        if (lane == 63) wsum[wid] = v;
+        __syncthreads();
+        uint32_t add = 0;
+        for (int k = 0; k < wid; k++) add += wsum[k];
+        colpre[threadIdx.x + 1] = add + v;
+        if (threadIdx.x == 0) colpre[0] = 0;
+    }
+    __syncthreads();
+    const int total = (int)colpre[ncol];
+    const int rows_hi = ylast - y0; /* rows 0..rows_hi of the slab are labelled */
+    int *qid = q_id[wid], *qrows = q_rows[wid], *qz = q_z[wid];
+
+    /* one queued atom per lane: its rows in turn */
+    auto process = [&](int count) {
+        if (lane >= count) return;
+        const int id = qid[lane], rr = qrows[lane], zr = qz[lane];
+        const float4 a = sx[id], ax = aux[id];
+        const float sN = ax.x, sD = ax.y;
+        const float ex = a.x - px;
+        const float X = ex * ex;
+        const int zlo = zr & 0xFFFF, zhi = zr >> 16;
+        const float zc = (a.z - G.oz) * inv_dx;
+        RowAtom A;
+        A.az = a.z;
+        A.dx = dx;
+        A.oz = G.oz;
+        int zb = iclamp((int)floorf(zc), zlo, zhi);
+        if (zb + 1 <= zhi && A.Zf(zb + 1) < A.Zf(zb)) zb = zb + 1;
+        const int r1 = rr >> 8;
+        for (int yr = rr & 0xFF; yr <= r1; yr++) {
+            const float ey = a.y - grid_pos(G.oy, y0 + yr, dx);
+            const float Y = ey * ey;
+            A.S = X + Y;
+            if (!(A.S < sN)) continue;
+            int L, Rr;
+            row_interval(A, sN, zlo, zhi, zb, zc, inv_dx, L, Rr);
+            if (L > Rr) continue;
+            lds_or_range(nearw + (size_t)yr * W, L, Rr);
+            if (!(A.S < sD)) continue;
+            row_interval(A, sD, L, Rr, zb, zc, inv_dx, L, Rr);
+            if (L <= Rr) lds_or_range(insw + (size_t)yr * W, L, Rr);
+        }
+    };
+
+    int qn = 0;
+    if (rows_hi >= 0) {
+        for (int base = wid * 64; base < total; base += 256) {
+            const int k = base + lane;
+            bool keep = false;
+            int id = 0, rows = 0, zr = 0;
+            if (k < total) {
+                int lo = 0, hi = ncol - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if ((int)colpre[mid] <= k) lo = mid;
+                    else hi = mid - 1;
+                }
+                id = (int)colst[lo] + (k - (int)colpre[lo]);
+                const float4 a = sx[id], ax = aux[id];
+                const float ex = a.x - px;
+                const float X = ex * ex;
+                if (X < ax.x) {
+                    const int packed = __float_as_int(ax.z);
+                    const int acy = packed >> 16, acz = packed & 0xFFFF;
+                    /* rows whose y cell admits the atom: cy >= acy - 1, and cy <= acy + 1 unless acy is the last cell */
+                    int ylo = yfirst[max(acy - 1, 0)];
+                    int yhi = (acy == gn - 1) ? g - 1 : yfirst[acy + 2] - 1;
+                    /* rows the atom can reach in y (a superset: the exact test is per row) */
+                    const float h = sqrtf(ax.x - X);
+                    ylo = max(ylo, (int)floorf((a.y - h - G.oy) * inv_dx) - 1);
+                    yhi = min(yhi, (int)ceilf((a.y + h - G.oy) * inv_dx) + 1);
+                    const int r0 = max(ylo - y0, 0), r1 = min(yhi - y0, rows_hi);
+                    /* voxels whose z cell admits the atom, labelled ones only (z <= g - 2) */
+                    const int zlo = zfirst[max(acz - 1, 0)];
+                    const int zhi = min((acz == gn - 1) ? g - 1 : zfirst[acz + 2] - 1, g - 2);
+                    keep = r0 <= r1 && zlo <= zhi;
+                    rows = r0 | (r1 << 8);
+                    zr = zlo | (zhi << 16);
+                }
+            }
+            const uint64_t km = __ballot(keep);
+            if (keep) {
+                const int slot = qn + __popcll(km & ((1ull << lane) - 1ull));
+                qid[slot] = id;
+                qrows[slot] = rows;
+                qz[slot] = zr;
+            }
+            qn += __popcll(km);
+            wave_lds_sync();
+            if (qn >= 64) {
+                process(64);
+                wave_lds_sync();
+                const int rest = qn - 64;
+                int m0 = 0, m1 = 0, m2 = 0;
+                if (lane < rest) {
+                    m0 = qid[64 + lane];
+                    m1 = qrows[64 + lane];
+                    m2 = qz[64 + lane];
+                }
+                wave_lds_sync();
+                if (lane < rest) {
+                    qid[lane] = m0;
+                    qrows[lane] = m1;
+                    qz[lane] = m2;
+                }
+                wave_lds_sync();
+                qn = rest;
+            }
+        }
+        if (qn > 0) process(qn);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < ny * W; t += 256) {
+        const int yr = t / W, w = t - yr * W;
+        const uint64_t valid = bits_below(g, w * 64);
+        uint64_t mi = 0ull, mn = 0ull;
+        if (yr <= rows_hi) {
+            mi = insw[t];
+            mn = nearw[t];
+        }
+        outm[wbase + t] = valid & ~(mi | mn);
+        shellm[wbase + t] = mn & ~mi;
     }
 }
 
@@ -1512,13 +1744,6 @@ struct MCView {
     }
 };
 
-/* bits of word w for z < L */
-__device__ __forceinline__ uint64_t bits_below(int L, int z0)
-{
-    int n = L - z0;
-    return n <= 0 ? 0ull : (n >= 64 ? ~0ull : ((1ull << n) - 1ull));
-}
-
 __device__ __forceinline__ uint64_t ld_word(const uint64_t *__restrict__ m, int g, int W, int x, int y, int w)
 {
     if (x < 0 || y < 0 || x >= g || y >= g || w < 0 || w >= W) return 0ull;
@@ -2116,16 +2341,39 @@ int qs_pass1_rows_per_wave(QsGrid G)
     return R < 1 ? 1 : (R > P1_RMAX ? P1_RMAX : R);
 }
 
-uint32_t qs_pass1_blocks(QsGrid G, int W, int R, bool direct)
+int qs_pass1_slab_rows(QsGrid G, int W)
 {
-    if (direct) return (uint32_t)W * nblk(G.g, 4) * (uint32_t)G.g;
-    return nblk(G.g, 4 * R) * (uint32_t)G.g;
+    /* up to 64 rows; both masks in at most 32 KB of LDS; at most 80 neighbour cells of y per
+       slab, so the candidate columns (3 x cells times the y cells + 2) stay within 256 */
+    int yb = P1S_MAXROWS;
+    yb = std::min(yb, std::max(1, 2048 / std::max(W, 1)));
+    yb = std::min(yb, std::max(1, (int)(80.0f * G.dxn / G.dx)));
+    return yb;
 }
 
-void qs_launch_pass1(QsBatch B, uint32_t nblocks, const uint32_t *cstart, const float4 *sx, const float4 *aux, uint64_t *outm,
-                     uint64_t *shellm, hipStream_t s)
+int qs_pass1_mode(bool direct, bool rows) { return direct ? 2 : (rows ? 1 : 0); }
+
+uint32_t qs_pass1_blocks(const QsMol &M, int mode)
 {
-    hipLaunchKernelGGL(k_pass1_rows, dim3(nblocks), dim3(256), 0, s, B, cstart, sx, aux, outm, shellm);
+    const QsGrid &G = M.G;
+    if (mode == 2) return M.W * nblk(G.g, 4) * (uint32_t)G.g;
+    if (mode == 1) return nblk(G.g, 4 * M.rows_per_wave) * (uint32_t)G.g;
+    return nblk(G.g, M.slab_rows) * (uint32_t)G.g;
+}
+
+size_t qs_pass1_lds_bytes(const QsMol &M, int mode)
+{
+    if (mode != 0) return 0;
+    return (size_t)16 * M.slab_rows * M.W + (size_t)8 * (M.G.gn + 2);
+}
+
+void qs_launch_pass1(QsBatch B, uint32_t nblocks, int mode, size_t lds_bytes, const uint32_t *cstart, const float4 *sx,
+                     const float4 *aux, uint64_t *outm, uint64_t *shellm, hipStream_t s)
+{
+    if (mode == 1)
+        hipLaunchKernelGGL(k_pass1_rows, dim3(nblocks), dim3(256), 0, s, B, cstart, sx, aux, outm, shellm);
+    else
+        hipLaunchKernelGGL(k_pass1_slab, dim3(nblocks), dim3(256), lds_bytes, s, B, cstart, sx, aux, outm, shellm);
 }
 
 void qs_launch_pass1_direct(QsBatch B, uint32_t nblocks, const int2 *cse, const float4 *sx, uint64_t *outm, uint64_t *shellm,

@@ -37,7 +37,7 @@ def test_float3_layout():
 
 
 def test_options_roundtrip(qs):
-    for key in ("lim_mode", "atom_ids_sorted", "pass1_direct", "pass2_direct", "trace", "timing"):
+    for key in ("lim_mode", "atom_ids_sorted", "pass1_direct", "pass1_rows", "pass2_direct", "trace", "timing"):
         old = qs.get_option(key)
         qs.set_option(key, 1)
         assert qs.get_option(key) == 1

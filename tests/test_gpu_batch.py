@@ -115,7 +115,7 @@ def test_batch_getters_agree(qs):
         _same_mesh(qs.batch_mesh(m), out[m])
 
 
-@pytest.mark.parametrize("opt", ["lim_mode", "atom_ids_sorted", "pass1_direct", "pass2_direct"])
+@pytest.mark.parametrize("opt", ["lim_mode", "atom_ids_sorted", "pass1_direct", "pass1_rows", "pass2_direct"])
 def test_batch_options_match_oracle(qs, orc, opt):
     """Each option in a batch: the per-voxel cross-check kernels and the reference's window and
     atom-id conventions give every molecule the oracle's mesh."""

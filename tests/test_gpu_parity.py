@@ -95,19 +95,22 @@ def test_lattice_ties_match_oracle(qs, orc, reso):
     _same_mesh(gpu, orc.compute_ses(pos, rad, reso, 1))
 
 
-def test_pass1_rows_equal_direct(qs):
+@pytest.mark.parametrize("opt", ["pass1_direct", "pass1_rows"])
+def test_pass1_slab_equals_other_kernels(qs, opt):
+    """The default slab-scatter pass 1 labels every voxel as the per-voxel 27-cell kernel and the
+    row-gather kernel do, at a fine and a coarse voxel (many and few rows per slab)."""
     pos, rad = _ball(qs, 36, 1200)
-    for reso in (0.5, 0.2):
-        print("case pass1 direct reso=%.1f" % reso, flush=True)
+    for reso in (0.5, 0.2, 1.3):
+        print("case pass1 %s reso=%.1f" % (opt, reso), flush=True)
         qs.compute_ses(pos, rad, reso, 0)
         g = qs.last_grid()["g"]
-        rows = qs.fetch("labels", np.uint8, g ** 3)
-        qs.set_option("pass1_direct", 1)
+        slab = qs.fetch("labels", np.uint8, g ** 3)
+        qs.set_option(opt, 1)
         try:
             qs.compute_ses(pos, rad, reso, 0)
         finally:
-            qs.set_option("pass1_direct", 0)
-        assert np.array_equal(qs.fetch("labels", np.uint8, g ** 3), rows)
+            qs.set_option(opt, 0)
+        assert np.array_equal(qs.fetch("labels", np.uint8, g ** 3), slab)
 
 
 def test_pass2_tile_equals_direct(qs, orc):
