@@ -1338,24 +1338,18 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *_
 
 /* Field pass 2 for a fixed window half-width R: loops unrolled, y and x passes register-blocked
    (each thread keeps one column of the previous pass in registers). */
-/* squared distance from bit c of row m to its nearest set bit within R, or INF */
+/* squared distance from bit z + R of row m to its nearest set bit within R, or INF; z < 32 and
+   2R < 32, so bits z .. z + 2R are the low bits of one 32-bit funnel shift.  Branch free: the
+   sentinel bits stand at distance R + 1. */
 template <int R, int INF>
-__device__ __forceinline__ int zdist2(uint64_t m, int c)
+__device__ __forceinline__ int zdist2(uint64_t m, int z)
 {
-    int best = INF;
-    if (m) {
-        const uint64_t up = m >> c;
-        if (up) {
-            const int d = __builtin_ctzll(up);
-            if (d <= R) best = d * d;
-        }
-        const uint64_t dn = m & ((2ull << c) - 1ull);
-        if (dn) {
-            const int d = c - (63 - __builtin_clzll(dn));
-            if (d <= R) best = min(best, d * d);
-        }
-    }
-    return best;
+    static_assert(2 * R < 31, "window wider than a funnel shift");
+    const uint32_t w = __builtin_amdgcn_alignbit((uint32_t)(m >> 32), (uint32_t)m, (uint32_t)z); /* bit k = m bit z + k */
+    const int up = __builtin_ctz((w >> R) | (1u << (R + 1)));              /* nearest set bit at or above the centre */
+    const int dn = __builtin_clz((w << (31 - R)) | (1u << (30 - R)));      /* at or below it */
+    const int d = min(up, dn);
+    return d <= R ? d * d : INF;
 }
 
 template <int R>
@@ -1452,7 +1446,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
             const int lx = col >> 5, z = col & 31;
             int v[LY];
 #pragma unroll
-            for (int ly = 0; ly < LY; ly++) v[ly] = zdist2<R, INF>(rows[lx * LY + ly], z + R);
+            for (int ly = 0; ly < LY; ly++) v[ly] = zdist2<R, INF>(rows[lx * LY + ly], z);
             u16x2 P[LY - 1];
 #pragma unroll
             for (int k = 0; k < LY - 1; k++) P[k] = __builtin_bit_cast(u16x2, (uint32_t)v[k] | ((uint32_t)v[k + 1] << 16));
@@ -1483,7 +1477,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         const int lx = col >> 5, z = col & 31;
         int v[LY];
 #pragma unroll
-        for (int ly = 0; ly < LY; ly++) v[ly] = zdist2<R, INF>(rows[lx * LY + ly], z + R);
+        for (int ly = 0; ly < LY; ly++) v[ly] = zdist2<R, INF>(rows[lx * LY + ly], z);
 #pragma unroll
         for (int yy = 0; yy < P2_TY; yy++) {
             int best = INF;
