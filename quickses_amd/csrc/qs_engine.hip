@@ -395,14 +395,19 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_TRY(ensure(vbw, nw * 4));
     QS_TRY(ensure(tbw, nw * 4));
     QS_TRY(ensure(alist, nw * 4));
+    QS_TRY(ensure(avb, nw * 4));
+    QS_TRY(ensure(atb, nw * 4));
+    QS_TRY(ensure(ntb, nw * 24));
+    QS_TRY(ensure(cidx, std::max<uint64_t>(vox_total, 1)));
     QS_TRY(ensure(totals, 16));
     if (nb_mc)
         qs_launch_mc_count(batch(at_mc, 0, nmol), nb_mc, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
-                           as<uint32_t>(vcnt), as<uint32_t>(tcnt), stream);
+                           as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint64_t>(ntb), as<uint8_t>(cidx), stream);
     if (nw_total) {
         QS_TRY(ensure(scan_tmp, 3 * qs_scan_tiles(nw_total) * sizeof(uint32_t)));
         qs_launch_mc_scans(as<uint32_t>(vcnt), as<uint32_t>(tcnt), nw_total, as<uint32_t>(scan_tmp), as<uint32_t>(vbw),
-                           as<uint32_t>(tbw), as<uint32_t>(alist), as<uint32_t>(totals), stream);
+                           as<uint32_t>(tbw), as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(atb), as<uint32_t>(totals),
+                           stream);
         QS_HIP(hipGetLastError());
     } else {
         QS_HIP(hipMemsetAsync(totals.p, 0, 12, stream));
@@ -457,13 +462,14 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_TRY(ensure(vert, (size_t)std::max<uint32_t>(nv, 1) * 12));
     QS_TRY(ensure(vkey, (size_t)std::max<uint32_t>(nv, 1) * 8));
     QS_TRY(ensure(tri, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
-    QS_TRY(ensure(cidx, std::max<uint64_t>(vox_total, 1)));
     QS_TRY(ensure(tbase, std::max<uint64_t>(vox_total, 1) * 4));
     const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
     const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 2 * ((size_t)nmol + 1);
-    qs_launch_mc_emit(as<QsMol>(d_mols), wpre, wcoarse, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
-                      as<uint32_t>(vbw), as<float>(field), as<uint32_t>(alist), nact, as<uint32_t>(tbw), as<float>(vert),
-                      as<uint64_t>(vkey), as<int>(tri), as<uint8_t>(cidx), as<uint32_t>(tbase), stream);
+    qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), nact, nv, as<uint64_t>(s4),
+                          as<float>(field), as<float>(vert), as<uint64_t>(vkey), stream);
+    qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), nact, nt_all,
+                           as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw), as<uint64_t>(ntb),
+                           as<uint8_t>(cidx), as<int>(tri), as<uint32_t>(tbase), stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
     if (with_atoms) {

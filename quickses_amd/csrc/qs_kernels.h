@@ -46,11 +46,19 @@ void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *ou
 
 /* marching cubes */
 uint32_t qs_mc_count_blocks(QsGrid G, int W);
+/* per row word: slot masks, vertex and triangle counts, triangle-count bit planes (words with
+   triangles only); the case of every polygonised cube into cidx */
 void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
-                        uint32_t *tcnt, hipStream_t s);
-void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint64_t *N, const uint64_t *Zm,
-                       const uint64_t *S4, const uint32_t *vbw, const float *F, const uint32_t *list, uint32_t nact,
-                       const uint32_t *tbw, float *vert, uint64_t *vkey, int *tri, uint8_t *cidx, uint32_t *tbase, hipStream_t s);
+                        uint32_t *tcnt, uint64_t *NT, uint8_t *cidx, hipStream_t s);
+/* one thread per vertex / per triangle over the active words (list, with their first vertex ids
+   avb and first triangle ids atb) */
+void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
+                           const uint32_t *avb, uint32_t nact, uint32_t nv, const uint64_t *S4, const float *F, float *vert,
+                           uint64_t *vkey, hipStream_t s);
+void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
+                            const uint32_t *atb, uint32_t nact, uint32_t ntri, const uint64_t *N, const uint64_t *Zm,
+                            const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT, const uint8_t *cidx, int *tri,
+                            uint32_t *tbase, hipStream_t s);
 
 /* per-vertex stages */
 void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint32_t *cstart,
@@ -76,7 +84,7 @@ void qs_launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
 /* marching-cubes word scans: vbw, tbw, the active-word list and totals {triangles, vertices,
    active words}; sums holds 3 * qs_scan_tiles(nw) entries */
 void qs_launch_mc_scans(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
-                        uint32_t *tbw, uint32_t *list, uint32_t *totals, hipStream_t s);
+                        uint32_t *tbw, uint32_t *list, uint32_t *avb, uint32_t *atb, uint32_t *totals, hipStream_t s);
 void qs_launch_gather_u32(const uint32_t *src, const uint64_t *idx, int n, uint32_t *out, hipStream_t s);
 
 #endif

@@ -291,7 +291,8 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_planes(uint32_t *__restrict__ 
 __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict__ vcnt, const uint32_t *__restrict__ tcnt,
                                                       uint64_t n, const uint32_t *__restrict__ sums, uint64_t ntiles,
                                                       uint32_t *__restrict__ vbw, uint32_t *__restrict__ tbw,
-                                                      uint32_t *__restrict__ list)
+                                                      uint32_t *__restrict__ list, uint32_t *__restrict__ avb,
+                                                      uint32_t *__restrict__ atb)
 {
     __shared__ uint32_t lds[SCAN_NT / 64 + 1];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_IPT;
@@ -316,7 +317,11 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict
         if (i < n) {
             vbw[i] = rv;
             tbw[i] = rt;
-            if (v[k] | t[k]) list[ra++] = (uint32_t)i;
+            if (v[k] | t[k]) {
+                avb[ra] = rv;
+                atb[ra] = rt;
+                list[ra++] = (uint32_t)i;
+            }
         }
         rv += v[k];
         rt += t[k];
@@ -1747,7 +1752,8 @@ __device__ __forceinline__ uint64_t ld_word(const uint64_t *__restrict__ m, int 
 /* countVertexPerCell MarchingCubes.cu:571-658 in word form: slot masks and per-word counts */
 __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__restrict__ N_all,
                                                   const uint64_t *__restrict__ Zm_all, uint64_t *__restrict__ S4_all,
-                                                  uint32_t *__restrict__ vcnt_all, uint32_t *__restrict__ tcnt_all)
+                                                  uint32_t *__restrict__ vcnt_all, uint32_t *__restrict__ tcnt_all,
+                                                  uint64_t *__restrict__ NT_all, uint8_t *__restrict__ cidx_all)
 {
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, xcd_block(blockIdx.x, gridDim.x), lb)];
@@ -1816,72 +1822,35 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
         const uint64_t all = c0 & c1 & c2 & c3 & c4 & c5 & c6 & c7;
         const uint64_t any = c0 | c1 | c2 | c3 | c4 | c5 | c6 | c7;
         uint64_t act = any & ~all & wlt;
+        /* triangles per cube as three bit planes (counts < 8), and each polygonised cube's case */
+        uint64_t p0 = 0, p1 = 0, p2 = 0;
+        uint8_t *__restrict__ cidx = cidx_all + M.vox_off + ((size_t)x * g + y) * g + z0;
         while (act) {
             int b = __builtin_ctzll(act);
             act &= act - 1ull;
             int ci = (int)((c0 >> b) & 1ull) | (int)((c1 >> b) & 1ull) << 1 | (int)((c2 >> b) & 1ull) << 2 |
                      (int)((c3 >> b) & 1ull) << 3 | (int)((c4 >> b) & 1ull) << 4 | (int)((c5 >> b) & 1ull) << 5 |
                      (int)((c6 >> b) & 1ull) << 6 | (int)((c7 >> b) & 1ull) << 7;
-            nt += (uint32_t)tri_count(ci);
+            const uint32_t k = (uint32_t)tri_count(ci);
+            if (!k) continue;
+            nt += k;
+            cidx[b] = (uint8_t)ci;
+            p0 |= (uint64_t)(k & 1u) << b;
+            p1 |= (uint64_t)((k >> 1) & 1u) << b;
+            p2 |= (uint64_t)((k >> 2) & 1u) << b;
+        }
+        if (nt) {
+            uint64_t *__restrict__ NT = NT_all + 3 * (M.word_off + wi);
+            NT[0] = p0;
+            NT[1] = p1;
+            NT[2] = p2;
         }
     }
     vcnt[wi] = nv;
     tcnt[wi] = nt;
 }
 
-/* The 2 x 2 x 2 row words that a row word's cubes touch (rows x..x+1, y..y+1, words w..w+1):
-   sign, zero and slot masks and first vertex ids, staged in LDS by emission so that cube indices
-   and corner vertex ids are LDS lookups.  Entry ((px - x) * 2 + (py - y)) * 2 + (word - w). */
-struct Hood {
-    uint64_t N[8], Z[8], S[8][4];
-    uint32_t vb[8];
-};
-
-struct HoodView {
-    const Hood *H;
-    int x, y, w;
-    __device__ __forceinline__ int ent(int px, int py, int pz) const
-    {
-        return ((px - x) * 2 + (py - y)) * 2 + ((pz >> 6) - w);
-    }
-    __device__ __forceinline__ bool nbit(int px, int py, int pz) const { return (H->N[ent(px, py, pz)] >> (pz & 63)) & 1ull; }
-    __device__ __forceinline__ bool zbit(int px, int py, int pz) const { return (H->Z[ent(px, py, pz)] >> (pz & 63)) & 1ull; }
-    __device__ __forceinline__ int cube_index(int cx, int cy, int cz) const
-    {
-        int ci = (int)nbit(cx, cy, cz);
-        ci |= (int)nbit(cx + 1, cy, cz) << 1;
-        ci |= (int)nbit(cx + 1, cy + 1, cz) << 2;
-        ci |= (int)nbit(cx, cy + 1, cz) << 3;
-        ci |= (int)nbit(cx, cy, cz + 1) << 4;
-        ci |= (int)nbit(cx + 1, cy, cz + 1) << 5;
-        ci |= (int)nbit(cx + 1, cy + 1, cz + 1) << 6;
-        ci |= (int)nbit(cx, cy + 1, cz + 1) << 7;
-        return ci;
-    }
-    __device__ __forceinline__ uint32_t slot_id(int px, int py, int pz, int sl) const
-    {
-        const int e = ent(px, py, pz), b = pz & 63;
-        const uint64_t below = (1ull << b) - 1ull;
-        const uint64_t m0 = H->S[e][0], m1 = H->S[e][1], m2 = H->S[e][2], m3 = H->S[e][3];
-        uint32_t id = H->vb[e] + (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below) + __popcll(m2 & below) +
-                                            __popcll(m3 & below));
-        if (sl > 0) id += (uint32_t)((m0 >> b) & 1ull);
-        if (sl > 1) id += (uint32_t)((m1 >> b) & 1ull);
-        if (sl > 2) id += (uint32_t)((m2 >> b) & 1ull);
-        return id;
-    }
-    __device__ __forceinline__ uint32_t corner_vid(int cx, int cy, int cz, int e) const
-    {
-        const int qx = cx + (int)((kEdgeLX >> e) & 1u), qy = cy + (int)((kEdgeLY >> e) & 1u),
-                  qz = cz + (int)((kEdgeLZ >> e) & 1u);
-        const int a = (int)((kEdgeAX >> (2 * e)) & 3u);
-        const bool qneg = nbit(qx, qy, qz);
-        const int nx = qneg ? qx + (a == 0) : qx, ny = qneg ? qy + (a == 1) : qy, nz = qneg ? qz + (a == 2) : qz;
-        if (zbit(nx, ny, nz)) return slot_id(nx, ny, nz, 3);
-        return slot_id(qx, qy, qz, a);
-    }
-};
-
+/* row word wi of a grid -> (x, y, word) */
 __device__ __forceinline__ void decode_word(uint32_t wi, int g, int W, int &x, int &y, int &w)
 {
     const int row = (int)(wi / (uint32_t)W);
@@ -1890,170 +1859,151 @@ __device__ __forceinline__ void decode_word(uint32_t wi, int g, int W, int &x, i
     y = row - x * g;
 }
 
-/* the 2 x 2 x 2 row words (rows x..x+1, y..y+1, words w..w+1) that a word's cubes touch */
-__device__ __forceinline__ void load_hood8(Hood &H, const MCView &V, int x, int y, int w, int lane)
+/* last index k in [lo, hi] with a[k] <= v, given a[lo] <= v; a non-decreasing */
+__device__ __forceinline__ uint32_t last_le(const uint32_t *__restrict__ a, uint32_t lo, uint32_t hi, uint32_t v)
 {
-    if (lane < 8 * 7) {
-        const int e8 = lane / 7, f = lane - 7 * (lane / 7);
-        const int dx = e8 >> 2, dy = (e8 >> 1) & 1, dw = e8 & 1;
-        const int e = e8;
-        const int xx = x + dx, yy = y + dy, ww = w + dw;
-        const bool ok = xx < V.g && yy < V.g && ww < V.W;
-        const size_t wi = ok ? (((size_t)xx * V.g + yy) * V.W + ww) : 0;
-        if (f == 0) H.N[e] = ok ? V.N[wi] : 0ull;
-        else if (f == 1) H.Z[e] = ok ? V.Zm[wi] : 0ull;
-        else if (f < 6) H.S[e][f - 2] = ok ? V.S4[4 * wi + (f - 2)] : 0ull;
-        else H.vb[e] = ok ? V.vbw[wi] : 0u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (a[mid] <= v) lo = mid;
+        else hi = mid - 1;
     }
+    return lo;
 }
 
-/* per wavefront of k_mc_emit: owner cube (lane) of each of the word's triangles (at most 5 per
-   cube), and each cube's packed case and first triangle within the word */
-struct EmitLds {
-    uint64_t pk[64];
-    uint16_t ex[64];
-    uint8_t owner[64 * 5];
-};
-
-/* vertices (vertexInterp/lerp MarchingCubes.cu:715-734, edge oriented low->high) and triangles
-   (generateTriangleVerticesSMEM MarchingCubes.cu:736-851 order); one wavefront per active word.
-   Each polygonised cube also records its case and first triangle id (read by the smoothing
-   pass), and each vertex its (point, slot) key. */
-/* one active word's vertices and triangles; its hood is staged in LDS */
-struct EmitWord {
-    MCView V;
-    QsGrid G;
-    const float *F;
-    uint8_t *cidx;
-    uint32_t *tbase;
-    uint32_t wi;
-    int x, y, w;
-};
-
-__device__ __forceinline__ void emit_setup(EmitWord &E, const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre,
-                                           const uint32_t *__restrict__ wcoarse, const uint64_t *N_all, const uint64_t *Zm_all, const uint64_t *S4_all,
-                                           const uint32_t *vbw_all, const float *F_all, uint8_t *cidx_all,
-                                           uint32_t *tbase_all, uint32_t wi)
+/* active word of item i of a launch over items [0, n): base[k] is the first item of active word k
+   (non-decreasing, base[0] = 0).  The block's first and last items bound the search. */
+__device__ __forceinline__ uint32_t item_word(const uint32_t *__restrict__ base, uint32_t nact, uint32_t n, uint32_t i,
+                                              uint32_t *s_range)
 {
+    if (threadIdx.x < 2) {
+        const uint32_t q = min(blockIdx.x * 256u + (threadIdx.x ? 255u : 0u), n - 1u);
+        s_range[threadIdx.x] = last_le(base, 0, nact - 1u, q);
+    }
+    __syncthreads();
+    return last_le(base, s_range[0], s_range[1], min(i, n - 1u));
+}
+
+/* vertices (vertexInterp/lerp MarchingCubes.cu:715-734, edge oriented low->high), one thread per
+   vertex: its word from the active words' first vertex ids, its point and slot from the word's
+   four slot masks (vertex ids run over the word's points in z order, slots x, y, z, point) */
+__global__ void __launch_bounds__(256) k_mc_vertices(const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre,
+                                                     const uint32_t *__restrict__ wcoarse, const uint32_t *__restrict__ list,
+                                                     const uint32_t *__restrict__ avb, uint32_t nact, uint32_t nv,
+                                                     const uint64_t *__restrict__ S4_all, const float *__restrict__ F_all,
+                                                     float *__restrict__ vert, uint64_t *__restrict__ vkey)
+{
+    __shared__ uint32_t s_range[2];
+    const uint32_t v = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t k = item_word(avb, nact, nv, v, s_range);
+    if (v >= nv) return;
+    const uint32_t wi = list[k];
+    const uint32_t local = v - avb[k];
     const QsMol &M = mols[seg_coarse<QS_COARSE_ITEMS>(wpre, wcoarse, (uint64_t)wi)];
-    E.G = M.G;
-    E.V.N = N_all + M.word_off;
-    E.V.Zm = Zm_all + M.word_off;
-    E.V.S4 = S4_all + 4 * M.word_off;
-    E.V.vbw = vbw_all + M.word_off;
-    E.V.g = E.G.g;
-    E.V.W = (int)M.W;
-    E.V.lim = E.G.lim;
-    E.F = F_all + M.vox_off;
-    E.cidx = cidx_all + M.vox_off;
-    E.tbase = tbase_all + M.vox_off;
-    E.wi = wi;
-    decode_word((uint32_t)(wi - M.word_off), E.G.g, (int)M.W, E.x, E.y, E.w);
-}
-
-__device__ __forceinline__ void emit_word(const EmitWord &E, const Hood &H, EmitLds &L, int lane,
-                                          const uint32_t *__restrict__ tbw, float *__restrict__ vert,
-                                          uint64_t *__restrict__ vkey, int *__restrict__ tri)
-{
-    const int x = E.x, y = E.y, w = E.w, g = E.G.g, lim = E.G.lim;
-    const HoodView hv{&H, x, y, w};
-    const int z = w * 64 + lane;
-    const float dx = E.G.dx;
-    const float *__restrict__ F = E.F;
-    const uint64_t m0 = H.S[0][0], m1 = H.S[0][1], m2 = H.S[0][2], m3 = H.S[0][3];
-    const unsigned sl = (unsigned)((m0 >> lane) & 1ull) | (unsigned)((m1 >> lane) & 1ull) << 1 |
-                        (unsigned)((m2 >> lane) & 1ull) << 2 | (unsigned)((m3 >> lane) & 1ull) << 3;
+    const QsGrid G = M.G;
+    const int g = G.g;
+    int x, y, w;
+    decode_word((uint32_t)(wi - M.word_off), g, (int)M.W, x, y, w);
+    const uint64_t *__restrict__ m = S4_all + 4 * (size_t)wi;
+    const uint64_t m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+    /* the point: smallest b whose slots below b + 1 exceed local */
+    int lo = 0, hi = 63;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const uint64_t below = (2ull << mid) - 1ull;
+        const uint32_t c = (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below) + __popcll(m2 & below) + __popcll(m3 & below));
+        if (c > local) hi = mid;
+        else lo = mid + 1;
+    }
+    const int b = lo;
+    const uint64_t below = (1ull << b) - 1ull;
+    uint32_t r = local - (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below) + __popcll(m2 & below) + __popcll(m3 & below));
+    const uint32_t bits = (uint32_t)((m0 >> b) & 1ull) | (uint32_t)((m1 >> b) & 1ull) << 1 |
+                          (uint32_t)((m2 >> b) & 1ull) << 2 | (uint32_t)((m3 >> b) & 1ull) << 3;
+    uint32_t bb = bits;
+    while (r) {
+        bb &= bb - 1u;
+        r--;
+    }
+    const int sl = bb ? __builtin_ctz(bb) : 3;
+    const int z = w * 64 + b;
+    const float dx = G.dx;
     const size_t pidx = ((size_t)x * g + y) * g + z;
-    if (sl) {
-        uint32_t v = hv.slot_id(x, y, z, 0);
-        const uint64_t pkey = (uint64_t)pidx << 2;
-        float A[3] = {grid_pos(E.G.ox, x, dx), grid_pos(E.G.oy, y, dx), grid_pos(E.G.oz, z, dx)};
-        const float f0 = fat(F, g, x, y, z);
+    const float A[3] = {grid_pos(G.ox, x, dx), grid_pos(G.oy, y, dx), grid_pos(G.oz, z, dx)};
+    float P[3] = {A[0], A[1], A[2]};
+    if (sl < 3) {
+        const float *__restrict__ F = F_all + M.vox_off;
+        const float f0 = F[pidx];
+        const float f1 = F[pidx + (sl == 0 ? (size_t)g * g : (sl == 1 ? (size_t)g : 1))];
+        float B[3] = {A[0], A[1], A[2]};
+        B[sl] = A[sl] + dx;
+        const float t = (0.0f - f0) / (f1 - f0);
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            if (!(sl & (1u << a))) continue;
-            const float f1 = fat(F, g, x + (a == 0), y + (a == 1), z + (a == 2));
-            float B[3] = {A[0], A[1], A[2]};
-            B[a] = A[a] + dx;
-            const float t = (0.0f - f0) / (f1 - f0);
-#pragma unroll
-            for (int c3 = 0; c3 < 3; c3++) {
-                float dd = B[c3] - A[c3];
-                float s = t * dd;
-                vert[3 * (size_t)v + c3] = A[c3] + s;
-            }
-            vkey[v] = pkey | (uint64_t)a;
-            v++;
-        }
-        if (sl & 8u) {
-            vert[3 * (size_t)v + 0] = A[0];
-            vert[3 * (size_t)v + 1] = A[1];
-            vert[3 * (size_t)v + 2] = A[2];
-            vkey[v] = pkey | 3ull;
+        for (int c3 = 0; c3 < 3; c3++) {
+            float dd = B[c3] - A[c3];
+            float s = t * dd;
+            P[c3] = A[c3] + s;
         }
     }
-    int ci = 0;
-    if (x < lim && y < lim && z < lim) ci = hv.cube_index(x, y, z);
-    const uint64_t packed = c_tri[ci];
-    const int nt = tri_count_packed(packed);
-    /* exclusive prefix of the (< 8) triangle counts over the wave, by bit planes */
-    const uint64_t below = (1ull << lane) - 1ull;
-    uint32_t ex = 0, ntw = 0;
-#pragma unroll
-    for (int b = 0; b < 3; b++) {
-        const uint64_t m = __ballot((nt >> b) & 1);
-        ex += (uint32_t)__popcll(m & below) << b;
-        ntw += (uint32_t)__popcll(m) << b;
-    }
-    const uint32_t tw = tbw[E.wi];
-    if (nt) {
-        E.cidx[pidx] = (uint8_t)ci;
-        E.tbase[pidx] = tw + ex;
-        for (int i = 0; i < nt; i++) L.owner[ex + i] = (uint8_t)lane;
-    }
-    L.pk[lane] = packed;
-    L.ex[lane] = (uint16_t)ex;
-    wave_lds_sync();
-    /* one triangle corner per lane: coalesced stores, no per-cube serial loop */
-    for (uint32_t j = (uint32_t)lane; j < 3 * ntw; j += 64) {
-        const uint32_t tt = j / 3u, cc = j - 3u * tt;
-        const int c = L.owner[tt];
-        const int e = tri_entry(L.pk[c], 3 * (int)(tt - L.ex[c]) + (int)cc);
-        tri[3 * (size_t)(tw + tt) + cc] = (int)hv.corner_vid(x, y, w * 64 + c, e);
-    }
-    wave_lds_sync(); /* L is reused by the wave's next word */
+    vert[3 * (size_t)v + 0] = P[0];
+    vert[3 * (size_t)v + 1] = P[1];
+    vert[3 * (size_t)v + 2] = P[2];
+    vkey[v] = ((uint64_t)pidx << 2) | (uint64_t)sl;
 }
 
-/* EMIT_K consecutive active words per wavefront: their neighbourhood loads are issued together,
-   so the wave waits on global memory once for all of them */
-#define EMIT_K 2
-__global__ void __launch_bounds__(256) k_mc_emit(const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre,
-                                                 const uint32_t *__restrict__ wcoarse,
-                                                 const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
-                                                 const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
-                                                 const float *__restrict__ F_all, const uint32_t *__restrict__ list,
-                                                 uint32_t nact, const uint32_t *__restrict__ tbw, float *__restrict__ vert,
-                                                 uint64_t *__restrict__ vkey, int *__restrict__ tri,
-                                                 uint8_t *__restrict__ cidx_all, uint32_t *__restrict__ tbase_all)
+/* triangles (generateTriangleVerticesSMEM MarchingCubes.cu:736-851 order), one thread per
+   triangle: its word from the active words' first triangle ids, its cube from the word's
+   triangle-count bit planes; each polygonised cube also records its first triangle id (read by
+   the smoothing pass) */
+__global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre,
+                                                      const uint32_t *__restrict__ wcoarse, const uint32_t *__restrict__ list,
+                                                      const uint32_t *__restrict__ atb, uint32_t nact, uint32_t ntri,
+                                                      const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
+                                                      const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
+                                                      const uint64_t *__restrict__ NT_all, const uint8_t *__restrict__ cidx_all,
+                                                      int *__restrict__ tri, uint32_t *__restrict__ tbase_all)
 {
-    __shared__ Hood hoods[4][EMIT_K];
-    __shared__ EmitLds elds[4];
-    /* everything up to the lane's z is wave-uniform: scalar registers */
-    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t k0 = (xcd_block(blockIdx.x, gridDim.x) * 4 + wid) * EMIT_K;
-    if (k0 >= nact) return; /* whole wavefront; each wave has its own hoods, so no block barrier */
-    EmitWord E[EMIT_K];
-#pragma unroll
-    for (int j = 0; j < EMIT_K; j++) {
-        const uint32_t k = min(k0 + j, nact - 1);
-        const uint32_t wi = __builtin_amdgcn_readfirstlane(list[k]);
-        emit_setup(E[j], mols, wpre, wcoarse, N_all, Zm_all, S4_all, vbw_all, F_all, cidx_all, tbase_all, wi);
-        if (k0 + j < nact) load_hood8(hoods[wid][j], E[j].V, E[j].x, E[j].y, E[j].w, lane);
+    __shared__ uint32_t s_range[2];
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t k = item_word(atb, nact, ntri, t, s_range);
+    if (t >= ntri) return;
+    const uint32_t wi = list[k];
+    const uint32_t local = t - atb[k];
+    const QsMol &M = mols[seg_coarse<QS_COARSE_ITEMS>(wpre, wcoarse, (uint64_t)wi)];
+    const int g = M.G.g;
+    int x, y, w;
+    decode_word((uint32_t)(wi - M.word_off), g, (int)M.W, x, y, w);
+    const uint64_t *__restrict__ np = NT_all + 3 * (size_t)wi;
+    const uint64_t p0 = np[0], p1 = np[1], p2 = np[2];
+    int lo = 0, hi = 63;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const uint64_t below = (2ull << mid) - 1ull;
+        const uint32_t c = (uint32_t)(__popcll(p0 & below) + 2 * __popcll(p1 & below) + 4 * __popcll(p2 & below));
+        if (c > local) hi = mid;
+        else lo = mid + 1;
     }
-    wave_lds_sync();
+    const int b = lo;
+    const uint64_t below = (1ull << b) - 1ull;
+    const int j = (int)(local - (uint32_t)(__popcll(p0 & below) + 2 * __popcll(p1 & below) + 4 * __popcll(p2 & below)));
+    const int z = w * 64 + b;
+    const size_t pidx = ((size_t)x * g + y) * g + z;
+    const int ci = cidx_all[M.vox_off + pidx];
+    const uint64_t packed = c_tri[ci];
+    MCView V;
+    V.N = N_all + M.word_off;
+    V.Zm = Zm_all + M.word_off;
+    V.S4 = S4_all + 4 * M.word_off;
+    V.vbw = vbw_all + M.word_off;
+    V.g = g;
+    V.W = (int)M.W;
+    V.lim = M.G.lim;
+    int c[3];
 #pragma unroll
-    for (int j = 0; j < EMIT_K; j++)
-        if (k0 + j < nact) emit_word(E[j], hoods[wid][j], elds[wid], lane, tbw, vert, vkey, tri);
+    for (int cc = 0; cc < 3; cc++) c[cc] = (int)V.corner_vid(x, y, z, tri_entry(packed, 3 * j + cc));
+    tri[3 * (size_t)t + 0] = c[0];
+    tri[3 * (size_t)t + 1] = c[1];
+    tri[3 * (size_t)t + 2] = c[2];
+    if (j == 0) tbase_all[M.vox_off + pidx] = t;
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -2435,18 +2385,28 @@ void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *ou
 uint32_t qs_mc_count_blocks(QsGrid G, int W) { return nblk((uint64_t)G.g * W, 256) * (uint32_t)G.g; }
 
 void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
-                        uint32_t *tcnt, hipStream_t s)
+                        uint32_t *tcnt, uint64_t *NT, uint8_t *cidx, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(256), 0, s, B, N, Zm, S4, vcnt, tcnt);
+    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(256), 0, s, B, N, Zm, S4, vcnt, tcnt, NT, cidx);
 }
 
-void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint64_t *N, const uint64_t *Zm,
-                       const uint64_t *S4, const uint32_t *vbw, const float *F, const uint32_t *list, uint32_t nact,
-                       const uint32_t *tbw, float *vert, uint64_t *vkey, int *tri, uint8_t *cidx, uint32_t *tbase, hipStream_t s)
+void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
+                           const uint32_t *avb, uint32_t nact, uint32_t nv, const uint64_t *S4, const float *F, float *vert,
+                           uint64_t *vkey, hipStream_t s)
 {
-    if (!nact) return;
-    hipLaunchKernelGGL(k_mc_emit, dim3(nblk(nact, 4 * EMIT_K)), dim3(256), 0, s, mols, wpre, wcoarse, N, Zm, S4, vbw, F, list,
-                       nact, tbw, vert, vkey, tri, cidx, tbase);
+    if (!nv || !nact) return;
+    hipLaunchKernelGGL(k_mc_vertices, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, avb, nact, nv, S4, F,
+                       vert, vkey);
+}
+
+void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
+                            const uint32_t *atb, uint32_t nact, uint32_t ntri, const uint64_t *N, const uint64_t *Zm,
+                            const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT, const uint8_t *cidx, int *tri,
+                            uint32_t *tbase, hipStream_t s)
+{
+    if (!ntri || !nact) return;
+    hipLaunchKernelGGL(k_mc_triangles, dim3(nblk(ntri, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, atb, nact, ntri, N,
+                       Zm, S4, vbw, NT, cidx, tri, tbase);
 }
 
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
@@ -2502,13 +2462,14 @@ void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *pos, int
 
 /* scan helpers */
 void qs_launch_mc_scans(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
-                        uint32_t *tbw, uint32_t *list, uint32_t *totals, hipStream_t s)
+                        uint32_t *tbw, uint32_t *list, uint32_t *avb, uint32_t *atb, uint32_t *totals, hipStream_t s)
 {
     const uint64_t ntiles = qs_scan_tiles(nw);
     if (!nw) return;
     hipLaunchKernelGGL(k_mc_reduce3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles);
     hipLaunchKernelGGL(k_scan_planes, dim3(3), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
-    hipLaunchKernelGGL(k_mc_scan3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles, vbw, tbw, list);
+    hipLaunchKernelGGL(k_mc_scan3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles, vbw, tbw, list,
+                       avb, atb);
 }
 
 uint64_t qs_scan_tiles(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
