@@ -465,11 +465,13 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_TRY(ensure(tbase, std::max<uint64_t>(vox_total, 1) * 4));
     const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
     const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 2 * ((size_t)nmol + 1);
-    qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), nact, nv, as<uint64_t>(s4),
-                          as<float>(field), as<float>(vert), as<uint64_t>(vkey), stream);
-    qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), nact, nt_all,
-                           as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw), as<uint64_t>(ntb),
-                           as<uint8_t>(cidx), as<int>(tri), as<uint32_t>(tbase), stream);
+    QS_TRY(ensure(vstart, ((size_t)nv / 256 + 1) * 4));
+    QS_TRY(ensure(tstart, ((size_t)nt_all / 256 + 1) * 4));
+    qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart), nact,
+                          nv, as<uint64_t>(s4), as<float>(field), as<float>(vert), as<uint64_t>(vkey), stream);
+    qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart),
+                           nact, nt_all, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
+                           as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri), as<uint32_t>(tbase), stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
     if (with_atoms) {

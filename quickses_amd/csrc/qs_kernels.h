@@ -51,14 +51,15 @@ uint32_t qs_mc_count_blocks(QsGrid G, int W);
 void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
                         uint32_t *tcnt, uint64_t *NT, uint8_t *cidx, hipStream_t s);
 /* one thread per vertex / per triangle over the active words (list, with their first vertex ids
-   avb and first triangle ids atb) */
+   avb and first triangle ids atb); vstart / tstart receive each 256-item block's first active word
+   (nv / 256 + 1 and ntri / 256 + 1 entries) */
 void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
-                           const uint32_t *avb, uint32_t nact, uint32_t nv, const uint64_t *S4, const float *F, float *vert,
-                           uint64_t *vkey, hipStream_t s);
+                           const uint32_t *avb, uint32_t *vstart, uint32_t nact, uint32_t nv, const uint64_t *S4,
+                           const float *F, float *vert, uint64_t *vkey, hipStream_t s);
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
-                            const uint32_t *atb, uint32_t nact, uint32_t ntri, const uint64_t *N, const uint64_t *Zm,
-                            const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT, const uint8_t *cidx, int *tri,
-                            uint32_t *tbase, hipStream_t s);
+                            const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
+                            const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
+                            const uint8_t *cidx, int *tri, uint32_t *tbase, hipStream_t s);
 
 /* per-vertex stages */
 void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint32_t *cstart,

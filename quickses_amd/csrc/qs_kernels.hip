@@ -1870,17 +1870,70 @@ __device__ __forceinline__ uint32_t last_le(const uint32_t *__restrict__ a, uint
     return lo;
 }
 
-/* active word of item i of a launch over items [0, n): base[k] is the first item of active word k
-   (non-decreasing, base[0] = 0).  The block's first and last items bound the search. */
-__device__ __forceinline__ uint32_t item_word(const uint32_t *__restrict__ base, uint32_t nact, uint32_t n, uint32_t i,
-                                              uint32_t *s_range)
+/* first active word of every 256-item block of a launch over items [0, n): active word k holds
+   items [base[k], base[k + 1]) (base[nact] = n), so block b starts in the word whose range holds
+   item 256 b */
+__global__ void __launch_bounds__(256) k_block_starts(const uint32_t *__restrict__ base, uint32_t nact, uint32_t n,
+                                                      uint32_t *__restrict__ start)
 {
-    if (threadIdx.x < 2) {
-        const uint32_t q = min(blockIdx.x * 256u + (threadIdx.x ? 255u : 0u), n - 1u);
-        s_range[threadIdx.x] = last_le(base, 0, nact - 1u, q);
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= nact) return;
+    const uint32_t b0 = (base[k] + 255u) >> 8;
+    const uint32_t e = (k + 1 < nact) ? base[k + 1] : n;
+    for (uint32_t b = b0; (b << 8) < e; b++) start[b] = k;
+}
+
+/* The active words that hold a block's items, staged in LDS: their first items and word
+   indices, and the block's molecule when one molecule holds them all (else -1). */
+struct ItemWords {
+    uint32_t k0, nk;
+    int mol;
+    uint32_t base[257], word[257];
+};
+
+__device__ __forceinline__ void stage_item_words(ItemWords &S, const uint32_t *__restrict__ base,
+                                                 const uint32_t *__restrict__ list, const uint32_t *__restrict__ start,
+                                                 uint32_t nact, uint32_t nblocks, const uint64_t *__restrict__ wpre,
+                                                 const uint32_t *__restrict__ wcoarse)
+{
+    const uint32_t k0 = start[blockIdx.x];
+    const uint32_t k1 = (blockIdx.x + 1 < nblocks) ? start[blockIdx.x + 1] : nact - 1u;
+    const uint32_t nk = k1 - k0 + 1u;
+    if (threadIdx.x == 0) {
+        S.k0 = k0;
+        S.nk = nk;
+        const int m0 = seg_coarse<QS_COARSE_ITEMS>(wpre, wcoarse, (uint64_t)list[k0]);
+        const int m1 = seg_coarse<QS_COARSE_ITEMS>(wpre, wcoarse, (uint64_t)list[k1]);
+        S.mol = (m0 == m1) ? m0 : -1;
     }
+    if (nk <= 257u)
+        for (uint32_t t = threadIdx.x; t < nk; t += 256) {
+            S.base[t] = base[k0 + t];
+            S.word[t] = list[k0 + t];
+        }
     __syncthreads();
-    return last_le(base, s_range[0], s_range[1], min(i, n - 1u));
+}
+
+/* active word (index into the active list, and its row word) of item i of the block */
+__device__ __forceinline__ void item_word(const ItemWords &S, const uint32_t *__restrict__ base,
+                                          const uint32_t *__restrict__ list, uint32_t i, uint32_t &k, uint32_t &wi,
+                                          uint32_t &first)
+{
+    if (S.nk <= 257u) {
+        uint32_t lo = 0, hi = S.nk - 1u;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (S.base[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        k = S.k0 + lo;
+        wi = S.word[lo];
+        first = S.base[lo];
+    } else {
+        k = last_le(base, S.k0, S.k0 + S.nk - 1u, i);
+        wi = list[k];
+        first = base[k];
+    }
 }
 
 /* vertices (vertexInterp/lerp MarchingCubes.cu:715-734, edge oriented low->high), one thread per
@@ -1888,17 +1941,19 @@ __device__ __forceinline__ uint32_t item_word(const uint32_t *__restrict__ base,
    four slot masks (vertex ids run over the word's points in z order, slots x, y, z, point) */
 __global__ void __launch_bounds__(256) k_mc_vertices(const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre,
                                                      const uint32_t *__restrict__ wcoarse, const uint32_t *__restrict__ list,
-                                                     const uint32_t *__restrict__ avb, uint32_t nact, uint32_t nv,
-                                                     const uint64_t *__restrict__ S4_all, const float *__restrict__ F_all,
-                                                     float *__restrict__ vert, uint64_t *__restrict__ vkey)
+                                                     const uint32_t *__restrict__ avb, const uint32_t *__restrict__ vstart,
+                                                     uint32_t nact, uint32_t nv, const uint64_t *__restrict__ S4_all,
+                                                     const float *__restrict__ F_all, float *__restrict__ vert,
+                                                     uint64_t *__restrict__ vkey)
 {
-    __shared__ uint32_t s_range[2];
+    __shared__ ItemWords S;
+    stage_item_words(S, avb, list, vstart, nact, gridDim.x, wpre, wcoarse);
     const uint32_t v = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t k = item_word(avb, nact, nv, v, s_range);
     if (v >= nv) return;
-    const uint32_t wi = list[k];
-    const uint32_t local = v - avb[k];
-    const QsMol &M = mols[seg_coarse<QS_COARSE_ITEMS>(wpre, wcoarse, (uint64_t)wi)];
+    uint32_t k, wi, first;
+    item_word(S, avb, list, v, k, wi, first);
+    const uint32_t local = v - first;
+    const QsMol &M = mols[S.mol >= 0 ? S.mol : seg_coarse<QS_COARSE_ITEMS>(wpre, wcoarse, (uint64_t)wi)];
     const QsGrid G = M.G;
     const int g = G.g;
     int x, y, w;
@@ -1956,19 +2011,21 @@ __global__ void __launch_bounds__(256) k_mc_vertices(const QsMol *__restrict__ m
    the smoothing pass) */
 __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre,
                                                       const uint32_t *__restrict__ wcoarse, const uint32_t *__restrict__ list,
-                                                      const uint32_t *__restrict__ atb, uint32_t nact, uint32_t ntri,
+                                                      const uint32_t *__restrict__ atb, const uint32_t *__restrict__ tstart,
+                                                      uint32_t nact, uint32_t ntri,
                                                       const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
                                                       const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
                                                       const uint64_t *__restrict__ NT_all, const uint8_t *__restrict__ cidx_all,
                                                       int *__restrict__ tri, uint32_t *__restrict__ tbase_all)
 {
-    __shared__ uint32_t s_range[2];
+    __shared__ ItemWords S;
+    stage_item_words(S, atb, list, tstart, nact, gridDim.x, wpre, wcoarse);
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t k = item_word(atb, nact, ntri, t, s_range);
     if (t >= ntri) return;
-    const uint32_t wi = list[k];
-    const uint32_t local = t - atb[k];
-    const QsMol &M = mols[seg_coarse<QS_COARSE_ITEMS>(wpre, wcoarse, (uint64_t)wi)];
+    uint32_t k, wi, first;
+    item_word(S, atb, list, t, k, wi, first);
+    const uint32_t local = t - first;
+    const QsMol &M = mols[S.mol >= 0 ? S.mol : seg_coarse<QS_COARSE_ITEMS>(wpre, wcoarse, (uint64_t)wi)];
     const int g = M.G.g;
     int x, y, w;
     decode_word((uint32_t)(wi - M.word_off), g, (int)M.W, x, y, w);
@@ -2391,22 +2448,24 @@ void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const ui
 }
 
 void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
-                           const uint32_t *avb, uint32_t nact, uint32_t nv, const uint64_t *S4, const float *F, float *vert,
-                           uint64_t *vkey, hipStream_t s)
+                           const uint32_t *avb, uint32_t *vstart, uint32_t nact, uint32_t nv, const uint64_t *S4,
+                           const float *F, float *vert, uint64_t *vkey, hipStream_t s)
 {
     if (!nv || !nact) return;
-    hipLaunchKernelGGL(k_mc_vertices, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, avb, nact, nv, S4, F,
-                       vert, vkey);
+    hipLaunchKernelGGL(k_block_starts, dim3(nblk(nact, 256)), dim3(256), 0, s, avb, nact, nv, vstart);
+    hipLaunchKernelGGL(k_mc_vertices, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, avb, vstart, nact, nv,
+                       S4, F, vert, vkey);
 }
 
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
-                            const uint32_t *atb, uint32_t nact, uint32_t ntri, const uint64_t *N, const uint64_t *Zm,
-                            const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT, const uint8_t *cidx, int *tri,
-                            uint32_t *tbase, hipStream_t s)
+                            const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
+                            const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
+                            const uint8_t *cidx, int *tri, uint32_t *tbase, hipStream_t s)
 {
     if (!ntri || !nact) return;
-    hipLaunchKernelGGL(k_mc_triangles, dim3(nblk(ntri, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, atb, nact, ntri, N,
-                       Zm, S4, vbw, NT, cidx, tri, tbase);
+    hipLaunchKernelGGL(k_block_starts, dim3(nblk(nact, 256)), dim3(256), 0, s, atb, nact, ntri, tstart);
+    hipLaunchKernelGGL(k_mc_triangles, dim3(nblk(ntri, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, atb, tstart, nact,
+                       ntri, N, Zm, S4, vbw, NT, cidx, tri, tbase);
 }
 
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
