@@ -1406,14 +1406,30 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         has_shell = __syncthreads_or(threadIdx.x < 128 && m != 0u);
     }
     if (!has_shell) {
-        /* outside -> probe, else -dx */
-        for (int tx = 0; tx < P2R_TX; tx++) {
-            int gx = tx0 + tx;
-            if (gx >= x1 || gx >= g) break;
-            bool out = (ownout[tx * P2_TY + ty] >> tz) & 1u;
-            float res = out ? QS_PROBE : -dx;
-            if (vyz) field[((size_t)gx * g + gy) * g + gz] = res;
-            emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, res);
+        /* outside -> probe, else -dx: the sign words are the complemented outside bits, and each
+           thread writes 4 consecutive z of 4 rows */
+        const uint32_t zvalid = (tz0 + P2_TZ <= g) ? ~0u : ((1u << (g - tz0)) - 1u);
+        if (threadIdx.x < P2R_TX * P2_TY) {
+            const int tx = threadIdx.x / P2_TY, ty_ = threadIdx.x - tx * P2_TY;
+            const int gx = tx0 + tx, gy_ = ty0 + ty_;
+            if (gx < x1 && gx < g && gy_ < g) {
+                const size_t w32 = ((size_t)gx * g + gy_) * (2 * (size_t)W) + (tz0 >> 5);
+                negm[w32] = ~ownout[threadIdx.x] & zvalid;
+                zerom[w32] = 0u;
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < P2R_TX * P2_TY * (P2_TZ / 4) / 256; it++) {
+            const int q = it * 256 + (int)threadIdx.x;
+            const int row = q >> 3, z4 = (q & 7) * 4;
+            const int tx = row / P2_TY, ty_ = row - tx * P2_TY;
+            const int gx = tx0 + tx, gy_ = ty0 + ty_;
+            if (gx >= x1 || gx >= g || gy_ >= g) continue;
+            const uint32_t ob = ownout[row] >> z4;
+            float *__restrict__ dst = field + ((size_t)gx * g + gy_) * g + tz0 + z4;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (tz0 + z4 + i < g) dst[i] = ((ob >> i) & 1u) ? QS_PROBE : -dx;
         }
         return;
     }
