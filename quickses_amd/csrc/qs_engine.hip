@@ -158,8 +158,9 @@ int Engine::reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts
     cells += 1;
     verts += 1;
     for (DevBuf *b : {&outm, &shellm, &negm, &zerom}) QS_TRY(ensure(*b, nw * 8));
-    for (DevBuf *b : {&vcnt, &tcnt, &vbw, &tbw, &alist}) QS_TRY(ensure(*b, nw * 4));
+    for (DevBuf *b : {&vcnt, &tcnt, &vbw, &tbw, &alist, &avb, &atb}) QS_TRY(ensure(*b, nw * 4));
     QS_TRY(ensure(s4, nw * 32));
+    QS_TRY(ensure(ntb, nw * 24));
     QS_TRY(ensure(field, vox * 4));
     QS_TRY(ensure(tbase, vox * 4));
     QS_TRY(ensure(cidx, vox));
@@ -172,7 +173,7 @@ int Engine::reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts
     QS_TRY(ensure(vkey, verts * 8));
     QS_TRY(ensure(atom, verts * 4));
     for (DevBuf *b : {&tri, &tri_out, &tri_loc}) QS_TRY(ensure(*b, tris * 12));
-    for (DevBuf *b : {&keep, &keeppos}) QS_TRY(ensure(*b, (tris + 1) * 4));
+    for (DevBuf *b : {&vstart, &deg_blk, &keeppos, &tstart}) QS_TRY(ensure(*b, (tris / 256 + 2) * 4));
     const size_t m = (size_t)nmol + 1;
     QS_TRY(ensure(d_off, m * 8 + 64));
     QS_TRY(ensure(bb_out, m * 8 * sizeof(float)));
@@ -467,11 +468,17 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 2 * ((size_t)nmol + 1);
     QS_TRY(ensure(vstart, ((size_t)nv / 256 + 1) * 4));
     QS_TRY(ensure(tstart, ((size_t)nt_all / 256 + 1) * 4));
+    /* degenerate triangles per 256-triangle block and in total (totals[3]), counted by the
+       triangle kernel */
+    QS_TRY(ensure(deg_blk, ((size_t)nt_all / 256 + 1) * 4));
+    QS_HIP(hipMemsetAsync(deg_blk.p, 0, ((size_t)nt_all / 256 + 1) * 4, stream));
+    QS_HIP(hipMemsetAsync(as<uint32_t>(totals) + 3, 0, 4, stream));
     qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart), nact,
                           nv, as<uint64_t>(s4), as<float>(field), as<float>(vert), as<uint64_t>(vkey), stream);
     qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart),
                            nact, nt_all, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
-                           as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri), as<uint32_t>(tbase), stream);
+                           as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri), as<uint32_t>(tbase), as<uint32_t>(deg_blk),
+                           as<uint32_t>(totals) + 3, stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
     if (with_atoms) {
@@ -610,9 +617,7 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     }
 
     /* ---- degenerate filter (CudaSurf.cu:820-829), per-molecule kept counts ---- */
-    /* count first: without degenerate triangles (the usual case) the filter is the identity */
-    QS_HIP(hipMemsetAsync(as<uint32_t>(totals) + 3, 0, 4, stream));
-    qs_launch_tri_count_degenerate(as<int>(tri), nt_all, as<uint32_t>(totals) + 3, stream);
+    /* the triangle kernel counted them: without degenerate triangles the filter is the identity */
     uint32_t ndeg = 0;
     QS_HIP(hipMemcpyAsync(&ndeg, as<uint32_t>(totals) + 3, 4, hipMemcpyDeviceToHost, stream));
     QS_HIP(hipStreamSynchronize(stream));
@@ -631,16 +636,16 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
         tm.report(mols[0].G, n);
         return 0;
     }
-    QS_TRY(ensure(keep, (size_t)std::max<uint32_t>(nt_all, 1) * 4));
-    QS_TRY(ensure(keeppos, ((size_t)nt_all + 1) * 4));
+    /* kept triangles go to 256 b - (degenerate ones in blocks before b) + rank in block b */
+    const uint64_t nblk_t = ((uint64_t)nt_all + 255) / 256;
+    QS_TRY(ensure(keeppos, (nblk_t + 1) * 4));
     QS_TRY(ensure(tri_out, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
-    qs_launch_tri_keep(as<int>(tri), nt_all, as<uint32_t>(keep), stream);
-    QS_TRY(scan(as<uint32_t>(keep), as<uint32_t>(keeppos), nt_all, as<uint32_t>(keeppos) + nt_all));
+    QS_TRY(scan(as<uint32_t>(deg_blk), as<uint32_t>(keeppos), nblk_t, as<uint32_t>(keeppos) + nblk_t));
     qs_launch_tri_compact(as<int>(tri), nt_all, as<uint32_t>(keeppos), as<int>(tri_out), stream);
     std::vector<uint64_t> idx(nmol + 1);
     for (int m = 0; m <= nmol; m++) idx[m] = (m < nmol) ? mols[m].tri_off : nt_all;
     QS_HIP(hipMemcpyAsync(bidx.p, idx.data(), (size_t)(nmol + 1) * 8, hipMemcpyHostToDevice, stream));
-    qs_launch_gather_u32(as<uint32_t>(keeppos), as<uint64_t>(bidx), nmol + 1, as<uint32_t>(bounds), stream);
+    qs_launch_tri_kept_before(as<int>(tri), as<uint32_t>(keeppos), as<uint64_t>(bidx), nmol + 1, as<uint32_t>(bounds), stream);
     QS_HIP(hipMemcpyAsync(tri_off.data(), bounds.p, (size_t)(nmol + 1) * 4, hipMemcpyDeviceToHost, stream));
     QS_HIP(hipStreamSynchronize(stream));
     tm.mark("filter");

@@ -59,7 +59,8 @@ void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                             const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
                             const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
-                            const uint8_t *cidx, int *tri, uint32_t *tbase, hipStream_t s);
+                            const uint8_t *cidx, int *tri, uint32_t *tbase, uint32_t *deg_blk, uint32_t *deg_total,
+                            hipStream_t s);
 
 /* per-vertex stages */
 void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint32_t *cstart,
@@ -69,10 +70,10 @@ void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *v
                       const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
                       const uint32_t *tbase, const int *tri, const float *src, float *dst, hipStream_t s);
 
-/* degenerate-triangle filter */
-void qs_launch_tri_keep(const int *tri, uint32_t nt, uint32_t *keep, hipStream_t s);
-void qs_launch_tri_count_degenerate(const int *tri, uint32_t nt, uint32_t *count, hipStream_t s);
-void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *pos, int *out, hipStream_t s);
+/* degenerate-triangle filter: dpre = exclusive prefix of the per-256-triangle-block degenerate
+   counts that the triangle kernel leaves */
+void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *dpre, int *out, hipStream_t s);
+void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint64_t *idx, int n, uint32_t *out, hipStream_t s);
 void qs_launch_tri_localize(const int *tri, uint32_t nt, const uint32_t *tpre, const uint32_t *vpre, int nmol, int *out,
                             hipStream_t s);
 

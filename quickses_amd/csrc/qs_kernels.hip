@@ -2032,7 +2032,8 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
                                                       const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
                                                       const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
                                                       const uint64_t *__restrict__ NT_all, const uint8_t *__restrict__ cidx_all,
-                                                      int *__restrict__ tri, uint32_t *__restrict__ tbase_all)
+                                                      int *__restrict__ tri, uint32_t *__restrict__ tbase_all,
+                                                      uint32_t *__restrict__ deg_blk, uint32_t *__restrict__ deg_total)
 {
     __shared__ ItemWords S;
     stage_item_words(S, atb, list, tstart, nact, gridDim.x, wpre, wcoarse);
@@ -2077,6 +2078,13 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
     tri[3 * (size_t)t + 1] = c[1];
     tri[3 * (size_t)t + 2] = c[2];
     if (j == 0) tbase_all[M.vox_off + pidx] = t;
+    /* triangles with a repeated corner (zero grid points) are dropped from the output
+       (CudaSurf.cu:820-829): counted here per block and in total */
+    const uint64_t dm = __ballot(c[0] == c[1] || c[1] == c[2] || c[0] == c[2]);
+    if (dm && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(dm)) {
+        atomicAdd(&deg_blk[blockIdx.x], (uint32_t)__popcll(dm));
+        atomicAdd(deg_total, (uint32_t)__popcll(dm));
+    }
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -2258,42 +2266,45 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* degenerate-triangle filter (CudaSurf.cu:820-829), order preserving                           */
-__global__ void __launch_bounds__(256) k_tri_keep(const int *__restrict__ tri, uint32_t nt, uint32_t *__restrict__ keep)
+/* degenerate-triangle filter (CudaSurf.cu:820-829), order preserving: block b of 256 triangles
+   writes its kept ones from position 256 b - dpre[b] on, dpre being the exclusive prefix of the
+   per-block degenerate counts */
+__device__ __forceinline__ bool tri_degenerate(const int *__restrict__ tri, uint32_t t)
 {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    int x = tri[3 * (size_t)t], y = tri[3 * (size_t)t + 1], z = tri[3 * (size_t)t + 2];
-    keep[t] = (x != y && y != z && x != z) ? 1u : 0u;
+    const int x = tri[3 * (size_t)t], y = tri[3 * (size_t)t + 1], z = tri[3 * (size_t)t + 2];
+    return !(x != y && y != z && x != z);
 }
 
-/* number of degenerate triangles (a repeated corner) added to *count: one atomic per wavefront
-   that has any */
-__global__ void __launch_bounds__(256) k_tri_count_degenerate(const int *__restrict__ tri, uint32_t nt,
-                                                              uint32_t *__restrict__ count)
+__global__ void __launch_bounds__(256) k_tri_compact(const int *__restrict__ tri, uint32_t nt,
+                                                     const uint32_t *__restrict__ dpre, int *__restrict__ out)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool deg = false;
-    if (t < nt) {
-        const int x = tri[3 * (size_t)t], y = tri[3 * (size_t)t + 1], z = tri[3 * (size_t)t + 2];
-        deg = !(x != y && y != z && x != z);
+    __shared__ uint32_t wkeep[4];
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const bool keep = t < nt && !tri_degenerate(tri, t);
+    const uint64_t km = __ballot(keep);
+    if (lane == 0) wkeep[wid] = (uint32_t)__popcll(km);
+    __syncthreads();
+    uint32_t o = blockIdx.x * 256u - dpre[blockIdx.x] + (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
+    for (int k = 0; k < wid; k++) o += wkeep[k];
+    if (keep) {
+        out[3 * (size_t)o] = tri[3 * (size_t)t];
+        out[3 * (size_t)o + 1] = tri[3 * (size_t)t + 1];
+        out[3 * (size_t)o + 2] = tri[3 * (size_t)t + 2];
     }
-    const uint64_t m = __ballot(deg);
-    if (m && (threadIdx.x & 63) == (uint32_t)(__builtin_ctzll(m))) atomicAdd(count, (uint32_t)__popcll(m));
 }
 
-__global__ void __launch_bounds__(256) k_tri_compact(const int *__restrict__ tri, uint32_t nt, const uint32_t *__restrict__ pos,
-                                                     int *__restrict__ out)
+/* kept triangles before each of n boundaries idx[i] (molecule starts and the end) */
+__global__ void __launch_bounds__(256) k_tri_kept_before(const int *__restrict__ tri, const uint32_t *__restrict__ dpre,
+                                                         const uint64_t *__restrict__ idx, int n, uint32_t *__restrict__ out)
 {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    int x = tri[3 * (size_t)t], y = tri[3 * (size_t)t + 1], z = tri[3 * (size_t)t + 2];
-    if (x != y && y != z && x != z) {
-        uint32_t o = pos[t];
-        out[3 * (size_t)o] = x;
-        out[3 * (size_t)o + 1] = y;
-        out[3 * (size_t)o + 2] = z;
-    }
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t t = (uint32_t)idx[i];
+    const uint32_t b = t >> 8;
+    uint32_t d = dpre[b];
+    for (uint32_t u = b << 8; u < t; u++) d += tri_degenerate(tri, u) ? 1u : 0u;
+    out[i] = t - d;
 }
 
 /* plain device copy (16-byte vectors, grid-stride): the reference point for traffic counters */
@@ -2476,12 +2487,13 @@ void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                             const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
                             const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
-                            const uint8_t *cidx, int *tri, uint32_t *tbase, hipStream_t s)
+                            const uint8_t *cidx, int *tri, uint32_t *tbase, uint32_t *deg_blk, uint32_t *deg_total,
+                            hipStream_t s)
 {
     if (!ntri || !nact) return;
     hipLaunchKernelGGL(k_block_starts, dim3(nblk(nact, 256)), dim3(256), 0, s, atb, nact, ntri, tstart);
     hipLaunchKernelGGL(k_mc_triangles, dim3(nblk(ntri, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, atb, tstart, nact,
-                       ntri, N, Zm, S4, vbw, NT, cidx, tri, tbase);
+                       ntri, N, Zm, S4, vbw, NT, cidx, tri, tbase, deg_blk, deg_total);
 }
 
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
@@ -2502,18 +2514,6 @@ void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint3
                        sorted_ids, out);
 }
 
-void qs_launch_tri_count_degenerate(const int *tri, uint32_t nt, uint32_t *count, hipStream_t s)
-{
-    if (!nt) return;
-    hipLaunchKernelGGL(k_tri_count_degenerate, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, count);
-}
-
-void qs_launch_tri_keep(const int *tri, uint32_t nt, uint32_t *keep, hipStream_t s)
-{
-    if (!nt) return;
-    hipLaunchKernelGGL(k_tri_keep, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, keep);
-}
-
 void qs_launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s)
 {
     const uint64_t n16 = bytes / 16;
@@ -2529,10 +2529,16 @@ void qs_launch_tri_localize(const int *tri, uint32_t nt, const uint32_t *tpre, c
     hipLaunchKernelGGL(k_tri_localize, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, tpre, vpre, nmol, out);
 }
 
-void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *pos, int *out, hipStream_t s)
+void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *dpre, int *out, hipStream_t s)
 {
     if (!nt) return;
-    hipLaunchKernelGGL(k_tri_compact, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, pos, out);
+    hipLaunchKernelGGL(k_tri_compact, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, dpre, out);
+}
+
+void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint64_t *idx, int n, uint32_t *out, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_tri_kept_before, dim3(nblk(n, 256)), dim3(256), 0, s, tri, dpre, idx, n, out);
 }
 
 /* scan helpers */
