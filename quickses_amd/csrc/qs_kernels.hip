@@ -1465,6 +1465,19 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         asm volatile("" : "+v"(one)); /* opaque, so that the mask bits stay packed arithmetic */
         for (int col = threadIdx.x; col < LX * P2_TZ; col += 256) {
             const int lx = col >> 5, z = col & 31;
+            /* no outside voxel within R of this column's z in any of its rows: every minimum is
+               INF (its tie mask is never read) */
+            uint32_t near_any = 0;
+#pragma unroll
+            for (int ly = 0; ly < LY; ly++) {
+                const uint64_t m = rows[lx * LY + ly];
+                near_any |= __builtin_amdgcn_alignbit((uint32_t)(m >> 32), (uint32_t)m, (uint32_t)z);
+            }
+            if (!(near_any & ((1u << (2 * R + 1)) - 1u))) {
+#pragma unroll
+                for (int yy = 0; yy < P2_TY; yy++) DY[(lx * P2_TY + yy) * P2_TZ + z] = (uint8_t)INF;
+                continue;
+            }
             int v[LY];
 #pragma unroll
             for (int ly = 0; ly < LY; ly++) v[ly] = zdist2<R, INF>(rows[lx * LY + ly], z);
