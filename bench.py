@@ -403,6 +403,14 @@ def main():
     qs.set_option("timing", 0)
     stage_avg = {k: v / n_stage for k, v in stage_sum.items()}
 
+    # per-vertex normals of the last mesh (an extension beyond the reference's outputs): kernel
+    # time from HIP events, 12 B written and 2-14 field values read per vertex
+    d_nrm = hipmem.alloc(max(nv, 1) * 12)
+    try:
+        normals_ms = min(qs.vertex_normals_device(d_nrm) for _ in range(3))
+    finally:
+        hipmem.free(d_nrm)
+
     # dominant single-kernel stage against the HBM roofline
     kern = {k: v for k, v in stage_avg.items() if algorithmic_bytes(k, V, W["atoms"], gn ** 3) is not None}
     dom = max(kern, key=kern.get)
@@ -461,6 +469,7 @@ def main():
         "meshes_per_s": round(args.steps / elapsed, 3),
         "api_host_to_host_voxels_per_s": round(V / api_s, 1),
         "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
+        "normals_ms": round(normals_ms, 4),
         "roofline": roofline,
         "other_configs": other,
         "batched": batched,

@@ -106,6 +106,18 @@ int qs_batch_offsets(unsigned int *vert_off, unsigned int *tri_off);
  * (3 * tri_off[nmol] ints, vertex indices local to each molecule), atom ids (vert_off[nmol]). */
 int qs_batch_fetch(float *vert, int *tri, int *atom_id);
 
+/* Per-vertex normals of the last computation (plain or batch; vertices in the same order as its
+ * vertex arrays): the unit gradient of the SES field at each vertex's marching-cubes position
+ * (before smoothing), pointing outwards.  The reference's MeshData declares such a normals array
+ * but leaves it commented out (SmoothMesh.h:34); there is no reference routine to follow.  The
+ * gradient at a grid point is the central difference of the field (one-sided on the grid's
+ * faces); an edge vertex blends its edge's two end points with its edge parameter.
+ * qs_vertex_normals copies 3 * nvert floats to host memory (nvert must equal the last mesh's
+ * vertex count; after API_computeSES that is *NVert).  qs_vertex_normals_device writes them to
+ * a device array and reports the kernel time in *ms (may be NULL).  Returns 0 or an error code. */
+int qs_vertex_normals(float *out, unsigned int nvert);
+int qs_vertex_normals_device(float *d_out, float *ms);
+
 /* Device-to-device copy of `bytes` (multiple of 16, 16-byte aligned device pointers) by a
  * plain vector-copy kernel on the library's stream; *ms = its time.  A measurement reference:
  * hardware traffic counters of this kernel calibrate those of the SES kernels. */

@@ -61,6 +61,8 @@ def lib():
         _lib.or_field_fp64.argtypes = [ctypes.POINTER(_Grid), vp, vp, ctypes.c_int, vp, vp]
         _lib.or_mc_vertices_fp64.argtypes = [ctypes.POINTER(_Grid), ctypes.c_int, vp, vp, vp, vp]
         _lib.or_mc_vertices_fp64.restype = ctypes.c_int64
+        _lib.or_vertex_normals_fp64.argtypes = [ctypes.POINTER(_Grid), ctypes.c_int, vp, vp]
+        _lib.or_vertex_normals_fp64.restype = ctypes.c_int64
         _lib.or_mesh_free.argtypes = [ctypes.POINTER(_Mesh)]
         _lib.or_num_threads.restype = ctypes.c_int
         _lib.or_bench_slab.argtypes = [vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -228,6 +230,14 @@ def vertices_fp64(G, lim, fld, f64, nvert):
                                   _p(np.ascontiguousarray(f64, np.float64)), _p(out), _p(point))
     assert n == nvert
     return out, point
+
+
+def vertex_normals_fp64(G, lim, fld, nvert):
+    """FP64 unit field gradients at the vertices of edge_weld(G, lim, fld), same order."""
+    out = np.empty((nvert, 3), np.float64)
+    n = lib().or_vertex_normals_fp64(ctypes.byref(G._s), lim, _p(np.ascontiguousarray(fld, np.float32)), _p(out))
+    assert n == nvert
+    return out
 
 
 def num_threads():

@@ -682,6 +682,51 @@ int64_t or_mc_vertices_fp64(const or_grid *G, int lim, const float *F, const dou
     return v;
 }
 
+/* Per-vertex normals in FP64 from the FP32 field, vertices in edge-weld order.  The reference
+   leaves MeshData's normals array commented out (SmoothMesh.h:34-35), so there is nothing of it to
+   follow; this is the definition the library documents: the field's central-difference gradient
+   (one-sided on the grid faces) at the vertex's grid point, blended over its edge with the edge
+   parameter t = -f0 / (f1 - f0), normalised.  Returns the vertex count. */
+static double grad_axis64(const float *F, int64_t p, int64_t stride, int i, int g, double dx)
+{
+    if (i > 0 && i < g - 1) return ((double)F[p + stride] - (double)F[p - stride]) / (2.0 * dx);
+    if (i == 0) return ((double)F[p + stride] - (double)F[p]) / dx;
+    return ((double)F[p] - (double)F[p - stride]) / dx;
+}
+
+int64_t or_vertex_normals_fp64(const or_grid *G, int lim, const float *F, double *out)
+{
+    const int g = G->g;
+    const double dx = (double)G->reso;
+    const int64_t stride[3] = {(int64_t)g * g, g, 1};
+    int64_t v = 0;
+    for (int i = 0; i < g; i++)
+        for (int j = 0; j < g; j++)
+            for (int k = 0; k < g; k++) {
+                unsigned mk = slot_mask(F, g, lim, i, j, k);
+                if (!mk) continue;
+                const int64_t p = ((int64_t)i * g + j) * g + k;
+                const int c[3] = {i, j, k};
+                double g0[3];
+                for (int a = 0; a < 3; a++) g0[a] = grad_axis64(F, p, stride[a], c[a], g, dx);
+                for (int s = 0; s < 4; s++) {
+                    if (!(mk & (1u << s))) continue;
+                    double n[3] = {g0[0], g0[1], g0[2]};
+                    if (s < 3) {
+                        const int64_t q = p + stride[s];
+                        const double f0 = F[p], f1 = F[q];
+                        const double t = (0.0 - f0) / (f1 - f0);
+                        for (int a = 0; a < 3; a++)
+                            n[a] = (1.0 - t) * g0[a] + t * grad_axis64(F, q, stride[a], c[a] + (a == s), g, dx);
+                    }
+                    const double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                    for (int a = 0; a < 3; a++) out[3 * v + a] = len > 0.0 ? n[a] / len : 0.0;
+                    v++;
+                }
+            }
+    return v;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* FP64: the same mathematics with positions, radii and distances in double precision.          */
 void or_field_fp64(const or_grid *G, const float *pos, const float *rad, int n, uint8_t *lab,

@@ -102,6 +102,10 @@ int or_mc_edge_weld(const or_grid *G, int lim, const float *field, or_mesh *m);
 int64_t or_mc_vertices_fp64(const or_grid *G, int lim, const float *field, const double *field64, double *out,
                             int64_t *point);
 
+/* per-vertex unit field gradients in FP64 from the FP32 field, in edge_weld vertex order
+   (the library's normals definition; the reference declares but never fills such an array) */
+int64_t or_vertex_normals_fp64(const or_grid *G, int lim, const float *field, double *out);
+
 /* Reference-form soup (one vertex per triangle corner, per-cube interpolation) and the
    reference's quantised weld: vertices sorted by quantised position. */
 int or_mc_soup(const or_grid *G, int lim, const float *field, float **soup, int64_t *n_soup);

@@ -100,6 +100,10 @@ def lib():
     L.qs_serve_set.restype = ctypes.c_int
     L.qs_serve_stats.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]
     L.qs_serve_stats.restype = ctypes.c_int
+    L.qs_vertex_normals.argtypes = [vp, ctypes.c_uint]
+    L.qs_vertex_normals.restype = ctypes.c_int
+    L.qs_vertex_normals_device.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.qs_vertex_normals_device.restype = ctypes.c_int
     L.qs_serve_close.argtypes = []
     L.qs_serve_close.restype = ctypes.c_int
     for name in ("qs_synth_ball", "qs_synth_shell"):
@@ -399,6 +403,23 @@ def copy_device(d_dst, d_src, nbytes):
     L = lib()
     ms = ctypes.c_float(0)
     rc = L.qs_copy_device(ctypes.c_void_p(d_dst), ctypes.c_void_p(d_src), int(nbytes), ctypes.byref(ms))
+    _check(rc)
+    return ms.value
+
+
+def vertex_normals(nvert):
+    """Unit field gradients (outward) at the last mesh's vertices, in vertex order, as a
+    (nvert, 3) float32 array; nvert is the last mesh's vertex count (all molecules of a batch)."""
+    out = np.empty((int(nvert), 3), np.float32)
+    rc = lib().qs_vertex_normals(ctypes.c_void_p(out.ctypes.data), int(nvert))
+    _check(rc)
+    return out
+
+
+def vertex_normals_device(d_out):
+    """Normals into a device array (3 floats per vertex); returns the kernel time in ms."""
+    ms = ctypes.c_float(0)
+    rc = lib().qs_vertex_normals_device(ctypes.c_void_p(d_out), ctypes.byref(ms))
     _check(rc)
     return ms.value
 

@@ -972,6 +972,54 @@ int qs_batch_mesh(int m, float *vert, int *tri, int *atom_id)
     return 0;
 }
 
+/* normals of the last computation's vertices (batch order), on the library's stream */
+static int launch_normals(Engine &E, float *d_out)
+{
+    if (!E.batch_valid) return set_error(3, "no mesh to take normals of");
+    const int nmol = (int)E.mols.size();
+    const uint32_t *d_vpre = as<uint32_t>(E.d_off) + (nmol + 1);
+    qs_launch_vertex_normals(as<QsMol>(E.d_mols), d_vpre, d_vpre + nmol + 1, as<uint64_t>(E.vkey), E.nv, as<float>(E.field), d_out,
+                             E.stream);
+    QS_HIP(hipGetLastError());
+    return 0;
+}
+
+int qs_vertex_normals_device(float *d_out, float *ms)
+{
+    g_err = 0;
+    Engine &E = g_eng;
+    if (!E.batch_valid) return set_error(3, "no mesh to take normals of");
+    if (!d_out && E.nv) return set_error(3, "normals: output pointer is NULL");
+    hipEvent_t a, b;
+    QS_HIP(hipEventCreate(&a));
+    QS_HIP(hipEventCreate(&b));
+    QS_HIP(hipEventRecord(a, E.stream));
+    const int rc = launch_normals(E, d_out);
+    QS_HIP(hipEventRecord(b, E.stream));
+    QS_HIP(hipEventSynchronize(b));
+    float t = 0.0f;
+    QS_HIP(hipEventElapsedTime(&t, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (ms) *ms = t;
+    return rc;
+}
+
+int qs_vertex_normals(float *out, unsigned int nvert)
+{
+    g_err = 0;
+    Engine &E = g_eng;
+    if (!E.batch_valid) return set_error(3, "no mesh to take normals of");
+    if (nvert != E.nv) return set_error(3, "normals: %u vertices asked, the last mesh has %u", nvert, E.nv);
+    if (!E.nv) return 0;
+    if (!out) return set_error(3, "normals: output pointer is NULL");
+    QS_TRY(ensure(E.normals, (size_t)E.nv * 12));
+    QS_TRY(launch_normals(E, as<float>(E.normals)));
+    QS_HIP(hipMemcpyAsync(out, E.normals.p, (size_t)E.nv * 12, hipMemcpyDeviceToHost, E.stream));
+    QS_HIP(hipStreamSynchronize(E.stream));
+    return 0;
+}
+
 int qs_copy_device(void *dst, const void *src, uint64_t bytes, float *ms)
 {
     g_err = 0;

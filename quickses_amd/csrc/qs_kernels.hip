@@ -2279,6 +2279,59 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* Per-vertex normals (the normals field that MeshData leaves commented out, SmoothMesh.h:34):
+ * the unit field gradient at the vertex's marching-cubes position.  The gradient at a grid point
+ * is the central difference (one-sided at the grid faces) of the field; an edge vertex blends
+ * its two end points' gradients with the vertex's edge parameter t, a zero-point vertex takes
+ * its point's.  The field grows outwards (negative inside), so the normals point outwards. */
+__device__ __forceinline__ float grad_axis(const float *__restrict__ F, size_t p, size_t stride, int i, int g, float dx)
+{
+    if (i > 0 && i < g - 1) return (F[p + stride] - F[p - stride]) / (2.0f * dx);
+    if (i == 0) return (F[p + stride] - F[p]) / dx;
+    return (F[p] - F[p - stride]) / dx;
+}
+
+__global__ void __launch_bounds__(256) k_vertex_normals(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
+                                                        const uint32_t *__restrict__ vcoarse, const uint64_t *__restrict__ vkey,
+                                                        uint32_t nv, const float *__restrict__ F_all, float *__restrict__ out)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
+    const int g = M.G.g;
+    const float dx = M.G.dx;
+    const float *__restrict__ F = F_all + M.vox_off;
+    const uint64_t key = vkey[v];
+    const int sl = (int)(key & 3ull);
+    const uint32_t p = (uint32_t)(key >> 2), ug = (uint32_t)g;
+    const uint32_t xy = p / ug;
+    int c[3];
+    c[2] = (int)(p - xy * ug);
+    c[0] = (int)(xy / ug);
+    c[1] = (int)(xy - (uint32_t)c[0] * ug);
+    const size_t stride[3] = {(size_t)g * g, (size_t)g, 1};
+    float n[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) n[a] = grad_axis(F, p, stride[a], c[a], g, dx);
+    if (sl < 3) {
+        const size_t q = p + stride[sl];
+        const float f0 = F[p], f1 = F[q];
+        const float t = (0.0f - f0) / (f1 - f0);
+        const float s = 1.0f - t;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float g1 = grad_axis(F, q, stride[a], c[a] + (a == sl), g, dx);
+            n[a] = s * n[a] + t * g1;
+        }
+    }
+    const float len = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const float inv = len > 0.0f ? 1.0f / len : 0.0f;
+    out[3 * (size_t)v + 0] = n[0] * inv;
+    out[3 * (size_t)v + 1] = n[1] * inv;
+    out[3 * (size_t)v + 2] = n[2] * inv;
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* degenerate-triangle filter (CudaSurf.cu:820-829), order preserving: block b of 256 triangles
    writes its kept ones from position 256 b - dpre[b] on, dpre being the exclusive prefix of the
    per-block degenerate counts */
@@ -2516,6 +2569,13 @@ void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *v
     if (!nv) return;
     hipLaunchKernelGGL(k_smooth, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, N, Zm, S4, vbw, vkey, nv, cidx, tbase,
                        tri, src, dst);
+}
+
+void qs_launch_vertex_normals(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *vkey, uint32_t nv,
+                              const float *F, float *out, hipStream_t s)
+{
+    if (!nv) return;
+    hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, vkey, nv, F, out);
 }
 
 void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint32_t *cstart,

@@ -104,3 +104,13 @@ def test_server_without_gpu_returns_errors(qs):
     with pytest.raises(qs.QuickSESError):
         qs.Server(hbm_budget_bytes=1 << 30, result_bytes=1 << 26, workers=1)
     assert L.qs_serve_close() < 0
+
+
+def test_normals_without_a_mesh_is_an_error(qs):
+    """Before any mesh exists the normals entry points return an error (no GPU work)."""
+    import quickses_amd as q
+    L = q.lib()
+    if L.qs_batch_offsets(None, None) >= 0:
+        pytest.skip("a mesh exists in this process")
+    assert L.qs_vertex_normals(None, 0) != 0
+    assert L.qs_vertex_normals_device(None, None) != 0

@@ -1,0 +1,58 @@
+"""Per-vertex normals (unit field gradients) from the GPU against the FP64 oracle, which
+evaluates the same definition on the same FP32 field; plain, batch and device entry points."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# float32 gradients, blend and normalisation against float64 on identical field values
+TOL = 1e-5
+
+
+@pytest.mark.parametrize("seed,n,reso", [(21, 800, 0.5), (22, 400, 0.2), (23, 1500, 1.0)])
+def test_normals_match_fp64_oracle(qs, orc, seed, n, reso):
+    pos, rad = qs.synth_ball(seed, n)
+    print("case normals seed=%d n=%d reso=%.1f" % (seed, n, reso), flush=True)
+    V = qs.compute_ses(pos, rad, reso, 1)[0]
+    got = qs.vertex_normals(len(V)).astype(np.float64)
+    G, perm, cs, ce, sx, lab, fld = orc.field(pos, rad, reso)
+    ref = orc.vertex_normals_fp64(G, G.g - 1, fld, len(V))
+    err = np.abs(got - ref).max(axis=1)
+    print("max component error %.3g" % err.max(), flush=True)
+    assert err.max() < TOL
+    nz = np.linalg.norm(ref, axis=1) > 0
+    assert np.abs(np.linalg.norm(got[nz], axis=1) - 1).max() < 1e-5
+
+
+def test_batch_normals_equal_plain(qs):
+    mols = [qs.synth_ball(30 + i, n) for i, n in enumerate([300, 900])]
+    reso = [0.5, 0.4]
+    print("case batch normals", flush=True)
+    qs.compute_batch(mols, reso, 1)
+    vo, _ = qs.batch_offsets()
+    batch = qs.vertex_normals(vo[-1])
+    for m, ((pos, rad), r) in enumerate(zip(mols, reso)):
+        V = qs.compute_ses(pos, rad, r, 1)[0]
+        assert np.array_equal(qs.vertex_normals(len(V)), batch[vo[m]:vo[m + 1]])
+
+
+def test_normals_device_entry(qs):
+    from quickses_amd import hipmem
+    pos, rad = qs.synth_ball(24, 700)
+    V = qs.compute_ses(pos, rad, 0.5, 1)[0]
+    host = qs.vertex_normals(len(V))
+    d = hipmem.alloc(host.nbytes)
+    try:
+        ms = qs.vertex_normals_device(d)
+        out = hipmem.download(d, host.shape, np.float32)
+    finally:
+        hipmem.free(d)
+    assert ms >= 0.0
+    assert np.array_equal(out, host)
+
+
+def test_normals_wrong_count_is_an_error(qs):
+    pos, rad = qs.synth_ball(25, 300)
+    V = qs.compute_ses(pos, rad, 0.5, 1)[0]
+    with pytest.raises(qs.QuickSESError):
+        qs.vertex_normals(len(V) + 1)

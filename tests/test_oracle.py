@@ -355,3 +355,49 @@ def test_filter_degenerate():
 
     T = np.array([[0, 1, 2], [3, 3, 4], [5, 6, 5], [7, 8, 9], [1, 1, 1]], np.int32)
     assert np.array_equal(orc.filter_degenerate(T), np.array([[0, 1, 2], [7, 8, 9]], np.int32))
+
+
+# --------------------------------------------------------------------------------------------
+# per-vertex normals (no reference routine: the library's definition, pinned by mathematics)
+
+def _two_atoms(d=12.0):
+    pos = np.array([[0.0, 0.0, 0.0], [d, 0.3, -0.2]], np.float32)
+    rad = np.array([1.7, 1.52], np.float32)
+    return pos, rad
+
+
+def test_normals_exact_on_a_linear_field(orc):
+    """Central and one-sided differences are exact on an affine field, so every normal is the
+    unit coefficient vector (an axis swap or a sign slip fails this)."""
+    pos, rad = _two_atoms()
+    G = orc.grid_setup(pos, rad, 0.5)
+    g = G.g
+    a = np.array([0.3, -0.5, 0.8])
+    i, j, k = np.meshgrid(np.arange(g), np.arange(g), np.arange(g), indexing="ij")
+    fld = (a[0] * i + a[1] * j + a[2] * k - 0.37 * g).astype(np.float32).ravel()
+    mesh = orc.edge_weld(G, g - 1, fld)
+    nv = len(mesh[0])
+    assert nv > 100
+    n = orc.vertex_normals_fp64(G, g - 1, fld, nv)
+    # float32 field values: the differences carry their rounding, far below this bound
+    assert np.abs(n - a / np.linalg.norm(a)).max() < 1e-5
+    assert np.abs(np.linalg.norm(n, axis=1) - 1).max() < 1e-12
+
+
+def test_normals_radial_on_separate_spheres(orc):
+    """Two atoms far apart: the SES is their two spheres, whose outward normals are radial.  The
+    field is a distance to the nearest outside voxel, so at 0.25 A (7 voxels per radius) the
+    lattice tilts the gradient by a few degrees; an axis swap or a sign slip is tens of degrees
+    or more off."""
+    pos, rad = _two_atoms()
+    G, perm, cs, ce, sx, lab, fld = orc.field(pos, rad, 0.25)
+    V, T = orc.edge_weld(G, G.g - 1, fld)[:2]
+    n = orc.vertex_normals_fp64(G, G.g - 1, fld, len(V))
+    c = pos[np.argmin(((V[:, None, :] - pos[None, :, :]) ** 2).sum(-1), axis=1)].astype(np.float64)
+    radial = (V - c) / np.linalg.norm(V - c, axis=1, keepdims=True)
+    ang = np.degrees(np.arccos(np.clip((n * radial).sum(1), -1, 1)))
+    assert np.median(ang) < 5.0 and ang.max() < 15.0, (np.median(ang), ang.max())
+    swapped = n[:, [1, 0, 2]]
+    assert np.median(np.degrees(np.arccos(np.clip((swapped * radial).sum(1), -1, 1)))) > 15.0
+    # a plausible slip, the gradient with its sign flipped, points inwards
+    assert np.median(np.degrees(np.arccos(np.clip((-n * radial).sum(1), -1, 1)))) > 170.0
