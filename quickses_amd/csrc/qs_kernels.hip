@@ -1784,6 +1784,10 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
                                                   uint32_t *__restrict__ vcnt_all, uint32_t *__restrict__ tcnt_all,
                                                   uint64_t *__restrict__ NT_all, uint8_t *__restrict__ cidx_all)
 {
+    /* triangles per case, in LDS: the loop over a word's cubes below looks one up per cube */
+    __shared__ uint8_t s_ntri[256];
+    s_ntri[threadIdx.x] = (uint8_t)tri_count((int)threadIdx.x);
+    __syncthreads();
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, xcd_block(blockIdx.x, gridDim.x), lb)];
     const QsGrid G = M.G;
@@ -1860,7 +1864,7 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
             int ci = (int)((c0 >> b) & 1ull) | (int)((c1 >> b) & 1ull) << 1 | (int)((c2 >> b) & 1ull) << 2 |
                      (int)((c3 >> b) & 1ull) << 3 | (int)((c4 >> b) & 1ull) << 4 | (int)((c5 >> b) & 1ull) << 5 |
                      (int)((c6 >> b) & 1ull) << 6 | (int)((c7 >> b) & 1ull) << 7;
-            const uint32_t k = (uint32_t)tri_count(ci);
+            const uint32_t k = s_ntri[ci];
             if (!k) continue;
             nt += k;
             cidx[b] = (uint8_t)ci;
