@@ -845,18 +845,6 @@ __device__ __forceinline__ void lds_or_range(uint64_t *row, int L, int R)
     }
 }
 
-/* first index i in [0, n) with t[i] >= v (n if none); t non-decreasing */
-__device__ __forceinline__ int lds_lower_bound(const int *t, int n, int v)
-{
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (t[mid] < v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
 /* Field pass 1 by scatter into LDS: one block per slab of up to P1S_MAXROWS consecutive y rows of
    one x plane, all z.  Each candidate atom (x cells cx-1..cx+1 times the slab's y cells, every z
    cell) takes the rows whose y cell admits it and, lane by lane, ORs its exact near and inside
