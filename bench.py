@@ -136,7 +136,10 @@ def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_0
         for ch in chunks:
             bounds.append((k, k + len(ch)))
             k += len(ch)
-        qs.compute_batch(chunks[0][:4], voxel, smooth)  # warm-up
+        # warm-up: every launch sequence once, so that the library's grow-only buffers have their
+        # steady-state sizes before the timed pass (a stream of batches allocates once)
+        for a, b in bounds:
+            qs.compute_batch_device(d_pos.ptr + 12 * int(off[a]), d_rad.ptr + 4 * int(off[a]), counts[a:b], voxel, smooth)
         # device-resident: batched
         hipmem.synchronize()
         t0 = time.perf_counter()
