@@ -671,6 +671,29 @@ int *g_triangles = nullptr;
 int *g_atom_ids = nullptr;
 unsigned g_ntri_idx = 0, g_nvert = 0;
 
+/* API_freeMesh keeps the result's host arrays (the largest set so far) for the next call: the
+   device-to-host copies then land in pages that are already mapped.  Into freshly allocated
+   memory every page faults during the copy, which for a mesh of a few hundred MB costs several
+   times the copy (4 GB/s against 17 GB/s into mapped pageable memory). */
+struct HostArray {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+HostArray g_cur[3], g_spare[3]; /* vertices, triangles, atom ids */
+
+void *host_array(int k, size_t bytes)
+{
+    HostArray &s = g_spare[k];
+    if (s.p && s.cap >= bytes) {
+        g_cur[k] = s;
+        s = HostArray{};
+    } else {
+        g_cur[k].p = malloc(bytes);
+        g_cur[k].cap = g_cur[k].p ? bytes : 0;
+    }
+    return g_cur[k].p;
+}
+
 /* one-molecule batch with a given grid (stage-wise entry points) */
 int single_grid(Engine &E, const QsGrid &G, size_t &at_mc, uint32_t &nb_mc, size_t &at_wpre)
 {
@@ -714,9 +737,16 @@ int check_batch_args(const unsigned *atom_off, int nmol, const float *reso, std:
 
 void clear_host_result()
 {
-    free(g_vertices);
-    free(g_triangles);
-    free(g_atom_ids);
+    for (int k = 0; k < 3; k++) {
+        HostArray &c = g_cur[k], &s = g_spare[k];
+        if (c.p && c.cap > s.cap) {
+            free(s.p);
+            s = c;
+        } else {
+            free(c.p);
+        }
+        c = HostArray{};
+    }
     g_vertices = nullptr;
     g_triangles = nullptr;
     g_atom_ids = nullptr;
@@ -773,9 +803,9 @@ void API_computeSES(float resoSES, qs_float3 *atomPos, float *atomRad, unsigned 
     clear_host_result();
     auto fail = [&]() {
         clear_host_result();
-        g_vertices = (float *)malloc(12);
-        g_triangles = (int *)malloc(12);
-        g_atom_ids = (int *)malloc(4);
+        g_vertices = (float *)host_array(0, 12);
+        g_triangles = (int *)host_array(1, 12);
+        g_atom_ids = (int *)host_array(2, 4);
     };
     if (N > 1 && (!atomPos || !atomRad)) {
         set_error(3, "null atom arrays");
@@ -819,9 +849,14 @@ void API_computeSES(float resoSES, qs_float3 *atomPos, float *atomRad, unsigned 
     }
     g_nvert = E.nv;
     g_ntri_idx = 3 * E.nt;
-    g_vertices = (float *)malloc((size_t)std::max<uint32_t>(E.nv, 1) * 12);
-    g_triangles = (int *)malloc((size_t)std::max<uint32_t>(E.nt, 1) * 12);
-    g_atom_ids = (int *)malloc((size_t)std::max<uint32_t>(E.nv, 1) * 4);
+    g_vertices = (float *)host_array(0, (size_t)std::max<uint32_t>(E.nv, 1) * 12);
+    g_triangles = (int *)host_array(1, (size_t)std::max<uint32_t>(E.nt, 1) * 12);
+    g_atom_ids = (int *)host_array(2, (size_t)std::max<uint32_t>(E.nv, 1) * 4);
+    if (!g_vertices || !g_triangles || !g_atom_ids) {
+        set_error(1, "out of host memory");
+        fail();
+        return;
+    }
     bool ok = true;
     if (E.nv) {
         ok = hipMemcpyAsync(g_vertices, E.res_vert, (size_t)E.nv * 12, hipMemcpyDeviceToHost, E.stream) == hipSuccess &&
