@@ -118,6 +118,13 @@ int qs_batch_fetch(float *vert, int *tri, int *atom_id);
 int qs_vertex_normals(float *out, unsigned int nvert);
 int qs_vertex_normals_device(float *d_out, float *ms);
 
+/* Per-vertex colours of the last computation: vertex v gets the RGBA (4 floats) of its closest
+ * atom (the atom id of API_getAtomIdPerVert / the batch getters) from atom_rgba, natoms * 4 floats
+ * laid out like the atom input (a batch's molecules concatenated); a vertex without a closest
+ * atom gets zeros.  MeshData declares a colors array but leaves it commented out
+ * (SmoothMesh.h:35).  out receives nvert * 4 floats.  Returns 0 or an error code. */
+int qs_vertex_colors(const float *atom_rgba, unsigned int natoms, float *out, unsigned int nvert);
+
 /* Device-to-device copy of `bytes` (multiple of 16, 16-byte aligned device pointers) by a
  * plain vector-copy kernel on the library's stream; *ms = its time.  A measurement reference:
  * hardware traffic counters of this kernel calibrate those of the SES kernels. */

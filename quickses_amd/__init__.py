@@ -104,6 +104,8 @@ def lib():
     L.qs_vertex_normals.restype = ctypes.c_int
     L.qs_vertex_normals_device.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.qs_vertex_normals_device.restype = ctypes.c_int
+    L.qs_vertex_colors.argtypes = [vp, ctypes.c_uint, vp, ctypes.c_uint]
+    L.qs_vertex_colors.restype = ctypes.c_int
     L.qs_serve_close.argtypes = []
     L.qs_serve_close.restype = ctypes.c_int
     for name in ("qs_synth_ball", "qs_synth_shell"):
@@ -412,6 +414,16 @@ def vertex_normals(nvert):
     (nvert, 3) float32 array; nvert is the last mesh's vertex count (all molecules of a batch)."""
     out = np.empty((int(nvert), 3), np.float32)
     rc = lib().qs_vertex_normals(ctypes.c_void_p(out.ctypes.data), int(nvert))
+    _check(rc)
+    return out
+
+
+def vertex_colors(atom_rgba, nvert):
+    """RGBA of each vertex's closest atom: atom_rgba is (natoms, 4) float32 in the atom order of
+    the last call (a batch's molecules concatenated); returns (nvert, 4) float32."""
+    tab = np.ascontiguousarray(atom_rgba, np.float32).reshape(-1, 4)
+    out = np.empty((int(nvert), 4), np.float32)
+    rc = lib().qs_vertex_colors(ctypes.c_void_p(tab.ctypes.data), tab.shape[0], ctypes.c_void_p(out.ctypes.data), int(nvert))
     _check(rc)
     return out
 

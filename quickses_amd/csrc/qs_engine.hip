@@ -1055,6 +1055,29 @@ int qs_vertex_normals(float *out, unsigned int nvert)
     return 0;
 }
 
+int qs_vertex_colors(const float *atom_rgba, unsigned int natoms, float *out, unsigned int nvert)
+{
+    g_err = 0;
+    Engine &E = g_eng;
+    if (!E.batch_valid) return set_error(3, "no mesh to colour");
+    if (nvert != E.nv) return set_error(3, "colours: %u vertices asked, the last mesh has %u", nvert, E.nv);
+    if (natoms != E.n_atoms_total) return set_error(3, "colours: %u atom colours given, the last mesh has %u atoms", natoms,
+                                                    E.n_atoms_total);
+    if (!E.nv) return 0;
+    if (!atom_rgba || !out) return set_error(3, "colours: NULL array");
+    QS_TRY(ensure(E.normals, (size_t)E.nv * 16));
+    QS_TRY(ensure(E.colors_in, (size_t)std::max(natoms, 1u) * 16));
+    QS_HIP(hipMemcpyAsync(E.colors_in.p, atom_rgba, (size_t)natoms * 16, hipMemcpyHostToDevice, E.stream));
+    const int nmol = (int)E.mols.size();
+    const uint32_t *d_vpre = as<uint32_t>(E.d_off) + (nmol + 1);
+    qs_launch_vertex_colors(as<QsMol>(E.d_mols), d_vpre, d_vpre + nmol + 1, E.res_atom, E.nv, as<float4>(E.colors_in),
+                            as<float4>(E.normals), E.stream);
+    QS_HIP(hipGetLastError());
+    QS_HIP(hipMemcpyAsync(out, E.normals.p, (size_t)E.nv * 16, hipMemcpyDeviceToHost, E.stream));
+    QS_HIP(hipStreamSynchronize(E.stream));
+    return 0;
+}
+
 int qs_copy_device(void *dst, const void *src, uint64_t bytes, float *ms)
 {
     g_err = 0;

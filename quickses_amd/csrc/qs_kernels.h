@@ -73,6 +73,9 @@ void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *v
 /* unit field gradient at each vertex's marching-cubes position, 3 floats per vertex */
 void qs_launch_vertex_normals(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *vkey, uint32_t nv,
                               const float *F, float *out, hipStream_t s);
+/* colour of each vertex's closest atom from a per-atom RGBA table (batch atom layout) */
+void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const int *atom, uint32_t nv,
+                             const float4 *table, float4 *out, hipStream_t s);
 /* degenerate-triangle filter: dpre = exclusive prefix of the per-256-triangle-block degenerate
    counts that the triangle kernel leaves */
 void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *dpre, int *out, hipStream_t s);

@@ -2323,6 +2323,20 @@ __global__ void __launch_bounds__(256) k_vertex_normals(const QsMol *__restrict_
     out[3 * (size_t)v + 2] = n[2] * inv;
 }
 
+/* Per-vertex colours (the colors field that MeshData leaves commented out, SmoothMesh.h:35): the
+   colour of the vertex's closest atom, from a per-atom RGBA table laid out like the atom input
+   (a batch's molecules concatenated); a vertex without a closest atom gets (0, 0, 0, 0). */
+__global__ void __launch_bounds__(256) k_vertex_colors(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
+                                                       const uint32_t *__restrict__ vcoarse, const int *__restrict__ atom,
+                                                       uint32_t nv, const float4 *__restrict__ table, float4 *__restrict__ out)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
+    const int a = atom[v];
+    out[v] = (a >= 0) ? table[(size_t)M.atom_off + (uint32_t)a] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* degenerate-triangle filter (CudaSurf.cu:820-829), order preserving: block b of 256 triangles
    writes its kept ones from position 256 b - dpre[b] on, dpre being the exclusive prefix of the
@@ -2568,6 +2582,13 @@ void qs_launch_vertex_normals(const QsMol *mols, const uint32_t *vpre, const uin
 {
     if (!nv) return;
     hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, vkey, nv, F, out);
+}
+
+void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const int *atom, uint32_t nv,
+                             const float4 *table, float4 *out, hipStream_t s)
+{
+    if (!nv) return;
+    hipLaunchKernelGGL(k_vertex_colors, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, atom, nv, table, out);
 }
 
 void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint32_t *cstart,

@@ -114,3 +114,11 @@ def test_normals_without_a_mesh_is_an_error(qs):
         pytest.skip("a mesh exists in this process")
     assert L.qs_vertex_normals(None, 0) != 0
     assert L.qs_vertex_normals_device(None, None) != 0
+
+
+def test_colors_without_a_mesh_is_an_error(qs):
+    import quickses_amd as q
+    L = q.lib()
+    if L.qs_batch_offsets(None, None) >= 0:
+        pytest.skip("a mesh exists in this process")
+    assert L.qs_vertex_colors(None, 0, None, 0) != 0

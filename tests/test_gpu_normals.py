@@ -56,3 +56,23 @@ def test_normals_wrong_count_is_an_error(qs):
     V = qs.compute_ses(pos, rad, 0.5, 1)[0]
     with pytest.raises(qs.QuickSESError):
         qs.vertex_normals(len(V) + 1)
+
+
+def test_colors_are_the_closest_atoms_colours(qs):
+    """qs_vertex_colors gathers a per-atom RGBA table through the atom ids, plain and batched
+    (batch tables concatenated like the atoms)."""
+    mols = [qs.synth_ball(40 + i, n) for i, n in enumerate([400, 700])]
+    rng = np.random.default_rng(3)
+    tabs = [rng.random((len(r), 4)).astype(np.float32) for _, r in mols]
+    for (pos, rad), tab in zip(mols, tabs):
+        V, T, A = qs.compute_ses(pos, rad, 0.5, 1)
+        col = qs.vertex_colors(tab, len(V))
+        assert np.array_equal(col, tab[A])
+    print("case batch colours", flush=True)
+    out = qs.compute_batch(mols, 0.5, 1)
+    vo, _ = qs.batch_offsets()
+    col = qs.vertex_colors(np.concatenate(tabs), vo[-1])
+    for m, (V, T, A) in enumerate(out):
+        assert np.array_equal(col[vo[m]:vo[m + 1]], tabs[m][A])
+    with pytest.raises(qs.QuickSESError):
+        qs.vertex_colors(np.concatenate(tabs)[:-1], vo[-1])
