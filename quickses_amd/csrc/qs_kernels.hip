@@ -1331,18 +1331,21 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *_
 
 /* Field pass 2 for a fixed window half-width R: loops unrolled, y and x passes register-blocked
    (each thread keeps one column of the previous pass in registers). */
-/* squared distance from bit z + R of row m to its nearest set bit within R, or INF; z < 32 and
-   2R < 32, so bits z .. z + 2R are the low bits of one 32-bit funnel shift.  Branch free: the
-   sentinel bits stand at distance R + 1. */
-template <int R, int INF>
-__device__ __forceinline__ int zdist2(uint64_t m, int z)
+/* squared distance from bit z + R of row m to its nearest set bit within R; z < 32 and 2R < 32,
+   so bits z .. z + 2R are the low bits of one 32-bit funnel shift.  Branch free: bits above the
+   window are cleared and a sentinel stands at bit 31, so a row with no set bit within R gives a
+   distance of at least 31 - R and a square of at least (31 - R)^2 >= 484, above INF = 255 (every
+   in-window norm is at most 3R^2 <= 243 for R <= 9); callers clamp it to INF. */
+template <int R>
+__device__ __forceinline__ uint32_t zdist2(uint64_t m, int z)
 {
-    static_assert(2 * R < 31, "window wider than a funnel shift");
+    static_assert(2 * R < 31 && (31 - R) * (31 - R) > 255, "window wider than a funnel shift");
     const uint32_t w = __builtin_amdgcn_alignbit((uint32_t)(m >> 32), (uint32_t)m, (uint32_t)z); /* bit k = m bit z + k */
-    const int up = __builtin_ctz((w >> R) | (1u << (R + 1)));              /* nearest set bit at or above the centre */
-    const int dn = __builtin_clz((w << (31 - R)) | (1u << (30 - R)));      /* at or below it */
-    const int d = min(up, dn);
-    return d <= R ? d * d : INF;
+    const uint32_t ws = (w & ((1u << (2 * R + 1)) - 1u)) | 0x80000000u;
+    const uint32_t up = (uint32_t)__builtin_ctz(ws >> R);               /* nearest set bit at or above the centre */
+    const uint32_t dn = (uint32_t)__builtin_clz((ws << (31 - R)) | 1u); /* at or below it (31: none) */
+    const uint32_t d = min(up, dn);
+    return d * d;
 }
 
 template <int R>
@@ -1449,8 +1452,10 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         /* two y outputs per 32-bit register (16-bit halves): the tap sums, minima and equality
            masks in packed 16-bit integer arithmetic; every value stays below 2^9 */
         typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
-        u16x2 one = {1, 1};
-        asm volatile("" : "+v"(one)); /* opaque, so that the mask bits stay packed arithmetic */
+        /* 2^1, 2^2, 2^4, 2^8 per lane, opaque so that the mask bits stay packed multiply-adds */
+        u16x2 pw[4] = {{2, 2}, {4, 4}, {16, 16}, {256, 256}};
+#pragma unroll
+        for (int i = 0; i < 4; i++) asm volatile("" : "+v"(pw[i]));
         for (int col = threadIdx.x; col < LX * P2_TZ; col += 256) {
             const int lx = col >> 5, z = col & 31;
             /* no outside voxel within R of this column's z in any of its rows: every minimum is
@@ -1466,32 +1471,45 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
                 for (int yy = 0; yy < P2_TY; yy++) DY[(lx * P2_TY + yy) * P2_TZ + z] = (uint8_t)INF;
                 continue;
             }
-            int v[LY];
+            uint32_t v[LY];
 #pragma unroll
-            for (int ly = 0; ly < LY; ly++) v[ly] = zdist2<R, INF>(rows[lx * LY + ly], z);
+            for (int ly = 0; ly < LY; ly++) v[ly] = zdist2<R>(rows[lx * LY + ly], z);
             u16x2 P[LY - 1];
 #pragma unroll
-            for (int k = 0; k < LY - 1; k++) P[k] = __builtin_bit_cast(u16x2, (uint32_t)v[k] | ((uint32_t)v[k + 1] << 16));
+            for (int k = 0; k < LY - 1; k++) P[k] = __builtin_bit_cast(u16x2, v[k] | (v[k + 1] << 16));
 #pragma unroll
             for (int yy = 0; yy < P2_TY; yy += 2) {
-                u16x2 sb[2 * R + 1];
-                u16x2 best = {(uint16_t)INF, (uint16_t)INF};
+                /* taps below 2^11 (z squares below 1024 plus 49): no 16-bit lane overflows */
+                constexpr int NT = 2 * R + 1;
+                u16x2 sb[NT], t[NT];
 #pragma unroll
-                for (int b = 0; b <= 2 * R; b++) {
+                for (int b = 0; b < NT; b++) {
                     const uint16_t c = (uint16_t)((b - R) * (b - R));
                     sb[b] = P[yy + b] + (u16x2){c, c};
-                    best = __builtin_elementwise_min(best, sb[b]);
+                    t[b] = sb[b];
                 }
-                u16x2 ne = {0, 0};
+                /* minimum as a tree (no chain of dependent packed operations) */
 #pragma unroll
-                for (int b = 0; b <= 2 * R; b++)
-                    ne |= __builtin_elementwise_min(sb[b] - best, one) << (u16x2){(uint16_t)b, (uint16_t)b};
-                const uint32_t bb = __builtin_bit_cast(uint32_t, best);
-                const uint32_t mm = ~__builtin_bit_cast(uint32_t, ne) & (((1u << (2 * R + 1)) - 1u) * 0x10001u);
+                for (int s = 1; s < NT; s *= 2)
+#pragma unroll
+                    for (int b = 0; b + s < NT; b += 2 * s) t[b] = __builtin_elementwise_min(t[b], t[b + s]);
+                const u16x2 best = t[0];
+                /* tie mask: best + 1 - tap saturates to 1 at a minimum and to 0 elsewhere; the
+                   bits are combined as a tree as well, pairs of taps first */
+                const u16x2 best1 = best + (u16x2){1, 1};
+#pragma unroll
+                for (int b = 0; b < NT; b++) t[b] = __builtin_elementwise_sub_sat(best1, sb[b]);
+#pragma unroll
+                for (int s = 1, l = 0; s < NT; s *= 2, l++)
+#pragma unroll
+                    for (int b = 0; b + s < NT; b += 2 * s) t[b] = t[b + s] * pw[l] + t[b];
+                const u16x2 mm = t[0];
+                const uint32_t bb = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(best, (u16x2){(uint16_t)INF, (uint16_t)INF}));
+                const uint32_t mw = __builtin_bit_cast(uint32_t, mm);
                 DY[(lx * P2_TY + yy) * P2_TZ + z] = (uint8_t)bb;
                 DY[(lx * P2_TY + yy + 1) * P2_TZ + z] = (uint8_t)(bb >> 16);
-                BM[(lx * P2_TY + yy) * P2_TZ + z] = (BMask)(mm & 0xFFFFu);
-                BM[(lx * P2_TY + yy + 1) * P2_TZ + z] = (BMask)(mm >> 16);
+                BM[(lx * P2_TY + yy) * P2_TZ + z] = (BMask)(mw & 0xFFFFu);
+                BM[(lx * P2_TY + yy + 1) * P2_TZ + z] = (BMask)(mw >> 16);
             }
         }
     } else
@@ -1499,7 +1517,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         const int lx = col >> 5, z = col & 31;
         int v[LY];
 #pragma unroll
-        for (int ly = 0; ly < LY; ly++) v[ly] = zdist2<R, INF>(rows[lx * LY + ly], z);
+        for (int ly = 0; ly < LY; ly++) v[ly] = (int)zdist2<R>(rows[lx * LY + ly], z);
 #pragma unroll
         for (int yy = 0; yy < P2_TY; yy++) {
             int best = INF;
