@@ -1550,7 +1550,9 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
             for (int a = 0; a <= 2 * R; a++) nmin = min(nmin, u[tx + a] + (a - R) * (a - R));
             if (nmin < INF) {
                 const float vx = grid_pos(G.ox, gx, dx);
-                float min_dist = 100000.0f;
+                /* the smallest squared distance, rooted once at the end: sqrtf rounds correctly
+                   and is monotone, so the root of the minimum is the minimum of the roots */
+                float min_sq = __builtin_inff();
                 const uint64_t fw = nmin < 64 ? nf.w[0] : (nmin < 128 ? nf.w[1] : (nmin < 192 ? nf.w[2] : nf.w[3]));
                 if ((fw >> (nmin & 63)) & 1ull) {
                     /* trace the minimisers back through the separable passes: the x offsets
@@ -1566,14 +1568,15 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
                         while (bmask) {
                             const int b = __builtin_ctz(bmask) - R;
                             bmask &= bmask - 1u;
+                            /* dz = c^2 exactly, c <= R: the hardware root is within an ulp of c */
                             const int dz = dy - b * b;
-                            const int c = (int)sqrtf((float)dz);
+                            const int c = (int)(__builtin_amdgcn_sqrtf((float)dz) + 0.5f);
                             const uint64_t m = rows[(tx + R + a) * LY + (ty + R + b)];
                             const float qx = grid_pos(G.ox, gx + a, dx), qy = grid_pos(G.oy, gy + b, dx);
                             if ((m >> (tz + R + c)) & 1ull)
-                                min_dist = fminf(dist3(qx, qy, grid_pos(G.oz, gz + c, dx), vx, vy, vz), min_dist);
+                                min_sq = fminf(dist3_sq(qx, qy, grid_pos(G.oz, gz + c, dx), vx, vy, vz), min_sq);
                             if (c && ((m >> (tz + R - c)) & 1ull))
-                                min_dist = fminf(dist3(qx, qy, grid_pos(G.oz, gz - c, dx), vx, vy, vz), min_dist);
+                                min_sq = fminf(dist3_sq(qx, qy, grid_pos(G.oz, gz - c, dx), vx, vy, vz), min_sq);
                         }
                     }
                 } else {
@@ -1583,11 +1586,12 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
                         int a = (int)(int8_t)(o & 0xFF), b = (int)(int8_t)((o >> 8) & 0xFF), c = (int)(int8_t)((o >> 16) & 0xFF);
                         uint64_t m = rows[(tx + R + a) * LY + (ty + R + b)];
                         if (!((m >> (tz + R + c)) & 1ull)) continue;
-                        float d = dist3(grid_pos(G.ox, gx + a, dx), grid_pos(G.oy, gy + b, dx), grid_pos(G.oz, gz + c, dx),
-                                        vx, vy, vz);
-                        min_dist = fminf(d, min_dist);
+                        float d = dist3_sq(grid_pos(G.ox, gx + a, dx), grid_pos(G.oy, gy + b, dx), grid_pos(G.oz, gz + c, dx),
+                                           vx, vy, vz);
+                        min_sq = fminf(d, min_sq);
                     }
                 }
+                const float min_dist = sqrtf(min_sq);
                 res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
             }
         }
