@@ -1348,6 +1348,30 @@ __device__ __forceinline__ uint32_t zdist2(uint64_t m, int z)
     return d * d;
 }
 
+/* entry it of a per-plane bit list: the plane lx with cpre[lx] <= it < cpre[lx + 1] (binary
+   search) and the z of the (it - cpre[lx])-th set bit of cmask[lx] */
+template <int LX>
+__device__ __forceinline__ void list_column(const uint32_t *cpre, const uint32_t *cmask, uint32_t it, int &lx, int &z)
+{
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+        if (lo + step < LX && cpre[lo + step] <= it) lo += step;
+    lx = lo;
+    uint32_t w = cmask[lo], k = it - cpre[lo];
+    int zz = 0;
+#pragma unroll
+    for (int h = 16; h >= 1; h >>= 1) {
+        const uint32_t c = (uint32_t)__builtin_popcount(w & ((1u << h) - 1u));
+        if (k >= c) {
+            k -= c;
+            w >>= h;
+            zz += h;
+        }
+    }
+    z = zz;
+}
+
 template <int R>
 __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__restrict__ outm_all,
                                                  const uint64_t *__restrict__ shellm_all, float *__restrict__ field_all,
@@ -1360,7 +1384,11 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     static_assert(LZ <= 64 && 3 * R * R < INF, "window too wide for this kernel");
     __shared__ uint64_t rows[LX * LY];
     __shared__ uint32_t own[P2R_TX * P2_TY], ownout[P2R_TX * P2_TY];
-    __shared__ uint8_t DY[LX * P2_TY * P2_TZ];
+    __shared__ uint32_t DYw[LX * P2_TY * P2_TZ / 4];
+    uint8_t *DY = (uint8_t *)DYw;
+    /* y-pass work list: per halo x plane lx, the z columns (bit z) that need y minima, and the
+       exclusive prefix of their counts (cpre[LX] = total) */
+    __shared__ uint32_t shx[P2R_TX], cmask[LX], cpre[LX + 1];
     /* y offsets (bit b + R) attaining each DY value */
     using BMask = typename std::conditional<(2 * R + 1 <= 16), uint16_t, uint32_t>::type;
     __shared__ BMask BM[LX * P2_TY * P2_TZ];
@@ -1392,8 +1420,16 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         const uint64_t *__restrict__ src = threadIdx.x < 128 ? shellm : outm;
         uint32_t m = 0u;
         if (gx < x1 && gx < g && gy_ < g) m = (uint32_t)row_bits(src + ((size_t)gx * g + gy_) * W, W, tz0, P2_TZ);
-        if (threadIdx.x < 128) own[t] = m;
-        else ownout[t] = m;
+        if (threadIdx.x < 128) {
+            own[t] = m;
+            /* shell bits of each own x plane, over its P2_TY rows (consecutive lanes) */
+            uint32_t sx = m;
+#pragma unroll
+            for (int k = 1; k < P2_TY; k *= 2) sx |= (uint32_t)__shfl_xor((int)sx, k);
+            if (ty_ == 0) shx[tx] = sx;
+        } else {
+            ownout[t] = m;
+        }
         has_shell = __syncthreads_or(threadIdx.x < 128 && m != 0u);
     }
     if (!has_shell) {
@@ -1435,6 +1471,16 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         rows[row] = m;
         fl |= (m != 0ull);
     }
+    /* every y minimum starts unreachable; only listed columns are computed */
+    for (int i = threadIdx.x; i < LX * P2_TY * P2_TZ / 4; i += 256) DYw[i] = 0x01010101u * INF;
+    /* a column (lx, z) is read by the x pass only for shell voxels of own planes lx - 2R .. lx */
+    uint32_t need = 0;
+    if (threadIdx.x < LX) {
+        const int lx = threadIdx.x;
+#pragma unroll
+        for (int tx = 0; tx < P2R_TX; tx++)
+            if (tx >= lx - 2 * R && tx <= lx) need |= shx[tx];
+    }
     if (!__syncthreads_or(fl)) {
         /* no outside voxel within reach: every voxel of the tile is -dx */
         for (int tx = 0; tx < P2R_TX; tx++) {
@@ -1446,31 +1492,49 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         return;
     }
 
-    /* z then y, one (lx, z) column per thread: the squared z distance of each row to its nearest
-       outside voxel within R, straight from the row bits, then the y minima over 2R+1 rows */
+    /* the work list: columns needed by a shell voxel with an outside voxel within R of their z
+       in some row (else every minimum is INF, as filled) */
+    static_assert(LX <= 64, "one lane per halo x plane");
+    if (threadIdx.x < 64) {
+        const int lx = threadIdx.x;
+        uint32_t c = 0;
+        if (lx < LX) {
+            uint64_t o = 0;
+#pragma unroll
+            for (int ly = 0; ly < LY; ly++) o |= rows[lx * LY + ly];
+            /* bit z: any of bits z .. z + 2R (bit k of a row is z = k - R) */
+            uint64_t near = 0;
+#pragma unroll
+            for (int k = 0; k <= 2 * R; k++) near |= o >> k;
+            c = (uint32_t)near & need;
+            cmask[lx] = c;
+        }
+        const uint32_t cnt = (uint32_t)__builtin_popcount(c);
+        uint32_t inc = cnt;
+#pragma unroll
+        for (int k = 1; k < 64; k *= 2) {
+            const uint32_t t = (uint32_t)__shfl_up((int)inc, k);
+            if (lx >= k) inc += t;
+        }
+        if (lx < LX) cpre[lx] = inc - cnt;
+        if (lx == LX - 1) cpre[LX] = inc;
+    }
+    __syncthreads();
+    const uint32_t ntodo = cpre[LX];
+
+    /* z then y, one listed (lx, z) column per thread: the squared z distance of each row to its
+       nearest outside voxel within R, straight from the row bits, then the y minima over 2R+1 rows */
     if constexpr (2 * R + 1 <= 16) {
         /* two y outputs per 32-bit register (16-bit halves): the tap sums, minima and equality
-           masks in packed 16-bit integer arithmetic; every value stays below 2^9 */
+           masks in packed 16-bit integer arithmetic */
         typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
         /* 2^1, 2^2, 2^4, 2^8 per lane, opaque so that the mask bits stay packed multiply-adds */
         u16x2 pw[4] = {{2, 2}, {4, 4}, {16, 16}, {256, 256}};
 #pragma unroll
         for (int i = 0; i < 4; i++) asm volatile("" : "+v"(pw[i]));
-        for (int col = threadIdx.x; col < LX * P2_TZ; col += 256) {
-            const int lx = col >> 5, z = col & 31;
-            /* no outside voxel within R of this column's z in any of its rows: every minimum is
-               INF (its tie mask is never read) */
-            uint32_t near_any = 0;
-#pragma unroll
-            for (int ly = 0; ly < LY; ly++) {
-                const uint64_t m = rows[lx * LY + ly];
-                near_any |= __builtin_amdgcn_alignbit((uint32_t)(m >> 32), (uint32_t)m, (uint32_t)z);
-            }
-            if (!(near_any & ((1u << (2 * R + 1)) - 1u))) {
-#pragma unroll
-                for (int yy = 0; yy < P2_TY; yy++) DY[(lx * P2_TY + yy) * P2_TZ + z] = (uint8_t)INF;
-                continue;
-            }
+        for (uint32_t it = threadIdx.x; it < ntodo; it += 256) {
+            int lx, z;
+            list_column<LX>(cpre, cmask, it, lx, z);
             uint32_t v[LY];
 #pragma unroll
             for (int ly = 0; ly < LY; ly++) v[ly] = zdist2<R>(rows[lx * LY + ly], z);
@@ -1513,8 +1577,9 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
             }
         }
     } else
-    for (int col = threadIdx.x; col < LX * P2_TZ; col += 256) {
-        const int lx = col >> 5, z = col & 31;
+    for (uint32_t it = threadIdx.x; it < ntodo; it += 256) {
+        int lx, z;
+        list_column<LX>(cpre, cmask, it, lx, z);
         int v[LY];
 #pragma unroll
         for (int ly = 0; ly < LY; ly++) v[ly] = (int)zdist2<R>(rows[lx * LY + ly], z);
