@@ -196,6 +196,8 @@ struct Engine {
     uint32_t nv = 0, nt_all = 0, nt = 0, n_active = 0;
     size_t p1_lds = 0; /* dynamic LDS of the field pass 1 launch (largest molecule of the batch) */
     bool batch_valid = false; /* result arrays, vert_off / tri_off and their device copies belong to run_batch */
+    /* field holds the shell voxels' values only (field pass 2); the others come from outm / shellm */
+    bool field_sparse = false;
     float *res_vert = nullptr;
     int *res_tri = nullptr, *res_atom = nullptr;
 
@@ -297,6 +299,7 @@ struct Engine {
     int run_batch(const float *d_pos, const float *d_rad, const std::vector<MolIn> &in, int smooth,
                   const std::vector<BBox> *host_bbox);
     int field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm);
+    int fill_field();
     int mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer &tm, bool with_atoms);
     int group_tables(P2Group &grp);
 };

@@ -381,7 +381,19 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
                         as<int>(grp.tab->norm), as<int>(grp.tab->end), grp.tab->flags, stream);
         QS_HIP(hipGetLastError());
     }
+    field_sparse = true;
     tm.mark("pass2");
+    return 0;
+}
+
+/* every voxel of every molecule's field (pass 2 leaves non-shell voxels unstored) */
+int Engine::fill_field()
+{
+    if (!field_sparse) return 0;
+    for (const QsMol &M : mols)
+        qs_launch_field_fill(as<uint64_t>(outm) + M.word_off, as<uint64_t>(shellm) + M.word_off, as<float>(field) + M.vox_off,
+                             M.G.g, (int)M.W, M.G.dx, stream);
+    QS_HIP(hipGetLastError());
     return 0;
 }
 
@@ -474,7 +486,8 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_HIP(hipMemsetAsync(deg_blk.p, 0, ((size_t)nt_all / 256 + 1) * 4, stream));
     QS_HIP(hipMemsetAsync(as<uint32_t>(totals) + 3, 0, 4, stream));
     qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart), nact,
-                          nv, as<uint64_t>(s4), as<float>(field), as<float>(vert), as<uint64_t>(vkey), stream);
+                          nv, as<uint64_t>(s4), as<float>(field), as<uint64_t>(outm),
+                          field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey), stream);
     qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart),
                            nact, nt_all, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
                            as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri), as<uint32_t>(tbase), as<uint32_t>(deg_blk),
@@ -1013,8 +1026,8 @@ static int launch_normals(Engine &E, float *d_out)
     if (!E.batch_valid) return set_error(3, "no mesh to take normals of");
     const int nmol = (int)E.mols.size();
     const uint32_t *d_vpre = as<uint32_t>(E.d_off) + (nmol + 1);
-    qs_launch_vertex_normals(as<QsMol>(E.d_mols), d_vpre, d_vpre + nmol + 1, as<uint64_t>(E.vkey), E.nv, as<float>(E.field), d_out,
-                             E.stream);
+    qs_launch_vertex_normals(as<QsMol>(E.d_mols), d_vpre, d_vpre + nmol + 1, as<uint64_t>(E.vkey), E.nv, as<float>(E.field),
+                             as<uint64_t>(E.outm), E.field_sparse ? as<uint64_t>(E.shellm) : nullptr, d_out, E.stream);
     QS_HIP(hipGetLastError());
     return 0;
 }
@@ -1166,7 +1179,12 @@ int64_t qs_fetch(const char *what, void *host, int64_t bytes)
             }
         return (int64_t)V0;
     }
-    else if (w == "field") { src = E.field.p; need = (int64_t)V * 4; }
+    else if (w == "field") {
+        if (E.fill_field() != 0) return -1;
+        if (hipStreamSynchronize(E.stream) != hipSuccess) return -1;
+        src = E.field.p;
+        need = (int64_t)V * 4;
+    }
     else if (w == "perm") { src = E.perm.p; need = (int64_t)E.n_atoms_total * 4; }
     else if (w == "cells") { src = E.cse.p; need = (int64_t)E.ncell_total * 8; }
     else if (w == "vert_unsmoothed") { src = E.vert.p; need = (int64_t)E.nv * 12; }
@@ -1221,6 +1239,8 @@ int qs_stage_pass2(const float *fp, const int *ip, const uint8_t *labels, float 
                         grp.tab->flags, E.stream);
         QS_HIP(hipGetLastError());
     }
+    E.field_sparse = true;
+    QS_TRY(E.fill_field());
     QS_HIP(hipStreamSynchronize(E.stream));
     QS_HIP(hipMemcpy(field, E.field.p, V * 4, hipMemcpyDeviceToHost));
     return 0;
@@ -1241,6 +1261,7 @@ int qs_stage_mesh(const float *fp, const int *ip, const float *field, unsigned i
     const uint64_t V = E.vox_total;
     QS_TRY(ensure(E.field, V * 4));
     QS_HIP(hipMemcpyAsync(E.field.p, field, V * 4, hipMemcpyHostToDevice, E.stream));
+    E.field_sparse = false; /* the caller's field holds every voxel */
     const int W = (int)E.mols[0].W;
     QS_TRY(ensure(E.negm, E.nw_total * 8));
     QS_TRY(ensure(E.zerom, E.nw_total * 8));

@@ -40,6 +40,10 @@ void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_
                      uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
                      QsNormFlags nf, hipStream_t s);
 
+/* field pass 2 stores the shell voxels' values only: the others (probe radius outside, -dx
+   inside) of one molecule's field, written from its outside / shell masks */
+void qs_launch_field_fill(const uint64_t *outm, const uint64_t *shellm, float *F, int g, int W, float dx, hipStream_t s);
+
 /* stage-wise helpers (one molecule) */
 void qs_launch_field_signs(const float *F, int g, int W, uint32_t *negm, uint32_t *zerom, hipStream_t s);
 void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *outm, uint64_t *shellm, hipStream_t s);
@@ -52,10 +56,12 @@ void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const ui
                         uint32_t *tcnt, uint64_t *NT, uint8_t *cidx, hipStream_t s);
 /* one thread per vertex / per triangle over the active words (list, with their first vertex ids
    avb and first triangle ids atb); vstart / tstart receive each 256-item block's first active word
-   (nv / 256 + 1 and ntri / 256 + 1 entries) */
+   (nv / 256 + 1 and ntri / 256 + 1 entries).  The field F holds the shell voxels' values only,
+   the others come from the outside / shell masks (shellm == nullptr: F holds every voxel). */
 void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                            const uint32_t *avb, uint32_t *vstart, uint32_t nact, uint32_t nv, const uint64_t *S4,
-                           const float *F, float *vert, uint64_t *vkey, hipStream_t s);
+                           const float *F, const uint64_t *outm, const uint64_t *shellm, float *vert, uint64_t *vkey,
+                           hipStream_t s);
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                             const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
                             const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
@@ -70,9 +76,10 @@ void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *v
                       const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
                       const uint32_t *tbase, const int *tri, const float *src, float *dst, hipStream_t s);
 
-/* unit field gradient at each vertex's marching-cubes position, 3 floats per vertex */
+/* unit field gradient at each vertex's marching-cubes position, 3 floats per vertex (field as for
+   the vertex kernel) */
 void qs_launch_vertex_normals(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *vkey, uint32_t nv,
-                              const float *F, float *out, hipStream_t s);
+                              const float *F, const uint64_t *outm, const uint64_t *shellm, float *out, hipStream_t s);
 /* colour of each vertex's closest atom from a per-atom RGBA table (batch atom layout) */
 void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const int *atom, uint32_t nv,
                              const float4 *table, float4 *out, hipStream_t s);

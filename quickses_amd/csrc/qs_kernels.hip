@@ -1433,8 +1433,8 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         has_shell = __syncthreads_or(threadIdx.x < 128 && m != 0u);
     }
     if (!has_shell) {
-        /* outside -> probe, else -dx: the sign words are the complemented outside bits, and each
-           thread writes 4 consecutive z of 4 rows */
+        /* no field values to store (only shell voxels hold one; the others are the probe radius
+           outside and -dx inside): the sign words are the complemented outside bits */
         const uint32_t zvalid = (tz0 + P2_TZ <= g) ? ~0u : ((1u << (g - tz0)) - 1u);
         if (threadIdx.x < P2R_TX * P2_TY) {
             const int tx = threadIdx.x / P2_TY, ty_ = threadIdx.x - tx * P2_TY;
@@ -1444,19 +1444,6 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
                 negm[w32] = ~ownout[threadIdx.x] & zvalid;
                 zerom[w32] = 0u;
             }
-        }
-#pragma unroll
-        for (int it = 0; it < P2R_TX * P2_TY * (P2_TZ / 4) / 256; it++) {
-            const int q = it * 256 + (int)threadIdx.x;
-            const int row = q >> 3, z4 = (q & 7) * 4;
-            const int tx = row / P2_TY, ty_ = row - tx * P2_TY;
-            const int gx = tx0 + tx, gy_ = ty0 + ty_;
-            if (gx >= x1 || gx >= g || gy_ >= g) continue;
-            const uint32_t ob = ownout[row] >> z4;
-            float *__restrict__ dst = field + ((size_t)gx * g + gy_) * g + tz0 + z4;
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (tz0 + z4 + i < g) dst[i] = ((ob >> i) & 1u) ? QS_PROBE : -dx;
         }
         return;
     }
@@ -1486,7 +1473,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         for (int tx = 0; tx < P2R_TX; tx++) {
             int gx = tx0 + tx;
             if (gx >= x1 || gx >= g) break;
-            if (vyz) field[((size_t)gx * g + gy) * g + gz] = -dx;
+            if (vyz && ((own[tx * P2_TY + ty] >> tz) & 1u)) field[((size_t)gx * g + gy) * g + gz] = -dx;
             emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, -dx);
         }
         return;
@@ -1660,7 +1647,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
                 res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
             }
         }
-        if (vyz) field[((size_t)gx * g + gy) * g + gz] = res;
+        if (vyz && shell) field[((size_t)gx * g + gy) * g + gz] = res;
         emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, res);
     }
 }
@@ -2044,6 +2031,39 @@ __device__ __forceinline__ void item_word(const ItemWords &S, const uint32_t *__
     }
 }
 
+/* Field value of voxel z of grid row `row` (= x * g + y).  Field pass 2 stores the shell voxels'
+   values only; every other voxel is the probe radius outside and -dx inside, read from the outside
+   and shell masks (W words per row).  shellm == nullptr: F holds every voxel. */
+__device__ __forceinline__ float field_at(const float *__restrict__ F, const uint64_t *__restrict__ outm,
+                                          const uint64_t *__restrict__ shellm, size_t row, int g, int W, int z, float dx)
+{
+    if (!shellm) return F[row * g + z];
+    const size_t wi = row * W + (z >> 6);
+    const uint64_t bit = 1ull << (z & 63);
+    if (shellm[wi] & bit) return F[row * g + z];
+    return (outm[wi] & bit) ? QS_PROBE : -dx;
+}
+
+/* every voxel of one molecule's field that field pass 2 leaves unstored, one thread per mask word */
+__global__ void __launch_bounds__(256) k_field_fill(const uint64_t *__restrict__ outm, const uint64_t *__restrict__ shellm,
+                                                    float *__restrict__ F, int g, int W, float dx)
+{
+    const size_t wi = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (wi >= (size_t)g * g * W) return;
+    const size_t row = wi / W;
+    const int z0 = (int)(wi - row * W) * 64;
+    const uint64_t o = outm[wi], sh = shellm[wi];
+    for (int b = 0; b < 64 && z0 + b < g; b++)
+        if (!((sh >> b) & 1ull)) F[row * g + z0 + b] = ((o >> b) & 1ull) ? QS_PROBE : -dx;
+}
+
+void qs_launch_field_fill(const uint64_t *outm, const uint64_t *shellm, float *F, int g, int W, float dx, hipStream_t s)
+{
+    const size_t n = (size_t)g * g * W;
+    if (!n) return;
+    k_field_fill<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(outm, shellm, F, g, W, dx);
+}
+
 /* vertices (vertexInterp/lerp MarchingCubes.cu:715-734, edge oriented low->high), one thread per
    vertex: its word from the active words' first vertex ids, its point and slot from the word's
    four slot masks (vertex ids run over the word's points in z order, slots x, y, z, point) */
@@ -2051,7 +2071,8 @@ __global__ void __launch_bounds__(256) k_mc_vertices(const QsMol *__restrict__ m
                                                      const uint32_t *__restrict__ wcoarse, const uint32_t *__restrict__ list,
                                                      const uint32_t *__restrict__ avb, const uint32_t *__restrict__ vstart,
                                                      uint32_t nact, uint32_t nv, const uint64_t *__restrict__ S4_all,
-                                                     const float *__restrict__ F_all, float *__restrict__ vert,
+                                                     const float *__restrict__ F_all, const uint64_t *__restrict__ outm_all,
+                                                     const uint64_t *__restrict__ shellm_all, float *__restrict__ vert,
                                                      uint64_t *__restrict__ vkey)
 {
     __shared__ ItemWords S;
@@ -2095,8 +2116,12 @@ __global__ void __launch_bounds__(256) k_mc_vertices(const QsMol *__restrict__ m
     float P[3] = {A[0], A[1], A[2]};
     if (sl < 3) {
         const float *__restrict__ F = F_all + M.vox_off;
-        const float f0 = F[pidx];
-        const float f1 = F[pidx + (sl == 0 ? (size_t)g * g : (sl == 1 ? (size_t)g : 1))];
+        const uint64_t *__restrict__ om = outm_all + M.word_off;
+        const uint64_t *__restrict__ sm = shellm_all ? shellm_all + M.word_off : nullptr;
+        const size_t row = (size_t)x * g + y;
+        const int W = (int)M.W;
+        const float f0 = field_at(F, om, sm, row, g, W, z, dx);
+        const float f1 = field_at(F, om, sm, row + (sl == 0 ? (size_t)g : (sl == 1 ? 1 : 0)), g, W, z + (sl == 2), dx);
         float B[3] = {A[0], A[1], A[2]};
         B[sl] = A[sl] + dx;
         const float t = (0.0f - f0) / (f1 - f0);
@@ -2363,23 +2388,38 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
  * is the central difference (one-sided at the grid faces) of the field; an edge vertex blends
  * its two end points' gradients with the vertex's edge parameter t, a zero-point vertex takes
  * its point's.  The field grows outwards (negative inside), so the normals point outwards. */
-__device__ __forceinline__ float grad_axis(const float *__restrict__ F, size_t p, size_t stride, int i, int g, float dx)
+/* the field at flat voxel index p (x-major, z fastest) */
+struct FieldView {
+    const float *F;
+    const uint64_t *outm, *shellm;
+    int g, W;
+    float dx;
+    __device__ __forceinline__ float operator()(size_t p) const
+    {
+        const size_t row = p / (size_t)g;
+        return field_at(F, outm, shellm, row, g, W, (int)(p - row * (size_t)g), dx);
+    }
+};
+
+__device__ __forceinline__ float grad_axis(const FieldView &F, size_t p, size_t stride, int i, int g, float dx)
 {
-    if (i > 0 && i < g - 1) return (F[p + stride] - F[p - stride]) / (2.0f * dx);
-    if (i == 0) return (F[p + stride] - F[p]) / dx;
-    return (F[p] - F[p - stride]) / dx;
+    if (i > 0 && i < g - 1) return (F(p + stride) - F(p - stride)) / (2.0f * dx);
+    if (i == 0) return (F(p + stride) - F(p)) / dx;
+    return (F(p) - F(p - stride)) / dx;
 }
 
 __global__ void __launch_bounds__(256) k_vertex_normals(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
                                                         const uint32_t *__restrict__ vcoarse, const uint64_t *__restrict__ vkey,
-                                                        uint32_t nv, const float *__restrict__ F_all, float *__restrict__ out)
+                                                        uint32_t nv, const float *__restrict__ F_all,
+                                                        const uint64_t *__restrict__ outm_all,
+                                                        const uint64_t *__restrict__ shellm_all, float *__restrict__ out)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
     const int g = M.G.g;
     const float dx = M.G.dx;
-    const float *__restrict__ F = F_all + M.vox_off;
+    const FieldView F{F_all + M.vox_off, outm_all + M.word_off, shellm_all ? shellm_all + M.word_off : nullptr, g, (int)M.W, dx};
     const uint64_t key = vkey[v];
     const int sl = (int)(key & 3ull);
     const uint32_t p = (uint32_t)(key >> 2), ug = (uint32_t)g;
@@ -2394,7 +2434,7 @@ __global__ void __launch_bounds__(256) k_vertex_normals(const QsMol *__restrict_
     for (int a = 0; a < 3; a++) n[a] = grad_axis(F, p, stride[a], c[a], g, dx);
     if (sl < 3) {
         const size_t q = p + stride[sl];
-        const float f0 = F[p], f1 = F[q];
+        const float f0 = F(p), f1 = F(q);
         const float t = (0.0f - f0) / (f1 - f0);
         const float s = 1.0f - t;
 #pragma unroll
@@ -2635,12 +2675,13 @@ void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const ui
 
 void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                            const uint32_t *avb, uint32_t *vstart, uint32_t nact, uint32_t nv, const uint64_t *S4,
-                           const float *F, float *vert, uint64_t *vkey, hipStream_t s)
+                           const float *F, const uint64_t *outm, const uint64_t *shellm, float *vert, uint64_t *vkey,
+                           hipStream_t s)
 {
     if (!nv || !nact) return;
     hipLaunchKernelGGL(k_block_starts, dim3(nblk(nact, 256)), dim3(256), 0, s, avb, nact, nv, vstart);
     hipLaunchKernelGGL(k_mc_vertices, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, avb, vstart, nact, nv,
-                       S4, F, vert, vkey);
+                       S4, F, outm, shellm, vert, vkey);
 }
 
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
@@ -2665,10 +2706,11 @@ void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *v
 }
 
 void qs_launch_vertex_normals(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *vkey, uint32_t nv,
-                              const float *F, float *out, hipStream_t s)
+                              const float *F, const uint64_t *outm, const uint64_t *shellm, float *out, hipStream_t s)
 {
     if (!nv) return;
-    hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, vkey, nv, F, out);
+    hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, vkey, nv, F, outm, shellm,
+                       out);
 }
 
 void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const int *atom, uint32_t nv,
