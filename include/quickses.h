@@ -142,7 +142,9 @@ int qs_batch_grid(int m, float *fparams, int *iparams);
 
 /* Copies of the last computation's intermediate arrays to host memory (stage-wise checks; for
  * a batch, the concatenated arrays, and "labels" of its first molecule):
- *   "labels" uint8[g^3] (0 outside, 1 shell, 2 inside), "field" float[g^3],
+ *   "labels" uint8[g^3] (0 outside, 1 shell, 2 inside), "field" float[g^3] (every voxel: field
+ *   pass 2 stores the shell voxels only, the others are filled in for the copy),
+ *   "shellm" uint64[g*g*ceil(g/64)] (shell mask, bit z%64 of word (x*g+y)*ceil(g/64)+z/64),
  *   "perm" int32[N] (cell-sorted order), "cells" int32[2*gn^3] (start,end pairs),
  *   "vert_unsmoothed" float[3*NVert], "tri_all" int32[3*T] (before the degenerate filter).
  * Returns the number of bytes copied, or -1. */

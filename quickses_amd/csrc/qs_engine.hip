@@ -1185,6 +1185,7 @@ int64_t qs_fetch(const char *what, void *host, int64_t bytes)
         src = E.field.p;
         need = (int64_t)V * 4;
     }
+    else if (w == "shellm") { src = E.shellm.p; need = (int64_t)E.nw_total * 8; }
     else if (w == "perm") { src = E.perm.p; need = (int64_t)E.n_atoms_total * 4; }
     else if (w == "cells") { src = E.cse.p; need = (int64_t)E.ncell_total * 8; }
     else if (w == "vert_unsmoothed") { src = E.vert.p; need = (int64_t)E.nv * 12; }

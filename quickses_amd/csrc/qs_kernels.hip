@@ -65,6 +65,20 @@ __device__ __forceinline__ float dist3_sq(float ax, float ay, float az, float bx
     return s + z;
 }
 
+/* sqrtf, correctly rounded.  For a normal x or +inf the hardware root is within one ulp of the
+   correctly rounded one, and Tuckerman's rounding test on the residuals of its two neighbours
+   (exact FMAs) selects it; zero, denormal and NaN inputs take the general routine. */
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    if (!(x >= 1.17549435e-38f)) return sqrtf(x);
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u), sp = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = s;
+    if (__builtin_fmaf(-sm, s, x) <= 0.0f) r = sm;
+    if (__builtin_fmaf(-sp, s, x) > 0.0f) r = sp;
+    return r;
+}
+
 __device__ __forceinline__ float dist3(float ax, float ay, float az, float bx, float by, float bz)
 {
     float ex = ax - bx, ey = ay - by, ez = az - bz;
@@ -1389,6 +1403,8 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     /* y-pass work list: per halo x plane lx, the z columns (bit z) that need y minima, and the
        exclusive prefix of their counts (cpre[LX] = total) */
     __shared__ uint32_t shx[P2R_TX], cmask[LX], cpre[LX + 1];
+    /* norms whose minimisers are traced back (bit n), from nf */
+    __shared__ uint32_t nfw[8];
     /* y offsets (bit b + R) attaining each DY value */
     using BMask = typename std::conditional<(2 * R + 1 <= 16), uint16_t, uint32_t>::type;
     __shared__ BMask BM[LX * P2_TY * P2_TZ];
@@ -1413,6 +1429,11 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
        without a shell voxel (outside the molecule or deep inside it) is finished from them alone,
        without its halo */
     static_assert(P2R_TX * P2_TY == 128, "one thread per own row of each mask");
+    if (threadIdx.x < 8) {
+        const int t = threadIdx.x;
+        const uint64_t w = t < 2 ? nf.w[0] : (t < 4 ? nf.w[1] : (t < 6 ? nf.w[2] : nf.w[3]));
+        nfw[t] = (uint32_t)(w >> (32 * (t & 1)));
+    }
     int has_shell;
     {
         const int t = threadIdx.x & 127, tx = t / P2_TY, ty_ = t - tx * P2_TY;
@@ -1585,70 +1606,128 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     __syncthreads();
 
     /* x, one (ty, tz) column per thread, then the field */
-    int u[LX];
-#pragma unroll
-    for (int lx = 0; lx < LX; lx++) u[lx] = DY[(lx * P2_TY + ty) * P2_TZ + tz];
     const float vy = grid_pos(G.oy, gy, dx), vz = grid_pos(G.oz, gz, dx);
-#pragma unroll
-    for (int tx = 0; tx < P2R_TX; tx++) {
+    /* voxel tx of the thread's column: its field value from the minimal norm nmin and the x
+       offsets attaining it (amask, bit a + R; used when nmin is traced back), then its stores */
+    auto finish = [&](int tx, int nmin, uint32_t amask) {
         const int gx = tx0 + tx;
-        if (gx >= x1 || gx >= g) break;
         const bool out = (rows[(tx + R) * LY + (ty + R)] >> (tz + R)) & 1ull;
-        const bool shell = (own[tx * P2_TY + ty] >> tz) & 1u;
+        const bool shell = (own[tx * P2_TY + ty] >> tz) & 1u; /* never outside */
         float res = out ? QS_PROBE : -dx;
-        if (vyz && shell && !out) {
-            int nmin = INF;
-#pragma unroll
-            for (int a = 0; a <= 2 * R; a++) nmin = min(nmin, u[tx + a] + (a - R) * (a - R));
-            if (nmin < INF) {
-                const float vx = grid_pos(G.ox, gx, dx);
-                /* the smallest squared distance, rooted once at the end: sqrtf rounds correctly
-                   and is monotone, so the root of the minimum is the minimum of the roots */
-                float min_sq = __builtin_inff();
-                const uint64_t fw = nmin < 64 ? nf.w[0] : (nmin < 128 ? nf.w[1] : (nmin < 192 ? nf.w[2] : nf.w[3]));
-                if ((fw >> (nmin & 63)) & 1ull) {
-                    /* trace the minimisers back through the separable passes: the x offsets
-                       attaining nmin from the registers, then their y offsets */
-                    uint32_t amask = 0;
-#pragma unroll
-                    for (int a = 0; a <= 2 * R; a++) amask |= (uint32_t)(u[tx + a] + (a - R) * (a - R) == nmin) << a;
-                    while (amask) {
-                        const int a = __builtin_ctz(amask) - R;
-                        amask &= amask - 1u;
-                        const int dy = nmin - a * a;
-                        uint32_t bmask = BM[((tx + R + a) * P2_TY + ty) * P2_TZ + tz];
-                        while (bmask) {
-                            const int b = __builtin_ctz(bmask) - R;
-                            bmask &= bmask - 1u;
-                            /* dz = c^2 exactly, c <= R: the hardware root is within an ulp of c */
-                            const int dz = dy - b * b;
-                            const int c = (int)(__builtin_amdgcn_sqrtf((float)dz) + 0.5f);
-                            const uint64_t m = rows[(tx + R + a) * LY + (ty + R + b)];
-                            const float qx = grid_pos(G.ox, gx + a, dx), qy = grid_pos(G.oy, gy + b, dx);
-                            if ((m >> (tz + R + c)) & 1ull)
-                                min_sq = fminf(dist3_sq(qx, qy, grid_pos(G.oz, gz + c, dx), vx, vy, vz), min_sq);
-                            if (c && ((m >> (tz + R - c)) & 1ull))
-                                min_sq = fminf(dist3_sq(qx, qy, grid_pos(G.oz, gz - c, dx), vx, vy, vz), min_sq);
-                        }
-                    }
-                } else {
-#pragma unroll 1
-                    for (int k = norm_start[nmin]; k < enum_end[nmin]; k++) {
-                        int o = offs[k];
-                        int a = (int)(int8_t)(o & 0xFF), b = (int)(int8_t)((o >> 8) & 0xFF), c = (int)(int8_t)((o >> 16) & 0xFF);
-                        uint64_t m = rows[(tx + R + a) * LY + (ty + R + b)];
-                        if (!((m >> (tz + R + c)) & 1ull)) continue;
-                        float d = dist3_sq(grid_pos(G.ox, gx + a, dx), grid_pos(G.oy, gy + b, dx), grid_pos(G.oz, gz + c, dx),
-                                           vx, vy, vz);
-                        min_sq = fminf(d, min_sq);
+        if (vyz && shell && nmin < INF) {
+            const float vx = grid_pos(G.ox, gx, dx);
+            /* the smallest squared distance, rooted once at the end: sqrtf rounds correctly and
+               is monotone, so the root of the minimum is the minimum of the roots */
+            float min_sq = __builtin_inff();
+            if ((nfw[nmin >> 5] >> (nmin & 31)) & 1u) {
+                /* trace the minimisers back through the separable passes: the x offsets, then
+                   their y offsets from the recorded masks */
+                while (amask) {
+                    const int a = __builtin_ctz(amask) - R;
+                    amask &= amask - 1u;
+                    const int dy = nmin - a * a;
+                    uint32_t bmask = BM[((tx + R + a) * P2_TY + ty) * P2_TZ + tz];
+                    while (bmask) {
+                        const int b = __builtin_ctz(bmask) - R;
+                        bmask &= bmask - 1u;
+                        /* dz = c^2 exactly, c <= R: the hardware root is within an ulp of c */
+                        const int dz = dy - b * b;
+                        const int c = (int)(__builtin_amdgcn_sqrtf((float)dz) + 0.5f);
+                        const uint64_t m = rows[(tx + R + a) * LY + (ty + R + b)];
+                        const float qx = grid_pos(G.ox, gx + a, dx), qy = grid_pos(G.oy, gy + b, dx);
+                        if ((m >> (tz + R + c)) & 1ull)
+                            min_sq = fminf(dist3_sq(qx, qy, grid_pos(G.oz, gz + c, dx), vx, vy, vz), min_sq);
+                        if (c && ((m >> (tz + R - c)) & 1ull))
+                            min_sq = fminf(dist3_sq(qx, qy, grid_pos(G.oz, gz - c, dx), vx, vy, vz), min_sq);
                     }
                 }
-                const float min_dist = sqrtf(min_sq);
-                res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
+            } else {
+#pragma unroll 1
+                for (int k = norm_start[nmin]; k < enum_end[nmin]; k++) {
+                    int o = offs[k];
+                    int a = (int)(int8_t)(o & 0xFF), b = (int)(int8_t)((o >> 8) & 0xFF), c = (int)(int8_t)((o >> 16) & 0xFF);
+                    uint64_t m = rows[(tx + R + a) * LY + (ty + R + b)];
+                    if (!((m >> (tz + R + c)) & 1ull)) continue;
+                    float d = dist3_sq(grid_pos(G.ox, gx + a, dx), grid_pos(G.oy, gy + b, dx), grid_pos(G.oz, gz + c, dx),
+                                       vx, vy, vz);
+                    min_sq = fminf(d, min_sq);
+                }
             }
+            const float min_dist = sqrt_rn(min_sq);
+            res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
         }
         if (vyz && shell) field[((size_t)gx * g + gy) * g + gz] = res;
         emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, res);
+    };
+
+    if constexpr (2 * R + 1 <= 16) {
+        /* two x outputs per 32-bit register: the minimum over the 2R+1 taps and the mask of the
+           taps attaining it, as in the y pass */
+        typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+        u16x2 pw[4] = {{2, 2}, {4, 4}, {16, 16}, {256, 256}};
+#pragma unroll
+        for (int i = 0; i < 4; i++) asm volatile("" : "+v"(pw[i]));
+        u16x2 U[LX - 1];
+        {
+            uint32_t u[LX];
+#pragma unroll
+            for (int lx = 0; lx < LX; lx++) u[lx] = DY[(lx * P2_TY + ty) * P2_TZ + tz];
+#pragma unroll
+            for (int k = 0; k < LX - 1; k++) U[k] = __builtin_bit_cast(u16x2, u[k] | (u[k + 1] << 16));
+        }
+        static_assert(P2R_TX % 2 == 0, "x outputs in pairs");
+#pragma unroll
+        for (int tx = 0; tx < P2R_TX; tx += 2) {
+            const int gx = tx0 + tx;
+            if (gx >= x1 || gx >= g) break;
+            const bool two = gx + 1 < g;
+            const uint32_t sh = ((own[tx * P2_TY + ty] >> tz) & 1u) | (two ? ((own[(tx + 1) * P2_TY + ty] >> tz) & 1u) << 1 : 0u);
+            uint32_t nm = (uint32_t)INF * 0x10001u, am = 0u;
+            if (vyz && sh) {
+                constexpr int NT = 2 * R + 1;
+                u16x2 sb[NT], t[NT];
+#pragma unroll
+                for (int a = 0; a < NT; a++) {
+                    const uint16_t c = (uint16_t)((a - R) * (a - R));
+                    sb[a] = U[tx + a] + (u16x2){c, c};
+                    t[a] = sb[a];
+                }
+#pragma unroll
+                for (int s = 1; s < NT; s *= 2)
+#pragma unroll
+                    for (int a = 0; a + s < NT; a += 2 * s) t[a] = __builtin_elementwise_min(t[a], t[a + s]);
+                const u16x2 best = t[0];
+                const u16x2 best1 = best + (u16x2){1, 1};
+#pragma unroll
+                for (int a = 0; a < NT; a++) t[a] = __builtin_elementwise_sub_sat(best1, sb[a]);
+#pragma unroll
+                for (int s = 1, l = 0; s < NT; s *= 2, l++)
+#pragma unroll
+                    for (int a = 0; a + s < NT; a += 2 * s) t[a] = t[a + s] * pw[l] + t[a];
+                nm = __builtin_bit_cast(uint32_t, best);
+                am = __builtin_bit_cast(uint32_t, t[0]);
+            }
+            finish(tx, (int)(nm & 0xFFFFu), am & 0xFFFFu);
+            if (two) finish(tx + 1, (int)(nm >> 16), am >> 16);
+        }
+    } else {
+        int u[LX];
+#pragma unroll
+        for (int lx = 0; lx < LX; lx++) u[lx] = DY[(lx * P2_TY + ty) * P2_TZ + tz];
+#pragma unroll
+        for (int tx = 0; tx < P2R_TX; tx++) {
+            const int gx = tx0 + tx;
+            if (gx >= x1 || gx >= g) break;
+            int nmin = INF;
+            uint32_t amask = 0;
+            if (vyz && ((own[tx * P2_TY + ty] >> tz) & 1u)) {
+#pragma unroll
+                for (int a = 0; a <= 2 * R; a++) nmin = min(nmin, u[tx + a] + (a - R) * (a - R));
+#pragma unroll
+                for (int a = 0; a <= 2 * R; a++) amask |= (uint32_t)(u[tx + a] + (a - R) * (a - R) == nmin) << a;
+            }
+            finish(tx, nmin, amask);
+        }
     }
 }
 
@@ -2388,24 +2467,23 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
  * is the central difference (one-sided at the grid faces) of the field; an edge vertex blends
  * its two end points' gradients with the vertex's edge parameter t, a zero-point vertex takes
  * its point's.  The field grows outwards (negative inside), so the normals point outwards. */
-/* the field at flat voxel index p (x-major, z fastest) */
+/* one molecule's field, addressed by grid row (x * g + y) and z */
 struct FieldView {
     const float *F;
     const uint64_t *outm, *shellm;
     int g, W;
     float dx;
-    __device__ __forceinline__ float operator()(size_t p) const
-    {
-        const size_t row = p / (size_t)g;
-        return field_at(F, outm, shellm, row, g, W, (int)(p - row * (size_t)g), dx);
-    }
+    __device__ __forceinline__ float at(size_t row, int z) const { return field_at(F, outm, shellm, row, g, W, z, dx); }
 };
 
-__device__ __forceinline__ float grad_axis(const FieldView &F, size_t p, size_t stride, int i, int g, float dx)
+/* derivative along axis a (0 x, 1 y, 2 z) at grid point (row, z), whose coordinate on that axis is i */
+__device__ __forceinline__ float grad_axis(const FieldView &F, size_t row, int z, int a, int i, int g, float dx)
 {
-    if (i > 0 && i < g - 1) return (F(p + stride) - F(p - stride)) / (2.0f * dx);
-    if (i == 0) return (F(p + stride) - F(p)) / dx;
-    return (F(p) - F(p - stride)) / dx;
+    const size_t dr = a == 0 ? (size_t)g : (a == 1 ? 1 : 0);
+    const int dz = a == 2;
+    if (i > 0 && i < g - 1) return (F.at(row + dr, z + dz) - F.at(row - dr, z - dz)) / (2.0f * dx);
+    if (i == 0) return (F.at(row + dr, z + dz) - F.at(row, z)) / dx;
+    return (F.at(row, z) - F.at(row - dr, z - dz)) / dx;
 }
 
 __global__ void __launch_bounds__(256) k_vertex_normals(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
@@ -2428,18 +2506,20 @@ __global__ void __launch_bounds__(256) k_vertex_normals(const QsMol *__restrict_
     c[2] = (int)(p - xy * ug);
     c[0] = (int)(xy / ug);
     c[1] = (int)(xy - (uint32_t)c[0] * ug);
-    const size_t stride[3] = {(size_t)g * g, (size_t)g, 1};
+    const size_t row = (size_t)c[0] * g + c[1];
+    const int z = c[2];
     float n[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++) n[a] = grad_axis(F, p, stride[a], c[a], g, dx);
+    for (int a = 0; a < 3; a++) n[a] = grad_axis(F, row, z, a, c[a], g, dx);
     if (sl < 3) {
-        const size_t q = p + stride[sl];
-        const float f0 = F(p), f1 = F(q);
+        const size_t rq = row + (sl == 0 ? (size_t)g : (sl == 1 ? 1 : 0));
+        const int zq = z + (sl == 2);
+        const float f0 = F.at(row, z), f1 = F.at(rq, zq);
         const float t = (0.0f - f0) / (f1 - f0);
         const float s = 1.0f - t;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            const float g1 = grad_axis(F, q, stride[a], c[a] + (a == sl), g, dx);
+            const float g1 = grad_axis(F, rq, zq, a, c[a] + (a == sl), g, dx);
             n[a] = s * n[a] + t * g1;
         }
     }

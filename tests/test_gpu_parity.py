@@ -58,6 +58,11 @@ def test_stages_match_oracle(qs, orc):
     cells = qs.fetch("cells", np.int32, 2 * G.gn ** 3).reshape(-1, 2)
     assert np.array_equal(cells[:, 0], cs) and np.array_equal(cells[:, 1], ce)
     assert np.array_equal(qs.fetch("labels", np.uint8, G.g ** 3), lab)
+    # the shell mask (the voxels whose field values pass 2 stores), bit z % 64 of word z / 64
+    W = (G.g + 63) // 64
+    sm = qs.fetch("shellm", np.uint64, G.g * G.g * W).reshape(G.g * G.g, W)
+    bits = np.unpackbits(sm.view(np.uint8), bitorder="little").reshape(G.g * G.g, 64 * W)[:, :G.g]
+    assert np.array_equal(bits.ravel(), (lab == 1).astype(np.uint8))
     f = qs.fetch("field", np.float32, G.g ** 3)
     assert np.array_equal(f.view(np.uint32), fld.view(np.uint32))
 
