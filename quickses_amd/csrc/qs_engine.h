@@ -180,7 +180,7 @@ struct Engine {
     DevBuf vstart, tstart; /* first active word of each 256-vertex / 256-triangle block */
     DevBuf normals;        /* staging for qs_vertex_normals / qs_vertex_colors */
     DevBuf colors_in;      /* per-atom colour table of qs_vertex_colors */
-    DevBuf vert, vert2, vert3, vkey, cidx, tbase, tri, tri_out, atom;
+    DevBuf vert, vert2, vert3, vkey, cidx, tri, tri_out, atom;
     DevBuf deg_blk, keeppos, tri_loc;
     DevBuf scan_tmp;
     std::vector<P2Group> groups;
@@ -207,7 +207,7 @@ struct Engine {
         for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start, &cursor,
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
                           &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey, &cidx,
-                          &tbase, &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp})
+                          &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp})
             release(*b);
         for (P2Tables *t : tables) {
             release(t->offs);

@@ -162,7 +162,6 @@ int Engine::reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts
     QS_TRY(ensure(s4, nw * 32));
     QS_TRY(ensure(ntb, nw * 24));
     QS_TRY(ensure(field, vox * 4));
-    QS_TRY(ensure(tbase, vox * 4));
     QS_TRY(ensure(cidx, vox));
     QS_TRY(ensure(in_pos, atoms * 12));
     for (DevBuf *b : {&in_rad, &key, &perm}) QS_TRY(ensure(*b, atoms * 4));
@@ -475,7 +474,6 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_TRY(ensure(vert, (size_t)std::max<uint32_t>(nv, 1) * 12));
     QS_TRY(ensure(vkey, (size_t)std::max<uint32_t>(nv, 1) * 8));
     QS_TRY(ensure(tri, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
-    QS_TRY(ensure(tbase, std::max<uint64_t>(vox_total, 1) * 4));
     const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
     const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 2 * ((size_t)nmol + 1);
     QS_TRY(ensure(vstart, ((size_t)nv / 256 + 1) * 4));
@@ -490,7 +488,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
                           field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey), stream);
     qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart),
                            nact, nt_all, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
-                           as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri), as<uint32_t>(tbase), as<uint32_t>(deg_blk),
+                           as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri), as<uint32_t>(deg_blk),
                            as<uint32_t>(totals) + 3, stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
@@ -613,7 +611,7 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
         float *dst = as<float>(vert2);
         for (int it = 0; it < smooth; it++) {
             qs_launch_smooth(as<QsMol>(d_mols), d_vpre, d_vpre + nmol + 1, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
-                             as<uint32_t>(vbw), as<uint64_t>(vkey), nv, as<uint8_t>(cidx), as<uint32_t>(tbase), as<int>(tri),
+                             as<uint32_t>(vbw), as<uint64_t>(vkey), nv, as<uint8_t>(cidx), as<uint64_t>(ntb), as<uint32_t>(tbw), as<int>(tri),
                              src, dst, stream);
             src = dst;
             dst = (dst == as<float>(vert2)) ? as<float>(vert3) : as<float>(vert2);

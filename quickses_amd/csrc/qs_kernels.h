@@ -65,7 +65,7 @@ void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                             const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
                             const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
-                            const uint8_t *cidx, int *tri, uint32_t *tbase, uint32_t *deg_blk, uint32_t *deg_total,
+                            const uint8_t *cidx, int *tri, uint32_t *deg_blk, uint32_t *deg_total,
                             hipStream_t s);
 
 /* per-vertex stages */
@@ -74,7 +74,7 @@ void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint3
                             const int *perm, const float *vert, uint32_t nv, int sorted_ids, int *out, hipStream_t s);
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
                       const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
-                      const uint32_t *tbase, const int *tri, const float *src, float *dst, hipStream_t s);
+                      const uint64_t *NT, const uint32_t *tbw, const int *tri, const float *src, float *dst, hipStream_t s);
 
 /* unit field gradient at each vertex's marching-cubes position, 3 floats per vertex (field as for
    the vertex kernel) */

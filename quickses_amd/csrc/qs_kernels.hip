@@ -128,6 +128,17 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
     return b - w + (w & 7u) * XCD_CHUNK + (w >> 3);
 }
 
+/* Block order for per-vertex gathers, whose neighbours lie a few thousand items (one grid plane)
+   away: each XCD runs one contiguous eighth of the launch's blocks, in order, so that the items a
+   block gathers were mostly fetched into the same L2 by the blocks before it.  Blocks past the
+   last multiple of 8 keep their number. */
+__device__ __forceinline__ uint32_t xcd_range(uint32_t b, uint32_t nb)
+{
+    const uint32_t per = nb / 8;
+    if (b >= per * 8) return b;
+    return (b & 7u) * per + (b >> 3);
+}
+
 /* molecule of block b of a batched launch, and the block's index inside the molecule */
 __device__ __forceinline__ int batch_mol(const QsBatch &B, uint32_t b, uint32_t &local)
 {
@@ -2228,8 +2239,8 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
                                                       const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
                                                       const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
                                                       const uint64_t *__restrict__ NT_all, const uint8_t *__restrict__ cidx_all,
-                                                      int *__restrict__ tri, uint32_t *__restrict__ tbase_all,
-                                                      uint32_t *__restrict__ deg_blk, uint32_t *__restrict__ deg_total)
+                                                      int *__restrict__ tri, uint32_t *__restrict__ deg_blk,
+                                                      uint32_t *__restrict__ deg_total)
 {
     __shared__ ItemWords S;
     stage_item_words(S, atb, list, tstart, nact, gridDim.x, wpre, wcoarse);
@@ -2273,7 +2284,6 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
     tri[3 * (size_t)t + 0] = c[0];
     tri[3 * (size_t)t + 1] = c[1];
     tri[3 * (size_t)t + 2] = c[2];
-    if (j == 0) tbase_all[M.vox_off + pidx] = t;
     /* triangles with a repeated corner (zero grid points) are dropped from the output
        (CudaSurf.cu:820-829): counted here per block and in total */
     const uint64_t dm = __ballot(c[0] == c[1] || c[1] == c[2] || c[0] == c[2]);
@@ -2291,7 +2301,7 @@ __global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ 
                                                       const float *__restrict__ vert, uint32_t nv, int sorted_ids,
                                                       int *__restrict__ out)
 {
-    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t v = xcd_range(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
     const QsGrid G = M.G;
@@ -2354,19 +2364,30 @@ __device__ __forceinline__ void push_pair(const float *__restrict__ src, uint32_
     sz = sz + q1[2];
 }
 
-/* one Jacobi pass, one thread per vertex.  Polygonised cubes carry their case (cidx) and first
-   triangle id (tbase), so each incident triangle's other corners are read from the triangle
-   array. */
+/* first triangle id of polygonised cube (x, y, z) (bit b of word wi): the word's first triangle
+   tbw[wi] plus the triangles of the word's cubes below b, from its triangle-count bit planes */
+__device__ __forceinline__ uint32_t cube_first_tri(const uint64_t *__restrict__ NT, const uint32_t *__restrict__ tbw, size_t wi,
+                                                   int b)
+{
+    const uint64_t *np = NT + 3 * wi;
+    const uint64_t below = (1ull << b) - 1ull;
+    return tbw[wi] + (uint32_t)(__popcll(np[0] & below) + 2 * __popcll(np[1] & below) + 4 * __popcll(np[2] & below));
+}
+
+/* one Jacobi pass, one thread per vertex.  Polygonised cubes carry their case (cidx) and, through
+   their word's triangle base and triangle-count planes, their first triangle id, so each incident
+   triangle's other corners are read from the triangle array. */
 __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
                                                 const uint32_t *__restrict__ vcoarse,
                                                 const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
                                                 const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
                                                 const uint64_t *__restrict__ vkey, uint32_t nv,
-                                                const uint8_t *__restrict__ cidx_all, const uint32_t *__restrict__ tbase_all,
-                                                const int *__restrict__ tri, const float *__restrict__ src,
+                                                const uint8_t *__restrict__ cidx_all, const uint64_t *__restrict__ NT_all,
+                                                const uint32_t *__restrict__ tbw_all, const int *__restrict__ tri,
+                                                const float *__restrict__ src,
                                                 float *__restrict__ dst)
 {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t v = xcd_range(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
     const QsGrid G = M.G;
@@ -2380,7 +2401,9 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
     V.W = (int)M.W;
     V.lim = lim;
     const uint8_t *__restrict__ cidx = cidx_all + M.vox_off;
-    const uint32_t *__restrict__ tbase = tbase_all + M.vox_off;
+    const uint64_t *__restrict__ NT = NT_all + 3 * M.word_off;
+    const uint32_t *__restrict__ tbw = tbw_all + M.word_off;
+    const int W = (int)M.W;
     const uint64_t key = vkey[v];
     const int sl = (int)(key & 3ull);
     /* key = grid point index << 2 | slot; a molecule's grid has fewer than 2^32 points */
@@ -2417,9 +2440,9 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
             ci4[k] = 0;
             tb4[k] = 0;
             if (cx < 0 || cy < 0 || cz < 0 || cx >= lim || cy >= lim || cz >= lim) continue;
-            const size_t c = ((size_t)cx * g + cy) * g + cz;
-            ci4[k] = cidx[c];
-            tb4[k] = tbase[c];
+            const size_t row = (size_t)cx * g + cy;
+            ci4[k] = cidx[row * g + cz];
+            tb4[k] = cube_first_tri(NT, tbw, row * W + (cz >> 6), cz & 63);
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -2445,7 +2468,7 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
             const int ci = V.cube_index(cx, cy, cz);
             const int nt = tri_count(ci);
             if (!nt) continue;
-            const uint32_t tb = tbase[((size_t)cx * g + cy) * g + cz];
+            const uint32_t tb = cube_first_tri(NT, tbw, ((size_t)cx * g + cy) * W + (cz >> 6), cz & 63);
             for (int t = 0; t < nt; t++) {
                 const int *tt = tri + 3 * (size_t)(tb + t);
                 const uint32_t a = (uint32_t)tt[0], b = (uint32_t)tt[1], c = (uint32_t)tt[2];
@@ -2767,22 +2790,22 @@ void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                             const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
                             const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
-                            const uint8_t *cidx, int *tri, uint32_t *tbase, uint32_t *deg_blk, uint32_t *deg_total,
+                            const uint8_t *cidx, int *tri, uint32_t *deg_blk, uint32_t *deg_total,
                             hipStream_t s)
 {
     if (!ntri || !nact) return;
     hipLaunchKernelGGL(k_block_starts, dim3(nblk(nact, 256)), dim3(256), 0, s, atb, nact, ntri, tstart);
     hipLaunchKernelGGL(k_mc_triangles, dim3(nblk(ntri, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, atb, tstart, nact,
-                       ntri, N, Zm, S4, vbw, NT, cidx, tri, tbase, deg_blk, deg_total);
+                       ntri, N, Zm, S4, vbw, NT, cidx, tri, deg_blk, deg_total);
 }
 
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
                       const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
-                      const uint32_t *tbase, const int *tri, const float *src, float *dst, hipStream_t s)
+                      const uint64_t *NT, const uint32_t *tbw, const int *tri, const float *src, float *dst, hipStream_t s)
 {
     if (!nv) return;
-    hipLaunchKernelGGL(k_smooth, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, N, Zm, S4, vbw, vkey, nv, cidx, tbase,
-                       tri, src, dst);
+    hipLaunchKernelGGL(k_smooth, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, N, Zm, S4, vbw, vkey, nv, cidx, NT,
+                       tbw, tri, src, dst);
 }
 
 void qs_launch_vertex_normals(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *vkey, uint32_t nv,
