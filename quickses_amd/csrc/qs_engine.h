@@ -186,7 +186,8 @@ struct Engine {
     std::vector<P2Group> groups;
     std::vector<P2Tables *> tables;
     /* host staging of uploads that kernels read later in the same call */
-    std::vector<uint32_t> h_meta, h_vpre, h_trioff;
+    std::vector<uint32_t> h_meta, h_vpre;
+    std::vector<uint64_t> h_bidx; /* first triangle of each molecule, staged for the filter */
 
     /* last result */
     std::vector<QsMol> mols;

@@ -628,40 +628,27 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     }
 
     /* ---- degenerate filter (CudaSurf.cu:820-829), per-molecule kept counts ---- */
-    /* the triangle kernel counted them: without degenerate triangles the filter is the identity */
-    uint32_t ndeg = 0;
-    QS_HIP(hipMemcpyAsync(&ndeg, as<uint32_t>(totals) + 3, 4, hipMemcpyDeviceToHost, stream));
-    QS_HIP(hipStreamSynchronize(stream));
-    if (ndeg == 0) {
-        for (int m = 0; m < nmol; m++) tri_off[m] = mols[m].tri_off;
-        tri_off[nmol] = nt_all;
-        /* device copy of the offsets (read by the batch getters); the staging vector is next
-           written after this call's stream work has been waited for */
-        h_trioff = tri_off;
-        QS_HIP(hipMemcpyAsync(bounds.p, h_trioff.data(), (size_t)(nmol + 1) * 4, hipMemcpyHostToDevice, stream));
-        tm.mark("filter");
-        nt = nt_all;
-        res_tri = as<int>(tri);
-        res_atom = as<int>(atom);
-        batch_valid = true;
-        tm.report(mols[0].G, n);
-        return 0;
-    }
-    /* kept triangles go to 256 b - (degenerate ones in blocks before b) + rank in block b */
+    /* The triangle kernel counted them (totals[3]).  The compaction runs without waiting for that
+       count: with none it writes nothing and the input array is the result.  Kept triangles go to
+       256 b - (degenerate ones in blocks before b) + rank in block b; the kept counts before each
+       molecule's first triangle give the offsets, equal to the input offsets when none drop. */
     const uint64_t nblk_t = ((uint64_t)nt_all + 255) / 256;
     QS_TRY(ensure(keeppos, (nblk_t + 1) * 4));
     QS_TRY(ensure(tri_out, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
     QS_TRY(scan(as<uint32_t>(deg_blk), as<uint32_t>(keeppos), nblk_t, as<uint32_t>(keeppos) + nblk_t));
-    qs_launch_tri_compact(as<int>(tri), nt_all, as<uint32_t>(keeppos), as<int>(tri_out), stream);
-    std::vector<uint64_t> idx(nmol + 1);
-    for (int m = 0; m <= nmol; m++) idx[m] = (m < nmol) ? mols[m].tri_off : nt_all;
-    QS_HIP(hipMemcpyAsync(bidx.p, idx.data(), (size_t)(nmol + 1) * 8, hipMemcpyHostToDevice, stream));
+    qs_launch_tri_compact(as<int>(tri), nt_all, as<uint32_t>(keeppos), as<uint32_t>(totals) + 3, as<int>(tri_out), stream);
+    /* staging vectors: next written after this call's stream work has been waited for */
+    h_bidx.resize(nmol + 1);
+    for (int m = 0; m <= nmol; m++) h_bidx[m] = (m < nmol) ? mols[m].tri_off : nt_all;
+    QS_HIP(hipMemcpyAsync(bidx.p, h_bidx.data(), (size_t)(nmol + 1) * 8, hipMemcpyHostToDevice, stream));
     qs_launch_tri_kept_before(as<int>(tri), as<uint32_t>(keeppos), as<uint64_t>(bidx), nmol + 1, as<uint32_t>(bounds), stream);
+    uint32_t ndeg = 0;
     QS_HIP(hipMemcpyAsync(tri_off.data(), bounds.p, (size_t)(nmol + 1) * 4, hipMemcpyDeviceToHost, stream));
+    QS_HIP(hipMemcpyAsync(&ndeg, as<uint32_t>(totals) + 3, 4, hipMemcpyDeviceToHost, stream));
     QS_HIP(hipStreamSynchronize(stream));
     tm.mark("filter");
     nt = tri_off[nmol];
-    res_tri = as<int>(tri_out);
+    res_tri = ndeg ? as<int>(tri_out) : as<int>(tri);
     res_atom = as<int>(atom);
     batch_valid = true; /* device copies: vertex offsets after the molecule prefixes in d_off, triangle offsets in bounds */
     tm.report(mols[0].G, n);

@@ -84,8 +84,9 @@ void qs_launch_vertex_normals(const QsMol *mols, const uint32_t *vpre, const uin
 void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const int *atom, uint32_t nv,
                              const float4 *table, float4 *out, hipStream_t s);
 /* degenerate-triangle filter: dpre = exclusive prefix of the per-256-triangle-block degenerate
-   counts that the triangle kernel leaves */
-void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *dpre, int *out, hipStream_t s);
+   counts that the triangle kernel leaves, ndeg their total (device); with ndeg == 0 the compaction
+   writes nothing */
+void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *dpre, const uint32_t *ndeg, int *out, hipStream_t s);
 void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint64_t *idx, int n, uint32_t *out, hipStream_t s);
 void qs_launch_tri_localize(const int *tri, uint32_t nt, const uint32_t *tpre, const uint32_t *vpre, int nmol, int *out,
                             hipStream_t s);

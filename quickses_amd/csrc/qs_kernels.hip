@@ -114,30 +114,22 @@ __device__ __forceinline__ int seg_coarse(const T *__restrict__ pre, const uint3
 }
 
 /* Workgroups are dealt to the eight XCDs round-robin (block b runs on XCD b % 8), and each XCD
-   has its own L2.  This renumbers the blocks of an nb-block launch in groups of 8*XCD_CHUNK: inside a
-   group each XCD runs XCD_CHUNK consecutive logical blocks, so neighbouring tiles share an L2 (their halo
-   reads hit there and their partial output lines merge there before going to HBM), while the
-   groups still spread every region of the grid over all eight XCDs.  A bijection on [0, nb): the
-   last, incomplete group keeps its order. */
-constexpr uint32_t XCD_CHUNK = 8;
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+   has its own L2.  This renumbers the blocks of an nb-block launch in groups of 8*C: inside a group
+   each XCD runs C consecutive logical blocks, so neighbouring tiles share an L2 (their halo reads
+   hit there and their partial output lines merge there before going to HBM), while the groups
+   still spread every region of the grid over all eight XCDs.  A bijection on [0, nb): the last,
+   incomplete group keeps its order.  Grid tiles use C = 8; per-vertex gathers, whose neighbours
+   lie about one grid plane (a few thousand vertices) away, use C = 64. */
+template <uint32_t C>
+__device__ __forceinline__ uint32_t xcd_group(uint32_t b, uint32_t nb)
 {
-    constexpr uint32_t G = 8 * XCD_CHUNK;
+    constexpr uint32_t G = 8 * C;
     if (b >= nb / G * G) return b;
     const uint32_t w = b % G;
-    return b - w + (w & 7u) * XCD_CHUNK + (w >> 3);
+    return b - w + (w & 7u) * C + (w >> 3);
 }
-
-/* Block order for per-vertex gathers, whose neighbours lie a few thousand items (one grid plane)
-   away: each XCD runs one contiguous eighth of the launch's blocks, in order, so that the items a
-   block gathers were mostly fetched into the same L2 by the blocks before it.  Blocks past the
-   last multiple of 8 keep their number. */
-__device__ __forceinline__ uint32_t xcd_range(uint32_t b, uint32_t nb)
-{
-    const uint32_t per = nb / 8;
-    if (b >= per * 8) return b;
-    return (b & 7u) * per + (b >> 3);
-}
+constexpr uint32_t XCD_CHUNK = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) { return xcd_group<XCD_CHUNK>(b, nb); }
 
 /* molecule of block b of a batched launch, and the block's index inside the molecule */
 __device__ __forceinline__ int batch_mol(const QsBatch &B, uint32_t b, uint32_t &local)
@@ -2301,7 +2293,7 @@ __global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ 
                                                       const float *__restrict__ vert, uint32_t nv, int sorted_ids,
                                                       int *__restrict__ out)
 {
-    uint32_t v = xcd_range(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    uint32_t v = xcd_group<64>(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
     const QsGrid G = M.G;
@@ -2387,7 +2379,7 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
                                                 const float *__restrict__ src,
                                                 float *__restrict__ dst)
 {
-    const uint32_t v = xcd_range(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const uint32_t v = xcd_group<64>(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
     const QsGrid G = M.G;
@@ -2578,9 +2570,11 @@ __device__ __forceinline__ bool tri_degenerate(const int *__restrict__ tri, uint
 }
 
 __global__ void __launch_bounds__(256) k_tri_compact(const int *__restrict__ tri, uint32_t nt,
-                                                     const uint32_t *__restrict__ dpre, int *__restrict__ out)
+                                                     const uint32_t *__restrict__ dpre, const uint32_t *__restrict__ ndeg,
+                                                     int *__restrict__ out)
 {
     __shared__ uint32_t wkeep[4];
+    if (*ndeg == 0) return; /* nothing to drop: the caller keeps the input array */
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const bool keep = t < nt && !tri_degenerate(tri, t);
@@ -2847,10 +2841,10 @@ void qs_launch_tri_localize(const int *tri, uint32_t nt, const uint32_t *tpre, c
     hipLaunchKernelGGL(k_tri_localize, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, tpre, vpre, nmol, out);
 }
 
-void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *dpre, int *out, hipStream_t s)
+void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *dpre, const uint32_t *ndeg, int *out, hipStream_t s)
 {
     if (!nt) return;
-    hipLaunchKernelGGL(k_tri_compact, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, dpre, out);
+    hipLaunchKernelGGL(k_tri_compact, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, dpre, ndeg, out);
 }
 
 void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint64_t *idx, int n, uint32_t *out, hipStream_t s)
