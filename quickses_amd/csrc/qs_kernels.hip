@@ -559,7 +559,8 @@ struct RowAtom {
 };
 
 /* exact [L, R] within [zlo, zhi] where the predicate holds; zb = its argmin in [zlo, zhi];
-   L > R if none.  zc is the atom's z in voxel units (estimate only). */
+   L > R if none.  zc is the atom's z in voxel units; zc -+ h only start the exact walks, so the
+   hardware root (within an ulp) serves for h. */
 __device__ __forceinline__ void row_interval(const RowAtom &A, float ss, int zlo, int zhi, int zb, float zc, float inv_dx,
                                              int &L, int &R)
 {
@@ -567,7 +568,7 @@ __device__ __forceinline__ void row_interval(const RowAtom &A, float ss, int zlo
     R = 0;
     if (zlo > zhi || !A.pred(zb, ss)) return;
     float h2 = ss - A.S;
-    float h = (h2 > 0.0f) ? sqrtf(h2) * inv_dx : 0.0f;
+    float h = (h2 > 0.0f) ? __builtin_amdgcn_sqrtf(h2) * inv_dx : 0.0f;
     int l = iclamp((int)ceilf(zc - h), zlo, zb);
     if (A.pred(l, ss)) {
         while (l > zlo && A.pred(l - 1, ss)) l--;
@@ -1008,8 +1009,9 @@ __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *_
                     /* rows whose y cell admits the atom: cy >= acy - 1, and cy <= acy + 1 unless acy is the last cell */
                     int ylo = yfirst[max(acy - 1, 0)];
                     int yhi = (acy == gn - 1) ? g - 1 : yfirst[acy + 2] - 1;
-                    /* rows the atom can reach in y (a superset: the exact test is per row) */
-                    const float h = sqrtf(ax.x - X);
+                    /* rows the atom can reach in y (a superset with a row of margin: the exact
+                       test is per row, so the hardware root serves) */
+                    const float h = __builtin_amdgcn_sqrtf(ax.x - X);
                     ylo = max(ylo, (int)floorf((a.y - h - G.oy) * inv_dx) - 1);
                     yhi = min(yhi, (int)ceilf((a.y + h - G.oy) * inv_dx) + 1);
                     const int r0 = max(ylo - y0, 0), r1 = min(yhi - y0, rows_hi);
@@ -2319,7 +2321,7 @@ __global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ 
                 const float4 at = sx[id];
                 const float sq = dist3_sq(at.x, at.y, at.z, px, py, pz);
                 if (sq < min_s) {
-                    const float d = sqrtf(sq);
+                    const float d = sqrt_rn(sq);
                     if (d < min_d) {
                         min_d = d;
                         min_s = sq;
