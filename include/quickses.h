@@ -105,6 +105,9 @@ int qs_batch_offsets(unsigned int *vert_off, unsigned int *tri_off);
 /* All meshes of the last batch at once: vertices (3 * vert_off[nmol] floats), triangles
  * (3 * tri_off[nmol] ints, vertex indices local to each molecule), atom ids (vert_off[nmol]). */
 int qs_batch_fetch(float *vert, int *tri, int *atom_id);
+/* The same arrays in host memory owned by the library (pinned, so the copies run at the link's
+ * speed), valid until the next qs_batch_fetch_host call; any pointer argument may be NULL. */
+int qs_batch_fetch_host(const float **vert, const int **tri, const int **atom_id);
 
 /* Per-vertex normals of the last computation (plain or batch; vertices in the same order as its
  * vertex arrays): the unit gradient of the SES field at each vertex's marching-cubes position

@@ -61,6 +61,8 @@ def lib():
     L.qs_batch_mesh.restype = ctypes.c_int
     L.qs_batch_fetch.argtypes = [vp, vp, vp]
     L.qs_batch_fetch.restype = ctypes.c_int
+    L.qs_batch_fetch_host.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    L.qs_batch_fetch_host.restype = ctypes.c_int
     L.qs_batch_offsets.argtypes = [u32p, u32p]
     L.qs_batch_offsets.restype = ctypes.c_int
     L.qs_copy_device.argtypes = [vp, vp, ctypes.c_uint64, f3p]
@@ -142,11 +144,12 @@ def get_option(key):
     return lib().qs_get_option(key.encode())
 
 
-def compute_ses(pos, rad, reso=0.5, smooth=1, invert_triangles=False):
+def compute_ses(pos, rad, reso=0.5, smooth=1, invert_triangles=False, copy=True):
     """API_computeSES + getters + API_freeMesh (CudaSurf.cu:787-872).
 
     pos: (N, 3) float32, rad: (N,) float32.  Returns (vertices (NV, 3) float32,
-    triangles (NT, 3) int32, atom_ids (NV,) int32)."""
+    triangles (NT, 3) int32, atom_ids (NV,) int32).  copy=False returns views of the library's
+    host arrays instead of copies, without API_freeMesh: valid until the next computation."""
     L = lib()
     pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
     rad = np.ascontiguousarray(rad, dtype=np.float32).reshape(-1)
@@ -163,11 +166,15 @@ def compute_ses(pos, rad, reso=0.5, smooth=1, invert_triangles=False):
         tp = L.API_getTriangles(bool(invert_triangles))
         vp = L.API_getVertices()
         ap = L.API_getAtomIdPerVert()
-        V = np.ctypeslib.as_array(vp, shape=(max(nv.value, 1) * 3,))[: nv.value * 3].reshape(-1, 3).copy()
-        T = np.ctypeslib.as_array(tp, shape=(max(nt.value, 1),))[: nt.value].reshape(-1, 3).copy()
-        A = np.ctypeslib.as_array(ap, shape=(max(nv.value, 1),))[: nv.value].copy()
+        V = np.ctypeslib.as_array(vp, shape=(max(nv.value, 1) * 3,))[: nv.value * 3].reshape(-1, 3)
+        T = np.ctypeslib.as_array(tp, shape=(max(nt.value, 1),))[: nt.value].reshape(-1, 3)
+        A = np.ctypeslib.as_array(ap, shape=(max(nv.value, 1),))[: nv.value]
+        if not copy:
+            return V, T, A
+        V, T, A = V.copy(), T.copy(), A.copy()
     finally:
-        L.API_freeMesh()
+        if copy or err:
+            L.API_freeMesh()
     return V, T, A
 
 
@@ -197,10 +204,11 @@ def _batch_desc(counts, reso):
     return off, reso
 
 
-def compute_batch(molecules, reso=0.5, smooth=1):
-    """qs_compute_batch + qs_batch_mesh: molecules is a list of (pos (N,3), rad (N,)) pairs,
+def compute_batch(molecules, reso=0.5, smooth=1, copy=True):
+    """qs_compute_batch + qs_batch_fetch: molecules is a list of (pos (N,3), rad (N,)) pairs,
     reso a voxel size or one per molecule.  Returns a list of (vertices, triangles, atom_ids),
-    each equal to compute_ses on that molecule alone."""
+    each equal to compute_ses on that molecule alone.  copy=False returns views of the library's
+    pinned host arrays (qs_batch_fetch_host), valid until the next batch fetch."""
     L = lib()
     pos = [np.ascontiguousarray(p, dtype=np.float32).reshape(-1, 3) for p, _ in molecules]
     rad = [np.ascontiguousarray(r, dtype=np.float32).reshape(-1) for _, r in molecules]
@@ -221,6 +229,15 @@ def compute_batch(molecules, reso=0.5, smooth=1):
     to = np.zeros(nmol + 1, dtype=np.int64)
     vo[1:] = np.cumsum(nv[:nmol])
     to[1:] = np.cumsum(nt[:nmol])
+    if not copy:
+        pv, pt, pa = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(L.qs_batch_fetch_host(ctypes.byref(pv), ctypes.byref(pt), ctypes.byref(pa)))
+        nvt, ntt = int(vo[-1]), int(to[-1])
+        V = np.ctypeslib.as_array(ctypes.cast(pv, ctypes.POINTER(ctypes.c_float)), shape=(max(nvt, 1) * 3,))[: 3 * nvt]
+        T = np.ctypeslib.as_array(ctypes.cast(pt, ctypes.POINTER(ctypes.c_int)), shape=(max(ntt, 1) * 3,))[: 3 * ntt]
+        A = np.ctypeslib.as_array(ctypes.cast(pa, ctypes.POINTER(ctypes.c_int)), shape=(max(nvt, 1),))[:nvt]
+        V, T = V.reshape(-1, 3), T.reshape(-1, 3)
+        return [(V[vo[m]:vo[m + 1]], T[to[m]:to[m + 1]], A[vo[m]:vo[m + 1]]) for m in range(nmol)]
     V = np.empty((int(vo[-1]), 3), dtype=np.float32)
     T = np.empty((int(to[-1]), 3), dtype=np.int32)
     A = np.empty(int(vo[-1]), dtype=np.int32)

@@ -669,15 +669,41 @@ int *g_triangles = nullptr;
 int *g_atom_ids = nullptr;
 unsigned g_ntri_idx = 0, g_nvert = 0;
 
-/* API_freeMesh keeps the result's host arrays (the largest set so far) for the next call: the
-   device-to-host copies then land in pages that are already mapped.  Into freshly allocated
-   memory every page faults during the copy, which for a mesh of a few hundred MB costs several
-   times the copy (4 GB/s against 17 GB/s into mapped pageable memory). */
+/* Host result arrays.  Arrays of 1 MB or more are pinned (hipHostMalloc), so the device-to-host
+   copies run at the link's speed; smaller ones (and every array of an empty mesh, which must not
+   touch the device) are ordinary memory.  API_freeMesh keeps the result's arrays (the largest set
+   so far) for the next call: pinning, like first-touch page faults in fresh pageable memory, costs
+   more than the copy itself. */
 struct HostArray {
     void *p = nullptr;
     size_t cap = 0;
+    bool pinned = false;
 };
 HostArray g_cur[3], g_spare[3]; /* vertices, triangles, atom ids */
+
+constexpr size_t PIN_MIN = 1u << 20;
+
+HostArray host_alloc(size_t bytes)
+{
+    HostArray a;
+    if (bytes >= PIN_MIN && hipHostMalloc(&a.p, bytes, hipHostMallocDefault) == hipSuccess) {
+        a.pinned = true;
+    } else {
+        (void)hipGetLastError();
+        a.p = malloc(bytes);
+    }
+    a.cap = a.p ? bytes : 0;
+    return a;
+}
+
+void host_free(HostArray &a)
+{
+    if (a.p) {
+        if (a.pinned) (void)hipHostFree(a.p);
+        else free(a.p);
+    }
+    a = HostArray{};
+}
 
 void *host_array(int k, size_t bytes)
 {
@@ -686,10 +712,22 @@ void *host_array(int k, size_t bytes)
         g_cur[k] = s;
         s = HostArray{};
     } else {
-        g_cur[k].p = malloc(bytes);
-        g_cur[k].cap = g_cur[k].p ? bytes : 0;
+        g_cur[k] = host_alloc(bytes);
     }
     return g_cur[k].p;
+}
+
+/* the last batch's meshes in library-owned host arrays (qs_batch_fetch_host), grow-only */
+HostArray g_batch_host[3];
+
+void *batch_host_array(int k, size_t bytes)
+{
+    HostArray &a = g_batch_host[k];
+    if (a.cap < bytes) {
+        host_free(a);
+        a = host_alloc(bytes);
+    }
+    return a.p;
 }
 
 /* one-molecule batch with a given grid (stage-wise entry points) */
@@ -738,12 +776,12 @@ void clear_host_result()
     for (int k = 0; k < 3; k++) {
         HostArray &c = g_cur[k], &s = g_spare[k];
         if (c.p && c.cap > s.cap) {
-            free(s.p);
+            host_free(s);
             s = c;
+            c = HostArray{};
         } else {
-            free(c.p);
+            host_free(c);
         }
-        c = HostArray{};
     }
     g_vertices = nullptr;
     g_triangles = nullptr;
@@ -986,6 +1024,33 @@ int qs_batch_fetch(float *vert, int *tri, int *atom_id)
         QS_HIP(hipMemcpyAsync(tri, E.tri_loc.p, (size_t)E.nt * 12, hipMemcpyDeviceToHost, E.stream));
         QS_HIP(hipStreamSynchronize(E.stream));
     }
+    return 0;
+}
+
+int qs_batch_fetch_host(const float **vert, const int **tri, const int **atom_id)
+{
+    Engine &E = g_eng;
+    if (!E.batch_valid) return set_error(3, "no batch result");
+    const int nmol = (int)E.mols.size();
+    float *hv = (float *)batch_host_array(0, (size_t)std::max<uint32_t>(E.nv, 1) * 12);
+    int *ht = (int *)batch_host_array(1, (size_t)std::max<uint32_t>(E.nt, 1) * 12);
+    int *ha = (int *)batch_host_array(2, (size_t)std::max<uint32_t>(E.nv, 1) * 4);
+    if (!hv || !ht || !ha) return set_error(1, "out of host memory");
+    if (E.nv) {
+        QS_HIP(hipMemcpyAsync(hv, E.res_vert, (size_t)E.nv * 12, hipMemcpyDeviceToHost, E.stream));
+        QS_HIP(hipMemcpyAsync(ha, E.res_atom, (size_t)E.nv * 4, hipMemcpyDeviceToHost, E.stream));
+    }
+    if (E.nt) {
+        QS_TRY(ensure(E.tri_loc, (size_t)E.nt * 12));
+        qs_launch_tri_localize(E.res_tri, E.nt, as<uint32_t>(E.bounds), as<uint32_t>(E.d_off) + (nmol + 1), nmol,
+                               as<int>(E.tri_loc), E.stream);
+        QS_HIP(hipGetLastError());
+        QS_HIP(hipMemcpyAsync(ht, E.tri_loc.p, (size_t)E.nt * 12, hipMemcpyDeviceToHost, E.stream));
+    }
+    QS_HIP(hipStreamSynchronize(E.stream));
+    if (vert) *vert = hv;
+    if (tri) *tri = ht;
+    if (atom_id) *atom_id = ha;
     return 0;
 }
 

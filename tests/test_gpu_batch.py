@@ -115,6 +115,26 @@ def test_batch_getters_agree(qs):
         _same_mesh(qs.batch_mesh(m), out[m])
 
 
+def test_library_host_arrays(qs):
+    """Views of the library's (pinned) host arrays equal the copied results: the batch fetch into
+    library-owned memory, and API_computeSES's arrays without a copy; a larger batch after a
+    smaller one grows the arrays, a smaller one after it reuses them."""
+    mols = _mixed(qs)
+    big = mols + [qs.synth_ball(790, 6000)]
+    for case in (mols, big, mols[:3]):
+        print("case host views nmol=%d" % len(case), flush=True)
+        ref = qs.compute_batch(case, 0.4, 1)
+        views = qs.compute_batch(case, 0.4, 1, copy=False)
+        assert len(views) == len(ref)
+        for a, b in zip(views, ref):
+            _same_mesh(a, b)
+    pos, rad = qs.synth_ball(791, 3000)
+    ref = qs.compute_ses(pos, rad, 0.3, 1)
+    view = qs.compute_ses(pos, rad, 0.3, 1, copy=False)
+    _same_mesh(view, ref)
+    qs.lib().API_freeMesh()
+
+
 @pytest.mark.parametrize("opt", ["lim_mode", "atom_ids_sorted", "pass1_direct", "pass1_rows", "pass2_direct"])
 def test_batch_options_match_oracle(qs, orc, opt):
     """Each option in a batch: the per-voxel cross-check kernels and the reference's window and
