@@ -139,9 +139,10 @@ def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_0
     sequence per chunk of molecules) on the two streams of the batched workload, at two
     boundaries: device-resident (atoms already in HBM, meshes left in HBM) and host arrays in /
     host arrays out, the C ABI's boundary (API_computeSES per molecule, its results in the
-    library's host arrays, against qs_compute_batch + qs_batch_fetch_host, all meshes of a launch
-    sequence in the library's host arrays).  Each leg runs once untimed first, so that the
-    library's grow-only device and host arrays have their steady-state sizes."""
+    library's host arrays, against qs_compute_batch + qs_batch_fetch_host_async /
+    qs_batch_wait_host over about four launch sequences, each sequence's copies to the library's
+    host arrays overlapping the next sequence's work).  Each leg runs once untimed first, so that
+    the library's grow-only device and host arrays have their steady-state sizes."""
     import numpy as np
 
     mols, frames = config4_sets(qs, 256, n_frames)
@@ -186,13 +187,13 @@ def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_0
         tpd = time.perf_counter() - t0
         d_pos.free()
         d_rad.free()
-        # host in, host out (views of the library's host arrays, valid until its next call)
-        for ch in chunks:
-            qs.compute_batch(ch, voxel, smooth, copy=False)
+        # host in, host out (views of the library's host arrays)
+        host_chunks = chunks_by_voxels(items, voxel, max(vox // 4, 1))
+        for _ in qs.compute_batches_host(host_chunks, voxel, smooth):
+            pass
         t0 = time.perf_counter()
         nv_b = 0
-        for ch in chunks:
-            out = qs.compute_batch(ch, voxel, smooth, copy=False)
+        for out in qs.compute_batches_host(host_chunks, voxel, smooth):
             nv_b += sum(len(o[0]) for o in out)
         tb = time.perf_counter() - t0
         big = max(range(len(items)), key=lambda m: counts[m])
@@ -208,7 +209,8 @@ def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_0
         n = len(items)
         print("batched stream %s: %d meshes, device-resident plain %.3f s batched %.3f s, host-to-host plain %.3f s "
               "batched %.3f s" % (name, n, tpd, tbd, tp, tb), file=sys.stderr, flush=True)
-        res[name] = dict(meshes=n, voxels=vox, launch_sequences=len(chunks), vertices=nv_b,
+        res[name] = dict(meshes=n, voxels=vox, launch_sequences=len(chunks), host_launch_sequences=len(host_chunks),
+                         vertices=nv_b,
                          first_sequence=dict(meshes=bounds[0][1] - bounds[0][0], stage_ms=chunk_stage_ms),
                          device_resident=dict(plain_meshes_per_s=round(n / tpd, 2), batched_meshes_per_s=round(n / tbd, 2),
                                               batched_voxels_per_s=round(vox / tbd, 1), speedup=round(tpd / tbd, 3)),

@@ -108,6 +108,14 @@ int qs_batch_fetch(float *vert, int *tri, int *atom_id);
 /* The same arrays in host memory owned by the library (pinned, so the copies run at the link's
  * speed), valid until the next qs_batch_fetch_host call; any pointer argument may be NULL. */
 int qs_batch_fetch_host(const float **vert, const int **tri, const int **atom_id);
+/* Pipelined form: qs_batch_fetch_host_async(slot) (slot 0 or 1) starts copying the last batch's
+ * meshes to the slot's pinned host arrays and returns at once; the next qs_compute_batch may
+ * start while the copies run.  qs_batch_wait_host(slot, ...) waits for them and gives the arrays
+ * (layout as qs_batch_fetch_host) and the vertex and triangle totals, valid until the slot's next
+ * fetch.  A slot's next fetch first waits for its previous copies. */
+int qs_batch_fetch_host_async(int slot);
+int qs_batch_wait_host(int slot, const float **vert, const int **tri, const int **atom_id, unsigned int *nvert,
+                       unsigned int *ntri);
 
 /* Per-vertex normals of the last computation (plain or batch; vertices in the same order as its
  * vertex arrays): the unit gradient of the SES field at each vertex's marching-cubes position

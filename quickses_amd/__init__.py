@@ -63,6 +63,10 @@ def lib():
     L.qs_batch_fetch.restype = ctypes.c_int
     L.qs_batch_fetch_host.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.qs_batch_fetch_host.restype = ctypes.c_int
+    L.qs_batch_fetch_host_async.argtypes = [ctypes.c_int]
+    L.qs_batch_fetch_host_async.restype = ctypes.c_int
+    L.qs_batch_wait_host.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), u32p, u32p]
+    L.qs_batch_wait_host.restype = ctypes.c_int
     L.qs_batch_offsets.argtypes = [u32p, u32p]
     L.qs_batch_offsets.restype = ctypes.c_int
     L.qs_copy_device.argtypes = [vp, vp, ctypes.c_uint64, f3p]
@@ -244,6 +248,61 @@ def compute_batch(molecules, reso=0.5, smooth=1, copy=True):
     rc = L.qs_batch_fetch(ctypes.c_void_p(V.ctypes.data), ctypes.c_void_p(T.ctypes.data), ctypes.c_void_p(A.ctypes.data))
     _check(rc)
     return [(V[vo[m]:vo[m + 1]], T[to[m]:to[m + 1]], A[vo[m]:vo[m + 1]]) for m in range(nmol)]
+
+
+def _run_batch(L, molecules, reso, smooth):
+    """qs_compute_batch on host arrays; returns the per-molecule vertex and triangle offsets"""
+    pos = [np.ascontiguousarray(p, dtype=np.float32).reshape(-1, 3) for p, _ in molecules]
+    rad = [np.ascontiguousarray(r, dtype=np.float32).reshape(-1) for _, r in molecules]
+    for p, r in zip(pos, rad):
+        if p.shape[0] != r.shape[0]:
+            raise ValueError("pos and rad disagree on the atom count")
+    off, reso = _batch_desc([p.shape[0] for p in pos], reso)
+    nmol = len(pos)
+    P = np.ascontiguousarray(np.concatenate(pos) if nmol else np.zeros((0, 3), np.float32))
+    R = np.ascontiguousarray(np.concatenate(rad) if nmol else np.zeros(0, np.float32))
+    nv = np.zeros(max(nmol, 1), dtype=np.uint32)
+    nt = np.zeros(max(nmol, 1), dtype=np.uint32)
+    u32p = ctypes.POINTER(ctypes.c_uint)
+    _check(L.qs_compute_batch(_fptr(P), _fptr(R), off.ctypes.data_as(u32p), nmol, _fptr(reso), int(smooth),
+                              nv.ctypes.data_as(u32p), nt.ctypes.data_as(u32p)))
+    vo = np.zeros(nmol + 1, dtype=np.int64)
+    to = np.zeros(nmol + 1, dtype=np.int64)
+    vo[1:] = np.cumsum(nv[:nmol])
+    to[1:] = np.cumsum(nt[:nmol])
+    return vo, to
+
+
+def _slot_views(L, slot, vo, to):
+    vp = ctypes.c_void_p
+    pv, pt, pa = vp(), vp(), vp()
+    nv, nt = ctypes.c_uint(0), ctypes.c_uint(0)
+    _check(L.qs_batch_wait_host(slot, ctypes.byref(pv), ctypes.byref(pt), ctypes.byref(pa), ctypes.byref(nv),
+                                ctypes.byref(nt)))
+    nvt, ntt = nv.value, nt.value
+    V = np.ctypeslib.as_array(ctypes.cast(pv, ctypes.POINTER(ctypes.c_float)), shape=(max(nvt, 1) * 3,))[: 3 * nvt]
+    T = np.ctypeslib.as_array(ctypes.cast(pt, ctypes.POINTER(ctypes.c_int)), shape=(max(ntt, 1) * 3,))[: 3 * ntt]
+    A = np.ctypeslib.as_array(ctypes.cast(pa, ctypes.POINTER(ctypes.c_int)), shape=(max(nvt, 1),))[:nvt]
+    V, T = V.reshape(-1, 3), T.reshape(-1, 3)
+    return [(V[vo[m]:vo[m + 1]], T[to[m]:to[m + 1]], A[vo[m]:vo[m + 1]]) for m in range(len(vo) - 1)]
+
+
+def compute_batches_host(chunks, reso=0.5, smooth=1):
+    """Mesh a sequence of batches (each a list of (pos, rad) pairs) with the copies of one batch
+    to the host overlapping the next batch's work (qs_batch_fetch_host_async / qs_batch_wait_host).
+    Yields, batch by batch, the list of (vertices, triangles, atom_ids) views of the library's
+    pinned host arrays; a batch's views stay valid until the generator resumes (copy them to keep
+    them)."""
+    L = lib()
+    prev = None
+    for k, ch in enumerate(chunks):
+        vo, to = _run_batch(L, ch, reso, smooth)
+        _check(L.qs_batch_fetch_host_async(k & 1))
+        if prev is not None:
+            yield _slot_views(L, *prev)
+        prev = (k & 1, vo, to)
+    if prev is not None:
+        yield _slot_views(L, *prev)
 
 
 def batch_offsets():

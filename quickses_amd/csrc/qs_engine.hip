@@ -720,6 +720,29 @@ void *host_array(int k, size_t bytes)
 /* the last batch's meshes in library-owned host arrays (qs_batch_fetch_host), grow-only */
 HostArray g_batch_host[3];
 
+/* pipelined host fetch (qs_batch_fetch_host_async / qs_batch_wait_host): two slots, each a
+   device staging copy of one batch's meshes and their pinned host arrays.  The staging copies run
+   on the library stream after the batch; the device-to-host copies run on a copy stream (default
+   priority), so that they overlap the next batch's work. */
+struct HostSlot {
+    DevBuf v, t, a;
+    HostArray hv, ht, ha;
+    hipEvent_t staged = nullptr, done = nullptr;
+    uint32_t nv = 0, nt = 0;
+    bool pending = false, ready = false;
+};
+HostSlot g_slot[2];
+hipStream_t g_copy = nullptr;
+int g_copy_dev = -1;
+
+void grow_host(HostArray &a, size_t bytes)
+{
+    if (a.cap < bytes) {
+        host_free(a);
+        a = host_alloc(bytes);
+    }
+}
+
 void *batch_host_array(int k, size_t bytes)
 {
     HostArray &a = g_batch_host[k];
@@ -1051,6 +1074,73 @@ int qs_batch_fetch_host(const float **vert, const int **tri, const int **atom_id
     if (vert) *vert = hv;
     if (tri) *tri = ht;
     if (atom_id) *atom_id = ha;
+    return 0;
+}
+
+int qs_batch_fetch_host_async(int slot)
+{
+    Engine &E = g_eng;
+    if (!E.batch_valid) return set_error(3, "no batch result");
+    if (slot < 0 || slot > 1) return set_error(3, "slot must be 0 or 1");
+    if (g_copy_dev != E.device || !g_copy) {
+        QS_HIP(hipStreamCreateWithFlags(&g_copy, hipStreamNonBlocking));
+        for (HostSlot &S : g_slot) {
+            QS_HIP(hipEventCreateWithFlags(&S.staged, hipEventDisableTiming));
+            QS_HIP(hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
+        }
+        g_copy_dev = E.device;
+    }
+    HostSlot &S = g_slot[slot];
+    if (S.pending) QS_HIP(hipEventSynchronize(S.done)); /* its previous copies still read the slot */
+    S.pending = S.ready = false;
+    const int nmol = (int)E.mols.size();
+    const size_t nv = E.nv, nt = E.nt;
+    QS_TRY(ensure(S.v, std::max<size_t>(nv, 1) * 12));
+    QS_TRY(ensure(S.a, std::max<size_t>(nv, 1) * 4));
+    QS_TRY(ensure(S.t, std::max<size_t>(nt, 1) * 12));
+    grow_host(S.hv, std::max<size_t>(nv, 1) * 12);
+    grow_host(S.ha, std::max<size_t>(nv, 1) * 4);
+    grow_host(S.ht, std::max<size_t>(nt, 1) * 12);
+    if (!S.hv.p || !S.ha.p || !S.ht.p) return set_error(1, "out of host memory");
+    if (nv) {
+        QS_HIP(hipMemcpyAsync(S.v.p, E.res_vert, nv * 12, hipMemcpyDeviceToDevice, E.stream));
+        QS_HIP(hipMemcpyAsync(S.a.p, E.res_atom, nv * 4, hipMemcpyDeviceToDevice, E.stream));
+    }
+    if (nt) {
+        qs_launch_tri_localize(E.res_tri, (uint32_t)nt, as<uint32_t>(E.bounds), as<uint32_t>(E.d_off) + (nmol + 1), nmol,
+                               as<int>(S.t), E.stream);
+        QS_HIP(hipGetLastError());
+    }
+    QS_HIP(hipEventRecord(S.staged, E.stream));
+    QS_HIP(hipStreamWaitEvent(g_copy, S.staged, 0));
+    if (nv) {
+        QS_HIP(hipMemcpyAsync(S.hv.p, S.v.p, nv * 12, hipMemcpyDeviceToHost, g_copy));
+        QS_HIP(hipMemcpyAsync(S.ha.p, S.a.p, nv * 4, hipMemcpyDeviceToHost, g_copy));
+    }
+    if (nt) QS_HIP(hipMemcpyAsync(S.ht.p, S.t.p, nt * 12, hipMemcpyDeviceToHost, g_copy));
+    QS_HIP(hipEventRecord(S.done, g_copy));
+    S.nv = (uint32_t)nv;
+    S.nt = (uint32_t)nt;
+    S.pending = true;
+    return 0;
+}
+
+int qs_batch_wait_host(int slot, const float **vert, const int **tri, const int **atom_id, unsigned int *nvert,
+                       unsigned int *ntri)
+{
+    if (slot < 0 || slot > 1) return set_error(3, "slot must be 0 or 1");
+    HostSlot &S = g_slot[slot];
+    if (S.pending) {
+        QS_HIP(hipEventSynchronize(S.done));
+        S.pending = false;
+        S.ready = true;
+    }
+    if (!S.ready) return set_error(3, "no fetch started on slot %d", slot);
+    if (vert) *vert = (const float *)S.hv.p;
+    if (tri) *tri = (const int *)S.ht.p;
+    if (atom_id) *atom_id = (const int *)S.ha.p;
+    if (nvert) *nvert = S.nv;
+    if (ntri) *ntri = S.nt;
     return 0;
 }
 

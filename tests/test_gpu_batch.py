@@ -135,6 +135,22 @@ def test_library_host_arrays(qs):
     qs.lib().API_freeMesh()
 
 
+def test_pipelined_host_fetch(qs):
+    """Batches fetched through the two pipelined slots, each batch's copies overlapping the next
+    batch's work, equal the batches fetched one at a time; sizes grow, shrink and grow again."""
+    mols = _mixed(qs)
+    chunks = [mols[:4], mols + [qs.synth_ball(792, 5000)], mols[2:5], mols[:1], mols]
+    ref = [qs.compute_batch(ch, 0.4, 1) for ch in chunks]
+    print("case pipelined host fetch, %d batches" % len(chunks), flush=True)
+    got = 0
+    for k, out in enumerate(qs.compute_batches_host(chunks, 0.4, 1)):
+        assert len(out) == len(ref[k])
+        for a, b in zip(out, ref[k]):
+            _same_mesh(a, b)
+        got += 1
+    assert got == len(chunks)
+
+
 @pytest.mark.parametrize("opt", ["lim_mode", "atom_ids_sorted", "pass1_direct", "pass1_rows", "pass2_direct"])
 def test_batch_options_match_oracle(qs, orc, opt):
     """Each option in a batch: the per-voxel cross-check kernels and the reference's window and
