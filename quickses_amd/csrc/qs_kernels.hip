@@ -1348,8 +1348,6 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *_
     }
 }
 
-/* Field pass 2 for a fixed window half-width R: loops unrolled, y and x passes register-blocked
-   (each thread keeps one column of the previous pass in registers). */
 /* squared distance from bit z + R of row m to its nearest set bit within R; z < 32 and 2R < 32,
    so bits z .. z + 2R are the low bits of one 32-bit funnel shift.  Branch free: bits above the
    window are cleared and a sentinel stands at bit 31, so a row with no set bit within R gives a
@@ -1366,6 +1364,17 @@ __device__ __forceinline__ uint32_t zdist2(uint64_t m, int z)
     const uint32_t d = min(up, dn);
     return d * d;
 }
+
+/* Field pass 2 for a fixed window half-width R (probeIntersection's second pass,
+   Kernels.cu:478-545), one 16 x 8 x 32 tile per block:
+   - the tile's own shell and outside bits; a tile without a shell voxel writes its sign words only;
+   - the outside rows of the tile and its R halo in LDS;
+   - the exact minimal squared norm n_min of every shell voxel by separable minima: z distances
+     from the row bits, y minima (with the masks of the attaining y offsets) for the listed
+     columns only, x minima two outputs per register;
+   - the exact float distance of the minimisers, traced back through the masks (or enumerated
+     from a norm-sorted table where another norm could tie), rooted once;
+   - the field of shell voxels (the others follow from the masks) and every voxel's sign bits. */
 
 /* entry it of a per-plane bit list: the plane lx with cpre[lx] <= it < cpre[lx + 1] (binary
    search) and the z of the (it - cpre[lx])-th set bit of cmask[lx] */
