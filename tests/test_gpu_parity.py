@@ -134,13 +134,13 @@ def test_pass2_tile_equals_direct(qs, orc):
         assert np.array_equal(direct.view(np.uint32), fld.view(np.uint32))
 
 
-@pytest.mark.parametrize("reso", [0.18, 0.16])
-def test_wide_windows_match_oracle(qs, orc, reso):
-    """Window half-widths 8 and 9 (2R + 1 above 16: the unpacked y and x passes of k_pass2_r):
-    field pass 2 from the oracle's labels and the whole mesh, bit for bit."""
+@pytest.mark.parametrize("reso", [0.18, 0.16, 1.5])
+def test_window_extremes_match_oracle(qs, orc, reso):
+    """Window half-widths 8 and 9 (2R + 1 above 16: the unpacked y and x passes of k_pass2_r) and
+    1 (the narrowest): field pass 2 from the oracle's labels and the whole mesh, bit for bit."""
     pos, rad = _ball(qs, 23, 120)
     G, perm, cs, ce, sx, lab, fld = orc.field(pos, rad, reso)
-    assert G.r in (8, 9)
+    assert G.r in (1, 8, 9)
     print("case wide window r=%d reso=%.2f g=%d" % (G.r, reso, G.g), flush=True)
     got = qs.stage_pass2(G.params(G.g - 1), lab)
     assert np.array_equal(got.view(np.uint32), fld.view(np.uint32))
