@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -188,6 +189,53 @@ struct Engine {
     /* host staging of uploads that kernels read later in the same call */
     std::vector<uint32_t> h_meta, h_vpre;
     std::vector<uint64_t> h_bidx; /* first triangle of each molecule, staged for the filter */
+    /* pinned staging for a run's small uploads (molecule records, prefixes, indices): copies from
+       it are queued at once, where the runtime stages pageable sources on the host first */
+    void *up = nullptr;
+    size_t up_cap = 0, up_used = 0, up_need = 0;
+    int upload(void *dst, const void *src, size_t bytes)
+    {
+        if (!bytes) return 0;
+        const size_t off = (up_used + 255) & ~(size_t)255;
+        if (up && off + bytes <= up_cap) {
+            memcpy((char *)up + off, src, bytes);
+            up_used = off + bytes;
+            QS_HIP(hipMemcpyAsync(dst, (char *)up + off, bytes, hipMemcpyHostToDevice, stream));
+            return 0;
+        }
+        up_need = std::max(up_need, off + bytes);
+        QS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+        return 0;
+    }
+    /* a pinned landing area for a download that is read after the next stream synchronisation,
+       or nullptr when the staging is full (the caller then copies into its own memory) */
+    void *stage(size_t bytes)
+    {
+        const size_t off = (up_used + 255) & ~(size_t)255;
+        if (up && off + bytes <= up_cap) {
+            up_used = off + bytes;
+            return (char *)up + off;
+        }
+        up_need = std::max(up_need, off + bytes);
+        return nullptr;
+    }
+    /* at the start of a run: the last run's uploads have been read (a run ends with a stream
+       synchronisation; one that stopped early is waited for here), and the staging grows to what
+       the last run needed */
+    int upload_reset()
+    {
+        if (up_used || up_need) QS_HIP(hipStreamSynchronize(stream));
+        if (up_need > up_cap) {
+            if (up) (void)hipHostFree(up);
+            up = nullptr;
+            up_cap = 0;
+            const size_t cap = std::max<size_t>(2 * up_need, 1 << 16);
+            if (hipHostMalloc(&up, cap, hipHostMallocDefault) == hipSuccess) up_cap = cap;
+            else (void)hipGetLastError();
+        }
+        up_used = up_need = 0;
+        return 0;
+    }
 
     /* last result */
     std::vector<QsMol> mols;
@@ -218,6 +266,10 @@ struct Engine {
         }
         tables.clear();
         groups.clear();
+        if (stream) hipStreamSynchronize(stream);
+        if (up) hipHostFree(up);
+        up = nullptr;
+        up_cap = up_used = up_need = 0;
         if (stream) hipStreamDestroy(stream);
         stream = nullptr;
         if (side) hipStreamDestroy(side);

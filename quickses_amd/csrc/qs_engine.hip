@@ -430,14 +430,17 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_TRY(ensure(bidx, ((size_t)nmol + 1) * 8));
     QS_TRY(ensure(bounds, ((size_t)nmol + 1) * 8));
     std::vector<uint32_t> hb(2 * (size_t)nmol + 3);
+    uint32_t *hbs = (uint32_t *)stage(hb.size() * 4);
+    uint32_t *hbd = hbs ? hbs : hb.data();
     if (nmol && nw_total) {
-        QS_HIP(hipMemcpyAsync(bidx.p, idx.data(), (size_t)nmol * 8, hipMemcpyHostToDevice, stream));
+        QS_TRY(upload(bidx.p, idx.data(), (size_t)nmol * 8));
         qs_launch_gather_u32(as<uint32_t>(vbw), as<uint64_t>(bidx), nmol, as<uint32_t>(bounds), stream);
         qs_launch_gather_u32(as<uint32_t>(tbw), as<uint64_t>(bidx), nmol, as<uint32_t>(bounds) + nmol, stream);
-        QS_HIP(hipMemcpyAsync(hb.data(), bounds.p, (size_t)nmol * 8, hipMemcpyDeviceToHost, stream));
+        QS_HIP(hipMemcpyAsync(hbd, bounds.p, (size_t)nmol * 8, hipMemcpyDeviceToHost, stream));
     }
-    QS_HIP(hipMemcpyAsync(hb.data() + 2 * (size_t)nmol, totals.p, 12, hipMemcpyDeviceToHost, stream));
+    QS_HIP(hipMemcpyAsync(hbd + 2 * (size_t)nmol, totals.p, 12, hipMemcpyDeviceToHost, stream));
     QS_HIP(hipStreamSynchronize(stream));
+    if (hbs) memcpy(hb.data(), hbs, hb.size() * 4);
     tm.mark("mc_count");
     nt_all = hb[2 * (size_t)nmol];
     nv = hb[2 * (size_t)nmol + 1];
@@ -461,7 +464,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
         mols[m].n_tri = tpre[m + 1] - tpre[m];
     }
     vert_off = vpre;
-    QS_HIP(hipMemcpyAsync(d_mols.p, mols.data(), mols.size() * sizeof(QsMol), hipMemcpyHostToDevice, stream));
+    QS_TRY(upload(d_mols.p, mols.data(), mols.size() * sizeof(QsMol)));
     /* vertex prefix and its coarse index for the per-vertex kernels, after the atom offsets */
     {
         std::vector<uint32_t> vc;
@@ -470,7 +473,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     }
     QS_TRY(ensure(d_off, ((size_t)nmol + 1) * 4 + vpre.size() * 4 + 64));
     uint32_t *d_vpre = as<uint32_t>(d_off) + (nmol + 1);
-    QS_HIP(hipMemcpyAsync(d_vpre, vpre.data(), vpre.size() * 4, hipMemcpyHostToDevice, stream));
+    QS_TRY(upload(d_vpre, vpre.data(), vpre.size() * 4));
     QS_TRY(ensure(vert, (size_t)std::max<uint32_t>(nv, 1) * 12));
     QS_TRY(ensure(vkey, (size_t)std::max<uint32_t>(nv, 1) * 8));
     QS_TRY(ensure(tri, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
@@ -518,6 +521,7 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
                       const std::vector<BBox> *host_bbox)
 {
     const int nmol = (int)in.size();
+    QS_TRY(upload_reset());
     batch_valid = false;
     nv = nt = nt_all = n_active = 0;
     res_vert = nullptr;
@@ -546,11 +550,13 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
         QS_TRY(ensure(d_off, ((size_t)nmol + 1) * 4 * 2 + 64));
         QS_TRY(ensure(bb_out, (size_t)nmol * 8 * sizeof(float)));
         QS_TRY(ensure(bb_part, qs_bbox_part_floats(nmol) * sizeof(float)));
-        QS_HIP(hipMemcpyAsync(d_off.p, off.data(), (nmol + 1) * 4, hipMemcpyHostToDevice, stream));
+        QS_TRY(upload(d_off.p, off.data(), (nmol + 1) * 4));
         qs_launch_bbox(d_pos, d_rad, as<uint32_t>(d_off), nmol, as<float>(bb_part), as<float>(bb_out), stream);
         std::vector<float> h((size_t)nmol * 8);
-        QS_HIP(hipMemcpyAsync(h.data(), bb_out.p, h.size() * 4, hipMemcpyDeviceToHost, stream));
+        float *hs = (float *)stage(h.size() * 4);
+        QS_HIP(hipMemcpyAsync(hs ? (void *)hs : (void *)h.data(), bb_out.p, h.size() * 4, hipMemcpyDeviceToHost, stream));
         QS_HIP(hipStreamSynchronize(stream));
+        if (hs) memcpy(h.data(), hs, h.size() * 4);
         for (int m = 0; m < nmol; m++) {
             if (in[m].n > 1 && h[8 * m + 7] != 0.0f)
                 return set_error(3, "molecule %d: atom positions or radii contain NaN or Inf", m);
@@ -568,8 +574,8 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     QS_TRY(build_meta(at_atoms, at_p1, at_mc, at_wpre, nb_atoms, nb_p1, nb_mc));
     QS_TRY(ensure(d_mols, mols.size() * sizeof(QsMol)));
     QS_TRY(ensure(d_meta, h_meta.size() * 4));
-    QS_HIP(hipMemcpyAsync(d_mols.p, mols.data(), mols.size() * sizeof(QsMol), hipMemcpyHostToDevice, stream));
-    QS_HIP(hipMemcpyAsync(d_meta.p, h_meta.data(), h_meta.size() * 4, hipMemcpyHostToDevice, stream));
+    QS_TRY(upload(d_mols.p, mols.data(), mols.size() * sizeof(QsMol)));
+    QS_TRY(upload(d_meta.p, h_meta.data(), h_meta.size() * 4));
 
     /* ---- atoms into cells (CudaSurf.cu:594-622), global over the batch ---- */
     const uint32_t n = n_atoms_total;
@@ -640,12 +646,19 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     /* staging vectors: next written after this call's stream work has been waited for */
     h_bidx.resize(nmol + 1);
     for (int m = 0; m <= nmol; m++) h_bidx[m] = (m < nmol) ? mols[m].tri_off : nt_all;
-    QS_HIP(hipMemcpyAsync(bidx.p, h_bidx.data(), (size_t)(nmol + 1) * 8, hipMemcpyHostToDevice, stream));
+    QS_TRY(upload(bidx.p, h_bidx.data(), (size_t)(nmol + 1) * 8));
     qs_launch_tri_kept_before(as<int>(tri), as<uint32_t>(keeppos), as<uint64_t>(bidx), nmol + 1, as<uint32_t>(bounds), stream);
     uint32_t ndeg = 0;
-    QS_HIP(hipMemcpyAsync(tri_off.data(), bounds.p, (size_t)(nmol + 1) * 4, hipMemcpyDeviceToHost, stream));
-    QS_HIP(hipMemcpyAsync(&ndeg, as<uint32_t>(totals) + 3, 4, hipMemcpyDeviceToHost, stream));
+    uint32_t *fs = (uint32_t *)stage((size_t)(nmol + 2) * 4);
+    QS_HIP(hipMemcpyAsync(fs ? (void *)fs : (void *)tri_off.data(), bounds.p, (size_t)(nmol + 1) * 4, hipMemcpyDeviceToHost,
+                          stream));
+    QS_HIP(hipMemcpyAsync(fs ? (void *)(fs + nmol + 1) : (void *)&ndeg, as<uint32_t>(totals) + 3, 4, hipMemcpyDeviceToHost,
+                          stream));
     QS_HIP(hipStreamSynchronize(stream));
+    if (fs) {
+        memcpy(tri_off.data(), fs, (size_t)(nmol + 1) * 4);
+        ndeg = fs[nmol + 1];
+    }
     tm.mark("filter");
     nt = tri_off[nmol];
     res_tri = ndeg ? as<int>(tri_out) : as<int>(tri);
