@@ -174,7 +174,7 @@ struct Engine {
     bool side_pending = false; /* closest-atom work on `side` not yet joined into `stream` */
 
     DevBuf in_pos, in_rad, d_off, bb_part, bb_out, d_mols, d_meta;
-    DevBuf key, count, start, cursor, perm, cse, sx, aux;
+    DevBuf key, count /* and the scatter cursors after the counts */, start, perm, cse, sx, aux;
     DevBuf labels, outm, shellm, field, negm, zerom;
     DevBuf s4, vcnt, tcnt, vbw, tbw, alist, totals, bounds, bidx;
     DevBuf ntb, avb, atb; /* triangle-count bit planes per word; first vertex / triangle of each active word */
@@ -253,7 +253,7 @@ struct Engine {
     /* frees every device buffer and the stream (the engine can be initialised again) */
     void release_all()
     {
-        for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start, &cursor,
+        for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start,
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
                           &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey, &cidx,
                           &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp})

@@ -166,7 +166,8 @@ int Engine::reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts
     QS_TRY(ensure(in_pos, atoms * 12));
     for (DevBuf *b : {&in_rad, &key, &perm}) QS_TRY(ensure(*b, atoms * 4));
     for (DevBuf *b : {&sx, &aux}) QS_TRY(ensure(*b, atoms * 16));
-    for (DevBuf *b : {&count, &start, &cursor}) QS_TRY(ensure(*b, (cells + 1) * 4));
+    QS_TRY(ensure(count, (cells + 1) * 8));
+    QS_TRY(ensure(start, (cells + 1) * 4));
     QS_TRY(ensure(cse, cells * 8));
     for (DevBuf *b : {&vert, &vert2, &vert3}) QS_TRY(ensure(*b, verts * 12));
     QS_TRY(ensure(vkey, verts * 8));
@@ -422,7 +423,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
                            stream);
         QS_HIP(hipGetLastError());
     } else {
-        QS_HIP(hipMemsetAsync(totals.p, 0, 12, stream));
+        QS_HIP(hipMemsetAsync(totals.p, 0, 16, stream));
     }
     /* per-molecule first vertex and triangle: the scans at each molecule's first word */
     std::vector<uint64_t> idx(nmol);
@@ -484,8 +485,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     /* degenerate triangles per 256-triangle block and in total (totals[3]), counted by the
        triangle kernel */
     QS_TRY(ensure(deg_blk, ((size_t)nt_all / 256 + 1) * 4));
-    QS_HIP(hipMemsetAsync(deg_blk.p, 0, ((size_t)nt_all / 256 + 1) * 4, stream));
-    QS_HIP(hipMemsetAsync(as<uint32_t>(totals) + 3, 0, 4, stream));
+    /* (zeroed by the triangle launch's block-start kernel; totals[3] by the scans) */
     qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart), nact,
                           nv, as<uint64_t>(s4), as<float>(field), as<uint64_t>(outm),
                           field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey), stream);
@@ -581,20 +581,20 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     const uint32_t n = n_atoms_total;
     const uint64_t ncell = ncell_total;
     QS_TRY(ensure(key, (size_t)std::max<uint32_t>(n, 1) * 4));
-    QS_TRY(ensure(count, ncell * 4));
+    /* per-cell counts and scatter cursors, one allocation cleared by one fill */
+    QS_TRY(ensure(count, ncell * 8));
     QS_TRY(ensure(start, (ncell + 1) * 4));
-    QS_TRY(ensure(cursor, ncell * 4));
+    uint32_t *cursor = as<uint32_t>(count) + ncell;
     QS_TRY(ensure(perm, (size_t)std::max<uint32_t>(n, 1) * 4));
     QS_TRY(ensure(cse, ncell * 8));
     QS_TRY(ensure(sx, (size_t)std::max<uint32_t>(n, 1) * 16));
     QS_TRY(ensure(aux, (size_t)std::max<uint32_t>(n, 1) * 16));
-    QS_HIP(hipMemsetAsync(count.p, 0, ncell * 4, stream));
-    QS_HIP(hipMemsetAsync(cursor.p, 0, ncell * 4, stream));
+    QS_HIP(hipMemsetAsync(count.p, 0, ncell * 8, stream));
     if (nb_atoms) qs_launch_hash_count(batch(at_atoms, 0, nmol), nb_atoms, d_pos, as<int>(key), as<uint32_t>(count), stream);
     /* start[c] = atoms in cells < c (global); start[ncell] = n */
     QS_TRY(scan(as<uint32_t>(count), as<uint32_t>(start), ncell, as<uint32_t>(start) + ncell));
     if (n) {
-        qs_launch_scatter(as<int>(key), (int)n, as<uint32_t>(start), as<uint32_t>(cursor), as<int>(perm), stream);
+        qs_launch_scatter(as<int>(key), (int)n, as<uint32_t>(start), cursor, as<int>(perm), stream);
         qs_launch_cell_finalize(as<uint32_t>(start), as<uint32_t>(count), (int)ncell, as<int>(perm), as<int2>(cse), stream);
         qs_launch_gather_atoms(as<int>(perm), d_pos, d_rad, (int)n, as<float4>(sx), stream);
         qs_launch_atom_aux(batch(at_atoms, 0, nmol), nb_atoms, as<float4>(sx), as<float4>(aux), stream);

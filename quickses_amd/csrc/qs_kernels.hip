@@ -277,7 +277,8 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_reduce3(const uint32_t *__restri
     }
 }
 
-/* block p: exclusive scan of plane p of the tile sums in place, total to totals[p] */
+/* block p: exclusive scan of plane p of the tile sums in place, total to totals[p]; the slot
+   after the plane totals (the degenerate-triangle count, accumulated later) starts at zero */
 __global__ void __launch_bounds__(SCAN_NT) k_scan_planes(uint32_t *__restrict__ sums, uint64_t ntiles,
                                                          uint32_t *__restrict__ totals)
 {
@@ -302,6 +303,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_planes(uint32_t *__restrict__ 
         carry += tot;
     }
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+    if (blockIdx.x == 0 && threadIdx.x == 0) totals[gridDim.x] = 0u;
 }
 
 /* per word: first vertex (vbw) and first triangle (tbw); active words listed in order */
@@ -2062,9 +2064,11 @@ __device__ __forceinline__ uint32_t last_le(const uint32_t *__restrict__ a, uint
    items [base[k], base[k + 1]) (base[nact] = n), so block b starts in the word whose range holds
    item 256 b */
 __global__ void __launch_bounds__(256) k_block_starts(const uint32_t *__restrict__ base, uint32_t nact, uint32_t n,
-                                                      uint32_t *__restrict__ start)
+                                                      uint32_t *__restrict__ start, uint32_t *__restrict__ zero,
+                                                      uint32_t nzero)
 {
     const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    for (uint32_t i = k; i < nzero; i += gridDim.x * 256u) zero[i] = 0u; /* counters the next kernel adds to */
     if (k >= nact) return;
     const uint32_t b0 = (base[k] + 255u) >> 8;
     const uint32_t e = (k + 1 < nact) ? base[k + 1] : n;
@@ -2787,7 +2791,7 @@ void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32
                            hipStream_t s)
 {
     if (!nv || !nact) return;
-    hipLaunchKernelGGL(k_block_starts, dim3(nblk(nact, 256)), dim3(256), 0, s, avb, nact, nv, vstart);
+    hipLaunchKernelGGL(k_block_starts, dim3(nblk(nact, 256)), dim3(256), 0, s, avb, nact, nv, vstart, nullptr, 0u);
     hipLaunchKernelGGL(k_mc_vertices, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, avb, vstart, nact, nv,
                        S4, F, outm, shellm, vert, vkey);
 }
@@ -2799,7 +2803,8 @@ void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint3
                             hipStream_t s)
 {
     if (!ntri || !nact) return;
-    hipLaunchKernelGGL(k_block_starts, dim3(nblk(nact, 256)), dim3(256), 0, s, atb, nact, ntri, tstart);
+    /* the per-block degenerate counts start at zero (the triangle kernel adds to them) */
+    hipLaunchKernelGGL(k_block_starts, dim3(nblk(nact, 256)), dim3(256), 0, s, atb, nact, ntri, tstart, deg_blk, ntri / 256 + 1);
     hipLaunchKernelGGL(k_mc_triangles, dim3(nblk(ntri, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, atb, tstart, nact,
                        ntri, N, Zm, S4, vbw, NT, cidx, tri, deg_blk, deg_total);
 }
