@@ -1916,13 +1916,15 @@ struct MCView {
         if (sl > 2) id += (uint32_t)((m2 >> b) & 1ull);
         return id;
     }
-    /* vertex of cube edge e of cube (cx, cy, cz) */
-    __device__ __forceinline__ uint32_t corner_vid(int cx, int cy, int cz, int e) const
+    /* vertex of cube edge e of cube (cx, cy, cz), whose case is ci: the sign of the edge's low
+       corner is case bit (z offset) * 4 + (0, 1, 2, 3 for xy offsets 00, 10, 11, 01) */
+    __device__ __forceinline__ uint32_t corner_vid(int cx, int cy, int cz, int e, int ci) const
     {
-        int qx = cx + (int)((kEdgeLX >> e) & 1u), qy = cy + (int)((kEdgeLY >> e) & 1u), qz = cz + (int)((kEdgeLZ >> e) & 1u);
+        const int lx = (int)((kEdgeLX >> e) & 1u), ly = (int)((kEdgeLY >> e) & 1u), lz = (int)((kEdgeLZ >> e) & 1u);
+        int qx = cx + lx, qy = cy + ly, qz = cz + lz;
         int a = (int)((kEdgeAX >> (2 * e)) & 3u);
         int rx = qx + (a == 0), ry = qy + (a == 1), rz = qz + (a == 2);
-        bool qneg = nbit(qx, qy, qz);
+        const bool qneg = (ci >> (4 * lz + (ly ? (lx ? 2 : 3) : lx))) & 1;
         int nx = qneg ? rx : qx, ny = qneg ? ry : qy, nz = qneg ? rz : qz;
         if (zbit(nx, ny, nz)) return slot_id(nx, ny, nz, 3);
         return slot_id(qx, qy, qz, a);
@@ -2287,7 +2289,7 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
     V.lim = M.G.lim;
     int c[3];
 #pragma unroll
-    for (int cc = 0; cc < 3; cc++) c[cc] = (int)V.corner_vid(x, y, z, tri_entry(packed, 3 * j + cc));
+    for (int cc = 0; cc < 3; cc++) c[cc] = (int)V.corner_vid(x, y, z, tri_entry(packed, 3 * j + cc), ci);
     tri[3 * (size_t)t + 0] = c[0];
     tri[3 * (size_t)t + 1] = c[1];
     tri[3 * (size_t)t + 2] = c[2];
