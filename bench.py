@@ -74,6 +74,23 @@ def profiled_traffic(stage):
     return None
 
 
+def profiled_valu_rate(stage):
+    """VALU wave instructions per CU per cycle of a stage's kernel, from the committed instruction
+    counter summary (SQ_INSTS_VALU over GRBM_GUI_ACTIVE / 8 cycles and 256 CUs), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_instructions_ses_50k_0.2A.md")
+    if stage not in STAGE_KERNEL or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 8 and cells[0].strip("`").startswith(STAGE_KERNEL[stage]):
+                try:
+                    return float(cells[7])
+                except ValueError:
+                    return None
+    return None
+
+
 def copy_bandwidth(qs, hipmem, nbytes=1 << 30, reps=3):
     """GB/s (read + write) of the library's plain copy kernel over nbytes."""
     a, b = hipmem.alloc(nbytes), hipmem.alloc(nbytes)
@@ -454,6 +471,7 @@ def main():
     fp_ms = stage_avg["pass1"] + stage_avg["pass2"]
     roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 5), traffic=profiled_traffic(dom), avg_ms=round(kern[dom], 4),
+                    valu_wave_instr_per_cu_cycle=profiled_valu_rate(dom),
                     algorithmic_bytes=abytes, shell_voxels=S, copy_kernel_GBps=round(copy_gbs, 1),
                     frac_of_copy=round(achieved / copy_gbs, 5),
                     field_pass=dict(kernels="pass1 + pass2", algorithmic_bytes=fp_bytes, ms=round(fp_ms, 4),
