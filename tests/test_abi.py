@@ -122,3 +122,22 @@ def test_colors_without_a_mesh_is_an_error(qs):
     if L.qs_batch_offsets(None, None) >= 0:
         pytest.skip("a mesh exists in this process")
     assert L.qs_vertex_colors(None, 0, None, 0) != 0
+
+
+def test_host_fetches_without_a_batch_are_errors(qs):
+    """The library-owned host fetches check their state before any GPU work: no batch result,
+    a slot other than 0 or 1, a wait on a slot that no fetch started."""
+    import ctypes
+    import quickses_amd as q
+    L = q.lib()
+    if L.qs_batch_offsets(None, None) >= 0:
+        pytest.skip("a mesh exists in this process")
+    vp = ctypes.c_void_p
+    pv, pt, pa = vp(), vp(), vp()
+    nv, nt = ctypes.c_uint(0), ctypes.c_uint(0)
+    assert L.qs_batch_fetch_host(ctypes.byref(pv), ctypes.byref(pt), ctypes.byref(pa)) != 0
+    assert L.qs_batch_fetch_host_async(0) != 0
+    assert L.qs_batch_wait_host(2, ctypes.byref(pv), ctypes.byref(pt), ctypes.byref(pa), ctypes.byref(nv),
+                                ctypes.byref(nt)) != 0
+    assert L.qs_batch_wait_host(0, ctypes.byref(pv), ctypes.byref(pt), ctypes.byref(pa), ctypes.byref(nv),
+                                ctypes.byref(nt)) != 0
