@@ -177,7 +177,8 @@ int Engine::reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts
     const size_t m = (size_t)nmol + 1;
     QS_TRY(ensure(d_off, m * 8 + 64));
     QS_TRY(ensure(bb_out, m * 8 * sizeof(float)));
-    QS_TRY(ensure(bb_part, qs_bbox_part_floats((int)m) * sizeof(float)));
+    QS_TRY(ensure(bb_part, qs_bbox_part_floats((int)m, qs_bbox_parts((uint32_t)std::min<uint64_t>(atoms, UINT32_MAX))) *
+                               sizeof(float)));
     QS_TRY(ensure(bidx, m * 8));
     QS_TRY(ensure(bounds, m * 8));
     QS_TRY(ensure(d_mols, m * sizeof(QsMol)));
@@ -549,9 +550,12 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
         off[nmol] = in[nmol - 1].atom_off + in[nmol - 1].n;
         QS_TRY(ensure(d_off, ((size_t)nmol + 1) * 4 * 2 + 64));
         QS_TRY(ensure(bb_out, (size_t)nmol * 8 * sizeof(float)));
-        QS_TRY(ensure(bb_part, qs_bbox_part_floats(nmol) * sizeof(float)));
+        uint32_t max_n = 0;
+        for (int m = 0; m < nmol; m++) max_n = std::max(max_n, in[m].n);
+        const int parts = qs_bbox_parts(max_n);
+        QS_TRY(ensure(bb_part, qs_bbox_part_floats(nmol, parts) * sizeof(float)));
         QS_TRY(upload(d_off.p, off.data(), (nmol + 1) * 4));
-        qs_launch_bbox(d_pos, d_rad, as<uint32_t>(d_off), nmol, as<float>(bb_part), as<float>(bb_out), stream);
+        qs_launch_bbox(d_pos, d_rad, as<uint32_t>(d_off), nmol, parts, as<float>(bb_part), as<float>(bb_out), stream);
         std::vector<float> h((size_t)nmol * 8);
         float *hs = (float *)stage(h.size() * 4);
         QS_HIP(hipMemcpyAsync(hs ? (void *)hs : (void *)h.data(), bb_out.p, h.size() * 4, hipMemcpyDeviceToHost, stream));

@@ -10,8 +10,10 @@
 #include "qs_common.h"
 
 /* atoms into cells */
-size_t qs_bbox_part_floats(int nmol);
-void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, int nmol, float *part, float *out,
+/* bounding boxes: `parts` partial boxes per molecule (qs_bbox_parts of the largest atom count) */
+int qs_bbox_parts(uint32_t max_atoms);
+size_t qs_bbox_part_floats(int nmol, int parts);
+void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, int nmol, int parts, float *part, float *out,
                     hipStream_t s);
 void qs_launch_hash_count(QsBatch B, uint32_t nblocks, const float *pos, int *key, uint32_t *count, hipStream_t s);
 void qs_launch_scatter(const int *key, int n, const uint32_t *start, uint32_t *cursor, int *perm, hipStream_t s);

@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict
 
 #define BB_NT 256
 #define BB_NV 8 /* min x,y,z, max x,y,z, max radius, count of non-finite inputs */
-#define BB_P 32 /* partial boxes per molecule */
+#define BB_PMAX 256 /* partial boxes per molecule at most */
 
 __device__ __forceinline__ void bb_combine(float *v, const float *w)
 {
@@ -361,15 +361,17 @@ __device__ __forceinline__ void bb_combine(float *v, const float *w)
     v[7] += w[7];
 }
 
-/* block (p, m): atoms off[m] + p*BB_NT + k*BB_P*BB_NT + t of molecule m -> part[m][p] */
+/* block (p, m) of P partial boxes per molecule: atoms off[m] + p*BB_NT + k*P*BB_NT + t of molecule m
+   -> part[m][p] */
 __global__ void __launch_bounds__(BB_NT) k_bbox_part(const float *__restrict__ pos, const float *__restrict__ rad,
                                                      const uint32_t *__restrict__ off, float *__restrict__ part)
 {
+    const uint32_t P = gridDim.x;
     __shared__ float s[BB_NV][BB_NT];
     const int m = blockIdx.y, p = blockIdx.x;
     float v[BB_NV] = {100000.0f, 100000.0f, 100000.0f, -100000.0f, -100000.0f, -100000.0f, 0.0f, 0.0f};
     const uint32_t end = off[m + 1];
-    for (uint32_t i = off[m] + p * BB_NT + threadIdx.x; i < end; i += BB_P * BB_NT) {
+    for (uint32_t i = off[m] + p * BB_NT + threadIdx.x; i < end; i += P * BB_NT) {
         for (int c = 0; c < 3; c++) {
             float q = pos[3 * (size_t)i + c];
             v[c] = fminf(v[c], q);
@@ -394,17 +396,17 @@ __global__ void __launch_bounds__(BB_NT) k_bbox_part(const float *__restrict__ p
         }
         __syncthreads();
     }
-    if (threadIdx.x < BB_NV) part[((size_t)m * BB_P + p) * BB_NV + threadIdx.x] = s[threadIdx.x][0];
+    if (threadIdx.x < BB_NV) part[((size_t)m * P + p) * BB_NV + threadIdx.x] = s[threadIdx.x][0];
 }
 
-/* one wavefront per molecule: combine its BB_P partial boxes */
-__global__ void __launch_bounds__(64) k_bbox_final(const float *__restrict__ part, float *__restrict__ out)
+/* one wavefront per molecule: combine its P partial boxes */
+__global__ void __launch_bounds__(64) k_bbox_final(const float *__restrict__ part, int P, float *__restrict__ out)
 {
     const int m = blockIdx.x, lane = threadIdx.x;
     if (lane >= BB_NV) return;
-    float acc = part[(size_t)m * BB_P * BB_NV + lane];
-    for (int p = 1; p < BB_P; p++) {
-        float w = part[((size_t)m * BB_P + p) * BB_NV + lane];
+    float acc = part[(size_t)m * P * BB_NV + lane];
+    for (int p = 1; p < P; p++) {
+        float w = part[((size_t)m * P + p) * BB_NV + lane];
         acc = (lane < 3) ? fminf(acc, w) : (lane < 7 ? fmaxf(acc, w) : acc + w);
     }
     out[BB_NV * m + lane] = acc;
@@ -2641,13 +2643,20 @@ __global__ void __launch_bounds__(256) k_tri_localize(const int *__restrict__ tr
 
 static inline unsigned nblk(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-size_t qs_bbox_part_floats(int nmol) { return (size_t)nmol * BB_P * BB_NV; }
+/* partial boxes per molecule: about 8 atoms per thread of the largest molecule, 1 to BB_PMAX */
+int qs_bbox_parts(uint32_t max_atoms)
+{
+    const uint32_t p = (max_atoms + 8 * BB_NT - 1) / (8 * BB_NT);
+    return (int)std::min<uint32_t>(std::max<uint32_t>(p, 1u), BB_PMAX);
+}
 
-void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, int nmol, float *part, float *out,
+size_t qs_bbox_part_floats(int nmol, int parts) { return (size_t)nmol * parts * BB_NV; }
+
+void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, int nmol, int parts, float *part, float *out,
                     hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bbox_part, dim3(BB_P, nmol), dim3(BB_NT), 0, s, pos, rad, off, part);
-    hipLaunchKernelGGL(k_bbox_final, dim3(nmol), dim3(64), 0, s, part, out);
+    hipLaunchKernelGGL(k_bbox_part, dim3(parts, nmol), dim3(BB_NT), 0, s, pos, rad, off, part);
+    hipLaunchKernelGGL(k_bbox_final, dim3(nmol), dim3(64), 0, s, part, parts, out);
 }
 
 void qs_launch_hash_count(QsBatch B, uint32_t nblocks, const float *pos, int *key, uint32_t *count, hipStream_t s)
