@@ -434,7 +434,8 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     std::vector<uint32_t> hb(2 * (size_t)nmol + 3);
     uint32_t *hbs = (uint32_t *)stage(hb.size() * 4);
     uint32_t *hbd = hbs ? hbs : hb.data();
-    if (nmol && nw_total) {
+    /* (one molecule: it starts at word 0, where the exclusive scans are 0) */
+    if (nmol > 1 && nw_total) {
         QS_TRY(upload(bidx.p, idx.data(), (size_t)nmol * 8));
         qs_launch_gather_u32(as<uint32_t>(vbw), as<uint64_t>(bidx), nmol, as<uint32_t>(bounds), stream);
         qs_launch_gather_u32(as<uint32_t>(tbw), as<uint64_t>(bidx), nmol, as<uint32_t>(bounds) + nmol, stream);
@@ -443,6 +444,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_HIP(hipMemcpyAsync(hbd + 2 * (size_t)nmol, totals.p, 12, hipMemcpyDeviceToHost, stream));
     QS_HIP(hipStreamSynchronize(stream));
     if (hbs) memcpy(hb.data(), hbs, hb.size() * 4);
+    if (nmol == 1) hb[0] = hb[1] = 0u;
     tm.mark("mc_count");
     nt_all = hb[2 * (size_t)nmol];
     nv = hb[2 * (size_t)nmol + 1];
