@@ -110,35 +110,35 @@ struct StageTimer {
         if (!on) return;
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return;
-        hipEventRecord(e, s);
+        (void)hipEventRecord(e, s);
         ev.emplace_back(name, e);
     }
     void report(const QsGrid &G, unsigned n)
     {
         if (!on || ev.size() < 2) return;
-        hipEventSynchronize(ev.back().second);
+        (void)hipEventSynchronize(ev.back().second);
         g_stage_ms.clear();
         for (size_t i = 1; i < ev.size(); i++) {
             float ms = 0;
-            hipEventElapsedTime(&ms, ev[i - 1].second, ev[i].second);
+            (void)hipEventElapsedTime(&ms, ev[i - 1].second, ev[i].second);
             g_stage_ms.emplace_back(ev[i].first, ms);
         }
         if (!g_opt.trace) {
-            for (auto &p : ev) hipEventDestroy(p.second);
+            for (auto &p : ev) (void)hipEventDestroy(p.second);
             ev.clear();
             return;
         }
         std::string line = "{\"qs_trace\": {\"n_atoms\": " + std::to_string(n) + ", \"g\": " + std::to_string(G.g);
         for (size_t i = 1; i < ev.size(); i++) {
             float ms = 0;
-            hipEventElapsedTime(&ms, ev[i - 1].second, ev[i].second);
+            (void)hipEventElapsedTime(&ms, ev[i - 1].second, ev[i].second);
             char buf[96];
             snprintf(buf, sizeof(buf), ", \"%s_ms\": %.4f", ev[i].first.c_str(), ms);
             line += buf;
         }
         line += "}}";
         fprintf(stderr, "%s\n", line.c_str());
-        for (auto &p : ev) hipEventDestroy(p.second);
+        for (auto &p : ev) (void)hipEventDestroy(p.second);
         ev.clear();
     }
 };
@@ -266,16 +266,16 @@ struct Engine {
         }
         tables.clear();
         groups.clear();
-        if (stream) hipStreamSynchronize(stream);
-        if (up) hipHostFree(up);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (up) (void)hipHostFree(up);
         up = nullptr;
         up_cap = up_used = up_need = 0;
-        if (stream) hipStreamDestroy(stream);
+        if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
-        if (side) hipStreamDestroy(side);
+        if (side) (void)hipStreamDestroy(side);
         side = nullptr;
-        if (ev_emit) hipEventDestroy(ev_emit);
-        if (ev_side) hipEventDestroy(ev_side);
+        if (ev_emit) (void)hipEventDestroy(ev_emit);
+        if (ev_side) (void)hipEventDestroy(ev_side);
         ev_emit = ev_side = nullptr;
         device = -1;
         batch_valid = false;

@@ -63,7 +63,7 @@ int ensure(DevBuf &b, size_t bytes)
 
 void release(DevBuf &b)
 {
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
 }
@@ -1206,8 +1206,8 @@ int qs_vertex_normals_device(float *d_out, float *ms)
     QS_HIP(hipEventSynchronize(b));
     float t = 0.0f;
     QS_HIP(hipEventElapsedTime(&t, a, b));
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     if (ms) *ms = t;
     return rc;
 }
@@ -1265,8 +1265,8 @@ int qs_copy_device(void *dst, const void *src, uint64_t bytes, float *ms)
     QS_HIP(hipEventSynchronize(b));
     float t = 0.0f;
     QS_HIP(hipEventElapsedTime(&t, a, b));
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     if (ms) *ms = t;
     return 0;
 }

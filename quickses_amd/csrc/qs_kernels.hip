@@ -385,7 +385,7 @@ __global__ void __launch_bounds__(BB_NT) k_bbox_part(const float *__restrict__ p
     for (int c = 0; c < BB_NV; c++) s[c][threadIdx.x] = v[c];
     __syncthreads();
     for (int st = BB_NT / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) {
+        if ((int)threadIdx.x < st) {
             float w[BB_NV];
             for (int c = 0; c < BB_NV; c++) {
                 v[c] = s[c][threadIdx.x];

@@ -216,7 +216,7 @@ int run_batch_requests(Server *S, Engine &E, std::vector<Request *> &batch)
 
 void worker_main(Server *S, int w)
 {
-    hipSetDevice(S->device);
+    if (hipSetDevice(S->device) != hipSuccess) (void)hipGetLastError(); /* the engine's first call reports it */
     g_grow_shift = 0; /* grow by doubling: a reallocation (hipFree) would stall the other workers */
     Engine &E = *S->engines[w];
     int init_rc = E.init();
