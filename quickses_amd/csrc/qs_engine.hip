@@ -602,8 +602,8 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     if (n) {
         qs_launch_scatter(as<int>(key), (int)n, as<uint32_t>(start), cursor, as<int>(perm), stream);
         qs_launch_cell_finalize(as<uint32_t>(start), as<uint32_t>(count), (int)ncell, as<int>(perm), as<int2>(cse), stream);
-        qs_launch_gather_atoms(as<int>(perm), d_pos, d_rad, (int)n, as<float4>(sx), stream);
-        qs_launch_atom_aux(batch(at_atoms, 0, nmol), nb_atoms, as<float4>(sx), as<float4>(aux), stream);
+        qs_launch_gather_aux(batch(at_atoms, 0, nmol), nb_atoms, as<int>(perm), d_pos, d_rad, as<float4>(sx), as<float4>(aux),
+                             stream);
     }
     QS_HIP(hipGetLastError());
     tm.mark("cells");

@@ -18,8 +18,9 @@ void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, int
 void qs_launch_hash_count(QsBatch B, uint32_t nblocks, const float *pos, int *key, uint32_t *count, hipStream_t s);
 void qs_launch_scatter(const int *key, int n, const uint32_t *start, uint32_t *cursor, int *perm, hipStream_t s);
 void qs_launch_cell_finalize(const uint32_t *start, const uint32_t *count, int ncell, int *perm, int2 *cse, hipStream_t s);
-void qs_launch_gather_atoms(const int *perm, const float *pos, const float *rad, int n, float4 *sx, hipStream_t s);
-void qs_launch_atom_aux(QsBatch B, uint32_t nblocks, const float4 *sx, float4 *aux, hipStream_t s);
+/* atoms in cell order (sx) with their exact thresholds and neighbour cells (aux) */
+void qs_launch_gather_aux(QsBatch B, uint32_t nblocks, const int *perm, const float *pos, const float *rad, float4 *sx,
+                          float4 *aux, hipStream_t s);
 
 /* field pass 1 */
 int qs_pass1_rows_per_wave(QsGrid G);

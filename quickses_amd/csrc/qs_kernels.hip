@@ -469,15 +469,6 @@ __global__ void __launch_bounds__(256) k_cell_finalize(const uint32_t *__restric
     cse[c] = make_int2((int)s, (int)(s + k));
 }
 
-__global__ void __launch_bounds__(256) k_gather_atoms(const int *__restrict__ perm, const float *__restrict__ pos,
-                                                      const float *__restrict__ rad, int n, float4 *__restrict__ sx)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    int id = perm[k];
-    sx[k] = make_float4(pos[3 * id], pos[3 * id + 1], pos[3 * id + 2], rad[id]);
-}
-
 /* ------------------------------------------------------------------------------------------ */
 /* field pass 1: probeIntersection + computeInOrOut Kernels.cu:344-475                         */
 /*
@@ -589,9 +580,12 @@ __device__ __forceinline__ void row_interval(const RowAtom &A, float ss, int zlo
     R = r;
 }
 
-/* per sorted atom, once: the exact float thresholds of the near (r + probe) and inside (r - dx)
-   tests, and its neighbour-cell y and z (packed) */
-__global__ void __launch_bounds__(256) k_atom_aux(QsBatch B, const float4 *__restrict__ sx, float4 *__restrict__ aux)
+/* per atom in cell order, once: the atom (perm gives its input index) as float4 x, y, z, r, the
+   exact float thresholds of its near (r + probe) and inside (r - dx) tests, and its neighbour-cell
+   y and z (packed) */
+__global__ void __launch_bounds__(256) k_gather_aux(QsBatch B, const int *__restrict__ perm, const float *__restrict__ pos,
+                                                    const float *__restrict__ rad, float4 *__restrict__ sx,
+                                                    float4 *__restrict__ aux)
 {
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
@@ -599,7 +593,9 @@ __global__ void __launch_bounds__(256) k_atom_aux(QsBatch B, const float4 *__res
     if (i >= M.n_atoms) return;
     const QsGrid G = M.G;
     const size_t k = (size_t)M.atom_off + i;
-    const float4 a = sx[k];
+    const int id = perm[k];
+    const float4 a = make_float4(pos[3 * (size_t)id], pos[3 * (size_t)id + 1], pos[3 * (size_t)id + 2], rad[id]);
+    sx[k] = a;
     const int acy = iclamp(cell_of(a.y, G.oy, G.dxn), 0, G.gn - 1);
     const int acz = iclamp(cell_of(a.z, G.oz, G.dxn), 0, G.gn - 1);
     aux[k] = make_float4(sqrt_threshold(QS_PROBE + a.w), sqrt_threshold(a.w - G.dx), __int_as_float((acy << 16) | acz), 0.0f);
@@ -2674,14 +2670,10 @@ void qs_launch_cell_finalize(const uint32_t *start, const uint32_t *count, int n
     hipLaunchKernelGGL(k_cell_finalize, dim3(nblk(ncell, 256)), dim3(256), 0, s, start, count, ncell, perm, cse);
 }
 
-void qs_launch_gather_atoms(const int *perm, const float *pos, const float *rad, int n, float4 *sx, hipStream_t s)
+void qs_launch_gather_aux(QsBatch B, uint32_t nblocks, const int *perm, const float *pos, const float *rad, float4 *sx,
+                          float4 *aux, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gather_atoms, dim3(nblk(n, 256)), dim3(256), 0, s, perm, pos, rad, n, sx);
-}
-
-void qs_launch_atom_aux(QsBatch B, uint32_t nblocks, const float4 *sx, float4 *aux, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_atom_aux, dim3(nblocks), dim3(256), 0, s, B, sx, aux);
+    hipLaunchKernelGGL(k_gather_aux, dim3(nblocks), dim3(256), 0, s, B, perm, pos, rad, sx, aux);
 }
 
 int qs_pass1_rows_per_wave(QsGrid G)
