@@ -2134,11 +2134,15 @@ __device__ __forceinline__ void item_word(const ItemWords &S, const uint32_t *__
 __device__ __forceinline__ float field_at(const float *__restrict__ F, const uint64_t *__restrict__ outm,
                                           const uint64_t *__restrict__ shellm, size_t row, int g, int W, int z, float dx)
 {
-    if (!shellm) return F[row * g + z];
+    /* the three loads are independent (F is allocated for every voxel; a non-shell voxel's stale
+       value is discarded) */
+    const float f = F[row * g + z];
+    if (!shellm) return f;
     const size_t wi = row * W + (z >> 6);
-    const uint64_t bit = 1ull << (z & 63);
-    if (shellm[wi] & bit) return F[row * g + z];
-    return (outm[wi] & bit) ? QS_PROBE : -dx;
+    const uint64_t sh = shellm[wi], ou = outm[wi];
+    const int b = z & 63;
+    if ((sh >> b) & 1ull) return f;
+    return ((ou >> b) & 1ull) ? QS_PROBE : -dx;
 }
 
 /* every voxel of one molecule's field that field pass 2 leaves unstored, one thread per mask word */
