@@ -1420,7 +1420,8 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     /* norms whose minimisers are traced back (bit n), from nf */
     __shared__ uint32_t nfw[8];
     /* y offsets (bit b + R) attaining each DY value */
-    using BMask = typename std::conditional<(2 * R + 1 <= 16), uint16_t, uint32_t>::type;
+    using BMask = typename std::conditional<(2 * R + 1 <= 8), uint8_t,
+                                            typename std::conditional<(2 * R + 1 <= 16), uint16_t, uint32_t>::type>::type;
     __shared__ BMask BM[LX * P2_TY * P2_TZ];
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, xcd_block(blockIdx.x, gridDim.x), lb)];
