@@ -2086,11 +2086,11 @@ struct ItemWords {
 
 __device__ __forceinline__ void stage_item_words(ItemWords &S, const uint32_t *__restrict__ base,
                                                  const uint32_t *__restrict__ list, const uint32_t *__restrict__ start,
-                                                 uint32_t nact, uint32_t nblocks, const uint64_t *__restrict__ wpre,
+                                                 uint32_t nact, uint32_t nblocks, uint32_t blk, const uint64_t *__restrict__ wpre,
                                                  const uint32_t *__restrict__ wcoarse)
 {
-    const uint32_t k0 = start[blockIdx.x];
-    const uint32_t k1 = (blockIdx.x + 1 < nblocks) ? start[blockIdx.x + 1] : nact - 1u;
+    const uint32_t k0 = start[blk];
+    const uint32_t k1 = (blk + 1 < nblocks) ? start[blk + 1] : nact - 1u;
     const uint32_t nk = k1 - k0 + 1u;
     if (threadIdx.x == 0) {
         S.k0 = k0;
@@ -2178,8 +2178,9 @@ __global__ void __launch_bounds__(256) k_mc_vertices(const QsMol *__restrict__ m
                                                      uint64_t *__restrict__ vkey)
 {
     __shared__ ItemWords S;
-    stage_item_words(S, avb, list, vstart, nact, gridDim.x, wpre, wcoarse);
-    const uint32_t v = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t blk = xcd_group<64>(blockIdx.x, gridDim.x); /* neighbouring rows' words share an L2 */
+    stage_item_words(S, avb, list, vstart, nact, gridDim.x, blk, wpre, wcoarse);
+    const uint32_t v = blk * 256u + threadIdx.x;
     if (v >= nv) return;
     uint32_t k, wi, first;
     item_word(S, avb, list, v, k, wi, first);
@@ -2255,8 +2256,9 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
                                                       uint32_t *__restrict__ deg_total)
 {
     __shared__ ItemWords S;
-    stage_item_words(S, atb, list, tstart, nact, gridDim.x, wpre, wcoarse);
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t blk = xcd_group<64>(blockIdx.x, gridDim.x); /* neighbouring rows' words share an L2 */
+    stage_item_words(S, atb, list, tstart, nact, gridDim.x, blk, wpre, wcoarse);
+    const uint32_t t = blk * 256u + threadIdx.x;
     if (t >= ntri) return;
     uint32_t k, wi, first;
     item_word(S, atb, list, t, k, wi, first);
@@ -2300,7 +2302,7 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
        (CudaSurf.cu:820-829): counted here per block and in total */
     const uint64_t dm = __ballot(c[0] == c[1] || c[1] == c[2] || c[0] == c[2]);
     if (dm && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(dm)) {
-        atomicAdd(&deg_blk[blockIdx.x], (uint32_t)__popcll(dm));
+        atomicAdd(&deg_blk[blk], (uint32_t)__popcll(dm));
         atomicAdd(deg_total, (uint32_t)__popcll(dm));
     }
 }
