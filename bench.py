@@ -106,49 +106,16 @@ def copy_bandwidth(qs, hipmem, nbytes=1 << 30, reps=3):
 def cpu_baseline(pos, rad, voxel, target_s=25.0):
     from oracle import oracle  # CPU baseline leg only
 
-    t, g, v = oracle.bench_slab(pos, rad, voxel, 4)
+    t, g, v, _ = oracle.bench_slab(pos, rad, voxel, 4)
     per_plane = t / 4.0
     planes = int(max(4, min(g - 1, target_s / max(per_plane, 1e-6))))
-    t, g, v = oracle.bench_slab(pos, rad, voxel, planes)
+    t, g, v, nt = oracle.bench_slab(pos, rad, voxel, planes)
     return dict(value=v / t, unit="voxels/s", cores=oracle.num_threads(), kind="port",
-                sample="CPU oracle (OpenMP, IEEE fp32): grid setup, hashing, cell order, field pass 1 and "
-                       "brute-force pass 2 and cube classification on the middle %d of %d x-planes of the same "
-                       "molecule (%d voxels, %.1f s)" % (planes, g, v, t))
-
-
-def estimated_voxels(pos, rad, voxel):
-    """g^3 of the molecule's grid by the grid rule (extent + 2 max radius + 4 probe)."""
-    import numpy as np
-
-    ext = float(np.max(pos.max(axis=0) - pos.min(axis=0)))
-    g = int(np.ceil((ext + 2.0 * float(rad.max()) + 4 * 1.4) / voxel))
-    return g ** 3
-
-
-def chunks_by_voxels(mols, voxel, budget):
-    """Consecutive groups of molecules whose grids together stay under `budget` voxels."""
-    out, cur, acc = [], [], 0
-    for pr in mols:
-        v = estimated_voxels(pr[0], pr[1], voxel)
-        if cur and acc + v > budget:
-            out.append(cur)
-            cur, acc = [], 0
-        cur.append(pr)
-        acc += v
-    if cur:
-        out.append(cur)
-    return out
-
-
-def config4_sets(qs, n_mol=256, n_frames=1000):
-    """256 molecules (sizes log-uniform 500-20,000, size seed 4, molecule m seed 4000+m) and
-    n_frames trajectory frames of one 10,000-atom molecule (seed 4500; frame f jittered by
-    N(0, 0.3 A) per coordinate with seed 5000+f)."""
-    sizes = qs.synth_sizes(4, n_mol, 500, 20000)
-    mols = [qs.synth_ball(4000 + m, int(n)) for m, n in enumerate(sizes)]
-    base_pos, base_rad = qs.synth_ball(4500, 10000)
-    frames = [(qs.synth_jitter(5000 + f, base_pos, 0.3), base_rad) for f in range(n_frames)]
-    return mols, frames
+                sample="CPU oracle (OpenMP for the field passes and closest atom, IEEE fp32), the reference's whole "
+                       "path on one slice as its slice loop runs it: grid setup, hashing, cell order, field passes 1 "
+                       "and 2 (brute-force window), triangle soup, 0.001 A quantised weld, closest atom, one in-place "
+                       "Laplacian pass, degenerate filter, on the middle %d of %d x-planes of the same molecule "
+                       "(%d voxels, %d triangles, %.1f s)" % (planes, g, v, nt, t))
 
 
 def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_000_000):
@@ -161,6 +128,8 @@ def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_0
     host arrays overlapping the next sequence's work).  Each leg runs once untimed first, so that
     the library's grow-only device and host arrays have their steady-state sizes."""
     import numpy as np
+
+    from quickses_amd.workloads import chunks_by_voxels, config4_sets, estimated_voxels
 
     mols, frames = config4_sets(qs, 256, n_frames)
     res = {}
@@ -272,17 +241,6 @@ def bench_other_plain(qs, hipmem, reps=5):
     return out
 
 
-def config5_requests(qs, k):
-    """k surface requests: atom counts log-uniform 1,000-200,000 (size seed 5, request i seed
-    50000+i), voxel size uniform over {0.3, 0.5, 1.0} A, one Laplacian pass."""
-    import numpy as np
-
-    sizes = qs.synth_sizes(5, k, 1000, 200000)
-    rng = np.random.default_rng(5)
-    voxels = rng.choice([0.3, 0.5, 1.0], size=k)
-    return [(qs.synth_ball(50000 + i, int(n)), float(v)) for i, (n, v) in enumerate(zip(sizes, voxels))]
-
-
 def _pct(xs, q):
     import numpy as np
 
@@ -298,6 +256,8 @@ def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=
     import threading
 
     import numpy as np
+
+    from quickses_amd.workloads import config5_requests
 
     reqs = config5_requests(qs, k)
     smooth = 1

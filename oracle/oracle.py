@@ -66,7 +66,7 @@ def lib():
         _lib.or_mesh_free.argtypes = [ctypes.POINTER(_Mesh)]
         _lib.or_num_threads.restype = ctypes.c_int
         _lib.or_bench_slab.argtypes = [vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
-                                       ctypes.POINTER(ctypes.c_int64)]
+                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
         _lib.or_bench_slab.restype = ctypes.c_double
     return _lib
 
@@ -245,11 +245,14 @@ def num_threads():
 
 
 def bench_slab(pos, rad, reso, planes):
-    """Seconds for setup + pass 1 + pass 2 + cube classification on `planes` middle x-planes;
-    returns (seconds, g, voxels)."""
+    """Seconds for the whole path (setup, field passes, soup, quantised weld, closest atom, one
+    in-place Laplacian pass, filter) on a slice of `planes` middle x-planes; returns
+    (seconds, g, voxels, triangles kept)."""
     pos = np.ascontiguousarray(pos, np.float32)
     rad = np.ascontiguousarray(rad, np.float32)
     g = ctypes.c_int(0)
     v = ctypes.c_int64(0)
-    t = lib().or_bench_slab(_p(pos), _p(rad), len(rad), ctypes.c_float(reso), int(planes), ctypes.byref(g), ctypes.byref(v))
-    return t, g.value, v.value
+    nt = ctypes.c_int64(0)
+    t = lib().or_bench_slab(_p(pos), _p(rad), len(rad), ctypes.c_float(reso), int(planes), ctypes.byref(g), ctypes.byref(v),
+                            ctypes.byref(nt))
+    return t, g.value, v.value, nt.value

@@ -826,8 +826,57 @@ static double now_s(void)
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
-double or_bench_slab(const float *pos, const float *rad, int n, float reso, int planes, int *g_out, int64_t *voxels)
+/* The reference's triangle soup (generateTriangleVerticesSMEM MarchingCubes.cu:736-851 order) of
+   the cubes with origin x in [x0, x0 + planes) (and y, z < lim) from a field holding planes
+   x0 .. x0 + planes (local plane i = global x0 + i). */
+static int64_t slab_soup(const or_grid *G, int lim, const float *F, int x0, int planes, float *soup)
 {
+    const int g = G->g;
+    const float dx = G->reso;
+    int64_t w = 0;
+    for (int i = x0; i < x0 + planes && i < lim; i++)
+        for (int j = 0; j < lim; j++)
+            for (int k = 0; k < lim; k++) {
+                float f[8];
+                int ci = 0;
+                for (int c = 0; c < 8; c++) {
+                    f[c] = FIELD_AT(F, g, i - x0 + CORNER_OFF[c][0], j + CORNER_OFF[c][1], k + CORNER_OFF[c][2]);
+                    ci += ((unsigned)(f[c] < 0.0f)) << c;
+                }
+                if (ci == 0 || ci == 255) continue;
+                const int ntri = tri_count(ci);
+                if (!soup) {
+                    w += 3 * ntri;
+                    continue;
+                }
+                float p[3] = {grid_pos(G->origin[0], i, dx), grid_pos(G->origin[1], j, dx), grid_pos(G->origin[2], k, dx)};
+                float v[8][3];
+                for (int c = 0; c < 8; c++)
+                    for (int a = 0; a < 3; a++) v[c][a] = p[a] + (CORNER_OFF[c][a] ? dx : 0.0f);
+                for (int tt = 0; tt < 3 * ntri; tt++) {
+                    int e = tri_entry(ci, tt);
+                    int c0 = EDGE_CORNERS[e][0], c1 = EDGE_CORNERS[e][1];
+                    float t = (0.0f - f[c0]) / (f[c1] - f[c0]);
+                    for (int a = 0; a < 3; a++) {
+                        float dd = v[c1][a] - v[c0][a];
+                        float s = t * dd;
+                        soup[3 * w + a] = v[c0][a] + s;
+                    }
+                    w++;
+                }
+            }
+    return w;
+}
+
+double or_bench_slab(const float *pos, const float *rad, int n, float reso, int planes, int *g_out, int64_t *voxels,
+                     int64_t *n_tri)
+{
+    /* The reference's whole path on one slice of `planes` middle x-planes, as its slice loop runs
+       it (CudaSurf.cu:653-754, then API_computeSES's filter 820-829): grid setup, hashing and cell
+       order (594-622), field passes 1 and 2 over the slice and its window halo, the triangle soup
+       of the slice's cubes, the quantised weld (428-458), the closest atom per welded vertex
+       (460-467), one in-place Laplacian pass (SmoothMesh.cpp:96-124) and the degenerate-triangle
+       filter. */
     double t0 = now_s();
     or_grid G;
     if (or_grid_setup(pos, rad, n, reso, &G)) return -1.0;
@@ -848,23 +897,24 @@ double or_bench_slab(const float *pos, const float *rad, int n, float reso, int 
     float *fld = (float *)malloc((size_t)(planes + 1) * plane * sizeof(float));
     or_pass1_range(&G, cs, ce, sx, lx0, lx1, lab);
     or_pass2_range(&G, lab, lx0, lx1, x0, x1 + 1, fld);
-    /* cube indices of planes [x0, x1): corners from fld planes 0..planes */
-    int64_t active = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : active)
-    for (int i = 0; i < planes; i++)
-        for (int j = 0; j < g - 1; j++)
-            for (int k = 0; k < g - 1; k++) {
-                int ci = 0;
-                for (int c = 0; c < 8; c++) {
-                    float f = fld[((int64_t)(i + CORNER_OFF[c][0]) * g + j + CORNER_OFF[c][1]) * g + k + CORNER_OFF[c][2]];
-                    ci += ((unsigned)(f < 0.0f)) << c;
-                }
-                active += (ci != 0 && ci != 255);
-            }
+    free(lab);
+    const int lim = g - 1;
+    const int64_t ns = slab_soup(&G, lim, fld, x0, planes, NULL);
+    float *soup = (float *)malloc((size_t)(ns ? ns : 1) * 3 * sizeof(float));
+    slab_soup(&G, lim, fld, x0, planes, soup);
+    free(fld);
+    or_mesh m;
+    or_quantised_weld(soup, ns, &m);
+    free(soup);
+    m.atom_id = (int *)malloc((size_t)(m.n_vert ? m.n_vert : 1) * sizeof(int));
+    or_closest_atom(&G, cs, ce, sx, m.vert, m.n_vert, m.atom_id);
+    or_smooth(m.vert, m.n_vert, m.tri, m.n_tri, 1, 1);
+    const int64_t kept = or_filter_degenerate(m.tri, m.n_tri);
     double t1 = now_s();
-    free(hash); free(perm); free(cs); free(ce); free(sx); free(lab); free(fld);
+    or_mesh_free(&m);
+    free(hash); free(perm); free(cs); free(ce); free(sx);
     if (g_out) *g_out = g;
     if (voxels) *voxels = (int64_t)planes * plane;
-    (void)active;
+    if (n_tri) *n_tri = kept;
     return t1 - t0;
 }

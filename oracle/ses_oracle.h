@@ -76,8 +76,10 @@ void or_pass2(const or_grid *G, const uint8_t *labels, float *field);
 /* the same on x-plane ranges (slab-local arrays) */
 void or_pass1_range(const or_grid *G, const int *cs, const int *ce, const float *sx, int x0, int x1, uint8_t *labels);
 void or_pass2_range(const or_grid *G, const uint8_t *labels, int lx0, int lx1, int x0, int x1, float *field);
-/* CPU timing sample on `planes` middle x-planes; returns seconds */
-double or_bench_slab(const float *pos, const float *rad, int n, float reso, int planes, int *g_out, int64_t *voxels);
+/* CPU timing sample: the whole path (field, soup, quantised weld, closest atom, one in-place
+   Laplacian pass, filter) on a slice of `planes` middle x-planes; returns seconds */
+double or_bench_slab(const float *pos, const float *rad, int n, float reso, int planes, int *g_out, int64_t *voxels,
+                     int64_t *n_tri);
 
 /* cube index per cube origin (g^3 entries, 0 outside [0,lim)^3) */
 void or_mc_classify(int g, int lim, const float *field, uint8_t *cubeidx);

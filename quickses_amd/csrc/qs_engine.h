@@ -247,6 +247,8 @@ struct Engine {
     bool batch_valid = false; /* result arrays, vert_off / tri_off and their device copies belong to run_batch */
     /* field holds the shell voxels' values only (field pass 2); the others come from outm / shellm */
     bool field_sparse = false;
+    /* the last run's atom ids are cell-sorted positions (option atom_ids_sorted at run time) */
+    bool ids_sorted = false;
     float *res_vert = nullptr;
     int *res_tri = nullptr, *res_atom = nullptr;
 

@@ -508,8 +508,9 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
             QS_HIP(hipEventRecord(ev_emit, stream));
             QS_HIP(hipStreamWaitEvent(side, ev_emit, 0));
         }
+        ids_sorted = g_opt.atom_ids_sorted != 0;
         qs_launch_closest_atom(as<QsMol>(d_mols), d_vpre, d_vpre + nmol + 1, as<uint32_t>(start), as<float4>(sx), as<int>(perm), as<float>(vert),
-                               nv, g_opt.atom_ids_sorted, as<int>(atom), cs);
+                               nv, ids_sorted ? 1 : 0, as<int>(atom), cs);
         QS_HIP(hipGetLastError());
         if (overlap) {
             QS_HIP(hipEventRecord(ev_side, side));
@@ -754,12 +755,61 @@ HostSlot g_slot[2];
 hipStream_t g_copy = nullptr;
 int g_copy_dev = -1;
 
+/* pinned host array of any size (ordinary memory only if pinning fails): a device-to-host copy
+   into pageable memory would run synchronously, and the pipelined fetch must return at once */
+HostArray host_alloc_pinned(size_t bytes)
+{
+    HostArray a;
+    if (hipHostMalloc(&a.p, bytes, hipHostMallocDefault) == hipSuccess) {
+        a.pinned = true;
+    } else {
+        (void)hipGetLastError();
+        a.p = malloc(bytes);
+    }
+    a.cap = a.p ? bytes : 0;
+    return a;
+}
+
 void grow_host(HostArray &a, size_t bytes)
 {
     if (a.cap < bytes) {
         host_free(a);
-        a = host_alloc(bytes);
+        a = host_alloc_pinned(bytes);
     }
+}
+
+/* the pipelined fetch's copy stream, events and slot buffers for the current device: when the
+   device changed, the old slots' copies are waited for and their stream, events and device
+   buffers released on the device that owns them */
+int copy_slots_for(int dev)
+{
+    if (g_copy && g_copy_dev == dev) return 0;
+    if (g_copy) {
+        int cur = 0;
+        QS_HIP(hipGetDevice(&cur));
+        QS_HIP(hipSetDevice(g_copy_dev));
+        for (HostSlot &S : g_slot) {
+            if (S.pending && S.done) (void)hipEventSynchronize(S.done);
+            S.pending = S.ready = false;
+            if (S.staged) (void)hipEventDestroy(S.staged);
+            if (S.done) (void)hipEventDestroy(S.done);
+            S.staged = S.done = nullptr;
+            release(S.v);
+            release(S.t);
+            release(S.a);
+        }
+        (void)hipStreamDestroy(g_copy);
+        g_copy = nullptr;
+        g_copy_dev = -1;
+        QS_HIP(hipSetDevice(cur));
+    }
+    QS_HIP(hipStreamCreateWithFlags(&g_copy, hipStreamNonBlocking));
+    for (HostSlot &S : g_slot) {
+        QS_HIP(hipEventCreateWithFlags(&S.staged, hipEventDisableTiming));
+        QS_HIP(hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
+    }
+    g_copy_dev = dev;
+    return 0;
 }
 
 void *batch_host_array(int k, size_t bytes)
@@ -767,7 +817,7 @@ void *batch_host_array(int k, size_t bytes)
     HostArray &a = g_batch_host[k];
     if (a.cap < bytes) {
         host_free(a);
-        a = host_alloc(bytes);
+        a = host_alloc_pinned(bytes);
     }
     return a.p;
 }
@@ -1101,14 +1151,7 @@ int qs_batch_fetch_host_async(int slot)
     Engine &E = g_eng;
     if (!E.batch_valid) return set_error(3, "no batch result");
     if (slot < 0 || slot > 1) return set_error(3, "slot must be 0 or 1");
-    if (g_copy_dev != E.device || !g_copy) {
-        QS_HIP(hipStreamCreateWithFlags(&g_copy, hipStreamNonBlocking));
-        for (HostSlot &S : g_slot) {
-            QS_HIP(hipEventCreateWithFlags(&S.staged, hipEventDisableTiming));
-            QS_HIP(hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
-        }
-        g_copy_dev = E.device;
-    }
+    QS_TRY(copy_slots_for(E.device));
     HostSlot &S = g_slot[slot];
     if (S.pending) QS_HIP(hipEventSynchronize(S.done)); /* its previous copies still read the slot */
     S.pending = S.ready = false;
@@ -1242,8 +1285,8 @@ int qs_vertex_colors(const float *atom_rgba, unsigned int natoms, float *out, un
     QS_HIP(hipMemcpyAsync(E.colors_in.p, atom_rgba, (size_t)natoms * 16, hipMemcpyHostToDevice, E.stream));
     const int nmol = (int)E.mols.size();
     const uint32_t *d_vpre = as<uint32_t>(E.d_off) + (nmol + 1);
-    qs_launch_vertex_colors(as<QsMol>(E.d_mols), d_vpre, d_vpre + nmol + 1, E.res_atom, E.nv, as<float4>(E.colors_in),
-                            as<float4>(E.normals), E.stream);
+    qs_launch_vertex_colors(as<QsMol>(E.d_mols), d_vpre, d_vpre + nmol + 1, E.res_atom, E.ids_sorted ? as<int>(E.perm) : nullptr,
+                            E.nv, as<float4>(E.colors_in), as<float4>(E.normals), E.stream);
     QS_HIP(hipGetLastError());
     QS_HIP(hipMemcpyAsync(out, E.normals.p, (size_t)E.nv * 16, hipMemcpyDeviceToHost, E.stream));
     QS_HIP(hipStreamSynchronize(E.stream));

@@ -83,9 +83,10 @@ void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *v
    the vertex kernel) */
 void qs_launch_vertex_normals(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *vkey, uint32_t nv,
                               const float *F, const uint64_t *outm, const uint64_t *shellm, float *out, hipStream_t s);
-/* colour of each vertex's closest atom from a per-atom RGBA table (batch atom layout) */
-void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const int *atom, uint32_t nv,
-                             const float4 *table, float4 *out, hipStream_t s);
+/* colour of each vertex's closest atom from a per-atom RGBA table (batch atom layout); perm maps
+   cell-sorted atom ids to input indices (nullptr: the ids are input indices) */
+void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const int *atom,
+                             const int *perm, uint32_t nv, const float4 *table, float4 *out, hipStream_t s);
 /* degenerate-triangle filter: dpre = exclusive prefix of the per-256-triangle-block degenerate
    counts that the triangle kernel leaves, ndeg their total (device); with ndeg == 0 the compaction
    writes nothing */

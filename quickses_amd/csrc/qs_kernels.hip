@@ -2569,16 +2569,24 @@ __global__ void __launch_bounds__(256) k_vertex_normals(const QsMol *__restrict_
 
 /* Per-vertex colours (the colors field that MeshData leaves commented out, SmoothMesh.h:35): the
    colour of the vertex's closest atom, from a per-atom RGBA table laid out like the atom input
-   (a batch's molecules concatenated); a vertex without a closest atom gets (0, 0, 0, 0). */
+   (a batch's molecules concatenated); a vertex without a closest atom gets (0, 0, 0, 0).  When the
+   ids are in the reference's cell-sorted convention (perm != nullptr), the id is mapped back to
+   the input order through perm (sorted position -> input index, batch-global) first. */
 __global__ void __launch_bounds__(256) k_vertex_colors(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
                                                        const uint32_t *__restrict__ vcoarse, const int *__restrict__ atom,
-                                                       uint32_t nv, const float4 *__restrict__ table, float4 *__restrict__ out)
+                                                       const int *__restrict__ perm, uint32_t nv,
+                                                       const float4 *__restrict__ table, float4 *__restrict__ out)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
     const int a = atom[v];
-    out[v] = (a >= 0) ? table[(size_t)M.atom_off + (uint32_t)a] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (a < 0) {
+        out[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const uint32_t in = perm ? (uint32_t)perm[M.atom_off + (uint32_t)a] : M.atom_off + (uint32_t)a;
+    out[v] = table[in];
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -2836,11 +2844,12 @@ void qs_launch_vertex_normals(const QsMol *mols, const uint32_t *vpre, const uin
                        out);
 }
 
-void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const int *atom, uint32_t nv,
-                             const float4 *table, float4 *out, hipStream_t s)
+void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const int *atom,
+                             const int *perm, uint32_t nv, const float4 *table, float4 *out, hipStream_t s)
 {
     if (!nv) return;
-    hipLaunchKernelGGL(k_vertex_colors, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, atom, nv, table, out);
+    hipLaunchKernelGGL(k_vertex_colors, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, atom, perm, nv, table,
+                       out);
 }
 
 void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint32_t *cstart,

@@ -76,3 +76,27 @@ def test_colors_are_the_closest_atoms_colours(qs):
         assert np.array_equal(col[vo[m]:vo[m + 1]], tabs[m][A])
     with pytest.raises(qs.QuickSESError):
         qs.vertex_colors(np.concatenate(tabs)[:-1], vo[-1])
+
+
+def test_colors_with_sorted_atom_ids(qs):
+    """With the reference's cell-sorted atom-id convention (atom_ids_sorted=1) the colours are
+    still those of the closest atom in input order: the same colours as in the default
+    convention, plain and batched."""
+    mols = [qs.synth_ball(45 + i, n) for i, n in enumerate([500, 800])]
+    rng = np.random.default_rng(4)
+    tabs = [rng.random((len(r), 4)).astype(np.float32) for _, r in mols]
+    ref = [tab[qs.compute_ses(pos, rad, 0.5, 1)[2]] for (pos, rad), tab in zip(mols, tabs)]
+    old = qs.get_option("atom_ids_sorted")
+    qs.set_option("atom_ids_sorted", 1)
+    try:
+        print("case colours, sorted ids", flush=True)
+        for (pos, rad), tab, want in zip(mols, tabs, ref):
+            V, T, A = qs.compute_ses(pos, rad, 0.5, 1)
+            assert np.array_equal(qs.vertex_colors(tab, len(V)), want)
+        qs.compute_batch(mols, 0.5, 1)
+        vo, _ = qs.batch_offsets()
+        col = qs.vertex_colors(np.concatenate(tabs), vo[-1])
+    finally:
+        qs.set_option("atom_ids_sorted", old)
+    for m in range(len(mols)):
+        assert np.array_equal(col[vo[m]:vo[m + 1]], ref[m])
