@@ -68,6 +68,12 @@ const char *qs_last_error_message(void);
  *   "pass2_direct"    1 = evaluate field pass 2 with the (2r+1)^3 window kernel (cross-check)
  *   "trace"           1 = print per-stage GPU times as JSON lines on stderr
  *   "timing"          1 = record per-stage GPU times (hipEvents on the library's stream)
+ *   "slab_planes"     N > 0 = mesh a single molecule in x-slabs of N planes (the reference's
+ *                     slices, CudaSurf.cu:626-754, with disjoint cube ownership: the mesh is the
+ *                     whole-grid mesh bit for bit); 0 = slabs only when the grid does not fit the
+ *                     memory budget or has 2^32 points or more (default)
+ *   "slab_mb"         device-memory budget of one slab's grid arrays in MiB; 0 = half the
+ *                     device's free memory (default)
  * Returns 0, or -1 for an unknown key. */
 int qs_set_option(const char *key, int value);
 int qs_get_option(const char *key);
@@ -149,6 +155,10 @@ int qs_stage_times(char *names, int names_cap, float *ms, int max_stages);
  * fparams = {origin x,y,z, dx, dxn}, iparams = {g, gn, r, lim}.  Returns 0, or -1 for no such
  * molecule. */
 int qs_last_grid(float *fparams, int *iparams);
+/* Number of x-slabs the last computation's grid ran in (1: the whole grid at once; 0: no
+ * result).  After a run in several slabs the field, labels and normals of the mesh are not
+ * available (qs_vertex_normals returns an error). */
+int qs_last_slab_count(void);
 int qs_batch_grid(int m, float *fparams, int *iparams);
 
 /* Copies of the last computation's intermediate arrays to host memory (stage-wise checks; for

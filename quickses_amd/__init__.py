@@ -72,6 +72,8 @@ def lib():
     L.qs_copy_device.argtypes = [vp, vp, ctypes.c_uint64, f3p]
     L.qs_copy_device.restype = ctypes.c_int
     L.qs_last_grid.argtypes = [f3p, ctypes.POINTER(ctypes.c_int)]
+    L.qs_last_slab_count.argtypes = []
+    L.qs_last_slab_count.restype = ctypes.c_int
     L.qs_batch_grid.argtypes = [ctypes.c_int, f3p, ctypes.POINTER(ctypes.c_int)]
     L.qs_batch_grid.restype = ctypes.c_int
     L.qs_stage_times.argtypes = [ctypes.c_char_p, ctypes.c_int, f3p, ctypes.c_int]
@@ -528,6 +530,11 @@ def last_grid():
     ip = (ctypes.c_int * 4)()
     L.qs_last_grid(fp, ip)
     return dict(origin=(fp[0], fp[1], fp[2]), dx=fp[3], dxn=fp[4], g=ip[0], gn=ip[1], r=ip[2], lim=ip[3])
+
+
+def last_slab_count():
+    """x-slabs the last computation's grid ran in (1: the whole grid at once)."""
+    return int(lib().qs_last_slab_count())
 
 
 def fetch(what, dtype, count):

@@ -41,6 +41,12 @@ struct QsMol {
     uint32_t tri_off, n_tri;    /* triangles (set after counting) */
     int rows_per_wave;          /* field pass 1, row kernel: rows per wavefront */
     int slab_rows;              /* field pass 1, slab kernel: y rows per block */
+    /* x planes each stage produces (the whole grid, [0, g), unless the grid runs in x-slabs):
+       field pass 1 labels [p1_lo, p1_hi), field pass 2 evaluates [p2_lo, p2_hi), the
+       marching-cubes count covers grid points [mc_lo, mc_hi), and cubes with origin
+       x < cube_hi are polygonised.  word_off and vox_off address global plane x, so a slab
+       whose arrays start at plane p1_lo has offsets that wrap below zero. */
+    int p1_lo, p1_hi, p2_lo, p2_hi, mc_lo, mc_hi, cube_hi;
 };
 
 /* Coarse indexes of prefix arrays: coarse[k] = the segment that holds item k << shift, with

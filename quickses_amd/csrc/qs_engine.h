@@ -51,6 +51,10 @@ struct Options {
     int pass2_direct = 0;
     int trace = 0;
     int timing = 0;
+    /* x-slabs: planes owned per slab (0: chosen from the memory budget), and that budget in MiB
+       (0: half the device's free memory) */
+    int slab_planes = 0;
+    int slab_mb = 0;
 };
 extern Options g_opt;
 
@@ -143,6 +147,17 @@ struct StageTimer {
     }
 };
 
+/* the whole grid as one slab: every stage over planes [0, g) */
+inline void full_planes(QsMol &M)
+{
+    const int g = M.G.g;
+    M.p1_lo = M.p2_lo = M.mc_lo = 0;
+    M.p1_hi = M.p2_hi = M.mc_hi = M.cube_hi = g;
+}
+
+/* the array index of a molecule's first stored word (word_off addresses global plane 0) */
+inline uint64_t phys_word_off(const QsMol &M) { return M.word_off + (uint64_t)M.p1_lo * M.G.g * M.W; }
+
 /* host description of one molecule of a batch */
 struct MolIn {
     uint32_t atom_off = 0, n = 0;
@@ -184,6 +199,7 @@ struct Engine {
     DevBuf vert, vert2, vert3, vkey, cidx, tri, tri_out, atom;
     DevBuf deg_blk, keeppos, tri_loc;
     DevBuf scan_tmp;
+    DevBuf inc_cnt, inc_start, inc_ent; /* per-vertex triangle incidences (smoothing of slab meshes) */
     std::vector<P2Group> groups;
     std::vector<P2Tables *> tables;
     /* host staging of uploads that kernels read later in the same call */
@@ -249,6 +265,12 @@ struct Engine {
     bool field_sparse = false;
     /* the last run's atom ids are cell-sorted positions (option atom_ids_sorted at run time) */
     bool ids_sorted = false;
+    /* row words the marching-cubes scans cover: [scan_w0, scan_w0 + scan_nw) */
+    uint64_t scan_w0 = 0, scan_nw = 0;
+    /* the last run meshed its grid in x-slabs (n_slabs of them): the grid arrays hold only the
+       last slab, so the field, labels and normals of the mesh are not available */
+    bool multi_slab = false;
+    int n_slabs = 1;
     float *res_vert = nullptr;
     int *res_tri = nullptr, *res_atom = nullptr;
 
@@ -258,7 +280,7 @@ struct Engine {
         for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start,
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
                           &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey, &cidx,
-                          &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp})
+                          &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent})
             release(*b);
         for (P2Tables *t : tables) {
             release(t->offs);
@@ -357,6 +379,11 @@ struct Engine {
     int fill_field();
     int mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer &tm, bool with_atoms);
     int group_tables(P2Group &grp);
+    /* x-slab form of one molecule's field and mesh stages: owned planes per slab (0: one pass) */
+    int slab_plan(const QsMol &M, int &planes);
+    int run_slabs(int planes, int smooth, StageTimer &tm);
+    int slab_mesh(size_t at_mc, uint32_t nb_mc, size_t at_wpre, uint64_t owned_word, uint32_t vbase, uint32_t tbase,
+                  uint32_t &n_owned, uint32_t &n_tri);
 };
 
 

@@ -206,7 +206,11 @@ int Engine::setup(const std::vector<MolIn> &in, const std::vector<BBox> &bb)
             grid_from_bbox(bb[m], in[m].reso, M.G);
             QsGrid &G = M.G;
             if (G.g < 2 || G.gn < 1) return set_error(3, "molecule %d: degenerate grid (g=%d)", m, G.g);
-            if ((uint64_t)G.g * G.g * G.g >= (1ull << 32)) return set_error(4, "molecule %d: grid of %d^3 points", m, G.g);
+            /* one molecule of any size runs in x-slabs; a batch's grids run whole */
+            if (nmol > 1 && (uint64_t)G.g * G.g * G.g >= (1ull << 32))
+                return set_error(4, "molecule %d: a grid of %d^3 points is meshed alone, not in a batch", m, G.g);
+            if ((uint64_t)G.g * G.g * ((G.g + 63) / 64) >= (1ull << 32))
+                return set_error(4, "molecule %d: grid of %d^3 points", m, G.g);
             int lim = g_opt.lim_mode ? (G.g - 2 * G.r - 1) : (G.g - 1);
             G.lim = std::max(lim, 0);
             M.W = (uint32_t)((G.g + 63) / 64);
@@ -219,6 +223,7 @@ int Engine::setup(const std::vector<MolIn> &in, const std::vector<BBox> &bb)
             M.rows_per_wave = 1;
             M.slab_rows = 1;
         }
+        full_planes(M);
         const uint64_t gg = (uint64_t)M.G.g;
         M.cell_off = (uint32_t)cell;
         M.word_off = word;
@@ -232,6 +237,8 @@ int Engine::setup(const std::vector<MolIn> &in, const std::vector<BBox> &bb)
     ncell_total = cell;
     nw_total = word;
     vox_total = vox;
+    scan_w0 = 0;
+    scan_nw = word;
     return 0;
 }
 
@@ -276,13 +283,13 @@ int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &a
     p1_lds = 0;
     for (const QsMol &M : mols)
         if (M.G.g) p1_lds = std::max(p1_lds, qs_pass1_lds_bytes(M, p1mode));
-    at_mc = prefix([](const QsMol &M) { return (uint64_t)(M.G.g ? qs_mc_count_blocks(M.G, (int)M.W) : 0); }, nb_mc);
+    at_mc = prefix([](const QsMol &M) { return (uint64_t)(M.G.g ? qs_mc_count_blocks(M) : 0); }, nb_mc);
     /* word prefix (64-bit) stored as pairs, then its coarse index */
     if (meta.size() & 1) meta.push_back(0u);
     at_wpre = meta.size();
     std::vector<uint64_t> wp(nmol + 1);
     for (int m = 0; m <= nmol; m++) {
-        uint64_t w = (m < nmol) ? mols[m].word_off : nw_total;
+        uint64_t w = (m < nmol) ? phys_word_off(mols[m]) : nw_total;
         wp[m] = w;
         meta.push_back((uint32_t)w);
         meta.push_back((uint32_t)(w >> 32));
@@ -317,7 +324,7 @@ int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &a
         uint64_t acc = 0;
         for (int m : grp.mols) {
             meta.push_back((uint32_t)acc);
-            acc += qs_pass2_blocks(mols[m].G, grp.path);
+            acc += qs_pass2_blocks(mols[m], grp.path);
         }
         meta.push_back((uint32_t)acc);
         grp.nblocks = (uint32_t)acc;
@@ -417,18 +424,18 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     if (nb_mc)
         qs_launch_mc_count(batch(at_mc, 0, nmol), nb_mc, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
                            as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint64_t>(ntb), as<uint8_t>(cidx), stream);
-    if (nw_total) {
-        QS_TRY(ensure(scan_tmp, 3 * qs_scan_tiles(nw_total) * sizeof(uint32_t)));
-        qs_launch_mc_scans(as<uint32_t>(vcnt), as<uint32_t>(tcnt), nw_total, as<uint32_t>(scan_tmp), as<uint32_t>(vbw),
-                           as<uint32_t>(tbw), as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(atb), as<uint32_t>(totals),
-                           stream);
+    if (scan_nw) {
+        QS_TRY(ensure(scan_tmp, 3 * qs_scan_tiles(scan_nw) * sizeof(uint32_t)));
+        qs_launch_mc_scans(as<uint32_t>(vcnt) + scan_w0, as<uint32_t>(tcnt) + scan_w0, scan_nw, as<uint32_t>(scan_tmp),
+                           as<uint32_t>(vbw) + scan_w0, as<uint32_t>(tbw) + scan_w0, as<uint32_t>(alist), as<uint32_t>(avb),
+                           as<uint32_t>(atb), as<uint32_t>(totals), (uint32_t)scan_w0, stream);
         QS_HIP(hipGetLastError());
     } else {
         QS_HIP(hipMemsetAsync(totals.p, 0, 16, stream));
     }
     /* per-molecule first vertex and triangle: the scans at each molecule's first word */
     std::vector<uint64_t> idx(nmol);
-    for (int m = 0; m < nmol; m++) idx[m] = std::min<uint64_t>(mols[m].word_off, nw_total ? nw_total - 1 : 0);
+    for (int m = 0; m < nmol; m++) idx[m] = std::min<uint64_t>(phys_word_off(mols[m]), nw_total ? nw_total - 1 : 0);
     QS_TRY(ensure(bidx, ((size_t)nmol + 1) * 8));
     QS_TRY(ensure(bounds, ((size_t)nmol + 1) * 8));
     std::vector<uint32_t> hb(2 * (size_t)nmol + 3);
@@ -455,7 +462,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     vpre.assign(nmol + 1, 0u);
     std::vector<uint32_t> tpre(nmol + 1);
     for (int m = 0; m < nmol; m++) {
-        const bool last_empty = mols[m].word_off >= nw_total;
+        const bool last_empty = phys_word_off(mols[m]) >= nw_total;
         vpre[m] = last_empty ? nv : hb[m];
         tpre[m] = last_empty ? nt_all : hb[nmol + m];
     }
@@ -494,7 +501,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
                           field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey), stream);
     qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart),
                            nact, nt_all, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
-                           as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri), as<uint32_t>(deg_blk),
+                           as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri), 0u, as<uint32_t>(deg_blk),
                            as<uint32_t>(totals) + 3, stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
@@ -521,6 +528,230 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     return 0;
 }
 
+/* grow a buffer to at least `bytes`, keeping its first `keep` bytes */
+static int grow_keep(DevBuf &b, size_t bytes, size_t keep, hipStream_t s)
+{
+    if (bytes <= b.cap && b.p) return 0;
+    DevBuf nb;
+    const size_t want = std::max<size_t>(bytes + (bytes >> 2), 256);
+    QS_HIP(hipMalloc(&nb.p, want));
+    nb.cap = want;
+    if (keep && b.p) {
+        QS_HIP(hipMemcpyAsync(nb.p, b.p, keep, hipMemcpyDeviceToDevice, s));
+        QS_HIP(hipStreamSynchronize(s));
+    }
+    release(b);
+    b = nb;
+    return 0;
+}
+
+/* device bytes of the grid arrays per x plane: per row word the outside, shell, sign and zero
+   masks (32 B), slot masks (32 B), triangle-count planes (24 B) and seven counts and scans
+   (28 B); per voxel the field (4 B) and the cube case (1 B) */
+static uint64_t plane_bytes(const QsMol &M)
+{
+    const uint64_t g = (uint64_t)M.G.g;
+    return g * M.W * 116 + g * g * 5;
+}
+
+/* Planes each slab owns (CudaSurf.cu:626-636 sizes its slices from the SLICE setting instead),
+   0 when the whole grid runs at once: that is when it fits the memory budget and its grid points
+   fit 32-bit indexes (the grid-walk smoothing keys them so).  Option slab_planes forces slabs of
+   that many planes; option slab_mb sets the budget (default: half the device's free memory). */
+int Engine::slab_plan(const QsMol &M, int &planes)
+{
+    planes = 0;
+    const int g = M.G.g, R = M.G.r;
+    if (g_opt.slab_planes > 0 && g_opt.slab_planes < g) {
+        planes = g_opt.slab_planes;
+        return 0;
+    }
+    const uint64_t pb = plane_bytes(M);
+    uint64_t budget = (uint64_t)g_opt.slab_mb << 20;
+    if (!budget) {
+        size_t free_b = 0, total_b = 0;
+        QS_HIP(hipMemGetInfo(&free_b, &total_b));
+        budget = free_b / 2;
+    }
+    const bool fits = pb * (uint64_t)g <= budget;
+    if (fits && (uint64_t)g * g * g < (1ull << 32)) return 0;
+    const uint64_t halo = 2 * (uint64_t)R + 3;
+    const uint64_t stored = budget / pb;
+    if (stored < halo + 8) return set_error(1, "grid of %d^3 points: not enough device memory for an x-slab", g);
+    planes = (int)std::min<uint64_t>((uint64_t)g, stored - halo);
+    return 0;
+}
+
+/* Marching cubes of one slab (CudaSurf.cu:373-458 for one slice): count the slab's words, scan
+   the owned grid points' words and those of the next slab's first plane (so the vertex ids the
+   triangles take there are the ones that slab emits), emit the owned vertices at global id
+   vbase + local id and the slab's triangles at tbase. */
+int Engine::slab_mesh(size_t at_mc, uint32_t nb_mc, size_t at_wpre, uint64_t owned_word, uint32_t vbase, uint32_t tbase,
+                      uint32_t &n_owned, uint32_t &n_tri)
+{
+    const uint64_t nw = std::max<uint64_t>(nw_total, 1);
+    QS_TRY(ensure(s4, nw * 32));
+    for (DevBuf *b : {&vcnt, &tcnt, &vbw, &tbw, &alist, &avb, &atb}) QS_TRY(ensure(*b, nw * 4));
+    QS_TRY(ensure(ntb, nw * 24));
+    QS_TRY(ensure(cidx, std::max<uint64_t>(vox_total, 1)));
+    QS_TRY(ensure(totals, 16));
+    if (nb_mc)
+        qs_launch_mc_count(batch(at_mc, 0, 1), nb_mc, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
+                           as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint64_t>(ntb), as<uint8_t>(cidx), stream);
+    QS_TRY(ensure(scan_tmp, 3 * qs_scan_tiles(std::max<uint64_t>(scan_nw, 1)) * sizeof(uint32_t)));
+    qs_launch_mc_scans(as<uint32_t>(vcnt) + scan_w0, as<uint32_t>(tcnt) + scan_w0, scan_nw, as<uint32_t>(scan_tmp),
+                       as<uint32_t>(vbw) + scan_w0, as<uint32_t>(tbw) + scan_w0, as<uint32_t>(alist), as<uint32_t>(avb),
+                       as<uint32_t>(atb), as<uint32_t>(totals), (uint32_t)scan_w0, stream);
+    QS_HIP(hipGetLastError());
+    uint32_t hb[4] = {0, 0, 0, 0};
+    uint32_t *hs = (uint32_t *)stage(sizeof(hb));
+    uint32_t *hd = hs ? hs : hb;
+    QS_HIP(hipMemcpyAsync(hd, totals.p, 12, hipMemcpyDeviceToHost, stream));
+    if (owned_word != ~0ull)
+        QS_HIP(hipMemcpyAsync(hd + 3, as<uint32_t>(vbw) + owned_word, 4, hipMemcpyDeviceToHost, stream));
+    QS_HIP(hipStreamSynchronize(stream));
+    if (hs) memcpy(hb, hs, sizeof(hb));
+    n_tri = hb[0];
+    const uint32_t nact = hb[2];
+    n_owned = (owned_word != ~0ull) ? hb[3] : hb[1];
+    if ((uint64_t)vbase + n_owned >= (1ull << 31) || 3 * ((uint64_t)tbase + n_tri) >= (1ull << 31))
+        return set_error(4, "mesh too large");
+    QS_TRY(grow_keep(vert, ((size_t)vbase + n_owned + 1) * 12, (size_t)vbase * 12, stream));
+    QS_TRY(grow_keep(tri, ((size_t)tbase + n_tri + 1) * 12, (size_t)tbase * 12, stream));
+    QS_TRY(ensure(vkey, ((size_t)n_owned + 1) * 8));
+    QS_TRY(ensure(vstart, ((size_t)n_owned / 256 + 1) * 4));
+    QS_TRY(ensure(tstart, ((size_t)n_tri / 256 + 1) * 4));
+    QS_TRY(ensure(deg_blk, ((size_t)n_tri / 256 + 1) * 4));
+    const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
+    const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 4;
+    qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart), nact,
+                          n_owned, as<uint64_t>(s4), as<float>(field), as<uint64_t>(outm), as<uint64_t>(shellm),
+                          as<float>(vert) + 3 * (size_t)vbase, as<uint64_t>(vkey), stream);
+    qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart), nact,
+                           n_tri, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
+                           as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri) + 3 * (size_t)tbase, vbase,
+                           as<uint32_t>(deg_blk), as<uint32_t>(totals) + 3, stream);
+    QS_HIP(hipGetLastError());
+    return 0;
+}
+
+/* One molecule in x-slabs (the reference's slice loop, CudaSurf.cu:626-754, with disjoint cube
+   ownership instead of overlapping slices).  Slab k owns the cubes with origin x in [x0, x1).
+   Its arrays hold planes [x0 - 1 - R, x1 + 2 + R): field pass 1 labels all of them, field pass 2
+   evaluates [x0 - 1, x1 + 2) (its window reads R planes either side), the marching-cubes count
+   covers grid points [x0, x1 + 1) and polygonises cubes below x1.  Every stage therefore sees
+   exactly the values the whole-grid run computes there.  Vertex ids are ranks in global
+   (point, slot) order and triangles follow global cube order, so the slabs' meshes concatenate
+   into the whole-grid mesh: the seams weld exactly.  The per-vertex stages then run on the whole
+   mesh: closest atom, Laplacian passes over the triangle array, the degenerate filter. */
+int Engine::run_slabs(int planes, int smooth, StageTimer &tm)
+{
+    QsMol &M = mols[0];
+    const int g = M.G.g, R = M.G.r;
+    const uint64_t plane_w = (uint64_t)g * M.W, plane_v = (uint64_t)g * g;
+    multi_slab = true;
+    n_slabs = (g + planes - 1) / planes;
+    uint64_t vbase = 0, tbase = 0;
+    StageTimer none;
+    for (int x0 = 0; x0 < g; x0 += planes) {
+        const int x1 = std::min(g, x0 + planes);
+        M.p1_lo = std::max(0, x0 - 1 - R);
+        M.p1_hi = std::min(g, x1 + 2 + R);
+        M.p2_lo = std::max(0, x0 - 1);
+        M.p2_hi = std::min(g, x1 + 2);
+        M.mc_lo = x0;
+        M.mc_hi = std::min(g, x1 + 1);
+        M.cube_hi = x1;
+        /* global plane x lands at array plane x - p1_lo (unsigned wrap-around) */
+        M.word_off = 0ull - (uint64_t)M.p1_lo * plane_w;
+        M.vox_off = 0ull - (uint64_t)M.p1_lo * plane_v;
+        nw_total = (uint64_t)(M.p1_hi - M.p1_lo) * plane_w;
+        vox_total = (uint64_t)(M.p1_hi - M.p1_lo) * plane_v;
+        scan_w0 = (uint64_t)(M.mc_lo - M.p1_lo) * plane_w;
+        scan_nw = (uint64_t)(M.mc_hi - M.mc_lo) * plane_w;
+        /* the first word of the next slab's first plane: the vertices before it are this slab's */
+        const uint64_t owned_word = (x1 < g) ? (uint64_t)(x1 - M.p1_lo) * plane_w : ~0ull;
+        size_t at_atoms, at_p1, at_mc, at_wpre;
+        uint32_t nb_atoms, nb_p1, nb_mc;
+        QS_TRY(build_meta(at_atoms, at_p1, at_mc, at_wpre, nb_atoms, nb_p1, nb_mc));
+        QS_TRY(ensure(d_meta, h_meta.size() * 4));
+        QS_TRY(upload(d_mols.p, mols.data(), sizeof(QsMol)));
+        QS_TRY(upload(d_meta.p, h_meta.data(), h_meta.size() * 4));
+        QS_TRY(field_stage(nb_p1, at_p1, none));
+        uint32_t n_owned = 0, n_tri = 0;
+        QS_TRY(slab_mesh(at_mc, nb_mc, at_wpre, owned_word, (uint32_t)vbase, (uint32_t)tbase, n_owned, n_tri));
+        vbase += n_owned;
+        tbase += n_tri;
+    }
+    tm.mark("slabs");
+    nv = (uint32_t)vbase;
+    nt_all = (uint32_t)tbase;
+    n_active = 0;
+    M.vert_off = 0;
+    M.n_vert = nv;
+    M.tri_off = 0;
+    M.n_tri = nt_all;
+    vert_off.assign(2, 0);
+    vert_off[1] = nv;
+    QS_TRY(upload(d_mols.p, mols.data(), sizeof(QsMol)));
+    std::vector<uint32_t> &vpre = h_vpre;
+    vpre.assign({0u, nv});
+    {
+        std::vector<uint32_t> vc;
+        append_coarse(vc, vpre.data(), 1, nv, QS_COARSE_ITEMS);
+        vpre.insert(vpre.end(), vc.begin(), vc.end());
+    }
+    QS_TRY(ensure(d_off, 2 * 4 + vpre.size() * 4 + 64));
+    uint32_t *d_vpre = as<uint32_t>(d_off) + 2;
+    QS_TRY(upload(d_vpre, vpre.data(), vpre.size() * 4));
+
+    /* closest atom per unsmoothed vertex, beside the smoothing passes unless stages are timed */
+    /* (vert and tri already hold the whole mesh: grow_keep sized them) */
+    QS_TRY(ensure(atom, (size_t)std::max<uint32_t>(nv, 1) * 4));
+    const bool overlap = !tm.on;
+    hipStream_t cs = overlap ? side : stream;
+    if (overlap) {
+        QS_HIP(hipEventRecord(ev_emit, stream));
+        QS_HIP(hipStreamWaitEvent(side, ev_emit, 0));
+    }
+    ids_sorted = g_opt.atom_ids_sorted != 0;
+    qs_launch_closest_atom(as<QsMol>(d_mols), d_vpre, d_vpre + 2, as<uint32_t>(start), as<float4>(sx), as<int>(perm),
+                           as<float>(vert), nv, ids_sorted ? 1 : 0, as<int>(atom), cs);
+    QS_HIP(hipGetLastError());
+    if (overlap) {
+        QS_HIP(hipEventRecord(ev_side, side));
+        side_pending = true;
+    }
+    tm.mark("closest_atom");
+
+    /* Laplacian passes (Jacobi) over the triangle array's incidences */
+    res_vert = as<float>(vert);
+    if (smooth > 0 && nv > 0 && nt_all > 0) {
+        QS_TRY(ensure(vert2, (size_t)nv * 12));
+        if (smooth > 1) QS_TRY(ensure(vert3, (size_t)nv * 12));
+        QS_TRY(ensure(inc_cnt, (size_t)nv * 4));
+        QS_TRY(ensure(inc_start, ((size_t)nv + 1) * 4));
+        QS_TRY(ensure(inc_ent, (size_t)nt_all * 12));
+        QS_HIP(hipMemsetAsync(inc_cnt.p, 0, (size_t)nv * 4, stream));
+        qs_launch_incid_count(as<int>(tri), nt_all, nv, as<uint32_t>(inc_cnt), stream);
+        QS_TRY(scan(as<uint32_t>(inc_cnt), as<uint32_t>(inc_start), nv, as<uint32_t>(inc_start) + nv));
+        QS_HIP(hipMemsetAsync(inc_cnt.p, 0, (size_t)nv * 4, stream));
+        qs_launch_smooth_incid_build(as<int>(tri), nt_all, nv, as<uint32_t>(inc_cnt), as<uint32_t>(inc_start),
+                                     as<uint32_t>(inc_ent), stream);
+        const float *src = as<float>(vert);
+        float *dst = as<float>(vert2);
+        for (int it = 0; it < smooth; it++) {
+            qs_launch_smooth_incid(as<uint32_t>(inc_start), as<uint32_t>(inc_ent), as<int>(tri), nv, src, dst, stream);
+            src = dst;
+            dst = (dst == as<float>(vert2)) ? as<float>(vert3) : as<float>(vert2);
+        }
+        QS_HIP(hipGetLastError());
+        res_vert = (float *)src;
+        tm.mark("smooth");
+    }
+    return 0;
+}
+
 int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<MolIn> &in, int smooth,
                       const std::vector<BBox> *host_bbox)
 {
@@ -531,6 +762,8 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     res_vert = nullptr;
     res_tri = nullptr;
     res_atom = nullptr;
+    multi_slab = false;
+    n_slabs = 1;
     vert_off.assign(nmol + 1, 0);
     tri_off.assign(nmol + 1, 0);
     mols.clear();
@@ -609,7 +842,12 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     QS_HIP(hipGetLastError());
     tm.mark("cells");
 
-    /* ---- field, mesh ---- */
+    /* ---- field, mesh: the whole grid at once, or one molecule in x-slabs ---- */
+    int slab_planes = 0;
+    if (nmol == 1 && mols[0].G.g) QS_TRY(slab_plan(mols[0], slab_planes));
+    if (slab_planes) {
+        QS_TRY(run_slabs(slab_planes, smooth, tm));
+    } else {
     QS_TRY(field_stage(nb_p1, at_p1, tm));
     QS_TRY(mesh_stage(at_mc, nb_mc, at_wpre, tm, true));
 
@@ -633,6 +871,7 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
         res_vert = (float *)src;
         tm.mark("smooth");
     }
+    }
 
     /* the closest-atom pass joins the main stream */
     if (side_pending) {
@@ -646,6 +885,13 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
        256 b - (degenerate ones in blocks before b) + rank in block b; the kept counts before each
        molecule's first triangle give the offsets, equal to the input offsets when none drop. */
     const uint64_t nblk_t = ((uint64_t)nt_all + 255) / 256;
+    if (multi_slab) {
+        /* the slabs' triangle launches counted per slab: count again over the whole array */
+        QS_TRY(ensure(deg_blk, (nblk_t + 1) * 4));
+        QS_HIP(hipMemsetAsync(as<uint32_t>(totals) + 3, 0, 4, stream));
+        qs_launch_deg_count(as<int>(tri), nt_all, as<uint32_t>(deg_blk), as<uint32_t>(totals) + 3, stream);
+        QS_HIP(hipGetLastError());
+    }
     QS_TRY(ensure(keeppos, (nblk_t + 1) * 4));
     QS_TRY(ensure(tri_out, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
     QS_TRY(scan(as<uint32_t>(deg_blk), as<uint32_t>(keeppos), nblk_t, as<uint32_t>(keeppos) + nblk_t));
@@ -831,10 +1077,14 @@ int single_grid(Engine &E, const QsGrid &G, size_t &at_mc, uint32_t &nb_mc, size
     M.W = (uint32_t)((G.g + 63) / 64);
     M.rows_per_wave = qs_pass1_rows_per_wave(G);
     M.slab_rows = qs_pass1_slab_rows(G, (int)M.W);
+    full_planes(M);
     E.n_atoms_total = 0;
     E.ncell_total = (uint64_t)G.gn * G.gn * G.gn;
     E.nw_total = (uint64_t)G.g * G.g * M.W;
     E.vox_total = (uint64_t)G.g * G.g * G.g;
+    E.scan_w0 = 0;
+    E.scan_nw = E.nw_total;
+    E.multi_slab = false;
     E.batch_valid = false;
     E.vert_off.assign(2, 0);
     E.tri_off.assign(2, 0);
@@ -901,6 +1151,8 @@ int qs_set_option(const char *key, int value)
     else if (k == "pass2_direct") g_opt.pass2_direct = value;
     else if (k == "trace") g_opt.trace = value;
     else if (k == "timing") g_opt.timing = value;
+    else if (k == "slab_planes") g_opt.slab_planes = value < 0 ? 0 : value;
+    else if (k == "slab_mb") g_opt.slab_mb = value < 0 ? 0 : value;
     else return -1;
     return 0;
 }
@@ -915,6 +1167,8 @@ int qs_get_option(const char *key)
     if (k == "pass2_direct") return g_opt.pass2_direct;
     if (k == "trace") return g_opt.trace;
     if (k == "timing") return g_opt.timing;
+    if (k == "slab_planes") return g_opt.slab_planes;
+    if (k == "slab_mb") return g_opt.slab_mb;
     return -1;
 }
 
@@ -1226,6 +1480,7 @@ int qs_batch_mesh(int m, float *vert, int *tri, int *atom_id)
 static int launch_normals(Engine &E, float *d_out)
 {
     if (!E.batch_valid) return set_error(3, "no mesh to take normals of");
+    if (E.multi_slab) return set_error(3, "normals: the last mesh was built in x-slabs; its field is not kept");
     const int nmol = (int)E.mols.size();
     const uint32_t *d_vpre = as<uint32_t>(E.d_off) + (nmol + 1);
     qs_launch_vertex_normals(as<QsMol>(E.d_mols), d_vpre, d_vpre + nmol + 1, as<uint64_t>(E.vkey), E.nv, as<float>(E.field),
@@ -1354,6 +1609,8 @@ int qs_batch_grid(int m, float *fp, int *ip)
 
 int qs_last_grid(float *fp, int *ip) { return qs_batch_grid(0, fp, ip); }
 
+int qs_last_slab_count(void) { return g_eng.batch_valid ? g_eng.n_slabs : 0; }
+
 int64_t qs_fetch(const char *what, void *host, int64_t bytes)
 {
     Engine &E = g_eng;
@@ -1363,6 +1620,8 @@ int64_t qs_fetch(const char *what, void *host, int64_t bytes)
     const uint64_t V = E.vox_total;
     const void *src = nullptr;
     int64_t need = 0;
+    const bool grid_kept = !E.multi_slab; /* a slab run keeps only its last slab's grid arrays */
+    if (!grid_kept && (w == "labels" || w == "field" || w == "shellm")) return -1;
     if (w == "labels") {
         /* molecule 0 of the last run, reconstructed from the outside / shell masks */
         const uint64_t V0 = (uint64_t)G.g * G.g * G.g;

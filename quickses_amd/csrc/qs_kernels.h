@@ -27,6 +27,7 @@ int qs_pass1_rows_per_wave(QsGrid G);
 int qs_pass1_slab_rows(QsGrid G, int W);
 /* field pass 1 kernel: 0 slab scatter (default), 1 row gather, 2 per-voxel */
 int qs_pass1_mode(bool direct, bool rows);
+/* blocks of one molecule's launch over its planes [p1_lo, p1_hi) */
 uint32_t qs_pass1_blocks(const QsMol &M, int mode);
 size_t qs_pass1_lds_bytes(const QsMol &M, int mode);
 void qs_launch_pass1(QsBatch B, uint32_t nblocks, int mode, size_t lds_bytes, const uint32_t *cstart, const float4 *sx,
@@ -38,7 +39,8 @@ void qs_launch_pass1_direct(QsBatch B, uint32_t nblocks, const int2 *cse, const 
 /* field pass 2 */
 size_t qs_pass2_lds_bytes(int r);
 int qs_pass2_path(int r, bool direct);
-uint32_t qs_pass2_blocks(QsGrid G, int path);
+/* blocks of one molecule's launch over its planes [p2_lo, p2_hi) */
+uint32_t qs_pass2_blocks(const QsMol &M, int path);
 void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm, float *field,
                      uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
                      QsNormFlags nf, hipStream_t s);
@@ -52,7 +54,8 @@ void qs_launch_field_signs(const float *F, int g, int W, uint32_t *negm, uint32_
 void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *outm, uint64_t *shellm, hipStream_t s);
 
 /* marching cubes */
-uint32_t qs_mc_count_blocks(QsGrid G, int W);
+/* blocks of one molecule's launch over its grid points of planes [mc_lo, mc_hi) */
+uint32_t qs_mc_count_blocks(const QsMol &M);
 /* per row word: slot masks, vertex and triangle counts, triangle-count bit planes (words with
    triangles only); the case of every polygonised cube into cidx */
 void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
@@ -68,7 +71,7 @@ void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                             const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
                             const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
-                            const uint8_t *cidx, int *tri, uint32_t *deg_blk, uint32_t *deg_total,
+                            const uint8_t *cidx, int *tri, uint32_t vbase, uint32_t *deg_blk, uint32_t *deg_total,
                             hipStream_t s);
 
 /* per-vertex stages */
@@ -78,6 +81,18 @@ void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint3
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
                       const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
                       const uint64_t *NT, const uint32_t *tbw, const int *tri, const float *src, float *dst, hipStream_t s);
+
+/* Laplacian passes over a triangle array (meshes built in x-slabs): per-vertex incidence counts
+   (cnt, nv entries, cleared by the caller) from the triangles, then, after an exclusive scan of
+   cnt into start (nv + 1) and a second clear of cnt, the incidence lists sorted by corner index
+   (ent, 3 nt entries); each pass reads src and writes dst */
+void qs_launch_incid_count(const int *tri, uint32_t nt, uint32_t nv, uint32_t *cnt, hipStream_t s);
+void qs_launch_smooth_incid_build(const int *tri, uint32_t nt, uint32_t nv, uint32_t *cnt, uint32_t *start, uint32_t *ent,
+                                  hipStream_t s);
+void qs_launch_smooth_incid(const uint32_t *start, const uint32_t *ent, const int *tri, uint32_t nv, const float *src,
+                            float *dst, hipStream_t s);
+/* degenerate triangles per 256-triangle block into deg_blk and their total added to *deg_total */
+void qs_launch_deg_count(const int *tri, uint32_t nt, uint32_t *deg_blk, uint32_t *deg_total, hipStream_t s);
 
 /* unit field gradient at each vertex's marching-cubes position, 3 floats per vertex (field as for
    the vertex kernel) */
@@ -102,9 +117,11 @@ void qs_launch_tile_scan(const uint32_t *in, uint32_t *out, uint64_t n, const ui
                          hipStream_t s);
 void qs_launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
 /* marching-cubes word scans: vbw, tbw, the active-word list and totals {triangles, vertices,
-   active words}; sums holds 3 * qs_scan_tiles(nw) entries */
+   active words}; sums holds 3 * qs_scan_tiles(nw) entries; list entries are list_base + the
+   word's index in vcnt */
 void qs_launch_mc_scans(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
-                        uint32_t *tbw, uint32_t *list, uint32_t *avb, uint32_t *atb, uint32_t *totals, hipStream_t s);
+                        uint32_t *tbw, uint32_t *list, uint32_t *avb, uint32_t *atb, uint32_t *totals, uint32_t list_base,
+                        hipStream_t s);
 void qs_launch_gather_u32(const uint32_t *src, const uint64_t *idx, int n, uint32_t *out, hipStream_t s);
 
 #endif

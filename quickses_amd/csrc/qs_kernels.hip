@@ -11,6 +11,7 @@
 
 #include "qs_common.h"
 #include "qs_kernels.h"
+#include <algorithm>
 #include <type_traits>
 
 #include "qs_mc_table.h"
@@ -311,7 +312,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict
                                                       uint64_t n, const uint32_t *__restrict__ sums, uint64_t ntiles,
                                                       uint32_t *__restrict__ vbw, uint32_t *__restrict__ tbw,
                                                       uint32_t *__restrict__ list, uint32_t *__restrict__ avb,
-                                                      uint32_t *__restrict__ atb)
+                                                      uint32_t *__restrict__ atb, uint32_t list_base)
 {
     __shared__ uint32_t lds[SCAN_NT / 64 + 1];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_IPT;
@@ -339,7 +340,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict
             if (v[k] | t[k]) {
                 avb[ra] = rv;
                 atb[ra] = rt;
-                list[ra++] = (uint32_t)i;
+                list[ra++] = list_base + (uint32_t)i;
             }
         }
         rv += v[k];
@@ -632,8 +633,8 @@ __global__ void __launch_bounds__(256) k_pass1_rows(QsBatch B, const uint32_t *_
        (molecule, rows, x and y cells) is paid once */
     const int nby = (g + 4 * R - 1) / (4 * R);
     const int by = (int)(lb % (uint32_t)nby);
-    const int x = (int)(lb / (uint32_t)nby);
-    const int x1 = g;
+    const int x = M.p1_lo + (int)(lb / (uint32_t)nby);
+    const int x1 = M.p1_hi;
     const int y0 = (by * 4 + wid) * R;
     if (y0 >= g || x >= x1) return; /* whole wavefront */
     const int ny = min(R, g - y0);
@@ -888,9 +889,9 @@ __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *_
     uint64_t *__restrict__ outm = outm_all + M.word_off;
     uint64_t *__restrict__ shellm = shellm_all + M.word_off;
     const int nyb = (g + YB - 1) / YB;
-    const int x = (int)(lb / (uint32_t)nyb);
+    const int x = M.p1_lo + (int)(lb / (uint32_t)nyb);
     const int y0 = (int)(lb % (uint32_t)nyb) * YB;
-    if (x >= g) return; /* whole block */
+    if (x >= M.p1_hi) return; /* whole block */
     const int ny = min(YB, g - y0);
     const size_t wbase = ((size_t)x * g + y0) * W;
     if (x >= g - 1) {
@@ -1084,8 +1085,8 @@ __global__ void __launch_bounds__(256) k_pass1_direct(QsBatch B, const int2 *__r
     const int nby = (g + 3) / 4;
     const int w = (int)(lb % (uint32_t)W);
     const int y = (int)((lb / (uint32_t)W) % (uint32_t)nby) * 4 + wid;
-    const int x = (int)(lb / ((uint32_t)W * nby));
-    if (y >= g || x >= g) return;
+    const int x = M.p1_lo + (int)(lb / ((uint32_t)W * nby));
+    if (y >= g || x >= M.p1_hi) return;
     const int z = w * 64 + lane;
     uint8_t res = QS_OUT;
     if (x < g - 1 && y < g - 1 && z < g - 1) {
@@ -1199,7 +1200,7 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *_
     float *__restrict__ field = field_all + M.vox_off;
     uint32_t *__restrict__ negm = negm_all + 2 * M.word_off;
     uint32_t *__restrict__ zerom = zerom_all + 2 * M.word_off;
-    const int x0 = 0, x1 = g;
+    const int x0 = M.p2_lo, x1 = M.p2_hi;
     const uint32_t nbx = (g + P2_TZ - 1) / P2_TZ, nby = (g + P2_TY - 1) / P2_TY;
     const int bidx = (int)(lb % nbx), bidy = (int)((lb / nbx) % nby), bidz = (int)(lb / (nbx * nby));
     const int LX = P2_TX + 2 * r, LY = P2_TY + 2 * r, LZ = P2_TZ + 2 * r;
@@ -1217,7 +1218,9 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *_
         int lx = row / LY, ly = row - lx * LY;
         int gx = tx0 - r + lx, gy = ty0 - r + ly;
         uint64_t m = 0ull;
-        if (gx >= 0 && gx < g && gy >= 0 && gy < g) m = row_bits(outm + ((size_t)gx * g + gy) * W, W, tz0 - r, LZ);
+        /* planes outside [p1_lo, p1_hi) are not stored (and are read by no output voxel) */
+        if (gx >= M.p1_lo && gx < M.p1_hi && gy >= 0 && gy < g)
+            m = row_bits(outm + ((size_t)gx * g + gy) * W, W, tz0 - r, LZ);
         rows[row] = m;
         if (m) fl |= 2;
     }
@@ -1432,9 +1435,10 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     float *__restrict__ field = field_all + M.vox_off;
     uint32_t *__restrict__ negm = negm_all + 2 * M.word_off;
     uint32_t *__restrict__ zerom = zerom_all + 2 * M.word_off;
-    const int x1 = g;
+    const int x1 = M.p2_hi;
     const uint32_t nbx = (g + P2_TZ - 1) / P2_TZ, nby = (g + P2_TY - 1) / P2_TY;
-    const int tz0 = (int)(lb % nbx) * P2_TZ, ty0 = (int)((lb / nbx) % nby) * P2_TY, tx0 = (int)(lb / (nbx * nby)) * P2R_TX;
+    const int tz0 = (int)(lb % nbx) * P2_TZ, ty0 = (int)((lb / nbx) % nby) * P2_TY;
+    const int tx0 = M.p2_lo + (int)(lb / (nbx * nby)) * P2R_TX;
     const int ty = threadIdx.x >> 5, tz = threadIdx.x & 31;
     const int gy = ty0 + ty, gz = tz0 + tz;
     const bool vyz = gy < g && gz < g;
@@ -1490,7 +1494,9 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         int lx = row / LY, ly = row - lx * LY;
         int gx = tx0 - R + lx, gy_ = ty0 - R + ly;
         uint64_t m = 0ull;
-        if (gx >= 0 && gx < g && gy_ >= 0 && gy_ < g) m = row_bits(outm + ((size_t)gx * g + gy_) * W, W, tz0 - R, LZ);
+        /* planes outside [p1_lo, p1_hi) are not stored (and are read by no output voxel) */
+        if (gx >= M.p1_lo && gx < M.p1_hi && gy_ >= 0 && gy_ < g)
+            m = row_bits(outm + ((size_t)gx * g + gy_) * W, W, tz0 - R, LZ);
         rows[row] = m;
         fl |= (m != 0ull);
     }
@@ -1695,7 +1701,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         for (int tx = 0; tx < P2R_TX; tx += 2) {
             const int gx = tx0 + tx;
             if (gx >= x1 || gx >= g) break;
-            const bool two = gx + 1 < g;
+            const bool two = gx + 1 < x1 && gx + 1 < g;
             const uint32_t sh = ((own[tx * P2_TY + ty] >> tz) & 1u) | (two ? ((own[(tx + 1) * P2_TY + ty] >> tz) & 1u) << 1 : 0u);
             uint32_t nm = (uint32_t)INF * 0x10001u, am = 0u;
             if (vyz && sh) {
@@ -1769,8 +1775,8 @@ __global__ void __launch_bounds__(256) k_pass2_direct(QsBatch B, const uint64_t 
     const int bidx = (int)(lb % nbx);
     int z = bidx * 64 + (threadIdx.x & 63);
     int y = (int)((lb / nbx) % nby) * 4 + (threadIdx.x >> 6);
-    int x = (int)(lb / (nbx * nby));
-    const int x1 = g;
+    int x = M.p2_lo + (int)(lb / (nbx * nby));
+    const int x1 = M.p2_hi;
     if (x >= x1 || y >= g) return;
     const bool valid = z < g;
     size_t vid = ((size_t)x * g + y) * g + z;
@@ -1957,8 +1963,8 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
     uint32_t *__restrict__ tcnt = tcnt_all + M.word_off;
     const uint32_t nbx = ((uint32_t)g * W + 255) / 256;
     const int idx = (int)(lb % nbx) * 256 + threadIdx.x;
-    const int x = (int)(lb / nbx);
-    if (idx >= g * W || x >= g) return;
+    const int x = M.mc_lo + (int)(lb / nbx);
+    if (idx >= g * W || x >= M.mc_hi) return;
     const int y = idx / W, w = idx - y * W;
     const int z0 = w * 64;
     const size_t wi = ((size_t)x * g + y) * W + w;
@@ -2003,7 +2009,7 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
     S4[4 * wi + 3] = pt;
     uint32_t nv = (uint32_t)(__popcll(ex) + __popcll(ey) + __popcll(ez) + __popcll(pt));
     uint32_t nt = 0;
-    if (x < lim && y < lim) {
+    if (x < lim && x < M.cube_hi && y < lim) {
         const uint64_t n11 = ld_word(N, g, W, x + 1, y + 1, w);
         const uint64_t c0 = n00, c1 = n10, c2 = n11, c3 = n01;
         const uint64_t c4 = n00_up;
@@ -2062,8 +2068,9 @@ __device__ __forceinline__ uint32_t last_le(const uint32_t *__restrict__ a, uint
 }
 
 /* first active word of every 256-item block of a launch over items [0, n): active word k holds
-   items [base[k], base[k + 1]) (base[nact] = n), so block b starts in the word whose range holds
-   item 256 b */
+   items [base[k], base[k + 1]) (the last one up to n), so block b starts in the word whose range
+   holds item 256 b.  Words may hold items beyond n (a slab's scan counts the vertices of the next
+   slab's first plane, which that slab emits): blocks stop at n. */
 __global__ void __launch_bounds__(256) k_block_starts(const uint32_t *__restrict__ base, uint32_t nact, uint32_t n,
                                                       uint32_t *__restrict__ start, uint32_t *__restrict__ zero,
                                                       uint32_t nzero)
@@ -2072,7 +2079,7 @@ __global__ void __launch_bounds__(256) k_block_starts(const uint32_t *__restrict
     for (uint32_t i = k; i < nzero; i += gridDim.x * 256u) zero[i] = 0u; /* counters the next kernel adds to */
     if (k >= nact) return;
     const uint32_t b0 = (base[k] + 255u) >> 8;
-    const uint32_t e = (k + 1 < nact) ? base[k + 1] : n;
+    const uint32_t e = (k + 1 < nact) ? min(base[k + 1], n) : n;
     for (uint32_t b = b0; (b << 8) < e; b++) start[b] = k;
 }
 
@@ -2252,7 +2259,7 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
                                                       const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
                                                       const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
                                                       const uint64_t *__restrict__ NT_all, const uint8_t *__restrict__ cidx_all,
-                                                      int *__restrict__ tri, uint32_t *__restrict__ deg_blk,
+                                                      int *__restrict__ tri, uint32_t vbase, uint32_t *__restrict__ deg_blk,
                                                       uint32_t *__restrict__ deg_total)
 {
     __shared__ ItemWords S;
@@ -2294,7 +2301,7 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
     V.lim = M.G.lim;
     int c[3];
 #pragma unroll
-    for (int cc = 0; cc < 3; cc++) c[cc] = (int)V.corner_vid(x, y, z, tri_entry(packed, 3 * j + cc), ci);
+    for (int cc = 0; cc < 3; cc++) c[cc] = (int)(vbase + V.corner_vid(x, y, z, tri_entry(packed, 3 * j + cc), ci));
     tri[3 * (size_t)t + 0] = c[0];
     tri[3 * (size_t)t + 1] = c[1];
     tri[3 * (size_t)t + 2] = c[2];
@@ -2499,6 +2506,73 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* Laplacian smoothing over a triangle array (SmoothMesh.cpp:45-63, 96-124), for meshes built in
+ * x-slabs, whose grid records do not outlive their slab.  Each vertex's incidences (corner index
+ * 3 t + j of triangle t) are listed per vertex and sorted, so the Jacobi sums run over the
+ * reference's neighbour list in its push order -- triangle by triangle, corner 0: 1,2; corner 1:
+ * 0,2; corner 2: 0,1 -- as k_smooth walks it from the grid records: the results are the same bits. */
+/* (corner ids outside [0, nv) are skipped: they cannot occur, and a mistake shows as a mesh
+   difference rather than a stray write) */
+__global__ void __launch_bounds__(256) k_incid_count(const int *__restrict__ tri, uint64_t ncorner, uint32_t nv,
+                                                     uint32_t *__restrict__ cnt)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= ncorner) return;
+    const uint32_t v = (uint32_t)tri[i];
+    if (v < nv) atomicAdd(&cnt[v], 1u);
+}
+
+__global__ void __launch_bounds__(256) k_incid_fill(const int *__restrict__ tri, uint64_t ncorner, uint32_t nv,
+                                                    const uint32_t *__restrict__ start, uint32_t *__restrict__ cursor,
+                                                    uint32_t *__restrict__ ent)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= ncorner) return;
+    const uint32_t v = (uint32_t)tri[i];
+    if (v < nv) ent[start[v] + atomicAdd(&cursor[v], 1u)] = (uint32_t)i;
+}
+
+/* each vertex's incidences in increasing corner index (insertion sort; a few per vertex) */
+__global__ void __launch_bounds__(256) k_incid_sort(const uint32_t *__restrict__ start, uint32_t nv, uint32_t *__restrict__ ent)
+{
+    const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+    if (v >= nv) return;
+    const uint32_t a = start[v], b = start[v + 1];
+    for (uint32_t i = a + 1; i < b; i++) {
+        const uint32_t x = ent[i];
+        uint32_t j = i;
+        while (j > a && ent[j - 1] > x) {
+            ent[j] = ent[j - 1];
+            j--;
+        }
+        ent[j] = x;
+    }
+}
+
+/* one Jacobi pass over the listed incidences */
+__global__ void __launch_bounds__(256) k_smooth_incid(const uint32_t *__restrict__ start, const uint32_t *__restrict__ ent,
+                                                      const int *__restrict__ tri, uint32_t nv, const float *__restrict__ src,
+                                                      float *__restrict__ dst)
+{
+    const uint32_t v = xcd_group<64>(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (v >= nv) return;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    uint32_t cnt = 0;
+    for (uint32_t k = start[v], e = start[v + 1]; k < e; k++) {
+        const uint32_t c = ent[k];
+        const uint32_t t = c / 3u, j = c - 3u * t;
+        const int *tt = tri + 3 * (size_t)t;
+        const uint32_t a = min((uint32_t)tt[j == 0 ? 1 : 0], nv - 1u), b = min((uint32_t)tt[j == 2 ? 1 : 2], nv - 1u);
+        push_pair(src, a, b, sx, sy, sz);
+        cnt += 2;
+    }
+    const float fc = (float)cnt;
+    dst[3 * (size_t)v + 0] = sx / fc;
+    dst[3 * (size_t)v + 1] = sy / fc;
+    dst[3 * (size_t)v + 2] = sz / fc;
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* Per-vertex normals (the normals field that MeshData leaves commented out, SmoothMesh.h:34):
  * the unit field gradient at the vertex's marching-cubes position.  The gradient at a grid point
  * is the central difference (one-sided at the grid faces) of the field; an edge vertex blends
@@ -2597,6 +2671,21 @@ __device__ __forceinline__ bool tri_degenerate(const int *__restrict__ tri, uint
 {
     const int x = tri[3 * (size_t)t], y = tri[3 * (size_t)t + 1], z = tri[3 * (size_t)t + 2];
     return !(x != y && y != z && x != z);
+}
+
+/* degenerate triangles per 256-triangle block (and their total, added to *deg_total), as the
+   triangle kernel counts them: for triangle arrays assembled from several launches */
+__global__ void __launch_bounds__(256) k_deg_count(const int *__restrict__ tri, uint32_t nt, uint32_t *__restrict__ deg_blk,
+                                                   uint32_t *__restrict__ deg_total)
+{
+    __shared__ uint32_t lds[256 / 64 + 1];
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t d = (t < nt && tri_degenerate(tri, t)) ? 1u : 0u;
+    const uint32_t sum = block_sum<256>(d, lds);
+    if (threadIdx.x == 0) {
+        deg_blk[blockIdx.x] = sum;
+        if (sum) atomicAdd(deg_total, sum);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_tri_compact(const int *__restrict__ tri, uint32_t nt,
@@ -2713,9 +2802,10 @@ int qs_pass1_mode(bool direct, bool rows) { return direct ? 2 : (rows ? 1 : 0); 
 uint32_t qs_pass1_blocks(const QsMol &M, int mode)
 {
     const QsGrid &G = M.G;
-    if (mode == 2) return M.W * nblk(G.g, 4) * (uint32_t)G.g;
-    if (mode == 1) return nblk(G.g, 4 * M.rows_per_wave) * (uint32_t)G.g;
-    return nblk(G.g, M.slab_rows) * (uint32_t)G.g;
+    const uint32_t nx = (uint32_t)std::max(M.p1_hi - M.p1_lo, 0);
+    if (mode == 2) return M.W * nblk(G.g, 4) * nx;
+    if (mode == 1) return nblk(G.g, 4 * M.rows_per_wave) * nx;
+    return nblk(G.g, M.slab_rows) * nx;
 }
 
 size_t qs_pass1_lds_bytes(const QsMol &M, int mode)
@@ -2749,10 +2839,12 @@ int qs_pass2_path(int r, bool direct)
     return 2;
 }
 
-uint32_t qs_pass2_blocks(QsGrid G, int path)
+uint32_t qs_pass2_blocks(const QsMol &M, int path)
 {
-    if (path == 2) return nblk(G.g, 64) * nblk(G.g, 4) * (uint32_t)G.g;
-    return nblk(G.g, P2_TZ) * nblk(G.g, P2_TY) * nblk(G.g, path == 0 ? P2R_TX : P2_TX);
+    const QsGrid &G = M.G;
+    const uint32_t nx = (uint32_t)std::max(M.p2_hi - M.p2_lo, 0);
+    if (path == 2) return nblk(G.g, 64) * nblk(G.g, 4) * nx;
+    return nblk(G.g, P2_TZ) * nblk(G.g, P2_TY) * nblk(nx, path == 0 ? P2R_TX : P2_TX);
 }
 
 /* one launch for a group of molecules sharing r (and so the path) */
@@ -2795,7 +2887,10 @@ void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *ou
     hipLaunchKernelGGL(k_labels_to_masks, grid, dim3(256), 0, s, labels, g, W, outm, shellm);
 }
 
-uint32_t qs_mc_count_blocks(QsGrid G, int W) { return nblk((uint64_t)G.g * W, 256) * (uint32_t)G.g; }
+uint32_t qs_mc_count_blocks(const QsMol &M)
+{
+    return nblk((uint64_t)M.G.g * M.W, 256) * (uint32_t)std::max(M.mc_hi - M.mc_lo, 0);
+}
 
 void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
                         uint32_t *tcnt, uint64_t *NT, uint8_t *cidx, hipStream_t s)
@@ -2817,14 +2912,14 @@ void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                             const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
                             const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
-                            const uint8_t *cidx, int *tri, uint32_t *deg_blk, uint32_t *deg_total,
+                            const uint8_t *cidx, int *tri, uint32_t vbase, uint32_t *deg_blk, uint32_t *deg_total,
                             hipStream_t s)
 {
     if (!ntri || !nact) return;
     /* the per-block degenerate counts start at zero (the triangle kernel adds to them) */
     hipLaunchKernelGGL(k_block_starts, dim3(nblk(nact, 256)), dim3(256), 0, s, atb, nact, ntri, tstart, deg_blk, ntri / 256 + 1);
     hipLaunchKernelGGL(k_mc_triangles, dim3(nblk(ntri, 256)), dim3(256), 0, s, mols, wpre, wcoarse, list, atb, tstart, nact,
-                       ntri, N, Zm, S4, vbw, NT, cidx, tri, deg_blk, deg_total);
+                       ntri, N, Zm, S4, vbw, NT, cidx, tri, vbase, deg_blk, deg_total);
 }
 
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
@@ -2890,14 +2985,15 @@ void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint6
 
 /* scan helpers */
 void qs_launch_mc_scans(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
-                        uint32_t *tbw, uint32_t *list, uint32_t *avb, uint32_t *atb, uint32_t *totals, hipStream_t s)
+                        uint32_t *tbw, uint32_t *list, uint32_t *avb, uint32_t *atb, uint32_t *totals, uint32_t list_base,
+                        hipStream_t s)
 {
     const uint64_t ntiles = qs_scan_tiles(nw);
     if (!nw) return;
     hipLaunchKernelGGL(k_mc_reduce3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles);
     hipLaunchKernelGGL(k_scan_planes, dim3(3), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
     hipLaunchKernelGGL(k_mc_scan3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles, vbw, tbw, list,
-                       avb, atb);
+                       avb, atb, list_base);
 }
 
 uint64_t qs_scan_tiles(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
@@ -2925,4 +3021,35 @@ void qs_launch_gather_u32(const uint32_t *src, const uint64_t *idx, int n, uint3
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_gather_u32, dim3(nblk(n, 256)), dim3(256), 0, s, src, idx, n, out);
+}
+
+void qs_launch_smooth_incid_build(const int *tri, uint32_t nt, uint32_t nv, uint32_t *cnt, uint32_t *start, uint32_t *ent,
+                                  hipStream_t s)
+{
+    const uint64_t nc = 3 * (uint64_t)nt;
+    if (!nc || !nv) return;
+    /* start[] (nv + 1 entries) holds the exclusive prefix of cnt[]; cnt[] is reused as the fill
+       cursors, so the caller clears it between the scan and this fill */
+    hipLaunchKernelGGL(k_incid_fill, dim3(nblk(nc, 256)), dim3(256), 0, s, tri, nc, nv, start, cnt, ent);
+    hipLaunchKernelGGL(k_incid_sort, dim3(nblk(nv, 256)), dim3(256), 0, s, start, nv, ent);
+}
+
+void qs_launch_incid_count(const int *tri, uint32_t nt, uint32_t nv, uint32_t *cnt, hipStream_t s)
+{
+    const uint64_t nc = 3 * (uint64_t)nt;
+    if (!nc || !nv) return;
+    hipLaunchKernelGGL(k_incid_count, dim3(nblk(nc, 256)), dim3(256), 0, s, tri, nc, nv, cnt);
+}
+
+void qs_launch_smooth_incid(const uint32_t *start, const uint32_t *ent, const int *tri, uint32_t nv, const float *src,
+                            float *dst, hipStream_t s)
+{
+    if (!nv) return;
+    hipLaunchKernelGGL(k_smooth_incid, dim3(nblk(nv, 256)), dim3(256), 0, s, start, ent, tri, nv, src, dst);
+}
+
+void qs_launch_deg_count(const int *tri, uint32_t nt, uint32_t *deg_blk, uint32_t *deg_total, hipStream_t s)
+{
+    if (!nt) return;
+    hipLaunchKernelGGL(k_deg_count, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, deg_blk, deg_total);
 }
