@@ -97,3 +97,35 @@ def test_fullsize_properties(qs, label, kind, seed, n, reso, smooth):
         out = qs.compute_batch([small, (pos, rad)], [0.5, reso], smooth)
         assert _digest(*out[1]) == first
     print("  closed and consistently oriented, Euler characteristic %d" % chi, flush=True)
+
+
+@pytest.mark.timeout(900)
+def test_million_atoms_at_0_2A_in_slabs(qs):
+    """The 1,000,000-atom shell at 0.2 A: g = 1,913, 7.0 G grid points (2^32 and more, so it runs
+    in the slab form).  The mesh is closed and consistently oriented, and the same bits in one
+    slab as in three."""
+    pos, rad = qs.synth_shell(3, 1000000, 120.0, 186.7)
+    print("case 1,000,000-atom shell at 0.2 A, slabs as planned", flush=True)
+    V, T, A, V0, grid = _run_device(qs, pos, rad, 0.2, 1)
+    g = grid["g"]
+    assert g ** 3 >= 2 ** 32
+    assert qs.last_slab_count() >= 1
+    nv = len(V)
+    Tall = qs.fetch("tri_all", np.int32, 3 * (4 * nv + 16)).reshape(-1, 3)
+    print("  g=%d slabs=%d vertices=%d triangles=%d" % (g, qs.last_slab_count(), nv, len(T)), flush=True)
+    assert nv > 0 and T.min() >= 0 and T.max() < nv and Tall.max() < nv
+    assert A.min() >= 0 and A.max() < len(rad)
+    first = _digest(V, T, A)
+    del V0
+    chi = _check_closed_oriented(Tall, nv)
+    del Tall
+    print("  closed and consistently oriented, Euler characteristic %d" % chi, flush=True)
+    old = qs.get_option("slab_planes")
+    qs.set_option("slab_planes", (g + 2) // 3)
+    try:
+        print("case 1,000,000-atom shell at 0.2 A in 3 slabs", flush=True)
+        V3, T3, A3, _, _ = _run_device(qs, pos, rad, 0.2, 1)
+        assert qs.last_slab_count() == 3
+    finally:
+        qs.set_option("slab_planes", old)
+    assert _digest(V3, T3, A3) == first
