@@ -85,6 +85,23 @@ int qs_get_option(const char *key);
 int qs_compute_device(const float *d_pos, const float *d_rad, unsigned int N, float resoSES, int doSmoothing,
                       unsigned int *nvert, unsigned int *nidx, float **d_vert, int **d_tri, int **d_atom_id);
 
+/* qs_compute_device without allocating: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  Every working buffer is carved from a fixed region of 256 MiB per device,
+ * a file-scope device array that each device the library runs on holds until the process ends;
+ * a molecule whose buffers do not fit (about 20 M grid points, a 20,000-atom molecule at 0.5 A
+ * takes about 100 MiB) is rejected with error 6 and a message.  The mesh is written to the
+ * caller's device buffers: d_vert (3 * vert_cap floats), d_atom_id (vert_cap ints), d_tri
+ * (3 * tri_cap ints); when it has more vertices or triangles than that, error 5 and nothing is
+ * written.  *nvert, *nidx as for qs_compute_device (nidx = 3 * triangles).  The same mesh as
+ * qs_compute_device, bit for bit; the call is synchronous.  Calls share the device's region
+ * (it keeps the last call's contents, and every run clears what it needs), so calls that
+ * overlap on one device race.  The first call on a device creates the library's streams and
+ * events for it. */
+int qs_compute_device_noalloc(const float *d_pos, const float *d_rad, unsigned int N, float resoSES, int doSmoothing,
+                              float *d_vert, unsigned int vert_cap, int *d_tri, unsigned int tri_cap, int *d_atom_id,
+                              unsigned int *nvert, unsigned int *nidx);
+
 /* Batched form: nmol molecules in one sequence of launches.  Molecule m owns atoms
  * [atom_off[m], atom_off[m+1]) of the concatenated pos (3 floats per atom) / rad arrays and is
  * meshed at voxel size reso[m]; its mesh equals what qs_compute_device gives for it alone.

@@ -52,6 +52,9 @@ def lib():
     L.qs_compute_device.argtypes = [vp, vp, ctypes.c_uint, ctypes.c_float, ctypes.c_int, u32p, u32p,
                                     ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.qs_compute_device.restype = ctypes.c_int
+    L.qs_compute_device_noalloc.argtypes = [vp, vp, ctypes.c_uint, ctypes.c_float, ctypes.c_int, vp, ctypes.c_uint, vp,
+                                            ctypes.c_uint, vp, u32p, u32p]
+    L.qs_compute_device_noalloc.restype = ctypes.c_int
     L.qs_compute_batch_device.argtypes = [vp, vp, u32p, ctypes.c_int, f3p, ctypes.c_int, u32p, u32p,
                                           ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.qs_compute_batch_device.restype = ctypes.c_int
@@ -196,6 +199,20 @@ def compute_device(d_pos, d_rad, n, reso=0.5, smooth=1):
     if rc:
         raise QuickSESError("quickses error %d: %s" % (rc, L.qs_last_error_message().decode()))
     return nv.value, ni.value, pv.value, pt.value, pa.value
+
+
+def compute_device_noalloc(d_pos, d_rad, n, reso, smooth, d_vert, vert_cap, d_tri, tri_cap, d_atom):
+    """qs_compute_device_noalloc: the mesh into the caller's device buffers (capacities in
+    vertices and triangles), no device memory allocated; returns (nvert, nidx)."""
+    L = lib()
+    nv = ctypes.c_uint(0)
+    ni = ctypes.c_uint(0)
+    rc = L.qs_compute_device_noalloc(ctypes.c_void_p(d_pos), ctypes.c_void_p(d_rad), int(n), ctypes.c_float(reso),
+                                     int(smooth), ctypes.c_void_p(d_vert), int(vert_cap), ctypes.c_void_p(d_tri),
+                                     int(tri_cap), ctypes.c_void_p(d_atom), ctypes.byref(nv), ctypes.byref(ni))
+    if rc:
+        raise QuickSESError("quickses error %d: %s" % (rc, L.qs_last_error_message().decode()))
+    return nv.value, ni.value
 
 
 def _batch_desc(counts, reso):

@@ -73,6 +73,15 @@ extern thread_local int g_grow_shift;
 int ensure(DevBuf &b, size_t bytes);
 void release(DevBuf &b);
 
+/* A fixed device region that ensure() carves buffers from, in place of allocating them, while a
+   thread's g_arena points at it (the no-allocation entry point).  Buffers are carved in order
+   and never freed; the region is reused from its start by the next run. */
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+extern thread_local Arena *g_arena;
+
 template <class T>
 inline T *as(DevBuf &b) { return (T *)b.p; }
 
@@ -241,7 +250,7 @@ struct Engine {
     int upload_reset()
     {
         if (up_used || up_need) QS_HIP(hipStreamSynchronize(stream));
-        if (up_need > up_cap) {
+        if (up_need > up_cap && !g_arena) { /* (a run in a fixed region allocates no staging) */
             if (up) (void)hipHostFree(up);
             up = nullptr;
             up_cap = 0;
@@ -273,6 +282,21 @@ struct Engine {
     int n_slabs = 1;
     float *res_vert = nullptr;
     int *res_tri = nullptr, *res_atom = nullptr;
+
+    /* drops every device buffer record without freeing it (buffers carved from a fixed region,
+       which the next run reuses from its start), and the cached pass-2 tables with them */
+    void forget_buffers()
+    {
+        for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start,
+                          &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
+                          &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey, &cidx,
+                          &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent})
+            *b = DevBuf{};
+        for (P2Tables *t : tables) delete t;
+        tables.clear();
+        groups.clear();
+        batch_valid = false;
+    }
 
     /* frees every device buffer and the stream (the engine can be initialised again) */
     void release_all()

@@ -35,6 +35,7 @@ thread_local int g_err = 0;
 thread_local char g_errmsg[512] = "";
 thread_local std::vector<std::pair<std::string, float>> g_stage_ms;
 thread_local int g_grow_shift = 3;
+thread_local Arena *g_arena = nullptr;
 Options g_opt;
 
 int set_error(int code, const char *fmt, ...)
@@ -50,6 +51,18 @@ int set_error(int code, const char *fmt, ...)
 int ensure(DevBuf &b, size_t bytes)
 {
     if (bytes <= b.cap && b.p) return 0;
+    if (g_arena) {
+        /* carve from the fixed region (the buffer's earlier piece, if any, stays unused) */
+        const size_t off = (g_arena->used + 255) & ~(size_t)255;
+        const size_t want = std::max<size_t>(bytes, 256);
+        if (off + want > g_arena->cap)
+            return set_error(6, "the fixed scratch region (%zu MiB) is too small for this molecule (%zu MiB and more needed)",
+                             g_arena->cap >> 20, (off + want + (1u << 20) - 1) >> 20);
+        b.p = g_arena->base + off;
+        b.cap = want;
+        g_arena->used = off + want;
+        return 0;
+    }
     if (b.p) {
         QS_HIP(hipFree(b.p));
         b.p = nullptr;
@@ -63,7 +76,7 @@ int ensure(DevBuf &b, size_t bytes)
 
 void release(DevBuf &b)
 {
-    if (b.p) (void)hipFree(b.p);
+    if (b.p && !g_arena) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
 }
@@ -772,7 +785,7 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
         return 0;
     }
     StageTimer tm;
-    tm.on = g_opt.trace != 0 || g_opt.timing != 0;
+    tm.on = (g_opt.trace != 0 || g_opt.timing != 0) && !g_arena; /* (timing events are not created in a fixed region's run) */
     tm.s = stream;
     tm.mark("start");
 
@@ -844,7 +857,9 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
 
     /* ---- field, mesh: the whole grid at once, or one molecule in x-slabs ---- */
     int slab_planes = 0;
-    if (nmol == 1 && mols[0].G.g) QS_TRY(slab_plan(mols[0], slab_planes));
+    if (nmol == 1 && mols[0].G.g && !g_arena) QS_TRY(slab_plan(mols[0], slab_planes));
+    if (g_arena && nmol == 1 && (uint64_t)mols[0].G.g * mols[0].G.g * mols[0].G.g >= (1ull << 32))
+        return set_error(6, "grid of %d^3 points: too large for the fixed scratch region", mols[0].G.g);
     if (slab_planes) {
         QS_TRY(run_slabs(slab_planes, smooth, tm));
     } else {
@@ -1133,6 +1148,21 @@ void clear_host_result()
 
 }  // namespace
 
+/* ------------------------------------------------------------------------------------------ */
+/* The fixed device region of qs_compute_device_noalloc.  Every device the library runs on holds
+ * one until the process ends.  In host code `scratch` only names the symbol: its device address,
+ * looked up once per device with hipGetSymbolAddress, is what buffers are carved from. */
+#define QS_SCRATCH_MB 256
+static __device__ __align__(256) unsigned char scratch[(size_t)QS_SCRATCH_MB << 20];
+
+namespace {
+
+constexpr int QS_MAX_DEV = 16;
+Engine g_eng_na[QS_MAX_DEV];        /* the no-allocation entry point's engine per device */
+void *g_scratch_ptr[QS_MAX_DEV];    /* device address of `scratch` on each device */
+
+}  // namespace
+
 /* ========================================================================================== */
 /* C ABI                                                                                        */
 
@@ -1289,6 +1319,57 @@ int qs_compute_device(const float *d_pos, const float *d_rad, unsigned int N, fl
     if (d_vert) *d_vert = g_eng.res_vert;
     if (d_tri) *d_tri = g_eng.res_tri;
     if (d_atom_id) *d_atom_id = g_eng.res_atom;
+    return 0;
+}
+
+int qs_compute_device_noalloc(const float *d_pos, const float *d_rad, unsigned int N, float resoSES, int doSmoothing,
+                              float *d_vert, unsigned int vert_cap, int *d_tri, unsigned int tri_cap, int *d_atom_id,
+                              unsigned int *nvert, unsigned int *nidx)
+{
+    g_err = 0;
+    g_errmsg[0] = 0;
+    if (nvert) *nvert = 0;
+    if (nidx) *nidx = 0;
+    if (N > (1u << 30)) return set_error(3, "too many atoms (%u)", N);
+    int dev = 0;
+    QS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= QS_MAX_DEV) return set_error(3, "device %d: the fixed scratch region serves devices 0-%d", dev,
+                                                       QS_MAX_DEV - 1);
+    Engine &E = g_eng_na[dev];
+    /* set-up, once per device with that device current: the device address of its region, the
+       engine's streams and events (no memory is allocated) */
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        QS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_error(2, "no device address for the fixed scratch region");
+        g_scratch_ptr[dev] = p;
+    }
+    QS_TRY(E.init());
+    Arena arena;
+    arena.base = (char *)g_scratch_ptr[dev];
+    arena.cap = (size_t)QS_SCRATCH_MB << 20;
+    E.forget_buffers();
+    struct ArenaScope {
+        explicit ArenaScope(Arena *a) { g_arena = a; }
+        ~ArenaScope() { g_arena = nullptr; }
+    } scope(&arena);
+    std::vector<MolIn> in(1);
+    in[0].atom_off = 0;
+    in[0].n = N;
+    in[0].reso = resoSES;
+    QS_TRY(E.run_batch(d_pos, d_rad, in, doSmoothing, nullptr));
+    if (E.nv > vert_cap || E.nt > tri_cap)
+        return set_error(5, "output capacity too small: %u vertices and %u triangles (capacity %u and %u)", E.nv, E.nt,
+                         vert_cap, tri_cap);
+    if ((E.nv && (!d_vert || !d_atom_id)) || (E.nt && !d_tri)) return set_error(3, "output pointer is NULL");
+    if (E.nv) {
+        QS_HIP(hipMemcpyAsync(d_vert, E.res_vert, (size_t)E.nv * 12, hipMemcpyDeviceToDevice, E.stream));
+        QS_HIP(hipMemcpyAsync(d_atom_id, E.res_atom, (size_t)E.nv * 4, hipMemcpyDeviceToDevice, E.stream));
+    }
+    if (E.nt) QS_HIP(hipMemcpyAsync(d_tri, E.res_tri, (size_t)E.nt * 12, hipMemcpyDeviceToDevice, E.stream));
+    QS_HIP(hipStreamSynchronize(E.stream));
+    if (nvert) *nvert = E.nv;
+    if (nidx) *nidx = 3 * E.nt;
     return 0;
 }
 
