@@ -209,6 +209,7 @@ struct Engine {
     DevBuf deg_blk, keeppos, tri_loc;
     DevBuf scan_tmp;
     DevBuf inc_cnt, inc_start, inc_ent; /* per-vertex triangle incidences (smoothing of slab meshes) */
+    DevBuf tpre_d;                      /* per-molecule first triangle (u64, nmol + 1), on the device */
     std::vector<P2Group> groups;
     std::vector<P2Tables *> tables;
     /* host staging of uploads that kernels read later in the same call */
@@ -280,6 +281,11 @@ struct Engine {
        last slab, so the field, labels and normals of the mesh are not available */
     bool multi_slab = false;
     int n_slabs = 1;
+    /* vertex and triangle pools of the current run, and the least pools later runs start from
+       (grown when a run needed more) */
+    uint32_t pool_v = 0, pool_t = 0;
+    uint64_t cap_v = 0, cap_t = 0;
+    int reruns = 0;
     float *res_vert = nullptr;
     int *res_tri = nullptr, *res_atom = nullptr;
 
@@ -290,7 +296,8 @@ struct Engine {
         for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start,
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
                           &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey, &cidx,
-                          &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent})
+                          &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
+                          &tpre_d})
             *b = DevBuf{};
         for (P2Tables *t : tables) delete t;
         tables.clear();
@@ -304,7 +311,8 @@ struct Engine {
         for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start,
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
                           &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey, &cidx,
-                          &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent})
+                          &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
+                          &tpre_d})
             release(*b);
         for (P2Tables *t : tables) {
             release(t->offs);
@@ -403,6 +411,8 @@ struct Engine {
     int fill_field();
     int mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer &tm, bool with_atoms);
     int group_tables(P2Group &grp);
+    void plan_pools();
+    bool pools_overflowed(uint32_t nv_, uint32_t nt_);
     /* x-slab form of one molecule's field and mesh stages: owned planes per slab (0: one pass) */
     int slab_plan(const QsMol &M, int &planes);
     int run_slabs(int planes, int smooth, StageTimer &tm);

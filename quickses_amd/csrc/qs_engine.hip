@@ -446,80 +446,52 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     } else {
         QS_HIP(hipMemsetAsync(totals.p, 0, 16, stream));
     }
-    /* per-molecule first vertex and triangle: the scans at each molecule's first word */
-    std::vector<uint64_t> idx(nmol);
-    for (int m = 0; m < nmol; m++) idx[m] = std::min<uint64_t>(phys_word_off(mols[m]), nw_total ? nw_total - 1 : 0);
+    tm.mark("mc_count");
+    /* Vertex and triangle pools.  The totals stay on the device: the emission, closest-atom,
+       smoothing and filter kernels read them there and run as persistent blocks over the pools,
+       so nothing waits for the host until the run's results are read back at its end, which is
+       also where a pool that was too small shows (the run is then repeated with larger pools). */
+    plan_pools();
+    /* per-molecule first vertex and triangle (the scans at each molecule's first word) and the
+       vertex coarse index, computed on the device */
+    std::vector<uint64_t> first(nmol);
+    for (int m = 0; m < nmol; m++) {
+        const uint64_t w = phys_word_off(mols[m]);
+        first[m] = (w < nw_total) ? w : ~0ull;
+    }
     QS_TRY(ensure(bidx, ((size_t)nmol + 1) * 8));
     QS_TRY(ensure(bounds, ((size_t)nmol + 1) * 8));
-    std::vector<uint32_t> hb(2 * (size_t)nmol + 3);
-    uint32_t *hbs = (uint32_t *)stage(hb.size() * 4);
-    uint32_t *hbd = hbs ? hbs : hb.data();
-    /* (one molecule: it starts at word 0, where the exclusive scans are 0) */
-    if (nmol > 1 && nw_total) {
-        QS_TRY(upload(bidx.p, idx.data(), (size_t)nmol * 8));
-        qs_launch_gather_u32(as<uint32_t>(vbw), as<uint64_t>(bidx), nmol, as<uint32_t>(bounds), stream);
-        qs_launch_gather_u32(as<uint32_t>(tbw), as<uint64_t>(bidx), nmol, as<uint32_t>(bounds) + nmol, stream);
-        QS_HIP(hipMemcpyAsync(hbd, bounds.p, (size_t)nmol * 8, hipMemcpyDeviceToHost, stream));
-    }
-    QS_HIP(hipMemcpyAsync(hbd + 2 * (size_t)nmol, totals.p, 12, hipMemcpyDeviceToHost, stream));
-    QS_HIP(hipStreamSynchronize(stream));
-    if (hbs) memcpy(hb.data(), hbs, hb.size() * 4);
-    if (nmol == 1) hb[0] = hb[1] = 0u;
-    tm.mark("mc_count");
-    nt_all = hb[2 * (size_t)nmol];
-    nv = hb[2 * (size_t)nmol + 1];
-    const uint32_t nact = hb[2 * (size_t)nmol + 2];
-    n_active = nact;
-    if ((uint64_t)nt_all * 3 >= (1ull << 31) || nv >= (1u << 31)) return set_error(4, "mesh too large");
-    std::vector<uint32_t> &vpre = h_vpre;
-    vpre.assign(nmol + 1, 0u);
-    std::vector<uint32_t> tpre(nmol + 1);
-    for (int m = 0; m < nmol; m++) {
-        const bool last_empty = phys_word_off(mols[m]) >= nw_total;
-        vpre[m] = last_empty ? nv : hb[m];
-        tpre[m] = last_empty ? nt_all : hb[nmol + m];
-    }
-    vpre[nmol] = nv;
-    tpre[nmol] = nt_all;
-    for (int m = 0; m < nmol; m++) {
-        mols[m].vert_off = vpre[m];
-        mols[m].n_vert = vpre[m + 1] - vpre[m];
-        mols[m].tri_off = tpre[m];
-        mols[m].n_tri = tpre[m + 1] - tpre[m];
-    }
-    vert_off = vpre;
-    QS_TRY(upload(d_mols.p, mols.data(), mols.size() * sizeof(QsMol)));
-    /* vertex prefix and its coarse index for the per-vertex kernels, after the atom offsets */
-    {
-        std::vector<uint32_t> vc;
-        append_coarse(vc, vpre.data(), nmol, nv, QS_COARSE_ITEMS);
-        vpre.insert(vpre.end(), vc.begin(), vc.end());
-    }
-    QS_TRY(ensure(d_off, ((size_t)nmol + 1) * 4 + vpre.size() * 4 + 64));
+    QS_TRY(ensure(tpre_d, ((size_t)nmol + 1) * 8));
+    QS_TRY(upload(bidx.p, first.data(), (size_t)nmol * 8));
+    const uint32_t ncoarse = (pool_v >> QS_COARSE_ITEMS) + 2;
+    QS_TRY(ensure(d_off, ((size_t)nmol + 1) * 8 + (size_t)ncoarse * 4 + 64));
     uint32_t *d_vpre = as<uint32_t>(d_off) + (nmol + 1);
-    QS_TRY(upload(d_vpre, vpre.data(), vpre.size() * 4));
-    QS_TRY(ensure(vert, (size_t)std::max<uint32_t>(nv, 1) * 12));
-    QS_TRY(ensure(vkey, (size_t)std::max<uint32_t>(nv, 1) * 8));
-    QS_TRY(ensure(tri, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
+    qs_launch_mesh_bases(as<uint32_t>(vbw), as<uint32_t>(tbw), as<uint64_t>(bidx), nmol, as<uint32_t>(totals), d_vpre,
+                         as<uint64_t>(tpre_d), d_vpre + nmol + 1, ncoarse, stream);
+    QS_TRY(ensure(vert, (size_t)pool_v * 12));
+    QS_TRY(ensure(vkey, (size_t)pool_v * 8));
+    QS_TRY(ensure(tri, (size_t)pool_t * 12));
     const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
     const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 2 * ((size_t)nmol + 1);
-    QS_TRY(ensure(vstart, ((size_t)nv / 256 + 1) * 4));
-    QS_TRY(ensure(tstart, ((size_t)nt_all / 256 + 1) * 4));
+    QS_TRY(ensure(vstart, ((size_t)pool_v / 256 + 1) * 4));
+    QS_TRY(ensure(tstart, ((size_t)pool_t / 256 + 1) * 4));
     /* degenerate triangles per 256-triangle block and in total (totals[3]), counted by the
-       triangle kernel */
-    QS_TRY(ensure(deg_blk, ((size_t)nt_all / 256 + 1) * 4));
-    /* (zeroed by the triangle launch's block-start kernel; totals[3] by the scans) */
-    qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart), nact,
-                          nv, as<uint64_t>(s4), as<float>(field), as<uint64_t>(outm),
-                          field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey), stream);
+       triangle kernel (the block counts zeroed by its block-start kernel, totals[3] by the scans) */
+    QS_TRY(ensure(deg_blk, ((size_t)pool_t / 256 + 1) * 4));
+    /* every scanned word may be active */
+    const uint32_t nact_max = (uint32_t)std::max<uint64_t>(scan_nw, 1);
+    const uint32_t *counts = as<uint32_t>(totals);
+    qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart),
+                          nact_max, pool_v, as<uint64_t>(s4), as<float>(field), as<uint64_t>(outm),
+                          field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey), counts, stream);
     qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart),
-                           nact, nt_all, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
+                           nact_max, pool_t, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
                            as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri), 0u, as<uint32_t>(deg_blk),
-                           as<uint32_t>(totals) + 3, stream);
+                           as<uint32_t>(totals) + 3, counts, stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
     if (with_atoms) {
-        QS_TRY(ensure(atom, (size_t)std::max<uint32_t>(nv, 1) * 4));
+        QS_TRY(ensure(atom, (size_t)pool_v * 4));
         /* the closest-atom pass only reads the unsmoothed vertices: without stage timing it runs
            on the side stream, overlapping the smoothing passes, and is joined before the filter */
         const bool overlap = !tm.on;
@@ -529,8 +501,8 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
             QS_HIP(hipStreamWaitEvent(side, ev_emit, 0));
         }
         ids_sorted = g_opt.atom_ids_sorted != 0;
-        qs_launch_closest_atom(as<QsMol>(d_mols), d_vpre, d_vpre + nmol + 1, as<uint32_t>(start), as<float4>(sx), as<int>(perm), as<float>(vert),
-                               nv, ids_sorted ? 1 : 0, as<int>(atom), cs);
+        qs_launch_closest_atom(as<QsMol>(d_mols), d_vpre, d_vpre + nmol + 1, as<uint32_t>(start), as<float4>(sx), as<int>(perm),
+                               as<float>(vert), pool_v, ids_sorted ? 1 : 0, as<int>(atom), counts + 1, cs);
         QS_HIP(hipGetLastError());
         if (overlap) {
             QS_HIP(hipEventRecord(ev_side, side));
@@ -539,6 +511,29 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
         tm.mark("closest_atom");
     }
     return 0;
+}
+
+/* Vertex and triangle pools of a run: about 8 A^2 of surface per atom (measured 2-6 A^2 on the
+   benchmark molecules, cavities included), at least what earlier runs needed (cap_v, cap_t) */
+void Engine::plan_pools()
+{
+    uint64_t ev = 4096;
+    for (const QsMol &M : mols)
+        if (M.G.g) ev += 256 + (uint64_t)(8.0 * M.n_atoms / ((double)M.G.dx * M.G.dx));
+    ev = std::min<uint64_t>(ev, 4 * vox_total + 4096);
+    ev = std::max<uint64_t>(ev, cap_v);
+    uint64_t et = std::max<uint64_t>(2 * ev + 4096, cap_t);
+    pool_v = (uint32_t)std::min<uint64_t>(ev, (1ull << 31) - 1);
+    pool_t = (uint32_t)std::min<uint64_t>(et, (1ull << 31) / 3 - 1);
+}
+
+/* after a run's results are read: did its pools hold the mesh?  If not, they grow to fit */
+bool Engine::pools_overflowed(uint32_t nv_, uint32_t nt_)
+{
+    if (nv_ <= pool_v && nt_ <= pool_t) return false;
+    cap_v = std::max<uint64_t>(cap_v, (uint64_t)nv_ + nv_ / 4 + 4096);
+    cap_t = std::max<uint64_t>(cap_t, (uint64_t)nt_ + nt_ / 4 + 4096);
+    return true;
 }
 
 /* grow a buffer to at least `bytes`, keeping its first `keep` bytes */
@@ -639,11 +634,11 @@ int Engine::slab_mesh(size_t at_mc, uint32_t nb_mc, size_t at_wpre, uint64_t own
     const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 4;
     qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart), nact,
                           n_owned, as<uint64_t>(s4), as<float>(field), as<uint64_t>(outm), as<uint64_t>(shellm),
-                          as<float>(vert) + 3 * (size_t)vbase, as<uint64_t>(vkey), stream);
+                          as<float>(vert) + 3 * (size_t)vbase, as<uint64_t>(vkey), nullptr, stream);
     qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart), nact,
                            n_tri, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
                            as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri) + 3 * (size_t)tbase, vbase,
-                           as<uint32_t>(deg_blk), as<uint32_t>(totals) + 3, stream);
+                           as<uint32_t>(deg_blk), as<uint32_t>(totals) + 3, nullptr, stream);
     QS_HIP(hipGetLastError());
     return 0;
 }
@@ -729,7 +724,7 @@ int Engine::run_slabs(int planes, int smooth, StageTimer &tm)
     }
     ids_sorted = g_opt.atom_ids_sorted != 0;
     qs_launch_closest_atom(as<QsMol>(d_mols), d_vpre, d_vpre + 2, as<uint32_t>(start), as<float4>(sx), as<int>(perm),
-                           as<float>(vert), nv, ids_sorted ? 1 : 0, as<int>(atom), cs);
+                           as<float>(vert), nv, ids_sorted ? 1 : 0, as<int>(atom), nullptr, cs);
     QS_HIP(hipGetLastError());
     if (overlap) {
         QS_HIP(hipEventRecord(ev_side, side));
@@ -866,19 +861,19 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     QS_TRY(field_stage(nb_p1, at_p1, tm));
     QS_TRY(mesh_stage(at_mc, nb_mc, at_wpre, tm, true));
 
-    /* ---- Laplacian passes (Jacobi) ---- */
+    /* ---- Laplacian passes (Jacobi), over the vertex pool ---- */
     res_vert = as<float>(vert);
     const uint32_t *d_vpre = as<uint32_t>(d_off) + (nmol + 1);
-    if (smooth > 0 && nv > 0 && nt_all > 0) {
-        QS_TRY(ensure(vert2, (size_t)nv * 12));
+    if (smooth > 0) {
+        QS_TRY(ensure(vert2, (size_t)pool_v * 12));
         /* `vert` keeps the unsmoothed positions; passes alternate between vert2 and vert3 */
-        if (smooth > 1) QS_TRY(ensure(vert3, (size_t)nv * 12));
+        if (smooth > 1) QS_TRY(ensure(vert3, (size_t)pool_v * 12));
         const float *src = as<float>(vert);
         float *dst = as<float>(vert2);
         for (int it = 0; it < smooth; it++) {
             qs_launch_smooth(as<QsMol>(d_mols), d_vpre, d_vpre + nmol + 1, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
-                             as<uint32_t>(vbw), as<uint64_t>(vkey), nv, as<uint8_t>(cidx), as<uint64_t>(ntb), as<uint32_t>(tbw), as<int>(tri),
-                             src, dst, stream);
+                             as<uint32_t>(vbw), as<uint64_t>(vkey), pool_v, as<uint8_t>(cidx), as<uint64_t>(ntb), as<uint32_t>(tbw), as<int>(tri),
+                             src, dst, as<uint32_t>(totals) + 1, stream);
             src = dst;
             dst = (dst == as<float>(vert2)) ? as<float>(vert3) : as<float>(vert2);
         }
@@ -898,36 +893,65 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     /* The triangle kernel counted them (totals[3]).  The compaction runs without waiting for that
        count: with none it writes nothing and the input array is the result.  Kept triangles go to
        256 b - (degenerate ones in blocks before b) + rank in block b; the kept counts before each
-       molecule's first triangle give the offsets, equal to the input offsets when none drop. */
-    const uint64_t nblk_t = ((uint64_t)nt_all + 255) / 256;
+       molecule's first triangle give the offsets, equal to the input offsets when none drop.  A
+       slab run knows its triangle count; otherwise the filter covers the triangle pool and takes
+       the count from the device. */
+    const uint32_t tcap = multi_slab ? nt_all : pool_t;
+    const uint64_t nblk_t = multi_slab ? ((uint64_t)nt_all + 255) / 256 : (uint64_t)pool_t / 256 + 1;
     if (multi_slab) {
         /* the slabs' triangle launches counted per slab: count again over the whole array */
         QS_TRY(ensure(deg_blk, (nblk_t + 1) * 4));
         QS_HIP(hipMemsetAsync(as<uint32_t>(totals) + 3, 0, 4, stream));
         qs_launch_deg_count(as<int>(tri), nt_all, as<uint32_t>(deg_blk), as<uint32_t>(totals) + 3, stream);
         QS_HIP(hipGetLastError());
+        /* the molecule's first triangle and the end, for the kept counts */
+        h_bidx.assign({0ull, (uint64_t)nt_all});
+        QS_TRY(ensure(tpre_d, 2 * 8));
+        QS_TRY(upload(tpre_d.p, h_bidx.data(), 2 * 8));
     }
     QS_TRY(ensure(keeppos, (nblk_t + 1) * 4));
-    QS_TRY(ensure(tri_out, (size_t)std::max<uint32_t>(nt_all, 1) * 12));
+    QS_TRY(ensure(tri_out, (size_t)std::max<uint32_t>(tcap, 1) * 12));
     QS_TRY(scan(as<uint32_t>(deg_blk), as<uint32_t>(keeppos), nblk_t, as<uint32_t>(keeppos) + nblk_t));
-    qs_launch_tri_compact(as<int>(tri), nt_all, as<uint32_t>(keeppos), as<uint32_t>(totals) + 3, as<int>(tri_out), stream);
-    /* staging vectors: next written after this call's stream work has been waited for */
-    h_bidx.resize(nmol + 1);
-    for (int m = 0; m <= nmol; m++) h_bidx[m] = (m < nmol) ? mols[m].tri_off : nt_all;
-    QS_TRY(upload(bidx.p, h_bidx.data(), (size_t)(nmol + 1) * 8));
-    qs_launch_tri_kept_before(as<int>(tri), as<uint32_t>(keeppos), as<uint64_t>(bidx), nmol + 1, as<uint32_t>(bounds), stream);
-    uint32_t ndeg = 0;
-    uint32_t *fs = (uint32_t *)stage((size_t)(nmol + 2) * 4);
-    QS_HIP(hipMemcpyAsync(fs ? (void *)fs : (void *)tri_off.data(), bounds.p, (size_t)(nmol + 1) * 4, hipMemcpyDeviceToHost,
-                          stream));
-    QS_HIP(hipMemcpyAsync(fs ? (void *)(fs + nmol + 1) : (void *)&ndeg, as<uint32_t>(totals) + 3, 4, hipMemcpyDeviceToHost,
-                          stream));
+    qs_launch_tri_compact(as<int>(tri), tcap, as<uint32_t>(keeppos), as<uint32_t>(totals) + 3, as<int>(tri_out),
+                          multi_slab ? nullptr : as<uint32_t>(totals), stream);
+    QS_TRY(ensure(bounds, ((size_t)nmol + 1) * 8));
+    qs_launch_tri_kept_before(as<int>(tri), as<uint32_t>(keeppos), as<uint64_t>(tpre_d), nmol + 1, tcap, as<uint32_t>(bounds),
+                              stream);
+    /* the run's one read-back: totals, per-molecule first vertices and kept-triangle offsets */
+    std::vector<uint32_t> hb(4 + 2 * ((size_t)nmol + 1));
+    uint32_t *fs = (uint32_t *)stage(hb.size() * 4);
+    uint32_t *fd = fs ? fs : hb.data();
+    QS_HIP(hipMemcpyAsync(fd, totals.p, 16, hipMemcpyDeviceToHost, stream));
+    QS_HIP(hipMemcpyAsync(fd + 4, as<uint32_t>(d_off) + (nmol + 1), ((size_t)nmol + 1) * 4, hipMemcpyDeviceToHost, stream));
+    QS_HIP(hipMemcpyAsync(fd + 4 + nmol + 1, bounds.p, ((size_t)nmol + 1) * 4, hipMemcpyDeviceToHost, stream));
     QS_HIP(hipStreamSynchronize(stream));
-    if (fs) {
-        memcpy(tri_off.data(), fs, (size_t)(nmol + 1) * 4);
-        ndeg = fs[nmol + 1];
-    }
+    if (fs) memcpy(hb.data(), fs, hb.size() * 4);
     tm.mark("filter");
+    if (!multi_slab) {
+        nt_all = hb[0];
+        nv = hb[1];
+        n_active = hb[2];
+        if (pools_overflowed(nv, nt_all)) {
+            /* the mesh did not fit the pools, which have grown to fit it: run again */
+            if (reruns) return set_error(4, "mesh pools overflowed again (%u vertices, %u triangles)", nv, nt_all);
+            reruns = 1;
+            if (g_arena) {
+                g_arena->used = 0;
+                forget_buffers();
+            }
+            const int rc = run_batch(d_pos, d_rad, in, smooth, host_bbox);
+            reruns = 0;
+            return rc;
+        }
+        if ((uint64_t)nt_all * 3 >= (1ull << 31) || nv >= (1u << 31)) return set_error(4, "mesh too large");
+    }
+    const uint32_t ndeg = hb[3];
+    vert_off.assign(hb.begin() + 4, hb.begin() + 4 + nmol + 1);
+    tri_off.assign(hb.begin() + 4 + nmol + 1, hb.end());
+    for (int m = 0; m < nmol; m++) {
+        mols[m].vert_off = vert_off[m];
+        mols[m].n_vert = vert_off[m + 1] - vert_off[m];
+    }
     nt = tri_off[nmol];
     res_tri = ndeg ? as<int>(tri_out) : as<int>(tri);
     res_atom = as<int>(atom);
@@ -1183,6 +1207,7 @@ int qs_set_option(const char *key, int value)
     else if (k == "timing") g_opt.timing = value;
     else if (k == "slab_planes") g_opt.slab_planes = value < 0 ? 0 : value;
     else if (k == "slab_mb") g_opt.slab_mb = value < 0 ? 0 : value;
+    else if (k == "persistent_blocks") qs_set_persistent_blocks((uint32_t)std::max(value, 8));
     else return -1;
     return 0;
 }
@@ -1199,6 +1224,7 @@ int qs_get_option(const char *key)
     if (k == "timing") return g_opt.timing;
     if (k == "slab_planes") return g_opt.slab_planes;
     if (k == "slab_mb") return g_opt.slab_mb;
+    if (k == "persistent_blocks") return (int)qs_persistent_blocks();
     return -1;
 }
 
@@ -1810,8 +1836,16 @@ int qs_stage_mesh(const float *fp, const int *ip, const float *field, unsigned i
     QS_TRY(ensure(E.zerom, E.nw_total * 8));
     qs_launch_field_signs(as<float>(E.field), G.g, W, as<uint32_t>(E.negm), as<uint32_t>(E.zerom), E.stream);
     StageTimer tm;
-    QS_TRY(E.mesh_stage(at_mc, nb_mc, at_wpre, tm, false));
-    QS_HIP(hipStreamSynchronize(E.stream));
+    for (int attempt = 0;; attempt++) {
+        QS_TRY(E.mesh_stage(at_mc, nb_mc, at_wpre, tm, false));
+        uint32_t hb[4];
+        QS_HIP(hipMemcpyAsync(hb, E.totals.p, 16, hipMemcpyDeviceToHost, E.stream));
+        QS_HIP(hipStreamSynchronize(E.stream));
+        E.nt_all = hb[0];
+        E.nv = hb[1];
+        if (!E.pools_overflowed(E.nv, E.nt_all)) break;
+        if (attempt) return set_error(4, "mesh pools overflowed again");
+    }
     if (nvert) *nvert = E.nv;
     if (ntri) *ntri = E.nt_all;
     if ((int64_t)E.nv * 3 > vert_cap || (int64_t)E.nt_all * 3 > tri_cap) return set_error(5, "output capacity too small");

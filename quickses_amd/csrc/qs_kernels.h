@@ -67,20 +67,32 @@ void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const ui
 void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                            const uint32_t *avb, uint32_t *vstart, uint32_t nact, uint32_t nv, const uint64_t *S4,
                            const float *F, const uint64_t *outm, const uint64_t *shellm, float *vert, uint64_t *vkey,
-                           hipStream_t s);
+                           const uint32_t *d_counts, hipStream_t s);
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                             const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
                             const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
                             const uint8_t *cidx, int *tri, uint32_t vbase, uint32_t *deg_blk, uint32_t *deg_total,
-                            hipStream_t s);
+                            const uint32_t *d_counts, hipStream_t s);
+
+/* Launches whose item counts live on the device take them from the d_* pointers (when given),
+   the host counts being the pools the launch covers: marching-cubes totals are counts[0]
+   triangles, counts[1] vertices, counts[2] active words. */
+
+/* per-molecule first vertex (vpre, u32) and first triangle (tpre, u64) from the word scans at each
+   molecule's first word (first[m], ~0 for none), nmol + 1 entries each ending with the totals,
+   and the vertex coarse index (ncoarse entries) */
+void qs_launch_mesh_bases(const uint32_t *vbw, const uint32_t *tbw, const uint64_t *first, int nmol, const uint32_t *counts,
+                          uint32_t *vpre, uint64_t *tpre, uint32_t *vcoarse, uint32_t ncoarse, hipStream_t s);
 
 /* per-vertex stages */
 void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint32_t *cstart,
                             const float4 *sx,
-                            const int *perm, const float *vert, uint32_t nv, int sorted_ids, int *out, hipStream_t s);
+                            const int *perm, const float *vert, uint32_t nv, int sorted_ids, int *out, const uint32_t *d_nv,
+                            hipStream_t s);
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
                       const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
-                      const uint64_t *NT, const uint32_t *tbw, const int *tri, const float *src, float *dst, hipStream_t s);
+                      const uint64_t *NT, const uint32_t *tbw, const int *tri, const float *src, float *dst,
+                      const uint32_t *d_nv, hipStream_t s);
 
 /* Laplacian passes over a triangle array (meshes built in x-slabs): per-vertex incidence counts
    (cnt, nv entries, cleared by the caller) from the triangles, then, after an exclusive scan of
@@ -105,10 +117,16 @@ void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint
 /* degenerate-triangle filter: dpre = exclusive prefix of the per-256-triangle-block degenerate
    counts that the triangle kernel leaves, ndeg their total (device); with ndeg == 0 the compaction
    writes nothing */
-void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *dpre, const uint32_t *ndeg, int *out, hipStream_t s);
-void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint64_t *idx, int n, uint32_t *out, hipStream_t s);
+void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *dpre, const uint32_t *ndeg, int *out,
+                           const uint32_t *d_nt, hipStream_t s);
+void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint64_t *idx, int n, uint32_t tcap, uint32_t *out,
+                               hipStream_t s);
 void qs_launch_tri_localize(const int *tri, uint32_t nt, const uint32_t *tpre, const uint32_t *vpre, int nmol, int *out,
                             hipStream_t s);
+
+/* blocks of the persistent launches (rounded down to a multiple of 8, at least 8) */
+void qs_set_persistent_blocks(uint32_t n);
+uint32_t qs_persistent_blocks(void);
 
 /* device-wide exclusive scan pieces */
 uint64_t qs_scan_tiles(uint64_t n);
