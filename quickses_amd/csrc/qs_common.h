@@ -47,6 +47,8 @@ struct QsMol {
        x < cube_hi are polygonised.  word_off and vox_off address global plane x, so a slab
        whose arrays start at plane p1_lo has offsets that wrap below zero. */
     int p1_lo, p1_hi, p2_lo, p2_hi, mc_lo, mc_hi, cube_hi;
+    /* first of the molecule's field-pass-2 tile flags (in pass-2 tile order; ~0: none kept) */
+    uint32_t tile_off;
 };
 
 /* Coarse indexes of prefix arrays: coarse[k] = the segment that holds item k << shift, with

@@ -210,6 +210,7 @@ struct Engine {
     DevBuf scan_tmp;
     DevBuf inc_cnt, inc_start, inc_ent; /* per-vertex triangle incidences (smoothing of slab meshes) */
     DevBuf tpre_d;                      /* per-molecule first triangle (u64, nmol + 1), on the device */
+    DevBuf tflag;                       /* field-pass-2 tiles holding a shell voxel (u8 per tile) */
     std::vector<P2Group> groups;
     std::vector<P2Tables *> tables;
     /* host staging of uploads that kernels read later in the same call */
@@ -284,6 +285,7 @@ struct Engine {
     /* vertex and triangle pools of the current run, and the least pools later runs start from
        (grown when a run needed more) */
     uint32_t pool_v = 0, pool_t = 0;
+    uint64_t ntflags = 0; /* tile flags of the run's fixed-R pass-2 molecules */
     uint64_t cap_v = 0, cap_t = 0;
     int reruns = 0;
     float *res_vert = nullptr;
@@ -297,7 +299,7 @@ struct Engine {
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
                           &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey, &cidx,
                           &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
-                          &tpre_d})
+                          &tpre_d, &tflag})
             *b = DevBuf{};
         for (P2Tables *t : tables) delete t;
         tables.clear();
@@ -312,7 +314,7 @@ struct Engine {
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
                           &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey, &cidx,
                           &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
-                          &tpre_d})
+                          &tpre_d, &tflag})
             release(*b);
         for (P2Tables *t : tables) {
             release(t->offs);

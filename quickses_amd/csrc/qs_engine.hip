@@ -332,6 +332,16 @@ int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &a
         }
         gs[k].mols.push_back(m);
     }
+    /* tile flags of the molecules that the fixed-R pass-2 kernel runs on, in its tile order */
+    ntflags = 0;
+    for (QsMol &M : mols) M.tile_off = ~0u;
+    for (auto &grp : gs)
+        if (grp.path == 0)
+            for (int m : grp.mols) {
+                mols[m].tile_off = (uint32_t)ntflags;
+                ntflags += qs_pass2_tiles(mols[m]);
+            }
+    if (ntflags >= (1ull << 32)) return set_error(4, "too many field-pass-2 tiles for one batch");
     for (auto &grp : gs) {
         grp.bpre_at = meta.size();
         uint64_t acc = 0;
@@ -384,14 +394,24 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
     QS_TRY(ensure(field, std::max<uint64_t>(vox_total, 1) * 4));
     QS_TRY(ensure(negm, std::max<uint64_t>(nw_total, 1) * 8));
     QS_TRY(ensure(zerom, std::max<uint64_t>(nw_total, 1) * 8));
+    /* the slab kernel of field pass 1 writes every voxel's default sign words and flags the
+       pass-2 tiles that hold a shell voxel; pass 2 then runs on the flagged tiles only */
+    const int p1mode = qs_pass1_mode(g_opt.pass1_direct != 0, g_opt.pass1_rows != 0);
+    uint8_t *tf = nullptr;
+    if (p1mode == 0 && ntflags) {
+        QS_TRY(ensure(tflag, ntflags));
+        QS_HIP(hipMemsetAsync(tflag.p, 0, ntflags, stream));
+        tf = as<uint8_t>(tflag);
+    }
     if (nb_p1) {
-        const int p1mode = qs_pass1_mode(g_opt.pass1_direct != 0, g_opt.pass1_rows != 0);
         if (p1mode == 2)
             qs_launch_pass1_direct(batch(at_p1, 0, nmol), nb_p1, as<int2>(cse), as<float4>(sx), as<uint64_t>(outm),
                                    as<uint64_t>(shellm), stream);
         else
             qs_launch_pass1(batch(at_p1, 0, nmol), nb_p1, p1mode, p1_lds, as<uint32_t>(start), as<float4>(sx),
-                            as<float4>(aux), as<uint64_t>(outm), as<uint64_t>(shellm), stream);
+                            as<float4>(aux), as<uint64_t>(outm), as<uint64_t>(shellm),
+                            p1mode == 0 ? as<uint64_t>(negm) : nullptr, p1mode == 0 ? as<uint64_t>(zerom) : nullptr, tf,
+                            stream);
         QS_HIP(hipGetLastError());
     }
     tm.mark("pass1");
@@ -399,7 +419,7 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
         QS_TRY(group_tables(grp));
         qs_launch_pass2(batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path, as<uint64_t>(outm),
                         as<uint64_t>(shellm), as<float>(field), as<uint32_t>(negm), as<uint32_t>(zerom), as<int>(grp.tab->offs),
-                        as<int>(grp.tab->norm), as<int>(grp.tab->end), grp.tab->flags, stream);
+                        as<int>(grp.tab->norm), as<int>(grp.tab->end), grp.tab->flags, tf, stream);
         QS_HIP(hipGetLastError());
     }
     field_sparse = true;
@@ -1805,7 +1825,7 @@ int qs_stage_pass2(const float *fp, const int *ip, const uint8_t *labels, float 
         qs_launch_pass2(E.batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path,
                         as<uint64_t>(E.outm), as<uint64_t>(E.shellm), as<float>(E.field), as<uint32_t>(E.negm),
                         as<uint32_t>(E.zerom), as<int>(grp.tab->offs), as<int>(grp.tab->norm), as<int>(grp.tab->end),
-                        grp.tab->flags, E.stream);
+                        grp.tab->flags, nullptr, E.stream);
         QS_HIP(hipGetLastError());
     }
     E.field_sparse = true;

@@ -30,9 +30,12 @@ int qs_pass1_mode(bool direct, bool rows);
 /* blocks of one molecule's launch over its planes [p1_lo, p1_hi) */
 uint32_t qs_pass1_blocks(const QsMol &M, int mode);
 size_t qs_pass1_lds_bytes(const QsMol &M, int mode);
+/* The slab kernel (mode 0) also writes the sign and zero words every voxel has when its pass-2
+   tile holds no shell voxel (negm, zerom), and flags the tiles that do (tflag, at each molecule's
+   tile_off); both may be NULL. */
 void qs_launch_pass1(QsBatch B, uint32_t nblocks, int mode, size_t lds_bytes, const uint32_t *cstart, const float4 *sx,
-                     const float4 *aux, uint64_t *outm,
-                     uint64_t *shellm, hipStream_t s);
+                     const float4 *aux, uint64_t *outm, uint64_t *shellm, uint64_t *negm, uint64_t *zerom, uint8_t *tflag,
+                     hipStream_t s);
 void qs_launch_pass1_direct(QsBatch B, uint32_t nblocks, const int2 *cse, const float4 *sx, uint64_t *outm, uint64_t *shellm,
                             hipStream_t s);
 
@@ -41,9 +44,12 @@ size_t qs_pass2_lds_bytes(int r);
 int qs_pass2_path(int r, bool direct);
 /* blocks of one molecule's launch over its planes [p2_lo, p2_hi) */
 uint32_t qs_pass2_blocks(const QsMol &M, int path);
+/* tflag (path 0 only, may be NULL): tiles left unflagged by field pass 1 are skipped */
 void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm, float *field,
                      uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
-                     QsNormFlags nf, hipStream_t s);
+                     QsNormFlags nf, const uint8_t *tflag, hipStream_t s);
+/* pass-2 tiles of the fixed-R kernel of one molecule (its tile-flag count) */
+uint32_t qs_pass2_tiles(const QsMol &M);
 
 /* field pass 2 stores the shell voxels' values only: the others (probe radius outside, -dx
    inside) of one molecule's field, written from its outside / shell masks */

@@ -16,6 +16,13 @@
 
 #include "qs_mc_table.h"
 
+/* field-pass-2 tiles: x, y, z extents (P2R_TX in x for the fixed-radius kernel, whose halo costs
+   (16 + 2R) / 16 in x) */
+#define P2_TX 8
+#define P2_TY 8
+#define P2_TZ 32
+#define P2R_TX 16 /* x extent of the fixed-radius kernel's tiles (halo cost (16 + 2R) / 16 in x) */
+
 __constant__ uint64_t c_tri[256] = QS_TRI_PACKED_DATA;
 
 /* cube corners and edges (see qs_mc_table.h): edge e runs from corner offset kEdgeLow[e] along
@@ -875,7 +882,9 @@ __device__ __forceinline__ void lds_or_range(uint64_t *row, int L, int R)
 #define P1S_MAXCOL 256
 __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *__restrict__ cstart_all,
                                                     const float4 *__restrict__ sx, const float4 *__restrict__ aux,
-                                                    uint64_t *__restrict__ outm_all, uint64_t *__restrict__ shellm_all)
+                                                    uint64_t *__restrict__ outm_all, uint64_t *__restrict__ shellm_all,
+                                                    uint64_t *__restrict__ negm_all, uint64_t *__restrict__ zerom_all,
+                                                    uint8_t *__restrict__ tflag)
 {
     extern __shared__ __align__(16) uint64_t p1s_dyn[];
     __shared__ uint32_t colpre[P1S_MAXCOL + 1], colst[P1S_MAXCOL];
@@ -894,12 +903,18 @@ __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *_
     if (x >= M.p1_hi) return; /* whole block */
     const int ny = min(YB, g - y0);
     const size_t wbase = ((size_t)x * g + y0) * W;
+    uint64_t *__restrict__ negm = negm_all ? negm_all + M.word_off : nullptr;
+    uint64_t *__restrict__ zerom = zerom_all ? zerom_all + M.word_off : nullptr;
     if (x >= g - 1) {
         /* the last plane is not labelled: outside */
         for (int t = threadIdx.x; t < ny * W; t += 256) {
             const int w = t % W;
             outm[wbase + t] = bits_below(g, w * 64);
             shellm[wbase + t] = 0ull;
+            if (negm) {
+                negm[wbase + t] = 0ull;
+                zerom[wbase + t] = 0ull;
+            }
         }
         return;
     }
@@ -1065,7 +1080,21 @@ __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *_
             mn = nearw[t];
         }
         outm[wbase + t] = valid & ~(mi | mn);
-        shellm[wbase + t] = mn & ~mi;
+        const uint64_t sh = mn & ~mi;
+        shellm[wbase + t] = sh;
+        if (negm) {
+            /* the sign words of a voxel without a shell voxel in its pass-2 tile: inside (-dx)
+               is negative, outside (the probe radius) is not, no field value is zero; field
+               pass 2 rewrites the tiles that hold a shell voxel, which it finds flagged here */
+            negm[wbase + t] = valid & (mi | mn);
+            zerom[wbase + t] = 0ull;
+            if (sh && tflag && M.tile_off != ~0u && x >= M.p2_lo && x < M.p2_hi) {
+                const uint32_t nbz = (uint32_t)(g + P2_TZ - 1) / P2_TZ, nby = (uint32_t)(g + P2_TY - 1) / P2_TY;
+                const uint32_t tyx = ((uint32_t)(x - M.p2_lo) / P2R_TX) * nby + (uint32_t)(y0 + yr) / P2_TY;
+                if ((uint32_t)sh) tflag[M.tile_off + tyx * nbz + 2 * w] = 1;
+                if (sh >> 32) tflag[M.tile_off + tyx * nbz + 2 * w + 1] = 1;
+            }
+        }
     }
 }
 
@@ -1139,10 +1168,6 @@ __global__ void __launch_bounds__(256) k_pass1_direct(QsBatch B, const int2 *__r
  * those candidates.  The result is the reference's value bit for bit.
  */
 
-#define P2_TX 8
-#define P2_TY 8
-#define P2_TZ 32
-#define P2R_TX 16 /* x extent of the fixed-radius kernel's tiles (halo cost (16 + 2R) / 16 in x) */
 
 /* bits [s, s + len) of a row of W mask words (bit i <-> z = s + i), zero outside [0, 64 W) */
 __device__ __forceinline__ uint64_t row_bits(const uint64_t *__restrict__ rp, int W, int s, int len)
@@ -1408,7 +1433,8 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
                                                  const uint64_t *__restrict__ shellm_all, float *__restrict__ field_all,
                                                  uint32_t *__restrict__ negm_all, uint32_t *__restrict__ zerom_all,
                                                  const int *__restrict__ offs, const int *__restrict__ norm_start,
-                                                 const int *__restrict__ enum_end, QsNormFlags nf)
+                                                 const int *__restrict__ enum_end, QsNormFlags nf,
+                                                 const uint8_t *__restrict__ tflag)
 {
     constexpr int LX = P2R_TX + 2 * R, LY = P2_TY + 2 * R, LZ = P2_TZ + 2 * R;
     constexpr int INF = 255;
@@ -1428,6 +1454,8 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     __shared__ BMask BM[LX * P2_TY * P2_TZ];
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, xcd_block(blockIdx.x, gridDim.x), lb)];
+    /* a tile without a shell voxel keeps the sign words field pass 1 wrote for it */
+    if (tflag && M.tile_off != ~0u && !tflag[M.tile_off + lb]) return;
     const QsGrid G = M.G;
     const int g = G.g, W = (int)M.W;
     const uint64_t *__restrict__ outm = outm_all + M.word_off;
@@ -2003,11 +2031,14 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
         if (x <= lim && y <= lim) nb |= (n00_up & wlt) | (n00_dn & zm1 & wle);
         pt = z00 & nb;
     }
-    S4[4 * wi + 0] = ex;
-    S4[4 * wi + 1] = ey;
-    S4[4 * wi + 2] = ez;
-    S4[4 * wi + 3] = pt;
     uint32_t nv = (uint32_t)(__popcll(ex) + __popcll(ey) + __popcll(ez) + __popcll(pt));
+    /* slot masks are read only for the words that hold vertices */
+    if (nv) {
+        S4[4 * wi + 0] = ex;
+        S4[4 * wi + 1] = ey;
+        S4[4 * wi + 2] = ez;
+        S4[4 * wi + 3] = pt;
+    }
     uint32_t nt = 0;
     if (x < lim && x < M.cube_hi && y < lim) {
         const uint64_t n11 = ld_word(N, g, W, x + 1, y + 1, w);
@@ -2914,12 +2945,14 @@ size_t qs_pass1_lds_bytes(const QsMol &M, int mode)
 }
 
 void qs_launch_pass1(QsBatch B, uint32_t nblocks, int mode, size_t lds_bytes, const uint32_t *cstart, const float4 *sx,
-                     const float4 *aux, uint64_t *outm, uint64_t *shellm, hipStream_t s)
+                     const float4 *aux, uint64_t *outm, uint64_t *shellm, uint64_t *negm, uint64_t *zerom, uint8_t *tflag,
+                     hipStream_t s)
 {
     if (mode == 1)
         hipLaunchKernelGGL(k_pass1_rows, dim3(nblocks), dim3(256), 0, s, B, cstart, sx, aux, outm, shellm);
     else
-        hipLaunchKernelGGL(k_pass1_slab, dim3(nblocks), dim3(256), lds_bytes, s, B, cstart, sx, aux, outm, shellm);
+        hipLaunchKernelGGL(k_pass1_slab, dim3(nblocks), dim3(256), lds_bytes, s, B, cstart, sx, aux, outm, shellm, negm, zerom,
+                           tflag);
 }
 
 void qs_launch_pass1_direct(QsBatch B, uint32_t nblocks, const int2 *cse, const float4 *sx, uint64_t *outm, uint64_t *shellm,
@@ -2949,13 +2982,13 @@ uint32_t qs_pass2_blocks(const QsMol &M, int path)
 /* one launch for a group of molecules sharing r (and so the path) */
 void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm, float *field,
                      uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
-                     QsNormFlags nf, hipStream_t s)
+                     QsNormFlags nf, const uint8_t *tflag, hipStream_t s)
 {
     if (path == 0) {
 #define QS_P2R(RR)                                                                                                    \
     case RR:                                                                                                          \
         hipLaunchKernelGGL(k_pass2_r<RR>, dim3(nblocks), dim3(256), 0, s, B, outm, shellm, field, negm, zerom, offs, \
-                           norm_start, enum_end, nf);                                                                 \
+                           norm_start, enum_end, nf, tflag);                                                          \
         return;
         switch (r) {
             QS_P2R(1)
@@ -3168,3 +3201,5 @@ void qs_launch_mesh_bases(const uint32_t *vbw, const uint32_t *tbw, const uint64
                        tpre);
     if (ncoarse) hipLaunchKernelGGL(k_vcoarse, dim3(nblk(ncoarse, 256)), dim3(256), 0, s, vpre, nmol, vcoarse, ncoarse);
 }
+
+uint32_t qs_pass2_tiles(const QsMol &M) { return qs_pass2_blocks(M, 0); }
