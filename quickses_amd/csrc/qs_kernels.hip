@@ -1452,6 +1452,10 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     using BMask = typename std::conditional<(2 * R + 1 <= 8), uint8_t,
                                             typename std::conditional<(2 * R + 1 <= 16), uint16_t, uint32_t>::type>::type;
     __shared__ BMask BM[LX * P2_TY * P2_TZ];
+    /* per wave, the voxels whose minimisers are still to be traced back (tx | row bit << 4 | z << 5),
+       and per own (x, y) row of the tile its sign and zero words */
+    __shared__ uint16_t tq[4][128];
+    __shared__ uint32_t sneg[P2R_TX * P2_TY], szero[P2R_TX * P2_TY];
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, xcd_block(blockIdx.x, gridDim.x), lb)];
     /* a tile without a shell voxel keeps the sign words field pass 1 wrote for it */
@@ -1654,68 +1658,106 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     }
     __syncthreads();
 
-    /* x, one (ty, tz) column per thread, then the field */
-    const float vy = grid_pos(G.oy, gy, dx), vz = grid_pos(G.oz, gz, dx);
-    /* voxel tx of the thread's column: its field value from the minimal norm nmin and the x
-       offsets attaining it (amask, bit a + R; used when nmin is traced back), then its stores */
-    auto finish = [&](int tx, int nmin, uint32_t amask) {
+    /* x, one (ty, tz) column per thread, in two phases.  Phase 1: each voxel's minimal squared
+       norm over its x window; a voxel with nothing to trace back gets its field value and sign
+       bits at once, the others join the wave's queue.  Phase 2: the wave's 64 lanes take queued
+       voxels 64 at a time (so lanes whose own voxels need nothing do not idle), find the x
+       offsets attaining the minimum again, trace the minimisers back through the y masks and
+       take the exact float distance.  The sign and zero words gather in LDS and are stored once
+       per row at the end. */
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint16_t *__restrict__ queue = tq[wid];
+    int qn = 0;
+    auto trace_items = [&](int count) {
+        if (lane >= count) return;
+        const uint32_t it = queue[lane];
+        const int tx = (int)(it & 15u), ty_ = 2 * wid + (int)((it >> 4) & 1u), tz_ = (int)(it >> 5);
+        const int gx = tx0 + tx, gy_ = ty0 + ty_, gz_ = tz0 + tz_;
+        /* the x offsets (bit a + R) attaining the minimal norm */
+        int nmin = INF;
+        uint32_t amask = 0;
+#pragma unroll
+        for (int a = 0; a <= 2 * R; a++) nmin = min(nmin, (int)DY[((tx + a) * P2_TY + ty_) * P2_TZ + tz_] + (a - R) * (a - R));
+#pragma unroll
+        for (int a = 0; a <= 2 * R; a++)
+            amask |= (uint32_t)((int)DY[((tx + a) * P2_TY + ty_) * P2_TZ + tz_] + (a - R) * (a - R) == nmin) << a;
+        const float vx = grid_pos(G.ox, gx, dx), vy = grid_pos(G.oy, gy_, dx), vz = grid_pos(G.oz, gz_, dx);
+        /* the smallest squared distance, rooted once at the end: sqrtf rounds correctly and is
+           monotone, so the root of the minimum is the minimum of the roots */
+        float min_sq = __builtin_inff();
+        if ((nfw[nmin >> 5] >> (nmin & 31)) & 1u) {
+            /* trace the minimisers back through the separable passes: the x offsets, then their y
+               offsets from the recorded masks */
+            while (amask) {
+                const int a = __builtin_ctz(amask) - R;
+                amask &= amask - 1u;
+                const int dy = nmin - a * a;
+                uint32_t bmask = BM[((tx + R + a) * P2_TY + ty_) * P2_TZ + tz_];
+                while (bmask) {
+                    const int b = __builtin_ctz(bmask) - R;
+                    bmask &= bmask - 1u;
+                    /* dz = c^2 exactly, c <= R: the hardware root is within an ulp of c */
+                    const int dz = dy - b * b;
+                    const int c = (int)(__builtin_amdgcn_sqrtf((float)dz) + 0.5f);
+                    const uint64_t m = rows[(tx + R + a) * LY + (ty_ + R + b)];
+                    const float qx = grid_pos(G.ox, gx + a, dx), qy = grid_pos(G.oy, gy_ + b, dx);
+                    if ((m >> (tz_ + R + c)) & 1ull)
+                        min_sq = fminf(dist3_sq(qx, qy, grid_pos(G.oz, gz_ + c, dx), vx, vy, vz), min_sq);
+                    if (c && ((m >> (tz_ + R - c)) & 1ull))
+                        min_sq = fminf(dist3_sq(qx, qy, grid_pos(G.oz, gz_ - c, dx), vx, vy, vz), min_sq);
+                }
+            }
+        } else {
+#pragma unroll 1
+            for (int k = norm_start[nmin]; k < enum_end[nmin]; k++) {
+                int o = offs[k];
+                int a = (int)(int8_t)(o & 0xFF), b = (int)(int8_t)((o >> 8) & 0xFF), c = (int)(int8_t)((o >> 16) & 0xFF);
+                uint64_t m = rows[(tx + R + a) * LY + (ty_ + R + b)];
+                if (!((m >> (tz_ + R + c)) & 1ull)) continue;
+                float d = dist3_sq(grid_pos(G.ox, gx + a, dx), grid_pos(G.oy, gy_ + b, dx), grid_pos(G.oz, gz_ + c, dx), vx, vy,
+                                   vz);
+                min_sq = fminf(d, min_sq);
+            }
+        }
+        const float min_dist = sqrt_rn(min_sq);
+        const float res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
+        field[((size_t)gx * g + gy_) * g + gz_] = res;
+        if (res < 0.0f) atomicOr(&sneg[tx * P2_TY + ty_], 1u << tz_);
+        if (res == 0.0f) atomicOr(&szero[tx * P2_TY + ty_], 1u << tz_);
+    };
+    /* voxel tx of the thread's column, whose minimal norm is nmin (INF: no outside voxel within
+       the window): settled now, or queued */
+    auto handle = [&](int tx, int nmin) {
         const int gx = tx0 + tx;
         const bool out = (rows[(tx + R) * LY + (ty + R)] >> (tz + R)) & 1ull;
         const bool shell = (own[tx * P2_TY + ty] >> tz) & 1u; /* never outside */
-        float res = out ? QS_PROBE : -dx;
-        if (vyz && shell && nmin < INF) {
-            const float vx = grid_pos(G.ox, gx, dx);
-            /* the smallest squared distance, rooted once at the end: sqrtf rounds correctly and
-               is monotone, so the root of the minimum is the minimum of the roots */
-            float min_sq = __builtin_inff();
-            if ((nfw[nmin >> 5] >> (nmin & 31)) & 1u) {
-                /* trace the minimisers back through the separable passes: the x offsets, then
-                   their y offsets from the recorded masks */
-                while (amask) {
-                    const int a = __builtin_ctz(amask) - R;
-                    amask &= amask - 1u;
-                    const int dy = nmin - a * a;
-                    uint32_t bmask = BM[((tx + R + a) * P2_TY + ty) * P2_TZ + tz];
-                    while (bmask) {
-                        const int b = __builtin_ctz(bmask) - R;
-                        bmask &= bmask - 1u;
-                        /* dz = c^2 exactly, c <= R: the hardware root is within an ulp of c */
-                        const int dz = dy - b * b;
-                        const int c = (int)(__builtin_amdgcn_sqrtf((float)dz) + 0.5f);
-                        const uint64_t m = rows[(tx + R + a) * LY + (ty + R + b)];
-                        const float qx = grid_pos(G.ox, gx + a, dx), qy = grid_pos(G.oy, gy + b, dx);
-                        if ((m >> (tz + R + c)) & 1ull)
-                            min_sq = fminf(dist3_sq(qx, qy, grid_pos(G.oz, gz + c, dx), vx, vy, vz), min_sq);
-                        if (c && ((m >> (tz + R - c)) & 1ull))
-                            min_sq = fminf(dist3_sq(qx, qy, grid_pos(G.oz, gz - c, dx), vx, vy, vz), min_sq);
-                    }
-                }
-            } else {
-#pragma unroll 1
-                for (int k = norm_start[nmin]; k < enum_end[nmin]; k++) {
-                    int o = offs[k];
-                    int a = (int)(int8_t)(o & 0xFF), b = (int)(int8_t)((o >> 8) & 0xFF), c = (int)(int8_t)((o >> 16) & 0xFF);
-                    uint64_t m = rows[(tx + R + a) * LY + (ty + R + b)];
-                    if (!((m >> (tz + R + c)) & 1ull)) continue;
-                    float d = dist3_sq(grid_pos(G.ox, gx + a, dx), grid_pos(G.oy, gy + b, dx), grid_pos(G.oz, gz + c, dx),
-                                       vx, vy, vz);
-                    min_sq = fminf(d, min_sq);
-                }
-            }
-            const float min_dist = sqrt_rn(min_sq);
-            res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
+        const bool traced = vyz && shell && nmin < INF;
+        /* outside: the probe radius; inside, or shell without an outside voxel in reach: -dx */
+        if (vyz && shell && !traced) field[((size_t)gx * g + gy) * g + gz] = -dx;
+        const uint64_t nb = __ballot(vyz && !traced && !out);
+        if ((lane & 31) == 0) {
+            sneg[tx * P2_TY + ty] = (uint32_t)(lane ? (nb >> 32) : nb);
+            szero[tx * P2_TY + ty] = 0u;
         }
-        if (vyz && shell) field[((size_t)gx * g + gy) * g + gz] = res;
-        emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, res);
+        const uint64_t tm = __ballot(traced);
+        if (traced) queue[qn + __popcll(tm & ((1ull << lane) - 1ull))] = (uint16_t)(tx | ((lane >> 5) << 4) | (tz << 5));
+        qn += __popcll(tm);
+        if (qn >= 64) {
+            wave_lds_sync();
+            trace_items(64);
+            wave_lds_sync();
+            const int rest = qn - 64;
+            const uint16_t mv = (lane < rest) ? queue[64 + lane] : (uint16_t)0;
+            wave_lds_sync();
+            if (lane < rest) queue[lane] = mv;
+            wave_lds_sync();
+            qn = rest;
+        }
     };
 
     if constexpr (2 * R + 1 <= 16) {
-        /* two x outputs per 32-bit register: the minimum over the 2R+1 taps and the mask of the
-           taps attaining it, as in the y pass */
+        /* two x outputs per 32-bit register: the minimum over the 2R+1 taps (16-bit lanes) */
         typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
-        u16x2 pw[4] = {{2, 2}, {4, 4}, {16, 16}, {256, 256}};
-#pragma unroll
-        for (int i = 0; i < 4; i++) asm volatile("" : "+v"(pw[i]));
         u16x2 U[LX - 1];
         {
             uint32_t u[LX];
@@ -1731,33 +1773,23 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
             if (gx >= x1 || gx >= g) break;
             const bool two = gx + 1 < x1 && gx + 1 < g;
             const uint32_t sh = ((own[tx * P2_TY + ty] >> tz) & 1u) | (two ? ((own[(tx + 1) * P2_TY + ty] >> tz) & 1u) << 1 : 0u);
-            uint32_t nm = (uint32_t)INF * 0x10001u, am = 0u;
+            uint32_t nm = (uint32_t)INF * 0x10001u;
             if (vyz && sh) {
                 constexpr int NT = 2 * R + 1;
-                u16x2 sb[NT], t[NT];
+                u16x2 t[NT];
 #pragma unroll
                 for (int a = 0; a < NT; a++) {
                     const uint16_t c = (uint16_t)((a - R) * (a - R));
-                    sb[a] = U[tx + a] + (u16x2){c, c};
-                    t[a] = sb[a];
+                    t[a] = U[tx + a] + (u16x2){c, c};
                 }
 #pragma unroll
-                for (int s = 1; s < NT; s *= 2)
+                for (int s2 = 1; s2 < NT; s2 *= 2)
 #pragma unroll
-                    for (int a = 0; a + s < NT; a += 2 * s) t[a] = __builtin_elementwise_min(t[a], t[a + s]);
-                const u16x2 best = t[0];
-                const u16x2 best1 = best + (u16x2){1, 1};
-#pragma unroll
-                for (int a = 0; a < NT; a++) t[a] = __builtin_elementwise_sub_sat(best1, sb[a]);
-#pragma unroll
-                for (int s = 1, l = 0; s < NT; s *= 2, l++)
-#pragma unroll
-                    for (int a = 0; a + s < NT; a += 2 * s) t[a] = t[a + s] * pw[l] + t[a];
-                nm = __builtin_bit_cast(uint32_t, best);
-                am = __builtin_bit_cast(uint32_t, t[0]);
+                    for (int a = 0; a + s2 < NT; a += 2 * s2) t[a] = __builtin_elementwise_min(t[a], t[a + s2]);
+                nm = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(t[0], (u16x2){(uint16_t)INF, (uint16_t)INF}));
             }
-            finish(tx, (int)(nm & 0xFFFFu), am & 0xFFFFu);
-            if (two) finish(tx + 1, (int)(nm >> 16), am >> 16);
+            handle(tx, (int)(nm & 0xFFFFu));
+            if (two) handle(tx + 1, (int)(nm >> 16));
         }
     } else {
         int u[LX];
@@ -1768,14 +1800,26 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
             const int gx = tx0 + tx;
             if (gx >= x1 || gx >= g) break;
             int nmin = INF;
-            uint32_t amask = 0;
             if (vyz && ((own[tx * P2_TY + ty] >> tz) & 1u)) {
 #pragma unroll
                 for (int a = 0; a <= 2 * R; a++) nmin = min(nmin, u[tx + a] + (a - R) * (a - R));
-#pragma unroll
-                for (int a = 0; a <= 2 * R; a++) amask |= (uint32_t)(u[tx + a] + (a - R) * (a - R) == nmin) << a;
             }
-            finish(tx, nmin, amask);
+            handle(tx, nmin);
+        }
+    }
+    if (qn > 0) {
+        wave_lds_sync();
+        trace_items(qn);
+    }
+    wave_lds_sync();
+    /* the wave's sign and zero words: lane = x (0-15) | row bit << 4 | zero word << 5 */
+    {
+        const int tx = lane & 15, ty_ = 2 * wid + ((lane >> 4) & 1);
+        const int gx = tx0 + tx, gy_ = ty0 + ty_;
+        if (gx < x1 && gx < g && gy_ < g) {
+            const size_t w32 = ((size_t)gx * g + gy_) * (2 * (size_t)W) + (tz0 >> 5);
+            if (lane < 32) negm[w32] = sneg[tx * P2_TY + ty_];
+            else zerom[w32] = szero[tx * P2_TY + ty_];
         }
     }
 }
