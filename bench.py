@@ -39,12 +39,6 @@ def algorithmic_bytes(stage, V, N, ncell, S):
     return None
 
 
-def dense_field_bytes(V):
-    """Bytes of the same field pass with a dense f32 field store (4 B per voxel) plus the 2-bit
-    masks in and out: the form whose stores pass 2 avoids."""
-    return V * 9 // 2
-
-
 def shell_voxels(qs, g):
     """Shell voxels of the last single-molecule computation, from its shell mask."""
     import numpy as np
@@ -118,7 +112,7 @@ def cpu_baseline(pos, rad, voxel, target_s=25.0):
                        "(%d voxels, %d triangles, %.1f s)" % (planes, g, v, nt, t))
 
 
-def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_000_000):
+def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=None):
     """meshes/s of the plain path (one call per molecule) against the batched path (one launch
     sequence per chunk of molecules) on the two streams of the batched workload, at two
     boundaries: device-resident (atoms already in HBM, meshes left in HBM) and host arrays in /
@@ -131,7 +125,10 @@ def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_0
 
     from quickses_amd.workloads import chunks_by_voxels, config4_sets, estimated_voxels
 
+    if budget is None:  # grid points per launch sequence: the tuned value, else 1.5 G
+        budget = int(qs.tuning().get("batch_voxels", {}).get("%g" % voxel, 1_500_000_000))
     mols, frames = config4_sets(qs, 256, n_frames)
+    link = hipmem.d2h_rate()  # device-to-host rate into pinned memory: the ceiling of host-to-host results
     res = {}
     for name, items in (("molecules_256", mols), ("frames_%d" % n_frames, frames)):
         vox = sum(estimated_voxels(p, r, voxel) for p, r in items)
@@ -178,10 +175,12 @@ def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_0
         for _ in qs.compute_batches_host(host_chunks, voxel, smooth):
             pass
         t0 = time.perf_counter()
-        nv_b = 0
+        nv_b = nt_b = 0
         for out in qs.compute_batches_host(host_chunks, voxel, smooth):
             nv_b += sum(len(o[0]) for o in out)
+            nt_b += sum(len(o[1]) for o in out)
         tb = time.perf_counter() - t0
+        d2h = 16 * nv_b + 12 * nt_b  # vertices and atom ids, triangles
         big = max(range(len(items)), key=lambda m: counts[m])
         qs.compute_ses(items[big][0], items[big][1], voxel, smooth, copy=False)
         t0 = time.perf_counter()
@@ -195,13 +194,15 @@ def bench_batched(qs, hipmem, voxel=0.5, smooth=1, n_frames=1000, budget=1_500_0
         n = len(items)
         print("batched stream %s: %d meshes, device-resident plain %.3f s batched %.3f s, host-to-host plain %.3f s "
               "batched %.3f s" % (name, n, tpd, tbd, tp, tb), file=sys.stderr, flush=True)
-        res[name] = dict(meshes=n, voxels=vox, launch_sequences=len(chunks), host_launch_sequences=len(host_chunks),
+        res[name] = dict(meshes=n, voxels=vox, budget_voxels=budget, launch_sequences=len(chunks), host_launch_sequences=len(host_chunks),
                          vertices=nv_b,
                          first_sequence=dict(meshes=bounds[0][1] - bounds[0][0], stage_ms=chunk_stage_ms),
                          device_resident=dict(plain_meshes_per_s=round(n / tpd, 2), batched_meshes_per_s=round(n / tbd, 2),
                                               batched_voxels_per_s=round(vox / tbd, 1), speedup=round(tpd / tbd, 3)),
                          host_to_host=dict(plain_meshes_per_s=round(n / tp, 2), batched_meshes_per_s=round(n / tb, 2),
-                                           batched_voxels_per_s=round(vox / tb, 1), speedup=round(tp / tb, 3)))
+                                           batched_voxels_per_s=round(vox / tb, 1), speedup=round(tp / tb, 3),
+                                           d2h_bytes=d2h, link_GBps=round(link, 2),
+                                           link_bound_meshes_per_s=round(n / (d2h / (link * 1e9)), 2)))
     streams = list(res.values())
     for key in ("device_resident", "host_to_host"):
         tot = sum(v["meshes"] for v in streams)
@@ -262,13 +263,16 @@ def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=
     reqs = config5_requests(qs, k)
     smooth = 1
 
+    # the plain leg copies each request's atoms into device buffers allocated once (as large as the
+    # largest request) and meshes it with qs_compute_device, mesh left in HBM: the same boundary
+    # as the server's, without an allocation per request
+    nmax = max(len(r[0][1]) for r in reqs)
+    d_pos, d_rad = hipmem.alloc(12 * nmax), hipmem.alloc(4 * nmax)
+
     def plain_one(pos, rad, reso):
-        dp, dr = hipmem.DeviceArray(pos), hipmem.DeviceArray(rad)
-        try:
-            qs.compute_device(dp.ptr, dr.ptr, len(rad), reso, smooth)
-        finally:
-            dp.free()
-            dr.free()
+        hipmem.upload(d_pos, pos)
+        hipmem.upload(d_rad, rad)
+        qs.compute_device(d_pos, d_rad, len(rad), reso, smooth)
 
     plain_one(reqs[0][0][0], reqs[0][0][1], reqs[0][1])  # warm-up
     t0 = time.perf_counter()
@@ -334,6 +338,8 @@ def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=
             lat_sat.append(srv.poll(rid, -1)["latency_ms"])
             srv.release(rid)
         wall_sat = time.perf_counter() - start
+    hipmem.free(d_pos)
+    hipmem.free(d_rad)
     return dict(
         requests=k, arrival_rate_per_s=round(lam, 2), load=load,
         plain_capacity_meshes_per_s=round(capacity, 2),
@@ -346,7 +352,8 @@ def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=
                        speedup=round(wall_sat and (k / wall_sat) / capacity, 3), served_p50_ms=_pct(lat_sat, 50),
                        served_p99_ms=_pct(lat_sat, 99)),
         workload="config-5 stream: atom counts log-uniform 1,000-200,000, voxel {0.3, 0.5, 1.0} A, 1 Laplacian "
-                 "pass, Poisson arrivals; latency = arrival -> mesh resident in HBM",
+                 "pass, Poisson arrivals; latency = arrival -> mesh resident in HBM; plain leg: atoms copied into "
+                 "device buffers allocated once, qs_compute_device per request",
     )
 
 
@@ -360,8 +367,9 @@ def main():
     ap.add_argument("--frames", type=int, default=1000, help="trajectory frames of the batched workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the served stream")
     ap.add_argument("--requests", type=int, default=400, help="requests of the served stream")
-    ap.add_argument("--serve-workers", type=int, default=2)
-    ap.add_argument("--serve-cap", type=int, default=150_000_000, help="grid points per served batch (0 = no cap)")
+    ap.add_argument("--serve-workers", type=int, default=None, help="server engines (default: tuned, else 2)")
+    ap.add_argument("--serve-cap", type=int, default=None,
+                    help="grid points per served batch, 0 = no cap (default: tuned, else 150 M)")
     ap.add_argument("--no-other", action="store_true", help="skip configs 1 and 3")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -373,6 +381,12 @@ def main():
 
     import quickses_amd as qs
     from quickses_amd import hipmem
+
+    tuned = qs.tuning().get("serve", {})
+    if args.serve_workers is None:
+        args.serve_workers = int(tuned.get("workers", 2))
+    if args.serve_cap is None:
+        args.serve_cap = int(tuned.get("max_batch_voxels", 150_000_000))
 
     if hipmem.device_count() < 1:
         print("no HIP device", file=sys.stderr)
@@ -419,25 +433,32 @@ def main():
     finally:
         hipmem.free(d_nrm)
 
-    # dominant single-kernel stage against the HBM roofline
+    # dominant kernel (field pass 2) against the HBM roofline.  achieved = the field pass's
+    # algorithmic bytes as stated for a field pass whose two halves meet on chip: 4 B per grid
+    # voxel (a dense f32 field over g^3 voxels) in the pass-2 kernel's time.  own_bytes: what this
+    # pass-2 kernel must move in its own form (2-bit masks in and out, 4 B per shell voxel).
     S = shell_voxels(qs, g)
     ab = lambda k: algorithmic_bytes(k, V, W["atoms"], gn ** 3, S)
-    kern = {k: v for k, v in stage_avg.items() if ab(k) is not None}
-    dom = max(kern, key=kern.get)
-    abytes = ab(dom)
-    achieved = abytes / (kern[dom] * 1e-3) / 1e9
+    t2 = stage_avg["pass2"]
+    dom = max(("pass1", "pass2"), key=lambda k: stage_avg[k])
+    v4 = 4 * V
+    achieved = v4 / (t2 * 1e-3) / 1e9
+    own = ab("pass2")
+    own_gbs = own / (t2 * 1e-3) / 1e9
     copy_gbs = copy_bandwidth(qs, hipmem)
-    fp_bytes = ab("pass1") + ab("pass2")
     fp_ms = stage_avg["pass1"] + stage_avg["pass2"]
-    roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=profiled_traffic(dom), avg_ms=round(kern[dom], 4),
-                    valu_wave_instr_per_cu_cycle=profiled_valu_rate(dom),
-                    algorithmic_bytes=abytes, shell_voxels=S, copy_kernel_GBps=round(copy_gbs, 1),
-                    frac_of_copy=round(achieved / copy_gbs, 5),
-                    field_pass=dict(kernels="pass1 + pass2", algorithmic_bytes=fp_bytes, ms=round(fp_ms, 4),
-                                    achieved=round(fp_bytes / (fp_ms * 1e-3) / 1e9, 2)),
-                    dense_field_equivalent=dict(bytes=dense_field_bytes(V),
-                                                GBps=round(dense_field_bytes(V) / (stage_avg["pass2"] * 1e-3) / 1e9, 2)))
+    roofline = dict(bound="hbm", kernel="pass2", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=profiled_traffic("pass2"), avg_ms=round(t2, 4),
+                    algorithmic_bytes=v4, bytes_definition="4 B per grid voxel (g^3): the field pass with its two halves "
+                                                           "meeting on chip",
+                    dominant_stage=dom, valu_wave_instr_per_cu_cycle=profiled_valu_rate("pass2"),
+                    own_bytes=dict(bytes=own, definition="2-bit outside/shell masks in, 2-bit sign/zero masks out, "
+                                                         "4 B per shell voxel", shell_voxels=S, GBps=round(own_gbs, 2),
+                                   frac=round(own_gbs / HBM_PEAK_GBS, 5)),
+                    copy_kernel_GBps=round(copy_gbs, 1), frac_of_copy=round(achieved / copy_gbs, 5),
+                    field_pass=dict(kernels="pass1 + pass2", algorithmic_bytes=v4, ms=round(fp_ms, 4),
+                                    achieved=round(v4 / (fp_ms * 1e-3) / 1e9, 2),
+                                    frac=round(v4 / (fp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)))
 
     # host-in / host-out rate (PCIe included) of the reference-compatible entry point, results in
     # the library's host arrays (one untimed call first: they grow to the mesh's size once)

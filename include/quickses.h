@@ -78,6 +78,17 @@ const char *qs_last_error_message(void);
 int qs_set_option(const char *key, int value);
 int qs_get_option(const char *key);
 
+/* Tuning (no reference counterpart; the reference has no tunables): per voxel size, rows per
+ * field-pass-1 block ("pass1_slab_rows"; an upper bound under the kernel's memory limits).
+ * qs_set_tuning sets one entry (returns 0, -1 for an unknown key or a full table);
+ * qs_load_tuning reads the "pass1_slab_rows" object of a JSON file such as the tuning.json that
+ * quickses_amd/autotune.py writes (returns the entries read, a negative error code when the file
+ * cannot be read; NULL clears the table).  The
+ * library reads tuning.json beside itself once per process on first use (environment variable
+ * QS_TUNING_FILE names another file, "none" none). */
+int qs_set_tuning(const char *key, float reso, int value);
+int qs_load_tuning(const char *path);
+
 /* Device-resident form of API_computeSES: d_pos (3*N floats) and d_rad (N floats) are device
  * pointers on the current HIP device; the mesh stays on the device in buffers owned by the
  * library, valid until the next call.  *nvert, *nidx as for API_computeSES.  Returns 0 or an

@@ -77,6 +77,10 @@ def lib():
     L.qs_last_grid.argtypes = [f3p, ctypes.POINTER(ctypes.c_int)]
     L.qs_last_slab_count.argtypes = []
     L.qs_last_slab_count.restype = ctypes.c_int
+    L.qs_set_tuning.argtypes = [ctypes.c_char_p, ctypes.c_float, ctypes.c_int]
+    L.qs_set_tuning.restype = ctypes.c_int
+    L.qs_load_tuning.argtypes = [ctypes.c_char_p]
+    L.qs_load_tuning.restype = ctypes.c_int
     L.qs_batch_grid.argtypes = [ctypes.c_int, f3p, ctypes.POINTER(ctypes.c_int)]
     L.qs_batch_grid.restype = ctypes.c_int
     L.qs_stage_times.argtypes = [ctypes.c_char_p, ctypes.c_int, f3p, ctypes.c_int]
@@ -547,6 +551,34 @@ def last_grid():
     ip = (ctypes.c_int * 4)()
     L.qs_last_grid(fp, ip)
     return dict(origin=(fp[0], fp[1], fp[2]), dx=fp[3], dxn=fp[4], g=ip[0], gn=ip[1], r=ip[2], lim=ip[3])
+
+
+TUNING_PATH = os.path.join(_HERE, "tuning.json")
+
+
+def set_tuning(key, reso, value):
+    """qs_set_tuning: one tuned value per voxel size (key "pass1_slab_rows")."""
+    if lib().qs_set_tuning(key.encode(), ctypes.c_float(reso), int(value)) != 0:
+        raise QuickSESError("tuning %s at %g A not set" % (key, reso))
+
+
+def load_tuning(path=TUNING_PATH):
+    """qs_load_tuning: the library's tuned values from a tuning file (None clears them); returns
+    the entries read."""
+    n = lib().qs_load_tuning(path.encode() if path else None)
+    if n < 0:
+        raise QuickSESError(lib().qs_last_error_message().decode())
+    return n
+
+
+def tuning(path=TUNING_PATH):
+    """The tuning file's contents as a dict ({} when there is none)."""
+    import json
+
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
 
 
 def last_slab_count():

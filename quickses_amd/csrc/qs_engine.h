@@ -96,6 +96,11 @@ void bbox_host(const float *pos, const float *rad, unsigned n, BBox &b, int &non
 
 void grid_from_bbox(const BBox &b, float reso, QsGrid &G);
 
+/* the tuning cache: pass-1 rows per voxel size from a JSON file (entries read, -1 unreadable),
+   and the one beside the library, read once per process */
+int load_tuning_file(const char *path);
+void load_default_tuning();
+
 /* ------------------------------------------------------------------------------------------ */
 /* pass-2 candidate tables: window offsets sorted by squared norm, and for each minimal norm the
    end of the range of norms whose float distance can still be the minimum                      */
@@ -341,6 +346,7 @@ struct Engine {
 
     int init()
     {
+        load_default_tuning();
         int dev = 0;
         QS_HIP(hipGetDevice(&dev));
         if (device == dev && stream) return 0;

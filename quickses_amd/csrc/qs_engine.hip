@@ -13,6 +13,7 @@
  * vertices and triangles (QsMol), and each launch maps its blocks to molecules by a prefix of
  * per-molecule block counts (QsBatch).
  */
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -79,6 +80,61 @@ void release(DevBuf &b)
     if (b.p && !g_arena) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
+}
+
+/* The tuning cache (tuning.json, written by quickses_amd/autotune.py): its "pass1_slab_rows" object,
+   voxel size -> rows per field-pass-1 block, e.g. {"0.2": 64, "0.5": 32}.  Returns the number of
+   entries read, or -1 when the file cannot be read. */
+int load_tuning_file(const char *path)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    std::string text;
+    char buf[4096];
+    size_t k;
+    while ((k = fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, k);
+    fclose(f);
+    const size_t key = text.find("\"pass1_slab_rows\"");
+    if (key == std::string::npos) return 0;
+    const size_t open = text.find('{', key), close = text.find('}', key);
+    if (open == std::string::npos || close == std::string::npos || close < open) return 0;
+    int n = 0;
+    size_t at = open + 1;
+    while (at < close) {
+        const size_t q0 = text.find('"', at);
+        if (q0 == std::string::npos || q0 >= close) break;
+        const size_t q1 = text.find('"', q0 + 1);
+        const size_t colon = text.find(':', q1);
+        if (q1 == std::string::npos || colon == std::string::npos || colon >= close) break;
+        const float dx = strtof(text.c_str() + q0 + 1, nullptr);
+        const int rows = (int)strtol(text.c_str() + colon + 1, nullptr, 10);
+        if (dx > 0.0f && rows > 0 && qs_set_tuned_rows(dx, rows) == 0) n++;
+        at = text.find(',', colon);
+        if (at == std::string::npos) break;
+        at++;
+    }
+    return n;
+}
+
+/* once per process: the tuning cache beside the library (QS_TUNING_FILE names another file;
+   "none" reads none) */
+void load_default_tuning()
+{
+    static bool done = false;
+    if (done) return;
+    done = true;
+    const char *env = getenv("QS_TUNING_FILE");
+    if (env && !strcmp(env, "none")) return;
+    if (env && *env) {
+        (void)load_tuning_file(env);
+        return;
+    }
+    Dl_info info;
+    if (!dladdr((void *)&load_default_tuning, &info) || !info.dli_fname) return;
+    std::string p = info.dli_fname;
+    const size_t slash = p.rfind('/');
+    p = (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) + "/tuning.json";
+    (void)load_tuning_file(p.c_str());
 }
 
 void bbox_host(const float *pos, const float *rad, unsigned n, BBox &b, int &nonfinite)
@@ -1214,6 +1270,25 @@ extern "C" {
 
 int API_lastError(void) { return g_err; }
 const char *qs_last_error_message(void) { return g_errmsg; }
+
+int qs_load_tuning(const char *path)
+{
+    g_err = 0;
+    if (!path) {
+        qs_set_tuned_rows(0.0f, 0);
+        return 0;
+    }
+    const int n = load_tuning_file(path);
+    if (n < 0) return -set_error(3, "tuning file %s cannot be read", path);
+    return n;
+}
+
+int qs_set_tuning(const char *key, float reso, int value)
+{
+    std::string k = key ? key : "";
+    if (k == "pass1_slab_rows") return qs_set_tuned_rows(reso, value);
+    return -1;
+}
 
 int qs_set_option(const char *key, int value)
 {

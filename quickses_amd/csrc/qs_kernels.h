@@ -25,6 +25,9 @@ void qs_launch_gather_aux(QsBatch B, uint32_t nblocks, const int *perm, const fl
 /* field pass 1 */
 int qs_pass1_rows_per_wave(QsGrid G);
 int qs_pass1_slab_rows(QsGrid G, int W);
+/* rows per pass-1 slab block at voxel size dx (an upper bound under the LDS and candidate-column
+   limits); dx <= 0 clears the table; -1 when it is full */
+int qs_set_tuned_rows(float dx, int rows);
 /* field pass 1 kernel: 0 slab scatter (default), 1 row gather, 2 per-voxel */
 int qs_pass1_mode(bool direct, bool rows);
 /* blocks of one molecule's launch over its planes [p1_lo, p1_hi) */

@@ -12,6 +12,7 @@
 #include "qs_common.h"
 #include "qs_kernels.h"
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "qs_mc_table.h"
@@ -2961,11 +2962,37 @@ int qs_pass1_rows_per_wave(QsGrid G)
     return R < 1 ? 1 : (R > P1_RMAX ? P1_RMAX : R);
 }
 
+/* tuned rows per pass-1 block by voxel size (qs_set_tuned_rows): a short list, looked up by the
+   nearest voxel size within 1 % */
+static float g_rows_dx[16];
+static int g_rows_v[16];
+static int g_rows_n = 0;
+
+int qs_set_tuned_rows(float dx, int rows)
+{
+    if (!(dx > 0.0f)) {
+        g_rows_n = 0; /* dx <= 0 clears the table */
+        return 0;
+    }
+    for (int i = 0; i < g_rows_n; i++)
+        if (std::fabs(g_rows_dx[i] - dx) <= 0.01f * dx) {
+            g_rows_v[i] = rows;
+            return 0;
+        }
+    if (g_rows_n == 16) return -1;
+    g_rows_dx[g_rows_n] = dx;
+    g_rows_v[g_rows_n++] = rows;
+    return 0;
+}
+
 int qs_pass1_slab_rows(QsGrid G, int W)
 {
-    /* up to 64 rows; both masks in at most 32 KB of LDS; at most 80 neighbour cells of y per
-       slab, so the candidate columns (3 x cells times the y cells + 2) stay within 256 */
+    /* up to 64 rows (or the tuned count for this voxel size); both masks in at most 32 KB of LDS;
+       at most 80 neighbour cells of y per slab, so the candidate columns (3 x cells times the y
+       cells + 2) stay within 256 */
     int yb = P1S_MAXROWS;
+    for (int i = 0; i < g_rows_n; i++)
+        if (std::fabs(g_rows_dx[i] - G.dx) <= 0.01f * G.dx && g_rows_v[i] >= 1) yb = std::min(yb, g_rows_v[i]);
     yb = std::min(yb, std::max(1, 2048 / std::max(W, 1)));
     yb = std::min(yb, std::max(1, (int)(80.0f * G.dxn / G.dx)));
     return yb;

@@ -26,6 +26,8 @@ def hip():
         h.hipDeviceSynchronize.argtypes = []
         h.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
         h.hipGetErrorString.restype = ctypes.c_char_p
+        h.hipHostMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t, ctypes.c_uint]
+        h.hipHostFree.argtypes = [vp]
         _hip = h
     return _hip
 
@@ -85,3 +87,31 @@ def download(ptr, shape, dtype):
     if out.nbytes:
         _ck(hip().hipMemcpy(out.ctypes.data, ptr, out.nbytes, D2H), "hipMemcpy D2H")
     return out
+
+
+def upload(ptr, host):
+    """Copy a numpy array into an existing device allocation (synchronous)."""
+    host = np.ascontiguousarray(host)
+    if host.nbytes:
+        _ck(hip().hipMemcpy(ptr, host.ctypes.data, host.nbytes, H2D), "hipMemcpy H2D")
+
+
+def d2h_rate(nbytes=1 << 30, reps=3):
+    """GB/s of a device-to-host copy into pinned host memory (the link's rate for result copies)."""
+    import time
+
+    d = alloc(nbytes)
+    h = ctypes.c_void_p()
+    _ck(hip().hipHostMalloc(ctypes.byref(h), nbytes, 0), "hipHostMalloc")
+    try:
+        _ck(hip().hipMemcpy(h.value, d, nbytes, D2H), "hipMemcpy D2H")
+        best = None
+        for _ in range(reps):
+            t = time.perf_counter()
+            _ck(hip().hipMemcpy(h.value, d, nbytes, D2H), "hipMemcpy D2H")
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+    finally:
+        hip().hipHostFree(h)
+        free(d)
+    return nbytes / best / 1e9
