@@ -879,7 +879,8 @@ __device__ __forceinline__ void lds_or_range(uint64_t *row, int L, int R)
    to the voxels whose z cell admits the atom, so each voxel sees exactly the atoms of its 27
    neighbour cells, as in probeIntersection (Kernels.cu:394-475); inside wins over near, which
    makes the result independent of the order of the ORs. */
-#define P1S_MAXROWS 64
+#define P1S_MAXROWS 256   /* rows per slab block at most (row indexes in the queue are 8-bit) */
+#define P1S_DEFROWS 64    /* rows per slab block when no tuned count is set */
 #define P1S_MAXCOL 256
 __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *__restrict__ cstart_all,
                                                     const float4 *__restrict__ sx, const float4 *__restrict__ aux,
@@ -2987,12 +2988,12 @@ int qs_set_tuned_rows(float dx, int rows)
 
 int qs_pass1_slab_rows(QsGrid G, int W)
 {
-    /* up to 64 rows (or the tuned count for this voxel size); both masks in at most 32 KB of LDS;
+    /* 64 rows (or the tuned count for this voxel size, up to 256); both masks in at most 32 KB of LDS;
        at most 80 neighbour cells of y per slab, so the candidate columns (3 x cells times the y
        cells + 2) stay within 256 */
-    int yb = P1S_MAXROWS;
+    int yb = P1S_DEFROWS;
     for (int i = 0; i < g_rows_n; i++)
-        if (std::fabs(g_rows_dx[i] - G.dx) <= 0.01f * G.dx && g_rows_v[i] >= 1) yb = std::min(yb, g_rows_v[i]);
+        if (std::fabs(g_rows_dx[i] - G.dx) <= 0.01f * G.dx && g_rows_v[i] >= 1) yb = std::min(P1S_MAXROWS, g_rows_v[i]);
     yb = std::min(yb, std::max(1, 2048 / std::max(W, 1)));
     yb = std::min(yb, std::max(1, (int)(80.0f * G.dxn / G.dx)));
     return yb;

@@ -19,7 +19,7 @@ def test_tuned_pass1_rows_keep_the_mesh(qs, tmp_path):
     qs.load_tuning(None)
     try:
         ref = [qs.compute_ses(p, r, reso, 1) for (p, r), reso in mols]
-        for rows in (1, 3, 8, 64):
+        for rows in (1, 3, 8, 64, 256):
             print("case tuned pass-1 rows %d" % rows, flush=True)
             for (p, r), reso in mols:
                 qs.set_tuning("pass1_slab_rows", reso, rows)
