@@ -193,6 +193,7 @@ struct P2Group {
     std::vector<int> mols;
     size_t bpre_at = 0, mid_at = 0;
     uint32_t nblocks = 0;
+    uint32_t xchunk = 1; /* tiles per XCD turn of the fixed-R kernel: 4 rows of z tiles */
     P2Tables *tab = nullptr;
 };
 
@@ -210,7 +211,7 @@ struct Engine {
     DevBuf vstart, tstart; /* first active word of each 256-vertex / 256-triangle block */
     DevBuf normals;        /* staging for qs_vertex_normals / qs_vertex_colors */
     DevBuf colors_in;      /* per-atom colour table of qs_vertex_colors */
-    DevBuf vert, vert2, vert3, vkey, cidx, tri, tri_out, atom;
+    DevBuf vert, vert2, vert3, vkey, tri, tri_out, atom;
     DevBuf deg_blk, keeppos, tri_loc;
     DevBuf scan_tmp;
     DevBuf inc_cnt, inc_start, inc_ent; /* per-vertex triangle incidences (smoothing of slab meshes) */
@@ -302,7 +303,7 @@ struct Engine {
     {
         for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start,
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
-                          &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey, &cidx,
+                          &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey,
                           &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
                           &tpre_d, &tflag})
             *b = DevBuf{};
@@ -317,7 +318,7 @@ struct Engine {
     {
         for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start,
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
-                          &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey, &cidx,
+                          &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey,
                           &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
                           &tpre_d, &tflag})
             release(*b);

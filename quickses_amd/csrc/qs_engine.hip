@@ -231,7 +231,6 @@ int Engine::reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts
     QS_TRY(ensure(s4, nw * 32));
     QS_TRY(ensure(ntb, nw * 24));
     QS_TRY(ensure(field, vox * 4));
-    QS_TRY(ensure(cidx, vox));
     QS_TRY(ensure(in_pos, atoms * 12));
     for (DevBuf *b : {&in_rad, &key, &perm}) QS_TRY(ensure(*b, atoms * 4));
     for (DevBuf *b : {&sx, &aux}) QS_TRY(ensure(*b, atoms * 16));
@@ -401,9 +400,11 @@ int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &a
     for (auto &grp : gs) {
         grp.bpre_at = meta.size();
         uint64_t acc = 0;
+        grp.xchunk = 1;
         for (int m : grp.mols) {
             meta.push_back((uint32_t)acc);
             acc += qs_pass2_blocks(mols[m], grp.path);
+            grp.xchunk = std::max(grp.xchunk, 4u * (uint32_t)qs_pass2_ztiles(mols[m]));
         }
         meta.push_back((uint32_t)acc);
         grp.nblocks = (uint32_t)acc;
@@ -475,7 +476,7 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
         QS_TRY(group_tables(grp));
         qs_launch_pass2(batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path, as<uint64_t>(outm),
                         as<uint64_t>(shellm), as<float>(field), as<uint32_t>(negm), as<uint32_t>(zerom), as<int>(grp.tab->offs),
-                        as<int>(grp.tab->norm), as<int>(grp.tab->end), grp.tab->flags, tf, stream);
+                        as<int>(grp.tab->norm), as<int>(grp.tab->end), grp.tab->flags, tf, grp.xchunk, stream);
         QS_HIP(hipGetLastError());
     }
     field_sparse = true;
@@ -508,11 +509,10 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_TRY(ensure(avb, nw * 4));
     QS_TRY(ensure(atb, nw * 4));
     QS_TRY(ensure(ntb, nw * 24));
-    QS_TRY(ensure(cidx, std::max<uint64_t>(vox_total, 1)));
     QS_TRY(ensure(totals, 16));
     if (nb_mc)
         qs_launch_mc_count(batch(at_mc, 0, nmol), nb_mc, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
-                           as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint64_t>(ntb), as<uint8_t>(cidx), stream);
+                           as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint64_t>(ntb), stream);
     if (scan_nw) {
         QS_TRY(ensure(scan_tmp, 3 * qs_scan_tiles(scan_nw) * sizeof(uint32_t)));
         qs_launch_mc_scans(as<uint32_t>(vcnt) + scan_w0, as<uint32_t>(tcnt) + scan_w0, scan_nw, as<uint32_t>(scan_tmp),
@@ -562,7 +562,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
                           field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey), counts, stream);
     qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart),
                            nact_max, pool_t, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
-                           as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri), 0u, as<uint32_t>(deg_blk),
+                           as<uint64_t>(ntb), as<int>(tri), 0u, as<uint32_t>(deg_blk),
                            as<uint32_t>(totals) + 3, counts, stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
@@ -631,11 +631,11 @@ static int grow_keep(DevBuf &b, size_t bytes, size_t keep, hipStream_t s)
 
 /* device bytes of the grid arrays per x plane: per row word the outside, shell, sign and zero
    masks (32 B), slot masks (32 B), triangle-count planes (24 B) and seven counts and scans
-   (28 B); per voxel the field (4 B) and the cube case (1 B) */
+   (28 B); per voxel the field (4 B) */
 static uint64_t plane_bytes(const QsMol &M)
 {
     const uint64_t g = (uint64_t)M.G.g;
-    return g * M.W * 116 + g * g * 5;
+    return g * M.W * 116 + g * g * 4;
 }
 
 /* Planes each slab owns (CudaSurf.cu:626-636 sizes its slices from the SLICE setting instead),
@@ -677,11 +677,10 @@ int Engine::slab_mesh(size_t at_mc, uint32_t nb_mc, size_t at_wpre, uint64_t own
     QS_TRY(ensure(s4, nw * 32));
     for (DevBuf *b : {&vcnt, &tcnt, &vbw, &tbw, &alist, &avb, &atb}) QS_TRY(ensure(*b, nw * 4));
     QS_TRY(ensure(ntb, nw * 24));
-    QS_TRY(ensure(cidx, std::max<uint64_t>(vox_total, 1)));
     QS_TRY(ensure(totals, 16));
     if (nb_mc)
         qs_launch_mc_count(batch(at_mc, 0, 1), nb_mc, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
-                           as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint64_t>(ntb), as<uint8_t>(cidx), stream);
+                           as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint64_t>(ntb), stream);
     QS_TRY(ensure(scan_tmp, 3 * qs_scan_tiles(std::max<uint64_t>(scan_nw, 1)) * sizeof(uint32_t)));
     qs_launch_mc_scans(as<uint32_t>(vcnt) + scan_w0, as<uint32_t>(tcnt) + scan_w0, scan_nw, as<uint32_t>(scan_tmp),
                        as<uint32_t>(vbw) + scan_w0, as<uint32_t>(tbw) + scan_w0, as<uint32_t>(alist), as<uint32_t>(avb),
@@ -713,7 +712,7 @@ int Engine::slab_mesh(size_t at_mc, uint32_t nb_mc, size_t at_wpre, uint64_t own
                           as<float>(vert) + 3 * (size_t)vbase, as<uint64_t>(vkey), nullptr, stream);
     qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart), nact,
                            n_tri, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
-                           as<uint64_t>(ntb), as<uint8_t>(cidx), as<int>(tri) + 3 * (size_t)tbase, vbase,
+                           as<uint64_t>(ntb), as<int>(tri) + 3 * (size_t)tbase, vbase,
                            as<uint32_t>(deg_blk), as<uint32_t>(totals) + 3, nullptr, stream);
     QS_HIP(hipGetLastError());
     return 0;
@@ -948,7 +947,7 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
         float *dst = as<float>(vert2);
         for (int it = 0; it < smooth; it++) {
             qs_launch_smooth(as<QsMol>(d_mols), d_vpre, d_vpre + nmol + 1, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
-                             as<uint32_t>(vbw), as<uint64_t>(vkey), pool_v, as<uint8_t>(cidx), as<uint64_t>(ntb), as<uint32_t>(tbw), as<int>(tri),
+                             as<uint32_t>(vbw), as<uint64_t>(vkey), pool_v, as<uint64_t>(ntb), as<uint32_t>(tbw), as<int>(tri),
                              src, dst, as<uint32_t>(totals) + 1, stream);
             src = dst;
             dst = (dst == as<float>(vert2)) ? as<float>(vert3) : as<float>(vert2);
@@ -1900,7 +1899,7 @@ int qs_stage_pass2(const float *fp, const int *ip, const uint8_t *labels, float 
         qs_launch_pass2(E.batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path,
                         as<uint64_t>(E.outm), as<uint64_t>(E.shellm), as<float>(E.field), as<uint32_t>(E.negm),
                         as<uint32_t>(E.zerom), as<int>(grp.tab->offs), as<int>(grp.tab->norm), as<int>(grp.tab->end),
-                        grp.tab->flags, nullptr, E.stream);
+                        grp.tab->flags, nullptr, grp.xchunk, E.stream);
         QS_HIP(hipGetLastError());
     }
     E.field_sparse = true;

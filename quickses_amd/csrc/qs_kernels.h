@@ -47,12 +47,15 @@ size_t qs_pass2_lds_bytes(int r);
 int qs_pass2_path(int r, bool direct);
 /* blocks of one molecule's launch over its planes [p2_lo, p2_hi) */
 uint32_t qs_pass2_blocks(const QsMol &M, int path);
-/* tflag (path 0 only, may be NULL): tiles left unflagged by field pass 1 are skipped */
+/* tflag (path 0 only, may be NULL): tiles left unflagged by field pass 1 are skipped; xchunk (path 0):
+   consecutive tiles each XCD takes in turn */
 void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm, float *field,
                      uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
-                     QsNormFlags nf, const uint8_t *tflag, hipStream_t s);
+                     QsNormFlags nf, const uint8_t *tflag, uint32_t xchunk, hipStream_t s);
 /* pass-2 tiles of the fixed-R kernel of one molecule (its tile-flag count) */
 uint32_t qs_pass2_tiles(const QsMol &M);
+/* z tiles per (x, y) tile row of the fixed-R pass-2 kernel */
+uint32_t qs_pass2_ztiles(const QsMol &M);
 
 /* field pass 2 stores the shell voxels' values only: the others (probe radius outside, -dx
    inside) of one molecule's field, written from its outside / shell masks */
@@ -65,10 +68,10 @@ void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *ou
 /* marching cubes */
 /* blocks of one molecule's launch over its grid points of planes [mc_lo, mc_hi) */
 uint32_t qs_mc_count_blocks(const QsMol &M);
-/* per row word: slot masks, vertex and triangle counts, triangle-count bit planes (words with
-   triangles only); the case of every polygonised cube into cidx */
+/* per row word: slot masks (words with vertices only), vertex and triangle counts, triangle-count
+   bit planes (words with triangles only) */
 void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
-                        uint32_t *tcnt, uint64_t *NT, uint8_t *cidx, hipStream_t s);
+                        uint32_t *tcnt, uint64_t *NT, hipStream_t s);
 /* one thread per vertex / per triangle over the active words (list, with their first vertex ids
    avb and first triangle ids atb); vstart / tstart receive each 256-item block's first active word
    (nv / 256 + 1 and ntri / 256 + 1 entries).  The field F holds the shell voxels' values only,
@@ -80,7 +83,7 @@ void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                             const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
                             const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
-                            const uint8_t *cidx, int *tri, uint32_t vbase, uint32_t *deg_blk, uint32_t *deg_total,
+                            int *tri, uint32_t vbase, uint32_t *deg_blk, uint32_t *deg_total,
                             const uint32_t *d_counts, hipStream_t s);
 
 /* Launches whose item counts live on the device take them from the d_* pointers (when given),
@@ -99,7 +102,7 @@ void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint3
                             const int *perm, const float *vert, uint32_t nv, int sorted_ids, int *out, const uint32_t *d_nv,
                             hipStream_t s);
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
-                      const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
+                      const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv,
                       const uint64_t *NT, const uint32_t *tbw, const int *tri, const float *src, float *dst,
                       const uint32_t *d_nv, hipStream_t s);
 

@@ -139,6 +139,14 @@ __device__ __forceinline__ uint32_t xcd_group(uint32_t b, uint32_t nb)
 }
 constexpr uint32_t XCD_CHUNK = 8;
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) { return xcd_group<XCD_CHUNK>(b, nb); }
+/* the same renumbering with a group size chosen at launch (c >= 1) */
+__device__ __forceinline__ uint32_t xcd_group_rt(uint32_t b, uint32_t nb, uint32_t c)
+{
+    const uint32_t G = 8 * c;
+    if (b >= nb / G * G) return b;
+    const uint32_t w = b % G;
+    return b - w + (w & 7u) * c + (w >> 3);
+}
 
 /* molecule of block b of a batched launch, and the block's index inside the molecule */
 __device__ __forceinline__ int batch_mol(const QsBatch &B, uint32_t b, uint32_t &local)
@@ -1436,7 +1444,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
                                                  uint32_t *__restrict__ negm_all, uint32_t *__restrict__ zerom_all,
                                                  const int *__restrict__ offs, const int *__restrict__ norm_start,
                                                  const int *__restrict__ enum_end, QsNormFlags nf,
-                                                 const uint8_t *__restrict__ tflag)
+                                                 const uint8_t *__restrict__ tflag, uint32_t xchunk)
 {
     constexpr int LX = P2R_TX + 2 * R, LY = P2_TY + 2 * R, LZ = P2_TZ + 2 * R;
     constexpr int INF = 255;
@@ -1459,7 +1467,9 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     __shared__ uint16_t tq[4][128];
     __shared__ uint32_t sneg[P2R_TX * P2_TY], szero[P2R_TX * P2_TY];
     uint32_t lb;
-    const QsMol &M = B.mols[batch_mol(B, xcd_block(blockIdx.x, gridDim.x), lb)];
+    /* each XCD takes xchunk consecutive tiles at a time (whole rows of z tiles: the y and z halos
+       of a tile are then mostly read by tiles of the same XCD, from its L2) */
+    const QsMol &M = B.mols[batch_mol(B, xcd_group_rt(blockIdx.x, gridDim.x, xchunk), lb)];
     /* a tile without a shell voxel keeps the sign words field pass 1 wrote for it */
     if (tflag && M.tile_off != ~0u && !tflag[M.tile_off + lb]) return;
     const QsGrid G = M.G;
@@ -1980,6 +1990,24 @@ struct MCView {
         ci |= (int)nbit(x, y + 1, z + 1) << 7;
         return ci;
     }
+    /* the same case from the four row words of the cube's x, y corners (and the next words when
+       the cube's top corners lie in them) */
+    __device__ __forceinline__ int cube_case(int x, int y, int z) const
+    {
+        const int b = z & 63;
+        const size_t w00 = widx(x, y, z), w10 = w00 + (size_t)g * W, w01 = w00 + W, w11 = w10 + W;
+        const uint64_t n00 = N[w00], n10 = N[w10], n11 = N[w11], n01 = N[w01];
+        int ci = (int)((n00 >> b) & 1ull) | (int)((n10 >> b) & 1ull) << 1 | (int)((n11 >> b) & 1ull) << 2 |
+                 (int)((n01 >> b) & 1ull) << 3;
+        if (b < 63) {
+            ci |= (int)((n00 >> (b + 1)) & 1ull) << 4 | (int)((n10 >> (b + 1)) & 1ull) << 5 |
+                  (int)((n11 >> (b + 1)) & 1ull) << 6 | (int)((n01 >> (b + 1)) & 1ull) << 7;
+        } else {
+            ci |= (int)(N[w00 + 1] & 1ull) << 4 | (int)(N[w10 + 1] & 1ull) << 5 | (int)(N[w11 + 1] & 1ull) << 6 |
+                  (int)(N[w01 + 1] & 1ull) << 7;
+        }
+        return ci;
+    }
     /* vertex id of slot s (0,1,2 edge x,y,z; 3 point) of point (x, y, z) */
     __device__ __forceinline__ uint32_t slot_id(int x, int y, int z, int sl) const
     {
@@ -2020,7 +2048,7 @@ __device__ __forceinline__ uint64_t ld_word(const uint64_t *__restrict__ m, int 
 __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__restrict__ N_all,
                                                   const uint64_t *__restrict__ Zm_all, uint64_t *__restrict__ S4_all,
                                                   uint32_t *__restrict__ vcnt_all, uint32_t *__restrict__ tcnt_all,
-                                                  uint64_t *__restrict__ NT_all, uint8_t *__restrict__ cidx_all)
+                                                  uint64_t *__restrict__ NT_all)
 {
     /* triangles per case, in LDS: the loop over a word's cubes below looks one up per cube */
     __shared__ uint8_t s_ntri[256];
@@ -2096,9 +2124,8 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
         const uint64_t all = c0 & c1 & c2 & c3 & c4 & c5 & c6 & c7;
         const uint64_t any = c0 | c1 | c2 | c3 | c4 | c5 | c6 | c7;
         uint64_t act = any & ~all & wlt;
-        /* triangles per cube as three bit planes (counts < 8), and each polygonised cube's case */
+        /* triangles per cube as three bit planes (counts < 8) */
         uint64_t p0 = 0, p1 = 0, p2 = 0;
-        uint8_t *__restrict__ cidx = cidx_all + M.vox_off + ((size_t)x * g + y) * g + z0;
         while (act) {
             int b = __builtin_ctzll(act);
             act &= act - 1ull;
@@ -2108,7 +2135,6 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
             const uint32_t k = s_ntri[ci];
             if (!k) continue;
             nt += k;
-            cidx[b] = (uint8_t)ci;
             p0 |= (uint64_t)(k & 1u) << b;
             p1 |= (uint64_t)((k >> 1) & 1u) << b;
             p2 |= (uint64_t)((k >> 2) & 1u) << b;
@@ -2345,16 +2371,14 @@ __global__ void __launch_bounds__(256) k_mc_vertices(const QsMol *__restrict__ m
 
 /* triangles (generateTriangleVerticesSMEM MarchingCubes.cu:736-851 order), one thread per
    triangle: its word from the active words' first triangle ids, its cube from the word's
-   triangle-count bit planes; each polygonised cube also records its first triangle id (read by
-   the smoothing pass) */
+   triangle-count bit planes, the cube's case from the sign words of its corners */
 __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ mols, const uint64_t *__restrict__ wpre,
                                                       const uint32_t *__restrict__ wcoarse, const uint32_t *__restrict__ list,
                                                       const uint32_t *__restrict__ atb, const uint32_t *__restrict__ tstart,
                                                       uint32_t nact, uint32_t ntri,
                                                       const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
                                                       const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
-                                                      const uint64_t *__restrict__ NT_all, const uint8_t *__restrict__ cidx_all,
-                                                      int *__restrict__ tri, uint32_t vbase, uint32_t *__restrict__ deg_blk,
+                                                      const uint64_t *__restrict__ NT_all, int *__restrict__ tri, uint32_t vbase, uint32_t *__restrict__ deg_blk,
                                                       uint32_t *__restrict__ deg_total, const uint32_t *__restrict__ d_counts)
 {
     __shared__ ItemWords S;
@@ -2392,9 +2416,6 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
     const uint64_t below = (1ull << b) - 1ull;
     const int j = (int)(local - (uint32_t)(__popcll(p0 & below) + 2 * __popcll(p1 & below) + 4 * __popcll(p2 & below)));
     const int z = w * 64 + b;
-    const size_t pidx = ((size_t)x * g + y) * g + z;
-    const int ci = cidx_all[M.vox_off + pidx];
-    const uint64_t packed = c_tri[ci];
     MCView V;
     V.N = N_all + M.word_off;
     V.Zm = Zm_all + M.word_off;
@@ -2403,6 +2424,8 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
     V.g = g;
     V.W = (int)M.W;
     V.lim = M.G.lim;
+    const int ci = V.cube_case(x, y, z);
+    const uint64_t packed = c_tri[ci];
 #pragma unroll
     for (int cc = 0; cc < 3; cc++) c[cc] = (int)(vbase + V.corner_vid(x, y, z, tri_entry(packed, 3 * j + cc), ci));
     tri[3 * (size_t)t + 0] = c[0];
@@ -2507,15 +2530,17 @@ __device__ __forceinline__ uint32_t cube_first_tri(const uint64_t *__restrict__ 
     return tbw[wi] + (uint32_t)(__popcll(np[0] & below) + 2 * __popcll(np[1] & below) + 4 * __popcll(np[2] & below));
 }
 
-/* one Jacobi pass, one thread per vertex.  Polygonised cubes carry their case (cidx) and, through
-   their word's triangle base and triangle-count planes, their first triangle id, so each incident
-   triangle's other corners are read from the triangle array. */
+/* one Jacobi pass, one thread per vertex.  A polygonised cube's triangles are consecutive from
+   its first triangle id (its word's triangle base plus the triangle-count planes below it), and
+   its count is its bit of those planes, so each incident triangle's other corners are read from
+   the triangle array.  An edge vertex is a corner of exactly the triangles whose case-table entry
+   is its cube edge (no other edge of the cube shares its id), so it finds them by id. */
 __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
                                                 const uint32_t *__restrict__ vcoarse,
                                                 const uint64_t *__restrict__ N_all, const uint64_t *__restrict__ Zm_all,
                                                 const uint64_t *__restrict__ S4_all, const uint32_t *__restrict__ vbw_all,
                                                 const uint64_t *__restrict__ vkey, uint32_t nv,
-                                                const uint8_t *__restrict__ cidx_all, const uint64_t *__restrict__ NT_all,
+                                                const uint64_t *__restrict__ NT_all,
                                                 const uint32_t *__restrict__ tbw_all, const int *__restrict__ tri,
                                                 const float *__restrict__ src,
                                                 float *__restrict__ dst, const uint32_t *__restrict__ d_nv)
@@ -2540,7 +2565,6 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
     V.g = g;
     V.W = (int)M.W;
     V.lim = lim;
-    const uint8_t *__restrict__ cidx = cidx_all + M.vox_off;
     const uint64_t *__restrict__ NT = NT_all + 3 * M.word_off;
     const uint32_t *__restrict__ tbw = tbw_all + M.word_off;
     const int W = (int)M.W;
@@ -2554,49 +2578,36 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     uint32_t cnt = 0;
     if (sl < 3) {
-        /* the 4 cubes around the edge in cube order, and the edge's number in each; all of them
-           that are polygonised contain the sign change, so their records exist.  The records of
-           all four are loaded before any is used. */
-        int ci4[4], e4[4];
+        /* the 4 cubes around the edge in cube order (offsets -1 / 0 of the two other axes, major
+           first); every one inside the grid contains the sign change, so it is polygonised and its
+           word's triangle records exist.  The records of all four are loaded before any is used. */
+        int nt4[4];
         uint32_t tb4[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const int hi = k >> 1, lo = k & 1; /* offsets -1 / 0 of the two other axes, major first */
-            int cx = x, cy = y, cz = z, e;
-            if (sl == 0) {
-                cy -= 1 - hi;
-                cz -= 1 - lo;
-                e = (1 - hi) * 2 + (1 - lo) * 4; /* corner offset (0,dy,dz): 0, 2, 4, 6 */
-            } else if (sl == 1) {
-                cx -= 1 - hi;
-                cz -= 1 - lo;
-                e = (1 - lo) ? ((1 - hi) ? 5 : 7) : ((1 - hi) ? 1 : 3); /* (dx,0,dz): 3 1 7 5 */
-            } else {
-                cx -= 1 - hi;
-                cy -= 1 - lo;
-                e = (1 - hi) ? ((1 - lo) ? 10 : 9) : ((1 - lo) ? 11 : 8); /* (dx,dy,0): 8 9 10 11 */
-            }
-            e4[k] = e;
-            ci4[k] = 0;
+            const int hi = k >> 1, lo = k & 1;
+            const int cx = x - (sl != 0 ? 1 - hi : 0);
+            const int cy = y - (sl == 0 ? 1 - hi : (sl == 2 ? 1 - lo : 0));
+            const int cz = z - (sl != 2 ? 1 - lo : 0);
+            nt4[k] = 0;
             tb4[k] = 0;
             if (cx < 0 || cy < 0 || cz < 0 || cx >= lim || cy >= lim || cz >= lim) continue;
-            const size_t row = (size_t)cx * g + cy;
-            ci4[k] = cidx[row * g + cz];
-            tb4[k] = cube_first_tri(NT, tbw, row * W + (cz >> 6), cz & 63);
+            const size_t wi = ((size_t)cx * g + cy) * W + (cz >> 6);
+            const int b = cz & 63;
+            const uint64_t *np = NT + 3 * wi;
+            const uint64_t p0 = np[0], p1 = np[1], p2 = np[2], below = (1ull << b) - 1ull;
+            tb4[k] = tbw[wi] + (uint32_t)(__popcll(p0 & below) + 2 * __popcll(p1 & below) + 4 * __popcll(p2 & below));
+            nt4[k] = (int)((p0 >> b) & 1ull) + 2 * (int)((p1 >> b) & 1ull) + 4 * (int)((p2 >> b) & 1ull);
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const int ci = ci4[k], e = e4[k];
-            const uint32_t tb = tb4[k];
-            const uint64_t packed = c_tri[ci];
-            const int nt = tri_count_packed(packed);
-            for (int t = 0; t < nt; t++) {
-                const int j = (tri_entry(packed, 3 * t) == e) ? 0
-                              : (tri_entry(packed, 3 * t + 1) == e) ? 1
-                              : (tri_entry(packed, 3 * t + 2) == e) ? 2 : -1;
-                if (j < 0) continue;
-                const int *tt = tri + 3 * (size_t)(tb + t);
-                push_pair(src, (uint32_t)tt[j == 0 ? 1 : 0], (uint32_t)tt[j == 2 ? 1 : 2], sx, sy, sz);
+            for (int t = 0; t < nt4[k]; t++) {
+                const int *tt = tri + 3 * (size_t)(tb4[k] + t);
+                const uint32_t a = (uint32_t)tt[0], b = (uint32_t)tt[1], c = (uint32_t)tt[2];
+                if (a == v) push_pair(src, b, c, sx, sy, sz);
+                else if (b == v) push_pair(src, a, c, sx, sy, sz);
+                else if (c == v) push_pair(src, a, b, sx, sy, sz);
+                else continue;
                 cnt += 2;
             }
         }
@@ -3054,13 +3065,14 @@ uint32_t qs_pass2_blocks(const QsMol &M, int path)
 /* one launch for a group of molecules sharing r (and so the path) */
 void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm, float *field,
                      uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
-                     QsNormFlags nf, const uint8_t *tflag, hipStream_t s)
+                     QsNormFlags nf, const uint8_t *tflag, uint32_t xchunk, hipStream_t s)
 {
     if (path == 0) {
+        xchunk = std::max(xchunk, 1u);
 #define QS_P2R(RR)                                                                                                    \
     case RR:                                                                                                          \
         hipLaunchKernelGGL(k_pass2_r<RR>, dim3(nblocks), dim3(256), 0, s, B, outm, shellm, field, negm, zerom, offs, \
-                           norm_start, enum_end, nf, tflag);                                                          \
+                           norm_start, enum_end, nf, tflag, xchunk);                                                  \
         return;
         switch (r) {
             QS_P2R(1)
@@ -3097,9 +3109,9 @@ uint32_t qs_mc_count_blocks(const QsMol &M)
 }
 
 void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
-                        uint32_t *tcnt, uint64_t *NT, uint8_t *cidx, hipStream_t s)
+                        uint32_t *tcnt, uint64_t *NT, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(256), 0, s, B, N, Zm, S4, vcnt, tcnt, NT, cidx);
+    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(256), 0, s, B, N, Zm, S4, vcnt, tcnt, NT);
 }
 
 void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
@@ -3117,7 +3129,7 @@ void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                             const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
                             const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
-                            const uint8_t *cidx, int *tri, uint32_t vbase, uint32_t *deg_blk, uint32_t *deg_total,
+                            int *tri, uint32_t vbase, uint32_t *deg_blk, uint32_t *deg_total,
                             const uint32_t *d_counts, hipStream_t s)
 {
     if (!ntri || !nact) return;
@@ -3127,16 +3139,16 @@ void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint3
                        nact, ntri, d_counts ? d_counts + 2 : nullptr, d_counts ? d_counts : nullptr, tstart, deg_blk,
                        ntri / 256 + 1);
     hipLaunchKernelGGL(k_mc_triangles, dim3(pgrid(nblk(ntri, 256))), dim3(256), 0, s, mols, wpre, wcoarse, list, atb, tstart, nact,
-                       ntri, N, Zm, S4, vbw, NT, cidx, tri, vbase, deg_blk, deg_total, d_counts);
+                       ntri, N, Zm, S4, vbw, NT, tri, vbase, deg_blk, deg_total, d_counts);
 }
 
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
-                      const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv, const uint8_t *cidx,
+                      const uint64_t *S4, const uint32_t *vbw, const uint64_t *vkey, uint32_t nv,
                       const uint64_t *NT, const uint32_t *tbw, const int *tri, const float *src, float *dst,
                       const uint32_t *d_nv, hipStream_t s)
 {
     if (!nv) return;
-    hipLaunchKernelGGL(k_smooth, dim3(pgrid(nblk(nv, 256))), dim3(256), 0, s, mols, vpre, vcoarse, N, Zm, S4, vbw, vkey, nv, cidx,
+    hipLaunchKernelGGL(k_smooth, dim3(pgrid(nblk(nv, 256))), dim3(256), 0, s, mols, vpre, vcoarse, N, Zm, S4, vbw, vkey, nv,
                        NT, tbw, tri, src, dst, d_nv);
 }
 
@@ -3275,3 +3287,4 @@ void qs_launch_mesh_bases(const uint32_t *vbw, const uint32_t *tbw, const uint64
 }
 
 uint32_t qs_pass2_tiles(const QsMol &M) { return qs_pass2_blocks(M, 0); }
+uint32_t qs_pass2_ztiles(const QsMol &M) { return (uint32_t)((M.G.g + P2_TZ - 1) / P2_TZ); }

@@ -120,13 +120,13 @@ struct Server {
 
 Server *g_srv = nullptr;
 
-/* device bytes of one request in an engine's workspace: grid arrays per voxel (field 4, cube
-   cases 1, bit masks and per-word counts about 1.4), atom arrays, neighbour cells, and the mesh
-   arrays for a surface of about 6 g^2 vertices */
+/* device bytes of one request in an engine's workspace: grid arrays per voxel (field 4, bit
+   masks and per-word counts about 1.4), atom arrays, neighbour cells, and the mesh arrays for a
+   surface of about 6 g^2 vertices */
 uint64_t estimate_bytes(const QsGrid &G, uint32_t n)
 {
     const uint64_t g = (uint64_t)G.g, gn = (uint64_t)G.gn;
-    return g * g * g * 13 / 2 + (uint64_t)n * 64 + gn * gn * gn * 20 + 6 * g * g * 140 + 4096;
+    return g * g * g * 11 / 2 + (uint64_t)n * 64 + gn * gn * gn * 20 + 6 * g * g * 140 + 4096;
 }
 
 void finish(Server *S, std::vector<Request *> &batch, const char *err)
