@@ -217,6 +217,8 @@ struct Engine {
     DevBuf inc_cnt, inc_start, inc_ent; /* per-vertex triangle incidences (smoothing of slab meshes) */
     DevBuf tpre_d;                      /* per-molecule first triangle (u64, nmol + 1), on the device */
     DevBuf tflag;                       /* field-pass-2 tiles holding a shell voxel (u8 per tile) */
+    DevBuf spre, stot;                  /* shell rank of each row word (u32), shell total (2 x u32) */
+    DevBuf field_dense;                 /* the whole field in grid order, expanded on request */
     std::vector<P2Group> groups;
     std::vector<P2Tables *> tables;
     /* host staging of uploads that kernels read later in the same call */
@@ -278,7 +280,8 @@ struct Engine {
     uint32_t nv = 0, nt_all = 0, nt = 0, n_active = 0;
     size_t p1_lds = 0; /* dynamic LDS of the field pass 1 launch (largest molecule of the batch) */
     bool batch_valid = false; /* result arrays, vert_off / tri_off and their device copies belong to run_batch */
-    /* field holds the shell voxels' values only (field pass 2); the others come from outm / shellm */
+    /* field holds the shell voxels' values only (field pass 2), densely in shell-rank order (spre);
+       the others come from outm / shellm.  Else field holds every voxel in grid order. */
     bool field_sparse = false;
     /* the last run's atom ids are cell-sorted positions (option atom_ids_sorted at run time) */
     bool ids_sorted = false;
@@ -305,7 +308,7 @@ struct Engine {
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
                           &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey,
                           &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
-                          &tpre_d, &tflag})
+                          &tpre_d, &tflag, &spre, &stot, &field_dense})
             *b = DevBuf{};
         for (P2Tables *t : tables) delete t;
         tables.clear();
@@ -320,7 +323,7 @@ struct Engine {
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
                           &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey,
                           &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
-                          &tpre_d, &tflag})
+                          &tpre_d, &tflag, &spre, &stot, &field_dense})
             release(*b);
         for (P2Tables *t : tables) {
             release(t->offs);
@@ -417,7 +420,7 @@ struct Engine {
     int run_batch(const float *d_pos, const float *d_rad, const std::vector<MolIn> &in, int smooth,
                   const std::vector<BBox> *host_bbox);
     int field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm);
-    int fill_field();
+    int fill_field(const void **dense);
     int mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer &tm, bool with_atoms);
     int group_tables(P2Group &grp);
     void plan_pools();

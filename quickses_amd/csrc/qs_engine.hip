@@ -231,6 +231,8 @@ int Engine::reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts
     QS_TRY(ensure(s4, nw * 32));
     QS_TRY(ensure(ntb, nw * 24));
     QS_TRY(ensure(field, vox * 4));
+    QS_TRY(ensure(spre, nw * 4));
+    QS_TRY(ensure(stot, 8));
     QS_TRY(ensure(in_pos, atoms * 12));
     for (DevBuf *b : {&in_rad, &key, &perm}) QS_TRY(ensure(*b, atoms * 4));
     for (DevBuf *b : {&sx, &aux}) QS_TRY(ensure(*b, atoms * 16));
@@ -451,6 +453,9 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
     QS_TRY(ensure(field, std::max<uint64_t>(vox_total, 1) * 4));
     QS_TRY(ensure(negm, std::max<uint64_t>(nw_total, 1) * 8));
     QS_TRY(ensure(zerom, std::max<uint64_t>(nw_total, 1) * 8));
+    QS_TRY(ensure(spre, std::max<uint64_t>(nw_total, 1) * 4));
+    QS_TRY(ensure(stot, 8));
+    QS_TRY(ensure(scan_tmp, qs_scan_tiles(std::max<uint64_t>(nw_total, 1)) * sizeof(uint32_t)));
     /* the slab kernel of field pass 1 writes every voxel's default sign words and flags the
        pass-2 tiles that hold a shell voxel; pass 2 then runs on the flagged tiles only */
     const int p1mode = qs_pass1_mode(g_opt.pass1_direct != 0, g_opt.pass1_rows != 0);
@@ -472,10 +477,13 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
         QS_HIP(hipGetLastError());
     }
     tm.mark("pass1");
+    /* every row word's shell rank: pass 2 stores the shell voxels' values densely in that order
+       (the shell voxels are at most the voxels, which the field is sized for) */
+    qs_launch_shell_scan(as<uint64_t>(shellm), nw_total, as<uint32_t>(scan_tmp), as<uint32_t>(stot), as<uint32_t>(spre), stream);
     for (auto &grp : groups) {
         QS_TRY(group_tables(grp));
         qs_launch_pass2(batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path, as<uint64_t>(outm),
-                        as<uint64_t>(shellm), as<float>(field), as<uint32_t>(negm), as<uint32_t>(zerom), as<int>(grp.tab->offs),
+                        as<uint64_t>(shellm), as<uint32_t>(spre), as<float>(field), as<uint32_t>(negm), as<uint32_t>(zerom), as<int>(grp.tab->offs),
                         as<int>(grp.tab->norm), as<int>(grp.tab->end), grp.tab->flags, tf, grp.xchunk, stream);
         QS_HIP(hipGetLastError());
     }
@@ -484,14 +492,19 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
     return 0;
 }
 
-/* every voxel of every molecule's field (pass 2 leaves non-shell voxels unstored) */
-int Engine::fill_field()
+/* every voxel of every molecule's field in grid order (pass 2 stores the shell voxels only):
+   *dense receives the device array, molecule m at its voxel offset */
+int Engine::fill_field(const void **dense)
 {
+    *dense = field.p;
     if (!field_sparse) return 0;
+    QS_TRY(ensure(field_dense, std::max<uint64_t>(vox_total, 1) * 4));
     for (const QsMol &M : mols)
-        qs_launch_field_fill(as<uint64_t>(outm) + M.word_off, as<uint64_t>(shellm) + M.word_off, as<float>(field) + M.vox_off,
-                             M.G.g, (int)M.W, M.G.dx, stream);
+        qs_launch_field_expand(as<float>(field), as<uint32_t>(spre) + M.word_off, as<uint64_t>(outm) + M.word_off,
+                               as<uint64_t>(shellm) + M.word_off, as<float>(field_dense) + M.vox_off, M.G.g, (int)M.W,
+                               M.G.dx, stream);
     QS_HIP(hipGetLastError());
+    *dense = field_dense.p;
     return 0;
 }
 
@@ -558,8 +571,9 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     const uint32_t nact_max = (uint32_t)std::max<uint64_t>(scan_nw, 1);
     const uint32_t *counts = as<uint32_t>(totals);
     qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart),
-                          nact_max, pool_v, as<uint64_t>(s4), as<float>(field), as<uint64_t>(outm),
-                          field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey), counts, stream);
+                          nact_max, pool_v, as<uint64_t>(s4), as<float>(field), field_sparse ? as<uint32_t>(spre) : nullptr,
+                          as<uint64_t>(outm), field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey),
+                          counts, stream);
     qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart),
                            nact_max, pool_t, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
                            as<uint64_t>(ntb), as<int>(tri), 0u, as<uint32_t>(deg_blk),
@@ -708,7 +722,7 @@ int Engine::slab_mesh(size_t at_mc, uint32_t nb_mc, size_t at_wpre, uint64_t own
     const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
     const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 4;
     qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart), nact,
-                          n_owned, as<uint64_t>(s4), as<float>(field), as<uint64_t>(outm), as<uint64_t>(shellm),
+                          n_owned, as<uint64_t>(s4), as<float>(field), as<uint32_t>(spre), as<uint64_t>(outm), as<uint64_t>(shellm),
                           as<float>(vert) + 3 * (size_t)vbase, as<uint64_t>(vkey), nullptr, stream);
     qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart), nact,
                            n_tri, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
@@ -1685,7 +1699,8 @@ static int launch_normals(Engine &E, float *d_out)
     const int nmol = (int)E.mols.size();
     const uint32_t *d_vpre = as<uint32_t>(E.d_off) + (nmol + 1);
     qs_launch_vertex_normals(as<QsMol>(E.d_mols), d_vpre, d_vpre + nmol + 1, as<uint64_t>(E.vkey), E.nv, as<float>(E.field),
-                             as<uint64_t>(E.outm), E.field_sparse ? as<uint64_t>(E.shellm) : nullptr, d_out, E.stream);
+                             E.field_sparse ? as<uint32_t>(E.spre) : nullptr, as<uint64_t>(E.outm),
+                             E.field_sparse ? as<uint64_t>(E.shellm) : nullptr, d_out, E.stream);
     QS_HIP(hipGetLastError());
     return 0;
 }
@@ -1842,9 +1857,8 @@ int64_t qs_fetch(const char *what, void *host, int64_t bytes)
         return (int64_t)V0;
     }
     else if (w == "field") {
-        if (E.fill_field() != 0) return -1;
+        if (E.fill_field(&src) != 0) return -1;
         if (hipStreamSynchronize(E.stream) != hipSuccess) return -1;
-        src = E.field.p;
         need = (int64_t)V * 4;
     }
     else if (w == "shellm") { src = E.shellm.p; need = (int64_t)E.nw_total * 8; }
@@ -1894,18 +1908,24 @@ int qs_stage_pass2(const float *fp, const int *ip, const uint8_t *labels, float 
     QS_TRY(ensure(E.field, V * 4));
     QS_TRY(ensure(E.negm, E.nw_total * 8));
     QS_TRY(ensure(E.zerom, E.nw_total * 8));
+    QS_TRY(ensure(E.spre, E.nw_total * 4));
+    QS_TRY(ensure(E.stot, 8));
+    QS_TRY(ensure(E.scan_tmp, qs_scan_tiles(E.nw_total) * sizeof(uint32_t)));
+    qs_launch_shell_scan(as<uint64_t>(E.shellm), E.nw_total, as<uint32_t>(E.scan_tmp), as<uint32_t>(E.stot), as<uint32_t>(E.spre),
+                         E.stream);
     for (auto &grp : E.groups) {
         QS_TRY(E.group_tables(grp));
         qs_launch_pass2(E.batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path,
-                        as<uint64_t>(E.outm), as<uint64_t>(E.shellm), as<float>(E.field), as<uint32_t>(E.negm),
+                        as<uint64_t>(E.outm), as<uint64_t>(E.shellm), as<uint32_t>(E.spre), as<float>(E.field), as<uint32_t>(E.negm),
                         as<uint32_t>(E.zerom), as<int>(grp.tab->offs), as<int>(grp.tab->norm), as<int>(grp.tab->end),
                         grp.tab->flags, nullptr, grp.xchunk, E.stream);
         QS_HIP(hipGetLastError());
     }
     E.field_sparse = true;
-    QS_TRY(E.fill_field());
+    const void *dense = nullptr;
+    QS_TRY(E.fill_field(&dense));
     QS_HIP(hipStreamSynchronize(E.stream));
-    QS_HIP(hipMemcpy(field, E.field.p, V * 4, hipMemcpyDeviceToHost));
+    QS_HIP(hipMemcpy(field, dense, V * 4, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -47,19 +47,26 @@ size_t qs_pass2_lds_bytes(int r);
 int qs_pass2_path(int r, bool direct);
 /* blocks of one molecule's launch over its planes [p2_lo, p2_hi) */
 uint32_t qs_pass2_blocks(const QsMol &M, int path);
-/* tflag (path 0 only, may be NULL): tiles left unflagged by field pass 1 are skipped; xchunk (path 0):
-   consecutive tiles each XCD takes in turn */
-void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm, float *field,
-                     uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
+/* the shell voxels' values go to field at their shell ranks (spre, from qs_launch_shell_scan);
+   tflag (path 0 only, may be NULL): tiles left unflagged by field pass 1 are skipped; xchunk
+   (path 0): consecutive tiles each XCD takes in turn */
+void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm,
+                     const uint32_t *spre, float *field, uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
                      QsNormFlags nf, const uint8_t *tflag, uint32_t xchunk, hipStream_t s);
 /* pass-2 tiles of the fixed-R kernel of one molecule (its tile-flag count) */
 uint32_t qs_pass2_tiles(const QsMol &M);
 /* z tiles per (x, y) tile row of the fixed-R pass-2 kernel */
 uint32_t qs_pass2_ztiles(const QsMol &M);
 
-/* field pass 2 stores the shell voxels' values only: the others (probe radius outside, -dx
-   inside) of one molecule's field, written from its outside / shell masks */
-void qs_launch_field_fill(const uint64_t *outm, const uint64_t *shellm, float *F, int g, int W, float dx, hipStream_t s);
+/* shell ranks (spre, one per word) of the words [0, nw) of shellm: the place of each word's first
+   shell voxel in the stored field (sums: qs_scan_tiles(nw) scratch entries; totals: 2 entries,
+   the total shell voxels first) */
+void qs_launch_shell_scan(const uint64_t *shellm, uint64_t nw, uint32_t *sums, uint32_t *totals, uint32_t *spre, hipStream_t s);
+/* field pass 2 stores the shell voxels' values only, in shell-rank order: one molecule's whole
+   field in grid order (out), from them and its outside / shell masks (spre, outm, shellm at the
+   molecule's first word; F the stored values of the whole launch) */
+void qs_launch_field_expand(const float *F, const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *out, int g,
+                            int W, float dx, hipStream_t s);
 
 /* stage-wise helpers (one molecule) */
 void qs_launch_field_signs(const float *F, int g, int W, uint32_t *negm, uint32_t *zerom, hipStream_t s);
@@ -75,11 +82,12 @@ void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const ui
 /* one thread per vertex / per triangle over the active words (list, with their first vertex ids
    avb and first triangle ids atb); vstart / tstart receive each 256-item block's first active word
    (nv / 256 + 1 and ntri / 256 + 1 entries).  The field F holds the shell voxels' values only,
-   the others come from the outside / shell masks (shellm == nullptr: F holds every voxel). */
+   at their shell ranks (spre), the others come from the outside / shell masks (shellm == nullptr:
+   F holds every voxel in grid order, spre is unused). */
 void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                            const uint32_t *avb, uint32_t *vstart, uint32_t nact, uint32_t nv, const uint64_t *S4,
-                           const float *F, const uint64_t *outm, const uint64_t *shellm, float *vert, uint64_t *vkey,
-                           const uint32_t *d_counts, hipStream_t s);
+                           const float *F, const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *vert,
+                           uint64_t *vkey, const uint32_t *d_counts, hipStream_t s);
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                             const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
                             const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
@@ -121,7 +129,8 @@ void qs_launch_deg_count(const int *tri, uint32_t nt, uint32_t *deg_blk, uint32_
 /* unit field gradient at each vertex's marching-cubes position, 3 floats per vertex (field as for
    the vertex kernel) */
 void qs_launch_vertex_normals(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *vkey, uint32_t nv,
-                              const float *F, const uint64_t *outm, const uint64_t *shellm, float *out, hipStream_t s);
+                              const float *F, const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *out,
+                              hipStream_t s);
 /* colour of each vertex's closest atom from a per-atom RGBA table (batch atom layout); perm maps
    cell-sorted atom ids to input indices (nullptr: the ids are input indices) */
 void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const int *atom,

@@ -265,6 +265,62 @@ __global__ void __launch_bounds__(SCAN_NT) k_tile_scan(const uint32_t *in, uint3
     if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = off + tot;
 }
 
+/* Shell ranks: spre[i] = shell voxels (set bits of shellm) in words [0, i).  Field pass 2 stores
+   the shell voxels' values only, densely in that order: voxel bit b of word i at
+   spre[i] + popcount(shellm[i] below b). */
+__global__ void __launch_bounds__(SCAN_NT) k_shell_reduce(const uint64_t *__restrict__ shellm, uint64_t n,
+                                                          uint32_t *__restrict__ sums)
+{
+    __shared__ uint32_t lds[SCAN_NT / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        const uint64_t i = base + (uint64_t)k * SCAN_NT + threadIdx.x;
+        if (i < n) s += (uint32_t)__popcll(shellm[i]);
+    }
+    const uint32_t t = block_sum<SCAN_NT>(s, lds);
+    if (threadIdx.x == 0) sums[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(SCAN_NT) k_shell_scan(const uint64_t *__restrict__ shellm, uint64_t n,
+                                                        const uint32_t *__restrict__ offsets, uint32_t *__restrict__ spre)
+{
+    /* wave w of the block takes words [w * 512, (w + 1) * 512) of the tile, 64 consecutive words
+       per load, and scans them in order: lane prefixes per group of 64, carried across groups */
+    __shared__ uint32_t wsum[SCAN_NT / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)wid * (64 * SCAN_IPT) + lane;
+    uint32_t v[SCAN_IPT];
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        const uint64_t i = base + 64 * k;
+        v[k] = (i < n) ? (uint32_t)__popcll(shellm[i]) : 0u;
+    }
+    uint32_t run = 0; /* words of the wave's earlier groups */
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        uint32_t x = v[k];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        const uint32_t tot = __shfl(x, 63, 64);
+        v[k] = run + x - v[k]; /* exclusive prefix inside the wave's range */
+        run += tot;
+    }
+    if (lane == 0) wsum[wid] = run;
+    __syncthreads();
+    uint32_t off = offsets[blockIdx.x];
+    for (int w = 0; w < wid; w++) off += wsum[w];
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        const uint64_t i = base + 64 * k;
+        if (i < n) spre[i] = off + v[k];
+    }
+}
+
 /* Marching-cubes word scans in one pass: per row word its vertex count, triangle count and
    "active" flag (either nonzero) are scanned together.  Plane 0 = triangles, 1 = vertices,
    2 = active words; tile sums are stored plane-major (3 x ntiles). */
@@ -1218,13 +1274,15 @@ __device__ __forceinline__ void emit_sign_bits(uint32_t *__restrict__ negm, uint
 
 template <typename DT>
 __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *__restrict__ outm_all,
-                                                    const uint64_t *__restrict__ shellm_all, float *__restrict__ field_all,
+                                                    const uint64_t *__restrict__ shellm_all, const uint32_t *__restrict__ spre_all,
+                                                    float *__restrict__ field,
                                                     uint32_t *__restrict__ negm_all, uint32_t *__restrict__ zerom_all,
                                                     const int *__restrict__ offs, const int *__restrict__ norm_start,
                                                     const int *__restrict__ enum_end)
 {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ int s_flags;
+    __shared__ uint32_t sbase[P2_TX * P2_TY]; /* shell rank of each own row's first tile voxel */
     constexpr int INF = (sizeof(DT) == 1) ? 0xFF : 0xFFFF;
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, blockIdx.x, lb)];
@@ -1232,7 +1290,7 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *_
     const int r = G.r, g = G.g, W = (int)M.W;
     const uint64_t *__restrict__ outm = outm_all + M.word_off;
     const uint64_t *__restrict__ shellm = shellm_all + M.word_off;
-    float *__restrict__ field = field_all + M.vox_off;
+    const uint32_t *__restrict__ spre = spre_all + M.word_off;
     uint32_t *__restrict__ negm = negm_all + 2 * M.word_off;
     uint32_t *__restrict__ zerom = zerom_all + 2 * M.word_off;
     const int x0 = M.p2_lo, x1 = M.p2_hi;
@@ -1265,7 +1323,11 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *_
         uint32_t m = 0u;
         if (gx < x1 && gx < g && gy < g) m = (uint32_t)row_bits(shellm + ((size_t)gx * g + gy) * W, W, tz0, P2_TZ);
         own[t] = m;
-        if (m) fl |= 1;
+        if (m) {
+            fl |= 1;
+            const size_t wi = ((size_t)gx * g + gy) * W + (tz0 >> 6);
+            sbase[t] = spre[wi] + (uint32_t)__popcll(shellm[wi] & bits_below(tz0 & 63, 0));
+        }
     }
     if (fl) atomicOr(&s_flags, fl);
     __syncthreads();
@@ -1282,7 +1344,8 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *_
             if (gx >= x1 || gx >= g) break;
             bool out = (rows[(size_t)(tx + r) * LY + (ty + r)] >> (tz + r)) & 1ull;
             float res = out ? QS_PROBE : -dx;
-            if (vyz) field[((size_t)gx * g + gy) * g + gz] = res;
+            const uint32_t om = own[tx * P2_TY + ty];
+            if (vyz && ((om >> tz) & 1u)) field[sbase[tx * P2_TY + ty] + (uint32_t)__popc(om & ((1u << tz) - 1u))] = res;
             emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, res);
         }
         return;
@@ -1329,7 +1392,6 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *_
     for (int tx = 0; tx < P2_TX; tx++) {
         int gx = tx0 + tx;
         if (gx >= x1 || gx >= g) break;
-        size_t vid = ((size_t)gx * g + gy) * g + gz;
         bool out = (rows[(size_t)(tx + r) * LY + (ty + r)] >> (tz + r)) & 1ull;
         bool shell = (own[tx * P2_TY + ty] >> tz) & 1u;
         float res;
@@ -1381,7 +1443,8 @@ __global__ void __launch_bounds__(256) k_pass2_tile(QsBatch B, const uint64_t *_
                 res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
             }
         }
-        if (vyz) field[vid] = res;
+        if (vyz && shell)
+            field[sbase[tx * P2_TY + ty] + (uint32_t)__popc(own[tx * P2_TY + ty] & ((1u << tz) - 1u))] = res;
         emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, res);
     }
 }
@@ -1440,17 +1503,20 @@ __device__ __forceinline__ void list_column(const uint32_t *cpre, const uint32_t
 
 template <int R>
 __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__restrict__ outm_all,
-                                                 const uint64_t *__restrict__ shellm_all, float *__restrict__ field_all,
-                                                 uint32_t *__restrict__ negm_all, uint32_t *__restrict__ zerom_all,
-                                                 const int *__restrict__ offs, const int *__restrict__ norm_start,
-                                                 const int *__restrict__ enum_end, QsNormFlags nf,
-                                                 const uint8_t *__restrict__ tflag, uint32_t xchunk)
+                                                 const uint64_t *__restrict__ shellm_all, const uint32_t *__restrict__ spre_all,
+                                                 float *__restrict__ field, uint32_t *__restrict__ negm_all,
+                                                 uint32_t *__restrict__ zerom_all, const int *__restrict__ offs,
+                                                 const int *__restrict__ norm_start, const int *__restrict__ enum_end,
+                                                 QsNormFlags nf, const uint8_t *__restrict__ tflag, uint32_t xchunk)
 {
     constexpr int LX = P2R_TX + 2 * R, LY = P2_TY + 2 * R, LZ = P2_TZ + 2 * R;
     constexpr int INF = 255;
     static_assert(LZ <= 64 && 3 * R * R < INF, "window too wide for this kernel");
     __shared__ uint64_t rows[LX * LY];
+    /* ownout: the own rows' outside bits, read only by a tile without a shell voxel; after that
+       test the array holds the own rows' shell ranks (sbase) instead */
     __shared__ uint32_t own[P2R_TX * P2_TY], ownout[P2R_TX * P2_TY];
+    uint32_t *const sbase = ownout;
     __shared__ uint32_t DYw[LX * P2_TY * P2_TZ / 4];
     uint8_t *DY = (uint8_t *)DYw;
     /* y-pass work list: per halo x plane lx, the z columns (bit z) that need y minima, and the
@@ -1476,7 +1542,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     const int g = G.g, W = (int)M.W;
     const uint64_t *__restrict__ outm = outm_all + M.word_off;
     const uint64_t *__restrict__ shellm = shellm_all + M.word_off;
-    float *__restrict__ field = field_all + M.vox_off;
+    const uint32_t *__restrict__ spre = spre_all + M.word_off;
     uint32_t *__restrict__ negm = negm_all + 2 * M.word_off;
     uint32_t *__restrict__ zerom = zerom_all + 2 * M.word_off;
     const int x1 = M.p2_hi;
@@ -1492,18 +1558,27 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
        without a shell voxel (outside the molecule or deep inside it) is finished from them alone,
        without its halo */
     static_assert(P2R_TX * P2_TY == 128, "one thread per own row of each mask");
+    static_assert(P2_TZ == 32, "a tile's z range is one half of a row word");
     if (threadIdx.x < 8) {
         const int t = threadIdx.x;
         const uint64_t w = t < 2 ? nf.w[0] : (t < 4 ? nf.w[1] : (t < 6 ? nf.w[2] : nf.w[3]));
         nfw[t] = (uint32_t)(w >> (32 * (t & 1)));
     }
     int has_shell;
+    uint32_t sb = 0u;
     {
         const int t = threadIdx.x & 127, tx = t / P2_TY, ty_ = t - tx * P2_TY;
         const int gx = tx0 + tx, gy_ = ty0 + ty_;
         const uint64_t *__restrict__ src = threadIdx.x < 128 ? shellm : outm;
         uint32_t m = 0u;
-        if (gx < x1 && gx < g && gy_ < g) m = (uint32_t)row_bits(src + ((size_t)gx * g + gy_) * W, W, tz0, P2_TZ);
+        if (gx < x1 && gx < g && gy_ < g) {
+            /* the tile's 32 z of the row are one half of a row word (tz0 is a multiple of 32); the
+               shell threads also take the row's shell rank there, loaded alongside */
+            const size_t wi = ((size_t)gx * g + gy_) * W + (tz0 >> 6);
+            const uint64_t wv = src[wi];
+            m = (uint32_t)(wv >> (tz0 & 63));
+            if (threadIdx.x < 128) sb = spre[wi] + (uint32_t)__popcll(wv & bits_below(tz0 & 63, 0));
+        }
         if (threadIdx.x < 128) {
             own[t] = m;
             /* shell bits of each own x plane, over its P2_TY rows (consecutive lanes) */
@@ -1531,6 +1606,14 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         }
         return;
     }
+    /* shell rank of each own row's first tile voxel: the row's shell voxels are stored from there
+       (ownout was last read before the barrier above) */
+    if (threadIdx.x < 128) sbase[threadIdx.x] = sb;
+    /* index of own voxel (tx, ty, tz) in the stored shell field */
+    auto fidx = [&](int tx, int ty_, int tz_) {
+        const int t = tx * P2_TY + ty_;
+        return sbase[t] + (uint32_t)__popc(own[t] & ((1u << tz_) - 1u));
+    };
 
     int fl = 0;
 #pragma unroll
@@ -1559,7 +1642,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         for (int tx = 0; tx < P2R_TX; tx++) {
             int gx = tx0 + tx;
             if (gx >= x1 || gx >= g) break;
-            if (vyz && ((own[tx * P2_TY + ty] >> tz) & 1u)) field[((size_t)gx * g + gy) * g + gz] = -dx;
+            if (vyz && ((own[tx * P2_TY + ty] >> tz) & 1u)) field[fidx(tx, ty, tz)] = -dx;
             emit_sign_bits(negm, zerom, W, g, gx, gy, tz0, vyz, -dx);
         }
         return;
@@ -1733,19 +1816,18 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         }
         const float min_dist = sqrt_rn(min_sq);
         const float res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
-        field[((size_t)gx * g + gy_) * g + gz_] = res;
+        field[fidx(tx, ty_, tz_)] = res;
         if (res < 0.0f) atomicOr(&sneg[tx * P2_TY + ty_], 1u << tz_);
         if (res == 0.0f) atomicOr(&szero[tx * P2_TY + ty_], 1u << tz_);
     };
     /* voxel tx of the thread's column, whose minimal norm is nmin (INF: no outside voxel within
        the window): settled now, or queued */
     auto handle = [&](int tx, int nmin) {
-        const int gx = tx0 + tx;
         const bool out = (rows[(tx + R) * LY + (ty + R)] >> (tz + R)) & 1ull;
         const bool shell = (own[tx * P2_TY + ty] >> tz) & 1u; /* never outside */
         const bool traced = vyz && shell && nmin < INF;
         /* outside: the probe radius; inside, or shell without an outside voxel in reach: -dx */
-        if (vyz && shell && !traced) field[((size_t)gx * g + gy) * g + gz] = -dx;
+        if (vyz && shell && !traced) field[fidx(tx, ty, tz)] = -dx;
         const uint64_t nb = __ballot(vyz && !traced && !out);
         if ((lane & 31) == 0) {
             sneg[tx * P2_TY + ty] = (uint32_t)(lane ? (nb >> 32) : nb);
@@ -1843,7 +1925,8 @@ __device__ __forceinline__ bool mask_bit(const uint64_t *__restrict__ m, int g, 
 
 /* direct (2r+1)^3 window, one thread per voxel: any r; also a cross-check of the tile kernel */
 __global__ void __launch_bounds__(256) k_pass2_direct(QsBatch B, const uint64_t *__restrict__ outm_all,
-                                                      const uint64_t *__restrict__ shellm_all, float *__restrict__ field_all,
+                                                      const uint64_t *__restrict__ shellm_all,
+                                                      const uint32_t *__restrict__ spre_all, float *__restrict__ field,
                                                       uint32_t *__restrict__ negm_all, uint32_t *__restrict__ zerom_all)
 {
     uint32_t lb;
@@ -1852,7 +1935,7 @@ __global__ void __launch_bounds__(256) k_pass2_direct(QsBatch B, const uint64_t 
     const int g = G.g, r = G.r, W = (int)M.W;
     const uint64_t *__restrict__ outm = outm_all + M.word_off;
     const uint64_t *__restrict__ shellm = shellm_all + M.word_off;
-    float *__restrict__ field = field_all + M.vox_off;
+    const uint32_t *__restrict__ spre = spre_all + M.word_off;
     uint32_t *__restrict__ negm = negm_all + 2 * M.word_off;
     uint32_t *__restrict__ zerom = zerom_all + 2 * M.word_off;
     const uint32_t nbx = (g + 63) / 64, nby = (g + 3) / 4;
@@ -1863,12 +1946,12 @@ __global__ void __launch_bounds__(256) k_pass2_direct(QsBatch B, const uint64_t 
     const int x1 = M.p2_hi;
     if (x >= x1 || y >= g) return;
     const bool valid = z < g;
-    size_t vid = ((size_t)x * g + y) * g + z;
     const float dx = G.dx;
     float res;
+    bool shell = false;
     if (!valid || mask_bit(outm, g, W, x, y, z)) {
         res = QS_PROBE;
-    } else if (!mask_bit(shellm, g, W, x, y, z)) {
+    } else if (!(shell = mask_bit(shellm, g, W, x, y, z))) {
         res = -dx;
     } else {
         float vx = grid_pos(G.ox, x, dx), vy = grid_pos(G.oy, y, dx), vz = grid_pos(G.oz, z, dx);
@@ -1887,7 +1970,10 @@ __global__ void __launch_bounds__(256) k_pass2_direct(QsBatch B, const uint64_t 
         }
         res = (min_dist < 999.0f) ? (QS_PROBE - min_dist) : -dx;
     }
-    if (valid) field[vid] = res;
+    if (shell) {
+        const size_t wi = ((size_t)x * g + y) * W + (z >> 6);
+        field[spre[wi] + (uint32_t)__popcll(shellm[wi] & bits_below(z & 63, 0))] = res;
+    }
     uint64_t n = __ballot(valid && res < 0.0f), zr = __ballot(valid && res == 0.0f);
     if ((threadIdx.x & 63) == 0) {
         size_t w32 = ((size_t)x * g + y) * (2 * (size_t)W) + 2 * (size_t)bidx;
@@ -2245,40 +2331,46 @@ __device__ __forceinline__ void item_word(const ItemWords &S, const uint32_t *__
 }
 
 /* Field value of voxel z of grid row `row` (= x * g + y).  Field pass 2 stores the shell voxels'
-   values only; every other voxel is the probe radius outside and -dx inside, read from the outside
-   and shell masks (W words per row).  shellm == nullptr: F holds every voxel. */
-__device__ __forceinline__ float field_at(const float *__restrict__ F, const uint64_t *__restrict__ outm,
-                                          const uint64_t *__restrict__ shellm, size_t row, int g, int W, int z, float dx)
+   values only, densely in shell-rank order (F[spre[word] + shell voxels of the word below z]);
+   every other voxel is the probe radius outside and -dx inside, read from the outside and shell
+   masks (W words per row).  shellm == nullptr: F holds every voxel, in grid order. */
+__device__ __forceinline__ float field_at(const float *__restrict__ F, const uint32_t *__restrict__ spre,
+                                          const uint64_t *__restrict__ outm, const uint64_t *__restrict__ shellm, size_t row,
+                                          int g, int W, int z, float dx)
 {
-    /* the three loads are independent (F is allocated for every voxel; a non-shell voxel's stale
-       value is discarded) */
-    const float f = F[row * g + z];
-    if (!shellm) return f;
+    if (!shellm) return F[row * g + z];
     const size_t wi = row * W + (z >> 6);
     const uint64_t sh = shellm[wi], ou = outm[wi];
+    const uint32_t base = spre[wi];
     const int b = z & 63;
-    if ((sh >> b) & 1ull) return f;
+    if ((sh >> b) & 1ull) return F[base + (uint32_t)__popcll(sh & ((1ull << b) - 1ull))];
     return ((ou >> b) & 1ull) ? QS_PROBE : -dx;
 }
 
-/* every voxel of one molecule's field that field pass 2 leaves unstored, one thread per mask word */
-__global__ void __launch_bounds__(256) k_field_fill(const uint64_t *__restrict__ outm, const uint64_t *__restrict__ shellm,
-                                                    float *__restrict__ F, int g, int W, float dx)
+/* every voxel of one molecule's field in grid order (out) from the stored shell values (F, global
+   shell ranks spre) and the masks, one thread per mask word */
+__global__ void __launch_bounds__(256) k_field_expand(const float *__restrict__ F, const uint32_t *__restrict__ spre,
+                                                      const uint64_t *__restrict__ outm, const uint64_t *__restrict__ shellm,
+                                                      float *__restrict__ out, int g, int W, float dx)
 {
     const size_t wi = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (wi >= (size_t)g * g * W) return;
     const size_t row = wi / W;
     const int z0 = (int)(wi - row * W) * 64;
     const uint64_t o = outm[wi], sh = shellm[wi];
-    for (int b = 0; b < 64 && z0 + b < g; b++)
-        if (!((sh >> b) & 1ull)) F[row * g + z0 + b] = ((o >> b) & 1ull) ? QS_PROBE : -dx;
+    uint32_t k = spre[wi];
+    for (int b = 0; b < 64 && z0 + b < g; b++) {
+        if ((sh >> b) & 1ull) out[row * g + z0 + b] = F[k++];
+        else out[row * g + z0 + b] = ((o >> b) & 1ull) ? QS_PROBE : -dx;
+    }
 }
 
-void qs_launch_field_fill(const uint64_t *outm, const uint64_t *shellm, float *F, int g, int W, float dx, hipStream_t s)
+void qs_launch_field_expand(const float *F, const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *out, int g,
+                            int W, float dx, hipStream_t s)
 {
     const size_t n = (size_t)g * g * W;
     if (!n) return;
-    k_field_fill<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(outm, shellm, F, g, W, dx);
+    k_field_expand<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(F, spre, outm, shellm, out, g, W, dx);
 }
 
 /* vertices (vertexInterp/lerp MarchingCubes.cu:715-734, edge oriented low->high), one thread per
@@ -2288,7 +2380,8 @@ __global__ void __launch_bounds__(256) k_mc_vertices(const QsMol *__restrict__ m
                                                      const uint32_t *__restrict__ wcoarse, const uint32_t *__restrict__ list,
                                                      const uint32_t *__restrict__ avb, const uint32_t *__restrict__ vstart,
                                                      uint32_t nact, uint32_t nv, const uint64_t *__restrict__ S4_all,
-                                                     const float *__restrict__ F_all, const uint64_t *__restrict__ outm_all,
+                                                     const float *__restrict__ F_all, const uint32_t *__restrict__ spre_all,
+                                                     const uint64_t *__restrict__ outm_all,
                                                      const uint64_t *__restrict__ shellm_all, float *__restrict__ vert,
                                                      uint64_t *__restrict__ vkey, const uint32_t *__restrict__ d_counts)
 {
@@ -2343,13 +2436,16 @@ __global__ void __launch_bounds__(256) k_mc_vertices(const QsMol *__restrict__ m
     const float A[3] = {grid_pos(G.ox, x, dx), grid_pos(G.oy, y, dx), grid_pos(G.oz, z, dx)};
     float P[3] = {A[0], A[1], A[2]};
     if (sl < 3) {
-        const float *__restrict__ F = F_all + M.vox_off;
+        /* stored shell values are addressed by global shell rank; a full field by the molecule's
+           voxel offset */
+        const float *__restrict__ F = shellm_all ? F_all : F_all + M.vox_off;
+        const uint32_t *__restrict__ sp = shellm_all ? spre_all + M.word_off : nullptr;
         const uint64_t *__restrict__ om = outm_all + M.word_off;
         const uint64_t *__restrict__ sm = shellm_all ? shellm_all + M.word_off : nullptr;
         const size_t row = (size_t)x * g + y;
         const int W = (int)M.W;
-        const float f0 = field_at(F, om, sm, row, g, W, z, dx);
-        const float f1 = field_at(F, om, sm, row + (sl == 0 ? (size_t)g : (sl == 1 ? 1 : 0)), g, W, z + (sl == 2), dx);
+        const float f0 = field_at(F, sp, om, sm, row, g, W, z, dx);
+        const float f1 = field_at(F, sp, om, sm, row + (sl == 0 ? (size_t)g : (sl == 1 ? 1 : 0)), g, W, z + (sl == 2), dx);
         float B[3] = {A[0], A[1], A[2]};
         B[sl] = A[sl] + dx;
         const float t = (0.0f - f0) / (f1 - f0);
@@ -2712,10 +2808,11 @@ __global__ void __launch_bounds__(256) k_smooth_incid(const uint32_t *__restrict
 /* one molecule's field, addressed by grid row (x * g + y) and z */
 struct FieldView {
     const float *F;
+    const uint32_t *spre;
     const uint64_t *outm, *shellm;
     int g, W;
     float dx;
-    __device__ __forceinline__ float at(size_t row, int z) const { return field_at(F, outm, shellm, row, g, W, z, dx); }
+    __device__ __forceinline__ float at(size_t row, int z) const { return field_at(F, spre, outm, shellm, row, g, W, z, dx); }
 };
 
 /* derivative along axis a (0 x, 1 y, 2 z) at grid point (row, z), whose coordinate on that axis is i */
@@ -2731,7 +2828,7 @@ __device__ __forceinline__ float grad_axis(const FieldView &F, size_t row, int z
 __global__ void __launch_bounds__(256) k_vertex_normals(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
                                                         const uint32_t *__restrict__ vcoarse, const uint64_t *__restrict__ vkey,
                                                         uint32_t nv, const float *__restrict__ F_all,
-                                                        const uint64_t *__restrict__ outm_all,
+                                                        const uint32_t *__restrict__ spre_all, const uint64_t *__restrict__ outm_all,
                                                         const uint64_t *__restrict__ shellm_all, float *__restrict__ out)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2739,7 +2836,8 @@ __global__ void __launch_bounds__(256) k_vertex_normals(const QsMol *__restrict_
     const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
     const int g = M.G.g;
     const float dx = M.G.dx;
-    const FieldView F{F_all + M.vox_off, outm_all + M.word_off, shellm_all ? shellm_all + M.word_off : nullptr, g, (int)M.W, dx};
+    const FieldView F{shellm_all ? F_all : F_all + M.vox_off, shellm_all ? spre_all + M.word_off : nullptr, outm_all + M.word_off,
+                      shellm_all ? shellm_all + M.word_off : nullptr, g, (int)M.W, dx};
     const uint64_t key = vkey[v];
     const int sl = (int)(key & 3ull);
     const uint32_t p = (uint32_t)(key >> 2), ug = (uint32_t)g;
@@ -3063,16 +3161,16 @@ uint32_t qs_pass2_blocks(const QsMol &M, int path)
 }
 
 /* one launch for a group of molecules sharing r (and so the path) */
-void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm, float *field,
-                     uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
+void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm,
+                     const uint32_t *spre, float *field, uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
                      QsNormFlags nf, const uint8_t *tflag, uint32_t xchunk, hipStream_t s)
 {
     if (path == 0) {
         xchunk = std::max(xchunk, 1u);
 #define QS_P2R(RR)                                                                                                    \
     case RR:                                                                                                          \
-        hipLaunchKernelGGL(k_pass2_r<RR>, dim3(nblocks), dim3(256), 0, s, B, outm, shellm, field, negm, zerom, offs, \
-                           norm_start, enum_end, nf, tflag, xchunk);                                                  \
+        hipLaunchKernelGGL(k_pass2_r<RR>, dim3(nblocks), dim3(256), 0, s, B, outm, shellm, spre, field, negm, zerom, \
+                           offs, norm_start, enum_end, nf, tflag, xchunk);                                            \
         return;
         switch (r) {
             QS_P2R(1)
@@ -3090,11 +3188,20 @@ void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_
 #undef QS_P2R
     }
     if (path <= 1) {
-        hipLaunchKernelGGL(k_pass2_tile<uint16_t>, dim3(nblocks), dim3(256), qs_pass2_lds_bytes(r), s, B, outm, shellm, field,
-                           negm, zerom, offs, norm_start, enum_end);
+        hipLaunchKernelGGL(k_pass2_tile<uint16_t>, dim3(nblocks), dim3(256), qs_pass2_lds_bytes(r), s, B, outm, shellm, spre,
+                           field, negm, zerom, offs, norm_start, enum_end);
         return;
     }
-    hipLaunchKernelGGL(k_pass2_direct, dim3(nblocks), dim3(256), 0, s, B, outm, shellm, field, negm, zerom);
+    hipLaunchKernelGGL(k_pass2_direct, dim3(nblocks), dim3(256), 0, s, B, outm, shellm, spre, field, negm, zerom);
+}
+
+void qs_launch_shell_scan(const uint64_t *shellm, uint64_t nw, uint32_t *sums, uint32_t *totals, uint32_t *spre, hipStream_t s)
+{
+    if (!nw) return;
+    const uint64_t ntiles = qs_scan_tiles(nw);
+    hipLaunchKernelGGL(k_shell_reduce, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, shellm, nw, sums);
+    hipLaunchKernelGGL(k_scan_planes, dim3(1), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
+    hipLaunchKernelGGL(k_shell_scan, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, shellm, nw, sums, spre);
 }
 
 void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *outm, uint64_t *shellm, hipStream_t s)
@@ -3116,14 +3223,14 @@ void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const ui
 
 void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
                            const uint32_t *avb, uint32_t *vstart, uint32_t nact, uint32_t nv, const uint64_t *S4,
-                           const float *F, const uint64_t *outm, const uint64_t *shellm, float *vert, uint64_t *vkey,
-                           const uint32_t *d_counts, hipStream_t s)
+                           const float *F, const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *vert,
+                           uint64_t *vkey, const uint32_t *d_counts, hipStream_t s)
 {
     if (!nv || !nact) return;
     hipLaunchKernelGGL(k_block_starts, dim3(pgrid(nblk(nact, 256))), dim3(256), 0, s, avb, nact, nv,
                        d_counts ? d_counts + 2 : nullptr, d_counts ? d_counts + 1 : nullptr, vstart, nullptr, 0u);
     hipLaunchKernelGGL(k_mc_vertices, dim3(pgrid(nblk(nv, 256))), dim3(256), 0, s, mols, wpre, wcoarse, list, avb, vstart, nact, nv,
-                       S4, F, outm, shellm, vert, vkey, d_counts);
+                       S4, F, spre, outm, shellm, vert, vkey, d_counts);
 }
 
 void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
@@ -3153,10 +3260,11 @@ void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *v
 }
 
 void qs_launch_vertex_normals(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *vkey, uint32_t nv,
-                              const float *F, const uint64_t *outm, const uint64_t *shellm, float *out, hipStream_t s)
+                              const float *F, const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *out,
+                              hipStream_t s)
 {
     if (!nv) return;
-    hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, vkey, nv, F, outm, shellm,
+    hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(nv, 256)), dim3(256), 0, s, mols, vpre, vcoarse, vkey, nv, F, spre, outm, shellm,
                        out);
 }
 
