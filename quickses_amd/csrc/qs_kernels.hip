@@ -386,37 +386,66 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict
                                                       uint32_t *__restrict__ list, uint32_t *__restrict__ avb,
                                                       uint32_t *__restrict__ atb, uint32_t list_base)
 {
-    __shared__ uint32_t lds[SCAN_NT / 64 + 1];
-    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_IPT;
-    uint32_t v[SCAN_IPT], t[SCAN_IPT];
-    uint32_t sv = 0, st = 0, sa = 0;
+    /* wave w of the block takes words [w * 512, (w + 1) * 512) of the tile, 64 consecutive words
+       per load (as k_shell_scan); the three scans run per group of 64 in order, carried across
+       groups, and each group's active words are listed in order by ballot */
+    __shared__ uint32_t wsum[3][SCAN_NT / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)wid * (64 * SCAN_IPT) + lane;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t v[SCAN_IPT], t[SCAN_IPT], pv[SCAN_IPT], pt[SCAN_IPT], pa[SCAN_IPT];
+    uint64_t am[SCAN_IPT];
 #pragma unroll
     for (int k = 0; k < SCAN_IPT; k++) {
-        const uint64_t i = base + k;
+        const uint64_t i = base + 64 * k;
         v[k] = (i < n) ? vcnt[i] : 0u;
         t[k] = (i < n) ? tcnt[i] : 0u;
-        sv += v[k];
-        st += t[k];
-        sa += (v[k] | t[k]) ? 1u : 0u;
     }
-    uint32_t tot;
-    uint32_t rt = sums[blockIdx.x] + block_excl_scan<SCAN_NT>(st, &tot, lds);
-    uint32_t rv = sums[ntiles + blockIdx.x] + block_excl_scan<SCAN_NT>(sv, &tot, lds);
-    uint32_t ra = sums[2 * ntiles + blockIdx.x] + block_excl_scan<SCAN_NT>(sa, &tot, lds);
+    uint32_t rv = 0, rt = 0, ra = 0; /* the wave's earlier groups */
 #pragma unroll
     for (int k = 0; k < SCAN_IPT; k++) {
-        const uint64_t i = base + k;
-        if (i < n) {
-            vbw[i] = rv;
-            tbw[i] = rt;
-            if (v[k] | t[k]) {
-                avb[ra] = rv;
-                atb[ra] = rt;
-                list[ra++] = list_base + (uint32_t)i;
+        uint32_t xv = v[k], xt = t[k];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t yv = __shfl_up(xv, d, 64), yt = __shfl_up(xt, d, 64);
+            if (lane >= d) {
+                xv += yv;
+                xt += yt;
             }
         }
-        rv += v[k];
-        rt += t[k];
+        am[k] = __ballot((v[k] | t[k]) != 0u);
+        pv[k] = rv + xv - v[k];
+        pt[k] = rt + xt - t[k];
+        pa[k] = ra + (uint32_t)__popcll(am[k] & below);
+        rv += __shfl(xv, 63, 64);
+        rt += __shfl(xt, 63, 64);
+        ra += (uint32_t)__popcll(am[k]);
+    }
+    if (lane == 0) {
+        wsum[0][wid] = rv;
+        wsum[1][wid] = rt;
+        wsum[2][wid] = ra;
+    }
+    __syncthreads();
+    uint32_t ov = sums[ntiles + blockIdx.x], ot = sums[blockIdx.x], oa = sums[2 * ntiles + blockIdx.x];
+    for (int w = 0; w < wid; w++) {
+        ov += wsum[0][w];
+        ot += wsum[1][w];
+        oa += wsum[2][w];
+    }
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        const uint64_t i = base + 64 * k;
+        if (i < n) {
+            vbw[i] = ov + pv[k];
+            tbw[i] = ot + pt[k];
+            if ((am[k] >> lane) & 1ull) {
+                const uint32_t r = oa + pa[k];
+                avb[r] = ov + pv[k];
+                atb[r] = ot + pt[k];
+                list[r] = list_base + (uint32_t)i;
+            }
+        }
     }
 }
 
