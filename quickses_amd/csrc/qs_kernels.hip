@@ -1065,34 +1065,69 @@ __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *_
     const int rows_hi = ylast - y0; /* rows 0..rows_hi of the slab are labelled */
     int *qid = q_id[wid], *qrows = q_rows[wid], *qz = q_z[wid];
 
-    /* one queued atom per lane: its rows in turn */
+    /* The wave's queued atoms (lane k < count holds atom k) as (atom, row) items: atom k's rows
+       r0..r1 are items pre_k .. pre_k + r1 - r0, and the lanes take the items 64 at a time, so
+       that atoms reaching few rows do not leave lanes idle while others walk many (the ORs are
+       order-independent).  Called by the whole wave. */
     auto process = [&](int count) {
-        if (lane >= count) return;
-        const int id = qid[lane], rr = qrows[lane], zr = qz[lane];
-        const float4 a = sx[id], ax = aux[id];
-        const float sN = ax.x, sD = ax.y;
-        const float ex = a.x - px;
-        const float X = ex * ex;
-        const int zlo = zr & 0xFFFF, zhi = zr >> 16;
-        const float zc = (a.z - G.oz) * inv_dx;
-        RowAtom A;
-        A.az = a.z;
-        A.dx = dx;
-        A.oz = G.oz;
-        int zb = iclamp((int)floorf(zc), zlo, zhi);
-        if (zb + 1 <= zhi && A.Zf(zb + 1) < A.Zf(zb)) zb = zb + 1;
-        const int r1 = rr >> 8;
-        for (int yr = rr & 0xFF; yr <= r1; yr++) {
-            const float ey = a.y - grid_pos(G.oy, y0 + yr, dx);
+        int rr = 0, nrows = 0, zr = 0, zb = 0;
+        float X = 0.0f, sN = 0.0f, sD = 0.0f, zc = 0.0f, az = 0.0f, ay = 0.0f;
+        if (lane < count) {
+            const int id = qid[lane];
+            rr = qrows[lane];
+            zr = qz[lane];
+            nrows = (rr >> 8) - (rr & 0xFF) + 1;
+            const float4 a = sx[id], ax = aux[id];
+            sN = ax.x;
+            sD = ax.y;
+            const float ex = a.x - px;
+            X = ex * ex;
+            zc = (a.z - G.oz) * inv_dx;
+            az = a.z;
+            ay = a.y;
+            const int zlo = zr & 0xFFFF, zhi = zr >> 16;
+            RowAtom A;
+            A.az = a.z;
+            A.dx = dx;
+            A.oz = G.oz;
+            zb = iclamp((int)floorf(zc), zlo, zhi);
+            if (zb + 1 <= zhi && A.Zf(zb + 1) < A.Zf(zb)) zb = zb + 1;
+        }
+        int pre = nrows;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(pre, d, 64);
+            if (lane >= d) pre += t;
+        }
+        const int nitems = __shfl(pre, 63, 64);
+        pre -= nrows; /* exclusive */
+        for (int base = 0; base < nitems; base += 64) {
+            const int j = base + lane;
+            /* the item's atom: the last lane whose first item is at or below j */
+            int o = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1)
+                if (__shfl(pre, o + step, 64) <= j) o += step;
+            const float oX = __shfl(X, o, 64), osN = __shfl(sN, o, 64), osD = __shfl(sD, o, 64);
+            const float ozc = __shfl(zc, o, 64), oaz = __shfl(az, o, 64), oay = __shfl(ay, o, 64);
+            const int orr = __shfl(rr, o, 64), ozr = __shfl(zr, o, 64), ozb = __shfl(zb, o, 64), opre = __shfl(pre, o, 64);
+            if (j >= nitems) continue;
+            const int yr = (orr & 0xFF) + (j - opre);
+            const int zlo = ozr & 0xFFFF, zhi = ozr >> 16;
+            RowAtom A;
+            A.az = oaz;
+            A.dx = dx;
+            A.oz = G.oz;
+            const float ey = oay - grid_pos(G.oy, y0 + yr, dx);
             const float Y = ey * ey;
-            A.S = X + Y;
-            if (!(A.S < sN)) continue;
+            A.S = oX + Y;
+            if (!(A.S < osN)) continue;
             int L, Rr;
-            row_interval(A, sN, zlo, zhi, zb, zc, inv_dx, L, Rr);
+            row_interval(A, osN, zlo, zhi, ozb, ozc, inv_dx, L, Rr);
             if (L > Rr) continue;
             lds_or_range(nearw + (size_t)yr * W, L, Rr);
-            if (!(A.S < sD)) continue;
-            row_interval(A, sD, L, Rr, zb, zc, inv_dx, L, Rr);
+            if (!(A.S < osD)) continue;
+            row_interval(A, osD, L, Rr, ozb, ozc, inv_dx, L, Rr);
             if (L <= Rr) lds_or_range(insw + (size_t)yr * W, L, Rr);
         }
     };
@@ -3010,14 +3045,20 @@ __global__ void __launch_bounds__(256) k_tri_kept_before(const int *__restrict__
                                                          const uint64_t *__restrict__ idx, int n, uint32_t tcap,
                                                          uint32_t *__restrict__ out)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    /* one wave per boundary: the degenerate triangles of the boundary's 256-triangle block below
+       it, four per lane, counted by ballot */
+    const int i = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (i >= n) return;
     /* (boundaries past the triangle pool occur only when the mesh overflowed it) */
     const uint32_t t = (uint32_t)min(idx[i], (uint64_t)tcap);
     const uint32_t b = t >> 8;
-    uint32_t d = dpre[b];
-    for (uint32_t u = b << 8; u < t; u++) d += tri_degenerate(tri, u) ? 1u : 0u;
-    out[i] = t - d;
+    uint32_t d = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t u = (b << 8) + (uint32_t)(64 * k + lane);
+        d += (uint32_t)__popcll(__ballot(u < t && tri_degenerate(tri, u)));
+    }
+    if (lane == 0) out[i] = t - dpre[b] - d;
 }
 
 /* plain device copy (16-byte vectors, grid-stride): the reference point for traffic counters */
@@ -3341,7 +3382,7 @@ void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint6
                                hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_tri_kept_before, dim3(nblk(n, 256)), dim3(256), 0, s, tri, dpre, idx, n, tcap, out);
+    hipLaunchKernelGGL(k_tri_kept_before, dim3(nblk(n, 4)), dim3(256), 0, s, tri, dpre, idx, n, tcap, out);
 }
 
 /* scan helpers */
