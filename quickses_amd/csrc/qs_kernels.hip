@@ -975,6 +975,12 @@ __device__ __forceinline__ void lds_or_range(uint64_t *row, int L, int R)
 #define P1S_MAXROWS 256   /* rows per slab block at most (row indexes in the queue are 8-bit) */
 #define P1S_DEFROWS 64    /* rows per slab block when no tuned count is set */
 #define P1S_MAXCOL 256
+/* a wave takes its queued atoms as (atom, row) items when K1 * (item rounds of 64) < K2 * (the
+   largest row count of one atom), i.e. when walking atoms would leave most lanes idle */
+#ifndef P1_FLAT_K1
+#define P1_FLAT_K1 3
+#define P1_FLAT_K2 2
+#endif
 __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *__restrict__ cstart_all,
                                                     const float4 *__restrict__ sx, const float4 *__restrict__ aux,
                                                     uint64_t *__restrict__ outm_all, uint64_t *__restrict__ shellm_all,
@@ -1065,10 +1071,30 @@ __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *_
     const int rows_hi = ylast - y0; /* rows 0..rows_hi of the slab are labelled */
     int *qid = q_id[wid], *qrows = q_rows[wid], *qz = q_z[wid];
 
-    /* The wave's queued atoms (lane k < count holds atom k) as (atom, row) items: atom k's rows
-       r0..r1 are items pre_k .. pre_k + r1 - r0, and the lanes take the items 64 at a time, so
-       that atoms reaching few rows do not leave lanes idle while others walk many (the ORs are
-       order-independent).  Called by the whole wave. */
+    /* One (atom, row) pair: the atom's exact near and inside z-intervals in the row, ORed into the
+       slab's masks (the ORs are order-independent). */
+    auto row_item = [&](int yr, float X, float sN, float sD, float zc, float az, float ay, int zr, int zb) {
+        const int zlo = zr & 0xFFFF, zhi = zr >> 16;
+        RowAtom A;
+        A.az = az;
+        A.dx = dx;
+        A.oz = G.oz;
+        const float ey = ay - grid_pos(G.oy, y0 + yr, dx);
+        const float Y = ey * ey;
+        A.S = X + Y;
+        if (!(A.S < sN)) return;
+        int L, Rr;
+        row_interval(A, sN, zlo, zhi, zb, zc, inv_dx, L, Rr);
+        if (L > Rr) return;
+        lds_or_range(nearw + (size_t)yr * W, L, Rr);
+        if (!(A.S < sD)) return;
+        row_interval(A, sD, L, Rr, zb, zc, inv_dx, L, Rr);
+        if (L <= Rr) lds_or_range(insw + (size_t)yr * W, L, Rr);
+    };
+    /* The wave's queued atoms, lane k < count holding atom k.  Each lane walks its own atom's rows
+       r0..r1, unless the row counts are so uneven that most lanes would idle: the atoms' rows are
+       then items pre_k .. pre_k + r1 - r0 that the lanes take 64 at a time, each with its atom's
+       values from the lane that holds it.  Called by the whole wave. */
     auto process = [&](int count) {
         int rr = 0, nrows = 0, zr = 0, zb = 0;
         float X = 0.0f, sN = 0.0f, sD = 0.0f, zc = 0.0f, az = 0.0f, ay = 0.0f;
@@ -1101,6 +1127,14 @@ __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *_
         }
         const int nitems = __shfl(pre, 63, 64);
         pre -= nrows; /* exclusive */
+        int maxrows = nrows;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) maxrows = max(maxrows, __shfl_xor(maxrows, d, 64));
+        if (P1_FLAT_K1 * ((nitems + 63) >> 6) >= P1_FLAT_K2 * maxrows) {
+            if (lane < count)
+                for (int yr = rr & 0xFF; yr <= (rr >> 8); yr++) row_item(yr, X, sN, sD, zc, az, ay, zr, zb);
+            return;
+        }
         for (int base = 0; base < nitems; base += 64) {
             const int j = base + lane;
             /* the item's atom: the last lane whose first item is at or below j */
@@ -1111,24 +1145,7 @@ __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *_
             const float oX = __shfl(X, o, 64), osN = __shfl(sN, o, 64), osD = __shfl(sD, o, 64);
             const float ozc = __shfl(zc, o, 64), oaz = __shfl(az, o, 64), oay = __shfl(ay, o, 64);
             const int orr = __shfl(rr, o, 64), ozr = __shfl(zr, o, 64), ozb = __shfl(zb, o, 64), opre = __shfl(pre, o, 64);
-            if (j >= nitems) continue;
-            const int yr = (orr & 0xFF) + (j - opre);
-            const int zlo = ozr & 0xFFFF, zhi = ozr >> 16;
-            RowAtom A;
-            A.az = oaz;
-            A.dx = dx;
-            A.oz = G.oz;
-            const float ey = oay - grid_pos(G.oy, y0 + yr, dx);
-            const float Y = ey * ey;
-            A.S = oX + Y;
-            if (!(A.S < osN)) continue;
-            int L, Rr;
-            row_interval(A, osN, zlo, zhi, ozb, ozc, inv_dx, L, Rr);
-            if (L > Rr) continue;
-            lds_or_range(nearw + (size_t)yr * W, L, Rr);
-            if (!(A.S < osD)) continue;
-            row_interval(A, osD, L, Rr, ozb, ozc, inv_dx, L, Rr);
-            if (L <= Rr) lds_or_range(insw + (size_t)yr * W, L, Rr);
+            if (j < nitems) row_item((orr & 0xFF) + (j - opre), oX, osN, osD, ozc, oaz, oay, ozr, ozb);
         }
     };
 
