@@ -1852,11 +1852,46 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         /* the x offsets (bit a + R) attaining the minimal norm */
         int nmin = INF;
         uint32_t amask = 0;
+        if constexpr (2 * R + 1 <= 16) {
+            /* taps a = 2k, 2k + 1 in the 16-bit halves of t[k] (an odd last tap is padded with the
+               largest value, which neither lowers the minimum nor equals it); the tie bits are
+               taken as in the y pass, then combined by packed multiply-adds: the low halves end up
+               holding the even taps' bits at their places, the high halves the odd ones' one place
+               lower */
+            typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+            constexpr int NT = 2 * R + 1, NP = (NT + 1) / 2;
+            u16x2 t[NP];
 #pragma unroll
-        for (int a = 0; a <= 2 * R; a++) nmin = min(nmin, (int)DY[((tx + a) * P2_TY + ty_) * P2_TZ + tz_] + (a - R) * (a - R));
+            for (int k = 0; k < NP; k++) {
+                const int a0 = 2 * k, a1 = 2 * k + 1;
+                const uint32_t lo = DY[((tx + a0) * P2_TY + ty_) * P2_TZ + tz_];
+                const uint32_t hi = (a1 < NT) ? (uint32_t)DY[((tx + a1) * P2_TY + ty_) * P2_TZ + tz_] : 0xFFFFu;
+                const uint16_t c0 = (uint16_t)((a0 - R) * (a0 - R)), c1 = (a1 < NT) ? (uint16_t)((a1 - R) * (a1 - R)) : (uint16_t)0;
+                t[k] = __builtin_bit_cast(u16x2, lo | (hi << 16)) + (u16x2){c0, c1};
+            }
+            u16x2 m = t[0];
 #pragma unroll
-        for (int a = 0; a <= 2 * R; a++)
-            amask |= (uint32_t)((int)DY[((tx + a) * P2_TY + ty_) * P2_TZ + tz_] + (a - R) * (a - R) == nmin) << a;
+            for (int k = 1; k < NP; k++) m = __builtin_elementwise_min(m, t[k]);
+            nmin = (int)min(m.x, m.y);
+            const u16x2 n1 = (u16x2){(uint16_t)(nmin + 1), (uint16_t)(nmin + 1)};
+#pragma unroll
+            for (int k = 0; k < NP; k++) t[k] = __builtin_elementwise_sub_sat(n1, t[k]);
+            /* 4, 16, 256 per lane, opaque so that the combination stays packed multiply-adds */
+            u16x2 pw[3] = {{4, 4}, {16, 16}, {256, 256}};
+#pragma unroll
+            for (int i = 0; i < 3; i++) asm volatile("" : "+v"(pw[i]));
+#pragma unroll
+            for (int st = 1, l = 0; st < NP; st *= 2, l++)
+#pragma unroll
+                for (int k = 0; k + st < NP; k += 2 * st) t[k] = t[k + st] * pw[l] + t[k];
+            amask = (uint32_t)t[0].x | ((uint32_t)t[0].y << 1);
+        } else {
+#pragma unroll
+            for (int a = 0; a <= 2 * R; a++) nmin = min(nmin, (int)DY[((tx + a) * P2_TY + ty_) * P2_TZ + tz_] + (a - R) * (a - R));
+#pragma unroll
+            for (int a = 0; a <= 2 * R; a++)
+                amask |= (uint32_t)((int)DY[((tx + a) * P2_TY + ty_) * P2_TZ + tz_] + (a - R) * (a - R) == nmin) << a;
+        }
         const float vx = grid_pos(G.ox, gx, dx), vy = grid_pos(G.oy, gy_, dx), vz = grid_pos(G.oz, gz_, dx);
         /* the smallest squared distance, rooted once at the end: sqrtf rounds correctly and is
            monotone, so the root of the minimum is the minimum of the roots */
