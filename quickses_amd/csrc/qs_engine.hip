@@ -480,6 +480,7 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
     /* every row word's shell rank: pass 2 stores the shell voxels' values densely in that order
        (the shell voxels are at most the voxels, which the field is sized for) */
     qs_launch_shell_scan(as<uint64_t>(shellm), nw_total, as<uint32_t>(scan_tmp), as<uint32_t>(stot), as<uint32_t>(spre), stream);
+    tm.mark("shell_scan");
     for (auto &grp : groups) {
         QS_TRY(group_tables(grp));
         qs_launch_pass2(batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path, as<uint64_t>(outm),
