@@ -33,7 +33,16 @@ CASES = [
     ("0.2 A (config 2), 300 atoms", 21, 300, 0.2),
     ("0.3 A (config 5), 600 atoms", 22, 600, 0.3),
     ("1.0 A (config 5), 2,000 atoms", 23, 2000, 1.0),
+    # config 3's geometry at its voxel size: a shell of the same density (12-20 A instead of
+    # 120-187 A), with an inner surface and the packing's cavities
+    ("config 3 shape: 1,300-atom shell (12-20 A), 0.5 A", 24, 1300, 0.5),
 ]
+
+
+def _molecule(qs, label, seed, n):
+    if "shell" in label:
+        return qs.synth_shell(seed, n, 12.0, 20.0)
+    return qs.synth_ball(seed, n)
 
 
 def _area_volume(V, T):
@@ -103,7 +112,7 @@ def _dist_to_mesh(P, V, T, k=12):
 def test_fp32_path_against_fp64(qs, orc, label, seed, n, reso):
     from scipy.ndimage import maximum_filter
 
-    pos, rad = qs.synth_ball(seed, n)
+    pos, rad = _molecule(qs, label, seed, n)
     G, perm, cs, ce, sx, lab, fld = orc.field(pos, rad, reso)
     lab64, f64 = orc.field_fp64(G, pos, rad)
     g, r = G.g, G.r
