@@ -2684,12 +2684,19 @@ __global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ 
     const int z0 = iclamp(cz - 1, 0, gn - 1), z1 = iclamp(cz + 1, 0, gn - 1);
     int best = -1;
     float min_d = 999999.0f, min_s = INFINITY;
-    for (int a = -1; a <= 1; a++) {
-        const int ix = iclamp(cx + a, 0, gn - 1);
-        for (int b = -1; b <= 1; b++) {
-            const int iy = iclamp(cy + b, 0, gn - 1);
-            const int h = (gn * gn * ix) + (gn * iy);
-            const int lo = (int)cstart[h + z0], hi = (int)cstart[h + z1 + 1];
+    /* the nine columns' ranges are loaded before any is walked */
+    int lo9[9], hi9[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int ix = iclamp(cx + k / 3 - 1, 0, gn - 1), iy = iclamp(cy + k % 3 - 1, 0, gn - 1);
+        const int h = (gn * gn * ix) + (gn * iy);
+        lo9[k] = (int)cstart[h + z0];
+        hi9[k] = (int)cstart[h + z1 + 1];
+    }
+    {
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const int lo = lo9[k], hi = hi9[k];
             for (int id = lo; id < hi; id++) {
                 const float4 at = sx[id];
                 const float sq = dist3_sq(at.x, at.y, at.z, px, py, pz);
