@@ -346,7 +346,8 @@ def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=
         plain_capacity_meshes_per_s=round(capacity, 2),
         plain=dict(meshes_per_s=round(k / wall_plain, 2), p50_ms=_pct(lat_plain, 50), p99_ms=_pct(lat_plain, 99)),
         served=dict(meshes_per_s=round(k / wall_served, 2), p50_ms=_pct(lat, 50), p99_ms=_pct(lat, 99),
-                    queue_p50_ms=_pct(queue, 50), mean_batch=round(float(np.mean(bsize)), 2),
+                    queue_p50_ms=_pct(queue, 50), queue_p99_ms=_pct(queue, 99),
+                    mean_batch=round(float(np.mean(bsize)), 2),
                     batches=st["batches"] - 1, workers=workers, max_busy_workers=st["max_busy_workers"],
                     hbm_budget_GiB=budget >> 30, max_batch_voxels=max_batch_voxels),
         saturated=dict(plain_meshes_per_s=round(capacity, 2), served_meshes_per_s=round(k / wall_sat, 2),
