@@ -249,11 +249,13 @@ def _pct(xs, q):
     return round(float(np.percentile(np.asarray(xs), q)), 3)
 
 
-def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=32 << 30, max_batch_voxels=0):
+def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=32 << 30, max_batch_voxels=0,
+                 express_voxels=0):
     """Poisson arrivals at `load` x the plain path's capacity.  Plain path: each request alone,
     atoms copied to HBM and meshed by qs_compute_device (mesh left in HBM), first come first
     served.  Served path: the request server (continuous batching over `workers` engines, HBM
-    budget admission, meshes resident in a device arena).  Latency = arrival -> mesh resident on
+    budget admission, meshes resident in a device arena; with express_voxels > 0 the last worker
+    takes only requests of at most that many grid points).  Latency = arrival -> mesh resident on
     the device."""
     import threading
 
@@ -303,7 +305,8 @@ def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=
 
     # served path
     lat, queue, bsize = [], [], []
-    with qs.Server(hbm_budget_bytes=budget, result_bytes=arena, workers=workers, max_batch_voxels=max_batch_voxels) as srv:
+    with qs.Server(hbm_budget_bytes=budget, result_bytes=arena, workers=workers, max_batch_voxels=max_batch_voxels,
+                   express_voxels=express_voxels) as srv:
         w = srv.submit(reqs[0][0][0][:2000], reqs[0][0][1][:2000], 1.0, smooth)  # warm-up of every engine
         srv.poll(w, -1)
         srv.release(w)
@@ -349,7 +352,8 @@ def bench_served(qs, hipmem, k=400, load=0.8, workers=2, budget=48 << 30, arena=
                     queue_p50_ms=_pct(queue, 50), queue_p99_ms=_pct(queue, 99),
                     mean_batch=round(float(np.mean(bsize)), 2),
                     batches=st["batches"] - 1, workers=workers, max_busy_workers=st["max_busy_workers"],
-                    hbm_budget_GiB=budget >> 30, max_batch_voxels=max_batch_voxels),
+                    hbm_budget_GiB=budget >> 30, max_batch_voxels=max_batch_voxels, express_voxels=express_voxels,
+                    express_batches=st["express_batches"]),
         saturated=dict(plain_meshes_per_s=round(capacity, 2), served_meshes_per_s=round(k / wall_sat, 2),
                        speedup=round(wall_sat and (k / wall_sat) / capacity, 3), served_p50_ms=_pct(lat_sat, 50),
                        served_p99_ms=_pct(lat_sat, 99)),
@@ -372,6 +376,8 @@ def main():
     ap.add_argument("--serve-workers", type=int, default=None, help="server engines (default: tuned, else 2)")
     ap.add_argument("--serve-cap", type=int, default=None,
                     help="grid points per served batch, 0 = no cap (default: tuned, else 150 M)")
+    ap.add_argument("--serve-express", type=int, default=None,
+                    help="request size cap of the server's express worker, 0 = none (default: tuned, else 0)")
     ap.add_argument("--no-other", action="store_true", help="skip configs 1 and 3")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -389,6 +395,8 @@ def main():
         args.serve_workers = int(tuned.get("workers", 2))
     if args.serve_cap is None:
         args.serve_cap = int(tuned.get("max_batch_voxels", 150_000_000))
+    if args.serve_express is None:
+        args.serve_express = int(tuned.get("express_voxels", 0))
 
     if hipmem.device_count() < 1:
         print("no HIP device", file=sys.stderr)
@@ -484,7 +492,7 @@ def main():
     served = None
     if not args.no_serve:
         served = bench_served(qs, hipmem, k=args.requests, workers=args.serve_workers,
-                              max_batch_voxels=args.serve_cap)
+                              max_batch_voxels=args.serve_cap, express_voxels=args.serve_express)
         print("served: %s" % json.dumps(served), file=sys.stderr, flush=True)
 
     cpu = None

@@ -220,10 +220,14 @@ int qs_stage_mesh(const float *fparams, const int *iparams, const float *field, 
  *   qs_serve_fetch   waits, then copies the mesh to host arrays (any may be NULL)
  *   qs_serve_release frees the request's result
  *   qs_serve_stats   {batches, requests batched, largest batch, most workers busy at once,
- *                    arena bytes in use, arena bytes, requests pending}; returns 7
+ *                    arena bytes in use, arena bytes, requests pending, express batches};
+ *                    returns 8
  *   qs_serve_set     "max_batch_voxels": cap on the grid points of one batch (0 = none, the
  *                    default; a request larger than the cap runs alone); "max_batch": cap on
- *                    the requests of one batch (1..256, default 256)
+ *                    the requests of one batch (1..256, default 256); "express_voxels": with 2
+ *                    workers or more, the last one takes only requests of at most that many
+ *                    grid points, oldest first, so that small requests need not wait behind
+ *                    large batches (0 = off, the default)
  *   qs_serve_close   finishes pending work, stops the workers, frees everything */
 int qs_serve_open(uint64_t hbm_budget_bytes, uint64_t result_bytes, int workers);
 int64_t qs_serve_submit(const float *pos, const float *rad, unsigned int N, float reso, int doSmoothing);

@@ -422,7 +422,7 @@ class Server:
     """Request server (qs_serve_*): submit surface requests at any time, poll for device-resident
     meshes.  Use as a context manager, or call close()."""
 
-    def __init__(self, hbm_budget_bytes=16 << 30, result_bytes=8 << 30, workers=2, max_batch_voxels=0):
+    def __init__(self, hbm_budget_bytes=16 << 30, result_bytes=8 << 30, workers=2, max_batch_voxels=0, express_voxels=0):
         L = lib()
         rc = L.qs_serve_open(int(hbm_budget_bytes), int(result_bytes), int(workers))
         if rc:
@@ -430,6 +430,8 @@ class Server:
         self.open = True
         if max_batch_voxels:
             self.set("max_batch_voxels", max_batch_voxels)
+        if express_voxels:
+            self.set("express_voxels", express_voxels)
 
     def set(self, key, value):
         rc = lib().qs_serve_set(key.encode(), int(value))
@@ -481,10 +483,10 @@ class Server:
             raise QuickSESError("quickses error %d: %s" % (-rc, lib().qs_last_error_message().decode()))
 
     def stats(self):
-        out = (ctypes.c_uint64 * 7)()
-        lib().qs_serve_stats(out, 7)
+        out = (ctypes.c_uint64 * 8)()
+        lib().qs_serve_stats(out, 8)
         keys = ("batches", "requests_batched", "largest_batch", "max_busy_workers", "arena_used", "arena_bytes",
-                "pending")
+                "pending", "express_batches")
         return {k: int(out[i]) for i, k in enumerate(keys)}
 
     def close(self):

@@ -6,8 +6,8 @@ library (which the library reads once per process for the values it uses itself)
   pass-1 stage time of the workload that uses that voxel size (one launch sequence over it);
 - batch_voxels, per voxel size: grid points per batched launch sequence, by the meshes/s of a
   stream of molecules through qs_compute_batch_device;
-- serve: worker engines and the grid-point cap of one served batch, by the p99 latency of a
-  served stream at 0.8 of the plain path's capacity.
+- serve: worker engines, the grid-point cap of one served batch and the express worker's request
+  size cap (0: none), by the p99 latency of a served stream at 0.8 of the plain path's capacity.
 
 usage: python -m quickses_amd.autotune [--out PATH] [--quick]
 """
@@ -109,7 +109,8 @@ def tune_batch_voxels(qs, hipmem, quick=False):
 
 
 def tune_serve(qs, hipmem, quick=False):
-    """Worker engines and per-batch grid-point cap of the server by p99 latency at 0.8 load."""
+    """Worker engines, per-batch grid-point cap and express cap of the server by p99 latency at
+    0.8 load."""
     import threading
 
     from .workloads import config5_requests
@@ -134,15 +135,16 @@ def tune_serve(qs, hipmem, quick=False):
         hipmem.free(d_rad)
     lam = 0.8 * cap_rate
     res = {}
-    for workers in (1, 2, 3, 4):
-        for cap in (0, 75_000_000, 150_000_000):
+    settings = [(w, cap, 0) for w in (1, 2, 3, 4) for cap in (0, 75_000_000, 150_000_000)]
+    settings += [(w, cap, 16_000_000) for w in (2, 3, 4) for cap in (0, 75_000_000, 150_000_000)]
+    for workers, cap, express in settings:
             p99s, p50s = [], []
             # two arrival streams per setting: a p99 over a short stream is one or two samples
             for rep in range(2):
                 arrivals = np.cumsum(np.random.default_rng(5 + rep).exponential(1.0 / lam, size=k))
                 lat = []
                 with qs.Server(hbm_budget_bytes=48 << 30, result_bytes=32 << 30, workers=workers,
-                               max_batch_voxels=cap) as srv:
+                               max_batch_voxels=cap, express_voxels=express) as srv:
                     w = srv.submit(reqs[0][0][0][:2000], reqs[0][0][1][:2000], 1.0, 1)
                     srv.poll(w, -1)
                     srv.release(w)
@@ -165,13 +167,15 @@ def tune_serve(qs, hipmem, quick=False):
                     th.join()
                 p99s.append(float(np.percentile(lat, 99)))
                 p50s.append(float(np.percentile(lat, 50)))
-            res[(workers, cap)] = (float(np.mean(p99s)), float(np.mean(p50s)))
-            print("serve workers=%d cap=%d: p50 %.3f ms p99 %.3f ms (mean of 2 streams)"
-                  % (workers, cap, res[(workers, cap)][1], res[(workers, cap)][0]), flush=True)
+            key = (workers, cap, express)
+            res[key] = (float(np.mean(p99s)), float(np.mean(p50s)))
+            print("serve workers=%d cap=%d express=%d: p50 %.3f ms p99 %.3f ms (mean of 2 streams)"
+                  % (workers, cap, express, res[key][1], res[key][0]), flush=True)
     best = min(res, key=lambda key: res[key][0])
-    return ({"workers": best[0], "max_batch_voxels": best[1]},
+    return ({"workers": best[0], "max_batch_voxels": best[1], "express_voxels": best[2]},
             {"arrival_rate_per_s": round(lam, 1), "requests": k, "streams": 2,
-             "p50_p99_ms": {"%d/%d" % key: [round(v[1], 3), round(v[0], 3)] for key, v in res.items()}})
+             "p50_p99_ms (workers/cap/express)": {"%d/%d/%d" % key: [round(v[1], 3), round(v[0], 3)]
+                                                   for key, v in res.items()}})
 
 
 def main():

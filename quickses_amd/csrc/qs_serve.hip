@@ -7,6 +7,9 @@
  * copy each mesh into a device-resident result arena where it stays until the client
  * releases it.  While one worker's batch runs, requests that arrive are taken by an idle
  * worker, so batches overlap on the device (continuous batching across the workers' slots).
+ * With an express threshold set, the last worker takes only requests of at most that many grid
+ * points (oldest first), so that a small request arriving while the other workers run large
+ * batches starts at once on its own stream; its kernels then share the device with theirs.
  *
  * Every result equals the mesh of the same request computed alone (the batched kernels give
  * each molecule exactly its single-molecule mesh).  The reference has no counterpart: its
@@ -111,10 +114,11 @@ struct Server {
     uint64_t budget_per_worker = 0;
     int max_batch = 256;
     uint64_t max_batch_voxels = 0; /* 0: no cap */
+    uint64_t express_voxels = 0;   /* > 0 (and 2 workers or more): the last worker's request size cap */
     int device = 0;
     bool closing = false;
     int64_t next_id = 0;
-    uint64_t batches = 0, batched_requests = 0, largest_batch = 0, busy_workers = 0, max_busy = 0;
+    uint64_t batches = 0, batched_requests = 0, largest_batch = 0, busy_workers = 0, max_busy = 0, express_batches = 0;
     int ready_workers = 0;
 };
 
@@ -228,31 +232,50 @@ void worker_main(Server *S, int w)
         S->ready_workers++;
     }
     S->cv_done.notify_all();
+    const int nworkers = (int)S->engines.size();
     for (;;) {
         std::vector<Request *> batch;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_work.wait(lk, [&] { return S->closing || !S->pending.empty(); });
-            if (S->pending.empty()) return; /* closing */
+            /* the express worker waits for a pending request within its size cap */
+            auto express = [&] { return S->express_voxels > 0 && nworkers >= 2 && w == nworkers - 1; };
+            auto small_pending = [&] {
+                for (Request *r : S->pending)
+                    if (r->voxels <= S->express_voxels) return true;
+                return false;
+            };
+            S->cv_work.wait(lk, [&] { return S->closing || (express() ? small_pending() : !S->pending.empty()); });
+            const bool ex = express();
+            /* closing, and nothing left that this worker takes (the express worker leaves larger
+               requests to the others, which finish every pending request before they stop) */
+            if (ex ? !small_pending() : S->pending.empty()) return;
             /* arrival order; the batch ends at the first request that does not fit the budget
-               (a request larger than the budget runs alone) or asks for another pass count */
+               (a request larger than the budget runs alone) or asks for another pass count.  The
+               express worker takes the requests within its cap, skipping the others. */
             uint64_t acc = 0, accv = 0;
-            const int smooth = S->pending.front()->smooth;
+            int smooth = -1;
             const auto now = Clock::now();
-            while (!S->pending.empty() && (int)batch.size() < S->max_batch) {
-                Request *r = S->pending.front();
+            for (auto it = S->pending.begin(); it != S->pending.end() && (int)batch.size() < S->max_batch;) {
+                Request *r = *it;
+                if (ex && r->voxels > S->express_voxels) {
+                    ++it;
+                    continue;
+                }
+                if (smooth < 0) smooth = r->smooth;
                 if (r->smooth != smooth) break;
                 if (!batch.empty() && acc + r->est_bytes > S->budget_per_worker) break;
                 /* work cap: a batch stays short, so requests arriving meanwhile wait less */
                 if (!batch.empty() && S->max_batch_voxels && accv + r->voxels > S->max_batch_voxels) break;
-                S->pending.pop_front();
+                it = S->pending.erase(it);
                 r->state = RUNNING;
                 r->t_start = now;
                 acc += r->est_bytes;
                 accv += r->voxels;
                 batch.push_back(r);
             }
+            if (batch.empty()) continue; /* (another worker took them) */
             S->batches++;
+            if (ex) S->express_batches++;
             S->batched_requests += batch.size();
             S->largest_batch = std::max<uint64_t>(S->largest_batch, batch.size());
             S->busy_workers++;
@@ -346,7 +369,8 @@ int64_t qs_serve_submit(const float *pos, const float *rad, unsigned int N, floa
         S->reqs[r->id] = r;
         S->pending.push_back(r);
     }
-    S->cv_work.notify_one();
+    /* every worker re-checks: the express worker takes only small requests */
+    S->cv_work.notify_all();
     return r->id;
 }
 
@@ -416,6 +440,7 @@ int qs_serve_set(const char *key, uint64_t value)
     std::lock_guard<std::mutex> lk(S->mu);
     const std::string k = key ? key : "";
     if (k == "max_batch_voxels") S->max_batch_voxels = value;
+    else if (k == "express_voxels") S->express_voxels = value;
     else if (k == "max_batch") S->max_batch = (int)std::max<uint64_t>(1, std::min<uint64_t>(value, 256));
     else return -set_error(3, "unknown server setting %s", k.c_str());
     return 0;
@@ -426,10 +451,10 @@ int qs_serve_stats(uint64_t *out, int n)
     if (check_server()) return -g_err;
     Server *S = g_srv;
     std::lock_guard<std::mutex> lk(S->mu);
-    const uint64_t v[7] = {S->batches, S->batched_requests, S->largest_batch, S->max_busy, S->arena.used, S->arena.size,
-                           (uint64_t)S->pending.size()};
-    for (int i = 0; i < n && i < 7; i++) out[i] = v[i];
-    return 7;
+    const uint64_t v[8] = {S->batches, S->batched_requests, S->largest_batch, S->max_busy, S->arena.used, S->arena.size,
+                           (uint64_t)S->pending.size(), S->express_batches};
+    for (int i = 0; i < n && i < 8; i++) out[i] = v[i];
+    return 8;
 }
 
 int qs_serve_close(void)

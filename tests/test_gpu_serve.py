@@ -30,12 +30,16 @@ def _requests(qs, k, seed=900):
     return out
 
 
-@pytest.mark.parametrize("workers,budget,cap", [(2, 8 << 30, 0), (1, 256 << 20, 0), (3, 96 << 20, 0),
-                                                (2, 8 << 30, 3_000_000)])
-def test_served_results_equal_alone(qs, workers, budget, cap):
+# (workers, budget, batch cap, express worker's request cap): the express worker (the last one)
+# takes only requests of at most that many grid points
+@pytest.mark.parametrize("workers,budget,cap,express", [(2, 8 << 30, 0, 0), (1, 256 << 20, 0, 0), (3, 96 << 20, 0, 0),
+                                                        (2, 8 << 30, 3_000_000, 0), (3, 8 << 30, 0, 2_000_000),
+                                                        (2, 8 << 30, 3_000_000, 500_000)])
+def test_served_results_equal_alone(qs, workers, budget, cap, express):
     reqs = _requests(qs, 14)
-    print("case serve workers=%d budget=%d cap=%d" % (workers, budget, cap), flush=True)
-    with qs.Server(hbm_budget_bytes=budget, result_bytes=1 << 30, workers=workers, max_batch_voxels=cap) as srv:
+    print("case serve workers=%d budget=%d cap=%d express=%d" % (workers, budget, cap, express), flush=True)
+    with qs.Server(hbm_budget_bytes=budget, result_bytes=1 << 30, workers=workers, max_batch_voxels=cap,
+                   express_voxels=express) as srv:
         ids = []
         for i, ((pos, rad), reso, smooth) in enumerate(reqs):
             ids.append(srv.submit(pos, rad, reso, smooth))
@@ -53,6 +57,27 @@ def test_served_results_equal_alone(qs, workers, budget, cap):
     for ((pos, rad), reso, smooth), got, inf in zip(reqs, served, info):
         assert inf["latency_ms"] >= inf["queue_ms"] >= 0.0 and inf["batch_size"] >= 1
         _same_mesh(got, qs.compute_ses(pos, rad, reso, smooth))
+
+
+def test_served_express_lane(qs):
+    """Small requests submitted while the other worker meshes a large one are taken by the express
+    worker (the large one never is), and every result equals the request meshed alone."""
+    big = qs.synth_ball(950, 200_000)  # 0.3 A: 688^3 grid points, a few ms
+    small = [qs.synth_ball(951 + i, 300) for i in range(4)]
+    with qs.Server(hbm_budget_bytes=16 << 30, result_bytes=2 << 30, workers=2, express_voxels=2_000_000) as srv:
+        print("case serve express lane", flush=True)
+        ids = [srv.submit(big[0], big[1], 0.3, 1)]
+        time.sleep(0.001)  # the normal worker has taken the large request and runs it
+        ids += [srv.submit(p, r, 1.0, 1) for p, r in small]
+        got = [srv.fetch(r) for r in ids]
+        info = [srv.poll(r, 0) for r in ids]
+        st = srv.stats()
+        for r in ids:
+            srv.release(r)
+    assert st["express_batches"] >= 1
+    assert info[0]["batch_size"] == 1
+    for (pos, rad), reso, res in zip([big] + small, [0.3] + [1.0] * 4, got):
+        _same_mesh(res, qs.compute_ses(pos, rad, reso, 1))
 
 
 def test_served_device_pointers(qs):
