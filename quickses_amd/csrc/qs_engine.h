@@ -206,7 +206,8 @@ struct Engine {
     DevBuf in_pos, in_rad, d_off, bb_part, bb_out, d_mols, d_meta;
     DevBuf key, count /* and the scatter cursors after the counts */, start, perm, cse, sx, aux;
     DevBuf labels, outm, shellm, field, negm, zerom;
-    DevBuf s4, vcnt, tcnt, vbw, tbw, alist, totals, bounds, bidx;
+    DevBuf s4, vcnt, tcnt, vbw, tbw, alist, totals, bounds;
+    DevBuf rback; /* the run's read-back: totals, vertex and kept-triangle offsets */
     DevBuf ntb, avb, atb; /* triangle-count bit planes per word; first vertex / triangle of each active word */
     DevBuf vstart, tstart; /* first active word of each 256-vertex / 256-triangle block */
     DevBuf normals;        /* staging for qs_vertex_normals / qs_vertex_colors */
@@ -271,6 +272,25 @@ struct Engine {
         up_used = up_need = 0;
         return 0;
     }
+    /* A run's atom inputs from host memory, queued before run_batch through the same staging
+       (run_batch then keeps them): upload_begin, then upload_input per array.  Arrays above
+       STAGE_INPUT_MAX are copied from the caller's memory directly. */
+    static constexpr size_t STAGE_INPUT_MAX = (size_t)8 << 20;
+    bool inputs_staged = false;
+    int upload_begin()
+    {
+        QS_TRY(upload_reset());
+        inputs_staged = true;
+        return 0;
+    }
+    int upload_input(void *dst, const void *src, size_t bytes)
+    {
+        if (bytes > STAGE_INPUT_MAX) {
+            QS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+            return 0;
+        }
+        return upload(dst, src, bytes);
+    }
 
     /* last result */
     std::vector<QsMol> mols;
@@ -295,6 +315,7 @@ struct Engine {
        (grown when a run needed more) */
     uint32_t pool_v = 0, pool_t = 0;
     uint64_t ntflags = 0; /* tile flags of the run's fixed-R pass-2 molecules */
+    uint8_t tmark = 0;    /* the field passes' tile mark, 1-255, changed every run */
     uint64_t cap_v = 0, cap_t = 0;
     int reruns = 0;
     float *res_vert = nullptr;
@@ -306,7 +327,7 @@ struct Engine {
     {
         for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start,
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
-                          &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey,
+                          &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &rback, &vert, &vert2, &vert3, &vkey,
                           &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
                           &tpre_d, &tflag, &spre, &stot, &field_dense})
             *b = DevBuf{};
@@ -321,7 +342,7 @@ struct Engine {
     {
         for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start,
                           &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
-                          &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &bidx, &vert, &vert2, &vert3, &vkey,
+                          &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &rback, &vert, &vert2, &vert3, &vkey,
                           &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
                           &tpre_d, &tflag, &spre, &stot, &field_dense})
             release(*b);
@@ -368,6 +389,11 @@ struct Engine {
     {
         if (n == 0) {
             if (d_total) QS_HIP(hipMemsetAsync(d_total, 0, sizeof(uint32_t), stream));
+            return 0;
+        }
+        if (qs_scan_tiles(n) > 1 && n <= QS_SCAN1_MAX) {
+            qs_launch_scan1(in, out, n, d_total, stream);
+            QS_HIP(hipGetLastError());
             return 0;
         }
         std::vector<uint64_t> lv;

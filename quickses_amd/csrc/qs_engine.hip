@@ -32,6 +32,11 @@
 
 namespace qs {
 
+/* field pass 1: a slab launch of fewer blocks than this (four per CU) splits its slabs, down to
+   slabs of this many rows */
+constexpr uint64_t P1_FILL_BLOCKS = 1024;
+constexpr int P1_FILL_MINROWS = 8;
+
 thread_local int g_err = 0;
 thread_local char g_errmsg[512] = "";
 thread_local std::vector<std::pair<std::string, float>> g_stage_ms;
@@ -249,8 +254,8 @@ int Engine::reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts
     QS_TRY(ensure(bb_out, m * 8 * sizeof(float)));
     QS_TRY(ensure(bb_part, qs_bbox_part_floats((int)m, qs_bbox_parts((uint32_t)std::min<uint64_t>(atoms, UINT32_MAX))) *
                                sizeof(float)));
-    QS_TRY(ensure(bidx, m * 8));
     QS_TRY(ensure(bounds, m * 8));
+    QS_TRY(ensure(rback, (4 + 2 * m) * 4));
     QS_TRY(ensure(d_mols, m * sizeof(QsMol)));
     return 0;
 }
@@ -349,6 +354,23 @@ int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &a
     };
     at_atoms = prefix([](const QsMol &M) { return (uint64_t)((M.n_atoms + 255) / 256); }, nb_atoms);
     const int p1mode = qs_pass1_mode(g_opt.pass1_direct != 0, g_opt.pass1_rows != 0);
+    /* A run whose slab launch would leave the device under-filled (a small molecule whose planes
+       are one slab each) takes slabs of fewer rows, down to P1_FILL_MINROWS: the masks do not
+       depend on the slab height, only the number of blocks does. */
+    if (p1mode == 0) {
+        for (;;) {
+            uint64_t total = 0;
+            int rows = 0;
+            for (const QsMol &M : mols)
+                if (M.G.g) {
+                    total += qs_pass1_blocks(M, 0);
+                    rows = std::max(rows, M.slab_rows);
+                }
+            if (total >= P1_FILL_BLOCKS || rows <= P1_FILL_MINROWS) break;
+            for (QsMol &M : mols)
+                if (M.G.g) M.slab_rows = std::max(P1_FILL_MINROWS, std::min(M.slab_rows, (rows + 1) / 2));
+        }
+    }
     at_p1 = prefix([&](const QsMol &M) { return (uint64_t)(M.G.g ? qs_pass1_blocks(M, p1mode) : 0); }, nb_p1);
     p1_lds = 0;
     for (const QsMol &M : mols)
@@ -462,9 +484,10 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
     uint8_t *tf = nullptr;
     if (p1mode == 0 && ntflags) {
         QS_TRY(ensure(tflag, ntflags));
-        QS_HIP(hipMemsetAsync(tflag.p, 0, ntflags, stream));
         tf = as<uint8_t>(tflag);
     }
+    /* this run's tile mark (the flags are not cleared: see qs_launch_pass1) */
+    tmark = (uint8_t)(tmark == 255 ? 1 : tmark + 1);
     if (nb_p1) {
         if (p1mode == 2)
             qs_launch_pass1_direct(batch(at_p1, 0, nmol), nb_p1, as<int2>(cse), as<float4>(sx), as<uint64_t>(outm),
@@ -473,7 +496,7 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
             qs_launch_pass1(batch(at_p1, 0, nmol), nb_p1, p1mode, p1_lds, as<uint32_t>(start), as<float4>(sx),
                             as<float4>(aux), as<uint64_t>(outm), as<uint64_t>(shellm),
                             p1mode == 0 ? as<uint64_t>(negm) : nullptr, p1mode == 0 ? as<uint64_t>(zerom) : nullptr, tf,
-                            stream);
+                            tmark, stream);
         QS_HIP(hipGetLastError());
     }
     tm.mark("pass1");
@@ -485,7 +508,7 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
         QS_TRY(group_tables(grp));
         qs_launch_pass2(batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path, as<uint64_t>(outm),
                         as<uint64_t>(shellm), as<uint32_t>(spre), as<float>(field), as<uint32_t>(negm), as<uint32_t>(zerom), as<int>(grp.tab->offs),
-                        as<int>(grp.tab->norm), as<int>(grp.tab->end), grp.tab->flags, tf, grp.xchunk, stream);
+                        as<int>(grp.tab->norm), as<int>(grp.tab->end), grp.tab->flags, tf, tmark, grp.xchunk, stream);
         QS_HIP(hipGetLastError());
     }
     field_sparse = true;
@@ -542,27 +565,20 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
        so nothing waits for the host until the run's results are read back at its end, which is
        also where a pool that was too small shows (the run is then repeated with larger pools). */
     plan_pools();
-    /* per-molecule first vertex and triangle (the scans at each molecule's first word) and the
-       vertex coarse index, computed on the device */
-    std::vector<uint64_t> first(nmol);
-    for (int m = 0; m < nmol; m++) {
-        const uint64_t w = phys_word_off(mols[m]);
-        first[m] = (w < nw_total) ? w : ~0ull;
-    }
-    QS_TRY(ensure(bidx, ((size_t)nmol + 1) * 8));
+    /* per-molecule first vertex and triangle (the scans at each molecule's first word, from the
+       word prefix) and the vertex coarse index, computed on the device */
+    const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
+    const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 2 * ((size_t)nmol + 1);
     QS_TRY(ensure(bounds, ((size_t)nmol + 1) * 8));
     QS_TRY(ensure(tpre_d, ((size_t)nmol + 1) * 8));
-    QS_TRY(upload(bidx.p, first.data(), (size_t)nmol * 8));
     const uint32_t ncoarse = (pool_v >> QS_COARSE_ITEMS) + 2;
     QS_TRY(ensure(d_off, ((size_t)nmol + 1) * 8 + (size_t)ncoarse * 4 + 64));
     uint32_t *d_vpre = as<uint32_t>(d_off) + (nmol + 1);
-    qs_launch_mesh_bases(as<uint32_t>(vbw), as<uint32_t>(tbw), as<uint64_t>(bidx), nmol, as<uint32_t>(totals), d_vpre,
+    qs_launch_mesh_bases(as<uint32_t>(vbw), as<uint32_t>(tbw), wpre, nw_total, nmol, as<uint32_t>(totals), d_vpre,
                          as<uint64_t>(tpre_d), d_vpre + nmol + 1, ncoarse, stream);
     QS_TRY(ensure(vert, (size_t)pool_v * 12));
     QS_TRY(ensure(vkey, (size_t)pool_v * 8));
     QS_TRY(ensure(tri, (size_t)pool_t * 12));
-    const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
-    const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 2 * ((size_t)nmol + 1);
     QS_TRY(ensure(vstart, ((size_t)pool_v / 256 + 1) * 4));
     QS_TRY(ensure(tstart, ((size_t)pool_t / 256 + 1) * 4));
     /* degenerate triangles per 256-triangle block and in total (totals[3]), counted by the
@@ -571,14 +587,11 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     /* every scanned word may be active */
     const uint32_t nact_max = (uint32_t)std::max<uint64_t>(scan_nw, 1);
     const uint32_t *counts = as<uint32_t>(totals);
-    qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart),
-                          nact_max, pool_v, as<uint64_t>(s4), as<float>(field), field_sparse ? as<uint32_t>(spre) : nullptr,
-                          as<uint64_t>(outm), field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey),
-                          counts, stream);
-    qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart),
-                           nact_max, pool_t, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
-                           as<uint64_t>(ntb), as<int>(tri), 0u, as<uint32_t>(deg_blk),
-                           as<uint32_t>(totals) + 3, counts, stream);
+    qs_launch_mc_emit(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), nact_max, as<uint32_t>(avb), as<uint32_t>(vstart),
+                      pool_v, as<uint64_t>(s4), as<float>(field), field_sparse ? as<uint32_t>(spre) : nullptr, as<uint64_t>(outm),
+                      field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey), as<uint32_t>(atb),
+                      as<uint32_t>(tstart), pool_t, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint32_t>(vbw),
+                      as<uint64_t>(ntb), as<int>(tri), 0u, as<uint32_t>(deg_blk), as<uint32_t>(totals) + 3, counts, stream);
     QS_HIP(hipGetLastError());
     tm.mark("mc_emit");
     if (with_atoms) {
@@ -722,13 +735,11 @@ int Engine::slab_mesh(size_t at_mc, uint32_t nb_mc, size_t at_wpre, uint64_t own
     QS_TRY(ensure(deg_blk, ((size_t)n_tri / 256 + 1) * 4));
     const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
     const uint32_t *wcoarse = as<uint32_t>(d_meta) + at_wpre + 4;
-    qs_launch_mc_vertices(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(avb), as<uint32_t>(vstart), nact,
-                          n_owned, as<uint64_t>(s4), as<float>(field), as<uint32_t>(spre), as<uint64_t>(outm), as<uint64_t>(shellm),
-                          as<float>(vert) + 3 * (size_t)vbase, as<uint64_t>(vkey), nullptr, stream);
-    qs_launch_mc_triangles(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), as<uint32_t>(atb), as<uint32_t>(tstart), nact,
-                           n_tri, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4), as<uint32_t>(vbw),
-                           as<uint64_t>(ntb), as<int>(tri) + 3 * (size_t)tbase, vbase,
-                           as<uint32_t>(deg_blk), as<uint32_t>(totals) + 3, nullptr, stream);
+    qs_launch_mc_emit(as<QsMol>(d_mols), wpre, wcoarse, as<uint32_t>(alist), nact, as<uint32_t>(avb), as<uint32_t>(vstart),
+                      n_owned, as<uint64_t>(s4), as<float>(field), as<uint32_t>(spre), as<uint64_t>(outm), as<uint64_t>(shellm),
+                      as<float>(vert) + 3 * (size_t)vbase, as<uint64_t>(vkey), as<uint32_t>(atb), as<uint32_t>(tstart), n_tri,
+                      as<uint64_t>(negm), as<uint64_t>(zerom), as<uint32_t>(vbw), as<uint64_t>(ntb),
+                      as<int>(tri) + 3 * (size_t)tbase, vbase, as<uint32_t>(deg_blk), as<uint32_t>(totals) + 3, nullptr, stream);
     QS_HIP(hipGetLastError());
     return 0;
 }
@@ -854,7 +865,9 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
                       const std::vector<BBox> *host_bbox)
 {
     const int nmol = (int)in.size();
-    QS_TRY(upload_reset());
+    /* (inputs queued through the staging by upload_begin stay) */
+    if (!inputs_staged) QS_TRY(upload_reset());
+    inputs_staged = false;
     batch_valid = false;
     nv = nt = nt_all = n_active = 0;
     res_vert = nullptr;
@@ -888,8 +901,10 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
         for (int m = 0; m < nmol; m++) max_n = std::max(max_n, in[m].n);
         const int parts = qs_bbox_parts(max_n);
         QS_TRY(ensure(bb_part, qs_bbox_part_floats(nmol, parts) * sizeof(float)));
-        QS_TRY(upload(d_off.p, off.data(), (nmol + 1) * 4));
-        qs_launch_bbox(d_pos, d_rad, as<uint32_t>(d_off), nmol, parts, as<float>(bb_part), as<float>(bb_out), stream);
+        /* (one molecule's atom range goes in the launch's arguments) */
+        if (nmol > 1) QS_TRY(upload(d_off.p, off.data(), (nmol + 1) * 4));
+        qs_launch_bbox(d_pos, d_rad, nmol > 1 ? as<uint32_t>(d_off) : nullptr, off[0], off[1], nmol, parts, as<float>(bb_part),
+                       as<float>(bb_out), stream);
         std::vector<float> h((size_t)nmol * 8);
         float *hs = (float *)stage(h.size() * 4);
         QS_HIP(hipMemcpyAsync(hs ? (void *)hs : (void *)h.data(), bb_out.p, h.size() * 4, hipMemcpyDeviceToHost, stream));
@@ -1005,15 +1020,15 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     qs_launch_tri_compact(as<int>(tri), tcap, as<uint32_t>(keeppos), as<uint32_t>(totals) + 3, as<int>(tri_out),
                           multi_slab ? nullptr : as<uint32_t>(totals), stream);
     QS_TRY(ensure(bounds, ((size_t)nmol + 1) * 8));
-    qs_launch_tri_kept_before(as<int>(tri), as<uint32_t>(keeppos), as<uint64_t>(tpre_d), nmol + 1, tcap, as<uint32_t>(bounds),
-                              stream);
-    /* the run's one read-back: totals, per-molecule first vertices and kept-triangle offsets */
+    /* the run's one read-back, gathered by the last kernel: totals, per-molecule first vertices
+       and kept-triangle offsets */
     std::vector<uint32_t> hb(4 + 2 * ((size_t)nmol + 1));
+    QS_TRY(ensure(rback, hb.size() * 4));
+    qs_launch_tri_kept_before(as<int>(tri), as<uint32_t>(keeppos), as<uint64_t>(tpre_d), nmol + 1, tcap, as<uint32_t>(bounds),
+                              as<uint32_t>(totals), as<uint32_t>(d_off) + (nmol + 1), as<uint32_t>(rback), stream);
     uint32_t *fs = (uint32_t *)stage(hb.size() * 4);
     uint32_t *fd = fs ? fs : hb.data();
-    QS_HIP(hipMemcpyAsync(fd, totals.p, 16, hipMemcpyDeviceToHost, stream));
-    QS_HIP(hipMemcpyAsync(fd + 4, as<uint32_t>(d_off) + (nmol + 1), ((size_t)nmol + 1) * 4, hipMemcpyDeviceToHost, stream));
-    QS_HIP(hipMemcpyAsync(fd + 4 + nmol + 1, bounds.p, ((size_t)nmol + 1) * 4, hipMemcpyDeviceToHost, stream));
+    QS_HIP(hipMemcpyAsync(fd, rback.p, hb.size() * 4, hipMemcpyDeviceToHost, stream));
     QS_HIP(hipStreamSynchronize(stream));
     if (fs) memcpy(hb.data(), fs, hb.size() * 4);
     tm.mark("filter");
@@ -1064,9 +1079,9 @@ int *g_triangles = nullptr;
 int *g_atom_ids = nullptr;
 unsigned g_ntri_idx = 0, g_nvert = 0;
 
-/* Host result arrays.  Arrays of 1 MB or more are pinned (hipHostMalloc), so the device-to-host
-   copies run at the link's speed; smaller ones (and every array of an empty mesh, which must not
-   touch the device) are ordinary memory.  API_freeMesh keeps the result's arrays (the largest set
+/* Host result arrays.  Arrays of PIN_MIN bytes or more are pinned (hipHostMalloc), so the
+   device-to-host copies are queued at once and run at the link's speed; smaller ones (and every
+   array of an empty mesh, which must not touch the device) are ordinary memory.  API_freeMesh keeps the result's arrays (the largest set
    so far) for the next call: pinning, like first-touch page faults in fresh pageable memory, costs
    more than the copy itself. */
 struct HostArray {
@@ -1076,7 +1091,7 @@ struct HostArray {
 };
 HostArray g_cur[3], g_spare[3]; /* vertices, triangles, atom ids */
 
-constexpr size_t PIN_MIN = 1u << 20;
+constexpr size_t PIN_MIN = 4096;
 
 HostArray host_alloc(size_t bytes)
 {
@@ -1381,10 +1396,9 @@ void API_computeSES(float resoSES, qs_float3 *atomPos, float *atomRad, unsigned 
     int rc = E.init();
     if (!rc) rc = ensure(E.in_pos, (size_t)N * 12);
     if (!rc) rc = ensure(E.in_rad, (size_t)N * 4);
-    if (!rc && hipMemcpyAsync(E.in_pos.p, hp, (size_t)N * 12, hipMemcpyHostToDevice, E.stream) != hipSuccess)
-        rc = set_error(2, "H2D copy failed");
-    if (!rc && hipMemcpyAsync(E.in_rad.p, atomRad, (size_t)N * 4, hipMemcpyHostToDevice, E.stream) != hipSuccess)
-        rc = set_error(2, "H2D copy failed");
+    if (!rc) rc = E.upload_begin();
+    if (!rc) rc = E.upload_input(E.in_pos.p, hp, (size_t)N * 12);
+    if (!rc) rc = E.upload_input(E.in_rad.p, atomRad, (size_t)N * 4);
     std::vector<MolIn> in(1);
     in[0].atom_off = 0;
     in[0].n = N;
@@ -1547,10 +1561,9 @@ int qs_compute_batch(const float *pos, const float *rad, const unsigned int *ato
     QS_TRY(E.init());
     QS_TRY(ensure(E.in_pos, (size_t)std::max(n, 1u) * 12));
     QS_TRY(ensure(E.in_rad, (size_t)std::max(n, 1u) * 4));
-    if (n) {
-        QS_HIP(hipMemcpyAsync(E.in_pos.p, pos, (size_t)n * 12, hipMemcpyHostToDevice, E.stream));
-        QS_HIP(hipMemcpyAsync(E.in_rad.p, rad, (size_t)n * 4, hipMemcpyHostToDevice, E.stream));
-    }
+    QS_TRY(E.upload_begin());
+    QS_TRY(E.upload_input(E.in_pos.p, pos, (size_t)n * 12));
+    QS_TRY(E.upload_input(E.in_rad.p, rad, (size_t)n * 4));
     QS_TRY(E.run_batch(as<float>(E.in_pos), as<float>(E.in_rad), in, doSmoothing, &bb));
     for (int m = 0; m < nmol; m++) {
         if (nvert) nvert[m] = E.vert_off[m + 1] - E.vert_off[m];
@@ -1919,7 +1932,7 @@ int qs_stage_pass2(const float *fp, const int *ip, const uint8_t *labels, float 
         qs_launch_pass2(E.batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path,
                         as<uint64_t>(E.outm), as<uint64_t>(E.shellm), as<uint32_t>(E.spre), as<float>(E.field), as<uint32_t>(E.negm),
                         as<uint32_t>(E.zerom), as<int>(grp.tab->offs), as<int>(grp.tab->norm), as<int>(grp.tab->end),
-                        grp.tab->flags, nullptr, grp.xchunk, E.stream);
+                        grp.tab->flags, nullptr, 0, grp.xchunk, E.stream);
         QS_HIP(hipGetLastError());
     }
     E.field_sparse = true;

@@ -13,8 +13,9 @@
 /* bounding boxes: `parts` partial boxes per molecule (qs_bbox_parts of the largest atom count) */
 int qs_bbox_parts(uint32_t max_atoms);
 size_t qs_bbox_part_floats(int nmol, int parts);
-void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, int nmol, int parts, float *part, float *out,
-                    hipStream_t s);
+/* off: per-molecule atom offsets (nmol + 1); or nullptr for one molecule of atoms [o0, o1) */
+void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, uint32_t o0, uint32_t o1, int nmol, int parts,
+                    float *part, float *out, hipStream_t s);
 void qs_launch_hash_count(QsBatch B, uint32_t nblocks, const float *pos, int *key, uint32_t *count, hipStream_t s);
 void qs_launch_scatter(const int *key, int n, const uint32_t *start, uint32_t *cursor, int *perm, hipStream_t s);
 void qs_launch_cell_finalize(const uint32_t *start, const uint32_t *count, int ncell, int *perm, int2 *cse, hipStream_t s);
@@ -34,11 +35,13 @@ int qs_pass1_mode(bool direct, bool rows);
 uint32_t qs_pass1_blocks(const QsMol &M, int mode);
 size_t qs_pass1_lds_bytes(const QsMol &M, int mode);
 /* The slab kernel (mode 0) also writes the sign and zero words every voxel has when its pass-2
-   tile holds no shell voxel (negm, zerom), and flags the tiles that do (tflag, at each molecule's
-   tile_off); both may be NULL. */
+   tile holds no shell voxel (negm, zerom), and marks the tiles that do (tflag, at each molecule's
+   tile_off, set to tmark); both may be NULL.  The flags are not cleared between runs: a run uses a
+   mark of its own, and a stale flag that equals it only makes pass 2 rewrite the same sign words
+   for a tile without shell voxels. */
 void qs_launch_pass1(QsBatch B, uint32_t nblocks, int mode, size_t lds_bytes, const uint32_t *cstart, const float4 *sx,
                      const float4 *aux, uint64_t *outm, uint64_t *shellm, uint64_t *negm, uint64_t *zerom, uint8_t *tflag,
-                     hipStream_t s);
+                     uint8_t tmark, hipStream_t s);
 void qs_launch_pass1_direct(QsBatch B, uint32_t nblocks, const int2 *cse, const float4 *sx, uint64_t *outm, uint64_t *shellm,
                             hipStream_t s);
 
@@ -48,11 +51,11 @@ int qs_pass2_path(int r, bool direct);
 /* blocks of one molecule's launch over its planes [p2_lo, p2_hi) */
 uint32_t qs_pass2_blocks(const QsMol &M, int path);
 /* the shell voxels' values go to field at their shell ranks (spre, from qs_launch_shell_scan);
-   tflag (path 0 only, may be NULL): tiles left unflagged by field pass 1 are skipped; xchunk
+   tflag (path 0 only, may be NULL): tiles field pass 1 did not mark with tmark are skipped; xchunk
    (path 0): consecutive tiles each XCD takes in turn */
 void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm,
                      const uint32_t *spre, float *field, uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
-                     QsNormFlags nf, const uint8_t *tflag, uint32_t xchunk, hipStream_t s);
+                     QsNormFlags nf, const uint8_t *tflag, uint8_t tmark, uint32_t xchunk, hipStream_t s);
 /* pass-2 tiles of the fixed-R kernel of one molecule (its tile-flag count) */
 uint32_t qs_pass2_tiles(const QsMol &M);
 /* z tiles per (x, y) tile row of the fixed-R pass-2 kernel */
@@ -80,29 +83,28 @@ uint32_t qs_mc_count_blocks(const QsMol &M);
 void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
                         uint32_t *tcnt, uint64_t *NT, hipStream_t s);
 /* one thread per vertex / per triangle over the active words (list, with their first vertex ids
-   avb and first triangle ids atb); vstart / tstart receive each 256-item block's first active word
-   (nv / 256 + 1 and ntri / 256 + 1 entries).  The field F holds the shell voxels' values only,
-   at their shell ranks (spre), the others come from the outside / shell masks (shellm == nullptr:
-   F holds every voxel in grid order, spre is unused). */
-void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
-                           const uint32_t *avb, uint32_t *vstart, uint32_t nact, uint32_t nv, const uint64_t *S4,
-                           const float *F, const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *vert,
-                           uint64_t *vkey, const uint32_t *d_counts, hipStream_t s);
-void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
-                            const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
-                            const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
-                            int *tri, uint32_t vbase, uint32_t *deg_blk, uint32_t *deg_total,
-                            const uint32_t *d_counts, hipStream_t s);
+   avb and first triangle ids atb), after one launch that gives vstart / tstart each 256-item
+   block's first active word (nv / 256 + 1 and ntri / 256 + 1 entries) and clears deg_blk.  The
+   field F holds the shell voxels' values only, at their shell ranks (spre), the others come from
+   the outside / shell masks (shellm == nullptr: F holds every voxel in grid order, spre is
+   unused).  Triangle corners are vertex ids plus vbase; degenerate triangles are counted per
+   256-triangle block (deg_blk) and in total. */
+void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list, uint32_t nact,
+                       const uint32_t *avb, uint32_t *vstart, uint32_t nv, const uint64_t *S4, const float *F,
+                       const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *vert, uint64_t *vkey,
+                       const uint32_t *atb, uint32_t *tstart, uint32_t ntri, const uint64_t *N, const uint64_t *Zm,
+                       const uint32_t *vbw, const uint64_t *NT, int *tri, uint32_t vbase, uint32_t *deg_blk,
+                       uint32_t *deg_total, const uint32_t *d_counts, hipStream_t s);
 
 /* Launches whose item counts live on the device take them from the d_* pointers (when given),
    the host counts being the pools the launch covers: marching-cubes totals are counts[0]
    triangles, counts[1] vertices, counts[2] active words. */
 
 /* per-molecule first vertex (vpre, u32) and first triangle (tpre, u64) from the word scans at each
-   molecule's first word (first[m], ~0 for none), nmol + 1 entries each ending with the totals,
-   and the vertex coarse index (ncoarse entries) */
-void qs_launch_mesh_bases(const uint32_t *vbw, const uint32_t *tbw, const uint64_t *first, int nmol, const uint32_t *counts,
-                          uint32_t *vpre, uint64_t *tpre, uint32_t *vcoarse, uint32_t ncoarse, hipStream_t s);
+   molecule's first word (wpre[m]; at or past the nw scanned words for none), nmol + 1 entries
+   each ending with the totals, and the vertex coarse index (ncoarse entries), in one launch */
+void qs_launch_mesh_bases(const uint32_t *vbw, const uint32_t *tbw, const uint64_t *wpre, uint64_t nw, int nmol,
+                          const uint32_t *counts, uint32_t *vpre, uint64_t *tpre, uint32_t *vcoarse, uint32_t ncoarse, hipStream_t s);
 
 /* per-vertex stages */
 void qs_launch_closest_atom(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint32_t *cstart,
@@ -140,8 +142,10 @@ void qs_launch_vertex_colors(const QsMol *mols, const uint32_t *vpre, const uint
    writes nothing */
 void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *dpre, const uint32_t *ndeg, int *out,
                            const uint32_t *d_nt, hipStream_t s);
+/* kept triangles before each of n boundaries (idx, triangle ids) into out; with rb also the run's
+   read-back gathered in one array: {totals[0..3], vpre[0..n), out[0..n)} */
 void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint64_t *idx, int n, uint32_t tcap, uint32_t *out,
-                               hipStream_t s);
+                               const uint32_t *totals, const uint32_t *vpre, uint32_t *rb, hipStream_t s);
 void qs_launch_tri_localize(const int *tri, uint32_t nt, const uint32_t *tpre, const uint32_t *vpre, int nmol, int *out,
                             hipStream_t s);
 
@@ -151,6 +155,10 @@ uint32_t qs_persistent_blocks(void);
 
 /* device-wide exclusive scan pieces */
 uint64_t qs_scan_tiles(uint64_t n);
+/* inputs of at most this many entries are scanned by one block in one launch (the generic scan,
+   the shell ranks and the marching-cubes word scans) */
+#define QS_SCAN1_MAX 32768u
+void qs_launch_scan1(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *total, hipStream_t s);
 void qs_launch_tile_reduce(const uint32_t *in, uint64_t n, uint32_t *sums, hipStream_t s);
 void qs_launch_tile_scan(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *offsets, uint32_t *total,
                          hipStream_t s);

@@ -449,6 +449,166 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict
     }
 }
 
+/* The same scans in one block of SCAN1_NT threads for at most QS_SCAN1_MAX entries: one launch
+   instead of three for a small grid.  The block runs over chunks of SCAN1_NT * SCAN_IPT entries in
+   order, each wave 64 consecutive entries per load as above, carrying the running totals. */
+#define SCAN1_NT 1024
+#define SCAN1_W (SCAN1_NT / 64)
+
+/* this wave's offset (waves below it) and the chunk total, from the per-wave sums */
+__device__ __forceinline__ void scan1_offsets(const uint32_t *wsum, int wid, uint32_t &below, uint32_t &all)
+{
+    below = 0u;
+    all = 0u;
+#pragma unroll
+    for (int w = 0; w < SCAN1_W; w++) {
+        const uint32_t s = wsum[w];
+        below += (w < wid) ? s : 0u;
+        all += s;
+    }
+}
+
+/* exclusive prefix of the wave's SCAN_IPT groups of 64 in place; returns the wave's total */
+__device__ __forceinline__ uint32_t wave_groups_scan(uint32_t (&v)[SCAN_IPT], int lane)
+{
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        uint32_t x = v[k];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        const uint32_t tot = __shfl(x, 63, 64);
+        v[k] = run + x - v[k];
+        run += tot;
+    }
+    return run;
+}
+
+/* generic exclusive scan (in may equal out); total to *total if given */
+__global__ void __launch_bounds__(SCAN1_NT) k_scan1(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *__restrict__ total)
+{
+    __shared__ uint32_t wsum[SCAN1_W];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (uint64_t c0 = 0; c0 < n; c0 += SCAN1_NT * SCAN_IPT) {
+        const uint64_t base = c0 + (uint64_t)wid * (64 * SCAN_IPT) + lane;
+        uint32_t v[SCAN_IPT];
+#pragma unroll
+        for (int k = 0; k < SCAN_IPT; k++) v[k] = (base + 64 * k < n) ? in[base + 64 * k] : 0u;
+        const uint32_t run = wave_groups_scan(v, lane);
+        if (lane == 0) wsum[wid] = run;
+        __syncthreads();
+        uint32_t below, all;
+        scan1_offsets(wsum, wid, below, all);
+#pragma unroll
+        for (int k = 0; k < SCAN_IPT; k++)
+            if (base + 64 * k < n) out[base + 64 * k] = carry + below + v[k];
+        carry += all;
+        __syncthreads(); /* wsum is written again for the next chunk */
+    }
+    if (total && threadIdx.x == 0) *total = carry;
+}
+
+/* shell ranks (as k_shell_reduce, k_scan_planes, k_shell_scan): totals = {shell voxels, 0} */
+__global__ void __launch_bounds__(SCAN1_NT) k_shell_scan1(const uint64_t *__restrict__ shellm, uint64_t n,
+                                                          uint32_t *__restrict__ totals, uint32_t *__restrict__ spre)
+{
+    __shared__ uint32_t wsum[SCAN1_W];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (uint64_t c0 = 0; c0 < n; c0 += SCAN1_NT * SCAN_IPT) {
+        const uint64_t base = c0 + (uint64_t)wid * (64 * SCAN_IPT) + lane;
+        uint32_t v[SCAN_IPT];
+#pragma unroll
+        for (int k = 0; k < SCAN_IPT; k++) v[k] = (base + 64 * k < n) ? (uint32_t)__popcll(shellm[base + 64 * k]) : 0u;
+        const uint32_t run = wave_groups_scan(v, lane);
+        if (lane == 0) wsum[wid] = run;
+        __syncthreads();
+        uint32_t below, all;
+        scan1_offsets(wsum, wid, below, all);
+#pragma unroll
+        for (int k = 0; k < SCAN_IPT; k++)
+            if (base + 64 * k < n) spre[base + 64 * k] = carry + below + v[k];
+        carry += all;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        totals[0] = carry;
+        totals[1] = 0u;
+    }
+}
+
+/* marching-cubes word scans (as k_mc_reduce3, k_scan_planes, k_mc_scan3): totals = {triangles,
+   vertices, active words, 0} */
+__global__ void __launch_bounds__(SCAN1_NT) k_mc_scan1(const uint32_t *__restrict__ vcnt, const uint32_t *__restrict__ tcnt,
+                                                       uint64_t n, uint32_t *__restrict__ vbw, uint32_t *__restrict__ tbw,
+                                                       uint32_t *__restrict__ list, uint32_t *__restrict__ avb,
+                                                       uint32_t *__restrict__ atb, uint32_t list_base,
+                                                       uint32_t *__restrict__ totals)
+{
+    __shared__ uint32_t wsum[3][SCAN1_W];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t lbelow = (1ull << lane) - 1ull;
+    uint32_t cv = 0, ct = 0, ca = 0;
+    for (uint64_t c0 = 0; c0 < n; c0 += SCAN1_NT * SCAN_IPT) {
+        const uint64_t base = c0 + (uint64_t)wid * (64 * SCAN_IPT) + lane;
+        uint32_t v[SCAN_IPT], t[SCAN_IPT], pa[SCAN_IPT];
+        uint64_t am[SCAN_IPT];
+        uint32_t ra = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_IPT; k++) {
+            const uint64_t i = base + 64 * k;
+            v[k] = (i < n) ? vcnt[i] : 0u;
+            t[k] = (i < n) ? tcnt[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < SCAN_IPT; k++) {
+            am[k] = __ballot((v[k] | t[k]) != 0u);
+            pa[k] = ra + (uint32_t)__popcll(am[k] & lbelow);
+            ra += (uint32_t)__popcll(am[k]);
+        }
+        const uint32_t rv = wave_groups_scan(v, lane), rt = wave_groups_scan(t, lane);
+        if (lane == 0) {
+            wsum[0][wid] = rv;
+            wsum[1][wid] = rt;
+            wsum[2][wid] = ra;
+        }
+        __syncthreads();
+        uint32_t bv, av, bt, at, ba, aa;
+        scan1_offsets(wsum[0], wid, bv, av);
+        scan1_offsets(wsum[1], wid, bt, at);
+        scan1_offsets(wsum[2], wid, ba, aa);
+#pragma unroll
+        for (int k = 0; k < SCAN_IPT; k++) {
+            const uint64_t i = base + 64 * k;
+            if (i < n) {
+                const uint32_t fv = cv + bv + v[k], ft = ct + bt + t[k];
+                vbw[i] = fv;
+                tbw[i] = ft;
+                if ((am[k] >> lane) & 1ull) {
+                    const uint32_t r = ca + ba + pa[k];
+                    avb[r] = fv;
+                    atb[r] = ft;
+                    list[r] = list_base + (uint32_t)i;
+                }
+            }
+        }
+        cv += av;
+        ct += at;
+        ca += aa;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        totals[0] = ct;
+        totals[1] = cv;
+        totals[2] = ca;
+        totals[3] = 0u;
+    }
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* bounding box (getMinMax CudaSurf.cu:149-174): partial boxes per molecule, then one wave    */
 
@@ -464,16 +624,17 @@ __device__ __forceinline__ void bb_combine(float *v, const float *w)
 }
 
 /* block (p, m) of P partial boxes per molecule: atoms off[m] + p*BB_NT + k*P*BB_NT + t of molecule m
-   -> part[m][p] */
+   -> part[m][p] (off == nullptr: one molecule, atoms [o0, o1)) */
 __global__ void __launch_bounds__(BB_NT) k_bbox_part(const float *__restrict__ pos, const float *__restrict__ rad,
-                                                     const uint32_t *__restrict__ off, float *__restrict__ part)
+                                                     const uint32_t *__restrict__ off, uint32_t o0, uint32_t o1,
+                                                     float *__restrict__ part)
 {
     const uint32_t P = gridDim.x;
     __shared__ float s[BB_NV][BB_NT];
     const int m = blockIdx.y, p = blockIdx.x;
     float v[BB_NV] = {100000.0f, 100000.0f, 100000.0f, -100000.0f, -100000.0f, -100000.0f, 0.0f, 0.0f};
-    const uint32_t end = off[m + 1];
-    for (uint32_t i = off[m] + p * BB_NT + threadIdx.x; i < end; i += P * BB_NT) {
+    const uint32_t beg = off ? off[m] : o0, end = off ? off[m + 1] : o1;
+    for (uint32_t i = beg + p * BB_NT + threadIdx.x; i < end; i += P * BB_NT) {
         for (int c = 0; c < 3; c++) {
             float q = pos[3 * (size_t)i + c];
             v[c] = fminf(v[c], q);
@@ -985,7 +1146,7 @@ __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *_
                                                     const float4 *__restrict__ sx, const float4 *__restrict__ aux,
                                                     uint64_t *__restrict__ outm_all, uint64_t *__restrict__ shellm_all,
                                                     uint64_t *__restrict__ negm_all, uint64_t *__restrict__ zerom_all,
-                                                    uint8_t *__restrict__ tflag)
+                                                    uint8_t *__restrict__ tflag, uint8_t tmark)
 {
     extern __shared__ __align__(16) uint64_t p1s_dyn[];
     __shared__ uint32_t colpre[P1S_MAXCOL + 1], colst[P1S_MAXCOL];
@@ -1238,8 +1399,8 @@ __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *_
             if (sh && tflag && M.tile_off != ~0u && x >= M.p2_lo && x < M.p2_hi) {
                 const uint32_t nbz = (uint32_t)(g + P2_TZ - 1) / P2_TZ, nby = (uint32_t)(g + P2_TY - 1) / P2_TY;
                 const uint32_t tyx = ((uint32_t)(x - M.p2_lo) / P2R_TX) * nby + (uint32_t)(y0 + yr) / P2_TY;
-                if ((uint32_t)sh) tflag[M.tile_off + tyx * nbz + 2 * w] = 1;
-                if (sh >> 32) tflag[M.tile_off + tyx * nbz + 2 * w + 1] = 1;
+                if ((uint32_t)sh) tflag[M.tile_off + tyx * nbz + 2 * w] = tmark;
+                if (sh >> 32) tflag[M.tile_off + tyx * nbz + 2 * w + 1] = tmark;
             }
         }
     }
@@ -1588,7 +1749,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
                                                  float *__restrict__ field, uint32_t *__restrict__ negm_all,
                                                  uint32_t *__restrict__ zerom_all, const int *__restrict__ offs,
                                                  const int *__restrict__ norm_start, const int *__restrict__ enum_end,
-                                                 QsNormFlags nf, const uint8_t *__restrict__ tflag, uint32_t xchunk)
+                                                 QsNormFlags nf, const uint8_t *__restrict__ tflag, uint8_t tmark, uint32_t xchunk)
 {
     constexpr int LX = P2R_TX + 2 * R, LY = P2_TY + 2 * R, LZ = P2_TZ + 2 * R;
     constexpr int INF = 255;
@@ -1618,7 +1779,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
        of a tile are then mostly read by tiles of the same XCD, from its L2) */
     const QsMol &M = B.mols[batch_mol(B, xcd_group_rt(blockIdx.x, gridDim.x, xchunk), lb)];
     /* a tile without a shell voxel keeps the sign words field pass 1 wrote for it */
-    if (tflag && M.tile_off != ~0u && !tflag[M.tile_off + lb]) return;
+    if (tflag && M.tile_off != ~0u && tflag[M.tile_off + lb] != tmark) return;
     const QsGrid G = M.G;
     const int g = G.g, W = (int)M.W;
     const uint64_t *__restrict__ outm = outm_all + M.word_off;
@@ -2375,21 +2536,31 @@ __device__ __forceinline__ uint32_t last_le(const uint32_t *__restrict__ a, uint
 /* first active word of every 256-item block of a launch over items [0, n): active word k holds
    items [base[k], base[k + 1]) (the last one up to n), so block b starts in the word whose range
    holds item 256 b.  Words may hold items beyond n (a slab's scan counts the vertices of the next
-   slab's first plane, which that slab emits): blocks stop at n. */
-__global__ void __launch_bounds__(256) k_block_starts(const uint32_t *__restrict__ base, uint32_t nact, uint32_t n,
-                                                      const uint32_t *__restrict__ d_nact, const uint32_t *__restrict__ d_n,
-                                                      uint32_t *__restrict__ start, uint32_t *__restrict__ zero,
-                                                      uint32_t nzero)
+   slab's first plane, which that slab emits): blocks stop at n.  One launch serves the vertex
+   launch (blockIdx.y 0) and the triangle launch (1). */
+struct QsStarts {
+    const uint32_t *base; /* first item of each active word */
+    uint32_t n;           /* items the launch covers */
+    const uint32_t *d_n;  /* the item count on the device (within n), or null */
+    uint32_t *start;      /* first active word of each 256-item block */
+    uint32_t *zero;       /* nzero counters the item kernel adds to, cleared here */
+    uint32_t nzero;
+};
+
+__global__ void __launch_bounds__(256) k_block_starts(QsStarts sv, QsStarts st, uint32_t nact,
+                                                      const uint32_t *__restrict__ d_nact)
 {
+    const QsStarts A = blockIdx.y ? st : sv;
     /* counts on the device (when given) within the launch's bounds nact, n */
     if (d_nact) nact = min(*d_nact, nact);
-    if (d_n) n = min(*d_n, n);
+    uint32_t n = A.n;
+    if (A.d_n) n = min(*A.d_n, n);
     const uint32_t k0 = blockIdx.x * 256u + threadIdx.x;
-    for (uint32_t i = k0; i < nzero; i += gridDim.x * 256u) zero[i] = 0u; /* counters the next kernel adds to */
+    for (uint32_t i = k0; i < A.nzero; i += gridDim.x * 256u) A.zero[i] = 0u;
     for (uint32_t k = k0; k < nact; k += gridDim.x * 256u) {
-        const uint32_t b0 = (base[k] + 255u) >> 8;
-        const uint32_t e = (k + 1 < nact) ? min(base[k + 1], n) : n;
-        for (uint32_t b = b0; (b << 8) < e; b++) start[b] = k;
+        const uint32_t b0 = (A.base[k] + 255u) >> 8;
+        const uint32_t e = (k + 1 < nact) ? min(A.base[k + 1], n) : n;
+        for (uint32_t b = b0; (b << 8) < e; b++) A.start[b] = k;
     }
 }
 
@@ -3026,36 +3197,42 @@ __device__ __forceinline__ bool tri_degenerate(const int *__restrict__ tri, uint
 }
 
 /* Per-molecule first vertex and first triangle of a batch, from the word scans: molecule m's are
-   the exclusive scans at its first row word (first[m]; ~0 for a molecule without words, whose
-   mesh is empty and starts at the totals).  vpre / tpre get nmol + 1 entries, the last one the
-   totals (counts[1] vertices, counts[0] triangles). */
-__global__ void __launch_bounds__(256) k_mesh_bases(const uint32_t *__restrict__ vbw, const uint32_t *__restrict__ tbw,
-                                                    const uint64_t *__restrict__ first, int nmol,
-                                                    const uint32_t *__restrict__ counts, uint32_t *__restrict__ vpre,
-                                                    uint64_t *__restrict__ tpre)
+   the exclusive scans at its first row word (wpre[m]; at or past the scanned words nw for a
+   molecule without words, whose mesh is empty and starts at the totals).  vpre / tpre get nmol + 1 entries, the last one the
+   totals (counts[1] vertices, counts[0] triangles).  In the same launch, the coarse index of the
+   vertex prefix: entry k (of ncoarse, enough for a pool of vertices) = the last molecule whose
+   first vertex is at or before vertex k << QS_COARSE_ITEMS, searched over the same first
+   vertices read from the word scan. */
+__device__ __forceinline__ uint32_t mol_first_vertex(const uint32_t *__restrict__ vbw, const uint64_t *__restrict__ wpre,
+                                                     uint64_t nw, int nmol, const uint32_t *__restrict__ counts, int m)
 {
-    const int m = blockIdx.x * 256 + threadIdx.x;
-    if (m > nmol) return;
-    const uint64_t w = (m < nmol) ? first[m] : ~0ull;
-    vpre[m] = (w != ~0ull) ? vbw[w] : counts[1];
-    tpre[m] = (w != ~0ull) ? tbw[w] : counts[0];
+    const uint64_t w = (m < nmol) ? wpre[m] : nw;
+    return (w < nw) ? vbw[w] : counts[1];
 }
 
-/* coarse index of the vertex prefix: entry k = the last molecule whose first vertex is at or
-   before vertex k << QS_COARSE_ITEMS (n entries, enough for a pool of vertices) */
-__global__ void __launch_bounds__(256) k_vcoarse(const uint32_t *__restrict__ vpre, int nmol, uint32_t *__restrict__ coarse,
-                                                 uint32_t n)
+__global__ void __launch_bounds__(256) k_mesh_bases(const uint32_t *__restrict__ vbw, const uint32_t *__restrict__ tbw,
+                                                    const uint64_t *__restrict__ wpre, uint64_t nw, int nmol,
+                                                    const uint32_t *__restrict__ counts, uint32_t *__restrict__ vpre,
+                                                    uint64_t *__restrict__ tpre, uint32_t *__restrict__ coarse,
+                                                    uint32_t ncoarse)
 {
-    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    const uint64_t t = (uint64_t)k << QS_COARSE_ITEMS;
-    int lo = 0, hi = nmol - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if ((uint64_t)vpre[mid] <= t) lo = mid;
-        else hi = mid - 1;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i <= (uint32_t)nmol) {
+        const int m = (int)i;
+        const uint64_t w = (m < nmol) ? wpre[m] : nw;
+        vpre[m] = (w < nw) ? vbw[w] : counts[1];
+        tpre[m] = (w < nw) ? tbw[w] : counts[0];
     }
-    coarse[k] = (uint32_t)lo;
+    if (i < ncoarse) {
+        const uint64_t t = (uint64_t)i << QS_COARSE_ITEMS;
+        int lo = 0, hi = nmol - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((uint64_t)mol_first_vertex(vbw, wpre, nw, nmol, counts, mid) <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        coarse[i] = (uint32_t)lo;
+    }
 }
 
 /* degenerate triangles per 256-triangle block (and their total, added to *deg_total), as the
@@ -3102,8 +3279,12 @@ __global__ void __launch_bounds__(256) k_tri_compact(const int *__restrict__ tri
 /* kept triangles before each of n boundaries idx[i] (molecule starts and the end) */
 __global__ void __launch_bounds__(256) k_tri_kept_before(const int *__restrict__ tri, const uint32_t *__restrict__ dpre,
                                                          const uint64_t *__restrict__ idx, int n, uint32_t tcap,
-                                                         uint32_t *__restrict__ out)
+                                                         uint32_t *__restrict__ out, const uint32_t *__restrict__ totals,
+                                                         const uint32_t *__restrict__ vpre, uint32_t *__restrict__ rb)
 {
+    /* with rb: the run's read-back in one array, {totals[0..3], vpre[0..n), out[0..n)} */
+    if (rb && blockIdx.x == 0)
+        for (int j = (int)threadIdx.x; j < 4 + n; j += 256) rb[j] = (j < 4) ? totals[j] : vpre[j - 4];
     /* one wave per boundary: the degenerate triangles of the boundary's 256-triangle block below
        it, four per lane, counted by ballot */
     const int i = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -3117,7 +3298,10 @@ __global__ void __launch_bounds__(256) k_tri_kept_before(const int *__restrict__
         const uint32_t u = (b << 8) + (uint32_t)(64 * k + lane);
         d += (uint32_t)__popcll(__ballot(u < t && tri_degenerate(tri, u)));
     }
-    if (lane == 0) out[i] = t - dpre[b] - d;
+    if (lane == 0) {
+        out[i] = t - dpre[b] - d;
+        if (rb) rb[4 + n + i] = t - dpre[b] - d;
+    }
 }
 
 /* plain device copy (16-byte vectors, grid-stride): the reference point for traffic counters */
@@ -3166,10 +3350,15 @@ int qs_bbox_parts(uint32_t max_atoms)
 
 size_t qs_bbox_part_floats(int nmol, int parts) { return (size_t)nmol * parts * BB_NV; }
 
-void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, int nmol, int parts, float *part, float *out,
-                    hipStream_t s)
+void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, uint32_t o0, uint32_t o1, int nmol, int parts,
+                    float *part, float *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bbox_part, dim3(parts, nmol), dim3(BB_NT), 0, s, pos, rad, off, part);
+    /* one partial box per molecule is its box (part[m][0] and out[m] are at the same offset) */
+    if (parts == 1) {
+        hipLaunchKernelGGL(k_bbox_part, dim3(1, nmol), dim3(BB_NT), 0, s, pos, rad, off, o0, o1, out);
+        return;
+    }
+    hipLaunchKernelGGL(k_bbox_part, dim3(parts, nmol), dim3(BB_NT), 0, s, pos, rad, off, o0, o1, part);
     hipLaunchKernelGGL(k_bbox_final, dim3(nmol), dim3(64), 0, s, part, parts, out);
 }
 
@@ -3256,13 +3445,13 @@ size_t qs_pass1_lds_bytes(const QsMol &M, int mode)
 
 void qs_launch_pass1(QsBatch B, uint32_t nblocks, int mode, size_t lds_bytes, const uint32_t *cstart, const float4 *sx,
                      const float4 *aux, uint64_t *outm, uint64_t *shellm, uint64_t *negm, uint64_t *zerom, uint8_t *tflag,
-                     hipStream_t s)
+                     uint8_t tmark, hipStream_t s)
 {
     if (mode == 1)
         hipLaunchKernelGGL(k_pass1_rows, dim3(nblocks), dim3(256), 0, s, B, cstart, sx, aux, outm, shellm);
     else
         hipLaunchKernelGGL(k_pass1_slab, dim3(nblocks), dim3(256), lds_bytes, s, B, cstart, sx, aux, outm, shellm, negm, zerom,
-                           tflag);
+                           tflag, tmark);
 }
 
 void qs_launch_pass1_direct(QsBatch B, uint32_t nblocks, const int2 *cse, const float4 *sx, uint64_t *outm, uint64_t *shellm,
@@ -3292,14 +3481,14 @@ uint32_t qs_pass2_blocks(const QsMol &M, int path)
 /* one launch for a group of molecules sharing r (and so the path) */
 void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm,
                      const uint32_t *spre, float *field, uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,
-                     QsNormFlags nf, const uint8_t *tflag, uint32_t xchunk, hipStream_t s)
+                     QsNormFlags nf, const uint8_t *tflag, uint8_t tmark, uint32_t xchunk, hipStream_t s)
 {
     if (path == 0) {
         xchunk = std::max(xchunk, 1u);
 #define QS_P2R(RR)                                                                                                    \
     case RR:                                                                                                          \
         hipLaunchKernelGGL(k_pass2_r<RR>, dim3(nblocks), dim3(256), 0, s, B, outm, shellm, spre, field, negm, zerom, \
-                           offs, norm_start, enum_end, nf, tflag, xchunk);                                            \
+                           offs, norm_start, enum_end, nf, tflag, tmark, xchunk);                                            \
         return;
         switch (r) {
             QS_P2R(1)
@@ -3327,6 +3516,10 @@ void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_
 void qs_launch_shell_scan(const uint64_t *shellm, uint64_t nw, uint32_t *sums, uint32_t *totals, uint32_t *spre, hipStream_t s)
 {
     if (!nw) return;
+    if (nw <= QS_SCAN1_MAX) {
+        hipLaunchKernelGGL(k_shell_scan1, dim3(1), dim3(SCAN1_NT), 0, s, shellm, nw, totals, spre);
+        return;
+    }
     const uint64_t ntiles = qs_scan_tiles(nw);
     hipLaunchKernelGGL(k_shell_reduce, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, shellm, nw, sums);
     hipLaunchKernelGGL(k_scan_planes, dim3(1), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
@@ -3350,32 +3543,27 @@ void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const ui
     hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(256), 0, s, B, N, Zm, S4, vcnt, tcnt, NT);
 }
 
-void qs_launch_mc_vertices(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
-                           const uint32_t *avb, uint32_t *vstart, uint32_t nact, uint32_t nv, const uint64_t *S4,
-                           const float *F, const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *vert,
-                           uint64_t *vkey, const uint32_t *d_counts, hipStream_t s)
+void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list, uint32_t nact,
+                       const uint32_t *avb, uint32_t *vstart, uint32_t nv, const uint64_t *S4, const float *F,
+                       const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *vert, uint64_t *vkey,
+                       const uint32_t *atb, uint32_t *tstart, uint32_t ntri, const uint64_t *N, const uint64_t *Zm,
+                       const uint32_t *vbw, const uint64_t *NT, int *tri, uint32_t vbase, uint32_t *deg_blk,
+                       uint32_t *deg_total, const uint32_t *d_counts, hipStream_t s)
 {
-    if (!nv || !nact) return;
-    hipLaunchKernelGGL(k_block_starts, dim3(pgrid(nblk(nact, 256))), dim3(256), 0, s, avb, nact, nv,
-                       d_counts ? d_counts + 2 : nullptr, d_counts ? d_counts + 1 : nullptr, vstart, nullptr, 0u);
-    hipLaunchKernelGGL(k_mc_vertices, dim3(pgrid(nblk(nv, 256))), dim3(256), 0, s, mols, wpre, wcoarse, list, avb, vstart, nact, nv,
-                       S4, F, spre, outm, shellm, vert, vkey, d_counts);
-}
-
-void qs_launch_mc_triangles(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list,
-                            const uint32_t *atb, uint32_t *tstart, uint32_t nact, uint32_t ntri, const uint64_t *N,
-                            const uint64_t *Zm, const uint64_t *S4, const uint32_t *vbw, const uint64_t *NT,
-                            int *tri, uint32_t vbase, uint32_t *deg_blk, uint32_t *deg_total,
-                            const uint32_t *d_counts, hipStream_t s)
-{
-    if (!ntri || !nact) return;
+    if (!nact || (!nv && !ntri)) return;
     /* the per-block degenerate counts start at zero (the triangle kernel adds to them), over the
        whole pool of ntri triangles */
-    hipLaunchKernelGGL(k_block_starts, dim3(pgrid(nblk(std::max<uint32_t>(nact, ntri / 256 + 1), 256))), dim3(256), 0, s, atb,
-                       nact, ntri, d_counts ? d_counts + 2 : nullptr, d_counts ? d_counts : nullptr, tstart, deg_blk,
-                       ntri / 256 + 1);
-    hipLaunchKernelGGL(k_mc_triangles, dim3(pgrid(nblk(ntri, 256))), dim3(256), 0, s, mols, wpre, wcoarse, list, atb, tstart, nact,
-                       ntri, N, Zm, S4, vbw, NT, tri, vbase, deg_blk, deg_total, d_counts);
+    const uint32_t nzero = ntri ? ntri / 256 + 1 : 0u;
+    const QsStarts sv = {avb, nv, d_counts ? d_counts + 1 : nullptr, vstart, nullptr, 0u};
+    const QsStarts st = {atb, ntri, d_counts ? d_counts : nullptr, tstart, deg_blk, nzero};
+    hipLaunchKernelGGL(k_block_starts, dim3(pgrid(nblk(std::max<uint32_t>(nact, nzero), 256)), 2), dim3(256), 0, s, sv, st, nact,
+                       d_counts ? d_counts + 2 : nullptr);
+    if (nv)
+        hipLaunchKernelGGL(k_mc_vertices, dim3(pgrid(nblk(nv, 256))), dim3(256), 0, s, mols, wpre, wcoarse, list, avb, vstart, nact,
+                           nv, S4, F, spre, outm, shellm, vert, vkey, d_counts);
+    if (ntri)
+        hipLaunchKernelGGL(k_mc_triangles, dim3(pgrid(nblk(ntri, 256))), dim3(256), 0, s, mols, wpre, wcoarse, list, atb, tstart,
+                           nact, ntri, N, Zm, S4, vbw, NT, tri, vbase, deg_blk, deg_total, d_counts);
 }
 
 void qs_launch_smooth(const QsMol *mols, const uint32_t *vpre, const uint32_t *vcoarse, const uint64_t *N, const uint64_t *Zm,
@@ -3438,10 +3626,10 @@ void qs_launch_tri_compact(const int *tri, uint32_t nt, const uint32_t *dpre, co
 }
 
 void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint64_t *idx, int n, uint32_t tcap, uint32_t *out,
-                               hipStream_t s)
+                               const uint32_t *totals, const uint32_t *vpre, uint32_t *rb, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_tri_kept_before, dim3(nblk(n, 4)), dim3(256), 0, s, tri, dpre, idx, n, tcap, out);
+    hipLaunchKernelGGL(k_tri_kept_before, dim3(nblk(n, 4)), dim3(256), 0, s, tri, dpre, idx, n, tcap, out, totals, vpre, rb);
 }
 
 /* scan helpers */
@@ -3449,8 +3637,13 @@ void qs_launch_mc_scans(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw,
                         uint32_t *tbw, uint32_t *list, uint32_t *avb, uint32_t *atb, uint32_t *totals, uint32_t list_base,
                         hipStream_t s)
 {
-    const uint64_t ntiles = qs_scan_tiles(nw);
     if (!nw) return;
+    if (nw <= QS_SCAN1_MAX) {
+        hipLaunchKernelGGL(k_mc_scan1, dim3(1), dim3(SCAN1_NT), 0, s, vcnt, tcnt, nw, vbw, tbw, list, avb, atb, list_base,
+                           totals);
+        return;
+    }
+    const uint64_t ntiles = qs_scan_tiles(nw);
     hipLaunchKernelGGL(k_mc_reduce3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles);
     hipLaunchKernelGGL(k_scan_planes, dim3(3), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
     hipLaunchKernelGGL(k_mc_scan3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles, vbw, tbw, list,
@@ -3462,6 +3655,11 @@ uint64_t qs_scan_tiles(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 void qs_launch_tile_reduce(const uint32_t *in, uint64_t n, uint32_t *sums, hipStream_t s)
 {
     hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)qs_scan_tiles(n)), dim3(SCAN_NT), 0, s, in, n, sums);
+}
+
+void qs_launch_scan1(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *total, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(SCAN1_NT), 0, s, in, out, n, total);
 }
 
 void qs_launch_tile_scan(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *offsets, uint32_t *total,
@@ -3515,12 +3713,11 @@ void qs_launch_deg_count(const int *tri, uint32_t nt, uint32_t *deg_blk, uint32_
     hipLaunchKernelGGL(k_deg_count, dim3(nblk(nt, 256)), dim3(256), 0, s, tri, nt, deg_blk, deg_total);
 }
 
-void qs_launch_mesh_bases(const uint32_t *vbw, const uint32_t *tbw, const uint64_t *first, int nmol, const uint32_t *counts,
-                          uint32_t *vpre, uint64_t *tpre, uint32_t *vcoarse, uint32_t ncoarse, hipStream_t s)
+void qs_launch_mesh_bases(const uint32_t *vbw, const uint32_t *tbw, const uint64_t *wpre, uint64_t nw, int nmol,
+                          const uint32_t *counts, uint32_t *vpre, uint64_t *tpre, uint32_t *vcoarse, uint32_t ncoarse, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mesh_bases, dim3(nblk((uint64_t)nmol + 1, 256)), dim3(256), 0, s, vbw, tbw, first, nmol, counts, vpre,
-                       tpre);
-    if (ncoarse) hipLaunchKernelGGL(k_vcoarse, dim3(nblk(ncoarse, 256)), dim3(256), 0, s, vpre, nmol, vcoarse, ncoarse);
+    hipLaunchKernelGGL(k_mesh_bases, dim3(nblk(std::max<uint64_t>((uint64_t)nmol + 1, ncoarse), 256)), dim3(256), 0, s, vbw, tbw,
+                       wpre, nw, nmol, counts, vpre, tpre, vcoarse, ncoarse);
 }
 
 uint32_t qs_pass2_tiles(const QsMol &M) { return qs_pass2_blocks(M, 0); }

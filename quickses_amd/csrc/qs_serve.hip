@@ -166,13 +166,12 @@ int run_batch_requests(Server *S, Engine &E, std::vector<Request *> &batch)
     }
     QS_TRY(ensure(E.in_pos, std::max<uint64_t>(n, 1) * 12));
     QS_TRY(ensure(E.in_rad, std::max<uint64_t>(n, 1) * 4));
+    QS_TRY(E.upload_begin());
     for (int i = 0; i < nb; i++) {
         const size_t k = batch[i]->rad.size();
         if (!k) continue;
-        QS_HIP(hipMemcpyAsync(as<float>(E.in_pos) + 3 * (size_t)in[i].atom_off, batch[i]->pos.data(), k * 12,
-                              hipMemcpyHostToDevice, E.stream));
-        QS_HIP(hipMemcpyAsync(as<float>(E.in_rad) + in[i].atom_off, batch[i]->rad.data(), k * 4, hipMemcpyHostToDevice,
-                              E.stream));
+        QS_TRY(E.upload_input(as<float>(E.in_pos) + 3 * (size_t)in[i].atom_off, batch[i]->pos.data(), k * 12));
+        QS_TRY(E.upload_input(as<float>(E.in_rad) + in[i].atom_off, batch[i]->rad.data(), k * 4));
     }
     QS_TRY(E.run_batch(as<float>(E.in_pos), as<float>(E.in_rad), in, batch[0]->smooth, &bbs));
     if (E.nt) {
