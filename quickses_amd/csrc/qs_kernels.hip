@@ -2829,22 +2829,12 @@ __global__ void __launch_bounds__(256) k_mc_triangles(const QsMol *__restrict__ 
 
 /* ------------------------------------------------------------------------------------------ */
 /* closest atom per vertex (computeClosestAtom Kernels.cu:547-604)                              */
-__global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
-                                                      const uint32_t *__restrict__ vcoarse, const uint32_t *__restrict__ cstart_all,
-                                                      const float4 *__restrict__ sx, const int *__restrict__ perm,
-                                                      const float *__restrict__ vert, uint32_t nv, int sorted_ids,
-                                                      int *__restrict__ out, const uint32_t *__restrict__ d_nv)
+/* closest atom of vertex v (position p) of molecule M, relative to its first atom (-1: none) */
+__device__ __forceinline__ int closest_one(const QsMol &M, float px, float py, float pz, const uint32_t *__restrict__ cstart_all,
+                                           const float4 *__restrict__ sx, const int *__restrict__ perm, int sorted_ids)
 {
-    if (d_nv) nv = min(*d_nv, nv); /* the device count within the launch's pool */
-    const uint32_t nb = (nv + 255u) >> 8;
-    /* persistent blocks over the 256-vertex blocks (grid a multiple of 8: see k_mc_vertices) */
-    for (uint32_t vb = blockIdx.x; vb < nb; vb += gridDim.x) {
-    const uint32_t v = xcd_group<64>(vb, nb) * 256u + threadIdx.x;
-    if (v >= nv) continue;
-    const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
     const QsGrid G = M.G;
     const uint32_t *__restrict__ cstart = cstart_all + M.cell_off;
-    float px = vert[3 * (size_t)v], py = vert[3 * (size_t)v + 1], pz = vert[3 * (size_t)v + 2];
     int cx = cell_of(px, G.ox, G.dxn), cy = cell_of(py, G.oy, G.dxn), cz = cell_of(pz, G.oz, G.dxn);
     const int gn = G.gn;
     /* The reference visits the 27 clamped cells x-major, z fastest, and keeps the first atom whose
@@ -2864,26 +2854,41 @@ __global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ 
         lo9[k] = (int)cstart[h + z0];
         hi9[k] = (int)cstart[h + z1 + 1];
     }
-    {
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            const int lo = lo9[k], hi = hi9[k];
-            for (int id = lo; id < hi; id++) {
-                const float4 at = sx[id];
-                const float sq = dist3_sq(at.x, at.y, at.z, px, py, pz);
-                if (sq < min_s) {
-                    const float d = sqrt_rn(sq);
-                    if (d < min_d) {
-                        min_d = d;
-                        min_s = sq;
-                        best = id;
-                    }
+    for (int k = 0; k < 9; k++) {
+        const int lo = lo9[k], hi = hi9[k];
+        for (int id = lo; id < hi; id++) {
+            const float4 at = sx[id];
+            const float sq = dist3_sq(at.x, at.y, at.z, px, py, pz);
+            if (sq < min_s) {
+                const float d = sqrt_rn(sq);
+                if (d < min_d) {
+                    min_d = d;
+                    min_s = sq;
+                    best = id;
                 }
             }
         }
     }
     /* atom indices are returned relative to the molecule's first atom */
-    out[v] = (best < 0) ? -1 : ((sorted_ids ? best : perm[best]) - (int)M.atom_off);
+    return (best < 0) ? -1 : ((sorted_ids ? best : perm[best]) - (int)M.atom_off);
+}
+
+__global__ void __launch_bounds__(256) k_closest_atom(const QsMol *__restrict__ mols, const uint32_t *__restrict__ vpre,
+                                                      const uint32_t *__restrict__ vcoarse, const uint32_t *__restrict__ cstart_all,
+                                                      const float4 *__restrict__ sx, const int *__restrict__ perm,
+                                                      const float *__restrict__ vert, uint32_t nv, int sorted_ids,
+                                                      int *__restrict__ out, const uint32_t *__restrict__ d_nv)
+{
+    if (d_nv) nv = min(*d_nv, nv); /* the device count within the launch's pool */
+    const uint32_t nb = (nv + 255u) >> 8;
+    /* persistent blocks over the 256-vertex blocks (grid a multiple of 8: see k_mc_vertices) */
+    for (uint32_t vb = blockIdx.x; vb < nb; vb += gridDim.x) {
+        const uint32_t v = xcd_group<64>(vb, nb) * 256u + threadIdx.x;
+        if (v >= nv) continue;
+        const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
+        out[v] = closest_one(M, vert[3 * (size_t)v], vert[3 * (size_t)v + 1], vert[3 * (size_t)v + 2], cstart_all, sx, perm,
+                             sorted_ids);
     }
 }
 
@@ -2920,6 +2925,115 @@ __device__ __forceinline__ uint32_t cube_first_tri(const uint64_t *__restrict__ 
     return tbw[wi] + (uint32_t)(__popcll(np[0] & below) + 2 * __popcll(np[1] & below) + 4 * __popcll(np[2] & below));
 }
 
+/* The grid records the smoothing walk reads (one molecule's view of the batch arrays) */
+struct SmoothView {
+    MCView V;
+    const uint64_t *__restrict__ NT;
+    const uint32_t *__restrict__ tbw;
+};
+
+__device__ __forceinline__ SmoothView smooth_view(const QsMol &M, const uint64_t *__restrict__ N_all,
+                                                  const uint64_t *__restrict__ Zm_all, const uint64_t *__restrict__ S4_all,
+                                                  const uint32_t *__restrict__ vbw_all, const uint64_t *__restrict__ NT_all,
+                                                  const uint32_t *__restrict__ tbw_all)
+{
+    SmoothView S;
+    S.V.N = N_all + M.word_off;
+    S.V.Zm = Zm_all + M.word_off;
+    S.V.S4 = S4_all + 4 * M.word_off;
+    S.V.vbw = vbw_all + M.word_off;
+    S.V.g = M.G.g;
+    S.V.W = (int)M.W;
+    S.V.lim = M.G.lim;
+    S.NT = NT_all + 3 * M.word_off;
+    S.tbw = tbw_all + M.word_off;
+    return S;
+}
+
+/* The incident cubes of an edge vertex: the 4 cubes around the edge in cube order (offsets -1 / 0
+   of the two other axes, major first); every one inside the grid contains the sign change, so it
+   is polygonised and its word's triangle records exist.  Their first triangles and counts (0 for
+   a cube outside the grid) are loaded here, all four before any is used. */
+__device__ __forceinline__ void smooth_edge_cubes(const SmoothView &S, int x, int y, int z, int sl, int (&nt4)[4],
+                                                  uint32_t (&tb4)[4])
+{
+    const int g = S.V.g, W = S.V.W, lim = S.V.lim;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int hi = k >> 1, lo = k & 1;
+        const int cx = x - (sl != 0 ? 1 - hi : 0);
+        const int cy = y - (sl == 0 ? 1 - hi : (sl == 2 ? 1 - lo : 0));
+        const int cz = z - (sl != 2 ? 1 - lo : 0);
+        nt4[k] = 0;
+        tb4[k] = 0;
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= lim || cy >= lim || cz >= lim) continue;
+        const size_t wi = ((size_t)cx * g + cy) * W + (cz >> 6);
+        const int b = cz & 63;
+        const uint64_t *np = S.NT + 3 * wi;
+        const uint64_t p0 = np[0], p1 = np[1], p2 = np[2], below = (1ull << b) - 1ull;
+        tb4[k] = S.tbw[wi] + (uint32_t)(__popcll(p0 & below) + 2 * __popcll(p1 & below) + 4 * __popcll(p2 & below));
+        nt4[k] = (int)((p0 >> b) & 1ull) + 2 * (int)((p1 >> b) & 1ull) + 4 * (int)((p2 >> b) & 1ull);
+    }
+}
+
+/* vertex v's Jacobi sum over the neighbour list walked from its cubes (edge vertex: the cubes of
+   smooth_edge_cubes; zero point: the 8 cubes around it), written to dst */
+__device__ __forceinline__ void smooth_sum(const SmoothView &S, uint32_t v, int x, int y, int z, int sl, const int (&nt4)[4],
+                                           const uint32_t (&tb4)[4], const int *__restrict__ tri,
+                                           const float *__restrict__ src, float *__restrict__ dst)
+{
+    const int g = S.V.g, W = S.V.W, lim = S.V.lim;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    uint32_t cnt = 0;
+    if (sl < 3) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            for (int t = 0; t < nt4[k]; t++) {
+                const int *tt = tri + 3 * (size_t)(tb4[k] + t);
+                const uint32_t a = (uint32_t)tt[0], b = (uint32_t)tt[1], c = (uint32_t)tt[2];
+                if (a == v) push_pair(src, b, c, sx, sy, sz);
+                else if (b == v) push_pair(src, a, c, sx, sy, sz);
+                else if (c == v) push_pair(src, a, b, sx, sy, sz);
+                else continue;
+                cnt += 2;
+            }
+        }
+    } else {
+        /* a zero grid point: the 8 cubes around it, every corner that is this vertex */
+        for (int k = 0; k < 8; k++) {
+            const int cx = x - 1 + (k >> 2), cy = y - 1 + ((k >> 1) & 1), cz = z - 1 + (k & 1);
+            if (cx < 0 || cy < 0 || cz < 0 || cx >= lim || cy >= lim || cz >= lim) continue;
+            const int ci = S.V.cube_index(cx, cy, cz);
+            const int nt = tri_count(ci);
+            if (!nt) continue;
+            const uint32_t tb = cube_first_tri(S.NT, S.tbw, ((size_t)cx * g + cy) * W + (cz >> 6), cz & 63);
+            for (int t = 0; t < nt; t++) {
+                const int *tt = tri + 3 * (size_t)(tb + t);
+                const uint32_t a = (uint32_t)tt[0], b = (uint32_t)tt[1], c = (uint32_t)tt[2];
+                if (a == v) { push_pair(src, b, c, sx, sy, sz); cnt += 2; }
+                if (b == v) { push_pair(src, a, c, sx, sy, sz); cnt += 2; }
+                if (c == v) { push_pair(src, a, b, sx, sy, sz); cnt += 2; }
+            }
+        }
+    }
+    const float fc = (float)cnt;
+    dst[3 * (size_t)v + 0] = sx / fc;
+    dst[3 * (size_t)v + 1] = sy / fc;
+    dst[3 * (size_t)v + 2] = sz / fc;
+}
+
+/* grid point (x, y, z) and slot of a vertex key (grid point index << 2 | slot; a molecule's grid
+   has fewer than 2^32 points) */
+__device__ __forceinline__ void vertex_point(uint64_t key, int g, int &x, int &y, int &z, int &sl)
+{
+    sl = (int)(key & 3ull);
+    const uint32_t p = (uint32_t)(key >> 2), ug = (uint32_t)g;
+    const uint32_t xy = p / ug;
+    z = (int)(p - xy * ug);
+    x = (int)(xy / ug);
+    y = (int)(xy - (uint32_t)x * ug);
+}
+
 /* one Jacobi pass, one thread per vertex.  A polygonised cube's triangles are consecutive from
    its first triangle id (its word's triangle base plus the triangle-count planes below it), and
    its count is its bit of those planes, so each incident triangle's other corners are read from
@@ -2942,87 +3056,16 @@ __global__ void __launch_bounds__(256) k_smooth(const QsMol *__restrict__ mols, 
     const uint32_t nb = (nv + 255u) >> 8;
     /* persistent blocks over the 256-vertex blocks (grid a multiple of 8: see k_mc_vertices) */
     for (uint32_t vb = blockIdx.x; vb < nb; vb += gridDim.x) {
-    const uint32_t v = xcd_group<64>(vb, nb) * 256u + threadIdx.x;
-    if (v >= nv) continue;
-    const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
-    const QsGrid G = M.G;
-    const int g = G.g, lim = G.lim;
-    MCView V;
-    V.N = N_all + M.word_off;
-    V.Zm = Zm_all + M.word_off;
-    V.S4 = S4_all + 4 * M.word_off;
-    V.vbw = vbw_all + M.word_off;
-    V.g = g;
-    V.W = (int)M.W;
-    V.lim = lim;
-    const uint64_t *__restrict__ NT = NT_all + 3 * M.word_off;
-    const uint32_t *__restrict__ tbw = tbw_all + M.word_off;
-    const int W = (int)M.W;
-    const uint64_t key = vkey[v];
-    const int sl = (int)(key & 3ull);
-    /* key = grid point index << 2 | slot; a molecule's grid has fewer than 2^32 points */
-    const uint32_t p = (uint32_t)(key >> 2), ug = (uint32_t)g;
-    const uint32_t xy = p / ug;
-    const int z = (int)(p - xy * ug);
-    const int x = (int)(xy / ug), y = (int)(xy - (uint32_t)x * ug);
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    uint32_t cnt = 0;
-    if (sl < 3) {
-        /* the 4 cubes around the edge in cube order (offsets -1 / 0 of the two other axes, major
-           first); every one inside the grid contains the sign change, so it is polygonised and its
-           word's triangle records exist.  The records of all four are loaded before any is used. */
-        int nt4[4];
-        uint32_t tb4[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int hi = k >> 1, lo = k & 1;
-            const int cx = x - (sl != 0 ? 1 - hi : 0);
-            const int cy = y - (sl == 0 ? 1 - hi : (sl == 2 ? 1 - lo : 0));
-            const int cz = z - (sl != 2 ? 1 - lo : 0);
-            nt4[k] = 0;
-            tb4[k] = 0;
-            if (cx < 0 || cy < 0 || cz < 0 || cx >= lim || cy >= lim || cz >= lim) continue;
-            const size_t wi = ((size_t)cx * g + cy) * W + (cz >> 6);
-            const int b = cz & 63;
-            const uint64_t *np = NT + 3 * wi;
-            const uint64_t p0 = np[0], p1 = np[1], p2 = np[2], below = (1ull << b) - 1ull;
-            tb4[k] = tbw[wi] + (uint32_t)(__popcll(p0 & below) + 2 * __popcll(p1 & below) + 4 * __popcll(p2 & below));
-            nt4[k] = (int)((p0 >> b) & 1ull) + 2 * (int)((p1 >> b) & 1ull) + 4 * (int)((p2 >> b) & 1ull);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            for (int t = 0; t < nt4[k]; t++) {
-                const int *tt = tri + 3 * (size_t)(tb4[k] + t);
-                const uint32_t a = (uint32_t)tt[0], b = (uint32_t)tt[1], c = (uint32_t)tt[2];
-                if (a == v) push_pair(src, b, c, sx, sy, sz);
-                else if (b == v) push_pair(src, a, c, sx, sy, sz);
-                else if (c == v) push_pair(src, a, b, sx, sy, sz);
-                else continue;
-                cnt += 2;
-            }
-        }
-    } else {
-        /* a zero grid point: the 8 cubes around it, every corner that is this vertex */
-        for (int k = 0; k < 8; k++) {
-            const int cx = x - 1 + (k >> 2), cy = y - 1 + ((k >> 1) & 1), cz = z - 1 + (k & 1);
-            if (cx < 0 || cy < 0 || cz < 0 || cx >= lim || cy >= lim || cz >= lim) continue;
-            const int ci = V.cube_index(cx, cy, cz);
-            const int nt = tri_count(ci);
-            if (!nt) continue;
-            const uint32_t tb = cube_first_tri(NT, tbw, ((size_t)cx * g + cy) * W + (cz >> 6), cz & 63);
-            for (int t = 0; t < nt; t++) {
-                const int *tt = tri + 3 * (size_t)(tb + t);
-                const uint32_t a = (uint32_t)tt[0], b = (uint32_t)tt[1], c = (uint32_t)tt[2];
-                if (a == v) { push_pair(src, b, c, sx, sy, sz); cnt += 2; }
-                if (b == v) { push_pair(src, a, c, sx, sy, sz); cnt += 2; }
-                if (c == v) { push_pair(src, a, b, sx, sy, sz); cnt += 2; }
-            }
-        }
-    }
-    const float fc = (float)cnt;
-    dst[3 * (size_t)v + 0] = sx / fc;
-    dst[3 * (size_t)v + 1] = sy / fc;
-    dst[3 * (size_t)v + 2] = sz / fc;
+        const uint32_t v = xcd_group<64>(vb, nb) * 256u + threadIdx.x;
+        if (v >= nv) continue;
+        const QsMol &M = mols[seg_of_wave(vpre, vcoarse, v)];
+        const SmoothView S = smooth_view(M, N_all, Zm_all, S4_all, vbw_all, NT_all, tbw_all);
+        int x, y, z, sl;
+        vertex_point(vkey[v], S.V.g, x, y, z, sl);
+        int nt4[4] = {0, 0, 0, 0};
+        uint32_t tb4[4] = {0u, 0u, 0u, 0u};
+        if (sl < 3) smooth_edge_cubes(S, x, y, z, sl, nt4, tb4);
+        smooth_sum(S, v, x, y, z, sl, nt4, tb4, tri, src, dst);
     }
 }
 

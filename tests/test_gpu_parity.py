@@ -204,3 +204,20 @@ def test_device_entry_point(qs):
     T = hipmem.download(pt, (ni // 3, 3), np.int32)
     A = hipmem.download(pa, (nv,), np.int32)
     assert np.array_equal(V, ref[0]) and np.array_equal(T, ref[1]) and np.array_equal(A, ref[2])
+
+
+def test_repeated_runs_with_stale_tile_marks(qs, orc):
+    """The pass-2 tile marks are never cleared: each run marks with its own value (1-255, in
+    turn), and a stale mark equal to it only makes pass 2 rewrite a tile without shell voxels.
+    Two molecules with different tile layouts alternate over more than two turns of the marks;
+    every mesh equals that molecule's first mesh bit for bit, and the first ones equal the
+    oracle's."""
+    a = qs.synth_ball(41, 700)
+    b = qs.synth_ball(42, 1500)
+    first = [qs.compute_ses(*a, 0.4, 1), qs.compute_ses(*b, 0.5, 1)]
+    _same_mesh(first[0], orc.compute_ses(*a, 0.4, 1))
+    _same_mesh(first[1], orc.compute_ses(*b, 0.5, 1))
+    print("case 600 alternating runs", flush=True)
+    for k in range(600):
+        got = qs.compute_ses(*a, 0.4, 1) if k % 2 == 0 else qs.compute_ses(*b, 0.5, 1)
+        _same_mesh(got, first[k % 2])
