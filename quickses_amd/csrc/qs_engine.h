@@ -396,6 +396,14 @@ struct Engine {
             QS_HIP(hipGetLastError());
             return 0;
         }
+        if (qs_scan_tiles(n) > 1 && qs_scan_tiles(n) <= QS_SCAN_SELF_TILES) {
+            /* the tile sums, then the tiles, each adding up the sums before it */
+            QS_TRY(ensure(scan_tmp, 64 * sizeof(uint32_t)));
+            qs_launch_tile_reduce(in, n, as<uint32_t>(scan_tmp), stream);
+            qs_launch_tile_scan(in, out, n, nullptr, d_total, stream, as<uint32_t>(scan_tmp));
+            QS_HIP(hipGetLastError());
+            return 0;
+        }
         std::vector<uint64_t> lv;
         uint64_t m = n;
         while (qs_scan_tiles(m) > 1) {

@@ -156,12 +156,16 @@ uint32_t qs_persistent_blocks(void);
 /* device-wide exclusive scan pieces */
 uint64_t qs_scan_tiles(uint64_t n);
 /* inputs of at most this many entries are scanned by one block in one launch (the generic scan,
-   the shell ranks and the marching-cubes word scans) */
-#define QS_SCAN1_MAX 32768u
+   the shell ranks and the marching-cubes word scans; one block's memory throughput makes more
+   entries slower than several blocks); inputs of at most QS_SCAN_SELF_TILES tiles of 2,048 are
+   scanned in two launches, each tile adding up the sums of the tiles before it itself */
+#define QS_SCAN1_MAX 8192u
+#define QS_SCAN_SELF_TILES 64u
 void qs_launch_scan1(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *total, hipStream_t s);
 void qs_launch_tile_reduce(const uint32_t *in, uint64_t n, uint32_t *sums, hipStream_t s);
+/* (self_sums: the raw sums of at most QS_SCAN_SELF_TILES tiles, offsets then nullptr) */
 void qs_launch_tile_scan(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *offsets, uint32_t *total,
-                         hipStream_t s);
+                         hipStream_t s, const uint32_t *self_sums = nullptr);
 void qs_launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
 /* marching-cubes word scans: vbw, tbw, the active-word list and totals {triangles, vertices,
    active words}; sums holds 3 * qs_scan_tiles(nw) entries; list entries are list_base + the

@@ -215,6 +215,17 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *lds)
     return t;
 }
 
+/* sum of sums[0 .. b) for b <= 64, in every lane of the wave (a scan whose tile count is small
+   takes its tiles' offsets this way instead of from a launch that scans the tile sums) */
+__device__ __forceinline__ uint32_t wave_sum_below(const uint32_t *__restrict__ sums, uint32_t b)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t x = (lane < b) ? sums[lane] : 0u;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += (uint32_t)__shfl_xor((int)x, d, 64);
+    return x;
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* device-wide exclusive scan of uint32 (reduce tiles, scan tile sums, scan tiles)              */
 
@@ -237,10 +248,11 @@ __global__ void __launch_bounds__(SCAN_NT) k_tile_reduce(const uint32_t *__restr
     if (threadIdx.x == 0) sums[blockIdx.x] = t;
 }
 
-/* out[i] = offsets[tile] + exclusive prefix inside the tile; in may alias out */
+/* out[i] = offsets[tile] + exclusive prefix inside the tile; in may alias out.  With self_sums
+   (at most QS_SCAN_SELF_TILES tiles) the tile's offset is the sum of the tile sums before it. */
 __global__ void __launch_bounds__(SCAN_NT) k_tile_scan(const uint32_t *in, uint32_t *out, uint64_t n,
                                                        const uint32_t *__restrict__ offsets,
-                                                       uint32_t *__restrict__ total)
+                                                       uint32_t *__restrict__ total, const uint32_t *__restrict__ self_sums)
 {
     __shared__ uint32_t lds[SCAN_NT / 64 + 1];
     uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_IPT;
@@ -254,7 +266,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_tile_scan(const uint32_t *in, uint3
     }
     uint32_t tot;
     uint32_t pre = block_excl_scan<SCAN_NT>(s, &tot, lds);
-    uint32_t off = offsets ? offsets[blockIdx.x] : 0u;
+    uint32_t off = offsets ? offsets[blockIdx.x] : (self_sums ? wave_sum_below(self_sums, blockIdx.x) : 0u);
     uint32_t run = off + pre;
 #pragma unroll
     for (int k = 0; k < SCAN_IPT; k++) {
@@ -283,8 +295,11 @@ __global__ void __launch_bounds__(SCAN_NT) k_shell_reduce(const uint64_t *__rest
     if (threadIdx.x == 0) sums[blockIdx.x] = t;
 }
 
+/* (self: offsets holds the raw tile sums of at most QS_SCAN_SELF_TILES tiles, each tile adds up
+   the ones before it, and the last tile writes totals = {shell voxels, 0}) */
 __global__ void __launch_bounds__(SCAN_NT) k_shell_scan(const uint64_t *__restrict__ shellm, uint64_t n,
-                                                        const uint32_t *__restrict__ offsets, uint32_t *__restrict__ spre)
+                                                        const uint32_t *__restrict__ offsets, uint32_t *__restrict__ spre,
+                                                        int self, uint32_t *__restrict__ totals)
 {
     /* wave w of the block takes words [w * 512, (w + 1) * 512) of the tile, 64 consecutive words
        per load, and scans them in order: lane prefixes per group of 64, carried across groups */
@@ -312,7 +327,13 @@ __global__ void __launch_bounds__(SCAN_NT) k_shell_scan(const uint64_t *__restri
     }
     if (lane == 0) wsum[wid] = run;
     __syncthreads();
-    uint32_t off = offsets[blockIdx.x];
+    uint32_t off = self ? wave_sum_below(offsets, blockIdx.x) : offsets[blockIdx.x];
+    if (self && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        uint32_t t = off;
+        for (int w = 0; w < SCAN_NT / 64; w++) t += wsum[w];
+        totals[0] = t;
+        totals[1] = 0u;
+    }
     for (int w = 0; w < wid; w++) off += wsum[w];
 #pragma unroll
     for (int k = 0; k < SCAN_IPT; k++) {
@@ -384,8 +405,11 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict
                                                       uint64_t n, const uint32_t *__restrict__ sums, uint64_t ntiles,
                                                       uint32_t *__restrict__ vbw, uint32_t *__restrict__ tbw,
                                                       uint32_t *__restrict__ list, uint32_t *__restrict__ avb,
-                                                      uint32_t *__restrict__ atb, uint32_t list_base)
+                                                      uint32_t *__restrict__ atb, uint32_t list_base, int self,
+                                                      uint32_t *__restrict__ totals)
 {
+    /* (self: sums holds the raw tile sums of at most QS_SCAN_SELF_TILES tiles, each tile adds up
+       the ones before it, and the last tile writes totals = {triangles, vertices, active, 0}) */
     /* wave w of the block takes words [w * 512, (w + 1) * 512) of the tile, 64 consecutive words
        per load (as k_shell_scan); the three scans run per group of 64 in order, carried across
        groups, and each group's active words are listed in order by ballot */
@@ -427,7 +451,28 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict
         wsum[2][wid] = ra;
     }
     __syncthreads();
-    uint32_t ov = sums[ntiles + blockIdx.x], ot = sums[blockIdx.x], oa = sums[2 * ntiles + blockIdx.x];
+    uint32_t ov, ot, oa;
+    if (self) {
+        ot = wave_sum_below(sums, blockIdx.x);
+        ov = wave_sum_below(sums + ntiles, blockIdx.x);
+        oa = wave_sum_below(sums + 2 * ntiles, blockIdx.x);
+        if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+            uint32_t tv = ov, tt = ot, ta = oa;
+            for (int w = 0; w < SCAN_NT / 64; w++) {
+                tv += wsum[0][w];
+                tt += wsum[1][w];
+                ta += wsum[2][w];
+            }
+            totals[0] = tt;
+            totals[1] = tv;
+            totals[2] = ta;
+            totals[3] = 0u;
+        }
+    } else {
+        ov = sums[ntiles + blockIdx.x];
+        ot = sums[blockIdx.x];
+        oa = sums[2 * ntiles + blockIdx.x];
+    }
     for (int w = 0; w < wid; w++) {
         ov += wsum[0][w];
         ot += wsum[1][w];
@@ -3564,9 +3609,10 @@ void qs_launch_shell_scan(const uint64_t *shellm, uint64_t nw, uint32_t *sums, u
         return;
     }
     const uint64_t ntiles = qs_scan_tiles(nw);
+    const int self = ntiles <= QS_SCAN_SELF_TILES;
     hipLaunchKernelGGL(k_shell_reduce, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, shellm, nw, sums);
-    hipLaunchKernelGGL(k_scan_planes, dim3(1), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
-    hipLaunchKernelGGL(k_shell_scan, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, shellm, nw, sums, spre);
+    if (!self) hipLaunchKernelGGL(k_scan_planes, dim3(1), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
+    hipLaunchKernelGGL(k_shell_scan, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, shellm, nw, sums, spre, self, totals);
 }
 
 void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *outm, uint64_t *shellm, hipStream_t s)
@@ -3687,10 +3733,11 @@ void qs_launch_mc_scans(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw,
         return;
     }
     const uint64_t ntiles = qs_scan_tiles(nw);
+    const int self = ntiles <= QS_SCAN_SELF_TILES;
     hipLaunchKernelGGL(k_mc_reduce3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles);
-    hipLaunchKernelGGL(k_scan_planes, dim3(3), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
+    if (!self) hipLaunchKernelGGL(k_scan_planes, dim3(3), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
     hipLaunchKernelGGL(k_mc_scan3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles, vbw, tbw, list,
-                       avb, atb, list_base);
+                       avb, atb, list_base, self, totals);
 }
 
 uint64_t qs_scan_tiles(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
@@ -3706,9 +3753,10 @@ void qs_launch_scan1(const uint32_t *in, uint32_t *out, uint64_t n, uint32_t *to
 }
 
 void qs_launch_tile_scan(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *offsets, uint32_t *total,
-                         hipStream_t s)
+                         hipStream_t s, const uint32_t *self_sums)
 {
-    hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)qs_scan_tiles(n)), dim3(SCAN_NT), 0, s, in, out, n, offsets, total);
+    hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)qs_scan_tiles(n)), dim3(SCAN_NT), 0, s, in, out, n, offsets, total,
+                       self_sums);
 }
 
 /* value of a uint32 array at given indices (per-molecule boundaries of scans) */
