@@ -411,7 +411,7 @@ int Engine::build_meta(size_t &at_atoms, size_t &at_p1, size_t &at_mc, size_t &a
         }
         gs[k].mols.push_back(m);
     }
-    /* tile flags of the molecules that the fixed-R pass-2 kernel runs on, in its tile order */
+    /* tile marks of the molecules that the fixed-R pass-2 kernel runs on, in its tile order */
     ntflags = 0;
     for (QsMol &M : mols) M.tile_off = ~0u;
     for (auto &grp : gs)
@@ -478,8 +478,8 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
     QS_TRY(ensure(spre, std::max<uint64_t>(nw_total, 1) * 4));
     QS_TRY(ensure(stot, 8));
     QS_TRY(ensure(scan_tmp, qs_scan_tiles(std::max<uint64_t>(nw_total, 1)) * sizeof(uint32_t)));
-    /* the slab kernel of field pass 1 writes every voxel's default sign words and flags the
-       pass-2 tiles that hold a shell voxel; pass 2 then runs on the flagged tiles only */
+    /* the slab kernel of field pass 1 writes every voxel's default sign words and marks the
+       pass-2 tiles that hold a shell voxel; pass 2 then runs on the marked tiles only */
     const int p1mode = qs_pass1_mode(g_opt.pass1_direct != 0, g_opt.pass1_rows != 0);
     uint8_t *tf = nullptr;
     if (p1mode == 0 && ntflags) {

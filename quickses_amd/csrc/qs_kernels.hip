@@ -1438,7 +1438,7 @@ __global__ void __launch_bounds__(256) k_pass1_slab(QsBatch B, const uint32_t *_
         if (negm) {
             /* the sign words of a voxel without a shell voxel in its pass-2 tile: inside (-dx)
                is negative, outside (the probe radius) is not, no field value is zero; field
-               pass 2 rewrites the tiles that hold a shell voxel, which it finds flagged here */
+               pass 2 rewrites the tiles that hold a shell voxel, which it finds marked here */
             negm[wbase + t] = valid & (mi | mn);
             zerom[wbase + t] = 0ull;
             if (sh && tflag && M.tile_off != ~0u && x >= M.p2_lo && x < M.p2_hi) {
