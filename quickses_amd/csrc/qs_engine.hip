@@ -1081,9 +1081,9 @@ unsigned g_ntri_idx = 0, g_nvert = 0;
 
 /* Host result arrays.  Arrays of PIN_MIN bytes or more are pinned (hipHostMalloc), so the
    device-to-host copies are queued at once and run at the link's speed; smaller ones (and every
-   array of an empty mesh, which must not touch the device) are ordinary memory.  API_freeMesh keeps the result's arrays (the largest set
-   so far) for the next call: pinning, like first-touch page faults in fresh pageable memory, costs
-   more than the copy itself. */
+   array of an empty mesh, which must not touch the device) are ordinary memory.  API_freeMesh
+   keeps the result's arrays (the largest set so far) for the next call: pinning, like first-touch
+   page faults in fresh pageable memory, costs more than the copy itself. */
 struct HostArray {
     void *p = nullptr;
     size_t cap = 0;
