@@ -200,8 +200,9 @@ struct P2Group {
 struct Engine {
     int device = -1;
     hipStream_t stream = nullptr, side = nullptr;
-    hipEvent_t ev_emit = nullptr, ev_side = nullptr;
+    hipEvent_t ev_emit = nullptr, ev_side = nullptr, ev_vert = nullptr;
     bool side_pending = false; /* closest-atom work on `side` not yet joined into `stream` */
+    bool vert_pending = false; /* the vertex kernel on `side` not yet joined into `stream` */
 
     DevBuf in_pos, in_rad, d_off, bb_part, bb_out, d_mols, d_meta;
     DevBuf key, count /* and the scatter cursors after the counts */, start, perm, cse, sx, aux;
@@ -364,7 +365,8 @@ struct Engine {
         side = nullptr;
         if (ev_emit) (void)hipEventDestroy(ev_emit);
         if (ev_side) (void)hipEventDestroy(ev_side);
-        ev_emit = ev_side = nullptr;
+        if (ev_vert) (void)hipEventDestroy(ev_vert);
+        ev_emit = ev_side = ev_vert = nullptr;
         device = -1;
         batch_valid = false;
     }
@@ -381,6 +383,7 @@ struct Engine {
         QS_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         QS_HIP(hipEventCreateWithFlags(&ev_emit, hipEventDisableTiming));
         QS_HIP(hipEventCreateWithFlags(&ev_side, hipEventDisableTiming));
+        QS_HIP(hipEventCreateWithFlags(&ev_vert, hipEventDisableTiming));
         return 0;
     }
 

@@ -88,13 +88,16 @@ void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const ui
    field F holds the shell voxels' values only, at their shell ranks (spre), the others come from
    the outside / shell masks (shellm == nullptr: F holds every voxel in grid order, spre is
    unused).  Triangle corners are vertex ids plus vbase; degenerate triangles are counted per
-   256-triangle block (deg_blk) and in total. */
+   256-triangle block (deg_blk) and in total.  The block starts and the triangle kernel run on s;
+   the vertex kernel on s_vert: when that is another stream it waits there for the block starts
+   (event ev_fork), and ev_vert is recorded on s_vert after it. */
 void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list, uint32_t nact,
                        const uint32_t *avb, uint32_t *vstart, uint32_t nv, const uint64_t *S4, const float *F,
                        const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *vert, uint64_t *vkey,
                        const uint32_t *atb, uint32_t *tstart, uint32_t ntri, const uint64_t *N, const uint64_t *Zm,
                        const uint32_t *vbw, const uint64_t *NT, int *tri, uint32_t vbase, uint32_t *deg_blk,
-                       uint32_t *deg_total, const uint32_t *d_counts, hipStream_t s);
+                       uint32_t *deg_total, const uint32_t *d_counts, hipStream_t s, hipStream_t s_vert = nullptr,
+                       hipEvent_t ev_fork = nullptr, hipEvent_t ev_vert = nullptr);
 
 /* Launches whose item counts live on the device take them from the d_* pointers (when given),
    the host counts being the pools the launch covers: marching-cubes totals are counts[0]

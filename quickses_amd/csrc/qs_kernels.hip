@@ -3637,9 +3637,19 @@ void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *
                        const uint32_t *spre, const uint64_t *outm, const uint64_t *shellm, float *vert, uint64_t *vkey,
                        const uint32_t *atb, uint32_t *tstart, uint32_t ntri, const uint64_t *N, const uint64_t *Zm,
                        const uint32_t *vbw, const uint64_t *NT, int *tri, uint32_t vbase, uint32_t *deg_blk,
-                       uint32_t *deg_total, const uint32_t *d_counts, hipStream_t s)
+                       uint32_t *deg_total, const uint32_t *d_counts, hipStream_t s, hipStream_t s_vert, hipEvent_t ev_fork,
+                       hipEvent_t ev_vert)
 {
-    if (!nact || (!nv && !ntri)) return;
+    if (!s_vert) s_vert = s;
+    if (!nact || (!nv && !ntri)) {
+        /* nothing to emit: s_vert still follows s up to here */
+        if (s_vert != s) {
+            (void)hipEventRecord(ev_fork, s);
+            (void)hipStreamWaitEvent(s_vert, ev_fork, 0);
+            (void)hipEventRecord(ev_vert, s_vert);
+        }
+        return;
+    }
     /* the per-block degenerate counts start at zero (the triangle kernel adds to them), over the
        whole pool of ntri triangles */
     const uint32_t nzero = ntri ? ntri / 256 + 1 : 0u;
@@ -3647,9 +3657,15 @@ void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *
     const QsStarts st = {atb, ntri, d_counts ? d_counts : nullptr, tstart, deg_blk, nzero};
     hipLaunchKernelGGL(k_block_starts, dim3(pgrid(nblk(std::max<uint32_t>(nact, nzero), 256)), 2), dim3(256), 0, s, sv, st, nact,
                        d_counts ? d_counts + 2 : nullptr);
+    /* the vertex kernel on s_vert (after the block starts), beside the triangle kernel on s */
+    if (s_vert != s) {
+        (void)hipEventRecord(ev_fork, s);
+        (void)hipStreamWaitEvent(s_vert, ev_fork, 0);
+    }
     if (nv)
-        hipLaunchKernelGGL(k_mc_vertices, dim3(pgrid(nblk(nv, 256))), dim3(256), 0, s, mols, wpre, wcoarse, list, avb, vstart, nact,
+        hipLaunchKernelGGL(k_mc_vertices, dim3(pgrid(nblk(nv, 256))), dim3(256), 0, s_vert, mols, wpre, wcoarse, list, avb, vstart, nact,
                            nv, S4, F, spre, outm, shellm, vert, vkey, d_counts);
+    if (s_vert != s) (void)hipEventRecord(ev_vert, s_vert);
     if (ntri)
         hipLaunchKernelGGL(k_mc_triangles, dim3(pgrid(nblk(ntri, 256))), dim3(256), 0, s, mols, wpre, wcoarse, list, atb, tstart,
                            nact, ntri, N, Zm, S4, vbw, NT, tri, vbase, deg_blk, deg_total, d_counts);
