@@ -221,3 +221,21 @@ def test_repeated_runs_with_stale_tile_marks(qs, orc):
     for k in range(600):
         got = qs.compute_ses(*a, 0.4, 1) if k % 2 == 0 else qs.compute_ses(*b, 0.5, 1)
         _same_mesh(got, first[k % 2])
+
+
+@pytest.mark.parametrize("seed,n,reso,smooth", [(51, 1500, 0.4, 1), (52, 3000, 0.25, 2), (53, 800, 0.6, 0)])
+def test_side_stream_runs_equal_one_stream(qs, seed, n, reso, smooth):
+    """Without stage timing one molecule's vertex kernel and closest-atom pass run on a second
+    stream beside the triangle kernel and smoothing; with it every kernel runs in order on one
+    stream.  Both give the same mesh, bit for bit, and repeated runs do too."""
+    pos, rad = qs.synth_ball(seed, n)
+    old = qs.get_option("timing")
+    try:
+        qs.set_option("timing", 1)
+        ref = qs.compute_ses(pos, rad, reso, smooth)
+        qs.set_option("timing", 0)
+        print("case side stream seed=%d n=%d reso=%.2f smooth=%d" % (seed, n, reso, smooth), flush=True)
+        for _ in range(3):
+            _same_mesh(qs.compute_ses(pos, rad, reso, smooth), ref)
+    finally:
+        qs.set_option("timing", old)
