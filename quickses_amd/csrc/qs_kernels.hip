@@ -3503,13 +3503,16 @@ int qs_set_tuned_rows(float dx, int rows)
 
 int qs_pass1_slab_rows(QsGrid G, int W)
 {
-    /* 64 rows (or the tuned count for this voxel size, up to 256); both masks in at most 32 KB of LDS;
-       at most 80 neighbour cells of y per slab, so the candidate columns (3 x cells times the y
-       cells + 2) stay within 256 */
+    /* 64 rows (or the tuned count for this voxel size, up to 256); both masks in at most 16 KB of LDS
+       (a tuned count from a workload of small grids would otherwise give a large grid slabs whose
+       LDS limits the blocks per CU: the million-atom shell at 0.5 A, 12 words per row, took 1.22 ms
+       in pass 1 with 256-row slabs, 0.81 ms with the 85 rows of this cap, 0.86 ms with 64); at
+       most 80 neighbour cells of y per slab, so the candidate columns (3 x cells times the y cells
+       + 2) stay within 256 */
     int yb = P1S_DEFROWS;
     for (int i = 0; i < g_rows_n; i++)
         if (std::fabs(g_rows_dx[i] - G.dx) <= 0.01f * G.dx && g_rows_v[i] >= 1) yb = std::min(P1S_MAXROWS, g_rows_v[i]);
-    yb = std::min(yb, std::max(1, 2048 / std::max(W, 1)));
+    yb = std::min(yb, std::max(1, 1024 / std::max(W, 1)));
     yb = std::min(yb, std::max(1, (int)(80.0f * G.dxn / G.dx)));
     return yb;
 }
