@@ -3503,16 +3503,18 @@ int qs_set_tuned_rows(float dx, int rows)
 
 int qs_pass1_slab_rows(QsGrid G, int W)
 {
-    /* 64 rows (or the tuned count for this voxel size, up to 256); both masks in at most 16 KB of LDS
-       (a tuned count from a workload of small grids would otherwise give a large grid slabs whose
-       LDS limits the blocks per CU: the million-atom shell at 0.5 A, 12 words per row, took 1.22 ms
-       in pass 1 with 256-row slabs, 0.81 ms with the 85 rows of this cap, 0.86 ms with 64); at
-       most 80 neighbour cells of y per slab, so the candidate columns (3 x cells times the y cells
-       + 2) stay within 256 */
+    /* 64 rows (or the tuned count for this voxel size, up to 256); both masks in at most 12 KB of LDS
+       (a tuned count from a workload of small grids would otherwise give a large grid slabs that
+       are too tall: pass 1 of the million-atom shell at 0.5 A, 12 words per row, took 1.22 ms with
+       256-row slabs and 0.86 ms with the 64 rows of this cap (0.81 ms with 85), of a 200,000-atom
+       molecule at 0.3 A, 11 words per row, 0.64 ms with 93 rows and 0.48 ms with 69, at 0.5 A, 7
+       words per row, about 0.30 ms with 146 rows and 0.25 ms with 109); at most 80 neighbour
+       cells of y per slab, so the candidate columns (3 x cells times the y cells + 2) stay
+       within 256 */
     int yb = P1S_DEFROWS;
     for (int i = 0; i < g_rows_n; i++)
         if (std::fabs(g_rows_dx[i] - G.dx) <= 0.01f * G.dx && g_rows_v[i] >= 1) yb = std::min(P1S_MAXROWS, g_rows_v[i]);
-    yb = std::min(yb, std::max(1, 1024 / std::max(W, 1)));
+    yb = std::min(yb, std::max(1, 768 / std::max(W, 1)));
     yb = std::min(yb, std::max(1, (int)(80.0f * G.dxn / G.dx)));
     return yb;
 }
