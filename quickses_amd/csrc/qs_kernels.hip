@@ -707,17 +707,32 @@ __global__ void __launch_bounds__(BB_NT) k_bbox_part(const float *__restrict__ p
     if (threadIdx.x < BB_NV) part[((size_t)m * P + p) * BB_NV + threadIdx.x] = s[threadIdx.x][0];
 }
 
-/* one wavefront per molecule: combine its P partial boxes */
+/* one wavefront per molecule: combine its P partial boxes, lane k taking partials k, k + 64, ...,
+   then across the lanes (minima, maxima and a sum of small integer counts: the same box in any
+   order) */
 __global__ void __launch_bounds__(64) k_bbox_final(const float *__restrict__ part, int P, float *__restrict__ out)
 {
     const int m = blockIdx.x, lane = threadIdx.x;
-    if (lane >= BB_NV) return;
-    float acc = part[(size_t)m * P * BB_NV + lane];
-    for (int p = 1; p < P; p++) {
-        float w = part[((size_t)m * P + p) * BB_NV + lane];
-        acc = (lane < 3) ? fminf(acc, w) : (lane < 7 ? fmaxf(acc, w) : acc + w);
+    float v[BB_NV] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0f};
+    for (int p = lane; p < P; p += 64) {
+        float w[BB_NV];
+#pragma unroll
+        for (int c = 0; c < BB_NV; c++) w[c] = part[((size_t)m * P + p) * BB_NV + c];
+        bb_combine(v, w);
     }
-    out[BB_NV * m + lane] = acc;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        float w[BB_NV];
+#pragma unroll
+        for (int c = 0; c < BB_NV; c++) w[c] = __shfl_xor(v[c], d, 64);
+        bb_combine(v, w);
+    }
+    if (lane < BB_NV) {
+        float r = v[0];
+#pragma unroll
+        for (int c = 1; c < BB_NV; c++) r = (lane == c) ? v[c] : r;
+        out[BB_NV * m + lane] = r;
+    }
 }
 
 /* ------------------------------------------------------------------------------------------ */
