@@ -205,7 +205,7 @@ struct Engine {
     bool vert_pending = false; /* the vertex kernel on `side` not yet joined into `stream` */
 
     DevBuf in_pos, in_rad, d_off, bb_part, bb_out, d_mols, d_meta;
-    DevBuf key, count /* and the scatter cursors after the counts */, start, perm, cse, sx, aux;
+    DevBuf key /* cell and rank within the cell per atom */, count, start, perm, cse, sx, aux;
     DevBuf labels, outm, shellm, field, negm, zerom;
     DevBuf s4, vcnt, tcnt, vbw, tbw, alist, totals, bounds;
     DevBuf rback; /* the run's read-back: totals, vertex and kept-triangle offsets */

@@ -239,9 +239,10 @@ int Engine::reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts
     QS_TRY(ensure(spre, nw * 4));
     QS_TRY(ensure(stot, 8));
     QS_TRY(ensure(in_pos, atoms * 12));
-    for (DevBuf *b : {&in_rad, &key, &perm}) QS_TRY(ensure(*b, atoms * 4));
+    for (DevBuf *b : {&in_rad, &perm}) QS_TRY(ensure(*b, atoms * 4));
+    QS_TRY(ensure(key, atoms * 8));
     for (DevBuf *b : {&sx, &aux}) QS_TRY(ensure(*b, atoms * 16));
-    QS_TRY(ensure(count, (cells + 1) * 8));
+    QS_TRY(ensure(count, (cells + 1) * 4));
     QS_TRY(ensure(start, (cells + 1) * 4));
     QS_TRY(ensure(cse, cells * 8));
     for (DevBuf *b : {&vert, &vert2, &vert3}) QS_TRY(ensure(*b, verts * 12));
@@ -939,21 +940,20 @@ int Engine::run_batch(const float *d_pos, const float *d_rad, const std::vector<
     /* ---- atoms into cells (CudaSurf.cu:594-622), global over the batch ---- */
     const uint32_t n = n_atoms_total;
     const uint64_t ncell = ncell_total;
-    QS_TRY(ensure(key, (size_t)std::max<uint32_t>(n, 1) * 4));
-    /* per-cell counts and scatter cursors, one allocation cleared by one fill */
-    QS_TRY(ensure(count, ncell * 8));
+    /* per atom its cell and rank within the cell; per cell its atom count (cleared by a fill) */
+    QS_TRY(ensure(key, (size_t)std::max<uint32_t>(n, 1) * 8));
+    QS_TRY(ensure(count, ncell * 4));
     QS_TRY(ensure(start, (ncell + 1) * 4));
-    uint32_t *cursor = as<uint32_t>(count) + ncell;
     QS_TRY(ensure(perm, (size_t)std::max<uint32_t>(n, 1) * 4));
     QS_TRY(ensure(cse, ncell * 8));
     QS_TRY(ensure(sx, (size_t)std::max<uint32_t>(n, 1) * 16));
     QS_TRY(ensure(aux, (size_t)std::max<uint32_t>(n, 1) * 16));
-    QS_HIP(hipMemsetAsync(count.p, 0, ncell * 8, stream));
-    if (nb_atoms) qs_launch_hash_count(batch(at_atoms, 0, nmol), nb_atoms, d_pos, as<int>(key), as<uint32_t>(count), stream);
+    QS_HIP(hipMemsetAsync(count.p, 0, ncell * 4, stream));
+    if (nb_atoms) qs_launch_hash_count(batch(at_atoms, 0, nmol), nb_atoms, d_pos, as<int2>(key), as<uint32_t>(count), stream);
     /* start[c] = atoms in cells < c (global); start[ncell] = n */
     QS_TRY(scan(as<uint32_t>(count), as<uint32_t>(start), ncell, as<uint32_t>(start) + ncell));
     if (n) {
-        qs_launch_scatter(as<int>(key), (int)n, as<uint32_t>(start), cursor, as<int>(perm), stream);
+        qs_launch_scatter(as<int2>(key), (int)n, as<uint32_t>(start), as<int>(perm), stream);
         qs_launch_cell_finalize(as<uint32_t>(start), as<uint32_t>(count), (int)ncell, as<int>(perm), as<int2>(cse), stream);
         qs_launch_gather_aux(batch(at_atoms, 0, nmol), nb_atoms, as<int>(perm), d_pos, d_rad, as<float4>(sx), as<float4>(aux),
                              stream);

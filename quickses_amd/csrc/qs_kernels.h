@@ -16,8 +16,9 @@ size_t qs_bbox_part_floats(int nmol, int parts);
 /* off: per-molecule atom offsets (nmol + 1); or nullptr for one molecule of atoms [o0, o1) */
 void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, uint32_t o0, uint32_t o1, int nmol, int parts,
                     float *part, float *out, hipStream_t s);
-void qs_launch_hash_count(QsBatch B, uint32_t nblocks, const float *pos, int *key, uint32_t *count, hipStream_t s);
-void qs_launch_scatter(const int *key, int n, const uint32_t *start, uint32_t *cursor, int *perm, hipStream_t s);
+/* key: (cell, rank within the cell) per atom; count: atoms per cell (cleared before) */
+void qs_launch_hash_count(QsBatch B, uint32_t nblocks, const float *pos, int2 *key, uint32_t *count, hipStream_t s);
+void qs_launch_scatter(const int2 *key, int n, const uint32_t *start, int *perm, hipStream_t s);
 void qs_launch_cell_finalize(const uint32_t *start, const uint32_t *count, int ncell, int *perm, int2 *cse, hipStream_t s);
 /* atoms in cell order (sx) with their exact thresholds and neighbour cells (aux) */
 void qs_launch_gather_aux(QsBatch B, uint32_t nblocks, const int *perm, const float *pos, const float *rad, float4 *sx,

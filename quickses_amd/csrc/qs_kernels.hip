@@ -738,8 +738,10 @@ __global__ void __launch_bounds__(64) k_bbox_final(const float *__restrict__ par
 /* ------------------------------------------------------------------------------------------ */
 /* atom hashing and cell ordering                                                              */
 
-/* hashAtoms Kernels.cu:185-216: cell id x-major (global over the batch); count atoms per cell */
-__global__ void __launch_bounds__(256) k_hash_count(QsBatch B, const float *__restrict__ pos, int *__restrict__ key,
+/* hashAtoms Kernels.cu:185-216: cell id x-major (global over the batch); count atoms per cell.
+   key[atom] = (cell, the atom's rank among its cell's atoms in the order the counts were taken):
+   the scatter places it with that rank, and the cell's atoms are put in index order afterwards. */
+__global__ void __launch_bounds__(256) k_hash_count(QsBatch B, const float *__restrict__ pos, int2 *__restrict__ key,
                                                     uint32_t *__restrict__ count)
 {
     uint32_t lb;
@@ -753,18 +755,16 @@ __global__ void __launch_bounds__(256) k_hash_count(QsBatch B, const float *__re
     int cy = iclamp(cell_of(pos[3 * ga + 1], G.oy, G.dxn), 0, G.gn - 1);
     int cz = iclamp(cell_of(pos[3 * ga + 2], G.oz, G.dxn), 0, G.gn - 1);
     int h = (int)M.cell_off + ((G.gn * G.gn * cx) + (G.gn * cy) + cz);
-    key[ga] = h;
-    atomicAdd(&count[h], 1u);
+    key[ga] = make_int2(h, (int)atomicAdd(&count[h], 1u));
 }
 
-__global__ void __launch_bounds__(256) k_scatter(const int *__restrict__ key, int n, const uint32_t *__restrict__ start,
-                                                 uint32_t *__restrict__ cursor, int *__restrict__ perm)
+__global__ void __launch_bounds__(256) k_scatter(const int2 *__restrict__ key, int n, const uint32_t *__restrict__ start,
+                                                 int *__restrict__ perm)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int h = key[i];
-    uint32_t slot = start[h] + atomicAdd(&cursor[h], 1u);
-    perm[slot] = i;
+    const int2 hr = key[i];
+    perm[start[hr.x] + (uint32_t)hr.y] = i;
 }
 
 /* Per cell: order its atoms by input index, so the whole order is (cell, index) -- the order a
@@ -3465,14 +3465,14 @@ void qs_launch_bbox(const float *pos, const float *rad, const uint32_t *off, uin
     hipLaunchKernelGGL(k_bbox_final, dim3(nmol), dim3(64), 0, s, part, parts, out);
 }
 
-void qs_launch_hash_count(QsBatch B, uint32_t nblocks, const float *pos, int *key, uint32_t *count, hipStream_t s)
+void qs_launch_hash_count(QsBatch B, uint32_t nblocks, const float *pos, int2 *key, uint32_t *count, hipStream_t s)
 {
     hipLaunchKernelGGL(k_hash_count, dim3(nblocks), dim3(256), 0, s, B, pos, key, count);
 }
 
-void qs_launch_scatter(const int *key, int n, const uint32_t *start, uint32_t *cursor, int *perm, hipStream_t s)
+void qs_launch_scatter(const int2 *key, int n, const uint32_t *start, int *perm, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, key, n, start, cursor, perm);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, key, n, start, perm);
 }
 
 void qs_launch_cell_finalize(const uint32_t *start, const uint32_t *count, int ncell, int *perm, int2 *cse, hipStream_t s)
