@@ -1856,9 +1856,28 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     const bool vyz = gy < g && gz < g;
     const float dx = G.dx;
 
-    /* the tile's own shell bits (threads 0-127) and outside bits (threads 128-255) first: a tile
-       without a shell voxel (outside the molecule or deep inside it) is finished from them alone,
-       without its halo */
+    /* the outside rows of the tile and its R halo into LDS; fl: some row has an outside voxel */
+    int fl = 0;
+    auto load_rows = [&]() {
+#pragma unroll
+        for (int row = threadIdx.x; row < LX * LY; row += 256) {
+            int lx = row / LY, ly = row - lx * LY;
+            int gx = tx0 - R + lx, gy_ = ty0 - R + ly;
+            uint64_t m = 0ull;
+            /* planes outside [p1_lo, p1_hi) are not stored (and are read by no output voxel) */
+            if (gx >= M.p1_lo && gx < M.p1_hi && gy_ >= 0 && gy_ < g)
+                m = row_bits(outm + ((size_t)gx * g + gy_) * W, W, tz0 - R, LZ);
+            rows[row] = m;
+            fl |= (m != 0ull);
+        }
+    };
+    /* a tile marked by field pass 1 has a shell voxel: its halo rows are loaded together with its
+       own rows, and the first barrier reduces fl */
+    const bool marked = tflag && M.tile_off != ~0u;
+
+    /* the tile's own shell bits (threads 0-127) and outside bits (threads 128-255) first: an
+       unmarked tile without a shell voxel (outside the molecule or deep inside it) is finished
+       from them alone, without its halo */
     static_assert(P2R_TX * P2_TY == 128, "one thread per own row of each mask");
     static_assert(P2_TZ == 32, "a tile's z range is one half of a row word");
     if (threadIdx.x < 8) {
@@ -1881,6 +1900,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
             m = (uint32_t)(wv >> (tz0 & 63));
             if (threadIdx.x < 128) sb = spre[wi] + (uint32_t)__popcll(wv & bits_below(tz0 & 63, 0));
         }
+        if (marked) load_rows();
         if (threadIdx.x < 128) {
             own[t] = m;
             /* shell bits of each own x plane, over its P2_TY rows (consecutive lanes) */
@@ -1891,9 +1911,11 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         } else {
             ownout[t] = m;
         }
-        has_shell = __syncthreads_or(threadIdx.x < 128 && m != 0u);
+        /* marked: the tile has a shell voxel, and the barrier tells whether an outside voxel is
+           within reach instead */
+        has_shell = __syncthreads_or(marked ? fl : (threadIdx.x < 128 && m != 0u));
     }
-    if (!has_shell) {
+    if (!marked && !has_shell) {
         /* no field values to store (only shell voxels hold one; the others are the probe radius
            outside and -dx inside): the sign words are the complemented outside bits */
         const uint32_t zvalid = (tz0 + P2_TZ <= g) ? ~0u : ((1u << (g - tz0)) - 1u);
@@ -1917,18 +1939,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         return sbase[t] + (uint32_t)__popc(own[t] & ((1u << tz_) - 1u));
     };
 
-    int fl = 0;
-#pragma unroll
-    for (int row = threadIdx.x; row < LX * LY; row += 256) {
-        int lx = row / LY, ly = row - lx * LY;
-        int gx = tx0 - R + lx, gy_ = ty0 - R + ly;
-        uint64_t m = 0ull;
-        /* planes outside [p1_lo, p1_hi) are not stored (and are read by no output voxel) */
-        if (gx >= M.p1_lo && gx < M.p1_hi && gy_ >= 0 && gy_ < g)
-            m = row_bits(outm + ((size_t)gx * g + gy_) * W, W, tz0 - R, LZ);
-        rows[row] = m;
-        fl |= (m != 0ull);
-    }
+    if (!marked) load_rows();
     /* every y minimum starts unreachable; only listed columns are computed */
     for (int i = threadIdx.x; i < LX * P2_TY * P2_TZ / 4; i += 256) DYw[i] = 0x01010101u * INF;
     /* a column (lx, z) is read by the x pass only for shell voxels of own planes lx - 2R .. lx */
@@ -1939,8 +1950,10 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
         for (int tx = 0; tx < P2R_TX; tx++)
             if (tx >= lx - 2 * R && tx <= lx) need |= shx[tx];
     }
-    if (!__syncthreads_or(fl)) {
-        /* no outside voxel within reach: every voxel of the tile is -dx */
+    if (!(marked ? has_shell : __syncthreads_or(fl))) {
+        /* no outside voxel within reach: every voxel of the tile is -dx (a marked tile has passed
+           no barrier since its shell ranks were written) */
+        if (marked) __syncthreads();
         for (int tx = 0; tx < P2R_TX; tx++) {
             int gx = tx0 + tx;
             if (gx >= x1 || gx >= g) break;
