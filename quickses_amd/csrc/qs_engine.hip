@@ -509,7 +509,8 @@ int Engine::field_stage(uint32_t nb_p1, size_t at_p1, StageTimer &tm)
         QS_TRY(group_tables(grp));
         qs_launch_pass2(batch(grp.bpre_at, grp.mid_at, (int)grp.mols.size()), grp.nblocks, grp.r, grp.path, as<uint64_t>(outm),
                         as<uint64_t>(shellm), as<uint32_t>(spre), as<float>(field), as<uint32_t>(negm), as<uint32_t>(zerom), as<int>(grp.tab->offs),
-                        as<int>(grp.tab->norm), as<int>(grp.tab->end), grp.tab->flags, tf, tmark, grp.xchunk, stream);
+                        as<int>(grp.tab->norm), as<int>(grp.tab->end), grp.tab->flags,
+                        (tf && grp.path == 0) ? tf + mols[grp.mols[0]].tile_off : nullptr, tmark, grp.xchunk, stream);
         QS_HIP(hipGetLastError());
     }
     field_sparse = true;

@@ -52,7 +52,8 @@ int qs_pass2_path(int r, bool direct);
 /* blocks of one molecule's launch over its planes [p2_lo, p2_hi) */
 uint32_t qs_pass2_blocks(const QsMol &M, int path);
 /* the shell voxels' values go to field at their shell ranks (spre, from qs_launch_shell_scan);
-   tflag (path 0 only, may be NULL): tiles field pass 1 did not mark with tmark are skipped; xchunk
+   tflag (path 0 only, may be NULL): the marks of the launch's tiles in its block order (from the
+   group's first molecule's tile_off on); tiles field pass 1 did not mark with tmark are skipped; xchunk
    (path 0): consecutive tiles each XCD takes in turn */
 void qs_launch_pass2(QsBatch B, uint32_t nblocks, int r, int path, const uint64_t *outm, const uint64_t *shellm,
                      const uint32_t *spre, float *field, uint32_t *negm, uint32_t *zerom, const int *offs, const int *norm_start, const int *enum_end,

@@ -1837,9 +1837,11 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     uint32_t lb;
     /* each XCD takes xchunk consecutive tiles at a time (whole rows of z tiles: the y and z halos
        of a tile are then mostly read by tiles of the same XCD, from its L2) */
-    const QsMol &M = B.mols[batch_mol(B, xcd_group_rt(blockIdx.x, gridDim.x, xchunk), lb)];
-    /* a tile without a shell voxel keeps the sign words field pass 1 wrote for it */
-    if (tflag && M.tile_off != ~0u && tflag[M.tile_off + lb] != tmark) return;
+    const uint32_t bt = xcd_group_rt(blockIdx.x, gridDim.x, xchunk);
+    /* a tile without a shell voxel keeps the sign words field pass 1 wrote for it (the launch's
+       tile marks are in its block order: an unmarked tile leaves after one load) */
+    if (tflag && tflag[bt] != tmark) return;
+    const QsMol &M = B.mols[batch_mol(B, bt, lb)];
     const QsGrid G = M.G;
     const int g = G.g, W = (int)M.W;
     const uint64_t *__restrict__ outm = outm_all + M.word_off;
@@ -1873,7 +1875,7 @@ __global__ void __launch_bounds__(256) k_pass2_r(QsBatch B, const uint64_t *__re
     };
     /* a tile marked by field pass 1 has a shell voxel: its halo rows are loaded together with its
        own rows, and the first barrier reduces fl */
-    const bool marked = tflag && M.tile_off != ~0u;
+    const bool marked = tflag != nullptr;
 
     /* the tile's own shell bits (threads 0-127) and outside bits (threads 128-255) first: an
        unmarked tile without a shell voxel (outside the molecule or deep inside it) is finished
