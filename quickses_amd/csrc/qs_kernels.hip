@@ -2488,10 +2488,6 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
                                                   uint32_t *__restrict__ vcnt_all, uint32_t *__restrict__ tcnt_all,
                                                   uint64_t *__restrict__ NT_all)
 {
-    /* triangles per case, in LDS: the loop over a word's cubes below looks one up per cube */
-    __shared__ uint8_t s_ntri[256];
-    s_ntri[threadIdx.x] = (uint8_t)tri_count((int)threadIdx.x);
-    __syncthreads();
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, xcd_block(blockIdx.x, gridDim.x), lb)];
     const QsGrid G = M.G;
@@ -2561,22 +2557,37 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
         const uint64_t c7 = (n01 >> 1) | (ld_word(N, g, W, x, y + 1, w + 1) << 63);
         const uint64_t all = c0 & c1 & c2 & c3 & c4 & c5 & c6 & c7;
         const uint64_t any = c0 | c1 | c2 | c3 | c4 | c5 | c6 | c7;
-        uint64_t act = any & ~all & wlt;
-        /* triangles per cube as three bit planes (counts < 8) */
-        uint64_t p0 = 0, p1 = 0, p2 = 0;
-        while (act) {
-            int b = __builtin_ctzll(act);
-            act &= act - 1ull;
-            int ci = (int)((c0 >> b) & 1ull) | (int)((c1 >> b) & 1ull) << 1 | (int)((c2 >> b) & 1ull) << 2 |
-                     (int)((c3 >> b) & 1ull) << 3 | (int)((c4 >> b) & 1ull) << 4 | (int)((c5 >> b) & 1ull) << 5 |
-                     (int)((c6 >> b) & 1ull) << 6 | (int)((c7 >> b) & 1ull) << 7;
-            const uint32_t k = s_ntri[ci];
-            if (!k) continue;
-            nt += k;
-            p0 |= (uint64_t)(k & 1u) << b;
-            p1 |= (uint64_t)((k >> 1) & 1u) << b;
-            p2 |= (uint64_t)((k >> 2) & 1u) << b;
-        }
+        const uint64_t act = any & ~all & wlt;
+        /* triangles per cube as three bit planes (counts < 8), for the word's 64 cubes at once.
+           For a cube with negative corners S (neither none nor all), the table's count is
+           |S| - 2 f - 4 h: f is the number of faces whose four corners are in S, and h marks the
+           cases whose two corners outside S are opposite ends of a body diagonal.  (The table
+           gives k - 2 triangles per component of S with k cut edges; the cut edges number
+           3 |S| - 2 e(S) for e(S) edges inside S, and the components |S| - e(S) + f, except for
+           the ring of six corners around a body diagonal: one component, which the table
+           triangulates as two, one triangle at each corner outside S.)  Checked against every
+           case of the table. */
+        const auto fadd = [](uint64_t a, uint64_t b, uint64_t c, uint64_t &carry) {
+            const uint64_t ab = a ^ b;
+            carry = (a & b) | (c & ab);
+            return ab ^ c;
+        };
+        uint64_t a1, b1, e1, t1;
+        const uint64_t a0 = fadd(c0, c1, c2, a1), b0 = fadd(c3, c4, c5, b1);
+        const uint64_t d0 = c6 ^ c7, d1 = c6 & c7;
+        const uint64_t s0 = fadd(a0, b0, d0, e1);    /* |S| bit 0 */
+        const uint64_t t0 = fadd(a1, b1, d1, t1);    /* weight-2 sum of the carries */
+        const uint64_t s1 = t0 ^ e1, s2 = t1 ^ (t0 & e1); /* |S| bits 1 and 2 (bit 3: all set) */
+        const uint64_t fz0 = c0 & c1 & c2 & c3, fz1 = c4 & c5 & c6 & c7, fy0 = c0 & c1 & c4 & c5;
+        const uint64_t fy1 = c2 & c3 & c6 & c7, fx0 = c0 & c3 & c4 & c7, fx1 = c1 & c2 & c5 & c6;
+        uint64_t x1, y1;
+        const uint64_t x0 = fadd(fz0, fz1, fy0, x1), y0 = fadd(fy1, fx0, fx1, y1);
+        const uint64_t f0 = x0 ^ y0, f1 = x1 ^ y1 ^ (x0 & y0); /* f mod 4 (f <= 3 unless all set) */
+        /* |S| = 6 with no full face: the two corners outside S are a body diagonal */
+        const uint64_t h = ~s0 & s1 & s2 & ~(fz0 | fz1 | fy0 | fy1 | fx0 | fx1);
+        /* k = |S| - 2 f - 4 h mod 8, bit by bit */
+        const uint64_t p0 = s0 & act, p1 = (s1 ^ f0) & act, p2 = (s2 ^ f1 ^ (~s1 & f0) ^ h) & act;
+        nt = (uint32_t)(__popcll(p0) + 2 * __popcll(p1) + 4 * __popcll(p2));
         if (nt) {
             uint64_t *__restrict__ NT = NT_all + 3 * (M.word_off + wi);
             NT[0] = p0;
