@@ -2476,12 +2476,6 @@ struct MCView {
     }
 };
 
-__device__ __forceinline__ uint64_t ld_word(const uint64_t *__restrict__ m, int g, int W, int x, int y, int w)
-{
-    if (x < 0 || y < 0 || x >= g || y >= g || w < 0 || w >= W) return 0ull;
-    return m[((size_t)x * g + y) * W + w];
-}
-
 /* countVertexPerCell MarchingCubes.cu:571-658 in word form: slot masks and per-word counts */
 __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__restrict__ N_all,
                                                   const uint64_t *__restrict__ Zm_all, uint64_t *__restrict__ S4_all,
@@ -2509,33 +2503,41 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
     const uint64_t wle = bits_below(lim + 1, z0) & valid; /* z <= lim */
     const uint64_t zm1 = (w == 0) ? ~1ull : ~0ull;         /* z >= 1 */
 
-    const uint64_t n00 = ld_word(N, g, W, x, y, w), n00p = ld_word(N, g, W, x, y, w + 1), n00m = ld_word(N, g, W, x, y, w - 1);
-    const uint64_t z00 = ld_word(Zm, g, W, x, y, w), z00p = ld_word(Zm, g, W, x, y, w + 1);
-    const uint64_t n10 = ld_word(N, g, W, x + 1, y, w), z10 = ld_word(Zm, g, W, x + 1, y, w);
-    const uint64_t n01 = ld_word(N, g, W, x, y + 1, w), z01 = ld_word(Zm, g, W, x, y + 1, w);
+    /* the word and its neighbours one step along x, y and w, loaded together without branches: a
+       neighbour outside the grid reads the word itself and is masked to zero (the point itself
+       is inside the grid) */
+    const size_t sx = (size_t)g * W;
+    const uint64_t *__restrict__ np = N + wi;
+    const uint64_t *__restrict__ zp = Zm + wi;
+    const bool hx = x + 1 < g, hy = y + 1 < g, hw = w + 1 < W;
+    const size_t ox = hx ? sx : 0, oy = hy ? (size_t)W : 0, ow = hw ? 1 : 0;
+    const uint64_t mx = hx ? ~0ull : 0ull, my = hy ? ~0ull : 0ull, mw = hw ? ~0ull : 0ull;
+    const uint64_t n00 = np[0], n00p = np[ow] & mw, n00m = np[w > 0 ? -1 : 0] & (w > 0 ? ~0ull : 0ull);
+    const uint64_t z00 = zp[0], z00p = zp[ow] & mw;
+    const uint64_t n10 = np[ox] & mx, z10 = zp[ox] & mx;
+    const uint64_t n01 = np[oy] & my, z01 = zp[oy] & my;
+    /* the cube's far x, y corners and the next words of its top corners */
+    const uint64_t n11 = np[ox + oy] & mx & my;
+    const uint64_t n10p = np[ox + ow] & mx & mw, n11p = np[ox + oy + ow] & mx & my & mw, n01p = np[oy + ow] & my & mw;
     const uint64_t n00_up = (n00 >> 1) | (n00p << 63); /* bit z = N[z+1] */
     const uint64_t n00_dn = (n00 << 1) | (n00m >> 63); /* bit z = N[z-1] */
     const uint64_t z00_up = (z00 >> 1) | (z00p << 63);
 
-    uint64_t ex = 0, ey = 0, ez = 0, pt = 0;
-    if (x + 1 < g && x < lim && y <= lim) {
-        uint64_t nz = (n00 & z10) | (~n00 & z00);
-        ex = (n00 ^ n10) & ~nz & wle;
-    }
-    if (y + 1 < g && x <= lim && y < lim) {
-        uint64_t nz = (n00 & z01) | (~n00 & z00);
-        ey = (n00 ^ n01) & ~nz & wle;
-    }
-    if (x <= lim && y <= lim) {
-        uint64_t nz = (n00 & z00_up) | (~n00 & z00);
-        ez = (n00 ^ n00_up) & ~nz & wlt;
-    }
+    /* edge masks under condition masks rather than branches (a branch would take its word's
+       load with it and wait on it alone) */
+    const uint64_t kx = (x + 1 < g && x < lim && y <= lim) ? ~0ull : 0ull;
+    const uint64_t ky = (y + 1 < g && x <= lim && y < lim) ? ~0ull : 0ull;
+    const uint64_t kz = (x <= lim && y <= lim) ? ~0ull : 0ull;
+    const uint64_t ex = (n00 ^ n10) & ~((n00 & z10) | (~n00 & z00)) & wle & kx;
+    const uint64_t ey = (n00 ^ n01) & ~((n00 & z01) | (~n00 & z00)) & wle & ky;
+    const uint64_t ez = (n00 ^ n00_up) & ~((n00 & z00_up) | (~n00 & z00)) & wlt & kz;
+    uint64_t pt = 0;
     if (z00) {
         uint64_t nb = 0;
         if (x + 1 < g && x < lim && y <= lim) nb |= n10 & wle;
-        if (x >= 1 && x - 1 < lim && y <= lim) nb |= ld_word(N, g, W, x - 1, y, w) & wle;
+        if (x >= 1 && x - 1 < lim && y <= lim) nb |= *(np - sx) & wle;
         if (y + 1 < g && x <= lim && y < lim) nb |= n01 & wle;
-        if (y >= 1 && x <= lim && y - 1 < lim) nb |= ld_word(N, g, W, x, y - 1, w) & wle;
+        if (y >= 1 && x <= lim && y - 1 < lim) nb |= *(np - W) & wle;
         if (x <= lim && y <= lim) nb |= (n00_up & wlt) | (n00_dn & zm1 & wle);
         pt = z00 & nb;
     }
@@ -2549,12 +2551,11 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
     }
     uint32_t nt = 0;
     if (x < lim && x < M.cube_hi && y < lim) {
-        const uint64_t n11 = ld_word(N, g, W, x + 1, y + 1, w);
         const uint64_t c0 = n00, c1 = n10, c2 = n11, c3 = n01;
         const uint64_t c4 = n00_up;
-        const uint64_t c5 = (n10 >> 1) | (ld_word(N, g, W, x + 1, y, w + 1) << 63);
-        const uint64_t c6 = (n11 >> 1) | (ld_word(N, g, W, x + 1, y + 1, w + 1) << 63);
-        const uint64_t c7 = (n01 >> 1) | (ld_word(N, g, W, x, y + 1, w + 1) << 63);
+        const uint64_t c5 = (n10 >> 1) | (n10p << 63);
+        const uint64_t c6 = (n11 >> 1) | (n11p << 63);
+        const uint64_t c7 = (n01 >> 1) | (n01p << 63);
         const uint64_t all = c0 & c1 & c2 & c3 & c4 & c5 & c6 & c7;
         const uint64_t any = c0 | c1 | c2 | c3 | c4 | c5 | c6 | c7;
         const uint64_t act = any & ~all & wlt;
