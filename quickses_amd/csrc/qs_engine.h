@@ -207,7 +207,7 @@ struct Engine {
     DevBuf in_pos, in_rad, d_off, bb_part, bb_out, d_mols, d_meta;
     DevBuf key /* cell and rank within the cell per atom */, count, start, perm, cse, sx, aux;
     DevBuf labels, outm, shellm, field, negm, zerom;
-    DevBuf s4, vcnt, tcnt, vbw, tbw, alist, totals, bounds;
+    DevBuf s4, wcnt /* per row word: vertex count | triangle count << 16 */, vbw, tbw, alist, totals, bounds;
     DevBuf rback; /* the run's read-back: totals, vertex and kept-triangle offsets */
     DevBuf ntb, avb, atb; /* triangle-count bit planes per word; first vertex / triangle of each active word */
     DevBuf vstart, tstart; /* first active word of each 256-vertex / 256-triangle block */
@@ -327,7 +327,7 @@ struct Engine {
     void forget_buffers()
     {
         for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start,
-                          &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
+                          &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &wcnt,
                           &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &rback, &vert, &vert2, &vert3, &vkey,
                           &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
                           &tpre_d, &tflag, &spre, &stot, &field_dense})
@@ -342,7 +342,7 @@ struct Engine {
     void release_all()
     {
         for (DevBuf *b : {&in_pos, &in_rad, &d_off, &bb_part, &bb_out, &d_mols, &d_meta, &key, &count, &start,
-                          &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &vcnt, &tcnt,
+                          &perm, &cse, &sx, &aux, &labels, &outm, &shellm, &field, &negm, &zerom, &s4, &wcnt,
                           &vbw, &tbw, &alist, &ntb, &avb, &atb, &vstart, &tstart, &normals, &colors_in, &totals, &bounds, &rback, &vert, &vert2, &vert3, &vkey,
                           &tri, &tri_out, &atom, &deg_blk, &keeppos, &tri_loc, &scan_tmp, &inc_cnt, &inc_start, &inc_ent,
                           &tpre_d, &tflag, &spre, &stot, &field_dense})

@@ -232,7 +232,7 @@ int Engine::reserve(uint64_t vox, uint64_t atoms, uint64_t cells, uint64_t verts
     cells += 1;
     verts += 1;
     for (DevBuf *b : {&outm, &shellm, &negm, &zerom}) QS_TRY(ensure(*b, nw * 8));
-    for (DevBuf *b : {&vcnt, &tcnt, &vbw, &tbw, &alist, &avb, &atb}) QS_TRY(ensure(*b, nw * 4));
+    for (DevBuf *b : {&wcnt, &vbw, &tbw, &alist, &avb, &atb}) QS_TRY(ensure(*b, nw * 4));
     QS_TRY(ensure(s4, nw * 32));
     QS_TRY(ensure(ntb, nw * 24));
     QS_TRY(ensure(field, vox * 4));
@@ -540,8 +540,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     const int nmol = (int)mols.size();
     const uint64_t nw = std::max<uint64_t>(nw_total, 1);
     QS_TRY(ensure(s4, nw * 32));
-    QS_TRY(ensure(vcnt, nw * 4));
-    QS_TRY(ensure(tcnt, nw * 4));
+    QS_TRY(ensure(wcnt, nw * 4));
     QS_TRY(ensure(vbw, nw * 4));
     QS_TRY(ensure(tbw, nw * 4));
     QS_TRY(ensure(alist, nw * 4));
@@ -551,10 +550,10 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_TRY(ensure(totals, 16));
     if (nb_mc)
         qs_launch_mc_count(batch(at_mc, 0, nmol), nb_mc, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
-                           as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint64_t>(ntb), stream);
+                           as<uint32_t>(wcnt), as<uint64_t>(ntb), stream);
     if (scan_nw) {
         QS_TRY(ensure(scan_tmp, 3 * qs_scan_tiles(scan_nw) * sizeof(uint32_t)));
-        qs_launch_mc_scans(as<uint32_t>(vcnt) + scan_w0, as<uint32_t>(tcnt) + scan_w0, scan_nw, as<uint32_t>(scan_tmp),
+        qs_launch_mc_scans(as<uint32_t>(wcnt) + scan_w0, scan_nw, as<uint32_t>(scan_tmp),
                            as<uint32_t>(vbw) + scan_w0, as<uint32_t>(tbw) + scan_w0, as<uint32_t>(alist), as<uint32_t>(avb),
                            as<uint32_t>(atb), as<uint32_t>(totals), (uint32_t)scan_w0, stream);
         QS_HIP(hipGetLastError());
@@ -666,12 +665,12 @@ static int grow_keep(DevBuf &b, size_t bytes, size_t keep, hipStream_t s)
 }
 
 /* device bytes of the grid arrays per x plane: per row word the outside, shell, sign and zero
-   masks (32 B), slot masks (32 B), triangle-count planes (24 B) and seven counts and scans
-   (28 B); per voxel the field (4 B) */
+   masks (32 B), slot masks (32 B), triangle-count planes (24 B) and six counts and scans
+   (24 B); per voxel the field (4 B) */
 static uint64_t plane_bytes(const QsMol &M)
 {
     const uint64_t g = (uint64_t)M.G.g;
-    return g * M.W * 116 + g * g * 4;
+    return g * M.W * 112 + g * g * 4;
 }
 
 /* Planes each slab owns (CudaSurf.cu:626-636 sizes its slices from the SLICE setting instead),
@@ -711,14 +710,14 @@ int Engine::slab_mesh(size_t at_mc, uint32_t nb_mc, size_t at_wpre, uint64_t own
 {
     const uint64_t nw = std::max<uint64_t>(nw_total, 1);
     QS_TRY(ensure(s4, nw * 32));
-    for (DevBuf *b : {&vcnt, &tcnt, &vbw, &tbw, &alist, &avb, &atb}) QS_TRY(ensure(*b, nw * 4));
+    for (DevBuf *b : {&wcnt, &vbw, &tbw, &alist, &avb, &atb}) QS_TRY(ensure(*b, nw * 4));
     QS_TRY(ensure(ntb, nw * 24));
     QS_TRY(ensure(totals, 16));
     if (nb_mc)
         qs_launch_mc_count(batch(at_mc, 0, 1), nb_mc, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
-                           as<uint32_t>(vcnt), as<uint32_t>(tcnt), as<uint64_t>(ntb), stream);
+                           as<uint32_t>(wcnt), as<uint64_t>(ntb), stream);
     QS_TRY(ensure(scan_tmp, 3 * qs_scan_tiles(std::max<uint64_t>(scan_nw, 1)) * sizeof(uint32_t)));
-    qs_launch_mc_scans(as<uint32_t>(vcnt) + scan_w0, as<uint32_t>(tcnt) + scan_w0, scan_nw, as<uint32_t>(scan_tmp),
+    qs_launch_mc_scans(as<uint32_t>(wcnt) + scan_w0, scan_nw, as<uint32_t>(scan_tmp),
                        as<uint32_t>(vbw) + scan_w0, as<uint32_t>(tbw) + scan_w0, as<uint32_t>(alist), as<uint32_t>(avb),
                        as<uint32_t>(atb), as<uint32_t>(totals), (uint32_t)scan_w0, stream);
     QS_HIP(hipGetLastError());

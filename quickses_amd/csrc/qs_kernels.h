@@ -82,8 +82,9 @@ void qs_launch_labels_to_masks(const uint8_t *labels, int g, int W, uint64_t *ou
 uint32_t qs_mc_count_blocks(const QsMol &M);
 /* per row word: slot masks (words with vertices only), vertex and triangle counts, triangle-count
    bit planes (words with triangles only) */
-void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
-                        uint32_t *tcnt, uint64_t *NT, hipStream_t s);
+/* cnt: per row word its vertex count | triangle count << 16 */
+void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *cnt,
+                        uint64_t *NT, hipStream_t s);
 /* one thread per vertex / per triangle over the active words (list, with their first vertex ids
    avb and first triangle ids atb), after one launch that gives vstart / tstart each 256-item
    block's first active word (nv / 256 + 1 and ntri / 256 + 1 entries) and clears deg_blk.  The
@@ -172,10 +173,10 @@ void qs_launch_tile_reduce(const uint32_t *in, uint64_t n, uint32_t *sums, hipSt
 void qs_launch_tile_scan(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *offsets, uint32_t *total,
                          hipStream_t s, const uint32_t *self_sums = nullptr);
 void qs_launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
-/* marching-cubes word scans: vbw, tbw, the active-word list and totals {triangles, vertices,
-   active words}; sums holds 3 * qs_scan_tiles(nw) entries; list entries are list_base + the
-   word's index in vcnt */
-void qs_launch_mc_scans(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
+/* marching-cubes word scans over the packed counts of qs_launch_mc_count: vbw, tbw, the
+   active-word list and totals {triangles, vertices, active words}; sums holds
+   3 * qs_scan_tiles(nw) entries; list entries are list_base + the word's index in cnt */
+void qs_launch_mc_scans(const uint32_t *cnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
                         uint32_t *tbw, uint32_t *list, uint32_t *avb, uint32_t *atb, uint32_t *totals, uint32_t list_base,
                         hipStream_t s);
 void qs_launch_gather_u32(const uint32_t *src, const uint64_t *idx, int n, uint32_t *out, hipStream_t s);

@@ -345,8 +345,8 @@ __global__ void __launch_bounds__(SCAN_NT) k_shell_scan(const uint64_t *__restri
 /* Marching-cubes word scans in one pass: per row word its vertex count, triangle count and
    "active" flag (either nonzero) are scanned together.  Plane 0 = triangles, 1 = vertices,
    2 = active words; tile sums are stored plane-major (3 x ntiles). */
-__global__ void __launch_bounds__(SCAN_NT) k_mc_reduce3(const uint32_t *__restrict__ vcnt, const uint32_t *__restrict__ tcnt,
-                                                        uint64_t n, uint32_t *__restrict__ sums, uint64_t ntiles)
+__global__ void __launch_bounds__(SCAN_NT) k_mc_reduce3(const uint32_t *__restrict__ cnt, uint64_t n, uint32_t *__restrict__ sums,
+                                                        uint64_t ntiles)
 {
     __shared__ uint32_t lds[SCAN_NT / 64 + 1];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_reduce3(const uint32_t *__restri
     for (int k = 0; k < SCAN_IPT; k++) {
         const uint64_t i = base + (uint64_t)k * SCAN_NT + threadIdx.x;
         if (i < n) {
-            const uint32_t v = vcnt[i], t = tcnt[i];
+            const uint32_t c = cnt[i], v = c & 0xFFFFu, t = c >> 16;
             sv += v;
             st += t;
             sa += (v | t) ? 1u : 0u;
@@ -401,8 +401,7 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_planes(uint32_t *__restrict__ 
 }
 
 /* per word: first vertex (vbw) and first triangle (tbw); active words listed in order */
-__global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict__ vcnt, const uint32_t *__restrict__ tcnt,
-                                                      uint64_t n, const uint32_t *__restrict__ sums, uint64_t ntiles,
+__global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict__ cnt, uint64_t n, const uint32_t *__restrict__ sums, uint64_t ntiles,
                                                       uint32_t *__restrict__ vbw, uint32_t *__restrict__ tbw,
                                                       uint32_t *__restrict__ list, uint32_t *__restrict__ avb,
                                                       uint32_t *__restrict__ atb, uint32_t list_base, int self,
@@ -422,8 +421,9 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict
 #pragma unroll
     for (int k = 0; k < SCAN_IPT; k++) {
         const uint64_t i = base + 64 * k;
-        v[k] = (i < n) ? vcnt[i] : 0u;
-        t[k] = (i < n) ? tcnt[i] : 0u;
+        const uint32_t c = (i < n) ? cnt[i] : 0u;
+        v[k] = c & 0xFFFFu;
+        t[k] = c >> 16;
     }
     uint32_t rv = 0, rt = 0, ra = 0; /* the wave's earlier groups */
 #pragma unroll
@@ -588,8 +588,7 @@ __global__ void __launch_bounds__(SCAN1_NT) k_shell_scan1(const uint64_t *__rest
 
 /* marching-cubes word scans (as k_mc_reduce3, k_scan_planes, k_mc_scan3): totals = {triangles,
    vertices, active words, 0} */
-__global__ void __launch_bounds__(SCAN1_NT) k_mc_scan1(const uint32_t *__restrict__ vcnt, const uint32_t *__restrict__ tcnt,
-                                                       uint64_t n, uint32_t *__restrict__ vbw, uint32_t *__restrict__ tbw,
+__global__ void __launch_bounds__(SCAN1_NT) k_mc_scan1(const uint32_t *__restrict__ cnt, uint64_t n, uint32_t *__restrict__ vbw, uint32_t *__restrict__ tbw,
                                                        uint32_t *__restrict__ list, uint32_t *__restrict__ avb,
                                                        uint32_t *__restrict__ atb, uint32_t list_base,
                                                        uint32_t *__restrict__ totals)
@@ -606,8 +605,9 @@ __global__ void __launch_bounds__(SCAN1_NT) k_mc_scan1(const uint32_t *__restric
 #pragma unroll
         for (int k = 0; k < SCAN_IPT; k++) {
             const uint64_t i = base + 64 * k;
-            v[k] = (i < n) ? vcnt[i] : 0u;
-            t[k] = (i < n) ? tcnt[i] : 0u;
+            const uint32_t c = (i < n) ? cnt[i] : 0u;
+            v[k] = c & 0xFFFFu;
+            t[k] = c >> 16;
         }
 #pragma unroll
         for (int k = 0; k < SCAN_IPT; k++) {
@@ -2479,8 +2479,7 @@ struct MCView {
 /* countVertexPerCell MarchingCubes.cu:571-658 in word form: slot masks and per-word counts */
 __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__restrict__ N_all,
                                                   const uint64_t *__restrict__ Zm_all, uint64_t *__restrict__ S4_all,
-                                                  uint32_t *__restrict__ vcnt_all, uint32_t *__restrict__ tcnt_all,
-                                                  uint64_t *__restrict__ NT_all)
+                                                  uint32_t *__restrict__ cnt_all, uint64_t *__restrict__ NT_all)
 {
     uint32_t lb;
     const QsMol &M = B.mols[batch_mol(B, xcd_block(blockIdx.x, gridDim.x), lb)];
@@ -2489,8 +2488,7 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
     const uint64_t *__restrict__ N = N_all + M.word_off;
     const uint64_t *__restrict__ Zm = Zm_all + M.word_off;
     uint64_t *__restrict__ S4 = S4_all + 4 * M.word_off;
-    uint32_t *__restrict__ vcnt = vcnt_all + M.word_off;
-    uint32_t *__restrict__ tcnt = tcnt_all + M.word_off;
+    uint32_t *__restrict__ cnt = cnt_all + M.word_off;
     const uint32_t nbx = ((uint32_t)g * W + 255) / 256;
     const int idx = (int)(lb % nbx) * 256 + threadIdx.x;
     const int x = M.mc_lo + (int)(lb / nbx);
@@ -2596,8 +2594,8 @@ __global__ void __launch_bounds__(256) k_mc_count(QsBatch B, const uint64_t *__r
             NT[2] = p2;
         }
     }
-    vcnt[wi] = nv;
-    tcnt[wi] = nt;
+    /* the word's vertex count (at most 4 x 64) and triangle count (at most 5 x 64) in one word */
+    cnt[wi] = nv | (nt << 16);
 }
 
 /* row word wi of a grid -> (x, y, word) */
@@ -3673,10 +3671,10 @@ uint32_t qs_mc_count_blocks(const QsMol &M)
     return nblk((uint64_t)M.G.g * M.W, 256) * (uint32_t)std::max(M.mc_hi - M.mc_lo, 0);
 }
 
-void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *vcnt,
-                        uint32_t *tcnt, uint64_t *NT, hipStream_t s)
+void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const uint64_t *Zm, uint64_t *S4, uint32_t *cnt,
+                        uint64_t *NT, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(256), 0, s, B, N, Zm, S4, vcnt, tcnt, NT);
+    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(256), 0, s, B, N, Zm, S4, cnt, NT);
 }
 
 void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *wcoarse, const uint32_t *list, uint32_t nact,
@@ -3785,21 +3783,21 @@ void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint6
 }
 
 /* scan helpers */
-void qs_launch_mc_scans(const uint32_t *vcnt, const uint32_t *tcnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
+void qs_launch_mc_scans(const uint32_t *cnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
                         uint32_t *tbw, uint32_t *list, uint32_t *avb, uint32_t *atb, uint32_t *totals, uint32_t list_base,
                         hipStream_t s)
 {
     if (!nw) return;
     if (nw <= QS_SCAN1_MAX) {
-        hipLaunchKernelGGL(k_mc_scan1, dim3(1), dim3(SCAN1_NT), 0, s, vcnt, tcnt, nw, vbw, tbw, list, avb, atb, list_base,
+        hipLaunchKernelGGL(k_mc_scan1, dim3(1), dim3(SCAN1_NT), 0, s, cnt, nw, vbw, tbw, list, avb, atb, list_base,
                            totals);
         return;
     }
     const uint64_t ntiles = qs_scan_tiles(nw);
     const int self = ntiles <= QS_SCAN_SELF_TILES;
-    hipLaunchKernelGGL(k_mc_reduce3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles);
+    hipLaunchKernelGGL(k_mc_reduce3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, cnt, nw, sums, ntiles);
     if (!self) hipLaunchKernelGGL(k_scan_planes, dim3(3), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
-    hipLaunchKernelGGL(k_mc_scan3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, vcnt, tcnt, nw, sums, ntiles, vbw, tbw, list,
+    hipLaunchKernelGGL(k_mc_scan3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, cnt, nw, sums, ntiles, vbw, tbw, list,
                        avb, atb, list_base, self, totals);
 }
 
