@@ -235,6 +235,63 @@ def test_case_table_properties():
     assert max(len(r) for r in rows) == 15
 
 
+CUBE_FACES = [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (3, 2, 6, 7), (0, 3, 7, 4), (1, 2, 6, 5)]
+
+
+def test_triangle_count_closed_form():
+    """The cube-count kernel takes a case's triangle count as |S| - 2 f - 4 h (S the negative
+    corners, f the faces with four corners in S, h: the two other corners are a body diagonal),
+    evaluated on bit planes that hold 64 cubes each.  Here the planes hold all 256 cases at once
+    (bit c = case c) and the same adder and subtraction logic must give the table's count for
+    every case but 0 and 255; without the h term exactly the four hexagon cases fail."""
+    rows = _table()
+    count = [len(r) // 3 for r in rows]
+    full = (1 << 256) - 1
+    plane = [sum(1 << c for c in range(256) if (c >> i) & 1) for i in range(8)]
+
+    def fadd(a, b, c):
+        ab = a ^ b
+        return ab ^ c, (a & b) | (c & ab)
+
+    a0, a1 = fadd(plane[0], plane[1], plane[2])
+    b0, b1 = fadd(plane[3], plane[4], plane[5])
+    d0, d1 = plane[6] ^ plane[7], plane[6] & plane[7]
+    s0, e1 = fadd(a0, b0, d0)
+    t0, t1 = fadd(a1, b1, d1)
+    s1, s2 = t0 ^ e1, t1 ^ (t0 & e1)
+    faces = []
+    for fc in CUBE_FACES:
+        m = full
+        for i in fc:
+            m &= plane[i]
+        faces.append(m)
+    u0, u1 = fadd(faces[0], faces[1], faces[2])
+    v0, v1 = fadd(faces[3], faces[4], faces[5])
+    f0, f1 = u0 ^ v0, u1 ^ v1 ^ (u0 & v0)
+    anyface = faces[0] | faces[1] | faces[2] | faces[3] | faces[4] | faces[5]
+    h = ~s0 & s1 & s2 & ~anyface & full
+    k0, k1 = s0, s1 ^ f0
+    k2 = (s2 ^ f1 ^ (~s1 & f0 & full) ^ h) & full
+    k2_without_h = (s2 ^ f1 ^ (~s1 & f0 & full)) & full
+
+    def at(p, c):
+        return (p >> c) & 1
+
+    wrong_without_h = []
+    for c in range(1, 255):
+        assert at(k0, c) + 2 * at(k1, c) + 4 * at(k2, c) == count[c], c
+        if at(k0, c) + 2 * at(k1, c) + 4 * at(k2_without_h, c) != count[c]:
+            wrong_without_h.append(c)
+        # the same count from the corner sets directly
+        S = [i for i in range(8) if (c >> i) & 1]
+        f = sum(all(i in S for i in fc) for fc in CUBE_FACES)
+        assert len(S) - 2 * f - 4 * (len(S) == 6 and f == 0) == count[c], c
+    assert len(wrong_without_h) == 4
+    for c in wrong_without_h:
+        off = [i for i in range(8) if not (c >> i) & 1]
+        assert sorted(off) in ([0, 6], [1, 7], [2, 4], [3, 5]), c
+
+
 def _mesh_stats(V, T):
     V = V.astype(np.float64)
     a, b, c = V[T[:, 0]], V[T[:, 1]], V[T[:, 2]]
