@@ -2401,11 +2401,6 @@ __global__ void __launch_bounds__(256) k_labels_to_masks(const uint8_t *__restri
  * the (point, slot) order.  Emission and smoothing run only on words that own something.
  */
 
-__device__ __forceinline__ float fat(const float *__restrict__ f, int g, int x, int y, int z)
-{
-    return f[((size_t)x * g + y) * g + z];
-}
-
 struct MCView {
     const uint64_t *__restrict__ N;
     const uint64_t *__restrict__ Zm;
