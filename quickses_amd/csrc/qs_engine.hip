@@ -548,24 +548,38 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_TRY(ensure(atb, nw * 4));
     QS_TRY(ensure(ntb, nw * 24));
     QS_TRY(ensure(totals, 16));
+    /* Vertex and triangle pools.  The totals stay on the device: the emission, closest-atom,
+       smoothing and filter kernels read them there and run as persistent blocks over the pools,
+       so nothing waits for the host until the run's results are read back at its end, which is
+       also where a pool that was too small shows (the run is then repeated with larger pools).
+       The pools are planned before the word scans, which write the emission launches' block
+       starts for them. */
+    plan_pools();
+    QS_TRY(ensure(vstart, ((size_t)pool_v / 256 + 1) * 4));
+    QS_TRY(ensure(tstart, ((size_t)pool_t / 256 + 1) * 4));
+    /* degenerate triangles per 256-triangle block and in total (totals[3]), counted by the
+       triangle kernel (the block counts zeroed by the word scans, totals[3] by the scans) */
+    QS_TRY(ensure(deg_blk, ((size_t)pool_t / 256 + 1) * 4));
     if (nb_mc)
         qs_launch_mc_count(batch(at_mc, 0, nmol), nb_mc, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint64_t>(s4),
                            as<uint32_t>(wcnt), as<uint64_t>(ntb), stream);
     if (scan_nw) {
         QS_TRY(ensure(scan_tmp, 3 * qs_scan_tiles(scan_nw) * sizeof(uint32_t)));
+        QsBlockStarts bs;
+        bs.vstart = as<uint32_t>(vstart);
+        bs.nv = pool_v / 256 + 1;
+        bs.tstart = as<uint32_t>(tstart);
+        bs.nt = pool_t / 256 + 1;
+        bs.zero = as<uint32_t>(deg_blk);
+        bs.nzero = pool_t / 256 + 1;
         qs_launch_mc_scans(as<uint32_t>(wcnt) + scan_w0, scan_nw, as<uint32_t>(scan_tmp),
                            as<uint32_t>(vbw) + scan_w0, as<uint32_t>(tbw) + scan_w0, as<uint32_t>(alist), as<uint32_t>(avb),
-                           as<uint32_t>(atb), as<uint32_t>(totals), (uint32_t)scan_w0, stream);
+                           as<uint32_t>(atb), as<uint32_t>(totals), (uint32_t)scan_w0, stream, bs);
         QS_HIP(hipGetLastError());
     } else {
         QS_HIP(hipMemsetAsync(totals.p, 0, 16, stream));
     }
     tm.mark("mc_count");
-    /* Vertex and triangle pools.  The totals stay on the device: the emission, closest-atom,
-       smoothing and filter kernels read them there and run as persistent blocks over the pools,
-       so nothing waits for the host until the run's results are read back at its end, which is
-       also where a pool that was too small shows (the run is then repeated with larger pools). */
-    plan_pools();
     /* per-molecule first vertex and triangle (the scans at each molecule's first word, from the
        word prefix) and the vertex coarse index, computed on the device */
     const uint64_t *wpre = (const uint64_t *)(as<uint32_t>(d_meta) + at_wpre);
@@ -580,11 +594,6 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
     QS_TRY(ensure(vert, (size_t)pool_v * 12));
     QS_TRY(ensure(vkey, (size_t)pool_v * 8));
     QS_TRY(ensure(tri, (size_t)pool_t * 12));
-    QS_TRY(ensure(vstart, ((size_t)pool_v / 256 + 1) * 4));
-    QS_TRY(ensure(tstart, ((size_t)pool_t / 256 + 1) * 4));
-    /* degenerate triangles per 256-triangle block and in total (totals[3]), counted by the
-       triangle kernel (the block counts zeroed by its block-start kernel, totals[3] by the scans) */
-    QS_TRY(ensure(deg_blk, ((size_t)pool_t / 256 + 1) * 4));
     /* every scanned word may be active */
     const uint32_t nact_max = (uint32_t)std::max<uint64_t>(scan_nw, 1);
     const uint32_t *counts = as<uint32_t>(totals);
@@ -600,7 +609,7 @@ int Engine::mesh_stage(size_t at_mc, uint32_t nb_mc, size_t at_wpre, StageTimer 
                       field_sparse ? as<uint64_t>(shellm) : nullptr, as<float>(vert), as<uint64_t>(vkey), as<uint32_t>(atb),
                       as<uint32_t>(tstart), pool_t, as<uint64_t>(negm), as<uint64_t>(zerom), as<uint32_t>(vbw),
                       as<uint64_t>(ntb), as<int>(tri), 0u, as<uint32_t>(deg_blk), as<uint32_t>(totals) + 3, counts, stream,
-                      fork_vertices ? side : stream, ev_emit, ev_vert);
+                      fork_vertices ? side : stream, ev_emit, ev_vert, scan_nw != 0);
     QS_HIP(hipGetLastError());
     vert_pending = fork_vertices;
     tm.mark("mc_emit");

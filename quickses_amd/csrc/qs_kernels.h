@@ -87,7 +87,8 @@ void qs_launch_mc_count(QsBatch B, uint32_t nblocks, const uint64_t *N, const ui
                         uint64_t *NT, hipStream_t s);
 /* one thread per vertex / per triangle over the active words (list, with their first vertex ids
    avb and first triangle ids atb), after one launch that gives vstart / tstart each 256-item
-   block's first active word (nv / 256 + 1 and ntri / 256 + 1 entries) and clears deg_blk.  The
+   block's first active word (nv / 256 + 1 and ntri / 256 + 1 entries) and clears deg_blk, unless
+   starts_ready says that the word scans did (QsBlockStarts).  The
    field F holds the shell voxels' values only, at their shell ranks (spre), the others come from
    the outside / shell masks (shellm == nullptr: F holds every voxel in grid order, spre is
    unused).  Triangle corners are vertex ids plus vbase; degenerate triangles are counted per
@@ -100,7 +101,7 @@ void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *
                        const uint32_t *atb, uint32_t *tstart, uint32_t ntri, const uint64_t *N, const uint64_t *Zm,
                        const uint32_t *vbw, const uint64_t *NT, int *tri, uint32_t vbase, uint32_t *deg_blk,
                        uint32_t *deg_total, const uint32_t *d_counts, hipStream_t s, hipStream_t s_vert = nullptr,
-                       hipEvent_t ev_fork = nullptr, hipEvent_t ev_vert = nullptr);
+                       hipEvent_t ev_fork = nullptr, hipEvent_t ev_vert = nullptr, bool starts_ready = false);
 
 /* Launches whose item counts live on the device take them from the d_* pointers (when given),
    the host counts being the pools the launch covers: marching-cubes totals are counts[0]
@@ -173,12 +174,26 @@ void qs_launch_tile_reduce(const uint32_t *in, uint64_t n, uint32_t *sums, hipSt
 void qs_launch_tile_scan(const uint32_t *in, uint32_t *out, uint64_t n, const uint32_t *offsets, uint32_t *total,
                          hipStream_t s, const uint32_t *self_sums = nullptr);
 void qs_launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s);
+/* The emission launches' block starts, written by the word scans when the pools are known
+   before them: vstart / tstart hold the first active word of each 256-vertex / 256-triangle block
+   (nv / nt entries), and the nzero counters at zero (the triangle kernel's degenerate counts) are
+   cleared.  A null vstart writes none of them (qs_launch_mc_emit then computes them). */
+struct QsBlockStarts {
+    uint32_t *vstart = nullptr;
+    uint32_t nv = 0;
+    uint32_t *tstart = nullptr;
+    uint32_t nt = 0;
+    uint32_t *zero = nullptr;
+    uint32_t nzero = 0;
+};
+
 /* marching-cubes word scans over the packed counts of qs_launch_mc_count: vbw, tbw, the
    active-word list and totals {triangles, vertices, active words}; sums holds
-   3 * qs_scan_tiles(nw) entries; list entries are list_base + the word's index in cnt */
+   3 * qs_scan_tiles(nw) entries; list entries are list_base + the word's index in cnt; and the
+   block starts of bs */
 void qs_launch_mc_scans(const uint32_t *cnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
                         uint32_t *tbw, uint32_t *list, uint32_t *avb, uint32_t *atb, uint32_t *totals, uint32_t list_base,
-                        hipStream_t s);
+                        hipStream_t s, QsBlockStarts bs = QsBlockStarts{});
 void qs_launch_gather_u32(const uint32_t *src, const uint64_t *idx, int n, uint32_t *out, hipStream_t s);
 
 #endif

@@ -400,13 +400,26 @@ __global__ void __launch_bounds__(SCAN_NT) k_scan_planes(uint32_t *__restrict__ 
     if (blockIdx.x == 0 && threadIdx.x == 0) totals[gridDim.x] = 0u;
 }
 
-/* per word: first vertex (vbw) and first triangle (tbw); active words listed in order */
+/* An active word with items [first, first + count) is where every 256-item block whose first
+   item lies in that range starts: it writes those blocks' start (its index in the active list),
+   below the start array's capacity cap. */
+__device__ __forceinline__ void mark_block_starts(uint32_t *__restrict__ start, uint32_t cap, uint32_t first, uint32_t count,
+                                                  uint32_t r)
+{
+    for (uint32_t b = (first + 255u) >> 8; b < cap && (b << 8) < first + count; b++) start[b] = r;
+}
+
+/* per word: first vertex (vbw) and first triangle (tbw); active words listed in order; with
+   bs.vstart, the first active word of every 256-vertex and 256-triangle block, and the triangle
+   blocks' counters cleared */
 __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict__ cnt, uint64_t n, const uint32_t *__restrict__ sums, uint64_t ntiles,
                                                       uint32_t *__restrict__ vbw, uint32_t *__restrict__ tbw,
                                                       uint32_t *__restrict__ list, uint32_t *__restrict__ avb,
                                                       uint32_t *__restrict__ atb, uint32_t list_base, int self,
-                                                      uint32_t *__restrict__ totals)
+                                                      uint32_t *__restrict__ totals, QsBlockStarts bs)
 {
+    if (bs.zero)
+        for (uint32_t i = blockIdx.x * SCAN_NT + threadIdx.x; i < bs.nzero; i += gridDim.x * SCAN_NT) bs.zero[i] = 0u;
     /* (self: sums holds the raw tile sums of at most QS_SCAN_SELF_TILES tiles, each tile adds up
        the ones before it, and the last tile writes totals = {triangles, vertices, active, 0}) */
     /* wave w of the block takes words [w * 512, (w + 1) * 512) of the tile, 64 consecutive words
@@ -489,6 +502,10 @@ __global__ void __launch_bounds__(SCAN_NT) k_mc_scan3(const uint32_t *__restrict
                 avb[r] = ov + pv[k];
                 atb[r] = ot + pt[k];
                 list[r] = list_base + (uint32_t)i;
+                if (bs.vstart) {
+                    mark_block_starts(bs.vstart, bs.nv, ov + pv[k], v[k], r);
+                    mark_block_starts(bs.tstart, bs.nt, ot + pt[k], t[k], r);
+                }
             }
         }
     }
@@ -591,15 +608,17 @@ __global__ void __launch_bounds__(SCAN1_NT) k_shell_scan1(const uint64_t *__rest
 __global__ void __launch_bounds__(SCAN1_NT) k_mc_scan1(const uint32_t *__restrict__ cnt, uint64_t n, uint32_t *__restrict__ vbw, uint32_t *__restrict__ tbw,
                                                        uint32_t *__restrict__ list, uint32_t *__restrict__ avb,
                                                        uint32_t *__restrict__ atb, uint32_t list_base,
-                                                       uint32_t *__restrict__ totals)
+                                                       uint32_t *__restrict__ totals, QsBlockStarts bs)
 {
     __shared__ uint32_t wsum[3][SCAN1_W];
+    if (bs.zero)
+        for (uint32_t i = threadIdx.x; i < bs.nzero; i += SCAN1_NT) bs.zero[i] = 0u;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint64_t lbelow = (1ull << lane) - 1ull;
     uint32_t cv = 0, ct = 0, ca = 0;
     for (uint64_t c0 = 0; c0 < n; c0 += SCAN1_NT * SCAN_IPT) {
         const uint64_t base = c0 + (uint64_t)wid * (64 * SCAN_IPT) + lane;
-        uint32_t v[SCAN_IPT], t[SCAN_IPT], pa[SCAN_IPT];
+        uint32_t v[SCAN_IPT], t[SCAN_IPT], pa[SCAN_IPT], raw[SCAN_IPT];
         uint64_t am[SCAN_IPT];
         uint32_t ra = 0;
 #pragma unroll
@@ -608,6 +627,7 @@ __global__ void __launch_bounds__(SCAN1_NT) k_mc_scan1(const uint32_t *__restric
             const uint32_t c = (i < n) ? cnt[i] : 0u;
             v[k] = c & 0xFFFFu;
             t[k] = c >> 16;
+            raw[k] = c;
         }
 #pragma unroll
         for (int k = 0; k < SCAN_IPT; k++) {
@@ -638,6 +658,10 @@ __global__ void __launch_bounds__(SCAN1_NT) k_mc_scan1(const uint32_t *__restric
                     avb[r] = fv;
                     atb[r] = ft;
                     list[r] = list_base + (uint32_t)i;
+                    if (bs.vstart) {
+                        mark_block_starts(bs.vstart, bs.nv, fv, raw[k] & 0xFFFFu, r);
+                        mark_block_starts(bs.tstart, bs.nt, ft, raw[k] >> 16, r);
+                    }
                 }
             }
         }
@@ -3678,7 +3702,7 @@ void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *
                        const uint32_t *atb, uint32_t *tstart, uint32_t ntri, const uint64_t *N, const uint64_t *Zm,
                        const uint32_t *vbw, const uint64_t *NT, int *tri, uint32_t vbase, uint32_t *deg_blk,
                        uint32_t *deg_total, const uint32_t *d_counts, hipStream_t s, hipStream_t s_vert, hipEvent_t ev_fork,
-                       hipEvent_t ev_vert)
+                       hipEvent_t ev_vert, bool starts_ready)
 {
     if (!s_vert) s_vert = s;
     if (!nact || (!nv && !ntri)) {
@@ -3695,8 +3719,9 @@ void qs_launch_mc_emit(const QsMol *mols, const uint64_t *wpre, const uint32_t *
     const uint32_t nzero = ntri ? ntri / 256 + 1 : 0u;
     const QsStarts sv = {avb, nv, d_counts ? d_counts + 1 : nullptr, vstart, nullptr, 0u};
     const QsStarts st = {atb, ntri, d_counts ? d_counts : nullptr, tstart, deg_blk, nzero};
-    hipLaunchKernelGGL(k_block_starts, dim3(pgrid(nblk(std::max<uint32_t>(nact, nzero), 256)), 2), dim3(256), 0, s, sv, st, nact,
-                       d_counts ? d_counts + 2 : nullptr);
+    if (!starts_ready)
+        hipLaunchKernelGGL(k_block_starts, dim3(pgrid(nblk(std::max<uint32_t>(nact, nzero), 256)), 2), dim3(256), 0, s, sv, st,
+                           nact, d_counts ? d_counts + 2 : nullptr);
     /* the vertex kernel on s_vert (after the block starts), beside the triangle kernel on s */
     if (s_vert != s) {
         (void)hipEventRecord(ev_fork, s);
@@ -3780,12 +3805,12 @@ void qs_launch_tri_kept_before(const int *tri, const uint32_t *dpre, const uint6
 /* scan helpers */
 void qs_launch_mc_scans(const uint32_t *cnt, uint64_t nw, uint32_t *sums, uint32_t *vbw,
                         uint32_t *tbw, uint32_t *list, uint32_t *avb, uint32_t *atb, uint32_t *totals, uint32_t list_base,
-                        hipStream_t s)
+                        hipStream_t s, QsBlockStarts bs)
 {
     if (!nw) return;
     if (nw <= QS_SCAN1_MAX) {
         hipLaunchKernelGGL(k_mc_scan1, dim3(1), dim3(SCAN1_NT), 0, s, cnt, nw, vbw, tbw, list, avb, atb, list_base,
-                           totals);
+                           totals, bs);
         return;
     }
     const uint64_t ntiles = qs_scan_tiles(nw);
@@ -3793,7 +3818,7 @@ void qs_launch_mc_scans(const uint32_t *cnt, uint64_t nw, uint32_t *sums, uint32
     hipLaunchKernelGGL(k_mc_reduce3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, cnt, nw, sums, ntiles);
     if (!self) hipLaunchKernelGGL(k_scan_planes, dim3(3), dim3(SCAN_NT), 0, s, sums, ntiles, totals);
     hipLaunchKernelGGL(k_mc_scan3, dim3((unsigned)ntiles), dim3(SCAN_NT), 0, s, cnt, nw, sums, ntiles, vbw, tbw, list,
-                       avb, atb, list_base, self, totals);
+                       avb, atb, list_base, self, totals, bs);
 }
 
 uint64_t qs_scan_tiles(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
